@@ -1,0 +1,141 @@
+/*
+ * regot_hip.h -- C-ABI of the MI355X (gfx950) EROT hot path.
+ *
+ * One shared library, libregot_hip.so, holds the HIP kernels and the host
+ * C++ solver drivers.  Everything crosses this boundary as plain pointers,
+ * sizes and int status codes (0 = ok, < 0 = error, message in
+ * regot_last_error()).  No torch or Eigen type appears here.
+ *
+ * Reference: yixuan/regot-python v0.0.3.  Each entry point names the
+ * reference interface it replaces (file:line).  Vectors passed as
+ * `const double*` / `double*` are HOST pointers unless the name says d_.
+ *
+ * Threading: a problem handle is not thread-safe; distinct handles may be
+ * used from different threads.  Calls block until their results are ready.
+ */
+#ifndef REGOT_HIP_H
+#define REGOT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REGOT_F32 0
+#define REGOT_F64 1
+
+/* Solver ids for regot_solve(). */
+#define REGOT_SOLVER_BCD 0
+#define REGOT_SOLVER_SSNS 1
+#define REGOT_SOLVER_SPARSE_NEWTON 2
+#define REGOT_SOLVER_SPLR 3
+
+typedef struct regot_problem regot_problem;
+typedef struct regot_result regot_result;
+
+/* Thread-local message of the last failed call ("" if none). */
+const char* regot_last_error(void);
+
+/* Library version, e.g. 100 for 0.1.0. */
+int regot_version(void);
+
+/* Number of visible GPUs (0 when the HIP runtime finds none). */
+int regot_device_count(void);
+
+/*
+ * Problem handle: M (n x m, row-major, row pitch ldm elements) resident on
+ * one GPU in the bulk precision, plus a, b, log a, log b and reg.
+ * Replaces the Sinkhorn::Problem constructor, src/sinkhorn_problem.h:36-44
+ * (and the pybind11 numpy -> Eigen conversion, src/solvers.cpp:359-373).
+ *
+ *   M_dtype   REGOT_F32 or REGOT_F64: element type of the caller's M.
+ *   bulk      REGOT_F32: M held in FP32, exp/sum in FP32, potentials FP64.
+ *             REGOT_F64: M held in FP64, everything in FP64.
+ *   M_device  0: M is host memory (copied once); 1: M is device memory on
+ *             `device` and is borrowed (must outlive the handle) when its
+ *             dtype equals bulk and rows are 16-byte aligned, else copied.
+ *   ldm       row pitch in elements (>= m); 0 means m.
+ *   a, b      host FP64 marginals (copied).
+ */
+int regot_problem_create(regot_problem** out, int n, int m, const void* M, int M_dtype,
+                         int bulk, int M_device, long long ldm, const double* a,
+                         const double* b, double reg, int device);
+int regot_problem_destroy(regot_problem* p);
+int regot_problem_dims(const regot_problem* p, int* n, int* m, int* bulk);
+
+/* alpha_i = reg (log a_i - LSE_j((beta_j - M_ij)/reg)).
+ * Replaces Problem::optimal_alpha, src/sinkhorn_problem_bcd.cpp:177-201. */
+int regot_optimal_alpha(regot_problem* p, const double* beta, double* alpha);
+
+/* beta_j = reg (log b_j - LSE_i((alpha_i - M_ij)/reg)).
+ * Replaces Problem::optimal_beta, src/sinkhorn_problem_bcd.cpp:149-174. */
+int regot_optimal_beta(regot_problem* p, const double* alpha, double* beta);
+
+/* f(gamma) = reg sum(T) - alpha'a - betat'bt, gamma = [alpha; beta_0..m-2].
+ * Replaces Problem::dual_obj, src/sinkhorn_problem_obj.cpp:324-342. */
+int regot_dual_obj(regot_problem* p, const double* gamma, double* obj);
+
+/* f and grad (n+m-1) at gamma.
+ * Replaces Problem::dual_obj_grad, src/sinkhorn_problem_obj.cpp:363-393. */
+int regot_dual_obj_grad(regot_problem* p, const double* gamma, double* obj, double* grad);
+
+/* plan = exp((alpha (+) beta - M)/reg), FP64 row-major n x m, beta_{m-1} = 0.
+ * Replaces SinkhornResult::get_plan, src/sinkhorn_result.h:32-50. */
+int regot_get_plan(regot_problem* p, const double* gamma, double* plan);
+
+/*
+ * Solver options; zero-initialise, then call regot_opts_default().
+ * Mirrors SinkhornSolverOpts, src/sinkhorn_solvers.h:14-38, and the kwargs
+ * parsed by parse_sinkhorn_opts, src/solvers.cpp:314-341.
+ */
+typedef struct {
+    int method;          /* linear solver; this build always runs CG on the GPU */
+    double mu0;          /* SSNS */
+    double shift;        /* sparse Newton / SPLR */
+    double density;      /* sparse Newton / SPLR */
+    double cg_tol;       /* <= 0: driver default */
+    int cg_max_iter;     /* <= 0: driver default */
+    const double* x0;    /* optional warm start (n+m-1), host; NULL = reference init */
+    int graph_batch;     /* BCD iterations per captured hipGraph launch; <= 0: default */
+} regot_opts;
+
+void regot_opts_default(regot_opts* o, int solver);
+
+/*
+ * Run a solver on a problem handle.
+ * Replaces sinkhorn_{bcd,ssns,sparse_newton,splr}_internal,
+ * src/sinkhorn_solvers.h:40-94 (drivers src/sinkhorn_bcd.cpp:14-124,
+ * src/sinkhorn_ssns.cpp:16-234, src/sinkhorn_sparse_newton.cpp:24-170,
+ * src/sinkhorn_splr.cpp:24-199).  The plan is not formed here; see
+ * regot_result_plan().  verbose >= 1 prints "iter = i, objval = f,
+ * ||grad|| = g" lines to stdout, as the reference does.
+ */
+int regot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,
+                const regot_opts* opts, regot_result** out);
+
+/* Result accessors (SinkhornResult, src/sinkhorn_result.h:11-51). */
+int regot_result_niter(const regot_result* r);
+int regot_result_hist_len(const regot_result* r);       /* niter + 1 */
+int regot_result_dens_len(const regot_result* r);       /* 0 for BCD */
+/* Copies out: dual (n+m-1), obj_vals, mar_errs, run_times (hist_len),
+ * densities (dens_len).  Any pointer may be NULL. */
+int regot_result_get(const regot_result* r, double* dual, double* obj_vals, double* mar_errs,
+                     double* run_times, double* densities);
+/* FP64 plan of the result's dual, row-major n x m, into host memory. */
+int regot_result_plan(regot_problem* p, const regot_result* r, double* plan);
+/* Marginal error ||grad|| re-evaluated at the result's dual in FP64 arithmetic
+ * (exp in FP64 regardless of bulk), plus the FP64 objective. */
+int regot_result_verify(regot_problem* p, const regot_result* r, double* mar_err_fp64,
+                        double* obj_fp64);
+void regot_result_free(regot_result* r);
+
+/* Device-side timing of the hot kernels (HIP events on the handle's stream).
+ * kind: 0 = row pass, 1 = column pass (partial + combine).  Enabled by
+ * regot_profile_enable(p, 1); totals accumulate until reset. */
+int regot_profile_enable(regot_problem* p, int on);
+int regot_profile_get(regot_problem* p, int kind, long long* launches, double* total_ms);
+int regot_profile_reset(regot_problem* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

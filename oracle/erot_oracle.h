@@ -1,0 +1,114 @@
+/*
+ * erot_oracle.h -- FP64 CPU restatement of the EROT path of
+ * yixuan/regot-python v0.0.3 (src/sinkhorn_*).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the parity checker: only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
+ * The product (libregot_hip.so and the regot package) never links or calls it.
+ *
+ * Storage: the cost matrix M and every dense n x m array here are row-major
+ * (C order), the layout the GPU build keeps in HBM.  The reference stores them
+ * column-major (Eigen); only the summation order differs.
+ *
+ * Every function names the reference file:line it restates.
+ */
+#ifndef EROT_ORACLE_H
+#define EROT_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Problem dense passes (src/sinkhorn_problem_bcd.cpp, _obj.cpp) ---- */
+void orc_optimal_beta(int n, int m, const double* M, const double* b, double reg,
+                      const double* alpha, double* beta);
+void orc_optimal_alpha(int n, int m, const double* M, const double* a, double reg,
+                       const double* beta, double* alpha);
+double orc_compute_T(int n, int m, const double* M, double reg,
+                     const double* gamma, double* T);
+void orc_compute_sums(int n, int m, const double* T, double* Tsums);
+double orc_dual_obj(int n, int m, const double* M, const double* a, const double* b,
+                    double reg, const double* gamma);
+double orc_dual_obj_grad(int n, int m, const double* M, const double* a, const double* b,
+                         double reg, const double* gamma, double* grad,
+                         double* T, int computeT);
+void orc_get_plan(int n, int m, const double* M, double reg, const double* gamma,
+                  double* plan);
+
+/* ---- Sparsifiers (src/sinkhorn_sparsify3.cpp, src/sinkhorn_sparsify4.cpp) ----
+ * Output: CSC of the n x (m-1) kept block of T.  colptr has m entries;
+ * *rowind and *val are malloc'ed here and released with orc_free(). */
+long long orc_sparsify_mat3(int n, int m, const double* T, double delta, double density_hint,
+                            int* colptr, int** rowind, double** val);
+long long orc_sparsify_density(int n, int m, const double* T, double density, int preserve,
+                               int* colptr, int** rowind, double** val);
+void orc_free(void* p);
+
+/* ---- Hessian operators (src/sinkhorn_hess.cpp:99-164) on an explicit CSC ---- */
+void orc_apply_sigmadx(int n, int m, const int* colptr, const int* rowind, const double* val,
+                       const double* x, double* res);
+void orc_apply_sigmadtx(int n, int m, const int* colptr, const int* rowind, const double* val,
+                        const double* x, double* res);
+void orc_apply_deltax(int n, int m, const double* h1, const double* h2,
+                      const int* colptr, const int* rowind, const double* val,
+                      const double* x, double shift, double* res);
+void orc_apply_hsx(int n, int m, const double* h1, const double* h2,
+                   const int* colptr, const int* rowind, const double* val,
+                   const double* x, double* res);
+int orc_hess_cg(int n, int m, const double* h1, const double* h2,
+                const int* colptr, const int* rowind, const double* val,
+                const double* rhs, double shift, double tol, int max_iter,
+                double* res, double* tol_error);
+int orc_hess_direct(int n, int m, const double* h1, const double* h2,
+                    const int* colptr, const int* rowind, const double* val,
+                    const double* rhs, double shift, double* res);
+
+/* ---- Line-search helpers (src/sinkhorn_problem_ls.cpp:72-223) ---- */
+double orc_cubic_minimizer(double a, double b, double fa, double fb, double ga, double gb,
+                           int* exists);
+double orc_step_selection(double al, double au, double at, double fl, double fu, double ft,
+                          double gl, double gu, double gt);
+
+/* ---- Solver drivers (src/sinkhorn_{bcd,ssns,sparse_newton,splr}.cpp) ----
+ * History arrays need max_iter + 1 slots; dual needs n + m - 1.
+ * Returns niter; *hist_len receives the number of history entries.
+ * lin_method: 0 = CG, otherwise a dense Cholesky solve of (Hs + shift I)
+ * (the mathematical equivalent of the reference's SimplicialLDLT; small sizes only).
+ * cg_tol <= 0 keeps the driver's hard-coded 1e-8. */
+typedef struct {
+    int lin_method;
+    double mu0;
+    double shift;
+    double density;
+    double cg_tol;
+    int cg_max_iter;      /* <= 0: Eigen's default 2 * HessianCG::cols() = 2m */
+    int lowrank_cg;       /* SPLR: 1 = CG for the two low-rank solves instead of direct */
+    int dense_T;          /* 1: keep the n x m T like the reference (needed by Newton drivers) */
+} orc_opts;
+
+int orc_sinkhorn_bcd(int n, int m, const double* M, const double* a, const double* b,
+                     double reg, double tol, int max_iter, int verbose,
+                     double* dual, double* obj_vals, double* mar_errs, double* run_times,
+                     int* hist_len);
+int orc_sinkhorn_ssns(int n, int m, const double* M, const double* a, const double* b,
+                      double reg, double tol, int max_iter, int verbose, const orc_opts* opts,
+                      double* dual, double* obj_vals, double* mar_errs, double* run_times,
+                      double* densities, int* hist_len);
+int orc_sinkhorn_sparse_newton(int n, int m, const double* M, const double* a, const double* b,
+                               double reg, double tol, int max_iter, int verbose,
+                               const orc_opts* opts,
+                               double* dual, double* obj_vals, double* mar_errs,
+                               double* run_times, double* densities, int* hist_len);
+int orc_sinkhorn_splr(int n, int m, const double* M, const double* a, const double* b,
+                      double reg, double tol, int max_iter, int verbose, const orc_opts* opts,
+                      double* dual, double* obj_vals, double* mar_errs, double* run_times,
+                      double* densities, int* hist_len);
+
+/* BCD iterations only (no convergence test), for timing a bounded CPU sample. */
+double orc_bcd_iterations(int n, int m, const double* M, const double* a, const double* b,
+                          double reg, int iters, double* gamma_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,282 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/erot_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the parity checker for the HIP build.  Only tests/,
+``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg import this
+module; the ``regot`` package never does.
+
+Every wrapper takes row-major float64 numpy arrays and names the reference
+function it restates (yixuan/regot-python v0.0.3, ``src/sinkhorn_*``).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "liberot_oracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [
+        ("lin_method", ctypes.c_int),
+        ("mu0", ctypes.c_double),
+        ("shift", ctypes.c_double),
+        ("density", ctypes.c_double),
+        ("cg_tol", ctypes.c_double),
+        ("cg_max_iter", ctypes.c_int),
+        ("lowrank_cg", ctypes.c_int),
+        ("dense_T", ctypes.c_int),
+    ]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = ctypes.CDLL(_SO)
+        for name in ("orc_dual_obj", "orc_dual_obj_grad", "orc_compute_T", "orc_bcd_iterations",
+                     "orc_cubic_minimizer", "orc_step_selection"):
+            getattr(L, name).restype = ctypes.c_double
+        for name in ("orc_sparsify_mat3", "orc_sparsify_density"):
+            getattr(L, name).restype = ctypes.c_longlong
+        _lib = L
+    return _lib
+
+
+def _d(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _p(x):
+    return x.ctypes.data_as(_dp)
+
+
+# --------------------------------------------------------------------------
+# Problem dense passes
+# --------------------------------------------------------------------------
+def optimal_beta(M, b, reg, alpha):
+    """src/sinkhorn_problem_bcd.cpp:149-174"""
+    M, b, alpha = _d(M), _d(b), _d(alpha)
+    n, m = M.shape
+    beta = np.empty(m)
+    lib().orc_optimal_beta(n, m, _p(M), _p(b), ctypes.c_double(reg), _p(alpha), _p(beta))
+    return beta
+
+
+def optimal_alpha(M, a, reg, beta):
+    """src/sinkhorn_problem_bcd.cpp:177-201"""
+    M, a, beta = _d(M), _d(a), _d(beta)
+    n, m = M.shape
+    alpha = np.empty(n)
+    lib().orc_optimal_alpha(n, m, _p(M), _p(a), ctypes.c_double(reg), _p(beta), _p(alpha))
+    return alpha
+
+
+def compute_T(M, reg, gamma):
+    """src/sinkhorn_problem_obj.cpp:139-188 -> (T, sum T)"""
+    M, gamma = _d(M), _d(gamma)
+    n, m = M.shape
+    T = np.empty((n, m))
+    s = lib().orc_compute_T(n, m, _p(M), ctypes.c_double(reg), _p(gamma), _p(T))
+    return T, s
+
+
+def dual_obj(M, a, b, reg, gamma):
+    """src/sinkhorn_problem_obj.cpp:324-342"""
+    M, a, b, gamma = _d(M), _d(a), _d(b), _d(gamma)
+    n, m = M.shape
+    return lib().orc_dual_obj(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg), _p(gamma))
+
+
+def dual_obj_grad(M, a, b, reg, gamma):
+    """src/sinkhorn_problem_obj.cpp:363-393 -> (f, grad)"""
+    M, a, b, gamma = _d(M), _d(a), _d(b), _d(gamma)
+    n, m = M.shape
+    g = np.empty(n + m - 1)
+    f = lib().orc_dual_obj_grad(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg), _p(gamma),
+                                _p(g), None, 1)
+    return f, g
+
+
+def get_plan(M, reg, gamma):
+    """src/sinkhorn_result.h:32-50"""
+    M, gamma = _d(M), _d(gamma)
+    n, m = M.shape
+    plan = np.empty((n, m))
+    lib().orc_get_plan(n, m, _p(M), ctypes.c_double(reg), _p(gamma), _p(plan))
+    return plan
+
+
+# --------------------------------------------------------------------------
+# Sparsifiers -> (colptr, rowind, val) CSC of the n x (m-1) block
+# --------------------------------------------------------------------------
+def _csc_from(call, n, m):
+    colptr = np.empty(m, dtype=np.int32)
+    ri = ctypes.POINTER(ctypes.c_int)()
+    va = ctypes.POINTER(ctypes.c_double)()
+    nnz = call(colptr.ctypes.data_as(_ip), ctypes.byref(ri), ctypes.byref(va))
+    rowind = np.ctypeslib.as_array(ri, shape=(max(nnz, 1),))[:nnz].copy()
+    val = np.ctypeslib.as_array(va, shape=(max(nnz, 1),))[:nnz].copy()
+    lib().orc_free(ri)
+    lib().orc_free(va)
+    return colptr, rowind, val
+
+
+def sparsify_threshold(T, delta, density_hint):
+    """SSNS threshold rule, src/sinkhorn_sparsify3.cpp:276-405"""
+    T = _d(T)
+    n, m = T.shape
+    return _csc_from(lambda cp, ri, va: lib().orc_sparsify_mat3(
+        n, m, _p(T), ctypes.c_double(delta), ctypes.c_double(density_hint), cp, ri, va), n, m)
+
+
+def sparsify_density(T, density, preserve=True):
+    """Global top-k rule, src/sinkhorn_sparsify4.cpp:77-174"""
+    T = _d(T)
+    n, m = T.shape
+    return _csc_from(lambda cp, ri, va: lib().orc_sparsify_density(
+        n, m, _p(T), ctypes.c_double(density), int(preserve), cp, ri, va), n, m)
+
+
+def csc_to_dense(n, m, colptr, rowind, val):
+    S = np.zeros((n, m - 1))
+    for j in range(m - 1):
+        for k in range(colptr[j], colptr[j + 1]):
+            S[rowind[k], j] = val[k]
+    return S
+
+
+# --------------------------------------------------------------------------
+# Hessian operators (CSC sigma), src/sinkhorn_hess.cpp:99-164, src/sinkhorn_cg.cpp
+# --------------------------------------------------------------------------
+def _csc_args(colptr, rowind, val):
+    colptr = np.ascontiguousarray(colptr, dtype=np.int32)
+    rowind = np.ascontiguousarray(rowind, dtype=np.int32)
+    val = _d(val)
+    return colptr, rowind, val
+
+
+def apply_deltax(n, m, h1, h2, csc, x, shift):
+    cp, ri, va = _csc_args(*csc)
+    h1, h2, x = _d(h1), _d(h2), _d(x)
+    res = np.empty(m - 1)
+    lib().orc_apply_deltax(n, m, _p(h1), _p(h2), cp.ctypes.data_as(_ip), ri.ctypes.data_as(_ip),
+                           _p(va), _p(x), ctypes.c_double(shift), _p(res))
+    return res
+
+
+def apply_hsx(n, m, h1, h2, csc, x):
+    cp, ri, va = _csc_args(*csc)
+    h1, h2, x = _d(h1), _d(h2), _d(x)
+    res = np.empty(n + m - 1)
+    lib().orc_apply_hsx(n, m, _p(h1), _p(h2), cp.ctypes.data_as(_ip), ri.ctypes.data_as(_ip),
+                        _p(va), _p(x), _p(res))
+    return res
+
+
+def hess_cg(n, m, h1, h2, csc, rhs, shift, tol=1e-8, max_iter=0):
+    """(Hs + shift I)^{-1} rhs via CG on the Schur complement -> (x, iters, rel. residual)"""
+    cp, ri, va = _csc_args(*csc)
+    h1, h2, rhs = _d(h1), _d(h2), _d(rhs)
+    res = np.empty(n + m - 1)
+    err = ctypes.c_double(0)
+    it = lib().orc_hess_cg(n, m, _p(h1), _p(h2), cp.ctypes.data_as(_ip), ri.ctypes.data_as(_ip),
+                           _p(va), _p(rhs), ctypes.c_double(shift), ctypes.c_double(tol),
+                           int(max_iter), _p(res), ctypes.byref(err))
+    return res, it, err.value
+
+
+def hess_direct(n, m, h1, h2, csc, rhs, shift):
+    cp, ri, va = _csc_args(*csc)
+    h1, h2, rhs = _d(h1), _d(h2), _d(rhs)
+    res = np.empty(n + m - 1)
+    rc = lib().orc_hess_direct(n, m, _p(h1), _p(h2), cp.ctypes.data_as(_ip),
+                               ri.ctypes.data_as(_ip), _p(va), _p(rhs), ctypes.c_double(shift),
+                               _p(res))
+    if rc != 0:
+        raise RuntimeError("dense Cholesky failed (%d)" % rc)
+    return res
+
+
+def cubic_minimizer(a, b, fa, fb, ga, gb):
+    ex = ctypes.c_int(0)
+    r = lib().orc_cubic_minimizer(*(ctypes.c_double(v) for v in (a, b, fa, fb, ga, gb)),
+                                  ctypes.byref(ex))
+    return r, bool(ex.value)
+
+
+def step_selection(al, au, at, fl, fu, ft, gl, gu, gt):
+    return lib().orc_step_selection(*(ctypes.c_double(v) for v in (al, au, at, fl, fu, ft, gl, gu, gt)))
+
+
+# --------------------------------------------------------------------------
+# Drivers -> dict with the SinkhornResult fields
+# --------------------------------------------------------------------------
+def _run(fn, M, a, b, reg, tol, max_iter, verbose, opts, with_dens):
+    M, a, b = _d(M), _d(a), _d(b)
+    n, m = M.shape
+    dual = np.empty(n + m - 1)
+    h = [np.empty(max_iter + 1) for _ in range(4)]
+    hl = ctypes.c_int(0)
+    args = [n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg), ctypes.c_double(tol),
+            int(max_iter), int(verbose)]
+    if opts is not None:
+        args.append(ctypes.byref(opts))
+    args += [_p(dual), _p(h[0]), _p(h[1]), _p(h[2])]
+    if with_dens:
+        args.append(_p(h[3]))
+    args.append(ctypes.byref(hl))
+    niter = fn(*args)
+    k = hl.value
+    out = dict(niter=niter, dual=dual, obj_vals=list(h[0][:k]), mar_errs=list(h[1][:k]),
+               run_times=list(h[2][:k]), densities=list(h[3][:k]) if with_dens else [])
+    return out
+
+
+def _opts(method=1, mu0=1.0, shift=1e-3, density=0.01, cg_tol=0.0, cg_max_iter=0,
+          lowrank_cg=False):
+    return _Opts(int(method), float(mu0), float(shift), float(density), float(cg_tol),
+                 int(cg_max_iter), int(lowrank_cg), 1)
+
+
+def sinkhorn_bcd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0):
+    """src/sinkhorn_bcd.cpp:14-124"""
+    return _run(lib().orc_sinkhorn_bcd, M, a, b, reg, tol, max_iter, verbose, None, False)
+
+
+def sinkhorn_ssns(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kw):
+    """src/sinkhorn_ssns.cpp:16-234 (method defaults to 1, src/solvers.cpp:425-441)"""
+    return _run(lib().orc_sinkhorn_ssns, M, a, b, reg, tol, max_iter, verbose, _opts(**kw), True)
+
+
+def sinkhorn_sparse_newton(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kw):
+    """src/sinkhorn_sparse_newton.cpp:24-170"""
+    return _run(lib().orc_sinkhorn_sparse_newton, M, a, b, reg, tol, max_iter, verbose,
+                _opts(**kw), True)
+
+
+def sinkhorn_splr(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kw):
+    """src/sinkhorn_splr.cpp:24-199"""
+    return _run(lib().orc_sinkhorn_splr, M, a, b, reg, tol, max_iter, verbose, _opts(**kw), True)
+
+
+def bcd_iterations(M, a, b, reg, iters):
+    """Wall seconds of ``iters`` BCD iterations (no convergence test)."""
+    M, a, b = _d(M), _d(a), _d(b)
+    n, m = M.shape
+    g = np.empty(n + m - 1)
+    return lib().orc_bcd_iterations(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg), int(iters),
+                                    _p(g))

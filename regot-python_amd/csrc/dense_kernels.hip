@@ -1,0 +1,666 @@
+// dense_kernels.hip -- streaming passes over the cost matrix M for gfx950.
+//
+// Every pass evaluates x_ij = (p_i + q_j - M_ij) / reg in FP64 (M is FP32 data
+// upcast exactly, or FP64 in FP64 mode), converts the shifted value to the
+// bulk type, and accumulates 2^(x - shift) with a per-lane adaptive shift:
+//
+//   * the shift starts at a caller-supplied guess (log2 of the expected sum),
+//   * it moves down once if the first values all sit more than 8 below it,
+//   * it moves up whenever a value exceeds it by more than 1,
+//
+// so the dominant terms are always formed within a few units of zero, where
+// the FP32 rounding of the exponent argument is ~1e-7 relative, and nothing
+// overflows or underflows whatever the guess.  In steady state (good guess)
+// the shift never moves and the inner loop is: 4 FP64-rate ops (cvt, add,
+// fma, cvt), one max, one exp2 and one add per element.
+//
+// Partial results combine as FP64 log2-sums: log2(sum) = L_max + log2(sum
+// 2^(L - L_max)).  All reductions run in a fixed order, so results are
+// bitwise reproducible run to run.
+#include "dense.hpp"
+
+#include <cmath>
+
+namespace regot {
+namespace {
+
+constexpr double kLog2e = 1.4426950408889634074;
+constexpr double kLn2 = 0.69314718055994530942;
+
+template <typename T>
+struct VecOf;
+template <>
+struct VecOf<float> {
+    using type = float4;
+    static constexpr int N = 4;
+};
+template <>
+struct VecOf<double> {
+    using type = double2;
+    static constexpr int N = 2;
+};
+
+__device__ __forceinline__ float ex2(float x) { return exp2f(x); }
+__device__ __forceinline__ double ex2(double x) { return exp2(x); }
+__device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double vmax(double a, double b) { return fmax(a, b); }
+
+template <typename T>
+__device__ __forceinline__ T ninf()
+{
+    return -static_cast<T>(INFINITY);
+}
+
+template <typename VT, typename T>
+__device__ __forceinline__ T vget(const VT& v, int i);
+template <>
+__device__ __forceinline__ float vget<float4, float>(const float4& v, int i)
+{
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+}
+template <>
+__device__ __forceinline__ double vget<double2, double>(const double2& v, int i)
+{
+    return i == 0 ? v.x : v.y;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum0(double v)
+{
+    // Butterfly sum; lane 0's value is used so the order is fixed.
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return __shfl(v, 0);
+}
+
+// log2 of sum over the wave of 2^L (L = -inf for empty lanes).
+__device__ __forceinline__ double wave_log2sum(double L)
+{
+    const double Lmax = wave_max(L);
+    if (Lmax == -INFINITY) return -INFINITY;
+    const double e = (L == -INFINITY) ? 0.0 : exp2(L - Lmax);
+    return Lmax + log2(wave_sum0(e));
+}
+
+template <typename TA>
+__device__ __forceinline__ double acc_log2(double sh, TA s)
+{
+    return (s > TA(0)) ? sh + log2(static_cast<double>(s)) : -INFINITY;
+}
+
+// Move the shift when the running max leaves [-8, 1].  Returns true if moved.
+template <typename TA>
+__device__ __forceinline__ bool reshift(double& sh, TA& s, TA& rm)
+{
+    if (rm > TA(1) || (rm < TA(-8) && rm > ninf<TA>())) {
+        const TA d = rm;
+        sh += static_cast<double>(d);
+        s *= ex2(-d);
+        rm = TA(0);
+        return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// Row pass
+// ---------------------------------------------------------------------------
+
+// One row x V columns: t_v = (bk_v - sh) - k M_v, masked entries = -inf.
+template <typename TM, int V, bool MASK>
+__device__ __forceinline__ void row_accumulate(double& sh, TM& s, TM& rm, const double* bk,
+                                               const typename VecOf<TM>::type& mv, double k,
+                                               int valid_cols)
+{
+    TM t[V];
+    TM cm = ninf<TM>();
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const double mvd = static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv, v));
+        t[v] = static_cast<TM>(fma(-k, mvd, bk[v] - sh));
+        if (MASK && v >= valid_cols) t[v] = ninf<TM>();
+        cm = vmax(cm, t[v]);
+    }
+    rm = vmax(rm, cm);
+    if (reshift(sh, s, rm)) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const double mvd = static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv, v));
+            t[v] = static_cast<TM>(fma(-k, mvd, bk[v] - sh));
+            if (MASK && v >= valid_cols) t[v] = ninf<TM>();
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++) s += ex2(t[v]);
+}
+
+// Each wave owns R consecutive rows and streams them left to right, lane l
+// reading the V-wide vector at column c*64V + l*V of each row (1 KiB per row
+// per step, fully coalesced).  The beta chunk is loaded once per step and
+// reused by all R rows.
+template <typename TM, int R>
+__global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long ldm, int n,
+                                                  int m, const double* __restrict__ beta,
+                                                  double k, RowEpi epi,
+                                                  const int* __restrict__ ctl)
+{
+    using VT = typename VecOf<TM>::type;
+    constexpr int V = VecOf<TM>::N;
+    if (ctl != nullptr && ctl[0] != 0) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const long row0 = (static_cast<long>(blockIdx.x) * 4 + wave) * R;
+    if (row0 >= n) return;
+
+    const double* rowpot = epi.rowpot;
+    double* aout = epi.alpha_out;
+    if (epi.parity) {
+        const int kk = ctl[1];
+        rowpot = epi.apair[kk & 1];
+        aout = epi.apair[(kk + 1) & 1];
+    }
+
+    double sh[R];
+    TM s[R], rm[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const long i = row0 + r;
+        double g = 0.0;
+        if (i < n && epi.loga != nullptr) {
+            g = epi.loga[i] * kLog2e;
+            if (rowpot != nullptr) g -= rowpot[i] * k;
+        }
+        sh[r] = g;
+        s[r] = TM(0);
+        rm[r] = ninf<TM>();
+    }
+    const TM* rowp[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) rowp[r] = M + (row0 + r < n ? row0 + r : row0) * ldm;
+
+    const int step = 64 * V;
+    const int nfull = m / step;
+#pragma unroll 2
+    for (int c = 0; c < nfull; c++) {
+        const int j0 = c * step + lane * V;
+        double bk[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) bk[v] = beta[j0 + v] * k;
+        VT mv[R];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (row0 + r < n) mv[r] = *reinterpret_cast<const VT*>(rowp[r] + j0);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (row0 + r < n) row_accumulate<TM, V, false>(sh[r], s[r], rm[r], bk, mv[r], k, V);
+    }
+    {
+        const int j0 = nfull * step + lane * V;
+        if (j0 < m) {
+            const int valid = (m - j0) < V ? (m - j0) : V;
+            double bk[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) bk[v] = (v < valid) ? beta[j0 + v] * k : 0.0;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if (row0 + r < n) {
+                    // ldm >= m rounded up to V, so the vector stays inside the row pitch.
+                    const VT mv = *reinterpret_cast<const VT*>(rowp[r] + j0);
+                    row_accumulate<TM, V, true>(sh[r], s[r], rm[r], bk, mv, k, valid);
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const long i = row0 + r;
+        if (i >= n) break;
+        const double L = wave_log2sum(acc_log2(sh[r], s[r]));
+        if (lane == 0) {
+            if (epi.lse2) epi.lse2[i] = L;
+            if (aout) aout[i] = epi.reg * (epi.loga[i] - kLn2 * L);
+            if (epi.rowsum) epi.rowsum[i] = exp2(rowpot[i] * k + L);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column pass
+// ---------------------------------------------------------------------------
+
+template <typename TM, int V, int CH, bool MASK>
+__device__ __forceinline__ void col_chunk(double* sh, TM* s, TM* rm,
+                                          const typename VecOf<TM>::type* mv, const double* ak,
+                                          double k, int valid_rows, int valid_cols)
+{
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        TM t[CH];
+        TM cm = ninf<TM>();
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            const double mvd = static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv[c], v));
+            t[c] = static_cast<TM>(fma(-k, mvd, ak[c] - sh[v]));
+            if (MASK && (c >= valid_rows || v >= valid_cols)) t[c] = ninf<TM>();
+            cm = vmax(cm, t[c]);
+        }
+        rm[v] = vmax(rm[v], cm);
+        if (reshift(sh[v], s[v], rm[v])) {
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                const double mvd =
+                    static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv[c], v));
+                t[c] = static_cast<TM>(fma(-k, mvd, ak[c] - sh[v]));
+                if (MASK && (c >= valid_rows || v >= valid_cols)) t[c] = ninf<TM>();
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CH; c++) s[v] += ex2(t[c]);
+    }
+}
+
+// Workgroup = 4 waves over one 64V-column tile and one block of rows; the
+// waves take CH-row slabs round robin.  Output: one FP64 log2 partial sum per
+// (row block, column).
+template <typename TM>
+__global__ __launch_bounds__(256) void k_col_partial(
+    const TM* __restrict__ M, long ldm, int n, int m, const double* __restrict__ alpha,
+    const double* __restrict__ ap0, const double* __restrict__ ap1,
+    const double* __restrict__ colpot, const double* __restrict__ logb, double k, int rpb,
+    double* __restrict__ part, const int* __restrict__ ctl)
+{
+    using VT = typename VecOf<TM>::type;
+    constexpr int V = VecOf<TM>::N;
+    constexpr int CH = 4;
+    __shared__ double red[4][64 * V];
+    if (ctl != nullptr && ctl[0] != 0) return;
+    if (ap0 != nullptr) alpha = (ctl[1] & 1) ? ap1 : ap0;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int j0 = blockIdx.x * 64 * V + lane * V;
+    const int valid_cols = (m - j0) < V ? (m - j0) : V;   // <= 0: lane has no column
+    const long rb = static_cast<long>(blockIdx.y) * rpb;
+    const long re = (rb + rpb < n) ? rb + rpb : n;
+
+    double sh[V];
+    TM s[V], rm[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const int j = j0 + v;
+        double g = 0.0;
+        if (v < valid_cols && logb != nullptr) {
+            g = logb[j] * kLog2e;
+            if (colpot != nullptr) g -= colpot[j] * k;
+        }
+        sh[v] = g;
+        s[v] = TM(0);
+        rm[v] = ninf<TM>();
+    }
+
+    if (valid_cols > 0) {
+        const TM* base = M + j0;
+        long r = rb + wave * CH;
+        for (; r + CH <= re; r += 4 * CH) {
+            VT mv[CH];
+            double ak[CH];
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                mv[c] = *reinterpret_cast<const VT*>(base + (r + c) * ldm);
+                ak[c] = alpha[r + c] * k;
+            }
+            if (valid_cols == V)
+                col_chunk<TM, V, CH, false>(sh, s, rm, mv, ak, k, CH, V);
+            else
+                col_chunk<TM, V, CH, true>(sh, s, rm, mv, ak, k, CH, valid_cols);
+        }
+        if (r < re) {
+            const int vr = static_cast<int>(re - r);
+            VT mv[CH];
+            double ak[CH];
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                const long rr = (c < vr) ? r + c : r;
+                mv[c] = *reinterpret_cast<const VT*>(base + rr * ldm);
+                ak[c] = alpha[rr] * k;
+            }
+            col_chunk<TM, V, CH, true>(sh, s, rm, mv, ak, k, vr, valid_cols);
+        }
+    }
+
+#pragma unroll
+    for (int v = 0; v < V; v++) red[wave][lane * V + v] = acc_log2(sh[v], s[v]);
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            if (v >= valid_cols) break;
+            double Lm = -INFINITY;
+#pragma unroll
+            for (int w = 0; w < 4; w++) Lm = fmax(Lm, red[w][lane * V + v]);
+            double out = -INFINITY;
+            if (Lm != -INFINITY) {
+                double S = 0.0;
+#pragma unroll
+                for (int w = 0; w < 4; w++) {
+                    const double L = red[w][lane * V + v];
+                    S += (L == -INFINITY) ? 0.0 : exp2(L - Lm);
+                }
+                out = Lm + log2(S);
+            }
+            part[static_cast<long>(blockIdx.y) * m + j0 + v] = out;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_col_combine(const double* __restrict__ part, int nblk,
+                                                     int m, double k, ColEpi epi,
+                                                     const int* __restrict__ ctl)
+{
+    if (ctl != nullptr && ctl[0] != 0) return;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= m) return;
+    double Lm = -INFINITY;
+    for (int p = 0; p < nblk; p++) Lm = fmax(Lm, part[static_cast<long>(p) * m + j]);
+    double L = -INFINITY;
+    if (Lm != -INFINITY) {
+        double S = 0.0;
+        for (int p = 0; p < nblk; p++) {
+            const double x = part[static_cast<long>(p) * m + j];
+            S += (x == -INFINITY) ? 0.0 : exp2(x - Lm);
+        }
+        L = Lm + log2(S);
+    }
+    if (epi.lse2) epi.lse2[j] = L;
+    if (epi.colsum) epi.colsum[j] = exp2(epi.colpot[j] * k + L);
+    if (epi.beta_out) {
+        const double bn = epi.reg * (epi.logb[j] - kLn2 * L);
+        epi.beta_out[j] = bn;
+        if (epi.resid) epi.resid[j] = epi.b[j] * expm1(bn / epi.reg + kLn2 * L - epi.logb[j]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 plan, BCD bookkeeping and small helpers
+// ---------------------------------------------------------------------------
+
+template <typename TM>
+__global__ __launch_bounds__(256) void k_plan(const TM* __restrict__ M, long ldm, int m,
+                                              const double* __restrict__ alpha,
+                                              const double* __restrict__ beta, double reg,
+                                              double* __restrict__ plan, long r0)
+{
+    const long i = r0 + blockIdx.y;
+    const double ai = alpha[i];
+    const TM* Mi = M + i * ldm;
+    double* Pi = plan + static_cast<long>(blockIdx.y) * m;
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256)
+        Pi[j] = exp(((ai + beta[j]) - static_cast<double>(Mi[j])) / reg);
+}
+
+__device__ __forceinline__ double block_sum_1024(double v, double* sm)
+{
+    const int t = threadIdx.x;
+    sm[t] = v;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (t < o) sm[t] += sm[t + o];
+        __syncthreads();
+    }
+    const double r = sm[0];
+    __syncthreads();
+    return r;
+}
+
+// Iterate k (ctl[1]) is (alpha_k = apair[k&1], beta_k = beta); rowsum holds
+// T_k 1_m from the row pass of this iteration and gcol the column residual of
+// the beta update that produced beta_k.  Records f_k, ||g_k|| and a timestamp,
+// then either stops (||g_k|| < tol or k == max_iter) or advances k.
+__global__ __launch_bounds__(1024) void k_bcd_finalize(BcdState st)
+{
+    __shared__ double sm[1024];
+    int* ctl = st.ctl;
+    if (ctl[0] != 0) return;
+    const int kk = ctl[1];
+    const double* alpha = st.apair[kk & 1];
+    const double bm = st.beta[st.m - 1];
+    double gn2 = 0.0, tsum = 0.0, da = 0.0, db = 0.0;
+    for (int i = threadIdx.x; i < st.n; i += 1024) {
+        const double r = st.rowsum[i];
+        const double g = r - st.a[i];
+        gn2 += g * g;
+        tsum += r;
+        da += (alpha[i] + bm) * st.a[i];
+    }
+    for (int j = threadIdx.x; j < st.m - 1; j += 1024) {
+        const double g = st.gcol[j];
+        gn2 += g * g;
+        db += (st.beta[j] - bm) * st.b[j];
+    }
+    gn2 = block_sum_1024(gn2, sm);
+    tsum = block_sum_1024(tsum, sm);
+    da = block_sum_1024(da, sm);
+    db = block_sum_1024(db, sm);
+    if (threadIdx.x == 0) {
+        const double f = st.reg * tsum - da - db;
+        const double gnorm = sqrt(gn2);
+        const int h = kk % st.hist_cap;
+        st.hist_f[h] = f;
+        st.hist_g[h] = gnorm;
+        st.hist_ts[h] = static_cast<long long>(wall_clock64());
+        if (gnorm < st.tol || kk >= st.max_iter) {
+            ctl[2] = kk;
+            ctl[0] = 1;
+        } else {
+            ctl[1] = kk + 1;
+        }
+    }
+}
+
+// One workgroup per row: FP64 row sums of exp((alpha_i + beta_j - M_ij)/reg).
+template <typename TM>
+__global__ __launch_bounds__(256) void k_verify_rows(const TM* __restrict__ M, long ldm, int m,
+                                                     const double* __restrict__ alpha,
+                                                     const double* __restrict__ beta, double reg,
+                                                     double* __restrict__ rowsum)
+{
+    __shared__ double sm[256];
+    const long i = blockIdx.x;
+    const TM* Mi = M + i * ldm;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < m; j += 256)
+        acc += exp(((alpha[i] + beta[j]) - static_cast<double>(Mi[j])) / reg);
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rowsum[i] = sm[0];
+}
+
+// One thread per column, rows in order: FP64 column sums.
+template <typename TM>
+__global__ __launch_bounds__(256) void k_verify_cols(const TM* __restrict__ M, long ldm, int n,
+                                                     int m, const double* __restrict__ alpha,
+                                                     const double* __restrict__ beta, double reg,
+                                                     double* __restrict__ colsum)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= m) return;
+    double acc = 0.0;
+    for (long i = 0; i < n; i++)
+        acc += exp(((alpha[i] + beta[j]) - static_cast<double>(M[i * ldm + j])) / reg);
+    colsum[j] = acc;
+}
+
+__global__ void k_fill(double* p, double v, long count)
+{
+    for (long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; i < count;
+         i += static_cast<long>(gridDim.x) * 256)
+        p[i] = v;
+}
+
+__global__ void k_fill_int(int* p, int v, long count)
+{
+    for (long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; i < count;
+         i += static_cast<long>(gridDim.x) * 256)
+        p[i] = v;
+}
+
+__global__ void k_stamp(long long* dst)
+{
+    if (threadIdx.x == 0) dst[0] = static_cast<long long>(wall_clock64());
+}
+
+__global__ void k_split_gamma(const double* gamma, int n, int m, double* alpha, double* beta)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < n) alpha[t] = gamma[t];
+    if (t < m) beta[t] = (t < m - 1) ? gamma[n + t] : 0.0;
+}
+
+int grid_for(long count)
+{
+    long g = (count + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    return static_cast<int>(g);
+}
+
+}  // namespace
+
+// Rows per wave of the row pass.
+template <typename TM>
+constexpr int kRowsPerWave = sizeof(TM) == 4 ? 4 : 2;
+
+void launch_row_pass(int bulk, const void* M, long ldm, int n, int m, const double* beta,
+                     double k, const RowEpi& epi, const int* ctl, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (bulk == 0) {
+        constexpr int R = kRowsPerWave<float>;
+        hipLaunchKernelGGL((k_row_pass<float, R>), dim3(ceil_div(n, 4 * R)), dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, beta, k, epi, ctl);
+    } else {
+        constexpr int R = kRowsPerWave<double>;
+        hipLaunchKernelGGL((k_row_pass<double, R>), dim3(ceil_div(n, 4 * R)), dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, n, m, beta, k, epi, ctl);
+    }
+    KERNEL_CHECK();
+}
+
+int col_pass_blocks(int n, int m)
+{
+    // Aim for >= 2048 workgroups (8 per CU on 256 CUs) with >= 16 rows each.
+    const int tiles = ceil_div(m, 256);
+    int nblk = ceil_div(2048, tiles);
+    const int maxblk = ceil_div(n, 16);
+    if (nblk > maxblk) nblk = maxblk;
+    if (nblk < 1) nblk = 1;
+    return nblk;
+}
+
+void launch_col_partial(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                        const double* alpha_pair0, const double* alpha_pair1,
+                        const double* colpot_guess, const double* logb, double k,
+                        double* part, int nblk, const int* ctl, hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return;
+    const int rpb = ceil_div(n, nblk);
+    if (bulk == 0) {
+        hipLaunchKernelGGL(k_col_partial<float>, dim3(ceil_div(m, 256), nblk), dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, alpha, alpha_pair0,
+                           alpha_pair1, colpot_guess, logb, k, rpb, part, ctl);
+    } else {
+        hipLaunchKernelGGL(k_col_partial<double>, dim3(ceil_div(m, 128), nblk), dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, n, m, alpha, alpha_pair0,
+                           alpha_pair1, colpot_guess, logb, k, rpb, part, ctl);
+    }
+    KERNEL_CHECK();
+}
+
+void launch_col_combine(const double* part, int nblk, int m, double k, const ColEpi& epi,
+                        const int* ctl, hipStream_t s)
+{
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_col_combine, dim3(ceil_div(m, 256)), dim3(256), 0, s, part, nblk, m, k,
+                       epi, ctl);
+    KERNEL_CHECK();
+}
+
+void launch_plan(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                 const double* beta, double reg, double* plan, long r0, int nr, hipStream_t s)
+{
+    (void)n;
+    if (nr <= 0 || m <= 0) return;
+    const int gx = ceil_div(m, 256) < 8 ? ceil_div(m, 256) : 8;
+    if (bulk == 0)
+        hipLaunchKernelGGL(k_plan<float>, dim3(gx, nr), dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, m, alpha, beta, reg, plan, r0);
+    else
+        hipLaunchKernelGGL(k_plan<double>, dim3(gx, nr), dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, m, alpha, beta, reg, plan, r0);
+    KERNEL_CHECK();
+}
+
+void launch_bcd_finalize(const BcdState& st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bcd_finalize, dim3(1), dim3(1024), 0, s, st);
+    KERNEL_CHECK();
+}
+
+void launch_verify_sums(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                        const double* beta, double reg, double* rowsum, double* colsum,
+                        hipStream_t s)
+{
+    if (bulk == 0) {
+        hipLaunchKernelGGL(k_verify_rows<float>, dim3(n), dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, m, alpha, beta, reg, rowsum);
+        hipLaunchKernelGGL(k_verify_cols<float>, dim3(ceil_div(m, 256)), dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, alpha, beta, reg, colsum);
+    } else {
+        hipLaunchKernelGGL(k_verify_rows<double>, dim3(n), dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, m, alpha, beta, reg, rowsum);
+        hipLaunchKernelGGL(k_verify_cols<double>, dim3(ceil_div(m, 256)), dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, n, m, alpha, beta, reg, colsum);
+    }
+    KERNEL_CHECK();
+}
+
+void launch_fill(double* p, double v, long count, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(count)), dim3(256), 0, s, p, v, count);
+    KERNEL_CHECK();
+}
+
+void launch_fill_int(int* p, int v, long count, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(count)), dim3(256), 0, s, p, v, count);
+    KERNEL_CHECK();
+}
+
+void launch_stamp(long long* dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, dst);
+    KERNEL_CHECK();
+}
+
+void launch_split_gamma(const double* gamma, int n, int m, double* alpha, double* beta,
+                        hipStream_t s)
+{
+    const int c = n > m ? n : m;
+    hipLaunchKernelGGL(k_split_gamma, dim3(ceil_div(c, 256)), dim3(256), 0, s, gamma, n, m,
+                       alpha, beta);
+    KERNEL_CHECK();
+}
+
+}  // namespace regot

@@ -1,0 +1,95 @@
+// problem.hpp -- the device-resident EROT problem behind the C-ABI handle.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "dense.hpp"
+
+struct regot_problem;
+
+namespace regot {
+
+struct SparseHessian;  // sparse.hpp
+
+// FP64 BLAS-1 helpers on device vectors (vec_kernels.hip).  Reductions run in
+// one workgroup in a fixed order, so they are reproducible.
+void vec_axpy(double* y, double alpha, const double* x, long count, hipStream_t s);
+void vec_axpby(double* out, double alpha, const double* x, double beta, const double* y,
+               long count, hipStream_t s);
+void vec_copy(double* dst, const double* src, long count, hipStream_t s);
+// out[0..npairs) = dot(x_i, y_i) for the listed pairs, one launch.
+void vec_dots(const double* const* xs, const double* const* ys, long count, int npairs,
+              double* out, hipStream_t s);
+// grad = [r - a; c[0:m-1] - b[0:m-1]], out[0] = reg sum r - alpha'a - betat'bt,
+// out[1] = ||grad||^2.
+void vec_objgrad_finish(const double* r, const double* c, const double* a, const double* b,
+                        const double* gamma, int n, int m, double reg, double* grad,
+                        double* out, hipStream_t s);
+// out[0] = reg sum r - alpha'a - betat'bt.
+void vec_obj_finish(const double* r, const double* a, const double* b, const double* gamma,
+                    int n, int m, double reg, double* out, hipStream_t s);
+
+struct ProfileSlot {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    long long launches = 0;
+    double total_ms = 0.0;
+};
+
+}  // namespace regot
+
+struct regot_problem {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n = 0, m = 0;
+    long ldm = 0;
+    int bulk = 0;               // 0 = FP32, 1 = FP64
+    void* dM = nullptr;
+    bool own_M = false;
+    double reg = 1.0;
+    double k = 1.0;             // log2(e) / reg
+    regot::DBuf<double> a, b, loga, logb;
+
+    // scratch for the operator entry points and the drivers
+    regot::DBuf<double> alpha, beta, rsum, csum, gamma, grad, part, scal;
+    regot::DBuf<int> ctl;
+    int nblk = 1;
+    regot::HBuf<double> hscal;
+    regot::HBuf<int> hctl;
+
+    // sparsified Hessian of the Newton-type drivers (sparse.hpp)
+    std::unique_ptr<regot::SparseHessian> hess;
+
+    // profiling of the hot passes: [0] row pass, [1] column pass
+    bool prof = false;
+    regot::ProfileSlot prof_slot[2];
+
+    regot_problem() = default;
+    regot_problem(const regot_problem&) = delete;
+    regot_problem& operator=(const regot_problem&) = delete;
+    ~regot_problem();
+
+    void setup(int n_, int m_, const void* M, int M_dtype, int bulk_, int M_device,
+               long long ldm_, const double* a_h, const double* b_h, double reg_, int device_);
+
+    // --- streaming passes (device pointers) ---
+    // Row pass over full-length column potential beta (m); see RowEpi.
+    void row_pass(const double* beta_full, const regot::RowEpi& epi, const int* ctl_p = nullptr);
+    // Column pass (partial + combine) at row potential alpha (n); see ColEpi.
+    void col_pass(const double* alpha_v, const double* colpot_guess, const regot::ColEpi& epi,
+                  const double* ap0 = nullptr, const double* ap1 = nullptr,
+                  const int* ctl_p = nullptr);
+
+    // f(gamma) for device gamma (n+m-1); row pass only.
+    double dev_obj(const double* d_gamma);
+    // f and grad (device, n+m-1) for device gamma; row and column passes.
+    // Row sums land in rsum, column sums in csum.  Returns f; *gnorm2 = ||g||^2.
+    double dev_obj_grad(const double* d_gamma, double* d_grad, double* gnorm2);
+
+    void sync();
+    void prof_begin(int kind, hipEvent_t* stop_out);
+    void prof_end(int kind, hipEvent_t stop);
+    void prof_collect();
+};

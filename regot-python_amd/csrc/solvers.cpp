@@ -1,0 +1,321 @@
+// solvers.cpp -- host C++ drivers of the EROT solvers and the solver-level
+// C-ABI.  The drivers own the control flow (tests, schedules, histories) in
+// FP64; every O(n m) step runs in the HIP kernels.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <deque>
+#include <vector>
+
+#include "problem.hpp"
+#include "regot_hip.h"
+#include "sparse.hpp"
+
+struct regot_result {
+    int solver = 0;
+    int n = 0, m = 0;
+    int niter = 0;
+    std::vector<double> dual, obj_vals, mar_errs, run_times, densities;
+};
+
+namespace regot {
+
+int set_error(int code, const char* what);
+
+namespace {
+
+// gamma = [alpha + beta_{m-1}; beta_{0..m-2} - beta_{m-1}]; src/sinkhorn_bcd.cpp:50-51
+void normalise_host(int n, int m, const double* alpha, const double* beta, double* gamma)
+{
+    for (int i = 0; i < n; i++) gamma[i] = alpha[i] + beta[m - 1];
+    for (int j = 0; j < m - 1; j++) gamma[n + j] = beta[j] - beta[m - 1];
+}
+
+// Pinned, device-mapped host ring that the finalize kernel writes directly.
+template <typename T>
+struct MappedRing {
+    T* h = nullptr;
+    T* d = nullptr;
+    int cap = 0;
+    explicit MappedRing(int c) : cap(c)
+    {
+        HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(T) * cap,
+                                hipHostMallocMapped));
+        HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&d), h, 0));
+    }
+    ~MappedRing()
+    {
+        if (h) (void)hipHostFree(h);
+    }
+    MappedRing(const MappedRing&) = delete;
+    MappedRing& operator=(const MappedRing&) = delete;
+};
+
+// Plain Sinkhorn / BCD with the whole loop on the GPU.
+//
+// Semantics follow sinkhorn_bcd_internal, src/sinkhorn_bcd.cpp:14-124: init
+// alpha = 0, beta = optimal_beta(alpha); at iterate k test ||g_k|| < tol first,
+// then alpha <- optimal_alpha(beta), beta <- optimal_beta(alpha).
+//
+// Each iteration is four launches on one stream:
+//   row pass     beta_k -> alpha_{k+1} (double buffer) and the row sums at
+//                iterate k, which give g_k's row part, f_k and ||g_k||;
+//   finalize     records f_k, ||g_k||, a timestamp; stops or advances k;
+//   column pass  alpha_{k+1} -> beta_{k+1}, plus the column residual of the
+//                update (g_{k+1}'s column part, zero up to FP64 rounding).
+// After the stop flag is set every later kernel returns at once, so the host
+// keeps two batches of iterations in flight and only polls the flag.
+void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const regot_opts& o,
+               regot_result* res)
+{
+    const int n = p->n, m = p->m;
+    hipStream_t s = p->stream;
+    const int K = o.graph_batch > 0 ? o.graph_batch : 8;
+    const int cap = std::max(4096, 4 * K + 8);
+
+    DBuf<double> ap0(n), ap1(n), beta(m), r(n), gcol(m);
+    DBuf<int> ctl(4);
+    MappedRing<double> hf(cap), hg(cap);
+    MappedRing<long long> hts(cap + 1);
+
+    if (o.x0) {
+        HIP_CHECK(hipMemcpyAsync(p->gamma, o.x0, (n + m - 1) * sizeof(double),
+                                 hipMemcpyHostToDevice, s));
+        launch_split_gamma(p->gamma, n, m, ap0, beta, s);
+        ColEpi c;
+        c.colpot = beta;
+        c.colsum = p->csum;
+        p->col_pass(ap0, beta, c);
+        vec_axpby(gcol, 1.0, p->csum, -1.0, p->b, m, s);
+    } else {
+        launch_fill(ap0, 0.0, n, s);
+        ColEpi c;
+        c.beta_out = beta;
+        c.resid = gcol;
+        p->col_pass(ap0, nullptr, c);
+    }
+    launch_fill_int(ctl, 0, 4, s);
+    launch_stamp(hts.d + cap, s);
+
+    BcdState st;
+    st.a = p->a;
+    st.b = p->b;
+    st.apair[0] = ap0;
+    st.apair[1] = ap1;
+    st.beta = beta;
+    st.rowsum = r;
+    st.gcol = gcol;
+    st.ctl = ctl;
+    st.hist_f = hf.d;
+    st.hist_g = hg.d;
+    st.hist_ts = hts.d;
+    st.hist_cap = cap;
+    st.tol = tol;
+    st.reg = p->reg;
+    st.max_iter = max_iter;
+    st.n = n;
+    st.m = m;
+
+    auto enqueue_iteration = [&]() {
+        RowEpi e;
+        e.loga = p->loga;
+        e.rowsum = r;
+        e.parity = true;
+        e.apair[0] = ap0;
+        e.apair[1] = ap1;
+        p->row_pass(beta, e, ctl);
+        launch_bcd_finalize(st, s);
+        ColEpi c;
+        c.beta_out = beta;
+        c.resid = gcol;
+        p->col_pass(nullptr, beta, c, ap0, ap1, ctl);
+    };
+
+    HBuf<int> hc;
+    hc.ensure(8);
+    hipEvent_t ev[2];
+    HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    auto launch_batch = [&](int slot) {
+        for (int it = 0; it < K; it++) enqueue_iteration();
+        HIP_CHECK(hipMemcpyAsync(hc.p + 4 * slot, ctl.p, 4 * sizeof(int), hipMemcpyDeviceToHost,
+                                 s));
+        HIP_CHECK(hipEventRecord(ev[slot], s));
+    };
+
+    int wall_khz = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, p->device));
+    const double tick_ms = wall_khz > 0 ? 1.0 / wall_khz : 0.0;
+
+    int h_read = 0, niter = 0;
+    long long ts_init = 0;
+    std::deque<int> q;
+    try {
+        launch_batch(0);
+        q.push_back(0);
+        launch_batch(1);
+        q.push_back(1);
+        for (;;) {
+            const int slot = q.front();
+            q.pop_front();
+            HIP_CHECK(hipEventSynchronize(ev[slot]));
+            const int* c = hc.p + 4 * slot;
+            const bool done = c[0] != 0;
+            const int avail = done ? c[2] + 1 : c[1];
+            if (h_read == 0 && avail > 0) ts_init = hts.h[cap];
+            for (; h_read < avail; h_read++) {
+                const int h = h_read % cap;
+                res->obj_vals.push_back(hf.h[h]);
+                res->mar_errs.push_back(hg.h[h]);
+                res->run_times.push_back(static_cast<double>(hts.h[h] - ts_init) * tick_ms);
+                if (verbose >= 1 && h_read < max_iter)
+                    printf("iter = %d, objval = %g, ||grad|| = %g\n", h_read, hf.h[h], hg.h[h]);
+            }
+            if (verbose >= 1) fflush(stdout);
+            if (done) {
+                niter = c[2];
+                break;
+            }
+            launch_batch(slot);
+            q.push_back(slot);
+        }
+        HIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        throw;
+    }
+    HIP_CHECK(hipEventDestroy(ev[0]));
+    HIP_CHECK(hipEventDestroy(ev[1]));
+
+    std::vector<double> ah(n), bh(m);
+    HIP_CHECK(hipMemcpy(ah.data(), (niter & 1) ? ap1.p : ap0.p, n * sizeof(double),
+                        hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(bh.data(), beta.p, m * sizeof(double), hipMemcpyDeviceToHost));
+    res->niter = niter;
+    res->dual.resize(n + m - 1);
+    normalise_host(n, m, ah.data(), bh.data(), res->dual.data());
+}
+
+}  // namespace
+}  // namespace regot
+
+using namespace regot;
+
+#define CAPI_BEGIN try {
+#define CAPI_END                                                                          \
+    }                                                                                     \
+    catch (const regot::Error& e) { return set_error(e.code, e.what()); }                 \
+    catch (const std::bad_alloc&) { return set_error(E_NOMEM, "host allocation failed"); } \
+    catch (const std::exception& e) { return set_error(E_INTERNAL, e.what()); }            \
+    return OK;
+
+extern "C" {
+
+void regot_opts_default(regot_opts* o, int solver)
+{
+    if (!o) return;
+    // SinkhornSolverOpts defaults, src/sinkhorn_solvers.h:35-37, with the
+    // per-solver method override of src/solvers.cpp:415,433,451.
+    o->method = (solver == REGOT_SOLVER_BCD) ? 1 : 1;
+    o->mu0 = 1.0;
+    o->shift = 1e-3;
+    o->density = 0.01;
+    o->cg_tol = 0.0;
+    o->cg_max_iter = 0;
+    o->x0 = nullptr;
+    o->graph_batch = 0;
+}
+
+int regot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,
+                const regot_opts* opts, regot_result** out)
+{
+    CAPI_BEGIN
+    if (!p) fail(E_INVALID, "null problem handle");
+    if (!out) fail(E_INVALID, "out is null");
+    *out = nullptr;
+    if (max_iter < 0) fail(E_INVALID, "max_iter must be >= 0");
+    HIP_CHECK(hipSetDevice(p->device));
+    regot_opts o;
+    if (opts)
+        o = *opts;
+    else
+        regot_opts_default(&o, solver);
+    auto res = std::make_unique<regot_result>();
+    res->solver = solver;
+    res->n = p->n;
+    res->m = p->m;
+    switch (solver) {
+        case REGOT_SOLVER_BCD:
+            solve_bcd(p, tol, max_iter, verbose, o, res.get());
+            break;
+        default:
+            fail(E_INVALID, "unknown solver id %d", solver);
+    }
+    *out = res.release();
+    CAPI_END
+}
+
+int regot_result_niter(const regot_result* r) { return r ? r->niter : -1; }
+
+int regot_result_hist_len(const regot_result* r)
+{
+    return r ? static_cast<int>(r->obj_vals.size()) : -1;
+}
+
+int regot_result_dens_len(const regot_result* r)
+{
+    return r ? static_cast<int>(r->densities.size()) : -1;
+}
+
+int regot_result_get(const regot_result* r, double* dual, double* obj_vals, double* mar_errs,
+                     double* run_times, double* densities)
+{
+    CAPI_BEGIN
+    if (!r) fail(E_INVALID, "null result");
+    auto cp = [](double* dst, const std::vector<double>& v) {
+        if (dst && !v.empty()) std::copy(v.begin(), v.end(), dst);
+    };
+    cp(dual, r->dual);
+    cp(obj_vals, r->obj_vals);
+    cp(mar_errs, r->mar_errs);
+    cp(run_times, r->run_times);
+    cp(densities, r->densities);
+    CAPI_END
+}
+
+int regot_result_plan(regot_problem* p, const regot_result* r, double* plan)
+{
+    if (!r) return set_error(E_INVALID, "null result");
+    if (!p || p->n != r->n || p->m != r->m)
+        return set_error(E_INVALID, "problem and result sizes differ");
+    return regot_get_plan(p, r->dual.data(), plan);
+}
+
+int regot_result_verify(regot_problem* p, const regot_result* r, double* mar_err_fp64,
+                        double* obj_fp64)
+{
+    CAPI_BEGIN
+    if (!p || !r) fail(E_INVALID, "null handle");
+    if (p->n != r->n || p->m != r->m) fail(E_INVALID, "problem and result sizes differ");
+    HIP_CHECK(hipSetDevice(p->device));
+    const int n = p->n, m = p->m;
+    hipStream_t s = p->stream;
+    HIP_CHECK(hipMemcpyAsync(p->gamma, r->dual.data(), (n + m - 1) * sizeof(double),
+                             hipMemcpyHostToDevice, s));
+    launch_split_gamma(p->gamma, n, m, p->alpha, p->beta, s);
+    launch_verify_sums(p->bulk, p->dM, p->ldm, n, m, p->alpha, p->beta, p->reg, p->rsum,
+                       p->csum, s);
+    vec_objgrad_finish(p->rsum, p->csum, p->a, p->b, p->gamma, n, m, p->reg, p->grad, p->scal,
+                       s);
+    HIP_CHECK(hipMemcpyAsync(p->hscal.p, p->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    p->sync();
+    if (obj_fp64) *obj_fp64 = p->hscal.p[0];
+    if (mar_err_fp64) *mar_err_fp64 = std::sqrt(p->hscal.p[1]);
+    CAPI_END
+}
+
+void regot_result_free(regot_result* r) { delete r; }
+
+}  // extern "C"

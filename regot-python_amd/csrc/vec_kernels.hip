@@ -1,0 +1,147 @@
+// vec_kernels.hip -- FP64 vector helpers for the host drivers.  Reductions use
+// one 1024-thread workgroup with a fixed tree, so every call is reproducible.
+#include "problem.hpp"
+
+namespace regot {
+namespace {
+
+__device__ __forceinline__ double block_reduce(double v, double* sm)
+{
+    const int t = threadIdx.x;
+    sm[t] = v;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (t < o) sm[t] += sm[t + o];
+        __syncthreads();
+    }
+    const double r = sm[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void k_axpby(double* out, double alpha, const double* x, double beta, const double* y,
+                        long count)
+{
+    for (long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; i < count;
+         i += static_cast<long>(gridDim.x) * 256)
+        out[i] = alpha * x[i] + (y ? beta * y[i] : 0.0);
+}
+
+__global__ void k_axpy(double* y, double alpha, const double* x, long count)
+{
+    for (long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; i < count;
+         i += static_cast<long>(gridDim.x) * 256)
+        y[i] += alpha * x[i];
+}
+
+struct DotArgs {
+    const double* x[8];
+    const double* y[8];
+};
+
+__global__ __launch_bounds__(1024) void k_dots(DotArgs d, long count, int npairs, double* out)
+{
+    __shared__ double sm[1024];
+    for (int p = 0; p < npairs; p++) {
+        double s = 0.0;
+        for (long i = threadIdx.x; i < count; i += 1024) s += d.x[p][i] * d.y[p][i];
+        s = block_reduce(s, sm);
+        if (threadIdx.x == 0) out[p] = s;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_objgrad_finish(const double* r, const double* c,
+                                                         const double* a, const double* b,
+                                                         const double* gamma, int n, int m,
+                                                         double reg, double* grad, double* out)
+{
+    __shared__ double sm[1024];
+    double tsum = 0.0, gn2 = 0.0, da = 0.0, db = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const double g = r[i] - a[i];
+        if (grad) grad[i] = g;
+        tsum += r[i];
+        gn2 += g * g;
+        da += gamma[i] * a[i];
+    }
+    for (int j = threadIdx.x; j < m - 1; j += 1024) {
+        if (c) {
+            const double g = c[j] - b[j];
+            if (grad) grad[n + j] = g;
+            gn2 += g * g;
+        }
+        db += gamma[n + j] * b[j];
+    }
+    tsum = block_reduce(tsum, sm);
+    gn2 = block_reduce(gn2, sm);
+    da = block_reduce(da, sm);
+    db = block_reduce(db, sm);
+    if (threadIdx.x == 0) {
+        out[0] = reg * tsum - da - db;
+        out[1] = gn2;
+    }
+}
+
+int grid_for(long count)
+{
+    long g = (count + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    return static_cast<int>(g);
+}
+
+}  // namespace
+
+void vec_axpy(double* y, double alpha, const double* x, long count, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_axpy, dim3(grid_for(count)), dim3(256), 0, s, y, alpha, x, count);
+    KERNEL_CHECK();
+}
+
+void vec_axpby(double* out, double alpha, const double* x, double beta, const double* y,
+               long count, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(count)), dim3(256), 0, s, out, alpha, x, beta, y,
+                       count);
+    KERNEL_CHECK();
+}
+
+void vec_copy(double* dst, const double* src, long count, hipStream_t s)
+{
+    if (count <= 0) return;
+    HIP_CHECK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToDevice, s));
+}
+
+void vec_dots(const double* const* xs, const double* const* ys, long count, int npairs,
+              double* out, hipStream_t s)
+{
+    if (npairs < 1 || npairs > 8) fail(E_INTERNAL, "vec_dots: %d pairs", npairs);
+    DotArgs d{};
+    for (int p = 0; p < npairs; p++) {
+        d.x[p] = xs[p];
+        d.y[p] = ys[p];
+    }
+    hipLaunchKernelGGL(k_dots, dim3(1), dim3(1024), 0, s, d, count, npairs, out);
+    KERNEL_CHECK();
+}
+
+void vec_objgrad_finish(const double* r, const double* c, const double* a, const double* b,
+                        const double* gamma, int n, int m, double reg, double* grad,
+                        double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_objgrad_finish, dim3(1), dim3(1024), 0, s, r, c, a, b, gamma, n, m, reg,
+                       grad, out);
+    KERNEL_CHECK();
+}
+
+void vec_obj_finish(const double* r, const double* a, const double* b, const double* gamma,
+                    int n, int m, double reg, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_objgrad_finish, dim3(1), dim3(1024), 0, s, r, nullptr, a, b, gamma, n,
+                       m, reg, nullptr, out);
+    KERNEL_CHECK();
+}
+
+}  // namespace regot

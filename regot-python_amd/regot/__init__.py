@@ -1,0 +1,304 @@
+"""regot -- MI355X (gfx950) drop-in for RegOT's entropic-OT solvers.
+
+Drop-in functions with the reference's signature and result object
+(yixuan/regot-python v0.0.3, src/solvers.cpp:507-528, :555-564)::
+
+    sinkhorn_bcd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs)
+    sinkhorn_ssns(...), sinkhorn_sparse_newton(...), sinkhorn_splr(...)
+
+Reference kwargs: ``method``, ``mu0``, ``shift``, ``density``
+(src/solvers.cpp:314-341); unknown kwargs are ignored, as there.
+
+Extensions (all optional):
+
+* ``bulk``: ``"fp32"`` or ``"fp64"``.  Default: ``"fp32"`` when ``M`` is a
+  float32 array, else ``"fp64"``.  FP32 bulk keeps M in FP32 and runs the
+  exp/sum in FP32 with every potential, objective and test in FP64; FP64 bulk
+  runs everything in FP64.
+* ``x0``: warm start of length n+m-1 (the reference's ``SinkhornSolverOpts::x0``).
+* ``cg_tol``, ``cg_max_iter``: CG controls of the Newton-type solvers.
+* ``device``: GPU ordinal; ``compute_plan``: form ``plan`` (default True).
+
+Every computation runs in libregot_hip.so on the GPU; there is no CPU path.
+"""
+import numpy as np
+
+from . import _lib
+from ._lib import RegotError, check, dptr, lib
+
+__all__ = [
+    "sinkhorn_bcd", "sinkhorn_ssns", "sinkhorn_sparse_newton", "sinkhorn_splr",
+    "sinkhorn_result", "Problem", "RegotError",
+]
+
+__version__ = "0.1.0"
+
+
+class sinkhorn_result:
+    """Result object with the fields of the reference's ``sinkhorn_result``.
+
+    As in the reference (pybind11 ``def_readwrite``), each attribute read
+    returns a fresh copy: ``plan`` a C-contiguous float64 (n, m) array,
+    ``dual`` a float64 (n+m-1,) array, the histories Python lists of float.
+    """
+
+    __slots__ = ("_niter", "_dual", "_plan", "_obj_vals", "_mar_errs", "_run_times",
+                 "_densities", "mar_err_fp64", "obj_fp64")
+
+    def __init__(self):
+        self._niter = 0
+        self._dual = np.zeros(0)
+        self._plan = np.zeros((0, 0))
+        self._obj_vals = []
+        self._mar_errs = []
+        self._run_times = []
+        self._densities = []
+        self.mar_err_fp64 = None
+        self.obj_fp64 = None
+
+    def _prop(name, kind):
+        attr = "_" + name
+
+        def get(self):
+            v = getattr(self, attr)
+            if kind == "arr":
+                return np.array(v, dtype=np.float64, copy=True)
+            if kind == "list":
+                return list(v)
+            return v
+
+        def put(self, v):
+            if kind == "arr":
+                v = np.array(v, dtype=np.float64, copy=True)
+            elif kind == "list":
+                v = [float(x) for x in v]
+            else:
+                v = int(v)
+            setattr(self, attr, v)
+
+        return property(get, put)
+
+    niter = _prop("niter", "int")
+    dual = _prop("dual", "arr")
+    plan = _prop("plan", "arr")
+    obj_vals = _prop("obj_vals", "list")
+    mar_errs = _prop("mar_errs", "list")
+    run_times = _prop("run_times", "list")
+    densities = _prop("densities", "list")
+    del _prop
+
+    def __repr__(self):
+        return "<regot.sinkhorn_result niter=%d>" % self._niter
+
+
+def _as_bulk(M, bulk):
+    M = np.asarray(M)
+    if M.ndim != 2:
+        raise ValueError("M must be a 2-D array (got shape %s)" % (M.shape,))
+    if bulk is None:
+        bulk = "fp32" if M.dtype == np.float32 else "fp64"
+    if bulk not in ("fp32", "fp64"):
+        raise ValueError("bulk must be 'fp32' or 'fp64'")
+    dt = np.float32 if bulk == "fp32" else np.float64
+    return np.ascontiguousarray(M, dtype=dt), bulk
+
+
+class Problem:
+    """One EROT problem resident on a GPU (``regot_problem`` handle).
+
+    Operators mirror ``Sinkhorn::Problem`` (src/sinkhorn_problem.h:9-147).
+    """
+
+    def __init__(self, M, a, b, reg, bulk=None, device=0):
+        Mc, bulk = _as_bulk(M, bulk)
+        n, m = Mc.shape
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+        if a.shape[0] != n or b.shape[0] != m:
+            raise ValueError("a must have %d entries and b %d (got %d, %d)"
+                             % (n, m, a.shape[0], b.shape[0]))
+        self.n, self.m, self.reg, self.bulk = n, m, float(reg), bulk
+        code = _lib.REGOT_F32 if bulk == "fp32" else _lib.REGOT_F64
+        h = _lib._vp()
+        check(lib().regot_problem_create(
+            _lib.ctypes.byref(h), n, m, Mc.ctypes.data, code, code, 0, 0, dptr(a), dptr(b),
+            float(reg), int(device)))
+        self._h = h
+
+    @classmethod
+    def from_device(cls, ptr, n, m, dtype, a, b, reg, bulk=None, ldm=0, device=0):
+        """Borrow (or convert) a device-resident M given by raw pointer."""
+        self = cls.__new__(cls)
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+        dcode = _lib.REGOT_F32 if np.dtype(dtype) == np.float32 else _lib.REGOT_F64
+        bulk = bulk or ("fp32" if dcode == _lib.REGOT_F32 else "fp64")
+        bcode = _lib.REGOT_F32 if bulk == "fp32" else _lib.REGOT_F64
+        self.n, self.m, self.reg, self.bulk = n, m, float(reg), bulk
+        h = _lib._vp()
+        check(lib().regot_problem_create(_lib.ctypes.byref(h), n, m, _lib._vp(ptr), dcode, bcode,
+                                         1, int(ldm), dptr(a), dptr(b), float(reg), int(device)))
+        self._h = h
+        return self
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().regot_problem_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- operators -------------------------------------------------------
+    def optimal_alpha(self, beta):
+        """src/sinkhorn_problem_bcd.cpp:177-201"""
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        out = np.empty(self.n)
+        check(lib().regot_optimal_alpha(self._h, dptr(beta), dptr(out)))
+        return out
+
+    def optimal_beta(self, alpha):
+        """src/sinkhorn_problem_bcd.cpp:149-174"""
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        out = np.empty(self.m)
+        check(lib().regot_optimal_beta(self._h, dptr(alpha), dptr(out)))
+        return out
+
+    def dual_obj(self, gamma):
+        """src/sinkhorn_problem_obj.cpp:324-342"""
+        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        f = _lib.ctypes.c_double()
+        check(lib().regot_dual_obj(self._h, dptr(gamma), _lib.ctypes.byref(f)))
+        return f.value
+
+    def dual_obj_grad(self, gamma):
+        """src/sinkhorn_problem_obj.cpp:363-393 -> (f, grad)"""
+        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        f = _lib.ctypes.c_double()
+        g = np.empty(self.n + self.m - 1)
+        check(lib().regot_dual_obj_grad(self._h, dptr(gamma), _lib.ctypes.byref(f), dptr(g)))
+        return f.value, g
+
+    def get_plan(self, gamma):
+        """src/sinkhorn_result.h:32-50"""
+        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        plan = np.empty((self.n, self.m))
+        check(lib().regot_get_plan(self._h, dptr(gamma), dptr(plan)))
+        return plan
+
+    def profile(self, on=True):
+        check(lib().regot_profile_enable(self._h, int(bool(on))))
+
+    def profile_stats(self, kind):
+        """(launches, total ms) of kind 0 = row pass, 1 = column pass."""
+        cnt = _lib.ctypes.c_longlong()
+        ms = _lib.ctypes.c_double()
+        check(lib().regot_profile_get(self._h, int(kind), _lib.ctypes.byref(cnt),
+                                      _lib.ctypes.byref(ms)))
+        return cnt.value, ms.value
+
+    def profile_reset(self):
+        check(lib().regot_profile_reset(self._h))
+
+    # --- solvers ---------------------------------------------------------
+    def solve(self, solver, tol=1e-6, max_iter=1000, verbose=0, compute_plan=True,
+              verify=False, **kwargs):
+        opts = _lib.Opts()
+        lib().regot_opts_default(_lib.ctypes.byref(opts), solver)
+        if "method" in kwargs:
+            opts.method = int(kwargs["method"])
+        if "mu0" in kwargs:
+            opts.mu0 = float(kwargs["mu0"])
+        if "shift" in kwargs:
+            opts.shift = float(kwargs["shift"])
+        if "density" in kwargs:
+            opts.density = float(kwargs["density"])
+        if "cg_tol" in kwargs:
+            opts.cg_tol = float(kwargs["cg_tol"])
+        if "cg_max_iter" in kwargs:
+            opts.cg_max_iter = int(kwargs["cg_max_iter"])
+        if "graph_batch" in kwargs:
+            opts.graph_batch = int(kwargs["graph_batch"])
+        x0 = kwargs.get("x0")
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)
+            if x0.shape[0] != self.n + self.m - 1:
+                raise ValueError("x0 must have n+m-1 = %d entries" % (self.n + self.m - 1))
+            opts.x0 = dptr(x0)
+        r = _lib._vp()
+        check(lib().regot_solve(self._h, int(solver), float(tol), int(max_iter), int(verbose),
+                                _lib.ctypes.byref(opts), _lib.ctypes.byref(r)))
+        try:
+            L = lib()
+            res = sinkhorn_result()
+            hl = L.regot_result_hist_len(r)
+            dl = L.regot_result_dens_len(r)
+            dual = np.empty(self.n + self.m - 1)
+            h = [np.empty(max(hl, 1)) for _ in range(3)]
+            dens = np.empty(max(dl, 1))
+            check(L.regot_result_get(r, dptr(dual), dptr(h[0]), dptr(h[1]), dptr(h[2]),
+                                     dptr(dens)))
+            res._niter = int(L.regot_result_niter(r))
+            res._dual = dual
+            res._obj_vals = [float(x) for x in h[0][:hl]]
+            res._mar_errs = [float(x) for x in h[1][:hl]]
+            res._run_times = [float(x) for x in h[2][:hl]]
+            res._densities = [float(x) for x in dens[:dl]]
+            if compute_plan:
+                plan = np.empty((self.n, self.m))
+                check(L.regot_result_plan(self._h, r, dptr(plan)))
+                res._plan = plan
+            if verify:
+                me = _lib.ctypes.c_double()
+                ob = _lib.ctypes.c_double()
+                check(L.regot_result_verify(self._h, r, _lib.ctypes.byref(me),
+                                            _lib.ctypes.byref(ob)))
+                res.mar_err_fp64, res.obj_fp64 = me.value, ob.value
+        finally:
+            lib().regot_result_free(r)
+        return res
+
+
+_HANDLE_KW = ("bulk", "device", "compute_plan", "verify")
+
+
+def _run(solver, M, a, b, reg, tol, max_iter, verbose, kwargs):
+    rest = {k: v for k, v in kwargs.items() if k not in _HANDLE_KW}
+    with Problem(M, a, b, reg, bulk=kwargs.get("bulk"), device=kwargs.get("device", 0)) as p:
+        return p.solve(solver, tol=tol, max_iter=max_iter, verbose=verbose,
+                       compute_plan=kwargs.get("compute_plan", True),
+                       verify=kwargs.get("verify", False), **rest)
+
+
+def sinkhorn_bcd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Plain Sinkhorn (block coordinate descent on the dual).
+
+    Drop-in for ``regot.sinkhorn_bcd`` (src/solvers.cpp:359-373,
+    src/sinkhorn_bcd.cpp:14-124).
+    """
+    return _run(_lib.SOLVER_BCD, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def sinkhorn_ssns(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Safe and sparse Newton (src/solvers.cpp:425-441, src/sinkhorn_ssns.cpp:16-234)."""
+    return _run(_lib.SOLVER_SSNS, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def sinkhorn_sparse_newton(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Sparse Newton (src/solvers.cpp:407-423, src/sinkhorn_sparse_newton.cpp:24-170)."""
+    return _run(_lib.SOLVER_SPARSE_NEWTON, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def sinkhorn_splr(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Sparse-plus-low-rank quasi-Newton (src/solvers.cpp:443-459, src/sinkhorn_splr.cpp:24-199)."""
+    return _run(_lib.SOLVER_SPLR, M, a, b, reg, tol, max_iter, verbose, kwargs)

@@ -1,0 +1,94 @@
+"""ctypes binding of libregot_hip.so (the C-ABI declared in include/regot_hip.h).
+
+The shared library is built in-tree next to this file by
+``regot-python_amd/build.py``.  There is no CPU fallback: if the library or a
+GPU is missing, every call raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libregot_hip.so")
+
+REGOT_F32 = 0
+REGOT_F64 = 1
+SOLVER_BCD = 0
+SOLVER_SSNS = 1
+SOLVER_SPARSE_NEWTON = 2
+SOLVER_SPLR = 3
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+class RegotError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+
+class Opts(ctypes.Structure):
+    """Mirror of regot_opts (include/regot_hip.h)."""
+    _fields_ = [
+        ("method", ctypes.c_int),
+        ("mu0", ctypes.c_double),
+        ("shift", ctypes.c_double),
+        ("density", ctypes.c_double),
+        ("cg_tol", ctypes.c_double),
+        ("cg_max_iter", ctypes.c_int),
+        ("x0", _dp),
+        ("graph_batch", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load the library once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RegotError(
+            "libregot_hip.so is not built (expected at %s); run "
+            "`python regot-python_amd/build.py` or __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    L.regot_last_error.restype = ctypes.c_char_p
+    L.regot_problem_create.argtypes = [
+        ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.c_longlong, _dp, _dp, ctypes.c_double, ctypes.c_int]
+    L.regot_problem_destroy.argtypes = [_vp]
+    L.regot_problem_dims.argtypes = [_vp, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    for name in ("regot_optimal_alpha", "regot_optimal_beta"):
+        getattr(L, name).argtypes = [_vp, _dp, _dp]
+    L.regot_dual_obj.argtypes = [_vp, _dp, _dp]
+    L.regot_dual_obj_grad.argtypes = [_vp, _dp, _dp, _dp]
+    L.regot_get_plan.argtypes = [_vp, _dp, _dp]
+    L.regot_opts_default.argtypes = [ctypes.POINTER(Opts), ctypes.c_int]
+    L.regot_opts_default.restype = None
+    L.regot_solve.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                              ctypes.POINTER(Opts), ctypes.POINTER(_vp)]
+    for name in ("regot_result_niter", "regot_result_hist_len", "regot_result_dens_len"):
+        getattr(L, name).argtypes = [_vp]
+    L.regot_result_get.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
+    L.regot_result_plan.argtypes = [_vp, _vp, _dp]
+    L.regot_result_verify.argtypes = [_vp, _vp, _dp, _dp]
+    L.regot_result_free.argtypes = [_vp]
+    L.regot_result_free.restype = None
+    L.regot_profile_enable.argtypes = [_vp, ctypes.c_int]
+    L.regot_profile_get.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _dp]
+    L.regot_profile_reset.argtypes = [_vp]
+    _lib = L
+    return L
+
+
+def check(status):
+    """Raise RegotError with the library's message on a negative status."""
+    if status < 0:
+        msg = lib().regot_last_error().decode("utf-8", "replace")
+        raise RegotError("libregot_hip error %d: %s" % (status, msg))
+    return status
+
+
+def dptr(arr):
+    return arr.ctypes.data_as(_dp)

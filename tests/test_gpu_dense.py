@@ -1,0 +1,124 @@
+"""GPU parity of the dense passes and the BCD solver against the FP64 oracle.
+
+Tolerances: FP64 bulk 1e-10 relative, FP32 bulk 1e-6 relative (scale =
+max(1, |reference|_inf)), as the port's correctness bar states.
+"""
+import numpy as np
+import pytest
+
+from conftest import cloud_problem, readme_example
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-10, "fp32": 1e-6}
+
+
+def rel_err(x, ref):
+    x, ref = np.asarray(x), np.asarray(ref)
+    return np.max(np.abs(x - ref)) / max(1.0, np.max(np.abs(ref)))
+
+
+def problem_as(M, bulk):
+    """The oracle solves the same problem: for FP32 bulk, M rounded to FP32."""
+    if bulk == "fp32":
+        Mc = M.astype(np.float32)
+        return Mc, Mc.astype(np.float64)
+    return M, M
+
+
+@pytest.mark.parametrize("bulk", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape,reg", [((300, 257), 0.1), ((129, 700), 0.01), ((64, 64), 1e-3)])
+def test_half_steps(gpu_lib, bulk, shape, reg):
+    n, m = shape
+    M, a, b = cloud_problem(n, m, seed=n + m)
+    Mg, Mo = problem_as(M, bulk)
+    rng = np.random.default_rng(1)
+    with gpu_lib.Problem(Mg, a, b, reg) as P:
+        beta = rng.standard_normal(m) * 0.3
+        alpha = P.optimal_alpha(beta)
+        assert rel_err(alpha, O.optimal_alpha(Mo, a, reg, beta)) < TOL[bulk]
+        alpha0 = np.zeros(n)
+        beta0 = P.optimal_beta(alpha0)
+        assert rel_err(beta0, O.optimal_beta(Mo, b, reg, alpha0)) < TOL[bulk]
+        # huge offsets: exercises the adaptive shift in both directions
+        beta_off = beta + 1e3 * reg
+        alpha2 = P.optimal_alpha(beta_off)
+        assert rel_err(alpha2, O.optimal_alpha(Mo, a, reg, beta_off)) < TOL[bulk]
+        alpha_off = alpha - 5e2 * reg
+        beta2 = P.optimal_beta(alpha_off)
+        assert rel_err(beta2, O.optimal_beta(Mo, b, reg, alpha_off)) < TOL[bulk]
+
+
+@pytest.mark.parametrize("bulk", ["fp64", "fp32"])
+def test_obj_grad_plan(gpu_lib, bulk):
+    n, m, reg = 200, 333, 0.05
+    M, a, b = cloud_problem(n, m, seed=7)
+    Mg, Mo = problem_as(M, bulk)
+    with gpu_lib.Problem(Mg, a, b, reg) as P:
+        alpha = np.zeros(n)
+        beta = O.optimal_beta(Mo, b, reg, alpha)
+        gamma = np.concatenate([alpha + beta[-1], beta[:-1] - beta[-1]])
+        gamma += np.random.default_rng(3).standard_normal(n + m - 1) * 0.01
+        f = P.dual_obj(gamma)
+        fo = O.dual_obj(Mo, a, b, reg, gamma)
+        assert abs(f - fo) / max(1.0, abs(fo)) < TOL[bulk]
+        f2, g = P.dual_obj_grad(gamma)
+        fo2, go = O.dual_obj_grad(Mo, a, b, reg, gamma)
+        assert abs(f2 - fo2) / max(1.0, abs(fo2)) < TOL[bulk]
+        # gradient entries are O(a_i): compare against the marginal scale
+        assert np.max(np.abs(g - go)) < TOL[bulk] * max(a.max(), b.max()) * 10
+        plan = P.get_plan(gamma)
+        po = O.get_plan(Mo, reg, gamma)
+        assert np.max(np.abs(plan - po) / np.maximum(po, 1e-300)) < 1e-12
+
+
+def test_bcd_readme_fp64_matches_oracle_history(gpu_lib):
+    M, a, b = readme_example()
+    r = gpu_lib.sinkhorn_bcd(M, a, b, 0.1, tol=1e-9, max_iter=2000)
+    o = O.sinkhorn_bcd(M, a, b, 0.1, tol=1e-9, max_iter=2000)
+    assert abs(r.niter - o["niter"]) <= 1
+    assert rel_err(r.dual, o["dual"]) < 1e-10
+    k = min(len(r.obj_vals), len(o["obj_vals"]))
+    assert len(r.obj_vals) == r.niter + 1 == len(r.mar_errs) == len(r.run_times)
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(r.mar_errs[:k], o["mar_errs"][:k], rtol=1e-6, atol=1e-14)
+    assert r.mar_errs[-1] < 1e-9
+    plan = r.plan
+    assert plan.shape == (100, 80) and plan.flags["C_CONTIGUOUS"]
+    np.testing.assert_allclose(plan, O.get_plan(M, 0.1, o["dual"]), rtol=1e-8, atol=1e-15)
+    assert r.densities == []
+    assert all(np.diff(r.run_times) >= 0)
+
+
+def test_bcd_fp32_cloud(gpu_lib):
+    n, m, reg = 1000, 900, 0.05
+    M, a, b = cloud_problem(n, m, seed=11)
+    M32 = M.astype(np.float32)
+    r = gpu_lib.sinkhorn_bcd(M32, a, b, reg, tol=1e-8, max_iter=5000, verify=True)
+    o = O.sinkhorn_bcd(M32.astype(np.float64), a, b, reg, tol=1e-8, max_iter=5000)
+    assert r.niter < 5000
+    assert abs(r.niter - o["niter"]) <= max(3, 0.02 * o["niter"])
+    assert rel_err(r.dual, o["dual"]) < 1e-6
+    assert r.mar_err_fp64 < 1e-7
+    po = O.get_plan(M32.astype(np.float64), reg, o["dual"])
+    assert np.max(np.abs(r.plan - po)) < 1e-6 * po.max()
+
+
+def test_bcd_edge_cases(gpu_lib):
+    M, a, b = readme_example(7, 5)
+    # immediate convergence: huge tol -> niter 0, one history entry
+    r = gpu_lib.sinkhorn_bcd(M, a, b, 0.1, tol=1e9)
+    assert r.niter == 0 and len(r.obj_vals) == 1
+    # max_iter exhaustion
+    r = gpu_lib.sinkhorn_bcd(M, a, b, 0.1, tol=0.0, max_iter=17)
+    o = O.sinkhorn_bcd(M, a, b, 0.1, tol=0.0, max_iter=17)
+    assert r.niter == 17 and len(r.mar_errs) == 18
+    assert rel_err(r.dual, o["dual"]) < 1e-10
+    # single row / single column problems
+    for shape in [(1, 6), (6, 1), (1, 1)]:
+        Mx, ax, bx = readme_example(*shape)
+        r = gpu_lib.sinkhorn_bcd(Mx, ax, bx, 0.1, tol=1e-10, max_iter=50)
+        o = O.sinkhorn_bcd(Mx, ax, bx, 0.1, tol=1e-10, max_iter=50)
+        assert r.niter == o["niter"]
+        assert rel_err(r.dual, o["dual"]) < 1e-10
