@@ -101,7 +101,9 @@ __device__ __forceinline__ bool reshift(double& sh, TA& s, TA& rm)
     if (rm > TA(1) || (rm < TA(-8) && rm > ninf<TA>())) {
         const TA d = rm;
         sh += static_cast<double>(d);
-        s *= ex2(-d);
+        // Rescale in FP64: on a downward move 2^-d alone can overflow the bulk
+        // type even though s * 2^-d cannot (s holds terms <= 2^d).
+        if (s != TA(0)) s = static_cast<TA>(static_cast<double>(s) * exp2(-static_cast<double>(d)));
         rm = TA(0);
         return true;
     }
