@@ -120,6 +120,34 @@ struct HBuf {
     }
 };
 
+// Pinned, device-mapped host array: kernels write it, the host reads it after
+// the writing work has completed (stream order + event).
+template <typename T>
+struct MappedRing {
+    T* h = nullptr;
+    T* d = nullptr;
+    int cap = 0;
+    MappedRing() = default;
+    MappedRing(const MappedRing&) = delete;
+    MappedRing& operator=(const MappedRing&) = delete;
+    ~MappedRing() { release(); }
+    void ensure(int c)
+    {
+        if (h && c <= cap) return;
+        release();
+        HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(T) * c, hipHostMallocMapped));
+        HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&d), h, 0));
+        cap = c;
+    }
+    void release()
+    {
+        if (h) (void)hipHostFree(h);
+        h = nullptr;
+        d = nullptr;
+        cap = 0;
+    }
+};
+
 inline int ceil_div(long a, long b) { return static_cast<int>((a + b - 1) / b); }
 
 }  // namespace regot

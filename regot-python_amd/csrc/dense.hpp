@@ -28,7 +28,7 @@ struct RowEpi {
     double* apair[2] = {nullptr, nullptr};
 };
 
-// Per-column epilogue of the column combine.
+// Per-column epilogue of the column pass, run by the combine kernel.
 struct ColEpi {
     double* lse2 = nullptr;          // log2 sum_i 2^((alpha_i - M_ij) k)
     double* beta_out = nullptr;      // reg * (log b_j - ln2 * lse2)
@@ -36,7 +36,13 @@ struct ColEpi {
     const double* b = nullptr;
     const double* colpot = nullptr;  // beta used for column sums / the shift guess
     double* colsum = nullptr;        // sum_i exp((alpha_i + beta_j - M_ij)/reg)
-    double* resid = nullptr;         // b_j * expm1(beta_out_j/reg + ln2 lse2 - log b_j)
+    // Column gradient: with beta_out, b_j * expm1(beta_out_j/reg + ln2 lse2 - log b_j)
+    // (the FP64 rounding residual of the update); otherwise colsum_j - b_j.
+    double* resid = nullptr;
+    // Per combine workgroup t (j < m-1 only): [2t] = sum resid_j^2,
+    // [2t+1] = sum beta_j b_j with beta = beta_out if set, else colpot.
+    // In BCD mode the half selected by the parity of k+1 is written.
+    double* tilepart = nullptr;
     double reg = 1.0;
 };
 
@@ -44,41 +50,52 @@ struct ColEpi {
 void launch_row_pass(int bulk, const void* M, long ldm, int n, int m, const double* beta,
                      double k, const RowEpi& epi, const int* ctl, hipStream_t s);
 
-// Column pass, stage 1: partial log2-sums per (row block, column) into part.
-// Returns the number of row blocks used (size of part must be nblk * m).
+struct BcdState;
+
+// Column pass: partial log2-sums per (row block, column) into part (nblk * m
+// doubles), then a combine kernel (4 threads per column) with the epilogue.
+// With st, the combine also takes the BCD stop decision (see BcdState).
 int col_pass_blocks(int n, int m);
-void launch_col_partial(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
-                        const double* alpha_pair0, const double* alpha_pair1,
-                        const double* colpot_guess, const double* logb, double k,
-                        double* part, int nblk, const int* ctl, hipStream_t s);
-// Column pass, stage 2: combine the partials and run the epilogue.
-void launch_col_combine(const double* part, int nblk, int m, double k, const ColEpi& epi,
-                        const int* ctl, hipStream_t s);
+int col_combine_blocks(int m);
+void launch_col_pass(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                     const double* alpha_pair0, const double* alpha_pair1,
+                     const double* colpot_guess, const double* logb, double k, double* part,
+                     int nblk, const ColEpi& epi, const BcdState* st, const int* ctl,
+                     hipStream_t s);
 
 // FP64 plan: plan_ij = exp((alpha_i + beta_j - M_ij) / reg) for rows [r0, r0+nr).
 void launch_plan(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
                  const double* beta, double reg, double* plan, long r0, int nr, hipStream_t s);
 
-// On-device BCD bookkeeping (single workgroup): objective, ||grad||, history,
-// convergence test.  See dense_kernels.hip for the exact update order.
+// On-device BCD bookkeeping.  The row pass (FIN) publishes per-workgroup sums
+// of (g^2, r, alpha a) and copies k into ctl[3]; every combine workgroup of
+// the same iteration then reduces them (with the column-tile sums of the
+// previous update, double-buffered by the parity of k) to the same f_k and
+// ||g_k|| and takes the same stop decision.
 struct BcdState {
     const double* a;
-    const double* b;
     double* apair[2];
     const double* beta;
-    const double* rowsum;
-    const double* gcol;
-    int* ctl;          // [0] done, [1] k, [2] niter
-    double* hist_f;
+    int* ctl;               // [0] done, [1] k, [2] niter, [3] k as seen by the row pass
+    double* hist_f;         // rings of hist_cap entries (device-visible)
     double* hist_g;
     long long* hist_ts;
-    int hist_cap;      // history arrays are rings of this many entries
+    int hist_cap;
+    double* rpart;          // per row-pass workgroup: sum g^2, sum r, sum alpha a
+    const double* cpart;    // 2 x ntiles x (sum resid^2, sum beta_j b_j), j < m-1
+    int nrb;                // row-pass workgroups
+    int ntiles;             // combine workgroups
+    double sum_a;           // sum_i a_i
+    double sum_b1;          // sum_{j < m-1} b_j
     double tol;
     double reg;
     int max_iter;
     int n, m;
 };
-void launch_bcd_finalize(const BcdState& st, hipStream_t s);
+// Row pass publishing the per-workgroup BCD sums (FIN variant).
+void launch_row_pass_bcd(int bulk, const void* M, long ldm, int n, int m, const double* beta,
+                         double k, const RowEpi& epi, const BcdState& st, hipStream_t s);
+int row_pass_blocks(int bulk, int n);
 
 // FP64-exp row and column sums of T at (alpha, beta full-length), for the
 // verification of a result independent of the bulk precision.

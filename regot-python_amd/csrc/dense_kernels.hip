@@ -94,16 +94,24 @@ __device__ __forceinline__ double acc_log2(double sh, TA s)
     return (s > TA(0)) ? sh + log2(static_cast<double>(s)) : -INFINITY;
 }
 
-// Move the shift when the running max leaves [-8, 1].  Returns true if moved.
+// Move the shift when the running max leaves [-8, 8]: values up to 2^8 above
+// the shift keep the bulk-type rounding of the exponent argument below
+// 2^-16 ulp-units (~2e-7 relative), and a lane moves at most once downwards
+// (first values) and rarely upwards.  Returns true if moved.
 template <typename TA>
 __device__ __forceinline__ bool reshift(double& sh, TA& s, TA& rm)
 {
-    if (rm > TA(1) || (rm < TA(-8) && rm > ninf<TA>())) {
+    if (rm > TA(8) || (rm < TA(-8) && rm > ninf<TA>())) {
         const TA d = rm;
         sh += static_cast<double>(d);
-        // Rescale in FP64: on a downward move 2^-d alone can overflow the bulk
-        // type even though s * 2^-d cannot (s holds terms <= 2^d).
-        if (s != TA(0)) s = static_cast<TA>(static_cast<double>(s) * exp2(-static_cast<double>(d)));
+        if (s != TA(0)) {
+            // s holds terms <= 2^d, so s * 2^-d is representable; only the factor
+            // 2^-d alone can overflow (d < -126), which needs the FP64 route.
+            if (d > TA(-120))
+                s *= ex2(-d);
+            else
+                s = static_cast<TA>(static_cast<double>(s) * exp2(-static_cast<double>(d)));
+        }
         rm = TA(0);
         return true;
     }
@@ -142,14 +150,44 @@ __device__ __forceinline__ void row_accumulate(double& sh, TM& s, TM& rm, const 
     for (int v = 0; v < V; v++) s += ex2(t[v]);
 }
 
+__device__ __forceinline__ double ld_agent(const double* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sums of NV values over the 256 threads of the workgroup: butterfly within
+// each wave, then the four wave totals in fixed order.  Every thread gets the
+// same (bitwise) results; one barrier pair per call.
+template <int NV>
+__device__ __forceinline__ void block_sums_256(double (&v)[NV])
+{
+    __shared__ double sm[4][NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double x = v[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+        v[q] = __shfl(x, 0);
+    }
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) sm[wave][q] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] = ((sm[0][q] + sm[1][q]) + sm[2][q]) + sm[3][q];
+}
+
 // Each wave owns R consecutive rows and streams them left to right, lane l
 // reading the V-wide vector at column c*64V + l*V of each row (1 KiB per row
 // per step, fully coalesced).  The beta chunk is loaded once per step and
-// reused by all R rows.
-template <typename TM, int R>
+// reused by all R rows.  With FIN, per-workgroup sums of (g^2, r, alpha a) are
+// published and the last workgroup to arrive runs the BCD bookkeeping.
+template <typename TM, int R, bool FIN>
 __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long ldm, int n,
                                                   int m, const double* __restrict__ beta,
-                                                  double k, RowEpi epi,
+                                                  double k, RowEpi epi, BcdState fin,
                                                   const int* __restrict__ ctl)
 {
     using VT = typename VecOf<TM>::type;
@@ -158,7 +196,6 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const long row0 = (static_cast<long>(blockIdx.x) * 4 + wave) * R;
-    if (row0 >= n) return;
 
     const double* rowpot = epi.rowpot;
     double* aout = epi.alpha_out;
@@ -167,68 +204,99 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
         rowpot = epi.apair[kk & 1];
         aout = epi.apair[(kk + 1) & 1];
     }
+    double wg2 = 0.0, wr = 0.0, waa = 0.0;
 
-    double sh[R];
-    TM s[R], rm[R];
+    if (row0 < n) {
+        double sh[R];
+        TM s[R], rm[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const long i = row0 + r;
-        double g = 0.0;
-        if (i < n && epi.loga != nullptr) {
-            g = epi.loga[i] * kLog2e;
-            if (rowpot != nullptr) g -= rowpot[i] * k;
+        for (int r = 0; r < R; r++) {
+            const long i = row0 + r;
+            double g = 0.0;
+            if (i < n && epi.loga != nullptr) {
+                g = epi.loga[i] * kLog2e;
+                if (rowpot != nullptr) g -= rowpot[i] * k;
+            }
+            sh[r] = g;
+            s[r] = TM(0);
+            rm[r] = ninf<TM>();
         }
-        sh[r] = g;
-        s[r] = TM(0);
-        rm[r] = ninf<TM>();
-    }
-    const TM* rowp[R];
+        const TM* rowp[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) rowp[r] = M + (row0 + r < n ? row0 + r : row0) * ldm;
+        for (int r = 0; r < R; r++) rowp[r] = M + (row0 + r < n ? row0 + r : row0) * ldm;
 
-    const int step = 64 * V;
-    const int nfull = m / step;
+        const int step = 64 * V;
+        const int nfull = m / step;
 #pragma unroll 2
-    for (int c = 0; c < nfull; c++) {
-        const int j0 = c * step + lane * V;
-        double bk[V];
-#pragma unroll
-        for (int v = 0; v < V; v++) bk[v] = beta[j0 + v] * k;
-        VT mv[R];
-#pragma unroll
-        for (int r = 0; r < R; r++)
-            if (row0 + r < n) mv[r] = *reinterpret_cast<const VT*>(rowp[r] + j0);
-#pragma unroll
-        for (int r = 0; r < R; r++)
-            if (row0 + r < n) row_accumulate<TM, V, false>(sh[r], s[r], rm[r], bk, mv[r], k, V);
-    }
-    {
-        const int j0 = nfull * step + lane * V;
-        if (j0 < m) {
-            const int valid = (m - j0) < V ? (m - j0) : V;
+        for (int c = 0; c < nfull; c++) {
+            const int j0 = c * step + lane * V;
             double bk[V];
 #pragma unroll
-            for (int v = 0; v < V; v++) bk[v] = (v < valid) ? beta[j0 + v] * k : 0.0;
+            for (int v = 0; v < V; v++) bk[v] = beta[j0 + v] * k;
+            VT mv[R];
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                if (row0 + r < n) {
-                    // ldm >= m rounded up to V, so the vector stays inside the row pitch.
-                    const VT mv = *reinterpret_cast<const VT*>(rowp[r] + j0);
-                    row_accumulate<TM, V, true>(sh[r], s[r], rm[r], bk, mv, k, valid);
+            for (int r = 0; r < R; r++)
+                if (row0 + r < n) mv[r] = *reinterpret_cast<const VT*>(rowp[r] + j0);
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if (row0 + r < n)
+                    row_accumulate<TM, V, false>(sh[r], s[r], rm[r], bk, mv[r], k, V);
+        }
+        {
+            const int j0 = nfull * step + lane * V;
+            if (j0 < m) {
+                const int valid = (m - j0) < V ? (m - j0) : V;
+                double bk[V];
+#pragma unroll
+                for (int v = 0; v < V; v++) bk[v] = (v < valid) ? beta[j0 + v] * k : 0.0;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if (row0 + r < n) {
+                        // ldm >= m rounded up to V: the vector stays inside the row pitch.
+                        const VT mv = *reinterpret_cast<const VT*>(rowp[r] + j0);
+                        row_accumulate<TM, V, true>(sh[r], s[r], rm[r], bk, mv, k, valid);
+                    }
+                }
+            }
+        }
+
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const long i = row0 + r;
+            if (i >= n) break;
+            const double L = wave_log2sum(acc_log2(sh[r], s[r]));
+            if (lane == 0) {
+                if (epi.lse2) epi.lse2[i] = L;
+                if (aout) aout[i] = epi.reg * (epi.loga[i] - kLn2 * L);
+                if (epi.rowsum || FIN) {
+                    const double ri = exp2(rowpot[i] * k + L);
+                    if (epi.rowsum) epi.rowsum[i] = ri;
+                    if (FIN) {
+                        const double g = ri - fin.a[i];
+                        wg2 += g * g;
+                        wr += ri;
+                        waa += rowpot[i] * fin.a[i];
+                    }
                 }
             }
         }
     }
 
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const long i = row0 + r;
-        if (i >= n) break;
-        const double L = wave_log2sum(acc_log2(sh[r], s[r]));
+    if (FIN) {
+        // Per-workgroup partial sums; the combine kernel of this iteration
+        // reduces them after the kernel boundary (no cross-XCD fences here).
+        __shared__ double red[4][3];
         if (lane == 0) {
-            if (epi.lse2) epi.lse2[i] = L;
-            if (aout) aout[i] = epi.reg * (epi.loga[i] - kLn2 * L);
-            if (epi.rowsum) epi.rowsum[i] = exp2(rowpot[i] * k + L);
+            red[wave][0] = wg2;
+            red[wave][1] = wr;
+            red[wave][2] = waa;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                fin.rpart[3 * blockIdx.x + q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+            if (blockIdx.x == 0) fin.ctl[3] = ctl[1];
         }
     }
 }
@@ -281,12 +349,13 @@ __global__ __launch_bounds__(256) void k_col_partial(
     using VT = typename VecOf<TM>::type;
     constexpr int V = VecOf<TM>::N;
     constexpr int CH = 4;
-    __shared__ double red[4][64 * V];
+    constexpr int TW = 64 * V;
+    __shared__ double red[4][TW];
     if (ctl != nullptr && ctl[0] != 0) return;
     if (ap0 != nullptr) alpha = (ctl[1] & 1) ? ap1 : ap0;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int j0 = blockIdx.x * 64 * V + lane * V;
+    const int j0 = blockIdx.x * TW + lane * V;
     const int valid_cols = (m - j0) < V ? (m - j0) : V;   // <= 0: lane has no column
     const long rb = static_cast<long>(blockIdx.y) * rpb;
     const long re = (rb + rpb < n) ? rb + rpb : n;
@@ -361,30 +430,125 @@ __global__ __launch_bounds__(256) void k_col_partial(
     }
 }
 
+constexpr int kCombineCols = 64;   // columns per combine workgroup (4 threads each)
+
+// Stop test of the on-device BCD loop for iterate k = ctl[3], computed
+// identically (same data, same order) by every combine workgroup:
+// g_k = [r - a ; column residual of the update that produced beta_k],
+// f_k = reg sum r - (alpha_k + beta_{k,m-1})'a - (beta_k - beta_{k,m-1})'b over
+// j < m-1.  The test order is that of src/sinkhorn_bcd.cpp:65-79.
+__device__ void bcd_decide(const BcdState& st, int kk, double* f_out, double* g_out,
+                           bool* done_out)
+{
+    const double* cp = st.cpart + 2L * st.ntiles * (kk & 1);
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < st.nrb; q += 256) {
+        v[0] += st.rpart[3 * q];
+        v[1] += st.rpart[3 * q + 1];
+        v[2] += st.rpart[3 * q + 2];
+    }
+    for (int q = threadIdx.x; q < st.ntiles; q += 256) {
+        v[3] += cp[2 * q];
+        v[4] += cp[2 * q + 1];
+    }
+    block_sums_256<5>(v);
+    const double bm = st.beta[st.m - 1];
+    const double gnorm = sqrt(v[0] + v[3]);
+    const double da = v[2] + bm * st.sum_a;
+    const double db = v[4] - bm * st.sum_b1;
+    *f_out = st.reg * v[1] - da - db;
+    *g_out = gnorm;
+    *done_out = (gnorm < st.tol) || (kk >= st.max_iter);
+}
+
+// Combine the nblk partials of each column (4 threads per column, fixed
+// order), then the column epilogue.  With BCD, first take the stop decision
+// for iterate ctl[3]; when it stops, beta is left untouched (beta_k is the
+// result), otherwise beta <- beta_{k+1} and the tile sums of the new column
+// residual go to the other parity half of cpart.  Workgroup 0 records the
+// history entry and advances or stops the loop.
+template <bool BCD>
 __global__ __launch_bounds__(256) void k_col_combine(const double* __restrict__ part, int nblk,
-                                                     int m, double k, ColEpi epi,
+                                                     int m, double k, ColEpi epi, BcdState st,
                                                      const int* __restrict__ ctl)
 {
-    if (ctl != nullptr && ctl[0] != 0) return;
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= m) return;
-    double Lm = -INFINITY;
-    for (int p = 0; p < nblk; p++) Lm = fmax(Lm, part[static_cast<long>(p) * m + j]);
-    double L = -INFINITY;
-    if (Lm != -INFINITY) {
-        double S = 0.0;
-        for (int p = 0; p < nblk; p++) {
-            const double x = part[static_cast<long>(p) * m + j];
-            S += (x == -INFINITY) ? 0.0 : exp2(x - Lm);
+    int kk = 0;
+    double f = 0.0, gnorm = 0.0;
+    bool done = false;
+    if (BCD) {
+        if (st.ctl[0] != 0) return;
+        kk = st.ctl[3];
+        bcd_decide(st, kk, &f, &gnorm, &done);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            const int h = kk % st.hist_cap;
+            st.hist_f[h] = f;
+            st.hist_g[h] = gnorm;
+            st.hist_ts[h] = static_cast<long long>(wall_clock64());
         }
-        L = Lm + log2(S);
+    } else if (ctl != nullptr && ctl[0] != 0) {
+        return;
     }
-    if (epi.lse2) epi.lse2[j] = L;
-    if (epi.colsum) epi.colsum[j] = exp2(epi.colpot[j] * k + L);
-    if (epi.beta_out) {
-        const double bn = epi.reg * (epi.logb[j] - kLn2 * L);
-        epi.beta_out[j] = bn;
-        if (epi.resid) epi.resid[j] = epi.b[j] * expm1(bn / epi.reg + kLn2 * L - epi.logb[j]);
+
+    const int sub = threadIdx.x & 3;
+    const int j = blockIdx.x * kCombineCols + (threadIdx.x >> 2);
+    double r2 = 0.0, bb = 0.0;
+    if (!done) {
+        double Lm = -INFINITY;
+        if (j < m)
+            for (int p = sub; p < nblk; p += 4) Lm = fmax(Lm, part[static_cast<long>(p) * m + j]);
+        Lm = fmax(Lm, __shfl_xor(Lm, 1));
+        Lm = fmax(Lm, __shfl_xor(Lm, 2));
+        double S = 0.0;
+        if (j < m && Lm != -INFINITY)
+            for (int p = sub; p < nblk; p += 4) {
+                const double x = part[static_cast<long>(p) * m + j];
+                S += (x == -INFINITY) ? 0.0 : exp2(x - Lm);
+            }
+        S += __shfl_xor(S, 1);
+        S += __shfl_xor(S, 2);
+        S = __shfl(S, threadIdx.x & ~3);
+        if (sub == 0 && j < m) {
+            const double L = (Lm == -INFINITY) ? -INFINITY : Lm + log2(S);
+            if (epi.lse2) epi.lse2[j] = L;
+            double cs = 0.0;
+            if (epi.colsum || (epi.resid && !epi.beta_out)) {
+                cs = exp2(epi.colpot[j] * k + L);
+                if (epi.colsum) epi.colsum[j] = cs;
+            }
+            double bj = epi.colpot ? epi.colpot[j] : 0.0;
+            double g;
+            if (epi.beta_out) {
+                bj = epi.reg * (epi.logb[j] - kLn2 * L);
+                epi.beta_out[j] = bj;
+                g = epi.b[j] * expm1(bj / epi.reg + kLn2 * L - epi.logb[j]);
+            } else {
+                g = cs - epi.b[j];
+            }
+            if (epi.resid) epi.resid[j] = g;
+            if (j < m - 1) {
+                r2 = g * g;
+                bb = bj * epi.b[j];
+            }
+        }
+        if (epi.tilepart) {
+            double v[2] = {r2, bb};
+            block_sums_256<2>(v);
+            r2 = v[0];
+            bb = v[1];
+            if (threadIdx.x == 0) {
+                double* tp = epi.tilepart + (BCD ? 2L * gridDim.x * ((kk + 1) & 1) : 0);
+                tp[2 * blockIdx.x] = r2;
+                tp[2 * blockIdx.x + 1] = bb;
+            }
+        }
+    }
+    if (BCD && blockIdx.x == 0 && threadIdx.x == 0) {
+        if (done) {
+            st.ctl[2] = kk;
+            st.ctl[0] = 1;
+        } else {
+            st.ctl[1] = kk + 1;
+        }
     }
 }
 
@@ -404,65 +568,6 @@ __global__ __launch_bounds__(256) void k_plan(const TM* __restrict__ M, long ldm
     double* Pi = plan + static_cast<long>(blockIdx.y) * m;
     for (int j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256)
         Pi[j] = exp(((ai + beta[j]) - static_cast<double>(Mi[j])) / reg);
-}
-
-__device__ __forceinline__ double block_sum_1024(double v, double* sm)
-{
-    const int t = threadIdx.x;
-    sm[t] = v;
-    __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-        if (t < o) sm[t] += sm[t + o];
-        __syncthreads();
-    }
-    const double r = sm[0];
-    __syncthreads();
-    return r;
-}
-
-// Iterate k (ctl[1]) is (alpha_k = apair[k&1], beta_k = beta); rowsum holds
-// T_k 1_m from the row pass of this iteration and gcol the column residual of
-// the beta update that produced beta_k.  Records f_k, ||g_k|| and a timestamp,
-// then either stops (||g_k|| < tol or k == max_iter) or advances k.
-__global__ __launch_bounds__(1024) void k_bcd_finalize(BcdState st)
-{
-    __shared__ double sm[1024];
-    int* ctl = st.ctl;
-    if (ctl[0] != 0) return;
-    const int kk = ctl[1];
-    const double* alpha = st.apair[kk & 1];
-    const double bm = st.beta[st.m - 1];
-    double gn2 = 0.0, tsum = 0.0, da = 0.0, db = 0.0;
-    for (int i = threadIdx.x; i < st.n; i += 1024) {
-        const double r = st.rowsum[i];
-        const double g = r - st.a[i];
-        gn2 += g * g;
-        tsum += r;
-        da += (alpha[i] + bm) * st.a[i];
-    }
-    for (int j = threadIdx.x; j < st.m - 1; j += 1024) {
-        const double g = st.gcol[j];
-        gn2 += g * g;
-        db += (st.beta[j] - bm) * st.b[j];
-    }
-    gn2 = block_sum_1024(gn2, sm);
-    tsum = block_sum_1024(tsum, sm);
-    da = block_sum_1024(da, sm);
-    db = block_sum_1024(db, sm);
-    if (threadIdx.x == 0) {
-        const double f = st.reg * tsum - da - db;
-        const double gnorm = sqrt(gn2);
-        const int h = kk % st.hist_cap;
-        st.hist_f[h] = f;
-        st.hist_g[h] = gnorm;
-        st.hist_ts[h] = static_cast<long long>(wall_clock64());
-        if (gnorm < st.tol || kk >= st.max_iter) {
-            ctl[2] = kk;
-            ctl[0] = 1;
-        } else {
-            ctl[1] = kk + 1;
-        }
-    }
 }
 
 // One workgroup per row: FP64 row sums of exp((alpha_i + beta_j - M_ij)/reg).
@@ -542,18 +647,43 @@ int grid_for(long count)
 template <typename TM>
 constexpr int kRowsPerWave = sizeof(TM) == 4 ? 4 : 2;
 
+int row_pass_blocks(int bulk, int n)
+{
+    return ceil_div(n, 4 * (bulk == 0 ? kRowsPerWave<float> : kRowsPerWave<double>));
+}
+
 void launch_row_pass(int bulk, const void* M, long ldm, int n, int m, const double* beta,
                      double k, const RowEpi& epi, const int* ctl, hipStream_t s)
 {
     if (n <= 0) return;
+    BcdState none{};
     if (bulk == 0) {
         constexpr int R = kRowsPerWave<float>;
-        hipLaunchKernelGGL((k_row_pass<float, R>), dim3(ceil_div(n, 4 * R)), dim3(256), 0, s,
-                           static_cast<const float*>(M), ldm, n, m, beta, k, epi, ctl);
+        hipLaunchKernelGGL((k_row_pass<float, R, false>), dim3(row_pass_blocks(bulk, n)),
+                           dim3(256), 0, s, static_cast<const float*>(M), ldm, n, m, beta, k, epi,
+                           none, ctl);
     } else {
         constexpr int R = kRowsPerWave<double>;
-        hipLaunchKernelGGL((k_row_pass<double, R>), dim3(ceil_div(n, 4 * R)), dim3(256), 0, s,
-                           static_cast<const double*>(M), ldm, n, m, beta, k, epi, ctl);
+        hipLaunchKernelGGL((k_row_pass<double, R, false>), dim3(row_pass_blocks(bulk, n)),
+                           dim3(256), 0, s, static_cast<const double*>(M), ldm, n, m, beta, k,
+                           epi, none, ctl);
+    }
+    KERNEL_CHECK();
+}
+
+void launch_row_pass_bcd(int bulk, const void* M, long ldm, int n, int m, const double* beta,
+                         double k, const RowEpi& epi, const BcdState& st, hipStream_t s)
+{
+    if (bulk == 0) {
+        constexpr int R = kRowsPerWave<float>;
+        hipLaunchKernelGGL((k_row_pass<float, R, true>), dim3(row_pass_blocks(bulk, n)),
+                           dim3(256), 0, s, static_cast<const float*>(M), ldm, n, m, beta, k, epi,
+                           st, st.ctl);
+    } else {
+        constexpr int R = kRowsPerWave<double>;
+        hipLaunchKernelGGL((k_row_pass<double, R, true>), dim3(row_pass_blocks(bulk, n)),
+                           dim3(256), 0, s, static_cast<const double*>(M), ldm, n, m, beta, k,
+                           epi, st, st.ctl);
     }
     KERNEL_CHECK();
 }
@@ -569,31 +699,32 @@ int col_pass_blocks(int n, int m)
     return nblk;
 }
 
-void launch_col_partial(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
-                        const double* alpha_pair0, const double* alpha_pair1,
-                        const double* colpot_guess, const double* logb, double k,
-                        double* part, int nblk, const int* ctl, hipStream_t s)
+int col_combine_blocks(int m) { return ceil_div(m, kCombineCols); }
+
+void launch_col_pass(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                     const double* alpha_pair0, const double* alpha_pair1,
+                     const double* colpot_guess, const double* logb, double k, double* part,
+                     int nblk, const ColEpi& epi, const BcdState* st, const int* ctl,
+                     hipStream_t s)
 {
     if (m <= 0 || n <= 0) return;
     const int rpb = ceil_div(n, nblk);
-    if (bulk == 0) {
-        hipLaunchKernelGGL(k_col_partial<float>, dim3(ceil_div(m, 256), nblk), dim3(256), 0, s,
+    const dim3 grid(ceil_div(m, bulk == 0 ? 256 : 128), nblk);
+    if (bulk == 0)
+        hipLaunchKernelGGL(k_col_partial<float>, grid, dim3(256), 0, s,
                            static_cast<const float*>(M), ldm, n, m, alpha, alpha_pair0,
                            alpha_pair1, colpot_guess, logb, k, rpb, part, ctl);
-    } else {
-        hipLaunchKernelGGL(k_col_partial<double>, dim3(ceil_div(m, 128), nblk), dim3(256), 0, s,
+    else
+        hipLaunchKernelGGL(k_col_partial<double>, grid, dim3(256), 0, s,
                            static_cast<const double*>(M), ldm, n, m, alpha, alpha_pair0,
                            alpha_pair1, colpot_guess, logb, k, rpb, part, ctl);
-    }
     KERNEL_CHECK();
-}
-
-void launch_col_combine(const double* part, int nblk, int m, double k, const ColEpi& epi,
-                        const int* ctl, hipStream_t s)
-{
-    if (m <= 0) return;
-    hipLaunchKernelGGL(k_col_combine, dim3(ceil_div(m, 256)), dim3(256), 0, s, part, nblk, m, k,
-                       epi, ctl);
+    if (st != nullptr)
+        hipLaunchKernelGGL(k_col_combine<true>, dim3(col_combine_blocks(m)), dim3(256), 0, s,
+                           part, nblk, m, k, epi, *st, ctl);
+    else
+        hipLaunchKernelGGL(k_col_combine<false>, dim3(col_combine_blocks(m)), dim3(256), 0, s,
+                           part, nblk, m, k, epi, BcdState{}, ctl);
     KERNEL_CHECK();
 }
 
@@ -609,12 +740,6 @@ void launch_plan(int bulk, const void* M, long ldm, int n, int m, const double* 
     else
         hipLaunchKernelGGL(k_plan<double>, dim3(gx, nr), dim3(256), 0, s,
                            static_cast<const double*>(M), ldm, m, alpha, beta, reg, plan, r0);
-    KERNEL_CHECK();
-}
-
-void launch_bcd_finalize(const BcdState& st, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_bcd_finalize, dim3(1), dim3(1024), 0, s, st);
     KERNEL_CHECK();
 }
 

@@ -40,6 +40,7 @@ using namespace regot;
 regot_problem::~regot_problem()
 {
     hess.reset();
+    bcd.reset();
     for (auto& slot : prof_slot) {
         for (auto& e : slot.pending) {
             (void)hipEventDestroy(e.first);
@@ -162,6 +163,7 @@ void regot_problem::setup(int n_, int m_, const void* M, int M_dtype, int bulk_,
     ctl.alloc(16);
     nblk = col_pass_blocks(n, m);
     part.alloc(static_cast<size_t>(nblk) * m);
+    ntiles = col_combine_blocks(m);
     hscal.ensure(64);
     hctl.ensure(16);
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -221,18 +223,27 @@ void regot_problem::row_pass(const double* beta_full, const RowEpi& epi, const i
 
 void regot_problem::col_pass(const double* alpha_v, const double* colpot_guess,
                              const ColEpi& epi, const double* ap0, const double* ap1,
-                             const int* ctl_p)
+                             const int* ctl_p, const BcdState* st)
 {
     hipEvent_t stop;
     prof_begin(1, &stop);
-    launch_col_partial(bulk, dM, ldm, n, m, alpha_v, ap0, ap1, colpot_guess, logb, k, part,
-                       nblk, ctl_p, stream);
     ColEpi e = epi;
     e.reg = reg;
     e.logb = logb;
     e.b = b;
-    launch_col_combine(part, nblk, m, k, e, ctl_p, stream);
+    launch_col_pass(bulk, dM, ldm, n, m, alpha_v, ap0, ap1, colpot_guess, logb, k, part, nblk, e,
+                    st, ctl_p, stream);
     prof_end(1, stop);
+}
+
+void regot_problem::row_pass_bcd(const double* beta_full, const RowEpi& epi, const BcdState& st)
+{
+    hipEvent_t stop;
+    prof_begin(0, &stop);
+    RowEpi e = epi;
+    e.reg = reg;
+    launch_row_pass_bcd(bulk, dM, ldm, n, m, beta_full, k, e, st, stream);
+    prof_end(0, stop);
 }
 
 double regot_problem::dev_obj(const double* d_gamma)

@@ -31,6 +31,21 @@ void vec_objgrad_finish(const double* r, const double* c, const double* a, const
 void vec_obj_finish(const double* r, const double* a, const double* b, const double* gamma,
                     int n, int m, double reg, double* out, hipStream_t s);
 
+// Buffers of the on-device BCD loop, kept across solves on one handle.
+struct BcdWork {
+    DBuf<double> ap0, ap1, beta, gcol, rpart, cpart;
+    DBuf<int> ctl;
+    MappedRing<double> hf, hg;
+    MappedRing<long long> hts;
+    HBuf<int> hc;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~BcdWork()
+    {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
 struct ProfileSlot {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
@@ -56,6 +71,8 @@ struct regot_problem {
     regot::DBuf<double> alpha, beta, rsum, csum, gamma, grad, part, scal;
     regot::DBuf<int> ctl;
     int nblk = 1;
+    int ntiles = 1;
+    std::unique_ptr<regot::BcdWork> bcd;
     regot::HBuf<double> hscal;
     regot::HBuf<int> hctl;
 
@@ -80,7 +97,10 @@ struct regot_problem {
     // Column pass (partial + combine) at row potential alpha (n); see ColEpi.
     void col_pass(const double* alpha_v, const double* colpot_guess, const regot::ColEpi& epi,
                   const double* ap0 = nullptr, const double* ap1 = nullptr,
-                  const int* ctl_p = nullptr);
+                  const int* ctl_p = nullptr, const regot::BcdState* st = nullptr);
+    // Row pass with the BCD bookkeeping fused (see BcdState).
+    void row_pass_bcd(const double* beta_full, const regot::RowEpi& epi,
+                      const regot::BcdState& st);
 
     // f(gamma) for device gamma (n+m-1); row pass only.
     double dev_obj(const double* d_gamma);

@@ -31,39 +31,19 @@ void normalise_host(int n, int m, const double* alpha, const double* beta, doubl
     for (int j = 0; j < m - 1; j++) gamma[n + j] = beta[j] - beta[m - 1];
 }
 
-// Pinned, device-mapped host ring that the finalize kernel writes directly.
-template <typename T>
-struct MappedRing {
-    T* h = nullptr;
-    T* d = nullptr;
-    int cap = 0;
-    explicit MappedRing(int c) : cap(c)
-    {
-        HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(T) * cap,
-                                hipHostMallocMapped));
-        HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&d), h, 0));
-    }
-    ~MappedRing()
-    {
-        if (h) (void)hipHostFree(h);
-    }
-    MappedRing(const MappedRing&) = delete;
-    MappedRing& operator=(const MappedRing&) = delete;
-};
-
 // Plain Sinkhorn / BCD with the whole loop on the GPU.
 //
 // Semantics follow sinkhorn_bcd_internal, src/sinkhorn_bcd.cpp:14-124: init
 // alpha = 0, beta = optimal_beta(alpha); at iterate k test ||g_k|| < tol first,
 // then alpha <- optimal_alpha(beta), beta <- optimal_beta(alpha).
 //
-// Each iteration is four launches on one stream:
+// Each iteration is two launches on one stream:
 //   row pass     beta_k -> alpha_{k+1} (double buffer) and the row sums at
-//                iterate k, which give g_k's row part, f_k and ||g_k||;
-//   finalize     records f_k, ||g_k||, a timestamp; stops or advances k;
+//                iterate k; its last workgroup records f_k, ||g_k|| and a
+//                timestamp, then stops or advances k;
 //   column pass  alpha_{k+1} -> beta_{k+1}, plus the column residual of the
 //                update (g_{k+1}'s column part, zero up to FP64 rounding).
-// After the stop flag is set every later kernel returns at once, so the host
+// Once the stop flag is set every later kernel returns at once, so the host
 // keeps two batches of iterations in flight and only polls the flag.
 void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const regot_opts& o,
                regot_result* res)
@@ -73,43 +53,65 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
     const int K = o.graph_batch > 0 ? o.graph_batch : 8;
     const int cap = std::max(4096, 4 * K + 8);
 
-    DBuf<double> ap0(n), ap1(n), beta(m), r(n), gcol(m);
-    DBuf<int> ctl(4);
-    MappedRing<double> hf(cap), hg(cap);
-    MappedRing<long long> hts(cap + 1);
+    if (!p->bcd) p->bcd = std::make_unique<BcdWork>();
+    BcdWork& w = *p->bcd;
+    const int nrb = row_pass_blocks(p->bulk, n);
+    w.ap0.ensure(n);
+    w.ap1.ensure(n);
+    w.beta.ensure(m);
+    w.gcol.ensure(m);
+    w.rpart.ensure(3L * nrb);
+    w.cpart.ensure(4L * p->ntiles);
+    w.ctl.ensure(4);
+    w.hf.ensure(cap);
+    w.hg.ensure(cap);
+    w.hts.ensure(cap + 1);
+    w.hc.ensure(8);
+    for (auto& e : w.ev)
+        if (!e) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
+    // sum a and sum_{j<m-1} b for the objective (host FP64, fixed order)
+    std::vector<double> ah(n), bh(m);
+    HIP_CHECK(hipMemcpyAsync(ah.data(), p->a.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(bh.data(), p->b.p, m * sizeof(double), hipMemcpyDeviceToHost, s));
+    p->sync();
+    double sum_a = 0.0, sum_b1 = 0.0;
+    for (int i = 0; i < n; i++) sum_a += ah[i];
+    for (int j = 0; j < m - 1; j++) sum_b1 += bh[j];
+
+    ColEpi c0;
+    c0.resid = w.gcol;
+    c0.tilepart = w.cpart;
     if (o.x0) {
         HIP_CHECK(hipMemcpyAsync(p->gamma, o.x0, (n + m - 1) * sizeof(double),
                                  hipMemcpyHostToDevice, s));
-        launch_split_gamma(p->gamma, n, m, ap0, beta, s);
-        ColEpi c;
-        c.colpot = beta;
-        c.colsum = p->csum;
-        p->col_pass(ap0, beta, c);
-        vec_axpby(gcol, 1.0, p->csum, -1.0, p->b, m, s);
+        launch_split_gamma(p->gamma, n, m, w.ap0, w.beta, s);
+        c0.colpot = w.beta;
+        p->col_pass(w.ap0, w.beta, c0);
     } else {
-        launch_fill(ap0, 0.0, n, s);
-        ColEpi c;
-        c.beta_out = beta;
-        c.resid = gcol;
-        p->col_pass(ap0, nullptr, c);
+        launch_fill(w.ap0, 0.0, n, s);
+        c0.beta_out = w.beta;
+        p->col_pass(w.ap0, nullptr, c0);
     }
-    launch_fill_int(ctl, 0, 4, s);
-    launch_stamp(hts.d + cap, s);
+    launch_fill_int(w.ctl, 0, 4, s);
+    launch_stamp(w.hts.d + cap, s);
 
-    BcdState st;
+    BcdState st{};
     st.a = p->a;
-    st.b = p->b;
-    st.apair[0] = ap0;
-    st.apair[1] = ap1;
-    st.beta = beta;
-    st.rowsum = r;
-    st.gcol = gcol;
-    st.ctl = ctl;
-    st.hist_f = hf.d;
-    st.hist_g = hg.d;
-    st.hist_ts = hts.d;
+    st.apair[0] = w.ap0;
+    st.apair[1] = w.ap1;
+    st.beta = w.beta;
+    st.ctl = w.ctl;
+    st.hist_f = w.hf.d;
+    st.hist_g = w.hg.d;
+    st.hist_ts = w.hts.d;
     st.hist_cap = cap;
+    st.rpart = w.rpart;
+    st.cpart = w.cpart;
+    st.nrb = nrb;
+    st.ntiles = p->ntiles;
+    st.sum_a = sum_a;
+    st.sum_b1 = sum_b1;
     st.tol = tol;
     st.reg = p->reg;
     st.max_iter = max_iter;
@@ -119,28 +121,23 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
     auto enqueue_iteration = [&]() {
         RowEpi e;
         e.loga = p->loga;
-        e.rowsum = r;
         e.parity = true;
-        e.apair[0] = ap0;
-        e.apair[1] = ap1;
-        p->row_pass(beta, e, ctl);
-        launch_bcd_finalize(st, s);
+        e.apair[0] = w.ap0;
+        e.apair[1] = w.ap1;
+        p->row_pass_bcd(w.beta, e, st);
         ColEpi c;
-        c.beta_out = beta;
-        c.resid = gcol;
-        p->col_pass(nullptr, beta, c, ap0, ap1, ctl);
+        c.beta_out = w.beta;
+        c.resid = w.gcol;
+        c.tilepart = w.cpart;
+        // The column pass runs before the stop decision advances k, so it must
+        // read alpha_{k+1} = apair[(k + 1) & 1]: hand the pair over swapped.
+        p->col_pass(nullptr, w.beta, c, w.ap1, w.ap0, w.ctl, &st);
     };
-
-    HBuf<int> hc;
-    hc.ensure(8);
-    hipEvent_t ev[2];
-    HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     auto launch_batch = [&](int slot) {
         for (int it = 0; it < K; it++) enqueue_iteration();
-        HIP_CHECK(hipMemcpyAsync(hc.p + 4 * slot, ctl.p, 4 * sizeof(int), hipMemcpyDeviceToHost,
-                                 s));
-        HIP_CHECK(hipEventRecord(ev[slot], s));
+        HIP_CHECK(hipMemcpyAsync(w.hc.p + 4 * slot, w.ctl.p, 4 * sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipEventRecord(w.ev[slot], s));
     };
 
     int wall_khz = 0;
@@ -158,18 +155,19 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
         for (;;) {
             const int slot = q.front();
             q.pop_front();
-            HIP_CHECK(hipEventSynchronize(ev[slot]));
-            const int* c = hc.p + 4 * slot;
+            HIP_CHECK(hipEventSynchronize(w.ev[slot]));
+            const int* c = w.hc.p + 4 * slot;
             const bool done = c[0] != 0;
             const int avail = done ? c[2] + 1 : c[1];
-            if (h_read == 0 && avail > 0) ts_init = hts.h[cap];
+            if (h_read == 0 && avail > 0) ts_init = w.hts.h[cap];
             for (; h_read < avail; h_read++) {
                 const int h = h_read % cap;
-                res->obj_vals.push_back(hf.h[h]);
-                res->mar_errs.push_back(hg.h[h]);
-                res->run_times.push_back(static_cast<double>(hts.h[h] - ts_init) * tick_ms);
+                res->obj_vals.push_back(w.hf.h[h]);
+                res->mar_errs.push_back(w.hg.h[h]);
+                res->run_times.push_back(static_cast<double>(w.hts.h[h] - ts_init) * tick_ms);
                 if (verbose >= 1 && h_read < max_iter)
-                    printf("iter = %d, objval = %g, ||grad|| = %g\n", h_read, hf.h[h], hg.h[h]);
+                    printf("iter = %d, objval = %g, ||grad|| = %g\n", h_read, w.hf.h[h],
+                           w.hg.h[h]);
             }
             if (verbose >= 1) fflush(stdout);
             if (done) {
@@ -182,17 +180,13 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
         HIP_CHECK(hipStreamSynchronize(s));
     } catch (...) {
         (void)hipStreamSynchronize(s);
-        (void)hipEventDestroy(ev[0]);
-        (void)hipEventDestroy(ev[1]);
         throw;
     }
-    HIP_CHECK(hipEventDestroy(ev[0]));
-    HIP_CHECK(hipEventDestroy(ev[1]));
 
-    std::vector<double> ah(n), bh(m);
-    HIP_CHECK(hipMemcpy(ah.data(), (niter & 1) ? ap1.p : ap0.p, n * sizeof(double),
-                        hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(bh.data(), beta.p, m * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpyAsync(ah.data(), (niter & 1) ? w.ap1.p : w.ap0.p, n * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(bh.data(), w.beta.p, m * sizeof(double), hipMemcpyDeviceToHost, s));
+    p->sync();
     res->niter = niter;
     res->dual.resize(n + m - 1);
     normalise_host(n, m, ah.data(), bh.data(), res->dual.data());
