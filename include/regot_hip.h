@@ -83,6 +83,42 @@ int regot_dual_obj_grad(regot_problem* p, const double* gamma, double* obj, doub
 int regot_get_plan(regot_problem* p, const double* gamma, double* plan);
 
 /*
+ * Sparsified Hessian Hs = [diag(h1) sigma; sigma' diag(h2)] at gamma (host),
+ * held on the device in CSR and CSC.  The gradient at gamma is computed here
+ * (h1, h2 are the exact row / column sums / reg).
+ *
+ * Threshold rule.  Replaces Problem::dual_sparsified_hess,
+ * src/sinkhorn_problem_obj.cpp:432-443 -> Hessian::compute_hess,
+ * src/sinkhorn_hess.cpp:15-33 -> sparsify_mat3, src/sinkhorn_sparsify3.cpp:276-405.
+ * *nnz counts kept entries as the reference does (exact FP64 zeros included). */
+int regot_sparsified_hess(regot_problem* p, const double* gamma, double delta,
+                          double density_hint, long long* nnz);
+/* Top-k rule, row 0 and column 0 preserved.  Replaces
+ * Problem::dual_sparsified_hess_with_density, src/sinkhorn_problem_obj.cpp:446-457 ->
+ * sparsify_mat_with_density, src/sinkhorn_sparsify4.cpp:77-174. */
+int regot_sparsified_hess_with_density(regot_problem* p, const double* gamma, double density,
+                                       long long* nnz);
+/* Stored entries, FP64-zero kept entries and Hessian::density(),
+ * src/sinkhorn_hess.h:75-80. */
+int regot_hess_info(regot_problem* p, long long* nnz_stored, long long* nnz_zero,
+                    double* density);
+/* Copy h1 (n), h2 (m-1), CSC (colptr m, rowind nnz, cval nnz) and CSR
+ * (rowptr n+1, colind nnz, rval nnz) to host; NULL skips an array.
+ * (Hessian::to_spmat, src/sinkhorn_hess.cpp:57-96, builds the same matrix.) */
+int regot_hess_export(regot_problem* p, double* h1, double* h2, int* colptr, int* rowind,
+                      double* cval, int* rowptr, int* colind, double* rval);
+/* res = Delta x, x and res of length m-1.  Replaces Hessian::apply_Deltax,
+ * src/sinkhorn_hess.cpp:136-148. */
+int regot_hess_apply_delta(regot_problem* p, const double* x, double shift, double* res);
+/* res = Hs x, length n+m-1.  Replaces Hessian::apply_Hsx, src/sinkhorn_hess.cpp:151-164. */
+int regot_hess_apply_hs(regot_problem* p, const double* x, double* res);
+/* res = (Hs + shift I)^{-1} rhs by block elimination and CG on Delta (FP64
+ * dots), relative residual tol, max_iter <= 0 -> 2m.  Replaces hess_cg,
+ * src/sinkhorn_cg.cpp:9-50. */
+int regot_hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
+                  double* res, int* iters, double* tol_error);
+
+/*
  * Solver options; zero-initialise, then call regot_opts_default().
  * Mirrors SinkhornSolverOpts, src/sinkhorn_solvers.h:14-38, and the kwargs
  * parsed by parse_sinkhorn_opts, src/solvers.cpp:314-341.

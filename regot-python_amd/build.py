@@ -17,7 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "regot", "libregot_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["dense_kernels.hip", "vec_kernels.hip", "problem.hip", "solvers.cpp"]
+SOURCES = ["dense_kernels.hip", "vec_kernels.hip", "sparse_kernels.hip", "hessian.hip",
+           "problem.hip", "solvers.cpp"]
 ARCH = os.environ.get("REGOT_ARCH", "gfx950")
 
 

@@ -1,0 +1,25 @@
+// hessian.hpp -- sparsified Hessian build, products and CG on a problem handle.
+#pragma once
+
+#include "problem.hpp"
+#include "sparse.hpp"
+
+namespace regot {
+
+SparseHessian& hess_of(regot_problem* p);
+void hess_ensure_transpose(regot_problem* p);
+// Threshold rule (SSNS): sparsify_mat3 semantics at (gamma, grad), device vectors.
+void hess_threshold(regot_problem* p, const double* d_gamma, const double* d_grad, double delta,
+                    double density_hint);
+// Top-k rule (sparse Newton, SPLR): sparsify_mat_with_density semantics, preserve = true.
+void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, double density);
+void hess_apply_delta(regot_problem* p, const double* x, double shift, double* out);
+void hess_apply_hs(regot_problem* p, const double* x, double* out);
+// CG on the Schur complement Delta (m-1); returns iterations.
+int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
+                  double* x, double* tol_error);
+// (Hs + shift I)^{-1} rhs (n+m-1) by block elimination; returns CG iterations.
+int hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
+            double* out, double* tol_error);
+
+}  // namespace regot

@@ -52,6 +52,7 @@ regot_problem::~regot_problem()
         }
     }
     if (own_M && dM) (void)hipFree(dM);
+    if (dMT) (void)hipFree(dMT);
     if (stream) (void)hipStreamDestroy(stream);
 }
 

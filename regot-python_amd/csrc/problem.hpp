@@ -63,6 +63,8 @@ struct regot_problem {
     int bulk = 0;               // 0 = FP32, 1 = FP64
     void* dM = nullptr;
     bool own_M = false;
+    void* dMT = nullptr;        // M transposed (m x n, pitch ldt), built on first use
+    long ldt = 0;
     double reg = 1.0;
     double k = 1.0;             // log2(e) / reg
     regot::DBuf<double> a, b, loga, logb;

@@ -1,14 +1,119 @@
-// sparse.hpp -- sparsified dual Hessian on the GPU (threshold and top-k rules,
-// CSR + CSC storage, Hessian-vector products and CG).
+// sparse.hpp -- the sparsified dual Hessian on the GPU.
+//
+//   Hs = [ diag(h1)  sigma    ]     h1 = r / reg, h2 = ct / reg,
+//        [ sigma'    diag(h2) ]     sigma = kept entries of T[:, 0:m-1] / reg
+//
+// sigma is held twice, as CSR (rows, for sigma x) and CSC (columns, for
+// sigma' y), so both products are deterministic gathers.  Both copies are
+// built atomic-free: count kept entries per row / column, exclusive scan,
+// write in index order.  The kept set is decided element-wise from values
+// recomputed from M (row kernels) or its transpose MT (column kernels) with
+// one shared element function, so both copies agree bit for bit.
 #pragma once
 
 #include "common.hpp"
 
 namespace regot {
 
+// What decides whether entry (i, j), j < m-1, of T is kept.
+struct KeepRule {
+    int mode = 0;                        // 0 = threshold rule, 1 = top-k rule
+    // threshold rule: kept <=> T_ij >= min(tau_j, rho_i), or T_ij == 0 in FP64
+    const double* tau = nullptr;         // per column
+    const double* rho = nullptr;         // per row
+    // top-k rule: kept <=> key > kstar, or key == kstar and the tie is among the
+    // first tie_take ties in row-major order, or (preserve) i == 0 or j == 0
+    unsigned long long kstar = 0;
+    long long tie_take = 0;
+    const long long* tie_off = nullptr;  // per row: ties in earlier rows
+    unsigned int* tie_bits = nullptr;    // row-major bitmask of kept ties
+    int tie_words = 0;                   // words per bitmask row
+    int preserve = 1;
+};
+
+// Everything the element function needs: T_ij from alpha_i, beta_j, M_ij.
+struct ElemCtx {
+    const double* alpha = nullptr;   // n
+    const double* beta = nullptr;    // m (beta[m-1] = 0)
+    double k = 0.0;                  // log2(e) / reg
+    double reg = 1.0;
+    int n = 0, m = 0;
+};
+
 struct SparseHessian {
     int n = 0, m = 0;
-    long long nnz = 0;
+    long long nnz = 0;        // stored entries of sigma
+    long long nnz_zero = 0;   // threshold rule: kept entries that are exactly 0 in FP64
+    DBuf<double> h1, h2;
+    DBuf<int> rowptr, colind, colptr, rowind;
+    DBuf<double> rval, cval;
+    // work
+    DBuf<double> tau, rho;
+    DBuf<int> rcnt, ccnt, zcnt;
+    DBuf<long long> ltmp;
+    DBuf<unsigned int> tiebits;
+    DBuf<unsigned long long> hist;   // global count histograms of the top-k radix passes
+    DBuf<double> vn, vm, vm2;        // alpha, beta (full) of the current build; scratch
+    DBuf<double> cg_r, cg_p, cg_q, cg_y_buf, cg_part, cg_scal;
+    DBuf<int> cg_ctl;
+    HBuf<int> cg_hctl;
+    HBuf<double> hbuf;
+    HBuf<long long> hll;
+    double density() const
+    {
+        const double N = static_cast<double>(n) + m - 1;
+        return (2.0 * static_cast<double>(nnz + nnz_zero) + n + m - 1) / N / N;
+    }
 };
+
+// M^T (m x n, row pitch ldt) from M (n x m, pitch ldm); bulk 0 = FP32, 1 = FP64.
+void launch_transpose(int bulk, const void* M, long ldm, int n, int m, void* MT, long ldt,
+                      hipStream_t s);
+
+// Threshold rule, step 1: tau_j for every column j < m-1 (one workgroup per
+// column over MT).  find_small: columns whose whole mass is <= delta keep
+// nothing (tau = +inf); otherwise they keep their largest entry.
+void launch_col_threshold(int bulk, const void* MT, long ldt, const ElemCtx& c, double delta,
+                          int find_small, double* tau, hipStream_t s);
+// Threshold rule, step 2: rho_i per row (dropped-mass correction to 2 delta),
+// stored-entry count per row and FP64-zero kept count per row.
+void launch_row_threshold(int bulk, const void* M, long ldm, const ElemCtx& c,
+                          const double* tau, double delta2, double* rho, int* cnt, int* zcnt,
+                          hipStream_t s);
+
+// Top-k rule: radix passes over all entries of T[:, 0:m-1] to the key of the
+// k-th largest value.  Fills *kstar and *n_greater (entries with key > kstar).
+void topk_select(int bulk, const void* M, long ldm, const ElemCtx& c, long long k,
+                 SparseHessian& H, unsigned long long* kstar, long long* n_greater,
+                 hipStream_t s);
+// Per row: ties (key == kstar) into cnt.
+void launch_row_ties(int bulk, const void* M, long ldm, const ElemCtx& c,
+                     unsigned long long kstar, int* cnt, hipStream_t s);
+
+// Kept-entry counts per row (CSR) / per column (CSC) under a rule.
+void launch_row_count(int bulk, const void* M, long ldm, const ElemCtx& c, const KeepRule& r,
+                      int* cnt, hipStream_t s);
+void launch_col_count(int bulk, const void* MT, long ldt, const ElemCtx& c, const KeepRule& r,
+                      int* cnt, hipStream_t s);
+// Write CSR / CSC (values T / reg) at the given pointers.
+void launch_row_write(int bulk, const void* M, long ldm, const ElemCtx& c, const KeepRule& r,
+                      const int* rowptr, int* colind, double* val, hipStream_t s);
+void launch_col_write(int bulk, const void* MT, long ldt, const ElemCtx& c, const KeepRule& r,
+                      const int* colptr, int* rowind, double* val, hipStream_t s);
+
+// Exclusive scan of count (int) into out (count + 1 entries, int64 total check).
+void launch_scan(const int* cnt, int count, int* out, long long* total, hipStream_t s);
+// Exclusive scan of int counts into int64 offsets (count entries).
+void launch_scan64(const int* cnt, int count, long long* out, hipStream_t s);
+
+// y = sigma x (x: m-1, y: n), optionally y /= (h1 + shift) when div.
+void launch_spmv_rows(const SparseHessian& H, const double* x, double* y, bool div,
+                      double shift, hipStream_t s);
+// Column products (y: n; z, out: m-1):
+//   mode 0: out = sigma' y
+//   mode 1: out = h2 .* z + shift z - sigma' y   (Delta form)
+//   mode 2: out = h2 .* z + sigma' y             (lower block of Hs)
+void launch_spmv_cols(const SparseHessian& H, const double* y, const double* z, double shift,
+                      int mode, double* out, hipStream_t s);
 
 }  // namespace regot

@@ -196,6 +196,70 @@ class Problem:
         check(lib().regot_get_plan(self._h, dptr(gamma), dptr(plan)))
         return plan
 
+    # --- sparsified Hessian ------------------------------------------------
+    def sparsified_hess(self, gamma, delta, density_hint=0.001):
+        """Threshold rule at gamma (src/sinkhorn_sparsify3.cpp:276-405); returns nnz."""
+        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        nnz = _lib.ctypes.c_longlong()
+        check(lib().regot_sparsified_hess(self._h, dptr(gamma), float(delta), float(density_hint),
+                                          _lib.ctypes.byref(nnz)))
+        return nnz.value
+
+    def sparsified_hess_with_density(self, gamma, density):
+        """Top-k rule at gamma (src/sinkhorn_sparsify4.cpp:77-174); returns stored nnz."""
+        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        nnz = _lib.ctypes.c_longlong()
+        check(lib().regot_sparsified_hess_with_density(self._h, dptr(gamma), float(density),
+                                                       _lib.ctypes.byref(nnz)))
+        return nnz.value
+
+    def hess_info(self):
+        """(stored nnz, FP64-zero kept entries, Hessian density)"""
+        a = _lib.ctypes.c_longlong()
+        z = _lib.ctypes.c_longlong()
+        d = _lib.ctypes.c_double()
+        check(lib().regot_hess_info(self._h, _lib.ctypes.byref(a), _lib.ctypes.byref(z),
+                                    _lib.ctypes.byref(d)))
+        return a.value, z.value, d.value
+
+    def hess_export(self):
+        """dict with h1, h2, CSC (colptr, rowind, cval) and CSR (rowptr, colind, rval)."""
+        nnz = self.hess_info()[0]
+        n, m = self.n, self.m
+        out = dict(h1=np.empty(n), h2=np.empty(m - 1), colptr=np.empty(m, np.int32),
+                   rowind=np.empty(max(nnz, 1), np.int32), cval=np.empty(max(nnz, 1)),
+                   rowptr=np.empty(n + 1, np.int32), colind=np.empty(max(nnz, 1), np.int32),
+                   rval=np.empty(max(nnz, 1)))
+        check(lib().regot_hess_export(self._h, dptr(out["h1"]), dptr(out["h2"]),
+                                      _lib.iptr(out["colptr"]), _lib.iptr(out["rowind"]),
+                                      dptr(out["cval"]), _lib.iptr(out["rowptr"]),
+                                      _lib.iptr(out["colind"]), dptr(out["rval"])))
+        for k in ("rowind", "cval", "colind", "rval"):
+            out[k] = out[k][:nnz]
+        return out
+
+    def hess_apply_delta(self, x, shift):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty(self.m - 1)
+        check(lib().regot_hess_apply_delta(self._h, dptr(x), float(shift), dptr(out)))
+        return out
+
+    def hess_apply_hs(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty(self.n + self.m - 1)
+        check(lib().regot_hess_apply_hs(self._h, dptr(x), dptr(out)))
+        return out
+
+    def hess_cg(self, rhs, shift, tol=1e-8, max_iter=0):
+        """(Hs + shift I)^{-1} rhs -> (x, iterations, relative residual)"""
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        out = np.empty(self.n + self.m - 1)
+        it = _lib.ctypes.c_int()
+        te = _lib.ctypes.c_double()
+        check(lib().regot_hess_cg(self._h, dptr(rhs), float(shift), float(tol), int(max_iter),
+                                  dptr(out), _lib.ctypes.byref(it), _lib.ctypes.byref(te)))
+        return out, it.value, te.value
+
     def profile(self, on=True):
         check(lib().regot_profile_enable(self._h, int(bool(on))))
 

@@ -75,6 +75,16 @@ def lib():
     L.regot_result_verify.argtypes = [_vp, _vp, _dp, _dp]
     L.regot_result_free.argtypes = [_vp]
     L.regot_result_free.restype = None
+    _llp = ctypes.POINTER(ctypes.c_longlong)
+    _ip = ctypes.POINTER(ctypes.c_int)
+    L.regot_sparsified_hess.argtypes = [_vp, _dp, ctypes.c_double, ctypes.c_double, _llp]
+    L.regot_sparsified_hess_with_density.argtypes = [_vp, _dp, ctypes.c_double, _llp]
+    L.regot_hess_info.argtypes = [_vp, _llp, _llp, _dp]
+    L.regot_hess_export.argtypes = [_vp, _dp, _dp, _ip, _ip, _dp, _ip, _ip, _dp]
+    L.regot_hess_apply_delta.argtypes = [_vp, _dp, ctypes.c_double, _dp]
+    L.regot_hess_apply_hs.argtypes = [_vp, _dp, _dp]
+    L.regot_hess_cg.argtypes = [_vp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
+                                _ip, _dp]
     L.regot_profile_enable.argtypes = [_vp, ctypes.c_int]
     L.regot_profile_get.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _dp]
     L.regot_profile_reset.argtypes = [_vp]
@@ -92,3 +102,7 @@ def check(status):
 
 def dptr(arr):
     return arr.ctypes.data_as(_dp)
+
+
+def iptr(arr):
+    return arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int))

@@ -1,0 +1,155 @@
+"""GPU parity of the sparsified Hessian, its products and the CG solve.
+
+FP64 bulk: the kept sets must equal the oracle's exactly (random inputs have
+no ties) and values agree to 1e-12.  FP32 bulk: the rule's invariants hold and
+Hessian products agree with the oracle's to the FP32 value accuracy.
+"""
+import numpy as np
+import pytest
+
+from conftest import cloud_problem
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def point(M, a, b, reg, iters=3, seed=0, noise=0.02):
+    """A dual point near a few BCD iterations, perturbed."""
+    n, m = M.shape
+    alpha = np.zeros(n)
+    beta = O.optimal_beta(M, b, reg, alpha)
+    for _ in range(iters):
+        alpha = O.optimal_alpha(M, a, reg, beta)
+        beta = O.optimal_beta(M, b, reg, alpha)
+    g = np.concatenate([alpha + beta[-1], beta[:-1] - beta[-1]])
+    return g + noise * reg * np.random.default_rng(seed).standard_normal(g.shape)
+
+
+def csc_dense(n, m, colptr, rowind, val):
+    return O.csc_to_dense(n, m, colptr, rowind, val)
+
+
+@pytest.mark.parametrize("hint", [0.001, 0.5])
+@pytest.mark.parametrize("reg", [0.05, 0.001])
+def test_threshold_rule_fp64_matches_oracle(gpu_lib, hint, reg):
+    n, m = 150, 121
+    M, a, b = cloud_problem(n, m, seed=5)
+    gamma = point(M, a, b, reg)
+    f, g = O.dual_obj_grad(M, a, b, reg, gamma)
+    delta = 0.01 * np.linalg.norm(g)
+    T, _ = O.compute_T(M, reg, gamma)
+    cp, ri, va = O.sparsify_threshold(T, delta, hint)
+    with gpu_lib.Problem(M, a, b, reg, bulk="fp64") as P:
+        nnz = P.sparsified_hess(gamma, delta, hint)
+        stored, zeros, dens = P.hess_info()
+        H = P.hess_export()
+    # the reference counts kept exact zeros of T as stored entries
+    assert nnz == len(ri)
+    assert stored + zeros == len(ri)
+    Sg = csc_dense(n, m, H["colptr"], H["rowind"], H["cval"])
+    So = csc_dense(n, m, cp, ri, va) / reg
+    kept_o = np.zeros((n, m - 1), bool)
+    for j in range(m - 1):
+        kept_o[ri[cp[j]:cp[j + 1]], j] = True
+    kept_g = Sg != 0
+    assert np.array_equal(kept_g, kept_o & (So != 0))
+    np.testing.assert_allclose(Sg, So, rtol=1e-12, atol=0)
+    N = n + m - 1
+    assert abs(dens - (2.0 * len(ri) + N) / N / N) < 1e-15
+    np.testing.assert_allclose(H["h1"], (g[:n] + a) / reg, rtol=1e-12)
+    np.testing.assert_allclose(H["h2"], (g[n:] + b[:-1]) / reg, rtol=1e-12)
+    # CSR holds the same matrix
+    Sr = np.zeros((n, m - 1))
+    for i in range(n):
+        Sr[i, H["colind"][H["rowptr"][i]:H["rowptr"][i + 1]]] = H["rval"][H["rowptr"][i]:H["rowptr"][i + 1]]
+    assert np.array_equal(Sr, Sg)
+
+
+def test_threshold_rule_fp32_invariants(gpu_lib):
+    n, m, reg = 300, 257, 0.02
+    M, a, b = cloud_problem(n, m, seed=9)
+    M32 = M.astype(np.float32)
+    Mo = M32.astype(np.float64)
+    gamma = point(Mo, a, b, reg)
+    f, g = O.dual_obj_grad(Mo, a, b, reg, gamma)
+    delta = 0.01 * np.linalg.norm(g)
+    T, _ = O.compute_T(Mo, reg, gamma)
+    with gpu_lib.Problem(M32, a, b, reg) as P:
+        P.sparsified_hess(gamma, delta, 0.001)
+        H = P.hess_export()
+    S = csc_dense(n, m, H["colptr"], H["rowind"], H["cval"]) * reg
+    kept = S != 0
+    dropped = np.where(kept, 0.0, T[:, :-1])
+    # per-row dropped mass <= 2 delta (row rule), per column the threshold rule
+    assert np.all(dropped.sum(1) <= 2 * delta * (1 + 1e-5) + 1e-300)
+    cp, ri, va = O.sparsify_threshold(T, delta, 0.001)
+    So = csc_dense(n, m, cp, ri, va)
+    diff = np.abs(S - So)
+    # entries may switch sides only at the threshold boundary: small mass
+    assert diff.sum() <= 4 * delta * (m - 1) * 1e-3 + 1e-6 * So.sum()
+    assert np.max(np.abs(S[kept] - T[:, :-1][kept]) / T[:, :-1][kept]) < 2e-6
+
+
+@pytest.mark.parametrize("bulk", ["fp64", "fp32"])
+@pytest.mark.parametrize("density", [0.0, 0.01, 0.1, 1.0])
+def test_topk_rule(gpu_lib, bulk, density):
+    n, m, reg = 97, 131, 0.03
+    M, a, b = cloud_problem(n, m, seed=3)
+    Mg = M.astype(np.float32) if bulk == "fp32" else M
+    Mo = Mg.astype(np.float64)
+    gamma = point(Mo, a, b, reg)
+    T, _ = O.compute_T(Mo, reg, gamma)
+    cp, ri, va = O.sparsify_density(T, density, True)
+    with gpu_lib.Problem(Mg, a, b, reg) as P:
+        nnz = P.sparsified_hess_with_density(gamma, density)
+        H = P.hess_export()
+    So = csc_dense(n, m, cp, ri, va)
+    Sg = csc_dense(n, m, H["colptr"], H["rowind"], H["cval"]) * reg
+    k = int(n * (m - 1) * density)
+    if bulk == "fp64":
+        assert nnz == len(ri)
+        assert np.array_equal(Sg != 0, So != 0)
+        np.testing.assert_allclose(Sg, So, rtol=1e-12)
+    else:
+        # FP32 values may reorder near the k-th value; the count is exact
+        assert nnz == len(ri)
+        assert np.sum((Sg != 0) != (So != 0)) <= max(4, 0.002 * k)
+    # preserve: row 0 and column 0 always stored
+    assert np.all(Sg[0, :] != 0) and np.all(Sg[:, 0] != 0)
+
+
+@pytest.mark.parametrize("bulk", ["fp64", "fp32"])
+def test_hessian_products_and_cg(gpu_lib, bulk):
+    n, m, reg = 180, 140, 0.05
+    M, a, b = cloud_problem(n, m, seed=13)
+    Mg = M.astype(np.float32) if bulk == "fp32" else M
+    gamma = point(Mg.astype(np.float64), a, b, reg)
+    rng = np.random.default_rng(2)
+    with gpu_lib.Problem(Mg, a, b, reg) as P:
+        P.sparsified_hess_with_density(gamma, 0.05)
+        H = P.hess_export()
+        csc = (H["colptr"], H["rowind"], H["cval"])
+        x = rng.standard_normal(m - 1)
+        shift = 1e-3
+        d_g = P.hess_apply_delta(x, shift)
+        d_o = O.apply_deltax(n, m, H["h1"], H["h2"], csc, x, shift)
+        np.testing.assert_allclose(d_g, d_o, rtol=1e-12, atol=1e-12 * np.abs(d_o).max())
+        v = rng.standard_normal(n + m - 1)
+        h_g = P.hess_apply_hs(v)
+        h_o = O.apply_hsx(n, m, H["h1"], H["h2"], csc, v)
+        np.testing.assert_allclose(h_g, h_o, rtol=1e-12, atol=1e-12 * np.abs(h_o).max())
+        # dense check of the operator itself
+        S = csc_dense(n, m, *csc)
+        Hd = np.block([[np.diag(H["h1"]), S], [S.T, np.diag(H["h2"])]])
+        np.testing.assert_allclose(h_g, Hd @ v, rtol=1e-10, atol=1e-10 * np.abs(h_g).max())
+        rhs = rng.standard_normal(n + m - 1)
+        sol, it, err = P.hess_cg(rhs, shift, tol=1e-10)
+        sol_o, it_o, err_o = O.hess_cg(n, m, H["h1"], H["h2"], csc, rhs, shift, tol=1e-10)
+        ref = np.linalg.solve(Hd + shift * np.eye(n + m - 1), rhs)
+        assert err < 1e-10
+        assert abs(it - it_o) <= 2
+        np.testing.assert_allclose(sol, ref, rtol=1e-6, atol=1e-8 * np.abs(ref).max())
+        np.testing.assert_allclose(sol, sol_o, rtol=1e-7, atol=1e-9 * np.abs(ref).max())
+        # iteration cap is honoured
+        _, it2, err2 = P.hess_cg(rhs, shift, tol=1e-14, max_iter=3)
+        assert it2 == 3 and err2 > 1e-14
