@@ -46,9 +46,12 @@ struct ColEpi {
     double reg = 1.0;
 };
 
-// Row pass: one wave per R rows, streaming whole rows.
+// Row pass: one wave per R rows, streaming whole rows.  precise (FP32 bulk
+// only): t, exp2 and the sums in FP64 over the FP32 M, for objectives and
+// gradients that line searches difference at the 1e-16 relative level.
 void launch_row_pass(int bulk, const void* M, long ldm, int n, int m, const double* beta,
-                     double k, const RowEpi& epi, const int* ctl, hipStream_t s);
+                     double k, const RowEpi& epi, const int* ctl, hipStream_t s,
+                     bool precise = false);
 
 struct BcdState;
 
@@ -61,7 +64,7 @@ void launch_col_pass(int bulk, const void* M, long ldm, int n, int m, const doub
                      const double* alpha_pair0, const double* alpha_pair1,
                      const double* colpot_guess, const double* logb, double k, double* part,
                      int nblk, const ColEpi& epi, const BcdState* st, const int* ctl,
-                     hipStream_t s);
+                     hipStream_t s, bool precise = false);
 
 // FP64 plan: plan_ij = exp((alpha_i + beta_j - M_ij) / reg) for rows [r0, r0+nr).
 void launch_plan(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
@@ -96,12 +99,6 @@ struct BcdState {
 void launch_row_pass_bcd(int bulk, const void* M, long ldm, int n, int m, const double* beta,
                          double k, const RowEpi& epi, const BcdState& st, hipStream_t s);
 int row_pass_blocks(int bulk, int n);
-
-// FP64-exp row and column sums of T at (alpha, beta full-length), for the
-// verification of a result independent of the bulk precision.
-void launch_verify_sums(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
-                        const double* beta, double reg, double* rowsum, double* colsum,
-                        hipStream_t s);
 
 void launch_fill(double* p, double v, long count, hipStream_t s);
 void launch_fill_int(int* p, int v, long count, hipStream_t s);

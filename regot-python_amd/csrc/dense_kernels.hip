@@ -123,18 +123,19 @@ __device__ __forceinline__ bool reshift(double& sh, TA& s, TA& rm)
 // ---------------------------------------------------------------------------
 
 // One row x V columns: t_v = (bk_v - sh) - k M_v, masked entries = -inf.
-template <typename TM, int V, bool MASK>
-__device__ __forceinline__ void row_accumulate(double& sh, TM& s, TM& rm, const double* bk,
+// TM is the storage type of M, TA the type t, exp2 and the sums run in.
+template <typename TM, typename TA, int V, bool MASK>
+__device__ __forceinline__ void row_accumulate(double& sh, TA& s, TA& rm, const double* bk,
                                                const typename VecOf<TM>::type& mv, double k,
                                                int valid_cols)
 {
-    TM t[V];
-    TM cm = ninf<TM>();
+    TA t[V];
+    TA cm = ninf<TA>();
 #pragma unroll
     for (int v = 0; v < V; v++) {
         const double mvd = static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv, v));
-        t[v] = static_cast<TM>(fma(-k, mvd, bk[v] - sh));
-        if (MASK && v >= valid_cols) t[v] = ninf<TM>();
+        t[v] = static_cast<TA>(fma(-k, mvd, bk[v] - sh));
+        if (MASK && v >= valid_cols) t[v] = ninf<TA>();
         cm = vmax(cm, t[v]);
     }
     rm = vmax(rm, cm);
@@ -142,8 +143,8 @@ __device__ __forceinline__ void row_accumulate(double& sh, TM& s, TM& rm, const 
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const double mvd = static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv, v));
-            t[v] = static_cast<TM>(fma(-k, mvd, bk[v] - sh));
-            if (MASK && v >= valid_cols) t[v] = ninf<TM>();
+            t[v] = static_cast<TA>(fma(-k, mvd, bk[v] - sh));
+            if (MASK && v >= valid_cols) t[v] = ninf<TA>();
         }
     }
 #pragma unroll
@@ -184,7 +185,7 @@ __device__ __forceinline__ void block_sums_256(double (&v)[NV])
 // per step, fully coalesced).  The beta chunk is loaded once per step and
 // reused by all R rows.  With FIN, per-workgroup sums of (g^2, r, alpha a) are
 // published and the last workgroup to arrive runs the BCD bookkeeping.
-template <typename TM, int R, bool FIN>
+template <typename TM, typename TA, int R, bool FIN>
 __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long ldm, int n,
                                                   int m, const double* __restrict__ beta,
                                                   double k, RowEpi epi, BcdState fin,
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
 
     if (row0 < n) {
         double sh[R];
-        TM s[R], rm[R];
+        TA s[R], rm[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const long i = row0 + r;
@@ -218,8 +219,8 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
                 if (rowpot != nullptr) g -= rowpot[i] * k;
             }
             sh[r] = g;
-            s[r] = TM(0);
-            rm[r] = ninf<TM>();
+            s[r] = TA(0);
+            rm[r] = ninf<TA>();
         }
         const TM* rowp[R];
 #pragma unroll
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
 #pragma unroll
             for (int r = 0; r < R; r++)
                 if (row0 + r < n)
-                    row_accumulate<TM, V, false>(sh[r], s[r], rm[r], bk, mv[r], k, V);
+                    row_accumulate<TM, TA, V, false>(sh[r], s[r], rm[r], bk, mv[r], k, V);
         }
         {
             const int j0 = nfull * step + lane * V;
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
                     if (row0 + r < n) {
                         // ldm >= m rounded up to V: the vector stays inside the row pitch.
                         const VT mv = *reinterpret_cast<const VT*>(rowp[r] + j0);
-                        row_accumulate<TM, V, true>(sh[r], s[r], rm[r], bk, mv, k, valid);
+                        row_accumulate<TM, TA, V, true>(sh[r], s[r], rm[r], bk, mv, k, valid);
                     }
                 }
             }
@@ -305,20 +306,20 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
 // Column pass
 // ---------------------------------------------------------------------------
 
-template <typename TM, int V, int CH, bool MASK>
-__device__ __forceinline__ void col_chunk(double* sh, TM* s, TM* rm,
+template <typename TM, typename TA, int V, int CH, bool MASK>
+__device__ __forceinline__ void col_chunk(double* sh, TA* s, TA* rm,
                                           const typename VecOf<TM>::type* mv, const double* ak,
                                           double k, int valid_rows, int valid_cols)
 {
 #pragma unroll
     for (int v = 0; v < V; v++) {
-        TM t[CH];
-        TM cm = ninf<TM>();
+        TA t[CH];
+        TA cm = ninf<TA>();
 #pragma unroll
         for (int c = 0; c < CH; c++) {
             const double mvd = static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv[c], v));
-            t[c] = static_cast<TM>(fma(-k, mvd, ak[c] - sh[v]));
-            if (MASK && (c >= valid_rows || v >= valid_cols)) t[c] = ninf<TM>();
+            t[c] = static_cast<TA>(fma(-k, mvd, ak[c] - sh[v]));
+            if (MASK && (c >= valid_rows || v >= valid_cols)) t[c] = ninf<TA>();
             cm = vmax(cm, t[c]);
         }
         rm[v] = vmax(rm[v], cm);
@@ -327,8 +328,8 @@ __device__ __forceinline__ void col_chunk(double* sh, TM* s, TM* rm,
             for (int c = 0; c < CH; c++) {
                 const double mvd =
                     static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv[c], v));
-                t[c] = static_cast<TM>(fma(-k, mvd, ak[c] - sh[v]));
-                if (MASK && (c >= valid_rows || v >= valid_cols)) t[c] = ninf<TM>();
+                t[c] = static_cast<TA>(fma(-k, mvd, ak[c] - sh[v]));
+                if (MASK && (c >= valid_rows || v >= valid_cols)) t[c] = ninf<TA>();
             }
         }
 #pragma unroll
@@ -339,7 +340,7 @@ __device__ __forceinline__ void col_chunk(double* sh, TM* s, TM* rm,
 // Workgroup = 4 waves over one 64V-column tile and one block of rows; the
 // waves take CH-row slabs round robin.  Output: one FP64 log2 partial sum per
 // (row block, column).
-template <typename TM>
+template <typename TM, typename TA>
 __global__ __launch_bounds__(256) void k_col_partial(
     const TM* __restrict__ M, long ldm, int n, int m, const double* __restrict__ alpha,
     const double* __restrict__ ap0, const double* __restrict__ ap1,
@@ -361,7 +362,7 @@ __global__ __launch_bounds__(256) void k_col_partial(
     const long re = (rb + rpb < n) ? rb + rpb : n;
 
     double sh[V];
-    TM s[V], rm[V];
+    TA s[V], rm[V];
 #pragma unroll
     for (int v = 0; v < V; v++) {
         const int j = j0 + v;
@@ -371,8 +372,8 @@ __global__ __launch_bounds__(256) void k_col_partial(
             if (colpot != nullptr) g -= colpot[j] * k;
         }
         sh[v] = g;
-        s[v] = TM(0);
-        rm[v] = ninf<TM>();
+        s[v] = TA(0);
+        rm[v] = ninf<TA>();
     }
 
     if (valid_cols > 0) {
@@ -387,9 +388,9 @@ __global__ __launch_bounds__(256) void k_col_partial(
                 ak[c] = alpha[r + c] * k;
             }
             if (valid_cols == V)
-                col_chunk<TM, V, CH, false>(sh, s, rm, mv, ak, k, CH, V);
+                col_chunk<TM, TA, V, CH, false>(sh, s, rm, mv, ak, k, CH, V);
             else
-                col_chunk<TM, V, CH, true>(sh, s, rm, mv, ak, k, CH, valid_cols);
+                col_chunk<TM, TA, V, CH, true>(sh, s, rm, mv, ak, k, CH, valid_cols);
         }
         if (r < re) {
             const int vr = static_cast<int>(re - r);
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(256) void k_col_partial(
                 mv[c] = *reinterpret_cast<const VT*>(base + rr * ldm);
                 ak[c] = alpha[rr] * k;
             }
-            col_chunk<TM, V, CH, true>(sh, s, rm, mv, ak, k, vr, valid_cols);
+            col_chunk<TM, TA, V, CH, true>(sh, s, rm, mv, ak, k, vr, valid_cols);
         }
     }
 
@@ -570,43 +571,6 @@ __global__ __launch_bounds__(256) void k_plan(const TM* __restrict__ M, long ldm
         Pi[j] = exp(((ai + beta[j]) - static_cast<double>(Mi[j])) / reg);
 }
 
-// One workgroup per row: FP64 row sums of exp((alpha_i + beta_j - M_ij)/reg).
-template <typename TM>
-__global__ __launch_bounds__(256) void k_verify_rows(const TM* __restrict__ M, long ldm, int m,
-                                                     const double* __restrict__ alpha,
-                                                     const double* __restrict__ beta, double reg,
-                                                     double* __restrict__ rowsum)
-{
-    __shared__ double sm[256];
-    const long i = blockIdx.x;
-    const TM* Mi = M + i * ldm;
-    double acc = 0.0;
-    for (int j = threadIdx.x; j < m; j += 256)
-        acc += exp(((alpha[i] + beta[j]) - static_cast<double>(Mi[j])) / reg);
-    sm[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) rowsum[i] = sm[0];
-}
-
-// One thread per column, rows in order: FP64 column sums.
-template <typename TM>
-__global__ __launch_bounds__(256) void k_verify_cols(const TM* __restrict__ M, long ldm, int n,
-                                                     int m, const double* __restrict__ alpha,
-                                                     const double* __restrict__ beta, double reg,
-                                                     double* __restrict__ colsum)
-{
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= m) return;
-    double acc = 0.0;
-    for (long i = 0; i < n; i++)
-        acc += exp(((alpha[i] + beta[j]) - static_cast<double>(M[i * ldm + j])) / reg);
-    colsum[j] = acc;
-}
-
 __global__ void k_fill(double* p, double v, long count)
 {
     for (long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; i < count;
@@ -653,20 +617,23 @@ int row_pass_blocks(int bulk, int n)
 }
 
 void launch_row_pass(int bulk, const void* M, long ldm, int n, int m, const double* beta,
-                     double k, const RowEpi& epi, const int* ctl, hipStream_t s)
+                     double k, const RowEpi& epi, const int* ctl, hipStream_t s, bool precise)
 {
     if (n <= 0) return;
     BcdState none{};
-    if (bulk == 0) {
+    const dim3 grid(row_pass_blocks(bulk, n));
+    if (bulk == 0 && !precise) {
         constexpr int R = kRowsPerWave<float>;
-        hipLaunchKernelGGL((k_row_pass<float, R, false>), dim3(row_pass_blocks(bulk, n)),
-                           dim3(256), 0, s, static_cast<const float*>(M), ldm, n, m, beta, k, epi,
-                           none, ctl);
+        hipLaunchKernelGGL((k_row_pass<float, float, R, false>), grid, dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, beta, k, epi, none, ctl);
+    } else if (bulk == 0) {
+        constexpr int R = kRowsPerWave<float>;
+        hipLaunchKernelGGL((k_row_pass<float, double, R, false>), grid, dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, beta, k, epi, none, ctl);
     } else {
         constexpr int R = kRowsPerWave<double>;
-        hipLaunchKernelGGL((k_row_pass<double, R, false>), dim3(row_pass_blocks(bulk, n)),
-                           dim3(256), 0, s, static_cast<const double*>(M), ldm, n, m, beta, k,
-                           epi, none, ctl);
+        hipLaunchKernelGGL((k_row_pass<double, double, R, false>), grid, dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, n, m, beta, k, epi, none, ctl);
     }
     KERNEL_CHECK();
 }
@@ -674,16 +641,15 @@ void launch_row_pass(int bulk, const void* M, long ldm, int n, int m, const doub
 void launch_row_pass_bcd(int bulk, const void* M, long ldm, int n, int m, const double* beta,
                          double k, const RowEpi& epi, const BcdState& st, hipStream_t s)
 {
+    const dim3 grid(row_pass_blocks(bulk, n));
     if (bulk == 0) {
         constexpr int R = kRowsPerWave<float>;
-        hipLaunchKernelGGL((k_row_pass<float, R, true>), dim3(row_pass_blocks(bulk, n)),
-                           dim3(256), 0, s, static_cast<const float*>(M), ldm, n, m, beta, k, epi,
-                           st, st.ctl);
+        hipLaunchKernelGGL((k_row_pass<float, float, R, true>), grid, dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, beta, k, epi, st, st.ctl);
     } else {
         constexpr int R = kRowsPerWave<double>;
-        hipLaunchKernelGGL((k_row_pass<double, R, true>), dim3(row_pass_blocks(bulk, n)),
-                           dim3(256), 0, s, static_cast<const double*>(M), ldm, n, m, beta, k,
-                           epi, st, st.ctl);
+        hipLaunchKernelGGL((k_row_pass<double, double, R, true>), grid, dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, n, m, beta, k, epi, st, st.ctl);
     }
     KERNEL_CHECK();
 }
@@ -705,17 +671,21 @@ void launch_col_pass(int bulk, const void* M, long ldm, int n, int m, const doub
                      const double* alpha_pair0, const double* alpha_pair1,
                      const double* colpot_guess, const double* logb, double k, double* part,
                      int nblk, const ColEpi& epi, const BcdState* st, const int* ctl,
-                     hipStream_t s)
+                     hipStream_t s, bool precise)
 {
     if (m <= 0 || n <= 0) return;
     const int rpb = ceil_div(n, nblk);
     const dim3 grid(ceil_div(m, bulk == 0 ? 256 : 128), nblk);
-    if (bulk == 0)
-        hipLaunchKernelGGL(k_col_partial<float>, grid, dim3(256), 0, s,
+    if (bulk == 0 && !precise)
+        hipLaunchKernelGGL((k_col_partial<float, float>), grid, dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, alpha, alpha_pair0,
+                           alpha_pair1, colpot_guess, logb, k, rpb, part, ctl);
+    else if (bulk == 0)
+        hipLaunchKernelGGL((k_col_partial<float, double>), grid, dim3(256), 0, s,
                            static_cast<const float*>(M), ldm, n, m, alpha, alpha_pair0,
                            alpha_pair1, colpot_guess, logb, k, rpb, part, ctl);
     else
-        hipLaunchKernelGGL(k_col_partial<double>, grid, dim3(256), 0, s,
+        hipLaunchKernelGGL((k_col_partial<double, double>), grid, dim3(256), 0, s,
                            static_cast<const double*>(M), ldm, n, m, alpha, alpha_pair0,
                            alpha_pair1, colpot_guess, logb, k, rpb, part, ctl);
     KERNEL_CHECK();
@@ -740,24 +710,6 @@ void launch_plan(int bulk, const void* M, long ldm, int n, int m, const double* 
     else
         hipLaunchKernelGGL(k_plan<double>, dim3(gx, nr), dim3(256), 0, s,
                            static_cast<const double*>(M), ldm, m, alpha, beta, reg, plan, r0);
-    KERNEL_CHECK();
-}
-
-void launch_verify_sums(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
-                        const double* beta, double reg, double* rowsum, double* colsum,
-                        hipStream_t s)
-{
-    if (bulk == 0) {
-        hipLaunchKernelGGL(k_verify_rows<float>, dim3(n), dim3(256), 0, s,
-                           static_cast<const float*>(M), ldm, m, alpha, beta, reg, rowsum);
-        hipLaunchKernelGGL(k_verify_cols<float>, dim3(ceil_div(m, 256)), dim3(256), 0, s,
-                           static_cast<const float*>(M), ldm, n, m, alpha, beta, reg, colsum);
-    } else {
-        hipLaunchKernelGGL(k_verify_rows<double>, dim3(n), dim3(256), 0, s,
-                           static_cast<const double*>(M), ldm, m, alpha, beta, reg, rowsum);
-        hipLaunchKernelGGL(k_verify_cols<double>, dim3(ceil_div(m, 256)), dim3(256), 0, s,
-                           static_cast<const double*>(M), ldm, n, m, alpha, beta, reg, colsum);
-    }
     KERNEL_CHECK();
 }
 

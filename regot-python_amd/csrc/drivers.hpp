@@ -1,0 +1,26 @@
+// drivers.hpp -- solver drivers behind regot_solve() and their result object.
+#pragma once
+
+#include <vector>
+
+#include "problem.hpp"
+#include "regot_hip.h"
+
+struct regot_result {
+    int solver = 0;
+    int n = 0, m = 0;
+    int niter = 0;
+    std::vector<double> dual, obj_vals, mar_errs, run_times, densities;
+    // extras, not in the reference result: total CG iterations of the Newton drivers
+    long long cg_iters = 0;
+};
+
+namespace regot {
+
+void solve_ssns(regot_problem* p, double tol, int max_iter, int verbose, const regot_opts& o,
+                regot_result* res);
+// splr = false: sparse Newton; true: SPLR.
+void solve_newton_family(regot_problem* p, bool splr, double tol, int max_iter, int verbose,
+                         const regot_opts& o, regot_result* res);
+
+}  // namespace regot

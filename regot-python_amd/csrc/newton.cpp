@@ -1,0 +1,509 @@
+// newton.cpp -- host drivers of the Newton-type EROT solvers (SSNS, sparse
+// Newton, SPLR).  Control flow, schedules and line searches run on the host
+// in FP64 and mirror the reference drivers step for step; every vector lives
+// on the device, every O(n m) evaluation and every Hessian product or CG
+// solve is a HIP kernel, and only scalars come back to the host.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+#include <vector>
+
+#include "drivers.hpp"
+#include "hessian.hpp"
+
+namespace regot {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+// Sets the handle's precise flag for the lifetime of a solve: objectives and
+// gradients the line searches compare are then FP64 sums even in FP32 bulk
+// (the sparsified Hessian keeps the bulk precision).
+struct PreciseScope {
+    regot_problem* p;
+    bool was;
+    explicit PreciseScope(regot_problem* pp) : p(pp), was(pp->precise) { p->precise = true; }
+    ~PreciseScope() { p->precise = was; }
+};
+
+// Device workspace of one Newton-type solve.
+struct Work {
+    regot_problem* p;
+    int n, m;
+    long N;
+    hipStream_t s;
+    DBuf<double> gamma, g, direc, step, Hstep, x, xg, rhs, y, sv, g_pre, gamma_pre, lr0, lr1;
+    DBuf<double> dscal;
+    HBuf<double> hscal;
+    long long cg_iters = 0;
+
+    explicit Work(regot_problem* pp)
+        : p(pp), n(pp->n), m(pp->m), N(pp->n + pp->m - 1), s(pp->stream)
+    {
+        for (DBuf<double>* b : {&gamma, &g, &direc, &step, &Hstep, &x, &xg, &rhs, &y, &sv, &g_pre,
+                                &gamma_pre, &lr0, &lr1})
+            b->alloc(N);
+        dscal.alloc(8);
+        hscal.ensure(8);
+    }
+
+    // Reference init (e.g. src/sinkhorn_ssns.cpp:53-62): alpha = 0,
+    // beta = optimal_beta(alpha), gamma normalised; or the caller's x0.
+    void init(const regot_opts& o)
+    {
+        if (o.x0) {
+            HIP_CHECK(hipMemcpyAsync(gamma, o.x0, N * sizeof(double), hipMemcpyHostToDevice, s));
+            return;
+        }
+        launch_fill(p->alpha, 0.0, n, s);
+        ColEpi c;
+        c.beta_out = p->beta;
+        p->col_pass(p->alpha, nullptr, c);
+        vec_normalise(p->alpha, p->beta, n, m, gamma, s);
+    }
+
+    double dot(const double* a, const double* b)
+    {
+        const double* xs[1] = {a};
+        const double* ys[1] = {b};
+        vec_dots(xs, ys, N, 1, dscal, s);
+        HIP_CHECK(hipMemcpyAsync(hscal.p, dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        return hscal.p[0];
+    }
+
+    // Several dots in one launch.
+    void dots(int k, const double* const* a, const double* const* b, double* out)
+    {
+        vec_dots(a, b, N, k, dscal, s);
+        HIP_CHECK(hipMemcpyAsync(hscal.p, dscal.p, k * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        for (int q = 0; q < k; q++) out[q] = hscal.p[q];
+    }
+
+    // out = x + t d
+    void move(double* out, const double* x0, double t, const double* d)
+    {
+        vec_axpby(out, 1.0, x0, t, d, N, s);
+    }
+
+    double obj(const double* at) { return p->dev_obj(at); }
+
+    double obj_grad(const double* at, double* grad, double* gnorm)
+    {
+        double gn2 = 0.0;
+        const double f = p->dev_obj_grad(at, grad, &gn2);
+        if (gnorm) *gnorm = std::sqrt(gn2);
+        return f;
+    }
+
+    // (Hs + shift I)^{-1} rhs.  method 0 is the reference's CG (relative
+    // residual 1e-8, Eigen's default cap of 2 * HessianCG::cols() = 2m).  The
+    // reference's default method (SimplicialLDLT) is an exact solve; CG stands
+    // in for it on the GPU with a 1e-10 residual and a cap high enough that the
+    // tolerance, not the cap, ends it on ill-conditioned Schur complements.
+    void lin_solve(const regot_opts& o, const double* r, double shift, double* out)
+    {
+        const double tol = o.cg_tol > 0 ? o.cg_tol : (o.method == 0 ? 1e-8 : 1e-10);
+        const int maxit = o.cg_max_iter > 0 ? o.cg_max_iter
+                                            : (o.method == 0 ? 2 * m : 10 * m + 1000);
+        cg_iters += hess_cg(p, r, shift, tol, maxit, out, nullptr);
+    }
+
+    // Inverse-BFGS correction of (Hs + shift I)^{-1} applied to r, with the
+    // curvature pair (s, y).  src/sinkhorn_linsolve.cpp:160-225 (two solves, then
+    // res = A^-1 r + ((1 + y'A^-1y / y's) s'r/y's - y'A^-1r / y's) s - (s'r/y's) A^-1 y).
+    void lin_solve_low_rank(const regot_opts& o, const double* r, double shift, const double* yv,
+                            const double* sv_, double* out)
+    {
+        lin_solve(o, r, shift, lr0);
+        lin_solve(o, yv, shift, lr1);
+        const double* xs[4] = {sv_, yv, yv, yv};
+        const double* ys[4] = {r, sv_, lr1.p, lr0.p};
+        double d[4];
+        dots(4, xs, ys, d);
+        const double rs = d[0], ys_ = d[1], yiAy = d[2], yiAr = d[3];
+        const double rs_ys = rs / ys_;
+        const double cs = (1.0 + yiAy / ys_) * rs_ys - yiAr / ys_;
+        // out = lr0 + cs s - rs_ys lr1
+        vec_axpby(out, 1.0, lr0, cs, sv_, N, s);
+        vec_axpy(out, -rs_ys, lr1, N, s);
+    }
+};
+
+// Minimiser of the quadratic through fa, ga (at a) and fb (at b);
+// src/sinkhorn_problem_ls.cpp:72-77.
+double quad_fg(double a, double b, double fa, double ga, double fb)
+{
+    const double ba = b - a;
+    const double w = 0.5 * ba * ga / (fa - fb + ba * ga);
+    return a + w * ba;
+}
+
+// Minimiser of the quadratic with slopes ga (at a) and gb (at b); :84-88.
+double quad_gg(double a, double b, double ga, double gb)
+{
+    const double w = ga / (ga - gb);
+    return a + w * (b - a);
+}
+
+// Local minimiser of the cubic through fa, ga, fb, gb; :93-153.
+double cubic_min(double a, double b, double fa, double fb, double ga, double gb, bool& exists)
+{
+    const double apb = a + b, ba = b - a, ba2 = ba * ba;
+    const double fba = fb - fa, gba = gb - ga;
+    const double z3 = (ga + gb) * ba - 2.0 * fba;
+    const double z2 = 0.5 * (gba * ba2 - 3.0 * apb * z3);
+    const double z1 = fba * ba2 - apb * z2 - (a * apb + b * b) * z3;
+    const double eps = std::numeric_limits<double>::epsilon();
+    if (std::fabs(z3) < eps * std::fabs(z2) || std::fabs(z3) < eps * std::fabs(z1)) {
+        exists = (z2 * ba > 0.0);
+        return exists ? (-0.5 * z1 / z2) : b;
+    }
+    const double u = z2 / (3.0 * z3), v = z1 / z2;
+    const double vu = v / u;
+    exists = (vu <= 1.0);
+    if (!exists) return b;
+    double r1, r2;
+    if (std::fabs(u) >= std::fabs(v)) {
+        const double w = 1.0 + std::sqrt(1.0 - vu);
+        r1 = -u * w;
+        r2 = -v / w;
+    } else {
+        const double sd = std::sqrt(std::fabs(u)) * std::sqrt(std::fabs(v)) * std::sqrt(1 - u / v);
+        r1 = -u - sd;
+        r2 = -u + sd;
+    }
+    return (z3 * ba > 0.0) ? std::max(r1, r2) : std::min(r1, r2);
+}
+
+// Next trial from the bracket (Case 1/3 of More-Thuente); :157-223.
+double step_select(double al, double au, double at, double fl, double fu, double ft, double gl,
+                   double gu, double gt)
+{
+    if (al == au) return al;
+    if (!std::isfinite(ft) || !std::isfinite(gt)) return (al + at) / 2.0;
+    bool ac_ok;
+    const double ac = cubic_min(al, at, fl, ft, gl, gt, ac_ok);
+    const double aq = quad_fg(al, at, fl, gl, ft);
+    if (ft > fl) {
+        if (!ac_ok) return aq;
+        return (std::fabs(ac - al) < std::fabs(aq - al)) ? ac : ((aq + ac) / 2.0);
+    }
+    const double as = quad_gg(al, at, gl, gt);
+    if (gt * gl < 0.0) return (std::fabs(ac - at) >= std::fabs(as - at)) ? ac : as;
+    const double dl = 1.1, du = 0.66;
+    if (std::fabs(gt) < std::fabs(gl)) {
+        const bool take_ac =
+            ac_ok && ((ac - at) * (at - al) > 0.0) && (std::fabs(ac - at) < std::fabs(as - at));
+        const double res = take_ac ? ac : as;
+        const double lim = at + du * (au - at);
+        return (at > al) ? std::min(lim, res) : std::max(lim, res);
+    }
+    if (!std::isfinite(au) || !std::isfinite(fu) || !std::isfinite(gu)) return at + dl * (at - al);
+    bool ae_ok;
+    const double ae = cubic_min(at, au, ft, fu, gt, gu, ae_ok);
+    const double lim = at + du * (au - at);
+    return (at > al) ? std::min(lim, ae) : std::max(lim, ae);
+}
+
+// Strong-Wolfe search (c1 = 1e-4, c2 = 0.9, <= 20 trials, steps in
+// [1e-10, 2]), the More-Thuente variant of Problem::line_search_wolfe,
+// src/sinkhorn_problem_ls.cpp:226-470.  Each trial is one dual_obj_grad pass
+// at gamma + step d into w.x / w.xg.  Returns the step; *reuse_last is true
+// when the returned step is the last trial evaluated, so its f and gradient
+// (left in *f_last and w.xg) are those of the new point (the reference's
+// recompute_T == false case).
+double wolfe(Work& w, double init_step, double f0, double dg0, bool* reuse_last, double* f_last,
+             double* gn_last)
+{
+    const double c1 = 1e-4, c2 = 0.9;
+    const int max_iter = 20;
+    const double step_min = 1e-10, step_max = 2.0;
+    *reuse_last = false;
+    init_step = std::min(std::max(init_step, step_min), step_max);
+    double step = init_step;
+    double fx = f0;
+    if (dg0 > 0.0) return init_step;
+    const double test_decr = c1 * dg0, test_curv = -c2 * dg0;
+    const double inf = std::numeric_limits<double>::infinity();
+    double I_lo = 0.0, I_hi = inf, fI_lo = 0.0, fI_hi = inf;
+    double gI_lo = (1.0 - c1) * dg0, gI_hi = inf, psiI_lo = 0.0;
+    bool bracketed = false, safeguard = true;
+    double I_width = inf, I_width_prev = inf;
+    int shrink_fail = 0;
+    const double delta_max = 1.1, delta_min = 7.0 / 12, shrink = 0.66;
+    double last_step = -1.0;
+    for (int it = 0; it < max_iter; it++) {
+        w.move(w.x, w.gamma, step, w.direc);
+        fx = w.obj_grad(w.x, w.xg, gn_last);
+        const double dg = w.dot(w.xg, w.direc);
+        last_step = step;
+        *f_last = fx;
+        const double psit = fx - f0 - step * test_decr;
+        const double dpsit = dg - test_decr;
+        if (psit <= 0.0 && std::fabs(dg) <= test_curv) {
+            *reuse_last = true;
+            return step;
+        }
+        if (step <= step_min && (psit > 0.0 || dpsit >= 0.0)) {
+            *reuse_last = true;
+            return step;
+        }
+        if (step >= step_max && (psit <= 0.0 && dpsit < 0.0)) {
+            *reuse_last = true;
+            return step;
+        }
+        const double ft = psit, gt = dpsit;
+        if (safeguard && (psit <= 0.0 && dpsit < 0.0)) safeguard = false;
+        double new_step;
+        const bool case2 = (psit <= psiI_lo) && (dpsit * (I_lo - step) > 0.0);
+        if (case2) {
+            new_step = std::min(step_max, step + delta_max * (step - I_lo));
+        } else {
+            new_step = step_select(I_lo, I_hi, step, fI_lo, fI_hi, ft, gI_lo, gI_hi, gt);
+            new_step = std::min(std::max(new_step, step_min), step_max);
+            if (safeguard) {
+                const double upper = std::max(step_min, delta_min * step);
+                new_step = std::min(std::max(new_step, step_min), upper);
+            }
+        }
+        if (psit > psiI_lo) {
+            I_hi = step;
+            fI_hi = ft;
+            gI_hi = gt;
+        } else if (case2) {
+            I_lo = step;
+            fI_lo = ft;
+            gI_lo = gt;
+            psiI_lo = psit;
+        } else {
+            I_hi = I_lo;
+            fI_hi = fI_lo;
+            gI_hi = gI_lo;
+            I_lo = step;
+            fI_lo = ft;
+            gI_lo = gt;
+            psiI_lo = psit;
+        }
+        if (!bracketed && !case2)
+            bracketed = (std::min(I_lo, I_hi) >= step_min && std::max(I_lo, I_hi) <= step_max);
+        if (bracketed) {
+            I_width_prev = I_width;
+            I_width = std::fabs(I_hi - I_lo);
+            if (I_width_prev < inf && I_width > shrink * I_width_prev)
+                shrink_fail += 1;
+            else
+                shrink_fail = 0;
+            if (shrink_fail >= 2) {
+                new_step = (I_lo + I_hi) / 2.0;
+                shrink_fail = 0;
+            }
+        }
+        step = new_step;
+    }
+    // Trials exhausted (:683-705): the step the last update produced if
+    // psi(last evaluated f, that step) <= psi(I_lo), else I_lo (or the initial
+    // step when I_lo is not positive).  As in the reference, the first answer
+    // keeps the last trial's f and gradient (recompute_T stays false there)
+    // even though the returned step was not itself evaluated.
+    (void)last_step;
+    const double psit = fx - f0 - step * test_decr;
+    if (psit <= psiI_lo) {
+        *reuse_last = true;
+        return step;
+    }
+    if (I_lo <= 0.0) return init_step;
+    return I_lo;
+}
+
+struct Timer {
+    Clock::time_point t0;
+    void tic() { t0 = Clock::now(); }
+    double toc() { return ms_since(t0); }
+};
+
+}  // namespace
+
+// SSNS.  Mirrors sinkhorn_ssns_internal, src/sinkhorn_ssns.cpp:16-234:
+// trust-region-like mu control (rho thresholds 0.25 / 0.75, mu x4 / x0.5,
+// floor 1e-3), shift = mu ||g||, step candidates {1, 0.5, 0.25, 0.1},
+// delta = 0.01 ||g||, threshold sparsification with the previous density as
+// the hint.  The gradient after an accepted step is evaluated at the new
+// point (the reference reuses T from the last evaluated candidate, which is
+// the same point whenever a candidate lowered f).
+void solve_ssns(regot_problem* p, double tol, int max_iter, int verbose, const regot_opts& o,
+                regot_result* res)
+{
+    const double rho_t1 = 0.25, rho_t2 = 0.75, mu_s1 = 4.0, mu_s2 = 0.5, kappa = 0.001,
+                 nu0 = 0.01;
+    const double cand[4] = {1.0, 0.5, 0.25, 0.1};
+    PreciseScope ps(p);
+    Work w(p);
+    w.init(o);
+    double mu = o.mu0, rho = 1.0;
+    Timer timer;
+    timer.tic();
+    double gnorm;
+    double f = w.obj_grad(w.gamma, w.g, &gnorm);
+    double delta = nu0 * gnorm;
+    hess_threshold(p, w.gamma, w.g, delta, 0.001);
+    res->obj_vals.push_back(f);
+    res->mar_errs.push_back(gnorm);
+    res->run_times.push_back(timer.toc());
+    res->densities.push_back(hess_of(p).density());
+    int i;
+    for (i = 0; i < max_iter; i++) {
+        if (verbose >= 1) {
+            printf("iter = %d, objval = %g, ||grad|| = %g\n", i, f, gnorm);
+            fflush(stdout);
+        }
+        timer.tic();
+        if (gnorm < tol) break;
+        const double shift = mu * gnorm;
+        vec_axpby(w.rhs, -1.0, w.g, 0.0, nullptr, w.N, w.s);
+        w.lin_solve(o, w.rhs, shift, w.direc);
+        // line selection, src/sinkhorn_problem_ls.cpp:13-41
+        double best = 1.0, newf = std::numeric_limits<double>::infinity();
+        for (double a : cand) {
+            w.move(w.x, w.gamma, a, w.direc);
+            const double fa = w.obj(w.x);
+            if (fa < newf) {
+                best = a;
+                newf = fa;
+            }
+            if (newf < f) break;
+        }
+        vec_axpby(w.step, best, w.direc, 0.0, nullptr, w.N, w.s);
+        hess_apply_hs(p, w.step, w.Hstep);
+        const double* xs[2] = {w.g.p, w.step.p};
+        const double* ys[2] = {w.step.p, w.Hstep.p};
+        double d[2];
+        w.dots(2, xs, ys, d);
+        const double numer = f - newf;
+        const double denom = -d[0] - 0.5 * d[1];
+        rho = numer / denom;
+        if (rho > 0.0) vec_axpy(w.gamma, 1.0, w.step, w.N, w.s);
+        if (rho < rho_t1)
+            mu = mu_s1 * mu;
+        else if (rho > rho_t2)
+            mu = std::max(mu_s2 * mu, kappa);
+        if (rho > 0.0) {
+            f = w.obj_grad(w.gamma, w.g, &gnorm);
+            delta = nu0 * gnorm;
+            const double hint = hess_of(p).density();
+            hess_threshold(p, w.gamma, w.g, delta, hint);
+        }
+        res->obj_vals.push_back(f);
+        res->mar_errs.push_back(gnorm);
+        res->run_times.push_back(res->run_times.back() + timer.toc());
+        res->densities.push_back(hess_of(p).density());
+    }
+    res->niter = i;
+    res->cg_iters = w.cg_iters;
+    res->dual.resize(w.N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), w.gamma.p, w.N * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+// Sparse Newton and SPLR.  Mirror sinkhorn_sparse_newton_internal,
+// src/sinkhorn_sparse_newton.cpp:24-170, and sinkhorn_splr_internal,
+// src/sinkhorn_splr.cpp:24-199: density starts at 0.1 density_max within
+// [0.01, 1] x density_max; shift = min(||g||, shift_max); Wolfe search from
+// min(1, 1.5 alpha_prev); sparse Newton's density x2 on a bad move (||g||
+// above 1.1 x the previous once below the initial), else x0.9; SPLR's x1.1 /
+// x0.99 and its BFGS correction when y's > 1e-6 y'y (not in the first
+// iteration).  densities report the target density_T2H(density).
+void solve_newton_family(regot_problem* p, bool splr, double tol, int max_iter, int verbose,
+                         const regot_opts& o, regot_result* res)
+{
+    const int n = p->n, m = p->m;
+    const double density_max = o.density, density_min = 0.01 * o.density;
+    double density = 0.1 * density_max;
+    const double shift_max = o.shift;
+    const double eps = 1e-6;
+    auto density_T2H = [&](double denT) {
+        const double nnz = 2.0 * denT * n * (m - 1) + n + m - 1;
+        return nnz / static_cast<double>(n + m - 1) / static_cast<double>(n + m - 1);
+    };
+    PreciseScope ps(p);
+    Work w(p);
+    w.init(o);
+    launch_fill(w.gamma_pre, 0.0, w.N, w.s);
+    launch_fill(w.g_pre, 0.0, w.N, w.s);
+    Timer timer;
+    timer.tic();
+    double gnorm;
+    double f = w.obj_grad(w.gamma, w.g, &gnorm);
+    const double gnorm_init = gnorm;
+    hess_topk(p, w.gamma, w.g, density);
+    res->obj_vals.push_back(f);
+    res->mar_errs.push_back(gnorm);
+    res->run_times.push_back(timer.toc());
+    res->densities.push_back(density_T2H(density));
+    double alpha = 1.0;
+    int i;
+    for (i = 0; i < max_iter; i++) {
+        if (verbose >= 1) {
+            printf("iter = %d, objval = %g, ||grad|| = %g\n", i, f, gnorm);
+            fflush(stdout);
+        }
+        timer.tic();
+        if (gnorm < tol || !std::isfinite(f)) break;
+        const double shift = std::min(gnorm, shift_max);
+        vec_axpby(w.rhs, -1.0, w.g, 0.0, nullptr, w.N, w.s);
+        bool low_rank = false;
+        if (splr) {
+            vec_axpby(w.y, 1.0, w.g, -1.0, w.g_pre, w.N, w.s);
+            vec_axpby(w.sv, 1.0, w.gamma, -1.0, w.gamma_pre, w.N, w.s);
+            const double* xs[2] = {w.y.p, w.y.p};
+            const double* ys[2] = {w.sv.p, w.y.p};
+            double d[2];
+            w.dots(2, xs, ys, d);
+            low_rank = (i > 0) && (d[0] > eps * d[1]);
+        }
+        if (low_rank)
+            w.lin_solve_low_rank(o, w.rhs, shift, w.y, w.sv, w.direc);
+        else
+            w.lin_solve(o, w.rhs, shift, w.direc);
+        const double dg0 = w.dot(w.g, w.direc);
+        bool reuse = false;
+        double f_last = 0.0, gn_last = 0.0;
+        alpha = wolfe(w, std::min(1.0, 1.5 * alpha), f, dg0, &reuse, &f_last, &gn_last);
+        if (splr) vec_copy(w.gamma_pre, w.gamma, w.N, w.s);
+        vec_axpy(w.gamma, alpha, w.direc, w.N, w.s);
+        if (splr) vec_copy(w.g_pre, w.g, w.N, w.s);
+        const double gnorm_pre = gnorm;
+        if (reuse) {
+            vec_copy(w.g, w.xg, w.N, w.s);
+            f = f_last;
+            gnorm = gn_last;
+        } else {
+            f = w.obj_grad(w.gamma, w.g, &gnorm);
+        }
+        if (splr) {
+            const bool bad = (gnorm_pre < gnorm_init) && (gnorm > gnorm_pre);
+            density *= bad ? 1.1 : 0.99;
+        } else {
+            const bool bad = (gnorm_pre < gnorm_init) && (gnorm > 1.1 * gnorm_pre);
+            density *= bad ? 2.0 : 0.9;
+        }
+        density = std::min(density_max, std::max(density_min, density));
+        hess_topk(p, w.gamma, w.g, density);
+        res->obj_vals.push_back(f);
+        res->mar_errs.push_back(gnorm);
+        res->run_times.push_back(res->run_times.back() + timer.toc());
+        res->densities.push_back(density_T2H(density));
+    }
+    res->niter = i;
+    res->cg_iters = w.cg_iters;
+    res->dual.resize(w.N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), w.gamma.p, w.N * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+}  // namespace regot

@@ -218,7 +218,7 @@ void regot_problem::row_pass(const double* beta_full, const RowEpi& epi, const i
     prof_begin(0, &stop);
     RowEpi e = epi;
     e.reg = reg;
-    launch_row_pass(bulk, dM, ldm, n, m, beta_full, k, e, ctl_p, stream);
+    launch_row_pass(bulk, dM, ldm, n, m, beta_full, k, e, ctl_p, stream, precise);
     prof_end(0, stop);
 }
 
@@ -233,7 +233,7 @@ void regot_problem::col_pass(const double* alpha_v, const double* colpot_guess,
     e.logb = logb;
     e.b = b;
     launch_col_pass(bulk, dM, ldm, n, m, alpha_v, ap0, ap1, colpot_guess, logb, k, part, nblk, e,
-                    st, ctl_p, stream);
+                    st, ctl_p, stream, precise && st == nullptr);
     prof_end(1, stop);
 }
 

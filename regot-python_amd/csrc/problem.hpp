@@ -19,6 +19,9 @@ void vec_axpy(double* y, double alpha, const double* x, long count, hipStream_t 
 void vec_axpby(double* out, double alpha, const double* x, double beta, const double* y,
                long count, hipStream_t s);
 void vec_copy(double* dst, const double* src, long count, hipStream_t s);
+// gamma = [alpha + beta_{m-1}; beta_{0..m-2} - beta_{m-1}]
+void vec_normalise(const double* alpha, const double* beta, int n, int m, double* gamma,
+                   hipStream_t s);
 // out[0..npairs) = dot(x_i, y_i) for the listed pairs, one launch.
 void vec_dots(const double* const* xs, const double* const* ys, long count, int npairs,
               double* out, hipStream_t s);
@@ -80,6 +83,10 @@ struct regot_problem {
 
     // sparsified Hessian of the Newton-type drivers (sparse.hpp)
     std::unique_ptr<regot::SparseHessian> hess;
+
+    // FP32 bulk: run the general row / column passes with FP64 exp and sums
+    // (objective and gradient evaluations of the Newton drivers, verification).
+    bool precise = false;
 
     // profiling of the hot passes: [0] row pass, [1] column pass
     bool prof = false;

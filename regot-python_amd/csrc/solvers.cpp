@@ -7,16 +7,10 @@
 #include <deque>
 #include <vector>
 
+#include "drivers.hpp"
 #include "problem.hpp"
 #include "regot_hip.h"
 #include "sparse.hpp"
-
-struct regot_result {
-    int solver = 0;
-    int n = 0, m = 0;
-    int niter = 0;
-    std::vector<double> dual, obj_vals, mar_errs, run_times, densities;
-};
 
 namespace regot {
 
@@ -212,7 +206,8 @@ void regot_opts_default(regot_opts* o, int solver)
     if (!o) return;
     // SinkhornSolverOpts defaults, src/sinkhorn_solvers.h:35-37, with the
     // per-solver method override of src/solvers.cpp:415,433,451.
-    o->method = (solver == REGOT_SOLVER_BCD) ? 1 : 1;
+    (void)solver;
+    o->method = 1;
     o->mu0 = 1.0;
     o->shift = 1e-3;
     o->density = 0.01;
@@ -243,6 +238,15 @@ int regot_solve(regot_problem* p, int solver, double tol, int max_iter, int verb
     switch (solver) {
         case REGOT_SOLVER_BCD:
             solve_bcd(p, tol, max_iter, verbose, o, res.get());
+            break;
+        case REGOT_SOLVER_SSNS:
+            solve_ssns(p, tol, max_iter, verbose, o, res.get());
+            break;
+        case REGOT_SOLVER_SPARSE_NEWTON:
+            solve_newton_family(p, false, tol, max_iter, verbose, o, res.get());
+            break;
+        case REGOT_SOLVER_SPLR:
+            solve_newton_family(p, true, tol, max_iter, verbose, o, res.get());
             break;
         default:
             fail(E_INVALID, "unknown solver id %d", solver);
@@ -298,15 +302,19 @@ int regot_result_verify(regot_problem* p, const regot_result* r, double* mar_err
     hipStream_t s = p->stream;
     HIP_CHECK(hipMemcpyAsync(p->gamma, r->dual.data(), (n + m - 1) * sizeof(double),
                              hipMemcpyHostToDevice, s));
-    launch_split_gamma(p->gamma, n, m, p->alpha, p->beta, s);
-    launch_verify_sums(p->bulk, p->dM, p->ldm, n, m, p->alpha, p->beta, p->reg, p->rsum,
-                       p->csum, s);
-    vec_objgrad_finish(p->rsum, p->csum, p->a, p->b, p->gamma, n, m, p->reg, p->grad, p->scal,
-                       s);
-    HIP_CHECK(hipMemcpyAsync(p->hscal.p, p->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    p->sync();
-    if (obj_fp64) *obj_fp64 = p->hscal.p[0];
-    if (mar_err_fp64) *mar_err_fp64 = std::sqrt(p->hscal.p[1]);
+    const bool was = p->precise;
+    p->precise = true;
+    double gn2 = 0.0;
+    double f = 0.0;
+    try {
+        f = p->dev_obj_grad(p->gamma, p->grad, &gn2);
+    } catch (...) {
+        p->precise = was;
+        throw;
+    }
+    p->precise = was;
+    if (obj_fp64) *obj_fp64 = f;
+    if (mar_err_fp64) *mar_err_fp64 = std::sqrt(gn2);
     CAPI_END
 }
 

@@ -82,6 +82,14 @@ __global__ __launch_bounds__(1024) void k_objgrad_finish(const double* r, const 
     }
 }
 
+__global__ void k_normalise(const double* alpha, const double* beta, int n, int m, double* gamma)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const double bm = beta[m - 1];
+    if (t < n) gamma[t] = alpha[t] + bm;
+    if (t < m - 1) gamma[n + t] = beta[t] - bm;
+}
+
 int grid_for(long count)
 {
     long g = (count + 255) / 256;
@@ -105,6 +113,15 @@ void vec_axpby(double* out, double alpha, const double* x, double beta, const do
     if (count <= 0) return;
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(count)), dim3(256), 0, s, out, alpha, x, beta, y,
                        count);
+    KERNEL_CHECK();
+}
+
+void vec_normalise(const double* alpha, const double* beta, int n, int m, double* gamma,
+                   hipStream_t s)
+{
+    const int c = n > m ? n : m;
+    hipLaunchKernelGGL(k_normalise, dim3((c + 255) / 256), dim3(256), 0, s, alpha, beta, n, m,
+                       gamma);
     KERNEL_CHECK();
 }
 
