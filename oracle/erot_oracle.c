@@ -805,6 +805,9 @@ static double density_T2H(double denT, int n, int m)
     return nnz / (double)(n + m - 1) / (double)(n + m - 1);
 }
 
+void orc_low_rank_combine(long N, const double* iAr, const double* iAy, const double* r,
+                          const double* y, const double* s, double* res);
+
 static double opt_cg_tol(const orc_opts* o) { return (o && o->cg_tol > 0) ? o->cg_tol : 1e-8; }
 
 /* SinkhornLinearSolver::solve, src/sinkhorn_linsolve.cpp:58-106 */
@@ -837,13 +840,22 @@ static void lin_solve_low_rank(const prob_t* P, const hess_t* H, const orc_opts*
         orc_hess_direct(P->n, P->m, H->h1, H->h2, H->colptr, H->rowind, H->val, rhs, shift, iAr);
         orc_hess_direct(P->n, P->m, H->h1, H->h2, H->colptr, H->rowind, H->val, y, shift, iAy);
     }
-    const double rs = dot(s, rhs, N), ys = dot(y, s, N);
+    orc_low_rank_combine(N, iAr, iAy, rhs, y, s, res);
+    free(iAr);
+    free(iAy);
+}
+
+/* res = iAr + ((1 + y'iAy / y's) s'r / y's - y'iAr / y's) s - (s'r / y's) iAy, the
+ * combination step of SinkhornLinearSolver::solve_low_rank,
+ * src/sinkhorn_linsolve.cpp:207-213, given iAr = A^-1 r and iAy = A^-1 y. */
+void orc_low_rank_combine(long N, const double* iAr, const double* iAy, const double* r,
+                          const double* y, const double* s, double* res)
+{
+    const double rs = dot(s, r, N), ys = dot(y, s, N);
     const double yiAy = dot(y, iAy, N), yiAr = dot(y, iAr, N);
     const double rs_ys = rs / ys;
     const double cs = (1.0 + yiAy / ys) * rs_ys - yiAr / ys;
     for (long i = 0; i < N; i++) res[i] = iAr[i] + cs * s[i] - rs_ys * iAy[i];
-    free(iAr);
-    free(iAy);
 }
 
 /* gamma = [alpha + beta_{m-1}; beta_{0..m-2} - beta_{m-1}];

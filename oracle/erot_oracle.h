@@ -63,6 +63,11 @@ int orc_hess_direct(int n, int m, const double* h1, const double* h2,
                     const int* colptr, const int* rowind, const double* val,
                     const double* rhs, double shift, double* res);
 
+/* SPLR's inverse-BFGS combination given A^-1 r and A^-1 y
+ * (src/sinkhorn_linsolve.cpp:207-213). */
+void orc_low_rank_combine(long N, const double* iAr, const double* iAy, const double* r,
+                          const double* y, const double* s, double* res);
+
 /* ---- Line-search helpers (src/sinkhorn_problem_ls.cpp:72-223) ---- */
 double orc_cubic_minimizer(double a, double b, double fa, double fb, double ga, double gb,
                            int* exists);

@@ -211,6 +211,15 @@ def hess_direct(n, m, h1, h2, csc, rhs, shift):
     return res
 
 
+def low_rank_combine(iAr, iAy, r, y, s):
+    """src/sinkhorn_linsolve.cpp:207-213"""
+    iAr, iAy, r, y, s = (_d(v) for v in (iAr, iAy, r, y, s))
+    out = np.empty_like(r)
+    lib().orc_low_rank_combine(ctypes.c_long(len(r)), _p(iAr), _p(iAy), _p(r), _p(y), _p(s),
+                               _p(out))
+    return out
+
+
 def cubic_minimizer(a, b, fa, fb, ga, gb):
     ex = ctypes.c_int(0)
     r = lib().orc_cubic_minimizer(*(ctypes.c_double(v) for v in (a, b, fa, fb, ga, gb)),
