@@ -131,7 +131,7 @@ typedef struct {
     double cg_tol;       /* <= 0: driver default */
     int cg_max_iter;     /* <= 0: driver default */
     const double* x0;    /* optional warm start (n+m-1), host; NULL = reference init */
-    int graph_batch;     /* BCD iterations per captured hipGraph launch; <= 0: default */
+    int bcd_batch;       /* BCD iterations enqueued per batch (two in flight); <= 0: 8 */
 } regot_opts;
 
 void regot_opts_default(regot_opts* o, int solver);

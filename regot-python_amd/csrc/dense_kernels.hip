@@ -12,7 +12,9 @@
 // the FP32 rounding of the exponent argument is ~1e-7 relative, and nothing
 // overflows or underflows whatever the guess.  In steady state (good guess)
 // the shift never moves and the inner loop is: 4 FP64-rate ops (cvt, add,
-// fma, cvt), one max, one exp2 and one add per element.
+// fma, cvt), one max, one exp2 and one add per element; every 16 terms the
+// FP32 lane sum is moved into an FP64 total.  M is read with nontemporal
+// loads (it is streamed once per pass).
 //
 // Partial results combine as FP64 log2-sums: log2(sum) = L_max + log2(sum
 // 2^(L - L_max)).  All reductions run in a fixed order, so results are
@@ -40,7 +42,11 @@ struct VecOf<double> {
     static constexpr int N = 2;
 };
 
-__device__ __forceinline__ float ex2(float x) { return exp2f(x); }
+// v_exp_f32 directly: exp2f adds a denormal-range rescale (compare, two
+// selects, add, ldexp) around it.  Results below 2^-126 flush to 0; with the
+// adaptive shift every summed term is relative to a running maximum >= 2^-8,
+// so a flushed term is below 2^-118 of the sum.
+__device__ __forceinline__ float ex2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ double ex2(double x) { return exp2(x); }
 __device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ double vmax(double a, double b) { return fmax(a, b); }
@@ -62,6 +68,22 @@ template <>
 __device__ __forceinline__ double vget<double2, double>(const double2& v, int i)
 {
     return i == 0 ? v.x : v.y;
+}
+
+// Streaming load of one vector of M.  Nontemporal: M is read once per pass
+// and far larger than L2 and MALL, so it should not displace the potentials;
+// measured on a 1 GiB row stream this is ~6% faster than a default load.
+__device__ __forceinline__ float4 ld_stream(const float* p)
+{
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ double2 ld_stream(const double* p)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 v = __builtin_nontemporal_load(reinterpret_cast<const d2*>(p));
+    return make_double2(v.x, v.y);
 }
 
 __device__ __forceinline__ double wave_max(double v)
@@ -88,30 +110,47 @@ __device__ __forceinline__ double wave_log2sum(double L)
     return Lmax + log2(wave_sum0(e));
 }
 
-template <typename TA>
-__device__ __forceinline__ double acc_log2(double sh, TA s)
+__device__ __forceinline__ double acc_log2(double sh, double s)
 {
-    return (s > TA(0)) ? sh + log2(static_cast<double>(s)) : -INFINITY;
+    return (s > 0.0) ? sh + log2(s) : -INFINITY;
+}
+
+// Each lane sums 2^t in the bulk type TA into s and, in FP32, moves s into an
+// FP64 total S every kFlush terms, so FP32 rounding accumulates over 16 terms
+// at most (<= 1e-6 relative, ~2e-7 typical) whatever the row or column length.
+constexpr int kFlushSteps = 4;   // steps of 4 terms between FP32 -> FP64 flushes
+
+template <typename TA>
+__device__ __forceinline__ void flush(TA& s, double& S)
+{
+    if (sizeof(TA) == 4) {
+        S += static_cast<double>(s);
+        s = TA(0);
+    }
 }
 
 // Move the shift when the running max leaves [-8, 8]: values up to 2^8 above
 // the shift keep the bulk-type rounding of the exponent argument below
 // 2^-16 ulp-units (~2e-7 relative), and a lane moves at most once downwards
-// (first values) and rarely upwards.  Returns true if moved.
+// (first values) and rarely upwards.  The partial sums are folded into S and
+// rescaled by 2^-d in FP64.  Returns true if moved.
 template <typename TA>
-__device__ __forceinline__ bool reshift(double& sh, TA& s, TA& rm)
+__device__ __forceinline__ bool reshift(double& sh, TA& s, double& S, TA& rm)
 {
     if (rm > TA(8) || (rm < TA(-8) && rm > ninf<TA>())) {
         const TA d = rm;
         sh += static_cast<double>(d);
-        if (s != TA(0)) {
-            // s holds terms <= 2^d, so s * 2^-d is representable; only the factor
-            // 2^-d alone can overflow (d < -126), which needs the FP64 route.
-            if (d > TA(-120))
-                s *= ex2(-d);
-            else
-                s = static_cast<TA>(static_cast<double>(s) * exp2(-static_cast<double>(d)));
-        }
+        const double tot = S + static_cast<double>(s);
+        // tot holds terms <= 2^d, so tot * 2^-d is representable; the factor
+        // alone needs FP64 when d < -126.  FP32 bulk takes the FP32 exp2 of the
+        // factor (one rounding, like one more term) unless it would overflow.
+        double f;
+        if (sizeof(TA) == 4 && d > TA(-120))
+            f = static_cast<double>(ex2(static_cast<float>(-d)));
+        else
+            f = exp2(-static_cast<double>(d));
+        S = (tot != 0.0) ? tot * f : 0.0;
+        s = TA(0);
         rm = TA(0);
         return true;
     }
@@ -125,7 +164,8 @@ __device__ __forceinline__ bool reshift(double& sh, TA& s, TA& rm)
 // One row x V columns: t_v = (bk_v - sh) - k M_v, masked entries = -inf.
 // TM is the storage type of M, TA the type t, exp2 and the sums run in.
 template <typename TM, typename TA, int V, bool MASK>
-__device__ __forceinline__ void row_accumulate(double& sh, TA& s, TA& rm, const double* bk,
+__device__ __forceinline__ void row_accumulate(double& sh, TA& s, double& S, TA& rm,
+                                               const double* bk,
                                                const typename VecOf<TM>::type& mv, double k,
                                                int valid_cols)
 {
@@ -139,7 +179,7 @@ __device__ __forceinline__ void row_accumulate(double& sh, TA& s, TA& rm, const 
         cm = vmax(cm, t[v]);
     }
     rm = vmax(rm, cm);
-    if (reshift(sh, s, rm)) {
+    if (reshift(sh, s, S, rm)) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const double mvd = static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv, v));
@@ -195,7 +235,8 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
     constexpr int V = VecOf<TM>::N;
     if (ctl != nullptr && ctl[0] != 0) return;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // wave-uniform: row tests below become scalar branches, not exec masks
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long row0 = (static_cast<long>(blockIdx.x) * 4 + wave) * R;
 
     const double* rowpot = epi.rowpot;
@@ -208,7 +249,7 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
     double wg2 = 0.0, wr = 0.0, waa = 0.0;
 
     if (row0 < n) {
-        double sh[R];
+        double sh[R], S[R];
         TA s[R], rm[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -220,6 +261,7 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
             }
             sh[r] = g;
             s[r] = TA(0);
+            S[r] = 0.0;
             rm[r] = ninf<TA>();
         }
         const TM* rowp[R];
@@ -228,20 +270,34 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
 
         const int step = 64 * V;
         const int nfull = m / step;
-#pragma unroll 2
+        // Rows past n read row0 again (in bounds) and are not accumulated.
+        // The next step's vectors are loaded before the current ones are
+        // consumed, so each wave keeps R loads in flight while it computes.
+        VT mv[R];
+        if (nfull > 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) mv[r] = ld_stream(rowp[r] + lane * V);
+        }
         for (int c = 0; c < nfull; c++) {
             const int j0 = c * step + lane * V;
+            VT nx[R];
+            if (c + 1 < nfull) {
+#pragma unroll
+                for (int r = 0; r < R; r++) nx[r] = ld_stream(rowp[r] + j0 + step);
+            }
             double bk[V];
 #pragma unroll
             for (int v = 0; v < V; v++) bk[v] = beta[j0 + v] * k;
-            VT mv[R];
-#pragma unroll
-            for (int r = 0; r < R; r++)
-                if (row0 + r < n) mv[r] = *reinterpret_cast<const VT*>(rowp[r] + j0);
 #pragma unroll
             for (int r = 0; r < R; r++)
                 if (row0 + r < n)
-                    row_accumulate<TM, TA, V, false>(sh[r], s[r], rm[r], bk, mv[r], k, V);
+                    row_accumulate<TM, TA, V, false>(sh[r], s[r], S[r], rm[r], bk, mv[r], k, V);
+            if (c % kFlushSteps == kFlushSteps - 1) {
+#pragma unroll
+                for (int r = 0; r < R; r++) flush(s[r], S[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) mv[r] = nx[r];
         }
         {
             const int j0 = nfull * step + lane * V;
@@ -254,8 +310,8 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
                 for (int r = 0; r < R; r++) {
                     if (row0 + r < n) {
                         // ldm >= m rounded up to V: the vector stays inside the row pitch.
-                        const VT mv = *reinterpret_cast<const VT*>(rowp[r] + j0);
-                        row_accumulate<TM, TA, V, true>(sh[r], s[r], rm[r], bk, mv, k, valid);
+                        const VT mv = ld_stream(rowp[r] + j0);
+                        row_accumulate<TM, TA, V, true>(sh[r], s[r], S[r], rm[r], bk, mv, k, valid);
                     }
                 }
             }
@@ -265,7 +321,7 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
         for (int r = 0; r < R; r++) {
             const long i = row0 + r;
             if (i >= n) break;
-            const double L = wave_log2sum(acc_log2(sh[r], s[r]));
+            const double L = wave_log2sum(acc_log2(sh[r], S[r] + static_cast<double>(s[r])));
             if (lane == 0) {
                 if (epi.lse2) epi.lse2[i] = L;
                 if (aout) aout[i] = epi.reg * (epi.loga[i] - kLn2 * L);
@@ -307,7 +363,7 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
 // ---------------------------------------------------------------------------
 
 template <typename TM, typename TA, int V, int CH, bool MASK>
-__device__ __forceinline__ void col_chunk(double* sh, TA* s, TA* rm,
+__device__ __forceinline__ void col_chunk(double* sh, TA* s, double* S, TA* rm,
                                           const typename VecOf<TM>::type* mv, const double* ak,
                                           double k, int valid_rows, int valid_cols)
 {
@@ -323,7 +379,7 @@ __device__ __forceinline__ void col_chunk(double* sh, TA* s, TA* rm,
             cm = vmax(cm, t[c]);
         }
         rm[v] = vmax(rm[v], cm);
-        if (reshift(sh[v], s[v], rm[v])) {
+        if (reshift(sh[v], s[v], S[v], rm[v])) {
 #pragma unroll
             for (int c = 0; c < CH; c++) {
                 const double mvd =
@@ -355,13 +411,13 @@ __global__ __launch_bounds__(256) void k_col_partial(
     if (ctl != nullptr && ctl[0] != 0) return;
     if (ap0 != nullptr) alpha = (ctl[1] & 1) ? ap1 : ap0;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int j0 = blockIdx.x * TW + lane * V;
     const int valid_cols = (m - j0) < V ? (m - j0) : V;   // <= 0: lane has no column
     const long rb = static_cast<long>(blockIdx.y) * rpb;
     const long re = (rb + rpb < n) ? rb + rpb : n;
 
-    double sh[V];
+    double sh[V], S[V];
     TA s[V], rm[V];
 #pragma unroll
     for (int v = 0; v < V; v++) {
@@ -373,24 +429,29 @@ __global__ __launch_bounds__(256) void k_col_partial(
         }
         sh[v] = g;
         s[v] = TA(0);
+        S[v] = 0.0;
         rm[v] = ninf<TA>();
     }
 
     if (valid_cols > 0) {
         const TM* base = M + j0;
         long r = rb + wave * CH;
-        for (; r + CH <= re; r += 4 * CH) {
+        for (int q = 0; r + CH <= re; r += 4 * CH, q++) {
             VT mv[CH];
             double ak[CH];
 #pragma unroll
             for (int c = 0; c < CH; c++) {
-                mv[c] = *reinterpret_cast<const VT*>(base + (r + c) * ldm);
+                mv[c] = ld_stream(base + (r + c) * ldm);
                 ak[c] = alpha[r + c] * k;
             }
             if (valid_cols == V)
-                col_chunk<TM, TA, V, CH, false>(sh, s, rm, mv, ak, k, CH, V);
+                col_chunk<TM, TA, V, CH, false>(sh, s, S, rm, mv, ak, k, CH, V);
             else
-                col_chunk<TM, TA, V, CH, true>(sh, s, rm, mv, ak, k, CH, valid_cols);
+                col_chunk<TM, TA, V, CH, true>(sh, s, S, rm, mv, ak, k, CH, valid_cols);
+            if (q % kFlushSteps == kFlushSteps - 1) {
+#pragma unroll
+                for (int v = 0; v < V; v++) flush(s[v], S[v]);
+            }
         }
         if (r < re) {
             const int vr = static_cast<int>(re - r);
@@ -399,15 +460,16 @@ __global__ __launch_bounds__(256) void k_col_partial(
 #pragma unroll
             for (int c = 0; c < CH; c++) {
                 const long rr = (c < vr) ? r + c : r;
-                mv[c] = *reinterpret_cast<const VT*>(base + rr * ldm);
+                mv[c] = ld_stream(base + rr * ldm);
                 ak[c] = alpha[rr] * k;
             }
-            col_chunk<TM, TA, V, CH, true>(sh, s, rm, mv, ak, k, vr, valid_cols);
+            col_chunk<TM, TA, V, CH, true>(sh, s, S, rm, mv, ak, k, vr, valid_cols);
         }
     }
 
 #pragma unroll
-    for (int v = 0; v < V; v++) red[wave][lane * V + v] = acc_log2(sh[v], s[v]);
+    for (int v = 0; v < V; v++)
+        red[wave][lane * V + v] = acc_log2(sh[v], S[v] + static_cast<double>(s[v]));
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
@@ -656,9 +718,11 @@ void launch_row_pass_bcd(int bulk, const void* M, long ldm, int n, int m, const 
 
 int col_pass_blocks(int n, int m)
 {
-    // Aim for >= 2048 workgroups (8 per CU on 256 CUs) with >= 16 rows each.
+    // Aim for >= 512 workgroups (2 per CU on 256 CUs) with >= 16 rows each:
+    // long column streams per workgroup measured fastest (16384^2 FP32:
+    // 163 us at 512 workgroups, 175 us at 2048).
     const int tiles = ceil_div(m, 256);
-    int nblk = ceil_div(2048, tiles);
+    int nblk = ceil_div(512, tiles);
     const int maxblk = ceil_div(n, 16);
     if (nblk > maxblk) nblk = maxblk;
     if (nblk < 1) nblk = 1;

@@ -144,6 +144,13 @@ void regot_problem::setup(int n_, int m_, const void* M, int M_dtype, int bulk_,
         }
     }
 
+    // sum a and sum_{j<m-1} b (host FP64, fixed order) for the objective
+    sum_a = 0.0;
+    sum_b1 = 0.0;
+    for (int i = 0; i < n; i++) sum_a += a_h[i];
+    for (int j = 0; j < m - 1; j++) sum_b1 += b_h[j];
+    HIP_CHECK(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device));
+
     a.alloc(n);
     b.alloc(m);
     loga.alloc(n);

@@ -70,6 +70,9 @@ struct regot_problem {
     long ldt = 0;
     double reg = 1.0;
     double k = 1.0;             // log2(e) / reg
+    double sum_a = 0.0;         // sum_i a_i
+    double sum_b1 = 0.0;        // sum_{j < m-1} b_j
+    int wall_khz = 0;           // device wall clock rate (timestamps)
     regot::DBuf<double> a, b, loga, logb;
 
     // scratch for the operator entry points and the drivers

@@ -31,12 +31,17 @@ void normalise_host(int n, int m, const double* alpha, const double* beta, doubl
 // alpha = 0, beta = optimal_beta(alpha); at iterate k test ||g_k|| < tol first,
 // then alpha <- optimal_alpha(beta), beta <- optimal_beta(alpha).
 //
-// Each iteration is two launches on one stream:
-//   row pass     beta_k -> alpha_{k+1} (double buffer) and the row sums at
-//                iterate k; its last workgroup records f_k, ||g_k|| and a
-//                timestamp, then stops or advances k;
-//   column pass  alpha_{k+1} -> beta_{k+1}, plus the column residual of the
-//                update (g_{k+1}'s column part, zero up to FP64 rounding).
+// Each iteration is three launches on one stream:
+//   row pass        beta_k -> alpha_{k+1} (double buffer) and the row sums at
+//                   iterate k; each workgroup writes its partials of
+//                   ||r - a||^2, sum r and alpha'a;
+//   column partial  per-tile log2 sums of column j at alpha_{k+1};
+//   column combine  every workgroup reduces the row partials and takes the
+//                   stop decision for iterate k itself (same data, same
+//                   order); unless stopping it writes beta_{k+1} and the
+//                   tile sums of the new column residual (g_{k+1}'s column
+//                   part, zero up to FP64 rounding).  Workgroup 0 records
+//                   f_k, ||g_k|| and a timestamp and stops or advances k.
 // Once the stop flag is set every later kernel returns at once, so the host
 // keeps two batches of iterations in flight and only polls the flag.
 void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const regot_opts& o,
@@ -44,7 +49,10 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
 {
     const int n = p->n, m = p->m;
     hipStream_t s = p->stream;
-    const int K = o.graph_batch > 0 ? o.graph_batch : 8;
+    // With kernel profiling on, enqueue one iteration at a time so that no
+    // launch is enqueued past the stop decision: every timed pass does work.
+    const bool serial = p->prof;
+    const int K = serial ? 1 : (o.bcd_batch > 0 ? o.bcd_batch : 8);
     const int cap = std::max(4096, 4 * K + 8);
 
     if (!p->bcd) p->bcd = std::make_unique<BcdWork>();
@@ -64,14 +72,7 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
     for (auto& e : w.ev)
         if (!e) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
-    // sum a and sum_{j<m-1} b for the objective (host FP64, fixed order)
     std::vector<double> ah(n), bh(m);
-    HIP_CHECK(hipMemcpyAsync(ah.data(), p->a.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(bh.data(), p->b.p, m * sizeof(double), hipMemcpyDeviceToHost, s));
-    p->sync();
-    double sum_a = 0.0, sum_b1 = 0.0;
-    for (int i = 0; i < n; i++) sum_a += ah[i];
-    for (int j = 0; j < m - 1; j++) sum_b1 += bh[j];
 
     ColEpi c0;
     c0.resid = w.gcol;
@@ -104,8 +105,8 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
     st.cpart = w.cpart;
     st.nrb = nrb;
     st.ntiles = p->ntiles;
-    st.sum_a = sum_a;
-    st.sum_b1 = sum_b1;
+    st.sum_a = p->sum_a;
+    st.sum_b1 = p->sum_b1;
     st.tol = tol;
     st.reg = p->reg;
     st.max_iter = max_iter;
@@ -134,9 +135,7 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
         HIP_CHECK(hipEventRecord(w.ev[slot], s));
     };
 
-    int wall_khz = 0;
-    HIP_CHECK(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, p->device));
-    const double tick_ms = wall_khz > 0 ? 1.0 / wall_khz : 0.0;
+    const double tick_ms = p->wall_khz > 0 ? 1.0 / p->wall_khz : 0.0;
 
     int h_read = 0, niter = 0;
     long long ts_init = 0;
@@ -144,8 +143,10 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
     try {
         launch_batch(0);
         q.push_back(0);
-        launch_batch(1);
-        q.push_back(1);
+        if (!serial) {
+            launch_batch(1);
+            q.push_back(1);
+        }
         for (;;) {
             const int slot = q.front();
             q.pop_front();
@@ -214,7 +215,7 @@ void regot_opts_default(regot_opts* o, int solver)
     o->cg_tol = 0.0;
     o->cg_max_iter = 0;
     o->x0 = nullptr;
-    o->graph_batch = 0;
+    o->bcd_batch = 0;
 }
 
 int regot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,

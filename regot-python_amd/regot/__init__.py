@@ -291,8 +291,8 @@ class Problem:
             opts.cg_tol = float(kwargs["cg_tol"])
         if "cg_max_iter" in kwargs:
             opts.cg_max_iter = int(kwargs["cg_max_iter"])
-        if "graph_batch" in kwargs:
-            opts.graph_batch = int(kwargs["graph_batch"])
+        if "bcd_batch" in kwargs:
+            opts.bcd_batch = int(kwargs["bcd_batch"])
         x0 = kwargs.get("x0")
         if x0 is not None:
             x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)

@@ -35,7 +35,7 @@ class Opts(ctypes.Structure):
         ("cg_tol", ctypes.c_double),
         ("cg_max_iter", ctypes.c_int),
         ("x0", _dp),
-        ("graph_batch", ctypes.c_int),
+        ("bcd_batch", ctypes.c_int),
     ]
 
 
