@@ -162,6 +162,10 @@ int regot_result_plan(regot_problem* p, const regot_result* r, double* plan);
  * (exp in FP64 regardless of bulk), plus the FP64 objective. */
 int regot_result_verify(regot_problem* p, const regot_result* r, double* mar_err_fp64,
                         double* obj_fp64);
+/* Per-phase statistics of the Newton-type drivers (no reference counterpart;
+ * zeros for BCD): out[8] = {objective/gradient ms, count, sparsification ms,
+ * count, CG ms, CG solves, CG iterations, 0}.  Host wall time per phase. */
+int regot_result_stats(const regot_result* r, double* out);
 void regot_result_free(regot_result* r);
 
 /* Device-side timing of the hot kernels (HIP events on the handle's stream).

@@ -11,7 +11,11 @@ struct regot_result {
     int n = 0, m = 0;
     int niter = 0;
     std::vector<double> dual, obj_vals, mar_errs, run_times, densities;
-    // extras, not in the reference result: total CG iterations of the Newton drivers
+    // extras, not in the reference result (Newton drivers): host wall time and
+    // counts per phase -- [0] objective / gradient passes, [1] Hessian
+    // sparsification, [2] CG solves -- and the total CG iterations
+    double phase_ms[3] = {0.0, 0.0, 0.0};
+    long long phase_n[3] = {0, 0, 0};
     long long cg_iters = 0;
 };
 

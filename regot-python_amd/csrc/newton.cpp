@@ -42,6 +42,9 @@ struct Work {
     DBuf<double> dscal;
     HBuf<double> hscal;
     long long cg_iters = 0;
+    // per-phase wall time (each phase ends in a host synchronisation)
+    double phase_ms[3] = {0.0, 0.0, 0.0};
+    long long phase_n[3] = {0, 0, 0};
 
     explicit Work(regot_problem* pp)
         : p(pp), n(pp->n), m(pp->m), N(pp->n + pp->m - 1), s(pp->stream)
@@ -93,14 +96,49 @@ struct Work {
         vec_axpby(out, 1.0, x0, t, d, N, s);
     }
 
-    double obj(const double* at) { return p->dev_obj(at); }
+    double obj(const double* at)
+    {
+        const auto t0 = Clock::now();
+        const double f = p->dev_obj(at);
+        phase_ms[0] += ms_since(t0);
+        phase_n[0] += 1;
+        return f;
+    }
 
     double obj_grad(const double* at, double* grad, double* gnorm)
     {
+        const auto t0 = Clock::now();
         double gn2 = 0.0;
         const double f = p->dev_obj_grad(at, grad, &gn2);
         if (gnorm) *gnorm = std::sqrt(gn2);
+        phase_ms[0] += ms_since(t0);
+        phase_n[0] += 1;
         return f;
+    }
+
+    void threshold(double delta, double hint)
+    {
+        const auto t0 = Clock::now();
+        hess_threshold(p, gamma, g, delta, hint);
+        phase_ms[1] += ms_since(t0);
+        phase_n[1] += 1;
+    }
+
+    void topk(double density)
+    {
+        const auto t0 = Clock::now();
+        hess_topk(p, gamma, g, density);
+        phase_ms[1] += ms_since(t0);
+        phase_n[1] += 1;
+    }
+
+    void report(regot_result* res) const
+    {
+        res->cg_iters = cg_iters;
+        for (int q = 0; q < 3; q++) {
+            res->phase_ms[q] = phase_ms[q];
+            res->phase_n[q] = phase_n[q];
+        }
     }
 
     // (Hs + shift I)^{-1} rhs.  method 0 is the reference's CG (relative
@@ -113,7 +151,10 @@ struct Work {
         const double tol = o.cg_tol > 0 ? o.cg_tol : (o.method == 0 ? 1e-8 : 1e-10);
         const int maxit = o.cg_max_iter > 0 ? o.cg_max_iter
                                             : (o.method == 0 ? 2 * m : 10 * m + 1000);
+        const auto t0 = Clock::now();
         cg_iters += hess_cg(p, r, shift, tol, maxit, out, nullptr);
+        phase_ms[2] += ms_since(t0);
+        phase_n[2] += 1;
     }
 
     // Inverse-BFGS correction of (Hs + shift I)^{-1} applied to r, with the
@@ -353,7 +394,7 @@ void solve_ssns(regot_problem* p, double tol, int max_iter, int verbose, const r
     double gnorm;
     double f = w.obj_grad(w.gamma, w.g, &gnorm);
     double delta = nu0 * gnorm;
-    hess_threshold(p, w.gamma, w.g, delta, 0.001);
+    w.threshold(delta, 0.001);
     res->obj_vals.push_back(f);
     res->mar_errs.push_back(gnorm);
     res->run_times.push_back(timer.toc());
@@ -397,8 +438,7 @@ void solve_ssns(regot_problem* p, double tol, int max_iter, int verbose, const r
         if (rho > 0.0) {
             f = w.obj_grad(w.gamma, w.g, &gnorm);
             delta = nu0 * gnorm;
-            const double hint = hess_of(p).density();
-            hess_threshold(p, w.gamma, w.g, delta, hint);
+            w.threshold(delta, hess_of(p).density());
         }
         res->obj_vals.push_back(f);
         res->mar_errs.push_back(gnorm);
@@ -406,7 +446,7 @@ void solve_ssns(regot_problem* p, double tol, int max_iter, int verbose, const r
         res->densities.push_back(hess_of(p).density());
     }
     res->niter = i;
-    res->cg_iters = w.cg_iters;
+    w.report(res);
     res->dual.resize(w.N);
     HIP_CHECK(hipMemcpy(res->dual.data(), w.gamma.p, w.N * sizeof(double), hipMemcpyDeviceToHost));
 }
@@ -441,7 +481,7 @@ void solve_newton_family(regot_problem* p, bool splr, double tol, int max_iter, 
     double gnorm;
     double f = w.obj_grad(w.gamma, w.g, &gnorm);
     const double gnorm_init = gnorm;
-    hess_topk(p, w.gamma, w.g, density);
+    w.topk(density);
     res->obj_vals.push_back(f);
     res->mar_errs.push_back(gnorm);
     res->run_times.push_back(timer.toc());
@@ -494,14 +534,14 @@ void solve_newton_family(regot_problem* p, bool splr, double tol, int max_iter, 
             density *= bad ? 2.0 : 0.9;
         }
         density = std::min(density_max, std::max(density_min, density));
-        hess_topk(p, w.gamma, w.g, density);
+        w.topk(density);
         res->obj_vals.push_back(f);
         res->mar_errs.push_back(gnorm);
         res->run_times.push_back(res->run_times.back() + timer.toc());
         res->densities.push_back(density_T2H(density));
     }
     res->niter = i;
-    res->cg_iters = w.cg_iters;
+    w.report(res);
     res->dual.resize(w.N);
     HIP_CHECK(hipMemcpy(res->dual.data(), w.gamma.p, w.N * sizeof(double), hipMemcpyDeviceToHost));
 }

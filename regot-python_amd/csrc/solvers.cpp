@@ -319,6 +319,18 @@ int regot_result_verify(regot_problem* p, const regot_result* r, double* mar_err
     CAPI_END
 }
 
+int regot_result_stats(const regot_result* r, double* out)
+{
+    if (!r || !out) return set_error(E_INVALID, "null argument");
+    for (int q = 0; q < 3; q++) {
+        out[2 * q] = r->phase_ms[q];
+        out[2 * q + 1] = static_cast<double>(r->phase_n[q]);
+    }
+    out[6] = static_cast<double>(r->cg_iters);
+    out[7] = 0.0;
+    return OK;
+}
+
 void regot_result_free(regot_result* r) { delete r; }
 
 }  // extern "C"

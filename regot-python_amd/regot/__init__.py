@@ -43,7 +43,7 @@ class sinkhorn_result:
     """
 
     __slots__ = ("_niter", "_dual", "_plan", "_obj_vals", "_mar_errs", "_run_times",
-                 "_densities", "mar_err_fp64", "obj_fp64")
+                 "_densities", "mar_err_fp64", "obj_fp64", "stats")
 
     def __init__(self):
         self._niter = 0
@@ -55,6 +55,7 @@ class sinkhorn_result:
         self._densities = []
         self.mar_err_fp64 = None
         self.obj_fp64 = None
+        self.stats = {}
 
     def _prop(name, kind):
         attr = "_" + name
@@ -318,6 +319,11 @@ class Problem:
             res._mar_errs = [float(x) for x in h[1][:hl]]
             res._run_times = [float(x) for x in h[2][:hl]]
             res._densities = [float(x) for x in dens[:dl]]
+            st = np.zeros(8)
+            check(L.regot_result_stats(r, dptr(st)))
+            res.stats = {"objgrad_ms": st[0], "objgrad_calls": int(st[1]),
+                         "sparsify_ms": st[2], "sparsify_calls": int(st[3]),
+                         "cg_ms": st[4], "cg_solves": int(st[5]), "cg_iters": int(st[6])}
             if compute_plan:
                 plan = np.empty((self.n, self.m))
                 check(L.regot_result_plan(self._h, r, dptr(plan)))

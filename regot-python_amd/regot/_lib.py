@@ -73,6 +73,7 @@ def lib():
     L.regot_result_get.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
     L.regot_result_plan.argtypes = [_vp, _vp, _dp]
     L.regot_result_verify.argtypes = [_vp, _vp, _dp, _dp]
+    L.regot_result_stats.argtypes = [_vp, _dp]
     L.regot_result_free.argtypes = [_vp]
     L.regot_result_free.restype = None
     _llp = ctypes.POINTER(ctypes.c_longlong)
