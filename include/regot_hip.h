@@ -168,6 +168,37 @@ int regot_result_verify(regot_problem* p, const regot_result* r, double* mar_err
 int regot_result_stats(const regot_result* r, double* out);
 void regot_result_free(regot_result* r);
 
+/*
+ * Batched BCD: many independent problems solved together (no reference
+ * counterpart; each problem follows sinkhorn_bcd_internal,
+ * src/sinkhorn_bcd.cpp:14-124, exactly as regot_solve with REGOT_SOLVER_BCD
+ * does, but only the final state is kept -- no per-iteration histories).
+ *
+ * regot_batch_create copies every problem into one packed device buffer:
+ *   n[p], m[p]      sizes of problem p;
+ *   M[p]            row-major n[p] x m[p] cost matrix (host memory, or device
+ *                   memory on `device` when M_device = 1), element type M_dtype;
+ *   bulk            REGOT_F32 or REGOT_F64, as for regot_problem_create;
+ *   a[p], b[p]      host FP64 marginals; reg[p] > 0.
+ */
+typedef struct regot_batch regot_batch;
+int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
+                       const void* const* M, int M_dtype, int bulk, int M_device,
+                       const double* const* a, const double* const* b, const double* reg,
+                       int device);
+int regot_batch_destroy(regot_batch* batch);
+/* Number of problems, total n*m elements, and the packed dual length
+ * sum_p (n[p] + m[p] - 1). */
+int regot_batch_info(const regot_batch* batch, int* nprob, long long* total_elems,
+                     long long* dual_len);
+/* Solve every problem from the reference's initial point to ||g|| < tol or
+ * max_iter rounds.  Per problem: niter, the final objective f and ||g||
+ * (that of iterate niter); dual (packed, problem p at sum_{q<p} (n[q]+m[q]-1))
+ * in the gamma layout.  Any output may be NULL.  *rounds receives the number
+ * of BCD rounds launched. */
+int regot_batch_solve_bcd(regot_batch* batch, double tol, int max_iter, int* niter,
+                          double* obj, double* mar_err, double* dual, long long* rounds);
+
 /* Device-side timing of the hot kernels (HIP events on the handle's stream).
  * kind: 0 = row pass, 1 = column pass (partial + combine).  Enabled by
  * regot_profile_enable(p, 1); totals accumulate until reset. */

@@ -1,0 +1,643 @@
+// batch.hip -- many independent EROT problems solved by BCD (Sinkhorn) in
+// shared launches.
+//
+// All cost matrices live in one packed device buffer (problem p row-major at
+// moff[p] with a 256-byte row pitch).  One BCD round of every active problem
+// is three launches whose workgroups come from tile tables:
+//
+//   row tiles      (p, t): rows 4R t .. 4R t + 4R - 1 of problem p -> alpha_{k+1}
+//                  and the row partials of g_k;
+//   column tiles   (p, s, b): column strip s (64V columns) over row block b of
+//                  problem p -> partial log2 column sums at alpha_{k+1};
+//   combine tiles  (p, c): columns 64c .. 64c + 63 of problem p -> every tile
+//                  of p takes p's stop decision for iterate k itself (same data,
+//                  same order), then beta_{k+1} and its column residual.
+//
+// Each problem follows the single-problem semantics of sinkhorn_bcd
+// (src/sinkhorn_bcd.cpp:14-124): test ||g_k|| < tol first, then update, and
+// niter = k at the stop.  A stopped problem's tiles return at once; every few
+// rounds the host drops them from the tables (the work queue shrinks as
+// problems converge), so the long tail of slow problems runs on a small grid.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <vector>
+
+#include "dense_device.hpp"
+#include "regot_hip.h"
+
+namespace regot {
+
+int set_error(int code, const char* what);
+
+namespace {
+
+using namespace dev;
+
+constexpr int kStripBlocks = 512;   // target rows per column-pass row block
+constexpr int kCombCols = 64;       // columns per combine tile (4 threads each)
+
+template <typename TM>
+constexpr int kRows = sizeof(TM) == 4 ? 4 : 2;   // rows per wave in row tiles
+
+struct Desc {
+    long moff;          // element offset of M_p in the packed buffer
+    int n, m, ldm;
+    int aoff, boff;     // offsets into the packed n-sized / m-sized vectors
+    int rt0, nrt;       // row tiles of p: partial slots rt0 .. rt0 + nrt - 1
+    int nblk, rpb;      // column-pass row blocks, rows per block
+    long poff;          // offset of p's column partials (nblk x m doubles)
+    int ct0, nct;       // combine tiles of p: residual slots (2 parities)
+    double reg, k;      // k = log2(e) / reg
+    double sum_a, sum_b1;
+};
+
+struct Vecs {
+    const double* a;
+    const double* b;
+    const double* loga;
+    const double* logb;
+    double* ap0;        // alpha, double buffer by round parity
+    double* ap1;
+    double* beta;
+    double* rpart;      // 3 per row tile
+    double* part;       // column partials
+    double* cpart;      // 2 per combine tile per parity
+    double* bm;         // beta_k[m-1] per problem (stored by row tile 0)
+    int* done;          // per problem
+    int* niter;
+    double* fval;       // f and ||g|| at the decision
+    double* gval;
+};
+
+template <typename TM>
+__global__ __launch_bounds__(256) void k_batch_row(const TM* __restrict__ M,
+                                                   const Desc* __restrict__ D,
+                                                   const int2* __restrict__ tiles, Vecs v, int k)
+{
+    constexpr int R = kRows<TM>;
+    const int2 tl = tiles[blockIdx.x];
+    const int p = tl.x;
+    if (v.done[p]) return;
+    const Desc d = D[p];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long row0 = (static_cast<long>(tl.y) * 4 + wave) * R;
+    const double* rowpot = ((k & 1) ? v.ap1 : v.ap0) + d.aoff;
+    double* aout = ((k & 1) ? v.ap0 : v.ap1) + d.aoff;
+    const double* loga = v.loga + d.aoff;
+    const double* a = v.a + d.aoff;
+    double wg2 = 0.0, wr = 0.0, waa = 0.0;
+    if (row0 < d.n) {
+        double L[R];
+        row_lse<TM, TM, R>(M + d.moff, d.ldm, d.n, d.m, v.beta + d.boff, d.k, loga, rowpot,
+                              row0, lane, L);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const long i = row0 + r;
+            if (i >= d.n) break;
+            if (lane == 0) {
+                aout[i] = d.reg * (loga[i] - kLn2 * L[r]);
+                const double ri = exp2(rowpot[i] * d.k + L[r]);
+                const double g = ri - a[i];
+                wg2 += g * g;
+                wr += ri;
+                waa += rowpot[i] * a[i];
+            }
+        }
+    }
+    __shared__ double red[4][3];
+    if (lane == 0) {
+        red[wave][0] = wg2;
+        red[wave][1] = wr;
+        red[wave][2] = waa;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            v.rpart[3L * (d.rt0 + tl.y) + q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+        if (tl.y == 0) v.bm[p] = v.beta[d.boff + d.m - 1];
+    }
+}
+
+// init: alpha_0 = 0 and no stop flag is read.
+template <typename TM>
+__global__ __launch_bounds__(256) void k_batch_colpart(const TM* __restrict__ M,
+                                                       const Desc* __restrict__ D,
+                                                       const int4* __restrict__ tiles, Vecs v,
+                                                       int k, int init)
+{
+    constexpr int TW = 64 * VecOf<TM>::N;
+    const int4 tl = tiles[blockIdx.x];
+    const int p = tl.x;
+    if (!init && v.done[p]) return;
+    const Desc d = D[p];
+    const double* alpha = (((k + 1) & 1) ? v.ap1 : v.ap0) + d.aoff;   // alpha_{k+1}
+    const long rb = static_cast<long>(tl.z) * d.rpb;
+    const long re = (rb + d.rpb < d.n) ? rb + d.rpb : d.n;
+    col_partial_tile<TM, TM>(M + d.moff, d.ldm, d.m, alpha, init ? nullptr : v.beta + d.boff,
+                                v.logb + d.boff, d.k, tl.y * TW, rb, re,
+                                v.part + d.poff + static_cast<long>(tl.z) * d.m);
+}
+
+// Stop decision for iterate k of problem p (identical in every tile of p):
+// g_k = [r - a ; residual of the update that produced beta_k].
+__device__ void batch_decide(const Desc& d, const Vecs& v, int p, int k, double tol,
+                             int max_iter, double* f, double* gn, bool* done)
+{
+    const double* cp = v.cpart + 2L * (2L * d.ct0 + static_cast<long>(k & 1) * d.nct);
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < d.nrt; q += 256) {
+        s[0] += v.rpart[3L * (d.rt0 + q)];
+        s[1] += v.rpart[3L * (d.rt0 + q) + 1];
+        s[2] += v.rpart[3L * (d.rt0 + q) + 2];
+    }
+    for (int q = threadIdx.x; q < d.nct; q += 256) {
+        s[3] += cp[2 * q];
+        s[4] += cp[2 * q + 1];
+    }
+    block_sums_256<5>(s);
+    const double bm = v.bm[p];
+    *gn = sqrt(s[0] + s[3]);
+    *f = d.reg * s[1] - (s[2] + bm * d.sum_a) - (s[4] - bm * d.sum_b1);
+    *done = (*gn < tol) || (k >= max_iter);
+}
+
+__global__ __launch_bounds__(256) void k_batch_combine(const Desc* __restrict__ D,
+                                                       const int2* __restrict__ tiles, Vecs v,
+                                                       int k, int init, double tol, int max_iter)
+{
+    const int2 tl = tiles[blockIdx.x];
+    const int p = tl.x;
+    if (!init && v.done[p]) return;
+    const Desc d = D[p];
+    if (!init) {
+        double f, gn;
+        bool done;
+        batch_decide(d, v, p, k, tol, max_iter, &f, &gn, &done);
+        if (tl.y == 0 && threadIdx.x == 0) {
+            v.fval[p] = f;
+            v.gval[p] = gn;
+            if (done) {
+                v.niter[p] = k;
+                v.done[p] = 1;
+            }
+        }
+        if (done) return;
+    }
+    const int sub = threadIdx.x & 3;
+    const int j = tl.y * kCombCols + (threadIdx.x >> 2);
+    const bool valid = j < d.m;
+    const double L = combine_partials(v.part + d.poff, d.nblk, d.m, j, valid, sub);
+    double r2 = 0.0, bb = 0.0;
+    if (sub == 0 && valid) {
+        const double lb = v.logb[d.boff + j];
+        const double bj = d.reg * (lb - kLn2 * L);
+        v.beta[d.boff + j] = bj;
+        // residual of the update in FP64: b_j (exp(beta_j/reg + ln2 L - log b_j) - 1)
+        const double g = v.b[d.boff + j] * expm1(bj / d.reg + kLn2 * L - lb);
+        if (j < d.m - 1) {
+            r2 = g * g;
+            bb = bj * v.b[d.boff + j];
+        }
+    }
+    double s[2] = {r2, bb};
+    block_sums_256<2>(s);
+    if (threadIdx.x == 0) {
+        // residual slots of beta_{k+1}: parity (k + 1) & 1 (init: beta_0, parity 0)
+        const int par = init ? 0 : ((k + 1) & 1);
+        double* cp = v.cpart + 2L * (2L * d.ct0 + static_cast<long>(par) * d.nct);
+        cp[2 * tl.y] = s[0];
+        cp[2 * tl.y + 1] = s[1];
+    }
+}
+
+// gamma_p = [alpha + beta_{m-1}; beta_{0..m-2} - beta_{m-1}] with alpha =
+// alpha_{niter}; one workgroup per problem.
+__global__ __launch_bounds__(256) void k_batch_dual(const Desc* __restrict__ D, Vecs v,
+                                                    double* __restrict__ dual)
+{
+    const int p = blockIdx.x;
+    const Desc d = D[p];
+    const int it = v.niter[p];
+    const double* alpha = ((it & 1) ? v.ap1 : v.ap0) + d.aoff;
+    const double* beta = v.beta + d.boff;
+    const double bm = beta[d.m - 1];
+    double* out = dual + (d.aoff + d.boff - p);
+    for (int i = threadIdx.x; i < d.n; i += 256) out[i] = alpha[i] + bm;
+    for (int j = threadIdx.x; j < d.m - 1; j += 256) out[d.n + j] = beta[j] - bm;
+}
+
+template <typename Tin, typename Tout>
+__global__ void k_pack(const Tin* __restrict__ src, long lds, Tout* __restrict__ dst, long ldd,
+                       int n, int m)
+{
+    const long total = static_cast<long>(n) * ldd;
+    for (long t = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; t < total;
+         t += static_cast<long>(gridDim.x) * 256) {
+        const long i = t / ldd, j = t % ldd;
+        dst[t] = (j < m) ? static_cast<Tout>(src[i * lds + j]) : Tout(0);
+    }
+}
+
+__global__ void k_batch_init(double* ap0, const double* a, double* loga, long na,
+                             const double* b, double* logb, long nb, int* done, int* niter,
+                             int np)
+{
+    const long t = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (t < na) {
+        ap0[t] = 0.0;
+        loga[t] = log(a[t]);
+    }
+    if (t < nb) logb[t] = log(b[t]);
+    if (t < np) {
+        done[t] = 0;
+        niter[t] = 0;
+    }
+}
+
+}  // namespace
+}  // namespace regot
+
+using namespace regot;
+
+struct regot_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nprob = 0;
+    int bulk = 0;
+    std::vector<Desc> desc;
+    long total_n = 0, total_m = 0, total_dual = 0;
+    DBuf<unsigned char> M;
+    DBuf<Desc> d_desc;
+    DBuf<double> a, b, loga, logb, ap0, ap1, beta, rpart, part, cpart, bm, fval, gval;
+    DBuf<int> done, niter;
+    DBuf<int2> t_row, t_comb;
+    DBuf<int4> t_col;
+    HBuf<int> h_done;
+    HBuf<int2> h_row[2], h_comb[2];
+    HBuf<int4> h_col[2];
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    long long total_elems = 0;
+
+    ~regot_batch()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    Vecs vecs()
+    {
+        Vecs v;
+        v.a = a;
+        v.b = b;
+        v.loga = loga;
+        v.logb = logb;
+        v.ap0 = ap0;
+        v.ap1 = ap1;
+        v.beta = beta;
+        v.rpart = rpart;
+        v.part = part;
+        v.cpart = cpart;
+        v.bm = bm;
+        v.done = done;
+        v.niter = niter;
+        v.fval = fval;
+        v.gval = gval;
+        return v;
+    }
+
+    // Tile tables of the listed problems into staging slot s.
+    void build_tables(const std::vector<int>& act, int s, int* nrow, int* ncol, int* ncomb)
+    {
+        long cr = 0, cc = 0, cb = 0;
+        const int vw = bulk == 0 ? 4 : 2;
+        for (int p : act) {
+            const Desc& d = desc[p];
+            cr += d.nrt;
+            cc += static_cast<long>(ceil_div(d.m, 64 * vw)) * d.nblk;
+            cb += d.nct;
+        }
+        h_row[s].ensure(std::max<long>(cr, 1));
+        h_col[s].ensure(std::max<long>(cc, 1));
+        h_comb[s].ensure(std::max<long>(cb, 1));
+        long ir = 0, ic = 0, ib = 0;
+        for (int p : act) {
+            const Desc& d = desc[p];
+            for (int t = 0; t < d.nrt; t++) h_row[s].p[ir++] = make_int2(p, t);
+            const int strips = ceil_div(d.m, 64 * vw);
+            for (int bk = 0; bk < d.nblk; bk++)
+                for (int st = 0; st < strips; st++) h_col[s].p[ic++] = make_int4(p, st, bk, 0);
+            for (int c = 0; c < d.nct; c++) h_comb[s].p[ib++] = make_int2(p, c);
+        }
+        *nrow = static_cast<int>(cr);
+        *ncol = static_cast<int>(cc);
+        *ncomb = static_cast<int>(cb);
+        HIP_CHECK(hipMemcpyAsync(t_row.p, h_row[s].p, sizeof(int2) * std::max<long>(cr, 1),
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(t_col.p, h_col[s].p, sizeof(int4) * std::max<long>(cc, 1),
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(t_comb.p, h_comb[s].p, sizeof(int2) * std::max<long>(cb, 1),
+                                 hipMemcpyHostToDevice, stream));
+    }
+
+    template <typename TM>
+    void launch_round(int k, int init, int nrow, int ncol, int ncomb, double tol, int max_iter)
+    {
+        const TM* Mp = reinterpret_cast<const TM*>(M.p);
+        const Vecs v = vecs();
+        if (!init && nrow > 0) {
+            hipLaunchKernelGGL(k_batch_row<TM>, dim3(nrow), dim3(256), 0, stream, Mp, d_desc.p,
+                               t_row.p, v, k);
+            KERNEL_CHECK();
+        }
+        if (ncol > 0) {
+            hipLaunchKernelGGL(k_batch_colpart<TM>, dim3(ncol), dim3(256), 0, stream, Mp,
+                               d_desc.p, t_col.p, v, k, init);
+            KERNEL_CHECK();
+        }
+        if (ncomb > 0) {
+            hipLaunchKernelGGL(k_batch_combine, dim3(ncomb), dim3(256), 0, stream, d_desc.p,
+                               t_comb.p, v, k, init, tol, max_iter);
+            KERNEL_CHECK();
+        }
+    }
+
+    void round(int k, int init, int nrow, int ncol, int ncomb, double tol, int max_iter)
+    {
+        if (bulk == 0)
+            launch_round<float>(k, init, nrow, ncol, ncomb, tol, max_iter);
+        else
+            launch_round<double>(k, init, nrow, ncol, ncomb, tol, max_iter);
+    }
+};
+
+namespace {
+
+#define CAPI_BEGIN try {
+#define CAPI_END                                                                          \
+    }                                                                                     \
+    catch (const regot::Error& e) { return set_error(e.code, e.what()); }                 \
+    catch (const std::bad_alloc&) { return set_error(E_NOMEM, "host allocation failed"); } \
+    catch (const std::exception& e) { return set_error(E_INTERNAL, e.what()); }            \
+    return OK;
+
+}  // namespace
+
+extern "C" {
+
+int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
+                       const void* const* M, int M_dtype, int bulk, int M_device,
+                       const double* const* a, const double* const* b, const double* reg,
+                       int device)
+{
+    CAPI_BEGIN
+    if (!out || nprob <= 0 || !n || !m || !M || !a || !b || !reg)
+        fail(E_INVALID, "regot_batch_create: null argument or nprob <= 0");
+    if ((M_dtype != REGOT_F32 && M_dtype != REGOT_F64) || (bulk != REGOT_F32 && bulk != REGOT_F64))
+        fail(E_INVALID, "regot_batch_create: dtype must be REGOT_F32 or REGOT_F64");
+    *out = nullptr;
+    int ndev = 0;
+    HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) fail(E_INVALID, "device %d not available", device);
+    HIP_CHECK(hipSetDevice(device));
+    auto B = std::make_unique<regot_batch>();
+    B->device = device;
+    B->nprob = nprob;
+    B->bulk = bulk;
+    HIP_CHECK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    const int esz = bulk == REGOT_F32 ? 4 : 8;
+    const int vw = bulk == REGOT_F32 ? 4 : 2;
+    const int rows_per_tile = 4 * (bulk == REGOT_F32 ? 4 : 2);
+    B->desc.resize(nprob);
+    long moff = 0, aoff = 0, boff = 0, rt = 0, poff = 0, ct = 0;
+    for (int p = 0; p < nprob; p++) {
+        if (n[p] < 1 || m[p] < 1 || !M[p] || !a[p] || !b[p] || !(reg[p] > 0.0))
+            fail(E_INVALID, "problem %d: bad sizes, null pointer or reg <= 0", p);
+        if (static_cast<long>(n[p]) * m[p] > (1L << 31))
+            fail(E_INVALID, "problem %d: n*m above 2^31", p);
+        Desc& d = B->desc[p];
+        d.n = n[p];
+        d.m = m[p];
+        d.ldm = static_cast<int>(ceil_div(m[p], 256 / esz) * (256 / esz));
+        d.moff = moff;
+        moff += static_cast<long>(d.n) * d.ldm;
+        d.aoff = static_cast<int>(aoff);
+        d.boff = static_cast<int>(boff);
+        aoff += d.n;
+        boff += d.m;
+        d.nrt = ceil_div(d.n, rows_per_tile);
+        d.rt0 = static_cast<int>(rt);
+        rt += d.nrt;
+        d.nblk = ceil_div(d.n, kStripBlocks);
+        d.rpb = ceil_div(d.n, d.nblk);
+        d.poff = poff;
+        poff += static_cast<long>(d.nblk) * d.m;
+        d.nct = ceil_div(d.m, kCombCols);
+        d.ct0 = static_cast<int>(ct);
+        ct += d.nct;
+        d.reg = reg[p];
+        d.k = 1.4426950408889634074 / reg[p];
+        double sa = 0.0, sb = 0.0;
+        for (int i = 0; i < d.n; i++) sa += a[p][i];
+        for (int j = 0; j < d.m - 1; j++) sb += b[p][j];
+        d.sum_a = sa;
+        d.sum_b1 = sb;
+        B->total_elems += static_cast<long long>(d.n) * d.m;
+    }
+    if (aoff > 0x7fffffffL || boff > 0x7fffffffL)
+        fail(E_INVALID, "batch too large (sum of n or m above 2^31)");
+    (void)vw;
+    B->total_n = aoff;
+    B->total_m = boff;
+    B->total_dual = aoff + boff - nprob;
+    B->M.alloc(static_cast<size_t>(moff) * esz);
+    B->d_desc.alloc(nprob);
+    HIP_CHECK(hipMemcpyAsync(B->d_desc.p, B->desc.data(), sizeof(Desc) * nprob,
+                             hipMemcpyHostToDevice, B->stream));
+    // pack M (convert / pad on the device; host sources go through one staging copy)
+    const int in_esz = M_dtype == REGOT_F32 ? 4 : 8;
+    DBuf<unsigned char> stage;
+    for (int p = 0; p < nprob; p++) {
+        const Desc& d = B->desc[p];
+        const void* src = M[p];
+        if (!M_device) {
+            stage.ensure(static_cast<size_t>(d.n) * d.m * in_esz);
+            HIP_CHECK(hipMemcpyAsync(stage.p, M[p], static_cast<size_t>(d.n) * d.m * in_esz,
+                                     hipMemcpyHostToDevice, B->stream));
+            src = stage.p;
+        }
+        unsigned char* dst = B->M.p + static_cast<size_t>(d.moff) * esz;
+        const long total = static_cast<long>(d.n) * d.ldm;
+        const int g = static_cast<int>(std::min<long>(ceil_div(total, 256L), 65536L));
+        if (M_dtype == REGOT_F32 && bulk == REGOT_F32)
+            hipLaunchKernelGGL((k_pack<float, float>), dim3(g), dim3(256), 0, B->stream,
+                               static_cast<const float*>(src), static_cast<long>(d.m),
+                               reinterpret_cast<float*>(dst), static_cast<long>(d.ldm), d.n, d.m);
+        else if (M_dtype == REGOT_F64 && bulk == REGOT_F32)
+            hipLaunchKernelGGL((k_pack<double, float>), dim3(g), dim3(256), 0, B->stream,
+                               static_cast<const double*>(src), static_cast<long>(d.m),
+                               reinterpret_cast<float*>(dst), static_cast<long>(d.ldm), d.n, d.m);
+        else if (M_dtype == REGOT_F32)
+            hipLaunchKernelGGL((k_pack<float, double>), dim3(g), dim3(256), 0, B->stream,
+                               static_cast<const float*>(src), static_cast<long>(d.m),
+                               reinterpret_cast<double*>(dst), static_cast<long>(d.ldm), d.n, d.m);
+        else
+            hipLaunchKernelGGL((k_pack<double, double>), dim3(g), dim3(256), 0, B->stream,
+                               static_cast<const double*>(src), static_cast<long>(d.m),
+                               reinterpret_cast<double*>(dst), static_cast<long>(d.ldm), d.n, d.m);
+        KERNEL_CHECK();
+        if (!M_device) HIP_CHECK(hipStreamSynchronize(B->stream));   // stage is reused
+    }
+    // marginals
+    std::vector<double> ha(aoff), hb(boff);
+    for (int p = 0; p < nprob; p++) {
+        std::memcpy(ha.data() + B->desc[p].aoff, a[p], sizeof(double) * n[p]);
+        std::memcpy(hb.data() + B->desc[p].boff, b[p], sizeof(double) * m[p]);
+    }
+    B->a.alloc(aoff);
+    B->b.alloc(boff);
+    B->loga.alloc(aoff);
+    B->logb.alloc(boff);
+    B->ap0.alloc(aoff);
+    B->ap1.alloc(aoff);
+    B->beta.alloc(boff);
+    B->rpart.alloc(3 * rt);
+    B->part.alloc(poff);
+    B->cpart.alloc(4 * ct);
+    B->bm.alloc(nprob);
+    B->fval.alloc(nprob);
+    B->gval.alloc(nprob);
+    B->done.alloc(nprob);
+    B->niter.alloc(nprob);
+    HIP_CHECK(hipMemcpyAsync(B->a.p, ha.data(), sizeof(double) * aoff, hipMemcpyHostToDevice,
+                             B->stream));
+    HIP_CHECK(hipMemcpyAsync(B->b.p, hb.data(), sizeof(double) * boff, hipMemcpyHostToDevice,
+                             B->stream));
+    long tr = 0, tc = 0, tb = 0;
+    for (const Desc& d : B->desc) {
+        tr += d.nrt;
+        tc += static_cast<long>(ceil_div(d.m, 64 * (bulk == REGOT_F32 ? 4 : 2))) * d.nblk;
+        tb += d.nct;
+    }
+    B->t_row.alloc(tr);
+    B->t_col.alloc(tc);
+    B->t_comb.alloc(tb);
+    B->h_done.ensure(2L * nprob);
+    for (auto& e : B->ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_CHECK(hipStreamSynchronize(B->stream));
+    *out = B.release();
+    CAPI_END
+}
+
+int regot_batch_destroy(regot_batch* B)
+{
+    CAPI_BEGIN
+    delete B;
+    CAPI_END
+}
+
+int regot_batch_info(const regot_batch* B, int* nprob, long long* total_elems,
+                     long long* dual_len)
+{
+    CAPI_BEGIN
+    if (!B) fail(E_INVALID, "null batch");
+    if (nprob) *nprob = B->nprob;
+    if (total_elems) *total_elems = B->total_elems;
+    if (dual_len) *dual_len = B->total_dual;
+    CAPI_END
+}
+
+int regot_batch_solve_bcd(regot_batch* B, double tol, int max_iter, int* niter, double* obj,
+                          double* mar_err, double* dual, long long* rounds)
+{
+    CAPI_BEGIN
+    if (!B) fail(E_INVALID, "null batch");
+    if (max_iter < 0) fail(E_INVALID, "max_iter < 0");
+    HIP_CHECK(hipSetDevice(B->device));
+    hipStream_t s = B->stream;
+    const int np = B->nprob;
+    const long nmax = std::max<long>(std::max(B->total_n, B->total_m), np);
+    hipLaunchKernelGGL(k_batch_init, dim3(static_cast<int>(ceil_div(nmax, 256L))), dim3(256), 0,
+                       s, B->ap0.p, B->a.p, B->loga.p, B->total_n, B->b.p, B->logb.p,
+                       B->total_m, B->done.p, B->niter.p, np);
+    KERNEL_CHECK();
+
+    std::vector<int> act(np);
+    for (int p = 0; p < np; p++) act[p] = p;
+    int slot = 0, nrow = 0, ncol = 0, ncomb = 0;
+    B->build_tables(act, slot, &nrow, &ncol, &ncomb);
+    // beta_0 = optimal_beta(alpha_0 = 0) and its residual (parity 0)
+    B->round(-1, 1, nrow, ncol, ncomb, tol, max_iter);
+
+    const int K = 8;   // rounds per batch; two batches in flight
+    int k = 0;
+    long long nround = 0;
+    auto launch_batch = [&](int e) {
+        for (int q = 0; q < K && k <= max_iter; q++, k++) {
+            B->round(k, 0, nrow, ncol, ncomb, tol, max_iter);
+            nround++;
+        }
+        HIP_CHECK(hipMemcpyAsync(B->h_done.p + static_cast<long>(e) * np, B->done.p,
+                                 sizeof(int) * np, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipEventRecord(B->ev[e], s));
+    };
+    std::deque<int> q;
+    try {
+        launch_batch(0);
+        q.push_back(0);
+        if (k <= max_iter) {
+            launch_batch(1);
+            q.push_back(1);
+        }
+        while (!q.empty()) {
+            const int e = q.front();
+            q.pop_front();
+            HIP_CHECK(hipEventSynchronize(B->ev[e]));
+            const int* dn = B->h_done.p + static_cast<long>(e) * np;
+            std::vector<int> still;
+            still.reserve(act.size());
+            for (int p : act)
+                if (!dn[p]) still.push_back(p);
+            if (still.empty() || k > max_iter) {
+                // every problem stopped, or every round up to max_iter is queued
+                if (still.empty()) break;
+                continue;
+            }
+            if (still.size() != act.size()) {
+                act.swap(still);
+                slot ^= 1;
+                B->build_tables(act, slot, &nrow, &ncol, &ncomb);
+            }
+            launch_batch(e);
+            q.push_back(e);
+        }
+        HIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        throw;
+    }
+
+    if (niter) HIP_CHECK(hipMemcpyAsync(niter, B->niter.p, sizeof(int) * np, hipMemcpyDeviceToHost, s));
+    if (obj) HIP_CHECK(hipMemcpyAsync(obj, B->fval.p, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+    if (mar_err)
+        HIP_CHECK(hipMemcpyAsync(mar_err, B->gval.p, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+    if (dual) {
+        DBuf<double> dd(static_cast<size_t>(B->total_dual));
+        hipLaunchKernelGGL(k_batch_dual, dim3(np), dim3(256), 0, s, B->d_desc.p, B->vecs(), dd.p);
+        KERNEL_CHECK();
+        HIP_CHECK(hipMemcpyAsync(dual, dd.p, sizeof(double) * B->total_dual,
+                                 hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (rounds) *rounds = nround;
+    CAPI_END
+}
+
+}  // extern "C"

@@ -85,6 +85,7 @@ struct BcdState {
     long long* hist_ts;
     int hist_cap;
     double* rpart;          // per row-pass workgroup: sum g^2, sum r, sum alpha a
+    double* bm;             // beta_k[m-1], stored by the row pass of iteration k
     const double* cpart;    // 2 x ntiles x (sum resid^2, sum beta_j b_j), j < m-1
     int nrb;                // row-pass workgroups
     int ntiles;             // combine workgroups

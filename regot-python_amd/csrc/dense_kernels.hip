@@ -1,238 +1,30 @@
-// dense_kernels.hip -- streaming passes over the cost matrix M for gfx950.
-//
-// Every pass evaluates x_ij = (p_i + q_j - M_ij) / reg in FP64 (M is FP32 data
-// upcast exactly, or FP64 in FP64 mode), converts the shifted value to the
-// bulk type, and accumulates 2^(x - shift) with a per-lane adaptive shift:
-//
-//   * the shift starts at a caller-supplied guess (log2 of the expected sum),
-//   * it moves down once if the first values all sit more than 8 below it,
-//   * it moves up whenever a value exceeds it by more than 1,
-//
-// so the dominant terms are always formed within a few units of zero, where
-// the FP32 rounding of the exponent argument is ~1e-7 relative, and nothing
-// overflows or underflows whatever the guess.  In steady state (good guess)
-// the shift never moves and the inner loop is: 4 FP64-rate ops (cvt, add,
-// fma, cvt), one max, one exp2 and one add per element; every 16 terms the
-// FP32 lane sum is moved into an FP64 total.  M is read with nontemporal
-// loads (it is streamed once per pass).
-//
-// Partial results combine as FP64 log2-sums: log2(sum) = L_max + log2(sum
-// 2^(L - L_max)).  All reductions run in a fixed order, so results are
-// bitwise reproducible run to run.
+// dense_kernels.hip -- single-problem streaming passes over the cost matrix M
+// (row pass, column pass, FP64 plan) and the on-device BCD bookkeeping.  The
+// sweeps themselves are in dense_device.hpp.
 #include "dense.hpp"
+#include "dense_device.hpp"
 
 #include <cmath>
 
 namespace regot {
 namespace {
 
-constexpr double kLog2e = 1.4426950408889634074;
-constexpr double kLn2 = 0.69314718055994530942;
-
-template <typename T>
-struct VecOf;
-template <>
-struct VecOf<float> {
-    using type = float4;
-    static constexpr int N = 4;
-};
-template <>
-struct VecOf<double> {
-    using type = double2;
-    static constexpr int N = 2;
-};
-
-// v_exp_f32 directly: exp2f adds a denormal-range rescale (compare, two
-// selects, add, ldexp) around it.  Results below 2^-126 flush to 0; with the
-// adaptive shift every summed term is relative to a running maximum >= 2^-8,
-// so a flushed term is below 2^-118 of the sum.
-__device__ __forceinline__ float ex2(float x) { return __builtin_amdgcn_exp2f(x); }
-__device__ __forceinline__ double ex2(double x) { return exp2(x); }
-__device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
-__device__ __forceinline__ double vmax(double a, double b) { return fmax(a, b); }
-
-template <typename T>
-__device__ __forceinline__ T ninf()
-{
-    return -static_cast<T>(INFINITY);
-}
-
-template <typename VT, typename T>
-__device__ __forceinline__ T vget(const VT& v, int i);
-template <>
-__device__ __forceinline__ float vget<float4, float>(const float4& v, int i)
-{
-    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
-}
-template <>
-__device__ __forceinline__ double vget<double2, double>(const double2& v, int i)
-{
-    return i == 0 ? v.x : v.y;
-}
-
-// Streaming load of one vector of M.  Nontemporal: M is read once per pass
-// and far larger than L2 and MALL, so it should not displace the potentials;
-// measured on a 1 GiB row stream this is ~6% faster than a default load.
-__device__ __forceinline__ float4 ld_stream(const float* p)
-{
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
-    return make_float4(v.x, v.y, v.z, v.w);
-}
-__device__ __forceinline__ double2 ld_stream(const double* p)
-{
-    typedef double d2 __attribute__((ext_vector_type(2)));
-    const d2 v = __builtin_nontemporal_load(reinterpret_cast<const d2*>(p));
-    return make_double2(v.x, v.y);
-}
-
-__device__ __forceinline__ double wave_max(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
-}
-
-__device__ __forceinline__ double wave_sum0(double v)
-{
-    // Butterfly sum; lane 0's value is used so the order is fixed.
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return __shfl(v, 0);
-}
-
-// log2 of sum over the wave of 2^L (L = -inf for empty lanes).
-__device__ __forceinline__ double wave_log2sum(double L)
-{
-    const double Lmax = wave_max(L);
-    if (Lmax == -INFINITY) return -INFINITY;
-    const double e = (L == -INFINITY) ? 0.0 : exp2(L - Lmax);
-    return Lmax + log2(wave_sum0(e));
-}
-
-__device__ __forceinline__ double acc_log2(double sh, double s)
-{
-    return (s > 0.0) ? sh + log2(s) : -INFINITY;
-}
-
-// Each lane sums 2^t in the bulk type TA into s and, in FP32, moves s into an
-// FP64 total S every kFlush terms, so FP32 rounding accumulates over 16 terms
-// at most (<= 1e-6 relative, ~2e-7 typical) whatever the row or column length.
-constexpr int kFlushSteps = 4;   // steps of 4 terms between FP32 -> FP64 flushes
-
-template <typename TA>
-__device__ __forceinline__ void flush(TA& s, double& S)
-{
-    if (sizeof(TA) == 4) {
-        S += static_cast<double>(s);
-        s = TA(0);
-    }
-}
-
-// Move the shift when the running max leaves [-8, 8]: values up to 2^8 above
-// the shift keep the bulk-type rounding of the exponent argument below
-// 2^-16 ulp-units (~2e-7 relative), and a lane moves at most once downwards
-// (first values) and rarely upwards.  The partial sums are folded into S and
-// rescaled by 2^-d in FP64.  Returns true if moved.
-template <typename TA>
-__device__ __forceinline__ bool reshift(double& sh, TA& s, double& S, TA& rm)
-{
-    if (rm > TA(8) || (rm < TA(-8) && rm > ninf<TA>())) {
-        const TA d = rm;
-        sh += static_cast<double>(d);
-        const double tot = S + static_cast<double>(s);
-        // tot holds terms <= 2^d, so tot * 2^-d is representable; the factor
-        // alone needs FP64 when d < -126.  FP32 bulk takes the FP32 exp2 of the
-        // factor (one rounding, like one more term) unless it would overflow.
-        double f;
-        if (sizeof(TA) == 4 && d > TA(-120))
-            f = static_cast<double>(ex2(static_cast<float>(-d)));
-        else
-            f = exp2(-static_cast<double>(d));
-        S = (tot != 0.0) ? tot * f : 0.0;
-        s = TA(0);
-        rm = TA(0);
-        return true;
-    }
-    return false;
-}
+using namespace dev;
 
 // ---------------------------------------------------------------------------
 // Row pass
 // ---------------------------------------------------------------------------
 
-// One row x V columns: t_v = (bk_v - sh) - k M_v, masked entries = -inf.
-// TM is the storage type of M, TA the type t, exp2 and the sums run in.
-template <typename TM, typename TA, int V, bool MASK>
-__device__ __forceinline__ void row_accumulate(double& sh, TA& s, double& S, TA& rm,
-                                               const double* bk,
-                                               const typename VecOf<TM>::type& mv, double k,
-                                               int valid_cols)
-{
-    TA t[V];
-    TA cm = ninf<TA>();
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-        const double mvd = static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv, v));
-        t[v] = static_cast<TA>(fma(-k, mvd, bk[v] - sh));
-        if (MASK && v >= valid_cols) t[v] = ninf<TA>();
-        cm = vmax(cm, t[v]);
-    }
-    rm = vmax(rm, cm);
-    if (reshift(sh, s, S, rm)) {
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-            const double mvd = static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv, v));
-            t[v] = static_cast<TA>(fma(-k, mvd, bk[v] - sh));
-            if (MASK && v >= valid_cols) t[v] = ninf<TA>();
-        }
-    }
-#pragma unroll
-    for (int v = 0; v < V; v++) s += ex2(t[v]);
-}
-
-__device__ __forceinline__ double ld_agent(const double* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Sums of NV values over the 256 threads of the workgroup: butterfly within
-// each wave, then the four wave totals in fixed order.  Every thread gets the
-// same (bitwise) results; one barrier pair per call.
-template <int NV>
-__device__ __forceinline__ void block_sums_256(double (&v)[NV])
-{
-    __shared__ double sm[4][NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-        double x = v[q];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-        v[q] = __shfl(x, 0);
-    }
-    const int wave = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0)
-#pragma unroll
-        for (int q = 0; q < NV; q++) sm[wave][q] = v[q];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NV; q++) v[q] = ((sm[0][q] + sm[1][q]) + sm[2][q]) + sm[3][q];
-}
-
-// Each wave owns R consecutive rows and streams them left to right, lane l
-// reading the V-wide vector at column c*64V + l*V of each row (1 KiB per row
-// per step, fully coalesced).  The beta chunk is loaded once per step and
-// reused by all R rows.  With FIN, per-workgroup sums of (g^2, r, alpha a) are
-// published and the last workgroup to arrive runs the BCD bookkeeping.
+// Each wave owns R consecutive rows (row_lse).  The epilogue writes, per row,
+// the log2 sum, alpha = reg (log a - ln2 L) and/or the row sum
+// r_i = 2^(k alpha_i + L).  With FIN, per-workgroup sums of (g^2, r, alpha a)
+// are published for the combine kernel of the same BCD iteration.
 template <typename TM, typename TA, int R, bool FIN>
 __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long ldm, int n,
                                                   int m, const double* __restrict__ beta,
                                                   double k, RowEpi epi, BcdState fin,
                                                   const int* __restrict__ ctl)
 {
-    using VT = typename VecOf<TM>::type;
-    constexpr int V = VecOf<TM>::N;
     if (ctl != nullptr && ctl[0] != 0) return;
     const int lane = threadIdx.x & 63;
     // wave-uniform: row tests below become scalar branches, not exec masks
@@ -249,84 +41,17 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
     double wg2 = 0.0, wr = 0.0, waa = 0.0;
 
     if (row0 < n) {
-        double sh[R], S[R];
-        TA s[R], rm[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const long i = row0 + r;
-            double g = 0.0;
-            if (i < n && epi.loga != nullptr) {
-                g = epi.loga[i] * kLog2e;
-                if (rowpot != nullptr) g -= rowpot[i] * k;
-            }
-            sh[r] = g;
-            s[r] = TA(0);
-            S[r] = 0.0;
-            rm[r] = ninf<TA>();
-        }
-        const TM* rowp[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) rowp[r] = M + (row0 + r < n ? row0 + r : row0) * ldm;
-
-        const int step = 64 * V;
-        const int nfull = m / step;
-        // Rows past n read row0 again (in bounds) and are not accumulated.
-        // The next step's vectors are loaded before the current ones are
-        // consumed, so each wave keeps R loads in flight while it computes.
-        VT mv[R];
-        if (nfull > 0) {
-#pragma unroll
-            for (int r = 0; r < R; r++) mv[r] = ld_stream(rowp[r] + lane * V);
-        }
-        for (int c = 0; c < nfull; c++) {
-            const int j0 = c * step + lane * V;
-            VT nx[R];
-            if (c + 1 < nfull) {
-#pragma unroll
-                for (int r = 0; r < R; r++) nx[r] = ld_stream(rowp[r] + j0 + step);
-            }
-            double bk[V];
-#pragma unroll
-            for (int v = 0; v < V; v++) bk[v] = beta[j0 + v] * k;
-#pragma unroll
-            for (int r = 0; r < R; r++)
-                if (row0 + r < n)
-                    row_accumulate<TM, TA, V, false>(sh[r], s[r], S[r], rm[r], bk, mv[r], k, V);
-            if (c % kFlushSteps == kFlushSteps - 1) {
-#pragma unroll
-                for (int r = 0; r < R; r++) flush(s[r], S[r]);
-            }
-#pragma unroll
-            for (int r = 0; r < R; r++) mv[r] = nx[r];
-        }
-        {
-            const int j0 = nfull * step + lane * V;
-            if (j0 < m) {
-                const int valid = (m - j0) < V ? (m - j0) : V;
-                double bk[V];
-#pragma unroll
-                for (int v = 0; v < V; v++) bk[v] = (v < valid) ? beta[j0 + v] * k : 0.0;
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    if (row0 + r < n) {
-                        // ldm >= m rounded up to V: the vector stays inside the row pitch.
-                        const VT mv = ld_stream(rowp[r] + j0);
-                        row_accumulate<TM, TA, V, true>(sh[r], s[r], S[r], rm[r], bk, mv, k, valid);
-                    }
-                }
-            }
-        }
-
+        double L[R];
+        row_lse<TM, TA, R>(M, ldm, n, m, beta, k, epi.loga, rowpot, row0, lane, L);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const long i = row0 + r;
             if (i >= n) break;
-            const double L = wave_log2sum(acc_log2(sh[r], S[r] + static_cast<double>(s[r])));
             if (lane == 0) {
-                if (epi.lse2) epi.lse2[i] = L;
-                if (aout) aout[i] = epi.reg * (epi.loga[i] - kLn2 * L);
+                if (epi.lse2) epi.lse2[i] = L[r];
+                if (aout) aout[i] = epi.reg * (epi.loga[i] - kLn2 * L[r]);
                 if (epi.rowsum || FIN) {
-                    const double ri = exp2(rowpot[i] * k + L);
+                    const double ri = exp2(rowpot[i] * k + L[r]);
                     if (epi.rowsum) epi.rowsum[i] = ri;
                     if (FIN) {
                         const double g = ri - fin.a[i];
@@ -353,7 +78,12 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
 #pragma unroll
             for (int q = 0; q < 3; q++)
                 fin.rpart[3 * blockIdx.x + q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
-            if (blockIdx.x == 0) fin.ctl[3] = ctl[1];
+            if (blockIdx.x == 0) {
+                fin.ctl[3] = ctl[1];
+                // beta_k[m-1] for the objective: the combine kernel of this
+                // iteration overwrites beta while other workgroups still decide
+                fin.bm[0] = beta[m - 1];
+            }
         }
     }
 }
@@ -362,40 +92,8 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
 // Column pass
 // ---------------------------------------------------------------------------
 
-template <typename TM, typename TA, int V, int CH, bool MASK>
-__device__ __forceinline__ void col_chunk(double* sh, TA* s, double* S, TA* rm,
-                                          const typename VecOf<TM>::type* mv, const double* ak,
-                                          double k, int valid_rows, int valid_cols)
-{
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-        TA t[CH];
-        TA cm = ninf<TA>();
-#pragma unroll
-        for (int c = 0; c < CH; c++) {
-            const double mvd = static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv[c], v));
-            t[c] = static_cast<TA>(fma(-k, mvd, ak[c] - sh[v]));
-            if (MASK && (c >= valid_rows || v >= valid_cols)) t[c] = ninf<TA>();
-            cm = vmax(cm, t[c]);
-        }
-        rm[v] = vmax(rm[v], cm);
-        if (reshift(sh[v], s[v], S[v], rm[v])) {
-#pragma unroll
-            for (int c = 0; c < CH; c++) {
-                const double mvd =
-                    static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv[c], v));
-                t[c] = static_cast<TA>(fma(-k, mvd, ak[c] - sh[v]));
-                if (MASK && (c >= valid_rows || v >= valid_cols)) t[c] = ninf<TA>();
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < CH; c++) s[v] += ex2(t[c]);
-    }
-}
-
-// Workgroup = 4 waves over one 64V-column tile and one block of rows; the
-// waves take CH-row slabs round robin.  Output: one FP64 log2 partial sum per
-// (row block, column).
+// Workgroup (x, y) = column tile x (64V columns) over row block y (rpb rows):
+// one FP64 log2 partial sum per (row block, column) into part[y * m + j].
 template <typename TM, typename TA>
 __global__ __launch_bounds__(256) void k_col_partial(
     const TM* __restrict__ M, long ldm, int n, int m, const double* __restrict__ alpha,
@@ -403,94 +101,13 @@ __global__ __launch_bounds__(256) void k_col_partial(
     const double* __restrict__ colpot, const double* __restrict__ logb, double k, int rpb,
     double* __restrict__ part, const int* __restrict__ ctl)
 {
-    using VT = typename VecOf<TM>::type;
-    constexpr int V = VecOf<TM>::N;
-    constexpr int CH = 4;
-    constexpr int TW = 64 * V;
-    __shared__ double red[4][TW];
+    constexpr int TW = 64 * VecOf<TM>::N;
     if (ctl != nullptr && ctl[0] != 0) return;
     if (ap0 != nullptr) alpha = (ctl[1] & 1) ? ap1 : ap0;
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int j0 = blockIdx.x * TW + lane * V;
-    const int valid_cols = (m - j0) < V ? (m - j0) : V;   // <= 0: lane has no column
     const long rb = static_cast<long>(blockIdx.y) * rpb;
     const long re = (rb + rpb < n) ? rb + rpb : n;
-
-    double sh[V], S[V];
-    TA s[V], rm[V];
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-        const int j = j0 + v;
-        double g = 0.0;
-        if (v < valid_cols && logb != nullptr) {
-            g = logb[j] * kLog2e;
-            if (colpot != nullptr) g -= colpot[j] * k;
-        }
-        sh[v] = g;
-        s[v] = TA(0);
-        S[v] = 0.0;
-        rm[v] = ninf<TA>();
-    }
-
-    if (valid_cols > 0) {
-        const TM* base = M + j0;
-        long r = rb + wave * CH;
-        for (int q = 0; r + CH <= re; r += 4 * CH, q++) {
-            VT mv[CH];
-            double ak[CH];
-#pragma unroll
-            for (int c = 0; c < CH; c++) {
-                mv[c] = ld_stream(base + (r + c) * ldm);
-                ak[c] = alpha[r + c] * k;
-            }
-            if (valid_cols == V)
-                col_chunk<TM, TA, V, CH, false>(sh, s, S, rm, mv, ak, k, CH, V);
-            else
-                col_chunk<TM, TA, V, CH, true>(sh, s, S, rm, mv, ak, k, CH, valid_cols);
-            if (q % kFlushSteps == kFlushSteps - 1) {
-#pragma unroll
-                for (int v = 0; v < V; v++) flush(s[v], S[v]);
-            }
-        }
-        if (r < re) {
-            const int vr = static_cast<int>(re - r);
-            VT mv[CH];
-            double ak[CH];
-#pragma unroll
-            for (int c = 0; c < CH; c++) {
-                const long rr = (c < vr) ? r + c : r;
-                mv[c] = ld_stream(base + rr * ldm);
-                ak[c] = alpha[rr] * k;
-            }
-            col_chunk<TM, TA, V, CH, true>(sh, s, S, rm, mv, ak, k, vr, valid_cols);
-        }
-    }
-
-#pragma unroll
-    for (int v = 0; v < V; v++)
-        red[wave][lane * V + v] = acc_log2(sh[v], S[v] + static_cast<double>(s[v]));
-    __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-            if (v >= valid_cols) break;
-            double Lm = -INFINITY;
-#pragma unroll
-            for (int w = 0; w < 4; w++) Lm = fmax(Lm, red[w][lane * V + v]);
-            double out = -INFINITY;
-            if (Lm != -INFINITY) {
-                double S = 0.0;
-#pragma unroll
-                for (int w = 0; w < 4; w++) {
-                    const double L = red[w][lane * V + v];
-                    S += (L == -INFINITY) ? 0.0 : exp2(L - Lm);
-                }
-                out = Lm + log2(S);
-            }
-            part[static_cast<long>(blockIdx.y) * m + j0 + v] = out;
-        }
-    }
+    col_partial_tile<TM, TA>(M, ldm, m, alpha, colpot, logb, k, blockIdx.x * TW, rb, re,
+                             part + static_cast<long>(blockIdx.y) * m);
 }
 
 constexpr int kCombineCols = 64;   // columns per combine workgroup (4 threads each)
@@ -515,7 +132,7 @@ __device__ void bcd_decide(const BcdState& st, int kk, double* f_out, double* g_
         v[4] += cp[2 * q + 1];
     }
     block_sums_256<5>(v);
-    const double bm = st.beta[st.m - 1];
+    const double bm = st.bm[0];
     const double gnorm = sqrt(v[0] + v[3]);
     const double da = v[2] + bm * st.sum_a;
     const double db = v[4] - bm * st.sum_b1;
@@ -556,22 +173,8 @@ __global__ __launch_bounds__(256) void k_col_combine(const double* __restrict__ 
     const int j = blockIdx.x * kCombineCols + (threadIdx.x >> 2);
     double r2 = 0.0, bb = 0.0;
     if (!done) {
-        double Lm = -INFINITY;
-        if (j < m)
-            for (int p = sub; p < nblk; p += 4) Lm = fmax(Lm, part[static_cast<long>(p) * m + j]);
-        Lm = fmax(Lm, __shfl_xor(Lm, 1));
-        Lm = fmax(Lm, __shfl_xor(Lm, 2));
-        double S = 0.0;
-        if (j < m && Lm != -INFINITY)
-            for (int p = sub; p < nblk; p += 4) {
-                const double x = part[static_cast<long>(p) * m + j];
-                S += (x == -INFINITY) ? 0.0 : exp2(x - Lm);
-            }
-        S += __shfl_xor(S, 1);
-        S += __shfl_xor(S, 2);
-        S = __shfl(S, threadIdx.x & ~3);
+        const double L = combine_partials(part, nblk, m, j, j < m, sub);
         if (sub == 0 && j < m) {
-            const double L = (Lm == -INFINITY) ? -INFINITY : Lm + log2(S);
             if (epi.lse2) epi.lse2[j] = L;
             double cs = 0.0;
             if (epi.colsum || (epi.resid && !epi.beta_out)) {

@@ -36,7 +36,7 @@ void vec_obj_finish(const double* r, const double* a, const double* b, const dou
 
 // Buffers of the on-device BCD loop, kept across solves on one handle.
 struct BcdWork {
-    DBuf<double> ap0, ap1, beta, gcol, rpart, cpart;
+    DBuf<double> ap0, ap1, beta, gcol, rpart, cpart, bm;
     DBuf<int> ctl;
     MappedRing<double> hf, hg;
     MappedRing<long long> hts;

@@ -65,6 +65,7 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
     w.rpart.ensure(3L * nrb);
     w.cpart.ensure(4L * p->ntiles);
     w.ctl.ensure(4);
+    w.bm.ensure(1);
     w.hf.ensure(cap);
     w.hg.ensure(cap);
     w.hts.ensure(cap + 1);
@@ -102,6 +103,7 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
     st.hist_ts = w.hts.d;
     st.hist_cap = cap;
     st.rpart = w.rpart;
+    st.bm = w.bm;
     st.cpart = w.cpart;
     st.nrb = nrb;
     st.ntiles = p->ntiles;

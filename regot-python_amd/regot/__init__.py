@@ -339,6 +339,118 @@ class Problem:
         return res
 
 
+class BatchProblem:
+    """Many independent EROT problems resident on one GPU, solved together by
+    BCD (``regot_batch`` handle; no counterpart in the reference).
+
+    ``Ms`` is a sequence of 2-D cost matrices (NumPy arrays, or
+    ``(device_ptr, n, m, dtype)`` tuples of device-resident row-major
+    matrices), ``as_``/``bs`` the marginals and ``regs`` one reg per problem
+    (or a scalar).  ``bulk`` as for :class:`Problem` (default: FP32 when every
+    M is float32).
+    """
+
+    def __init__(self, Ms, as_, bs, regs, bulk=None, device=0):
+        Ms = list(Ms)
+        P = len(Ms)
+        if P == 0:
+            raise ValueError("empty batch")
+        regs = np.broadcast_to(np.asarray(regs, dtype=np.float64), (P,)).copy()
+        on_dev = all(isinstance(x, tuple) for x in Ms)
+        if not on_dev and any(isinstance(x, tuple) for x in Ms):
+            raise ValueError("mix of host arrays and device pointers")
+        keep = []
+        if on_dev:
+            dts = {np.dtype(x[3]) for x in Ms}
+            if len(dts) != 1:
+                raise ValueError("device matrices must share one dtype")
+            mdt = dts.pop()
+            ns = np.array([x[1] for x in Ms], dtype=np.int32)
+            ms = np.array([x[2] for x in Ms], dtype=np.int32)
+            ptrs = [int(x[0]) for x in Ms]
+        else:
+            if bulk is None:
+                bulk = "fp32" if all(np.asarray(x).dtype == np.float32 for x in Ms) else "fp64"
+            mdt = np.dtype(np.float32 if bulk == "fp32" else np.float64)
+            arrs = [np.ascontiguousarray(x, dtype=mdt) for x in Ms]
+            if any(x.ndim != 2 for x in arrs):
+                raise ValueError("every M must be 2-D")
+            keep += arrs
+            ns = np.array([x.shape[0] for x in arrs], dtype=np.int32)
+            ms = np.array([x.shape[1] for x in arrs], dtype=np.int32)
+            ptrs = [x.ctypes.data for x in arrs]
+        bulk = bulk or ("fp32" if mdt == np.float32 else "fp64")
+        if bulk not in ("fp32", "fp64"):
+            raise ValueError("bulk must be 'fp32' or 'fp64'")
+        av = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in as_]
+        bv = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in bs]
+        if len(av) != P or len(bv) != P:
+            raise ValueError("need one a and one b per problem")
+        for p in range(P):
+            if av[p].shape[0] != ns[p] or bv[p].shape[0] != ms[p]:
+                raise ValueError("problem %d: marginal sizes do not match M" % p)
+        keep += av + bv
+        VP = _lib._vp * P
+        mcode = _lib.REGOT_F32 if mdt == np.float32 else _lib.REGOT_F64
+        bcode = _lib.REGOT_F32 if bulk == "fp32" else _lib.REGOT_F64
+        h = _lib._vp()
+        check(lib().regot_batch_create(
+            _lib.ctypes.byref(h), P, _lib.iptr(ns), _lib.iptr(ms), VP(*ptrs), mcode, bcode,
+            int(on_dev), VP(*[x.ctypes.data for x in av]), VP(*[x.ctypes.data for x in bv]),
+            dptr(regs), int(device)))
+        self._h = h
+        self.nprob, self.bulk = P, bulk
+        self.n, self.m = ns.copy(), ms.copy()
+        tot = _lib.ctypes.c_longlong()
+        dl = _lib.ctypes.c_longlong()
+        check(lib().regot_batch_info(h, None, _lib.ctypes.byref(tot), _lib.ctypes.byref(dl)))
+        self.total_elems, self.dual_len = tot.value, dl.value
+
+    def solve_bcd(self, tol=1e-6, max_iter=1000, dual=True):
+        """BCD on every problem.  Returns a dict of per-problem arrays:
+        ``niter``, ``obj`` and ``mar_err`` (at the final iterate), ``dual``
+        (a list of (n+m-1,) arrays, or None), and ``rounds`` launched."""
+        P = self.nprob
+        it = np.zeros(P, dtype=np.int32)
+        ob = np.zeros(P)
+        me = np.zeros(P)
+        du = np.zeros(self.dual_len) if dual else None
+        rounds = _lib.ctypes.c_longlong()
+        check(lib().regot_batch_solve_bcd(self._h, float(tol), int(max_iter), _lib.iptr(it),
+                                          dptr(ob), dptr(me), dptr(du) if dual else None,
+                                          _lib.ctypes.byref(rounds)))
+        out = {"niter": it, "obj": ob, "mar_err": me, "rounds": rounds.value, "dual": None}
+        if dual:
+            sizes = self.n.astype(np.int64) + self.m - 1
+            out["dual"] = np.split(du, np.cumsum(sizes)[:-1])
+        return out
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().regot_batch_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def sinkhorn_bcd_batch(Ms, as_, bs, regs, tol=1e-6, max_iter=1000, bulk=None, device=0):
+    """Batched ``sinkhorn_bcd`` over independent problems (one GPU).  Returns
+    the per-problem dict of :meth:`BatchProblem.solve_bcd`."""
+    with BatchProblem(Ms, as_, bs, regs, bulk=bulk, device=device) as B:
+        return B.solve_bcd(tol=tol, max_iter=max_iter)
+
+
 _HANDLE_KW = ("bulk", "device", "compute_plan", "verify")
 
 

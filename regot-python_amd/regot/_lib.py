@@ -42,11 +42,30 @@ class Opts(ctypes.Structure):
 _lib = None
 
 
+def _torch_runtime_first():
+    """PyTorch-ROCm bundles its own HIP runtime.  Two HIP runtimes can share a
+    process and a device, but only if torch's initialises first (the other
+    order leaves torch without GPUs), so when torch is installed its device
+    runtime is brought up before this library is loaded."""
+    if os.environ.get("REGOT_SKIP_TORCH_INIT") == "1":
+        return
+    try:
+        import torch
+    except ImportError:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
+
+
 def lib():
     """Load the library once; raise if it is not built."""
     global _lib
     if _lib is not None:
         return _lib
+    _torch_runtime_first()
     if not os.path.exists(LIB_PATH):
         raise RegotError(
             "libregot_hip.so is not built (expected at %s); run "
@@ -86,6 +105,14 @@ def lib():
     L.regot_hess_apply_hs.argtypes = [_vp, _dp, _dp]
     L.regot_hess_cg.argtypes = [_vp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
                                 _ip, _dp]
+    _pp = ctypes.POINTER(_vp)
+    L.regot_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, _ip, _ip, _pp,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, _pp, _pp, _dp,
+                                     ctypes.c_int]
+    L.regot_batch_destroy.argtypes = [_vp]
+    L.regot_batch_info.argtypes = [_vp, _ip, _llp, _llp]
+    L.regot_batch_solve_bcd.argtypes = [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp,
+                                        _llp]
     L.regot_profile_enable.argtypes = [_vp, ctypes.c_int]
     L.regot_profile_get.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _dp]
     L.regot_profile_reset.argtypes = [_vp]
