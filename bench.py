@@ -7,7 +7,9 @@ stored FP32 (1 GiB, resident in HBM before timing), a, b = 0.9 Dirichlet(1)
 + 0.1/n (normalised), reg = 0.01; FP32 bulk with FP64 potentials.
 
 Prints one JSON line.  `value` is seconds to reach the tolerance (lower is
-better).  `roofline` is the dominant streaming kernel's achieved HBM rate from
+better).  `batched` reports the batched workload (512 problems of uneven
+n, m ~ U{64..4096} in one batched BCD solve to mar_err <= 1e-6, solves/s)
+beside the same problems solved one after another by the single-problem path.  `roofline` is the dominant streaming kernel's achieved HBM rate from
 HIP events on the library's stream (algorithmic bytes n*m*4 per launch).
 `cpu_baseline` times the FP64 CPU oracle (one thread) on a bounded sample of
 BCD iterations of the same problem and scales to the GPU's iteration count.
@@ -45,6 +47,68 @@ def make_problem(n, m, d, seed):
     return M32, a / a.sum(), b / b.sum()
 
 
+def batched_workload(P, seed=0):
+    """The batched EROT recipe: P problems, n, m ~ U{64..4096}, X, Y ~ N(0, I_8),
+    M = sqeuclid / max in FP32, a, b = 0.9 Dirichlet(1) + 0.1 uniform,
+    reg ~ logU[0.005, 0.1].  M is formed on the GPU (torch) and stays there."""
+    import torch
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(64, 4097, size=P)
+    ms = rng.integers(64, 4097, size=P)
+    regs = np.exp(rng.uniform(np.log(0.005), np.log(0.1), size=P))
+    As, Bs, dev = [], [], []
+    for p in range(P):
+        n, m = int(ns[p]), int(ms[p])
+        X = torch.from_numpy(rng.standard_normal((n, 8))).to("cuda:0")
+        Y = torch.from_numpy(rng.standard_normal((m, 8))).to("cuda:0")
+        M = torch.cdist(X, Y).square_()
+        M /= M.max()
+        dev.append(M.to(torch.float32).contiguous())
+        a = 0.9 * rng.dirichlet(np.ones(n)) + 0.1 / n
+        b = 0.9 * rng.dirichlet(np.ones(m)) + 0.1 / m
+        As.append(a / a.sum())
+        Bs.append(b / b.sum())
+    torch.cuda.synchronize()
+    return ns, ms, regs, As, Bs, dev
+
+
+def bench_batched(P, steps, tol, sequential=True):
+    import torch
+    import regot
+    from regot import _lib
+    ns, ms, regs, As, Bs, dev = batched_workload(P)
+    descs = [(t.data_ptr(), t.shape[0], t.shape[1], np.float32) for t in dev]
+    B = regot.BatchProblem(descs, As, Bs, regs)
+    out = B.solve_bcd(tol=tol, max_iter=100000)          # warmup
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = B.solve_bcd(tol=tol, max_iter=100000)
+    dt = (time.perf_counter() - t0) / steps
+    it = out["niter"].astype(np.float64)
+    streamed = float((it * ns * ms).sum()) * 2 * 4
+    res = {"problems": P, "elements": int((ns * ms).sum()), "tol": tol,
+           "ms_per_batch": dt * 1e3, "solves_per_s": P / dt, "rounds": out["rounds"],
+           "niter_median": float(np.median(it)), "niter_max": int(it.max()),
+           "mar_err_max": float(out["mar_err"].max()),
+           "streamed_GB": streamed / 1e9, "streamed_GBps": streamed / dt / 1e9}
+    B.close()
+    if sequential:
+        # the plain GPU form: one regot.Problem per problem, solved one after another
+        probs = [regot.Problem.from_device(t.data_ptr(), t.shape[0], t.shape[1], np.float32,
+                                           As[p], Bs[p], regs[p]) for p, t in enumerate(dev)]
+        t0 = time.perf_counter()
+        niter_seq = [P_.solve(_lib.SOLVER_BCD, tol=tol, max_iter=100000, compute_plan=False).niter
+                     for P_ in probs]
+        ds = time.perf_counter() - t0
+        res["sequential_solves_per_s"] = P / ds
+        res["sequential_niter_equal"] = bool(np.array_equal(np.array(niter_seq), out["niter"]))
+        for P_ in probs:
+            P_.close()
+    del dev
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +120,9 @@ def main():
     ap.add_argument("--max-iter", type=int, default=20000)
     ap.add_argument("--cpu-iters", type=int, default=2,
                     help="BCD iterations of the CPU oracle sample (0 = skip)")
+    ap.add_argument("--batch-problems", type=int, default=512,
+                    help="problems of the batched workload (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); the multi-GPU "
@@ -112,6 +179,12 @@ def main():
                          "(%.3f s/iter), scaled to the GPU's %d iterations"
                          % (args.cpu_iters, n, per_iter, niter)}
 
+    batched = None
+    if args.batch_problems > 0:
+        print("bench: batched workload, %d problems" % args.batch_problems, file=sys.stderr,
+              flush=True)
+        batched = bench_batched(args.batch_problems, args.batch_steps, 1e-6)
+
     out = {
         "metric": "s to mar_err<=1e-8, sinkhorn_bcd, n=m=%d EROT" % n,
         "value": ms_per_step / 1e3,
@@ -138,6 +211,7 @@ def main():
                      "row_pass_avg_ms": avg_row, "col_pass_avg_ms": avg_col,
                      "bytes_per_launch": bytes_per_pass},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out))
 

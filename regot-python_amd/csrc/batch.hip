@@ -17,7 +17,10 @@
 // (src/sinkhorn_bcd.cpp:14-124): test ||g_k|| < tol first, then update, and
 // niter = k at the stop.  A stopped problem's tiles return at once; every few
 // rounds the host drops them from the tables (the work queue shrinks as
-// problems converge), so the long tail of slow problems runs on a small grid.
+// problems converge).  When few tiles remain (the long tail of slow
+// problems), the tables switch to fine tiles -- a row tile of R rows whose 4
+// waves split the columns, 64-row column blocks -- so that a few problems
+// still spread over the whole chip with short per-wave sweeps.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -36,8 +39,10 @@ namespace {
 
 using namespace dev;
 
-constexpr int kStripBlocks = 512;   // target rows per column-pass row block
-constexpr int kCombCols = 64;       // columns per combine tile (4 threads each)
+constexpr int kBlockRows = 512;      // rows per column-pass row block (coarse tiles)
+constexpr int kBlockRowsFine = 64;   // rows per column-pass row block (fine tiles)
+constexpr int kCombCols = 64;        // columns per combine tile (4 threads each)
+constexpr long kFineBelow = 2048;    // coarse row tiles below which fine tiles are used
 
 template <typename TM>
 constexpr int kRows = sizeof(TM) == 4 ? 4 : 2;   // rows per wave in row tiles
@@ -47,6 +52,7 @@ struct Desc {
     int n, m, ldm;
     int aoff, boff;     // offsets into the packed n-sized / m-sized vectors
     int rt0, nrt;       // row tiles of p: partial slots rt0 .. rt0 + nrt - 1
+    int rpt;            // rows per row tile (4R coarse, R fine)
     int nblk, rpb;      // column-pass row blocks, rows per block
     long poff;          // offset of p's column partials (nblk x m doubles)
     int ct0, nct;       // combine tiles of p: residual slots (2 parities)
@@ -72,28 +78,59 @@ struct Vecs {
     double* gval;
 };
 
-template <typename TM>
+// Coarse (SPLIT = false): each wave owns R whole rows, 4R rows per tile.
+// Fine (SPLIT = true): the tile is R rows and wave w sweeps the w-th quarter
+// of the columns; the four partial log2 sums of a row merge through LDS.
+template <typename TM, bool SPLIT>
 __global__ __launch_bounds__(256) void k_batch_row(const TM* __restrict__ M,
                                                    const Desc* __restrict__ D,
                                                    const int2* __restrict__ tiles, Vecs v, int k)
 {
     constexpr int R = kRows<TM>;
+    constexpr int STEP = 64 * VecOf<TM>::N;
     const int2 tl = tiles[blockIdx.x];
     const int p = tl.x;
     if (v.done[p]) return;
     const Desc d = D[p];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long row0 = (static_cast<long>(tl.y) * 4 + wave) * R;
     const double* rowpot = ((k & 1) ? v.ap1 : v.ap0) + d.aoff;
     double* aout = ((k & 1) ? v.ap0 : v.ap1) + d.aoff;
     const double* loga = v.loga + d.aoff;
     const double* a = v.a + d.aoff;
+    __shared__ double lw[4][R];
     double wg2 = 0.0, wr = 0.0, waa = 0.0;
-    if (row0 < d.n) {
-        double L[R];
-        row_lse<TM, TM, R>(M + d.moff, d.ldm, d.n, d.m, v.beta + d.boff, d.k, loga, rowpot,
-                              row0, lane, L);
+    // rows of this wave (coarse) or of the tile (fine)
+    const long row0 = SPLIT ? static_cast<long>(tl.y) * R : (static_cast<long>(tl.y) * 4 + wave) * R;
+    double L[R];
+    if (SPLIT) {
+        const int chunk = ((d.m + 4 * STEP - 1) / (4 * STEP)) * STEP;
+        const int c0 = wave * chunk;
+        const int c1 = (c0 + chunk < d.m) ? c0 + chunk : d.m;
+        row_lse<TM, TM, R>(M + d.moff, d.ldm, d.n, c0, c1, v.beta + d.boff, d.k, loga, rowpot,
+                           row0, lane, L);
+        if (lane == 0)
+#pragma unroll
+            for (int r = 0; r < R; r++) lw[wave][r] = L[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            double Lm = -INFINITY;
+#pragma unroll
+            for (int w = 0; w < 4; w++) Lm = fmax(Lm, lw[w][r]);
+            double S = 0.0;
+            if (Lm != -INFINITY)
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                    S += (lw[w][r] == -INFINITY) ? 0.0 : exp2(lw[w][r] - Lm);
+            L[r] = (Lm == -INFINITY) ? -INFINITY : Lm + log2(S);
+        }
+    } else if (row0 < d.n) {
+        row_lse<TM, TM, R>(M + d.moff, d.ldm, d.n, 0, d.m, v.beta + d.boff, d.k, loga, rowpot,
+                           row0, lane, L);
+    }
+    // fine: wave 0 finishes the tile's rows; coarse: every wave its own rows
+    if ((!SPLIT || wave == 0) && row0 < d.n) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const long i = row0 + r;
@@ -280,8 +317,11 @@ struct regot_batch {
     HBuf<int> h_done;
     HBuf<int2> h_row[2], h_comb[2];
     HBuf<int4> h_col[2];
+    HBuf<Desc> h_desc[2];
+    int hd_slot = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     long long total_elems = 0;
+    bool fine = false;   // tile granularity of the current tables
 
     ~regot_batch()
     {
@@ -312,9 +352,33 @@ struct regot_batch {
         return v;
     }
 
-    // Tile tables of the listed problems into staging slot s.
+    // Tile granularity of every problem's descriptor (uploaded on change).
+    void set_mode(bool f)
+    {
+        const int R = bulk == 0 ? 4 : 2;
+        for (Desc& d : desc) {
+            d.rpt = f ? R : 4 * R;
+            d.nrt = ceil_div(d.n, d.rpt);
+            d.nblk = ceil_div(d.n, f ? kBlockRowsFine : kBlockRows);
+            d.rpb = ceil_div(d.n, d.nblk);
+        }
+        fine = f;
+        h_desc[hd_slot].ensure(desc.size());
+        std::memcpy(h_desc[hd_slot].p, desc.data(), sizeof(Desc) * desc.size());
+        HIP_CHECK(hipMemcpyAsync(d_desc.p, h_desc[hd_slot].p, sizeof(Desc) * desc.size(),
+                                 hipMemcpyHostToDevice, stream));
+        hd_slot ^= 1;
+    }
+
+    // Tile tables of the listed problems into staging slot s; switches to
+    // fine tiles when the coarse tables would be small.
     void build_tables(const std::vector<int>& act, int s, int* nrow, int* ncol, int* ncomb)
     {
+        long coarse_rows = 0;
+        const int R = bulk == 0 ? 4 : 2;
+        for (int p : act) coarse_rows += ceil_div(desc[p].n, 4 * R);
+        const bool want_fine = coarse_rows < kFineBelow;
+        if (want_fine != fine) set_mode(want_fine);
         long cr = 0, cc = 0, cb = 0;
         const int vw = bulk == 0 ? 4 : 2;
         for (int p : act) {
@@ -352,8 +416,12 @@ struct regot_batch {
         const TM* Mp = reinterpret_cast<const TM*>(M.p);
         const Vecs v = vecs();
         if (!init && nrow > 0) {
-            hipLaunchKernelGGL(k_batch_row<TM>, dim3(nrow), dim3(256), 0, stream, Mp, d_desc.p,
-                               t_row.p, v, k);
+            if (fine)
+                hipLaunchKernelGGL((k_batch_row<TM, true>), dim3(nrow), dim3(256), 0, stream, Mp,
+                                   d_desc.p, t_row.p, v, k);
+            else
+                hipLaunchKernelGGL((k_batch_row<TM, false>), dim3(nrow), dim3(256), 0, stream, Mp,
+                                   d_desc.p, t_row.p, v, k);
             KERNEL_CHECK();
         }
         if (ncol > 0) {
@@ -412,8 +480,7 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
     B->bulk = bulk;
     HIP_CHECK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
     const int esz = bulk == REGOT_F32 ? 4 : 8;
-    const int vw = bulk == REGOT_F32 ? 4 : 2;
-    const int rows_per_tile = 4 * (bulk == REGOT_F32 ? 4 : 2);
+    const int R = bulk == REGOT_F32 ? 4 : 2;
     B->desc.resize(nprob);
     long moff = 0, aoff = 0, boff = 0, rt = 0, poff = 0, ct = 0;
     for (int p = 0; p < nprob; p++) {
@@ -431,13 +498,15 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
         d.boff = static_cast<int>(boff);
         aoff += d.n;
         boff += d.m;
-        d.nrt = ceil_div(d.n, rows_per_tile);
+        // partial slots sized for fine tiles (the larger count)
+        d.rpt = 4 * R;
+        d.nrt = ceil_div(d.n, d.rpt);
         d.rt0 = static_cast<int>(rt);
-        rt += d.nrt;
-        d.nblk = ceil_div(d.n, kStripBlocks);
+        rt += ceil_div(d.n, R);
+        d.nblk = ceil_div(d.n, kBlockRows);
         d.rpb = ceil_div(d.n, d.nblk);
         d.poff = poff;
-        poff += static_cast<long>(d.nblk) * d.m;
+        poff += static_cast<long>(ceil_div(d.n, kBlockRowsFine)) * d.m;
         d.nct = ceil_div(d.m, kCombCols);
         d.ct0 = static_cast<int>(ct);
         ct += d.nct;
@@ -450,9 +519,8 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
         d.sum_b1 = sb;
         B->total_elems += static_cast<long long>(d.n) * d.m;
     }
-    if (aoff > 0x7fffffffL || boff > 0x7fffffffL)
+    if (aoff > 0x7fffffffL || boff > 0x7fffffffL || rt > 0x7fffffffL)
         fail(E_INVALID, "batch too large (sum of n or m above 2^31)");
-    (void)vw;
     B->total_n = aoff;
     B->total_m = boff;
     B->total_dual = aoff + boff - nprob;
@@ -519,10 +587,12 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
                              B->stream));
     HIP_CHECK(hipMemcpyAsync(B->b.p, hb.data(), sizeof(double) * boff, hipMemcpyHostToDevice,
                              B->stream));
+    // tables sized for fine tiles (the larger counts)
     long tr = 0, tc = 0, tb = 0;
     for (const Desc& d : B->desc) {
-        tr += d.nrt;
-        tc += static_cast<long>(ceil_div(d.m, 64 * (bulk == REGOT_F32 ? 4 : 2))) * d.nblk;
+        tr += ceil_div(d.n, R);
+        tc += static_cast<long>(ceil_div(d.m, 64 * (bulk == REGOT_F32 ? 4 : 2))) *
+              ceil_div(d.n, kBlockRowsFine);
         tb += d.nct;
     }
     B->t_row.alloc(tr);

@@ -105,15 +105,6 @@ __device__ __forceinline__ double wave_sum0(double v)
     return __shfl(v, 0);
 }
 
-// log2 of sum over the wave of 2^L (L = -inf for empty lanes).
-__device__ __forceinline__ double wave_log2sum(double L)
-{
-    const double Lmax = wave_max(L);
-    if (Lmax == -INFINITY) return -INFINITY;
-    const double e = (L == -INFINITY) ? 0.0 : exp2(L - Lmax);
-    return Lmax + log2(wave_sum0(e));
-}
-
 __device__ __forceinline__ double acc_log2(double sh, double s)
 {
     return (s > 0.0) ? sh + log2(s) : -INFINITY;
@@ -161,11 +152,35 @@ __device__ __forceinline__ bool reshift(double& sh, TA& s, double& S, TA& rm)
     return false;
 }
 
+// Wave-uniform variant for a row shared by the 64 lanes of a wave: the shift
+// sh is the same on every lane, and moves (by the wave maximum of rm) when
+// any lane's running maximum exceeds it by more than 8, or every lane's sits
+// more than 8 below it.  The lanes' totals then merge by a plain sum.  Lanes
+// far below the wave maximum lose nothing that matters: a term 2^-126 below
+// the maximum is below the FP32 resolution of the sum.  Every lane of the
+// wave must call it (wave-wide ballot and shuffles).
+template <typename TA>
+__device__ __forceinline__ bool reshift_wave(double& sh, TA& s, double& S, TA& rm)
+{
+    const bool up = __ballot(rm > TA(8)) != 0;
+    const bool down = __ballot(rm >= TA(-8)) == 0;
+    if (!(up || down)) return false;
+    const double d = wave_max(static_cast<double>(rm));
+    if (!(d > 8.0 || (d < -8.0 && d > -INFINITY))) return false;
+    sh += d;
+    const double tot = S + static_cast<double>(s);
+    S = (tot != 0.0) ? tot * exp2(-d) : 0.0;
+    s = TA(0);
+    rm = static_cast<TA>(static_cast<double>(rm) - d);
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // Row and column accumulation
 // ---------------------------------------------------------------------------
 
-// One row x V columns: t_v = (bk_v - sh) - k M_v, masked entries = -inf.
+// One row x V columns: t_v = (bk_v - sh) - k M_v, masked entries = -inf, with
+// the wave-uniform shift of reshift_wave (every lane of the wave calls it).
 // TM is the storage type of M, TA the type t, exp2 and the sums run in.
 template <typename TM, typename TA, int V, bool MASK>
 __device__ __forceinline__ void row_accumulate(double& sh, TA& s, double& S, TA& rm,
@@ -183,7 +198,7 @@ __device__ __forceinline__ void row_accumulate(double& sh, TA& s, double& S, TA&
         cm = vmax(cm, t[v]);
     }
     rm = vmax(rm, cm);
-    if (reshift(sh, s, S, rm)) {
+    if (reshift_wave(sh, s, S, rm)) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const double mvd = static_cast<double>(vget<typename VecOf<TM>::type, TM>(mv, v));
@@ -254,15 +269,16 @@ __device__ __forceinline__ void col_chunk(double* sh, TA* s, double* S, TA* rm,
 // Whole-row and column-tile sweeps
 // ---------------------------------------------------------------------------
 
-// One wave, R consecutive rows row0.. of an n x m matrix: L[r] = log2 sum_j
-// 2^(k (beta_j - M_ij)) (-inf past n; the same value on every lane).  Lane l
-// reads the V-wide vector at column c*64V + l*V of each row (1 KiB per row
-// per step, fully coalesced); the beta chunk is loaded once per step and
-// reused by all R rows.  The shift guess is log2 a_i - k rowpot_i (loga or
-// rowpot may be null).
+// One wave, R consecutive rows row0.. of an n x m matrix, columns
+// [c_begin, c_end) (c_begin a multiple of 64V): L[r] = log2 sum_j
+// 2^(k (beta_j - M_ij)) (-inf past n or for an empty range; the same value on
+// every lane).  Lane l reads the V-wide vector at column c_begin + c*64V + l*V
+// of each row (1 KiB per row per step, fully coalesced); the beta chunk is
+// loaded once per step and reused by all R rows.  The shift guess is
+// log2 a_i - k rowpot_i (loga or rowpot may be null).
 template <typename TM, typename TA, int R>
-__device__ __forceinline__ void row_lse(const TM* __restrict__ M, long ldm, int n, int m,
-                                        const double* __restrict__ beta, double k,
+__device__ __forceinline__ void row_lse(const TM* __restrict__ M, long ldm, int n, int c_begin,
+                                        int c_end, const double* __restrict__ beta, double k,
                                         const double* __restrict__ loga,
                                         const double* __restrict__ rowpot, long row0, int lane,
                                         double (&L)[R])
@@ -289,17 +305,17 @@ __device__ __forceinline__ void row_lse(const TM* __restrict__ M, long ldm, int 
     for (int r = 0; r < R; r++) rowp[r] = M + (row0 + r < n ? row0 + r : row0) * ldm;
 
     const int step = 64 * V;
-    const int nfull = m / step;
+    const int nfull = c_end > c_begin ? (c_end - c_begin) / step : 0;
     // Rows past n read row0 again (in bounds) and are not accumulated.  The
     // next step's vectors are loaded before the current ones are consumed, so
     // each wave keeps R loads in flight while it computes.
     VT mv[R];
     if (nfull > 0) {
 #pragma unroll
-        for (int r = 0; r < R; r++) mv[r] = ld_stream(rowp[r] + lane * V);
+        for (int r = 0; r < R; r++) mv[r] = ld_stream(rowp[r] + c_begin + lane * V);
     }
     for (int c = 0; c < nfull; c++) {
-        const int j0 = c * step + lane * V;
+        const int j0 = c_begin + c * step + lane * V;
         VT nx[R];
         if (c + 1 < nfull) {
 #pragma unroll
@@ -320,26 +336,33 @@ __device__ __forceinline__ void row_lse(const TM* __restrict__ M, long ldm, int 
         for (int r = 0; r < R; r++) mv[r] = nx[r];
     }
     {
-        const int j0 = nfull * step + lane * V;
-        if (j0 < m) {
-            const int valid = (m - j0) < V ? (m - j0) : V;
+        // partial last step: every lane takes part (the shift is wave-wide);
+        // lanes past c_end load an in-bounds vector and mask all of it
+        const int jt = c_begin + nfull * step;
+        if (jt < c_end) {
+            const int j0 = jt + lane * V;
+            int valid = c_end - j0;
+            valid = valid < 0 ? 0 : (valid > V ? V : valid);
+            const int jl = valid > 0 ? j0 : jt;
             double bk[V];
 #pragma unroll
             for (int v = 0; v < V; v++) bk[v] = (v < valid) ? beta[j0 + v] * k : 0.0;
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 if (row0 + r < n) {
-                    // the row pitch is a multiple of V: the vector stays inside it
-                    const VT mv1 = ld_stream(rowp[r] + j0);
+                    // jl is a multiple of V below m <= pitch (a multiple of V)
+                    const VT mv1 = ld_stream(rowp[r] + jl);
                     row_accumulate<TM, TA, V, true>(sh[r], s[r], S[r], rm[r], bk, mv1, k, valid);
                 }
             }
         }
     }
+    // the shift is wave-uniform: the lanes' totals add up directly
 #pragma unroll
-    for (int r = 0; r < R; r++)
-        L[r] = (row0 + r < n) ? wave_log2sum(acc_log2(sh[r], S[r] + static_cast<double>(s[r])))
-                              : -INFINITY;
+    for (int r = 0; r < R; r++) {
+        const double tot = wave_sum0(S[r] + static_cast<double>(s[r]));
+        L[r] = (row0 + r < n && tot > 0.0) ? sh[r] + log2(tot) : -INFINITY;
+    }
 }
 
 // One workgroup (4 waves) over one column tile of 64V columns starting at

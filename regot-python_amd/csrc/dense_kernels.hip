@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
 
     if (row0 < n) {
         double L[R];
-        row_lse<TM, TA, R>(M, ldm, n, m, beta, k, epi.loga, rowpot, row0, lane, L);
+        row_lse<TM, TA, R>(M, ldm, n, 0, m, beta, k, epi.loga, rowpot, row0, lane, L);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const long i = row0 + r;
