@@ -199,6 +199,32 @@ int regot_batch_info(const regot_batch* batch, int* nprob, long long* total_elem
 int regot_batch_solve_bcd(regot_batch* batch, double tol, int max_iter, int* niter,
                           double* obj, double* mar_err, double* dual, long long* rounds);
 
+/*
+ * Squared-Euclidean cost from BF16 point clouds on the matrix cores (no
+ * reference counterpart; the cost of the differentiable loss):
+ * M[i][j] = |x_i|^2 + |y_j|^2 - 2 x_i.y_j, X (n x d, row pitch ldx) and Y
+ * (m x d, pitch ldy) BF16 device arrays, M FP32 device array (pitch ldm).
+ * The contraction accumulates in FP32; norms and the sum are FP64.
+ */
+int regot_sqeuclid_bf16(const void* X, long long ldx, const void* Y, long long ldy, int n, int m,
+                        int d, float* M, long long ldm, int device);
+
+/*
+ * Gradient of a loss of the optimal plan with respect to M, by the implicit
+ * function theorem on the dual optimality conditions (no reference
+ * counterpart).  gamma (host, n+m-1) is the dual optimum of the handle's
+ * problem; loss 0 = entropic loss <T,M> - reg h(T) (= -f(gamma), the entropy
+ * of SinkhornProblem::primal_val, src/sinkhorn_problem_obj.cpp:300-319),
+ * 1 = sharp loss <T,M>.  Solves H w = dL/dgamma by the device CG of the
+ * sparsified Hessian with every entry kept (cg_tol <= 0: 1e-10), then writes
+ * G = dL/dM (n x m, row pitch ldg, REGOT_F32 or REGOT_F64 device array).
+ * Optional outputs: the loss value, w (host, n+m-1), CG iterations and the
+ * final relative residual.
+ */
+int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double cg_tol,
+                        int cg_max_iter, void* G, int G_dtype, long long ldg, double* loss_value,
+                        double* w_out, int* cg_iters, double* cg_err);
+
 /* Device-side timing of the hot kernels (HIP events on the handle's stream).
  * kind: 0 = row pass, 1 = column pass (partial + combine).  Enabled by
  * regot_profile_enable(p, 1); totals accumulate until reset. */

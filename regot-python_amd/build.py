@@ -18,7 +18,8 @@ OUT = os.path.join(HERE, "regot", "libregot_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
 SOURCES = ["dense_kernels.hip", "vec_kernels.hip", "sparse_kernels.hip", "hessian.hip",
-           "problem.hip", "batch.hip", "solvers.cpp", "newton.cpp"]
+           "problem.hip", "batch.hip", "cost_kernels.hip", "grad.hip", "solvers.cpp",
+           "newton.cpp"]
 ARCH = os.environ.get("REGOT_ARCH", "gfx950")
 
 

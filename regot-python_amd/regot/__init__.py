@@ -190,6 +190,25 @@ class Problem:
         check(lib().regot_dual_obj_grad(self._h, dptr(gamma), _lib.ctypes.byref(f), dptr(g)))
         return f.value, g
 
+    def implicit_grad(self, gamma, G_ptr, G_dtype, ldg=0, loss="entropic", cg_tol=1e-10,
+                      cg_max_iter=0):
+        """dL/dM at the dual optimum ``gamma`` by the implicit function theorem,
+        written to the device array at ``G_ptr`` (n x m, row pitch ``ldg``).
+        ``loss`` is "entropic" (<T,M> - reg h(T)) or "sharp" (<T,M>).
+        Returns (loss value, w, CG iterations, CG relative residual)."""
+        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        code = {"entropic": 0, "sharp": 1}[loss]
+        gcode = _lib.REGOT_F32 if np.dtype(G_dtype) == np.float32 else _lib.REGOT_F64
+        lv = _lib.ctypes.c_double()
+        w = np.empty(self.n + self.m - 1)
+        it = _lib.ctypes.c_int()
+        err = _lib.ctypes.c_double()
+        check(lib().regot_implicit_grad(self._h, dptr(gamma), code, float(cg_tol),
+                                        int(cg_max_iter), _lib._vp(int(G_ptr)), gcode,
+                                        int(ldg or self.m), _lib.ctypes.byref(lv), dptr(w),
+                                        _lib.ctypes.byref(it), _lib.ctypes.byref(err)))
+        return lv.value, w, it.value, err.value
+
     def get_plan(self, gamma):
         """src/sinkhorn_result.h:32-50"""
         gamma = np.ascontiguousarray(gamma, dtype=np.float64)
@@ -442,6 +461,14 @@ class BatchProblem:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def sqeuclid_bf16(X_ptr, ldx, Y_ptr, ldy, n, m, d, M_ptr, ldm=0, device=0):
+    """M = |x|^2 + |y|^2 - 2 X Y' from BF16 device arrays on the matrix cores
+    (FP32 device output at ``M_ptr``); see :func:`regot.autograd.sqeuclid`."""
+    check(lib().regot_sqeuclid_bf16(_lib._vp(int(X_ptr)), int(ldx), _lib._vp(int(Y_ptr)),
+                                    int(ldy), int(n), int(m), int(d), _lib._vp(int(M_ptr)),
+                                    int(ldm or m), int(device)))
 
 
 def sinkhorn_bcd_batch(Ms, as_, bs, regs, tol=1e-6, max_iter=1000, bulk=None, device=0):

@@ -113,6 +113,11 @@ def lib():
     L.regot_batch_info.argtypes = [_vp, _ip, _llp, _llp]
     L.regot_batch_solve_bcd.argtypes = [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp,
                                         _llp]
+    L.regot_sqeuclid_bf16.argtypes = [_vp, ctypes.c_longlong, _vp, ctypes.c_longlong,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
+                                      ctypes.c_longlong, ctypes.c_int]
+    L.regot_implicit_grad.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                      _vp, ctypes.c_int, ctypes.c_longlong, _dp, _dp, _ip, _dp]
     L.regot_profile_enable.argtypes = [_vp, ctypes.c_int]
     L.regot_profile_get.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _dp]
     L.regot_profile_reset.argtypes = [_vp]

@@ -1,0 +1,53 @@
+"""Dense FP64 implicit-differentiation reference for the EROT losses (CPU).
+
+Given the dual optimum gamma of min f (the oracle's solvers), form the plan T,
+the dense dual Hessian H ((n+m-1) square, beta_{m-1} fixed to 0), solve
+H w = dL/dgamma with numpy, and return G = dL/dM = dL/dM|gamma - w' dg/dM.
+Test infrastructure: the checker of the GPU adjoint.
+"""
+import numpy as np
+
+
+def plan(M, reg, gamma):
+    n, m = M.shape
+    beta = np.append(gamma[n:], 0.0)
+    return np.exp((gamma[:n, None] + beta[None, :] - M) / reg)
+
+
+def dual_hessian(T, reg):
+    n, m = T.shape
+    N = n + m - 1
+    H = np.zeros((N, N))
+    H[:n, :n] = np.diag(T.sum(1))
+    H[n:, n:] = np.diag(T.sum(0)[: m - 1])
+    H[:n, n:] = T[:, : m - 1]
+    H[n:, :n] = T[:, : m - 1].T
+    return H / reg
+
+
+def loss_value(M, reg, gamma, loss):
+    T = plan(M, reg, gamma)
+    if loss == "sharp":
+        return float((T * M).sum())
+    # <T, M> - reg h(T), h(T) = sum T (1 - log T)
+    return float((T * M).sum() - reg * (T * (1.0 - np.log(T))).sum())
+
+
+def implicit_grad(M, reg, gamma, loss):
+    n, m = M.shape
+    T = plan(M, reg, gamma)
+    H = dual_hessian(T, reg)
+    alpha = gamma[:n]
+    beta = np.append(gamma[n:], 0.0)
+    if loss == "entropic":
+        rhs = H @ gamma
+        direct = T * (1.0 - (alpha[:, None] + beta[None, :]) / reg)
+    else:
+        TM = T * M
+        rhs = np.concatenate([TM.sum(1), TM.sum(0)[: m - 1]]) / reg
+        direct = T * (1.0 - M / reg)
+    w = np.linalg.solve(H, rhs)
+    wn = w[:n]
+    wm = np.append(w[n:], 0.0)
+    G = direct + T * (wn[:, None] + wm[None, :]) / reg
+    return G, w
