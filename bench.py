@@ -9,7 +9,10 @@ stored FP32 (1 GiB, resident in HBM before timing), a, b = 0.9 Dirichlet(1)
 Prints one JSON line.  `value` is seconds to reach the tolerance (lower is
 better).  `batched` reports the batched workload (512 problems of uneven
 n, m ~ U{64..4096} in one batched BCD solve to mar_err <= 1e-6, solves/s)
-beside the same problems solved one after another by the single-problem path.  `roofline` is the dominant streaming kernel's achieved HBM rate from
+beside the same problems solved one after another by the single-problem path.
+`differentiable` reports the differentiable loss between two BF16 clouds of
+4096 points in R^64 (reg 0.05, tol 1e-6): matrix-core cost kernel, forward
+(cost + solve + implicit adjoint) and backward, beside the plain solve.  `roofline` is the dominant streaming kernel's achieved HBM rate from
 HIP events on the library's stream (algorithmic bytes n*m*4 per launch).
 `cpu_baseline` times the FP64 CPU oracle (one thread) on a bounded sample of
 BCD iterations of the same problem and scales to the GPU's iteration count.
@@ -109,6 +112,65 @@ def bench_batched(P, steps, tol, sequential=True):
     return res
 
 
+def bench_differentiable(n, d, reg, tol, steps):
+    """Differentiable loss: X, Y ~ N(0, I_d) in BF16 (seed 0), a = b uniform,
+    cost |x|^2 + |y|^2 - 2 X Y' on the matrix cores, BCD to mar_err <= tol,
+    implicit-function adjoint, gradients w.r.t. X and Y.  Timed beside the
+    plain GPU form (the same BCD solve on the formed cost)."""
+    import torch
+    import regot
+    from regot import _lib
+    from regot.autograd import sinkhorn_loss, sqeuclid
+    g = torch.Generator().manual_seed(0)
+    X = torch.randn(n, d, generator=g).to(torch.bfloat16).to("cuda:0").requires_grad_(True)
+    Y = torch.randn(n, d, generator=g).to(torch.bfloat16).to("cuda:0").requires_grad_(True)
+    a = np.full(n, 1.0 / n)
+    # plain form: cost formed once, solve only
+    M = sqeuclid(X, Y)
+    torch.cuda.synchronize()
+    P = regot.Problem.from_device(M.data_ptr(), n, n, np.float32, a, a, reg, bulk="fp32")
+    P.solve(_lib.SOLVER_BCD, tol=tol, max_iter=200000, compute_plan=False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = P.solve(_lib.SOLVER_BCD, tol=tol, max_iter=200000, compute_plan=False)
+    plain = (time.perf_counter() - t0) / steps
+    P.close()
+    # the sqeuclid kernel alone
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        M = sqeuclid(X, Y)
+    torch.cuda.synchronize()
+    cost_ms = (time.perf_counter() - t0) / 20 * 1e3
+    # differentiable form: loss forward (cost + solve + adjoint) and backward
+    info = {}
+    L = sinkhorn_loss(X, Y, reg=reg, tol=tol, max_iter=200000, info=info)
+    L.backward()
+    torch.cuda.synchronize()
+    fw = bw = 0.0
+    for _ in range(steps):
+        X.grad = None
+        Y.grad = None
+        t0 = time.perf_counter()
+        L = sinkhorn_loss(X, Y, reg=reg, tol=tol, max_iter=200000, info=info)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        fw += t1 - t0
+        bw += time.perf_counter() - t1
+    res = {"n": n, "m": n, "d": d, "reg": reg, "tol": tol, "niter": info.get("niter"),
+           "loss": L.item(), "cost_kernel_ms": cost_ms,
+           "cost_kernel_GBps": 4.0 * n * n / (cost_ms * 1e-3) / 1e9,
+           "forward_ms": fw / steps * 1e3, "backward_ms": bw / steps * 1e3,
+           "plain_solve_ms": plain * 1e3,
+           "adjoint_cg_iters": info.get("cg_iters"), "adjoint_residual": info.get("cg_err"),
+           "grad_finite": bool(torch.isfinite(X.grad.float()).all().item())}
+    del M
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +185,9 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=512,
                     help="problems of the batched workload (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--diff-n", type=int, default=4096,
+                    help="points per cloud of the differentiable workload (0 = skip)")
+    ap.add_argument("--diff-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); the multi-GPU "
@@ -185,6 +250,12 @@ def main():
               flush=True)
         batched = bench_batched(args.batch_problems, args.batch_steps, 1e-6)
 
+    diff = None
+    if args.diff_n > 0:
+        print("bench: differentiable workload, n=m=%d" % args.diff_n, file=sys.stderr,
+              flush=True)
+        diff = bench_differentiable(args.diff_n, 64, 0.05, 1e-6, args.diff_steps)
+
     out = {
         "metric": "s to mar_err<=1e-8, sinkhorn_bcd, n=m=%d EROT" % n,
         "value": ms_per_step / 1e3,
@@ -212,6 +283,7 @@ def main():
                      "bytes_per_launch": bytes_per_pass},
         "cpu_baseline": cpu,
         "batched": batched,
+        "differentiable": diff,
     }
     print(json.dumps(out))
 

@@ -87,7 +87,6 @@ __global__ __launch_bounds__(256) void k_batch_row(const TM* __restrict__ M,
                                                    const int2* __restrict__ tiles, Vecs v, int k)
 {
     constexpr int R = kRows<TM>;
-    constexpr int STEP = 64 * VecOf<TM>::N;
     const int2 tl = tiles[blockIdx.x];
     const int p = tl.x;
     if (v.done[p]) return;
@@ -98,33 +97,16 @@ __global__ __launch_bounds__(256) void k_batch_row(const TM* __restrict__ M,
     double* aout = ((k & 1) ? v.ap0 : v.ap1) + d.aoff;
     const double* loga = v.loga + d.aoff;
     const double* a = v.a + d.aoff;
-    __shared__ double lw[4][R];
     double wg2 = 0.0, wr = 0.0, waa = 0.0;
     // rows of this wave (coarse) or of the tile (fine)
     const long row0 = SPLIT ? static_cast<long>(tl.y) * R : (static_cast<long>(tl.y) * 4 + wave) * R;
     double L[R];
     if (SPLIT) {
-        const int chunk = ((d.m + 4 * STEP - 1) / (4 * STEP)) * STEP;
-        const int c0 = wave * chunk;
-        const int c1 = (c0 + chunk < d.m) ? c0 + chunk : d.m;
+        int c0, c1;
+        split_range<TM>(d.m, wave, &c0, &c1);
         row_lse<TM, TM, R>(M + d.moff, d.ldm, d.n, c0, c1, v.beta + d.boff, d.k, loga, rowpot,
                            row0, lane, L);
-        if (lane == 0)
-#pragma unroll
-            for (int r = 0; r < R; r++) lw[wave][r] = L[r];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            double Lm = -INFINITY;
-#pragma unroll
-            for (int w = 0; w < 4; w++) Lm = fmax(Lm, lw[w][r]);
-            double S = 0.0;
-            if (Lm != -INFINITY)
-#pragma unroll
-                for (int w = 0; w < 4; w++)
-                    S += (lw[w][r] == -INFINITY) ? 0.0 : exp2(lw[w][r] - Lm);
-            L[r] = (Lm == -INFINITY) ? -INFINITY : Lm + log2(S);
-        }
+        merge_split_rows<R>(L, wave, lane);
     } else if (row0 < d.n) {
         row_lse<TM, TM, R>(M + d.moff, d.ldm, d.n, 0, d.m, v.beta + d.boff, d.k, loga, rowpot,
                            row0, lane, L);

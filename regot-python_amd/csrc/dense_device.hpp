@@ -365,6 +365,40 @@ __device__ __forceinline__ void row_lse(const TM* __restrict__ M, long ldm, int 
     }
 }
 
+// The column range [c0, c1) of wave w when the 4 waves of a workgroup split
+// rows of m columns into quarters (c0 a multiple of 64V; empty ranges allowed).
+template <typename TM>
+__device__ __forceinline__ void split_range(int m, int wave, int* c0, int* c1)
+{
+    constexpr int STEP = 64 * VecOf<TM>::N;
+    const int chunk = ((m + 4 * STEP - 1) / (4 * STEP)) * STEP;
+    *c0 = wave * chunk;
+    *c1 = (*c0 + chunk < m) ? *c0 + chunk : m;
+}
+
+// Merge of the 4 waves' log2 sums of R rows through LDS (every thread of the
+// workgroup calls it; L on entry: this wave's partial, on exit: the total).
+template <int R>
+__device__ __forceinline__ void merge_split_rows(double (&L)[R], int wave, int lane)
+{
+    __shared__ double lw[4][R];
+    if (lane == 0)
+#pragma unroll
+        for (int r = 0; r < R; r++) lw[wave][r] = L[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        double Lm = -INFINITY;
+#pragma unroll
+        for (int w = 0; w < 4; w++) Lm = fmax(Lm, lw[w][r]);
+        double S = 0.0;
+        if (Lm != -INFINITY)
+#pragma unroll
+            for (int w = 0; w < 4; w++) S += (lw[w][r] == -INFINITY) ? 0.0 : exp2(lw[w][r] - Lm);
+        L[r] = (Lm == -INFINITY) ? -INFINITY : Lm + log2(S);
+    }
+}
+
 // One workgroup (4 waves) over one column tile of 64V columns starting at
 // tile_col0 and rows [rb, re): per column j, part_row[j] = log2 sum_i
 // 2^(k (alpha_i - M_ij)) over those rows.  Lane l owns the V adjacent columns

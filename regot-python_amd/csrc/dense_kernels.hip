@@ -15,11 +15,14 @@ using namespace dev;
 // Row pass
 // ---------------------------------------------------------------------------
 
-// Each wave owns R consecutive rows (row_lse).  The epilogue writes, per row,
-// the log2 sum, alpha = reg (log a - ln2 L) and/or the row sum
-// r_i = 2^(k alpha_i + L).  With FIN, per-workgroup sums of (g^2, r, alpha a)
-// are published for the combine kernel of the same BCD iteration.
-template <typename TM, typename TA, int R, bool FIN>
+// Each wave owns R consecutive rows (row_lse); with SPLIT (fewer rows than
+// CUs x 16 waves) the workgroup owns R rows and its 4 waves sweep column
+// quarters, so that the grid still fills the chip and each wave's sweep stays
+// short.  The epilogue writes, per row, the log2 sum, alpha = reg (log a -
+// ln2 L) and/or the row sum r_i = 2^(k alpha_i + L).  With FIN, per-workgroup
+// sums of (g^2, r, alpha a) are published for the combine kernel of the same
+// BCD iteration.
+template <typename TM, typename TA, int R, bool FIN, bool SPLIT>
 __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long ldm, int n,
                                                   int m, const double* __restrict__ beta,
                                                   double k, RowEpi epi, BcdState fin,
@@ -29,7 +32,8 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
     const int lane = threadIdx.x & 63;
     // wave-uniform: row tests below become scalar branches, not exec masks
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long row0 = (static_cast<long>(blockIdx.x) * 4 + wave) * R;
+    const long row0 = SPLIT ? static_cast<long>(blockIdx.x) * R
+                            : (static_cast<long>(blockIdx.x) * 4 + wave) * R;
 
     const double* rowpot = epi.rowpot;
     double* aout = epi.alpha_out;
@@ -40,9 +44,17 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
     }
     double wg2 = 0.0, wr = 0.0, waa = 0.0;
 
-    if (row0 < n) {
-        double L[R];
+    double L[R];
+    if (SPLIT) {
+        int c0, c1;
+        split_range<TM>(m, wave, &c0, &c1);
+        row_lse<TM, TA, R>(M, ldm, n, c0, c1, beta, k, epi.loga, rowpot, row0, lane, L);
+        merge_split_rows<R>(L, wave, lane);
+    } else if (row0 < n) {
         row_lse<TM, TA, R>(M, ldm, n, 0, m, beta, k, epi.loga, rowpot, row0, lane, L);
+    }
+    // split: wave 0 finishes the workgroup's rows; otherwise every wave its own
+    if ((!SPLIT || wave == 0) && row0 < n) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const long i = row0 + r;
@@ -276,9 +288,35 @@ int grid_for(long count)
 template <typename TM>
 constexpr int kRowsPerWave = sizeof(TM) == 4 ? 4 : 2;
 
+// Whole-row waves while they give >= 1024 workgroups (4 per CU), else split rows.
+constexpr int kSplitBelow = 1024;
+
+bool row_pass_split(int bulk, int n)
+{
+    const int R = bulk == 0 ? kRowsPerWave<float> : kRowsPerWave<double>;
+    return ceil_div(n, 4 * R) < kSplitBelow;
+}
+
 int row_pass_blocks(int bulk, int n)
 {
-    return ceil_div(n, 4 * (bulk == 0 ? kRowsPerWave<float> : kRowsPerWave<double>));
+    const int R = bulk == 0 ? kRowsPerWave<float> : kRowsPerWave<double>;
+    return row_pass_split(bulk, n) ? ceil_div(n, R) : ceil_div(n, 4 * R);
+}
+
+template <typename TM, typename TA, bool FIN>
+void row_pass_launch(const void* M, long ldm, int n, int m, const double* beta, double k,
+                     const RowEpi& epi, const BcdState& st, const int* ctl, hipStream_t s,
+                     int bulk)
+{
+    constexpr int R = kRowsPerWave<TM>;
+    const dim3 grid(row_pass_blocks(bulk, n));
+    if (row_pass_split(bulk, n))
+        hipLaunchKernelGGL((k_row_pass<TM, TA, R, FIN, true>), grid, dim3(256), 0, s,
+                           static_cast<const TM*>(M), ldm, n, m, beta, k, epi, st, ctl);
+    else
+        hipLaunchKernelGGL((k_row_pass<TM, TA, R, FIN, false>), grid, dim3(256), 0, s,
+                           static_cast<const TM*>(M), ldm, n, m, beta, k, epi, st, ctl);
+    KERNEL_CHECK();
 }
 
 void launch_row_pass(int bulk, const void* M, long ldm, int n, int m, const double* beta,
@@ -286,44 +324,29 @@ void launch_row_pass(int bulk, const void* M, long ldm, int n, int m, const doub
 {
     if (n <= 0) return;
     BcdState none{};
-    const dim3 grid(row_pass_blocks(bulk, n));
-    if (bulk == 0 && !precise) {
-        constexpr int R = kRowsPerWave<float>;
-        hipLaunchKernelGGL((k_row_pass<float, float, R, false>), grid, dim3(256), 0, s,
-                           static_cast<const float*>(M), ldm, n, m, beta, k, epi, none, ctl);
-    } else if (bulk == 0) {
-        constexpr int R = kRowsPerWave<float>;
-        hipLaunchKernelGGL((k_row_pass<float, double, R, false>), grid, dim3(256), 0, s,
-                           static_cast<const float*>(M), ldm, n, m, beta, k, epi, none, ctl);
-    } else {
-        constexpr int R = kRowsPerWave<double>;
-        hipLaunchKernelGGL((k_row_pass<double, double, R, false>), grid, dim3(256), 0, s,
-                           static_cast<const double*>(M), ldm, n, m, beta, k, epi, none, ctl);
-    }
-    KERNEL_CHECK();
+    if (bulk == 0 && !precise)
+        row_pass_launch<float, float, false>(M, ldm, n, m, beta, k, epi, none, ctl, s, bulk);
+    else if (bulk == 0)
+        row_pass_launch<float, double, false>(M, ldm, n, m, beta, k, epi, none, ctl, s, bulk);
+    else
+        row_pass_launch<double, double, false>(M, ldm, n, m, beta, k, epi, none, ctl, s, bulk);
 }
 
 void launch_row_pass_bcd(int bulk, const void* M, long ldm, int n, int m, const double* beta,
                          double k, const RowEpi& epi, const BcdState& st, hipStream_t s)
 {
-    const dim3 grid(row_pass_blocks(bulk, n));
-    if (bulk == 0) {
-        constexpr int R = kRowsPerWave<float>;
-        hipLaunchKernelGGL((k_row_pass<float, float, R, true>), grid, dim3(256), 0, s,
-                           static_cast<const float*>(M), ldm, n, m, beta, k, epi, st, st.ctl);
-    } else {
-        constexpr int R = kRowsPerWave<double>;
-        hipLaunchKernelGGL((k_row_pass<double, double, R, true>), grid, dim3(256), 0, s,
-                           static_cast<const double*>(M), ldm, n, m, beta, k, epi, st, st.ctl);
-    }
-    KERNEL_CHECK();
+    if (bulk == 0)
+        row_pass_launch<float, float, true>(M, ldm, n, m, beta, k, epi, st, st.ctl, s, bulk);
+    else
+        row_pass_launch<double, double, true>(M, ldm, n, m, beta, k, epi, st, st.ctl, s, bulk);
 }
 
 int col_pass_blocks(int n, int m)
 {
     // Aim for >= 512 workgroups (2 per CU on 256 CUs) with >= 16 rows each:
     // long column streams per workgroup measured fastest (16384^2 FP32:
-    // 163 us at 512 workgroups, 175 us at 2048).
+    // 163 us at 512 workgroups, 166 us at 1024, 175 us at 2048; 4096^2: 35 us
+    // at 512, 40 us at 2048).
     const int tiles = ceil_div(m, 256);
     int nblk = ceil_div(512, tiles);
     const int maxblk = ceil_div(n, 16);

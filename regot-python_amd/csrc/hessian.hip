@@ -353,7 +353,7 @@ void hess_apply_hs(regot_problem* p, const double* x, double* out)
 // stopping rule ||r||^2 < max(tol^2 ||b||^2, DBL_MIN) and the iteration cap
 // counted as Eigen's ConjugateGradient counts it (src/sinkhorn_cg.cpp:29-38).
 int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
-                  double* x, double* tol_error)
+                  double* x, double* tol_error, const double* x0)
 {
     SparseHessian& H = hess_of(p);
     const int n = p->n, m1 = p->m - 1;
@@ -375,26 +375,36 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     H.cg_hctl.ensure(16);
     H.hbuf.ensure(8);
 
-    launch_fill(x, 0.0, m1, s);
-    vec_copy(H.cg_r, rhs, m1, s);
-    vec_copy(H.cg_p, rhs, m1, s);
-    const double* xs[1] = {rhs};
-    vec_dots(xs, xs, m1, 1, H.cg_scal.p + 3, s);
-    HIP_CHECK(hipMemcpyAsync(H.hbuf.p, H.cg_scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, s));
+    // x = x0 (or 0), r = p = rhs - Delta x
+    if (x0) {
+        vec_copy(x, x0, m1, s);
+        hess_apply_delta(p, x0, shift, H.cg_q);
+        vec_axpby(H.cg_r, 1.0, rhs, -1.0, H.cg_q, m1, s);
+    } else {
+        launch_fill(x, 0.0, m1, s);
+        vec_copy(H.cg_r, rhs, m1, s);
+    }
+    vec_copy(H.cg_p, H.cg_r, m1, s);
+    const double* xs[2] = {rhs, H.cg_r.p};
+    vec_dots(xs, xs, m1, 2, H.cg_scal.p + 3, s);
+    HIP_CHECK(hipMemcpyAsync(H.hbuf.p, H.cg_scal.p + 3, 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     const double rhs_norm2 = H.hbuf.p[0];
+    const double rr0 = H.hbuf.p[1];
     if (rhs_norm2 == 0.0) {
+        if (x0) launch_fill(x, 0.0, m1, s);
         if (tol_error) *tol_error = 0.0;
         return 0;
     }
     double thr = tol * tol * rhs_norm2;
     if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
-    // rr = rhs_norm2 >= thr unless tol >= 1; then no iteration runs.
-    if (rhs_norm2 < thr) {
-        if (tol_error) *tol_error = 1.0;
+    // converged at the start (tol >= 1 without a guess, or an exact guess)
+    if (rr0 < thr) {
+        if (tol_error) *tol_error = std::sqrt(rr0 / rhs_norm2);
         return 0;
     }
-    double hs[4] = {rhs_norm2, thr, rhs_norm2, rhs_norm2};
+    double hs[4] = {rr0, thr, rr0, rhs_norm2};
     HIP_CHECK(hipMemcpyAsync(H.cg_scal.p, hs, 4 * sizeof(double), hipMemcpyHostToDevice, s));
     int hc[4] = {0, 0, 0, 0};
     HIP_CHECK(hipMemcpyAsync(H.cg_ctl.p, hc, 4 * sizeof(int), hipMemcpyHostToDevice, s));
@@ -479,7 +489,7 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
 // (Hs + shift I)^{-1} rhs by block elimination (src/sinkhorn_cg.cpp:9-50):
 // v = w / (h1 + s); y = sigma' v; solve Delta r2 = z - y; r1 = (w - sigma r2) / (h1 + s).
 int hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
-            double* out, double* tol_error)
+            double* out, double* tol_error, const double* x0)
 {
     SparseHessian& H = hess_of(p);
     const int n = p->n, m1 = p->m - 1;
@@ -493,7 +503,8 @@ int hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int m
     KERNEL_CHECK();
     launch_spmv_cols(H, H.cg_y_buf, nullptr, 0.0, 0, H.vm2, s);
     vec_axpby(H.vm2, -1.0, H.vm2, 1.0, rhs + n, m1, s);
-    const int it = hess_cg_delta(p, H.vm2, shift, tol, max_iter, out + n, tol_error);
+    const int it = hess_cg_delta(p, H.vm2, shift, tol, max_iter, out + n, tol_error,
+                                 x0 ? x0 + n : nullptr);
     launch_spmv_rows(H, out + n, H.cg_y_buf, false, 0.0, s);
     hipLaunchKernelGGL(k_sub_div, dim3(ceil_div(n, 256)), dim3(256), 0, s, rhs, H.cg_y_buf.p,
                        H.h1.p, shift, n, out);

@@ -15,11 +15,13 @@ void hess_threshold(regot_problem* p, const double* d_gamma, const double* d_gra
 void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, double density);
 void hess_apply_delta(regot_problem* p, const double* x, double shift, double* out);
 void hess_apply_hs(regot_problem* p, const double* x, double* out);
-// CG on the Schur complement Delta (m-1); returns iterations.
+// CG on the Schur complement Delta (m-1) from x0 (device, m-1) or zero;
+// relative residual tol against rhs; returns iterations.
 int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
-                  double* x, double* tol_error);
-// (Hs + shift I)^{-1} rhs (n+m-1) by block elimination; returns CG iterations.
+                  double* x, double* tol_error, const double* x0 = nullptr);
+// (Hs + shift I)^{-1} rhs (n+m-1) by block elimination; x0 (device, n+m-1,
+// only its last m-1 entries are used) warm-starts the CG; returns CG iterations.
 int hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
-            double* out, double* tol_error);
+            double* out, double* tol_error, const double* x0 = nullptr);
 
 }  // namespace regot

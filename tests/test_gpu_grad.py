@@ -46,7 +46,8 @@ def test_implicit_grad_matches_dense_reference(gpu_lib, loss):
     value, w, it, err = P.implicit_grad(gamma, G.data_ptr(), np.float64, m, loss=loss,
                                         cg_tol=1e-13)
     P.close()
-    assert err < 1e-12 and it > 0
+    # the entropic adjoint is warm-started at its exact solution (the dual)
+    assert err < 1e-12 and (it > 0 or loss == "entropic")
     np.testing.assert_allclose(w, w_ref, rtol=1e-8, atol=1e-10)
     np.testing.assert_allclose(G.cpu().numpy(), G_ref, rtol=1e-8, atol=1e-12)
     assert abs(value - loss_value(M, reg, gamma, loss)) < 1e-10 * max(1.0, abs(value))
