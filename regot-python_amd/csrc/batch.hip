@@ -18,9 +18,10 @@
 // niter = k at the stop.  A stopped problem's tiles return at once; every few
 // rounds the host drops them from the tables (the work queue shrinks as
 // problems converge).  When few tiles remain (the long tail of slow
-// problems), the tables switch to fine tiles -- a row tile of R rows whose 4
-// waves split the columns, 64-row column blocks -- so that a few problems
-// still spread over the whole chip with short per-wave sweeps.
+// problems), the tables switch to fine tiles -- one-row waves, 64-row column
+// blocks -- so that a few problems still spread over the whole chip, and M
+// is read through the caches once the active matrices fit the 256 MB
+// Infinity Cache.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -78,15 +79,13 @@ struct Vecs {
     double* gval;
 };
 
-// Coarse (SPLIT = false): each wave owns R whole rows, 4R rows per tile.
-// Fine (SPLIT = true): the tile is R rows and wave w sweeps the w-th quarter
-// of the columns; the four partial log2 sums of a row merge through LDS.
-template <typename TM, bool SPLIT>
+// Each wave owns R whole rows, 4R rows per tile (R = kRows coarse, 1 fine).
+template <typename TM, int R>
 __global__ __launch_bounds__(256) void k_batch_row(const TM* __restrict__ M,
                                                    const Desc* __restrict__ D,
-                                                   const int2* __restrict__ tiles, Vecs v, int k)
+                                                   const int2* __restrict__ tiles, Vecs v, int k,
+                                                   int nt)
 {
-    constexpr int R = kRows<TM>;
     const int2 tl = tiles[blockIdx.x];
     const int p = tl.x;
     if (v.done[p]) return;
@@ -98,21 +97,11 @@ __global__ __launch_bounds__(256) void k_batch_row(const TM* __restrict__ M,
     const double* loga = v.loga + d.aoff;
     const double* a = v.a + d.aoff;
     double wg2 = 0.0, wr = 0.0, waa = 0.0;
-    // rows of this wave (coarse) or of the tile (fine)
-    const long row0 = SPLIT ? static_cast<long>(tl.y) * R : (static_cast<long>(tl.y) * 4 + wave) * R;
-    double L[R];
-    if (SPLIT) {
-        int c0, c1;
-        split_range<TM>(d.m, wave, &c0, &c1);
-        row_lse<TM, TM, R>(M + d.moff, d.ldm, d.n, c0, c1, v.beta + d.boff, d.k, loga, rowpot,
-                           row0, lane, L);
-        merge_split_rows<R>(L, wave, lane);
-    } else if (row0 < d.n) {
+    const long row0 = (static_cast<long>(tl.y) * 4 + wave) * R;
+    if (row0 < d.n) {
+        double L[R];
         row_lse<TM, TM, R>(M + d.moff, d.ldm, d.n, 0, d.m, v.beta + d.boff, d.k, loga, rowpot,
-                           row0, lane, L);
-    }
-    // fine: wave 0 finishes the tile's rows; coarse: every wave its own rows
-    if ((!SPLIT || wave == 0) && row0 < d.n) {
+                           row0, lane, L, nt != 0);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const long i = row0 + r;
@@ -147,7 +136,7 @@ template <typename TM>
 __global__ __launch_bounds__(256) void k_batch_colpart(const TM* __restrict__ M,
                                                        const Desc* __restrict__ D,
                                                        const int4* __restrict__ tiles, Vecs v,
-                                                       int k, int init)
+                                                       int k, int init, int nt)
 {
     constexpr int TW = 64 * VecOf<TM>::N;
     const int4 tl = tiles[blockIdx.x];
@@ -159,7 +148,7 @@ __global__ __launch_bounds__(256) void k_batch_colpart(const TM* __restrict__ M,
     const long re = (rb + d.rpb < d.n) ? rb + d.rpb : d.n;
     col_partial_tile<TM, TM>(M + d.moff, d.ldm, d.m, alpha, init ? nullptr : v.beta + d.boff,
                                 v.logb + d.boff, d.k, tl.y * TW, rb, re,
-                                v.part + d.poff + static_cast<long>(tl.z) * d.m);
+                                v.part + d.poff + static_cast<long>(tl.z) * d.m, nt != 0);
 }
 
 // Stop decision for iterate k of problem p (identical in every tile of p):
@@ -304,6 +293,7 @@ struct regot_batch {
     hipEvent_t ev[2] = {nullptr, nullptr};
     long long total_elems = 0;
     bool fine = false;   // tile granularity of the current tables
+    bool nt = true;      // nontemporal loads of M (active matrices beyond the cache)
 
     ~regot_batch()
     {
@@ -339,7 +329,7 @@ struct regot_batch {
     {
         const int R = bulk == 0 ? 4 : 2;
         for (Desc& d : desc) {
-            d.rpt = f ? R : 4 * R;
+            d.rpt = f ? 4 : 4 * R;
             d.nrt = ceil_div(d.n, d.rpt);
             d.nblk = ceil_div(d.n, f ? kBlockRowsFine : kBlockRows);
             d.rpb = ceil_div(d.n, d.nblk);
@@ -357,8 +347,13 @@ struct regot_batch {
     void build_tables(const std::vector<int>& act, int s, int* nrow, int* ncol, int* ncomb)
     {
         long coarse_rows = 0;
+        double bytes = 0.0;
         const int R = bulk == 0 ? 4 : 2;
-        for (int p : act) coarse_rows += ceil_div(desc[p].n, 4 * R);
+        for (int p : act) {
+            coarse_rows += ceil_div(desc[p].n, 4 * R);
+            bytes += static_cast<double>(desc[p].n) * desc[p].ldm * (bulk == 0 ? 4 : 8);
+        }
+        nt = bytes > static_cast<double>(256L << 20);
         const bool want_fine = coarse_rows < kFineBelow;
         if (want_fine != fine) set_mode(want_fine);
         long cr = 0, cc = 0, cb = 0;
@@ -397,18 +392,19 @@ struct regot_batch {
     {
         const TM* Mp = reinterpret_cast<const TM*>(M.p);
         const Vecs v = vecs();
+        const int ntf = nt ? 1 : 0;
         if (!init && nrow > 0) {
             if (fine)
-                hipLaunchKernelGGL((k_batch_row<TM, true>), dim3(nrow), dim3(256), 0, stream, Mp,
-                                   d_desc.p, t_row.p, v, k);
+                hipLaunchKernelGGL((k_batch_row<TM, 1>), dim3(nrow), dim3(256), 0, stream, Mp,
+                                   d_desc.p, t_row.p, v, k, ntf);
             else
-                hipLaunchKernelGGL((k_batch_row<TM, false>), dim3(nrow), dim3(256), 0, stream, Mp,
-                                   d_desc.p, t_row.p, v, k);
+                hipLaunchKernelGGL((k_batch_row<TM, kRows<TM>>), dim3(nrow), dim3(256), 0, stream,
+                                   Mp, d_desc.p, t_row.p, v, k, ntf);
             KERNEL_CHECK();
         }
         if (ncol > 0) {
             hipLaunchKernelGGL(k_batch_colpart<TM>, dim3(ncol), dim3(256), 0, stream, Mp,
-                               d_desc.p, t_col.p, v, k, init);
+                               d_desc.p, t_col.p, v, k, init, ntf);
             KERNEL_CHECK();
         }
         if (ncomb > 0) {
@@ -484,7 +480,7 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
         d.rpt = 4 * R;
         d.nrt = ceil_div(d.n, d.rpt);
         d.rt0 = static_cast<int>(rt);
-        rt += ceil_div(d.n, R);
+        rt += ceil_div(d.n, 4);
         d.nblk = ceil_div(d.n, kBlockRows);
         d.rpb = ceil_div(d.n, d.nblk);
         d.poff = poff;
@@ -572,7 +568,7 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
     // tables sized for fine tiles (the larger counts)
     long tr = 0, tc = 0, tb = 0;
     for (const Desc& d : B->desc) {
-        tr += ceil_div(d.n, R);
+        tr += ceil_div(d.n, 4);
         tc += static_cast<long>(ceil_div(d.m, 64 * (bulk == REGOT_F32 ? 4 : 2))) *
               ceil_div(d.n, kBlockRowsFine);
         tb += d.nct;

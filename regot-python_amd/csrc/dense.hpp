@@ -7,16 +7,14 @@
 //   * Log-sums are returned in log2 units: lse2 = log2 sum 2^(x).
 //   * k = log2(e) / reg converts (potential - M) / reg into log2 units.
 //   * ctl (nullable) is the device control block of a running BCD solve:
-//     ctl[0] = done flag (kernels return at once when set), ctl[1] = k.
+//     ctl[0] = done flag (kernels return at once when set).
 #pragma once
 
 #include "common.hpp"
 
 namespace regot {
 
-// Per-row epilogue of the row pass.  With parity set, the row potential and
-// the output potential come from apair[k & 1] and apair[(k + 1) & 1], where k
-// is ctl[1]: the double buffer of the on-device BCD loop.
+// Per-row epilogue of the row pass.
 struct RowEpi {
     double* lse2 = nullptr;          // log2 sum_j 2^((beta_j - M_ij) k)
     double* alpha_out = nullptr;     // reg * (log a_i - ln2 * lse2)
@@ -24,8 +22,6 @@ struct RowEpi {
     const double* rowpot = nullptr;  // alpha used for row sums / the shift guess
     double* rowsum = nullptr;        // sum_j exp((alpha_i + beta_j - M_ij)/reg)
     double reg = 1.0;
-    bool parity = false;
-    double* apair[2] = {nullptr, nullptr};
 };
 
 // Per-column epilogue of the column pass, run by the combine kernel.
@@ -61,7 +57,6 @@ struct BcdState;
 int col_pass_blocks(int n, int m);
 int col_combine_blocks(int m);
 void launch_col_pass(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
-                     const double* alpha_pair0, const double* alpha_pair1,
                      const double* colpot_guess, const double* logb, double k, double* part,
                      int nblk, const ColEpi& epi, const BcdState* st, const int* ctl,
                      hipStream_t s, bool precise = false);
@@ -71,15 +66,16 @@ void launch_plan(int bulk, const void* M, long ldm, int n, int m, const double* 
                  const double* beta, double reg, double* plan, long r0, int nr, hipStream_t s);
 
 // On-device BCD bookkeeping.  The row pass (FIN) publishes per-workgroup sums
-// of (g^2, r, alpha a) and copies k into ctl[3]; every combine workgroup of
-// the same iteration then reduces them (with the column-tile sums of the
-// previous update, double-buffered by the parity of k) to the same f_k and
-// ||g_k|| and takes the same stop decision.
+// of (g^2, r, alpha a); every combine workgroup of the same iteration then
+// reduces them (with the column-tile sums of the previous update,
+// double-buffered by the parity of k) to the same f_k and ||g_k|| and takes
+// the same stop decision.  k is the iteration the launch belongs to (the host
+// enqueues iterations in order; launches past the stop return at once).
 struct BcdState {
     const double* a;
-    double* apair[2];
     const double* beta;
-    int* ctl;               // [0] done, [1] k, [2] niter, [3] k as seen by the row pass
+    int k;                  // iteration of this launch
+    int* ctl;               // [0] done, [1] iterations completed, [2] niter
     double* hist_f;         // rings of hist_cap entries (device-visible)
     double* hist_g;
     long long* hist_ts;

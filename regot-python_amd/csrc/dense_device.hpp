@@ -90,6 +90,16 @@ __device__ __forceinline__ double2 ld_stream(const double* p)
     return make_double2(v.x, v.y);
 }
 
+// Load of one vector of M: nontemporal (nt) for matrices far larger than the
+// last-level cache, which are streamed once per pass; cached otherwise, so a
+// matrix that fits the 256 MB Infinity Cache stays there between passes.
+template <typename T>
+__device__ __forceinline__ typename VecOf<T>::type ld_m(const T* p, bool nt)
+{
+    using VT = typename VecOf<T>::type;
+    return nt ? ld_stream(p) : *reinterpret_cast<const VT*>(p);
+}
+
 __device__ __forceinline__ double wave_max(double v)
 {
 #pragma unroll
@@ -281,10 +291,29 @@ __device__ __forceinline__ void row_lse(const TM* __restrict__ M, long ldm, int 
                                         int c_end, const double* __restrict__ beta, double k,
                                         const double* __restrict__ loga,
                                         const double* __restrict__ rowpot, long row0, int lane,
-                                        double (&L)[R])
+                                        double (&L)[R], bool nt)
 {
     using VT = typename VecOf<TM>::type;
     constexpr int V = VecOf<TM>::N;
+    const TM* rowp[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) rowp[r] = M + (row0 + r < n ? row0 + r : row0) * ldm;
+
+    const int step = 64 * V;
+    const int nfull = c_end > c_begin ? (c_end - c_begin) / step : 0;
+    // Rows past n read row0 again (in bounds) and are not accumulated.  The
+    // next step's vectors of M and beta are loaded before the current ones are
+    // consumed, so each wave keeps its loads in flight while it computes; the
+    // first step's loads go out before the shift guesses are read.
+    VT mv[R];
+    double bc[V];
+    if (nfull > 0) {
+#pragma unroll
+        for (int r = 0; r < R; r++) mv[r] = ld_m(rowp[r] + c_begin + lane * V, nt);
+#pragma unroll
+        for (int v = 0; v < V; v++) bc[v] = beta[c_begin + lane * V + v];
+    }
+
     double sh[R], S[R];
     TA s[R], rm[R];
 #pragma unroll
@@ -300,30 +329,20 @@ __device__ __forceinline__ void row_lse(const TM* __restrict__ M, long ldm, int 
         S[r] = 0.0;
         rm[r] = ninf<TA>();
     }
-    const TM* rowp[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) rowp[r] = M + (row0 + r < n ? row0 + r : row0) * ldm;
 
-    const int step = 64 * V;
-    const int nfull = c_end > c_begin ? (c_end - c_begin) / step : 0;
-    // Rows past n read row0 again (in bounds) and are not accumulated.  The
-    // next step's vectors are loaded before the current ones are consumed, so
-    // each wave keeps R loads in flight while it computes.
-    VT mv[R];
-    if (nfull > 0) {
-#pragma unroll
-        for (int r = 0; r < R; r++) mv[r] = ld_stream(rowp[r] + c_begin + lane * V);
-    }
     for (int c = 0; c < nfull; c++) {
         const int j0 = c_begin + c * step + lane * V;
         VT nx[R];
+        double bn[V];
         if (c + 1 < nfull) {
 #pragma unroll
-            for (int r = 0; r < R; r++) nx[r] = ld_stream(rowp[r] + j0 + step);
+            for (int r = 0; r < R; r++) nx[r] = ld_m(rowp[r] + j0 + step, nt);
+#pragma unroll
+            for (int v = 0; v < V; v++) bn[v] = beta[j0 + step + v];
         }
         double bk[V];
 #pragma unroll
-        for (int v = 0; v < V; v++) bk[v] = beta[j0 + v] * k;
+        for (int v = 0; v < V; v++) bk[v] = bc[v] * k;
 #pragma unroll
         for (int r = 0; r < R; r++)
             if (row0 + r < n)
@@ -334,6 +353,8 @@ __device__ __forceinline__ void row_lse(const TM* __restrict__ M, long ldm, int 
         }
 #pragma unroll
         for (int r = 0; r < R; r++) mv[r] = nx[r];
+#pragma unroll
+        for (int v = 0; v < V; v++) bc[v] = bn[v];
     }
     {
         // partial last step: every lane takes part (the shift is wave-wide);
@@ -351,7 +372,7 @@ __device__ __forceinline__ void row_lse(const TM* __restrict__ M, long ldm, int 
             for (int r = 0; r < R; r++) {
                 if (row0 + r < n) {
                     // jl is a multiple of V below m <= pitch (a multiple of V)
-                    const VT mv1 = ld_stream(rowp[r] + jl);
+                    const VT mv1 = ld_m(rowp[r] + jl, nt);
                     row_accumulate<TM, TA, V, true>(sh[r], s[r], S[r], rm[r], bk, mv1, k, valid);
                 }
             }
@@ -362,40 +383,6 @@ __device__ __forceinline__ void row_lse(const TM* __restrict__ M, long ldm, int 
     for (int r = 0; r < R; r++) {
         const double tot = wave_sum0(S[r] + static_cast<double>(s[r]));
         L[r] = (row0 + r < n && tot > 0.0) ? sh[r] + log2(tot) : -INFINITY;
-    }
-}
-
-// The column range [c0, c1) of wave w when the 4 waves of a workgroup split
-// rows of m columns into quarters (c0 a multiple of 64V; empty ranges allowed).
-template <typename TM>
-__device__ __forceinline__ void split_range(int m, int wave, int* c0, int* c1)
-{
-    constexpr int STEP = 64 * VecOf<TM>::N;
-    const int chunk = ((m + 4 * STEP - 1) / (4 * STEP)) * STEP;
-    *c0 = wave * chunk;
-    *c1 = (*c0 + chunk < m) ? *c0 + chunk : m;
-}
-
-// Merge of the 4 waves' log2 sums of R rows through LDS (every thread of the
-// workgroup calls it; L on entry: this wave's partial, on exit: the total).
-template <int R>
-__device__ __forceinline__ void merge_split_rows(double (&L)[R], int wave, int lane)
-{
-    __shared__ double lw[4][R];
-    if (lane == 0)
-#pragma unroll
-        for (int r = 0; r < R; r++) lw[wave][r] = L[r];
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        double Lm = -INFINITY;
-#pragma unroll
-        for (int w = 0; w < 4; w++) Lm = fmax(Lm, lw[w][r]);
-        double S = 0.0;
-        if (Lm != -INFINITY)
-#pragma unroll
-            for (int w = 0; w < 4; w++) S += (lw[w][r] == -INFINITY) ? 0.0 : exp2(lw[w][r] - Lm);
-        L[r] = (Lm == -INFINITY) ? -INFINITY : Lm + log2(S);
     }
 }
 
@@ -411,7 +398,7 @@ __device__ __forceinline__ void col_partial_tile(const TM* __restrict__ M, long 
                                                  const double* __restrict__ colpot,
                                                  const double* __restrict__ logb, double k,
                                                  int tile_col0, long rb, long re,
-                                                 double* __restrict__ part_row)
+                                                 double* __restrict__ part_row, bool nt)
 {
     using VT = typename VecOf<TM>::type;
     constexpr int V = VecOf<TM>::N;
@@ -442,14 +429,31 @@ __device__ __forceinline__ void col_partial_tile(const TM* __restrict__ M, long 
     if (valid_cols > 0) {
         const TM* base = M + j0;
         long r = rb + wave * CH;
-        for (int q = 0; r + CH <= re; r += 4 * CH, q++) {
-            VT mv[CH];
-            double ak[CH];
+        // chunks of CH rows; the next chunk's rows of M and alpha are loaded
+        // before the current chunk is consumed
+        VT mv[CH];
+        double ar[CH];
+        if (r + CH <= re) {
 #pragma unroll
             for (int c = 0; c < CH; c++) {
-                mv[c] = ld_stream(base + (r + c) * ldm);
-                ak[c] = alpha[r + c] * k;
+                mv[c] = ld_m(base + (r + c) * ldm, nt);
+                ar[c] = alpha[r + c];
             }
+        }
+        for (int q = 0; r + CH <= re; r += 4 * CH, q++) {
+            VT nx[CH];
+            double an[CH];
+            const bool more = r + 4 * CH + CH <= re;
+            if (more) {
+#pragma unroll
+                for (int c = 0; c < CH; c++) {
+                    nx[c] = ld_m(base + (r + 4 * CH + c) * ldm, nt);
+                    an[c] = alpha[r + 4 * CH + c];
+                }
+            }
+            double ak[CH];
+#pragma unroll
+            for (int c = 0; c < CH; c++) ak[c] = ar[c] * k;
             if (valid_cols == V)
                 col_chunk<TM, TA, V, CH, false>(sh, s, S, rm, mv, ak, k, CH, V);
             else
@@ -457,6 +461,13 @@ __device__ __forceinline__ void col_partial_tile(const TM* __restrict__ M, long 
             if (q % kFlushSteps == kFlushSteps - 1) {
 #pragma unroll
                 for (int v = 0; v < V; v++) flush(s[v], S[v]);
+            }
+            if (more) {
+#pragma unroll
+                for (int c = 0; c < CH; c++) {
+                    mv[c] = nx[c];
+                    ar[c] = an[c];
+                }
             }
         }
         if (r < re) {
@@ -466,7 +477,7 @@ __device__ __forceinline__ void col_partial_tile(const TM* __restrict__ M, long 
 #pragma unroll
             for (int c = 0; c < CH; c++) {
                 const long rr = (c < vr) ? r + c : r;
-                mv[c] = ld_stream(base + rr * ldm);
+                mv[c] = ld_m(base + rr * ldm, nt);
                 ak[c] = alpha[rr] * k;
             }
             col_chunk<TM, TA, V, CH, true>(sh, s, S, rm, mv, ak, k, vr, valid_cols);

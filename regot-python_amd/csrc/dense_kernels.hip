@@ -15,46 +15,30 @@ using namespace dev;
 // Row pass
 // ---------------------------------------------------------------------------
 
-// Each wave owns R consecutive rows (row_lse); with SPLIT (fewer rows than
-// CUs x 16 waves) the workgroup owns R rows and its 4 waves sweep column
-// quarters, so that the grid still fills the chip and each wave's sweep stays
-// short.  The epilogue writes, per row, the log2 sum, alpha = reg (log a -
-// ln2 L) and/or the row sum r_i = 2^(k alpha_i + L).  With FIN, per-workgroup
-// sums of (g^2, r, alpha a) are published for the combine kernel of the same
-// BCD iteration.
-template <typename TM, typename TA, int R, bool FIN, bool SPLIT>
+// Each wave owns R consecutive rows (row_lse; R = 1 for matrices too small to
+// fill the chip with 4-row waves).  The epilogue writes, per row, the log2 sum,
+// alpha = reg (log a - ln2 L) and/or the row sum r_i = 2^(k alpha_i + L).
+// With FIN, per-workgroup sums of (g^2, r, alpha a) are published for the
+// combine kernel of the same BCD iteration.  nt: nontemporal loads of M.
+template <typename TM, typename TA, int R, bool FIN>
 __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long ldm, int n,
                                                   int m, const double* __restrict__ beta,
                                                   double k, RowEpi epi, BcdState fin,
-                                                  const int* __restrict__ ctl)
+                                                  const int* __restrict__ ctl, int nt)
 {
     if (ctl != nullptr && ctl[0] != 0) return;
     const int lane = threadIdx.x & 63;
     // wave-uniform: row tests below become scalar branches, not exec masks
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long row0 = SPLIT ? static_cast<long>(blockIdx.x) * R
-                            : (static_cast<long>(blockIdx.x) * 4 + wave) * R;
+    const long row0 = (static_cast<long>(blockIdx.x) * 4 + wave) * R;
 
     const double* rowpot = epi.rowpot;
     double* aout = epi.alpha_out;
-    if (epi.parity) {
-        const int kk = ctl[1];
-        rowpot = epi.apair[kk & 1];
-        aout = epi.apair[(kk + 1) & 1];
-    }
     double wg2 = 0.0, wr = 0.0, waa = 0.0;
 
-    double L[R];
-    if (SPLIT) {
-        int c0, c1;
-        split_range<TM>(m, wave, &c0, &c1);
-        row_lse<TM, TA, R>(M, ldm, n, c0, c1, beta, k, epi.loga, rowpot, row0, lane, L);
-        merge_split_rows<R>(L, wave, lane);
-    } else if (row0 < n) {
-        row_lse<TM, TA, R>(M, ldm, n, 0, m, beta, k, epi.loga, rowpot, row0, lane, L);
-    }
-    // split: wave 0 finishes the workgroup's rows; otherwise every wave its own
-    if ((!SPLIT || wave == 0) && row0 < n) {
+    if (row0 < n) {
+        double L[R];
+        row_lse<TM, TA, R>(M, ldm, n, 0, m, beta, k, epi.loga, rowpot, row0, lane, L, nt != 0);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const long i = row0 + r;
@@ -90,12 +74,9 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
 #pragma unroll
             for (int q = 0; q < 3; q++)
                 fin.rpart[3 * blockIdx.x + q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
-            if (blockIdx.x == 0) {
-                fin.ctl[3] = ctl[1];
-                // beta_k[m-1] for the objective: the combine kernel of this
-                // iteration overwrites beta while other workgroups still decide
-                fin.bm[0] = beta[m - 1];
-            }
+            // beta_k[m-1] for the objective: the combine kernel of this
+            // iteration overwrites beta while other workgroups still decide
+            if (blockIdx.x == 0) fin.bm[0] = beta[m - 1];
         }
     }
 }
@@ -109,22 +90,20 @@ __global__ __launch_bounds__(256) void k_row_pass(const TM* __restrict__ M, long
 template <typename TM, typename TA>
 __global__ __launch_bounds__(256) void k_col_partial(
     const TM* __restrict__ M, long ldm, int n, int m, const double* __restrict__ alpha,
-    const double* __restrict__ ap0, const double* __restrict__ ap1,
     const double* __restrict__ colpot, const double* __restrict__ logb, double k, int rpb,
-    double* __restrict__ part, const int* __restrict__ ctl)
+    double* __restrict__ part, const int* __restrict__ ctl, int nt)
 {
     constexpr int TW = 64 * VecOf<TM>::N;
     if (ctl != nullptr && ctl[0] != 0) return;
-    if (ap0 != nullptr) alpha = (ctl[1] & 1) ? ap1 : ap0;
     const long rb = static_cast<long>(blockIdx.y) * rpb;
     const long re = (rb + rpb < n) ? rb + rpb : n;
     col_partial_tile<TM, TA>(M, ldm, m, alpha, colpot, logb, k, blockIdx.x * TW, rb, re,
-                             part + static_cast<long>(blockIdx.y) * m);
+                             part + static_cast<long>(blockIdx.y) * m, nt != 0);
 }
 
 constexpr int kCombineCols = 64;   // columns per combine workgroup (4 threads each)
 
-// Stop test of the on-device BCD loop for iterate k = ctl[3], computed
+// Stop test of the on-device BCD loop for iterate k = st.k, computed
 // identically (same data, same order) by every combine workgroup:
 // g_k = [r - a ; column residual of the update that produced beta_k],
 // f_k = reg sum r - (alpha_k + beta_{k,m-1})'a - (beta_k - beta_{k,m-1})'b over
@@ -155,7 +134,7 @@ __device__ void bcd_decide(const BcdState& st, int kk, double* f_out, double* g_
 
 // Combine the nblk partials of each column (4 threads per column, fixed
 // order), then the column epilogue.  With BCD, first take the stop decision
-// for iterate ctl[3]; when it stops, beta is left untouched (beta_k is the
+// for iterate st.k; when it stops, beta is left untouched (beta_k is the
 // result), otherwise beta <- beta_{k+1} and the tile sums of the new column
 // residual go to the other parity half of cpart.  Workgroup 0 records the
 // history entry and advances or stops the loop.
@@ -169,7 +148,7 @@ __global__ __launch_bounds__(256) void k_col_combine(const double* __restrict__ 
     bool done = false;
     if (BCD) {
         if (st.ctl[0] != 0) return;
-        kk = st.ctl[3];
+        kk = st.k;
         bcd_decide(st, kk, &f, &gnorm, &done);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             const int h = kk % st.hist_cap;
@@ -288,19 +267,23 @@ int grid_for(long count)
 template <typename TM>
 constexpr int kRowsPerWave = sizeof(TM) == 4 ? 4 : 2;
 
-// Whole-row waves while they give >= 1024 workgroups (4 per CU), else split rows.
-constexpr int kSplitBelow = 1024;
+// R-row waves while they give >= 1024 workgroups (4 per CU), else 1-row waves.
+constexpr int kWideBelow = 1024;
 
-bool row_pass_split(int bulk, int n)
+int rows_per_wave(int bulk, int n)
 {
     const int R = bulk == 0 ? kRowsPerWave<float> : kRowsPerWave<double>;
-    return ceil_div(n, 4 * R) < kSplitBelow;
+    return ceil_div(n, 4 * R) >= kWideBelow ? R : 1;
 }
 
-int row_pass_blocks(int bulk, int n)
+int row_pass_blocks(int bulk, int n) { return ceil_div(n, 4 * rows_per_wave(bulk, n)); }
+
+// Nontemporal loads above the size where M could stay in the 256 MB
+// Infinity Cache between passes (measured: cached loads are 10-15% faster
+// at 4096^2 and 8192^2 FP32, nontemporal ones 8% faster at 16384^2).
+bool stream_nt(int bulk, int n, long ldm)
 {
-    const int R = bulk == 0 ? kRowsPerWave<float> : kRowsPerWave<double>;
-    return row_pass_split(bulk, n) ? ceil_div(n, R) : ceil_div(n, 4 * R);
+    return static_cast<long>(n) * ldm * (bulk == 0 ? 4 : 8) > (256L << 20);
 }
 
 template <typename TM, typename TA, bool FIN>
@@ -310,12 +293,13 @@ void row_pass_launch(const void* M, long ldm, int n, int m, const double* beta, 
 {
     constexpr int R = kRowsPerWave<TM>;
     const dim3 grid(row_pass_blocks(bulk, n));
-    if (row_pass_split(bulk, n))
-        hipLaunchKernelGGL((k_row_pass<TM, TA, R, FIN, true>), grid, dim3(256), 0, s,
-                           static_cast<const TM*>(M), ldm, n, m, beta, k, epi, st, ctl);
+    const int nt = stream_nt(bulk, n, ldm) ? 1 : 0;
+    if (rows_per_wave(bulk, n) == R)
+        hipLaunchKernelGGL((k_row_pass<TM, TA, R, FIN>), grid, dim3(256), 0, s,
+                           static_cast<const TM*>(M), ldm, n, m, beta, k, epi, st, ctl, nt);
     else
-        hipLaunchKernelGGL((k_row_pass<TM, TA, R, FIN, false>), grid, dim3(256), 0, s,
-                           static_cast<const TM*>(M), ldm, n, m, beta, k, epi, st, ctl);
+        hipLaunchKernelGGL((k_row_pass<TM, TA, 1, FIN>), grid, dim3(256), 0, s,
+                           static_cast<const TM*>(M), ldm, n, m, beta, k, epi, st, ctl, nt);
     KERNEL_CHECK();
 }
 
@@ -358,7 +342,6 @@ int col_pass_blocks(int n, int m)
 int col_combine_blocks(int m) { return ceil_div(m, kCombineCols); }
 
 void launch_col_pass(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
-                     const double* alpha_pair0, const double* alpha_pair1,
                      const double* colpot_guess, const double* logb, double k, double* part,
                      int nblk, const ColEpi& epi, const BcdState* st, const int* ctl,
                      hipStream_t s, bool precise)
@@ -366,18 +349,19 @@ void launch_col_pass(int bulk, const void* M, long ldm, int n, int m, const doub
     if (m <= 0 || n <= 0) return;
     const int rpb = ceil_div(n, nblk);
     const dim3 grid(ceil_div(m, bulk == 0 ? 256 : 128), nblk);
+    const int nt = stream_nt(bulk, n, ldm) ? 1 : 0;
     if (bulk == 0 && !precise)
         hipLaunchKernelGGL((k_col_partial<float, float>), grid, dim3(256), 0, s,
-                           static_cast<const float*>(M), ldm, n, m, alpha, alpha_pair0,
-                           alpha_pair1, colpot_guess, logb, k, rpb, part, ctl);
+                           static_cast<const float*>(M), ldm, n, m, alpha, colpot_guess, logb,
+                           k, rpb, part, ctl, nt);
     else if (bulk == 0)
         hipLaunchKernelGGL((k_col_partial<float, double>), grid, dim3(256), 0, s,
-                           static_cast<const float*>(M), ldm, n, m, alpha, alpha_pair0,
-                           alpha_pair1, colpot_guess, logb, k, rpb, part, ctl);
+                           static_cast<const float*>(M), ldm, n, m, alpha, colpot_guess, logb,
+                           k, rpb, part, ctl, nt);
     else
         hipLaunchKernelGGL((k_col_partial<double, double>), grid, dim3(256), 0, s,
-                           static_cast<const double*>(M), ldm, n, m, alpha, alpha_pair0,
-                           alpha_pair1, colpot_guess, logb, k, rpb, part, ctl);
+                           static_cast<const double*>(M), ldm, n, m, alpha, colpot_guess, logb,
+                           k, rpb, part, ctl, nt);
     KERNEL_CHECK();
     if (st != nullptr)
         hipLaunchKernelGGL(k_col_combine<true>, dim3(col_combine_blocks(m)), dim3(256), 0, s,

@@ -230,8 +230,7 @@ void regot_problem::row_pass(const double* beta_full, const RowEpi& epi, const i
 }
 
 void regot_problem::col_pass(const double* alpha_v, const double* colpot_guess,
-                             const ColEpi& epi, const double* ap0, const double* ap1,
-                             const int* ctl_p, const BcdState* st)
+                             const ColEpi& epi, const int* ctl_p, const BcdState* st)
 {
     hipEvent_t stop;
     prof_begin(1, &stop);
@@ -239,8 +238,8 @@ void regot_problem::col_pass(const double* alpha_v, const double* colpot_guess,
     e.reg = reg;
     e.logb = logb;
     e.b = b;
-    launch_col_pass(bulk, dM, ldm, n, m, alpha_v, ap0, ap1, colpot_guess, logb, k, part, nblk, e,
-                    st, ctl_p, stream, precise && st == nullptr);
+    launch_col_pass(bulk, dM, ldm, n, m, alpha_v, colpot_guess, logb, k, part, nblk, e, st,
+                    ctl_p, stream, precise && st == nullptr);
     prof_end(1, stop);
 }
 

@@ -108,7 +108,6 @@ struct regot_problem {
     void row_pass(const double* beta_full, const regot::RowEpi& epi, const int* ctl_p = nullptr);
     // Column pass (partial + combine) at row potential alpha (n); see ColEpi.
     void col_pass(const double* alpha_v, const double* colpot_guess, const regot::ColEpi& epi,
-                  const double* ap0 = nullptr, const double* ap1 = nullptr,
                   const int* ctl_p = nullptr, const regot::BcdState* st = nullptr);
     // Row pass with the BCD bookkeeping fused (see BcdState).
     void row_pass_bcd(const double* beta_full, const regot::RowEpi& epi,

@@ -94,8 +94,6 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
 
     BcdState st{};
     st.a = p->a;
-    st.apair[0] = w.ap0;
-    st.apair[1] = w.ap1;
     st.beta = w.beta;
     st.ctl = w.ctl;
     st.hist_f = w.hf.d;
@@ -115,23 +113,26 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
     st.n = n;
     st.m = m;
 
+    // alpha_k lives in ap[k & 1]; iteration k writes alpha_{k+1} to the other
+    int k_enq = 0;
     auto enqueue_iteration = [&]() {
+        const int kk = k_enq++;
+        double* ak = (kk & 1) ? w.ap1.p : w.ap0.p;
+        double* ak1 = (kk & 1) ? w.ap0.p : w.ap1.p;
+        st.k = kk;
         RowEpi e;
         e.loga = p->loga;
-        e.parity = true;
-        e.apair[0] = w.ap0;
-        e.apair[1] = w.ap1;
+        e.rowpot = ak;
+        e.alpha_out = ak1;
         p->row_pass_bcd(w.beta, e, st);
         ColEpi c;
         c.beta_out = w.beta;
         c.resid = w.gcol;
         c.tilepart = w.cpart;
-        // The column pass runs before the stop decision advances k, so it must
-        // read alpha_{k+1} = apair[(k + 1) & 1]: hand the pair over swapped.
-        p->col_pass(nullptr, w.beta, c, w.ap1, w.ap0, w.ctl, &st);
+        p->col_pass(ak1, w.beta, c, w.ctl, &st);
     };
     auto launch_batch = [&](int slot) {
-        for (int it = 0; it < K; it++) enqueue_iteration();
+        for (int it = 0; it < K && k_enq <= max_iter; it++) enqueue_iteration();
         HIP_CHECK(hipMemcpyAsync(w.hc.p + 4 * slot, w.ctl.p, 4 * sizeof(int),
                                  hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipEventRecord(w.ev[slot], s));
