@@ -404,7 +404,7 @@ struct regot_batch {
         }
         if (ncol > 0) {
             hipLaunchKernelGGL(k_batch_colpart<TM>, dim3(ncol), dim3(256), 0, stream, Mp,
-                               d_desc.p, t_col.p, v, k, init, ntf);
+                               d_desc.p, t_col.p, v, k, init, 1);
             KERNEL_CHECK();
         }
         if (ncomb > 0) {

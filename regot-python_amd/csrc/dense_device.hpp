@@ -429,31 +429,14 @@ __device__ __forceinline__ void col_partial_tile(const TM* __restrict__ M, long 
     if (valid_cols > 0) {
         const TM* base = M + j0;
         long r = rb + wave * CH;
-        // chunks of CH rows; the next chunk's rows of M and alpha are loaded
-        // before the current chunk is consumed
-        VT mv[CH];
-        double ar[CH];
-        if (r + CH <= re) {
+        for (int q = 0; r + CH <= re; r += 4 * CH, q++) {
+            VT mv[CH];
+            double ak[CH];
 #pragma unroll
             for (int c = 0; c < CH; c++) {
                 mv[c] = ld_m(base + (r + c) * ldm, nt);
-                ar[c] = alpha[r + c];
+                ak[c] = alpha[r + c] * k;
             }
-        }
-        for (int q = 0; r + CH <= re; r += 4 * CH, q++) {
-            VT nx[CH];
-            double an[CH];
-            const bool more = r + 4 * CH + CH <= re;
-            if (more) {
-#pragma unroll
-                for (int c = 0; c < CH; c++) {
-                    nx[c] = ld_m(base + (r + 4 * CH + c) * ldm, nt);
-                    an[c] = alpha[r + 4 * CH + c];
-                }
-            }
-            double ak[CH];
-#pragma unroll
-            for (int c = 0; c < CH; c++) ak[c] = ar[c] * k;
             if (valid_cols == V)
                 col_chunk<TM, TA, V, CH, false>(sh, s, S, rm, mv, ak, k, CH, V);
             else
@@ -461,13 +444,6 @@ __device__ __forceinline__ void col_partial_tile(const TM* __restrict__ M, long 
             if (q % kFlushSteps == kFlushSteps - 1) {
 #pragma unroll
                 for (int v = 0; v < V; v++) flush(s[v], S[v]);
-            }
-            if (more) {
-#pragma unroll
-                for (int c = 0; c < CH; c++) {
-                    mv[c] = nx[c];
-                    ar[c] = an[c];
-                }
             }
         }
         if (r < re) {
