@@ -33,6 +33,21 @@ for p in (ROOT, os.path.join(ROOT, "regot-python_amd")):
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of the dominant pass at n = m = 16384 FP32, from the
+    committed rocprofv3 PMC measurement (FETCH_SIZE and WRITE_SIZE runs scaled
+    by a copy kernel of known size); None for other sizes."""
+    path = os.path.join(ROOT, "profiles", "bcd_n16384_fp32_pmc_traffic.json")
+    if n != 16384 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f)["kernels"]
+    if kernel == "row_pass":
+        return k["k_row_pass<float, float, 4, true>"]["traffic_bytes"]
+    return (k["k_col_partial<float, float>"]["traffic_bytes"] +
+            k["k_col_combine<true>"]["traffic_bytes"])
+
+
 def make_problem(n, m, d, seed):
     rng = np.random.default_rng(seed)
     X = rng.standard_normal((n, d))
@@ -188,6 +203,8 @@ def main():
     ap.add_argument("--diff-n", type=int, default=4096,
                     help="points per cloud of the differentiable workload (0 = skip)")
     ap.add_argument("--diff-steps", type=int, default=2)
+    ap.add_argument("--only", choices=["batched", "differentiable"], default=None,
+                    help="run one side workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); the multi-GPU "
@@ -196,6 +213,15 @@ def main():
 
     import regot
     from regot import _lib
+
+    if args.only == "batched":
+        print(json.dumps({"batched": bench_batched(args.batch_problems, args.batch_steps, 1e-6,
+                                                   sequential=False)}))
+        return
+    if args.only == "differentiable":
+        print(json.dumps({"differentiable": bench_differentiable(args.diff_n, 64, 0.05, 1e-6,
+                                                                 args.diff_steps)}))
+        return
 
     n = m = args.n
     print("bench: building problem n=m=%d" % n, file=sys.stderr, flush=True)
@@ -278,7 +304,9 @@ def main():
                    "mar_err_inloop": res.mar_errs[-1], "mar_err_fp64": res_p.mar_err_fp64},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_ms": dom_ms,
+                     "traffic": measured_traffic(dom_name, n),
+                     "traffic_source": "profiles/bcd_n16384_fp32_pmc_traffic.md",
+                     "avg_ms": dom_ms,
                      "row_pass_avg_ms": avg_row, "col_pass_avg_ms": avg_col,
                      "bytes_per_launch": bytes_per_pass},
         "cpu_baseline": cpu,
