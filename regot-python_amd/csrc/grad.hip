@@ -205,7 +205,8 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
         double err = 0.0;
         // For the entropic loss H gamma = rhs: the dual itself is the warm start
         // (the CG then only confirms the residual); the sharp loss starts at 0.
-        const int it = hess_cg(p, rhs, 0.0, tol, maxit, w, &err, loss == 0 ? p->gamma.p : nullptr);
+        const int it = hess_cg(p, rhs, 0.0, tol, maxit, w, &err, loss == 0 ? p->gamma.p : nullptr,
+                               true);
         hipLaunchKernelGGL(k_pq, dim3(ceil_div(std::max(n, m), 256)), dim3(256), 0, s, w.p,
                            alpha.p, beta.p, n, m, p->reg, loss == 0 ? 1.0 : 0.0, P.p, Q.p);
         KERNEL_CHECK();

@@ -39,8 +39,11 @@ __global__ void k_sub_div(const double* w, const double* y, const double* h1, do
 }
 
 // ---- CG on Delta, device-controlled --------------------------------------
-// ctl: [0] done, [1] iterations.  scal: [0] rr (current), [1] threshold,
-// [2] rr (next, published by the direction kernel), [3] rhs norm^2.
+// Conjugate gradients, optionally with the Jacobi preconditioner z = D^-1 r
+// (dinv = 1 / diag(Delta); null: plain CG, z = r).
+// ctl: [0] done, [1] iterations.  scal: [0] rho = r.z (current), [1] stop
+// threshold on r.r, [2] rho (next, published by the direction kernel),
+// [3] rhs norm^2, [4] r.r of the last update.
 struct CgArgs {
     const int* rowptr;
     const int* colind;
@@ -62,7 +65,69 @@ struct CgArgs {
     int* ctl;
     int max_iter;
     int nbc;        // column workgroups (4 columns each)
+    int nbu;        // update workgroups (1024 entries each)
+    const double* dinv;   // Jacobi preconditioner, or null
 };
+
+// dinv_j = 1 / (h2_j + shift - sum_i sigma_ij^2 / (h1_i + shift)), the diagonal
+// of Delta (>= shift since sigma_ij <= h1_i and sum_i sigma_ij <= h2_j); one
+// wave per column.
+__global__ __launch_bounds__(256) void k_delta_diag(const int* __restrict__ colptr,
+                                                    const int* __restrict__ rowind,
+                                                    const double* __restrict__ cval,
+                                                    const double* __restrict__ h1,
+                                                    const double* __restrict__ h2, double shift,
+                                                    int m1, double* __restrict__ dinv)
+{
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= m1) return;
+    double s = 0.0;
+    for (int k = colptr[j] + lane; k < colptr[j + 1]; k += 64) {
+        const double v = cval[k];
+        s += v * v / (h1[rowind[k]] + shift);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) {
+        double d = h2[j] + shift - s;
+        const double floor = 1e-12 * (h2[j] + shift);
+        if (!(d > floor)) d = (h2[j] + shift > 0.0) ? h2[j] + shift : 1.0;
+        dinv[j] = 1.0 / d;
+    }
+}
+
+// z = dinv .* r (or r), p = z, partials of r.z and r.r per 1024 entries.
+__global__ __launch_bounds__(256) void k_pcg_start(const double* __restrict__ r,
+                                                   const double* __restrict__ dinv, int m1,
+                                                   double* __restrict__ p,
+                                                   double* __restrict__ part)
+{
+    __shared__ double sm[2][256];
+    double rz = 0.0, rr = 0.0;
+    for (int j = blockIdx.x * 1024 + threadIdx.x; j < (blockIdx.x + 1) * 1024 && j < m1;
+         j += 256) {
+        const double rj = r[j];
+        const double zj = dinv ? rj * dinv[j] : rj;
+        p[j] = zj;
+        rz += rj * zj;
+        rr += rj * rj;
+    }
+    sm[0][threadIdx.x] = rz;
+    sm[1][threadIdx.x] = rr;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sm[0][0];
+        part[2 * blockIdx.x + 1] = sm[1][0];
+    }
+}
 
 // y = (sigma p) / (h1 + shift); workgroup 0 also moves the next rr into place.
 __global__ __launch_bounds__(256) void k_cg_rows(CgArgs a)
@@ -112,62 +177,79 @@ __global__ __launch_bounds__(256) void k_cg_cols(CgArgs a)
 }
 
 // Every workgroup reduces the p.q partials (same order, same value), then
-// x += alpha p, r -= alpha q over its 1024 entries; partial r.r.
+// x += alpha p, r -= alpha q over its 1024 entries; partials of r.z and r.r.
 __global__ __launch_bounds__(256) void k_cg_update(CgArgs a)
 {
-    __shared__ double sm[256];
+    __shared__ double sm[2][256];
     if (a.ctl[0]) return;
     double v = 0.0;
     for (int k = threadIdx.x; k < a.nbc; k += 256) v += a.part[k];
-    sm[threadIdx.x] = v;
+    sm[0][threadIdx.x] = v;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        if (threadIdx.x < o) sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
         __syncthreads();
     }
-    const double pq = sm[0];
+    const double pq = sm[0][0];
     const double alpha = a.scal[0] / pq;
     __syncthreads();
-    double rr = 0.0;
+    double rr = 0.0, rz = 0.0;
     for (int j = blockIdx.x * 1024 + threadIdx.x; j < (blockIdx.x + 1) * 1024 && j < a.m1; j += 256) {
         a.x[j] += alpha * a.p[j];
         const double rj = a.r[j] - alpha * a.q[j];
         a.r[j] = rj;
         rr += rj * rj;
+        rz += a.dinv ? rj * rj * a.dinv[j] : 0.0;
     }
-    sm[threadIdx.x] = rr;
+    sm[0][threadIdx.x] = rr;
+    sm[1][threadIdx.x] = rz;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        if (threadIdx.x < o) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + o];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) a.part[a.nbc + blockIdx.x] = sm[0];
+    if (threadIdx.x == 0) {
+        a.part[a.nbc + 2 * blockIdx.x] = sm[0][0];
+        a.part[a.nbc + 2 * blockIdx.x + 1] = sm[1][0];
+    }
 }
 
-// Every workgroup reduces r.r; stop (rr < threshold, or the iteration cap
-// after this update), else p = r + (rr_new / rr) p.
-__global__ __launch_bounds__(256) void k_cg_dir(CgArgs a, int nbu)
+// Every workgroup reduces r.r and r.z; stop (rr < threshold, or the
+// iteration cap after this update), else p = z + (rho_new / rho) p.
+__global__ __launch_bounds__(256) void k_cg_dir(CgArgs a)
 {
-    __shared__ double sm[256];
+    __shared__ double sm[2][256];
     if (a.ctl[0]) return;
-    double v = 0.0;
-    for (int k = threadIdx.x; k < nbu; k += 256) v += a.part[a.nbc + k];
-    sm[threadIdx.x] = v;
+    double v = 0.0, w = 0.0;
+    for (int k = threadIdx.x; k < a.nbu; k += 256) {
+        v += a.part[a.nbc + 2 * k];
+        w += a.part[a.nbc + 2 * k + 1];
+    }
+    sm[0][threadIdx.x] = v;
+    sm[1][threadIdx.x] = w;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        if (threadIdx.x < o) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + o];
+        }
         __syncthreads();
     }
-    const double rr_new = sm[0];
+    const double rr_new = sm[0][0];
+    const double rho_new = a.dinv ? sm[1][0] : rr_new;
     const bool conv = rr_new < a.scal[1];
     if (!conv) {
-        const double beta = rr_new / a.scal[0];
+        const double beta = rho_new / a.scal[0];
         for (int j = blockIdx.x * 1024 + threadIdx.x; j < (blockIdx.x + 1) * 1024 && j < a.m1;
              j += 256)
-            a.p[j] = a.r[j] + beta * a.p[j];
+            a.p[j] = (a.dinv ? a.r[j] * a.dinv[j] : a.r[j]) + beta * a.p[j];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.scal[2] = rr_new;
+        a.scal[2] = rho_new;
+        a.scal[4] = rr_new;
         if (conv) {
             a.ctl[0] = 1;
         } else {
@@ -353,7 +435,7 @@ void hess_apply_hs(regot_problem* p, const double* x, double* out)
 // stopping rule ||r||^2 < max(tol^2 ||b||^2, DBL_MIN) and the iteration cap
 // counted as Eigen's ConjugateGradient counts it (src/sinkhorn_cg.cpp:29-38).
 int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
-                  double* x, double* tol_error, const double* x0)
+                  double* x, double* tol_error, const double* x0, bool jacobi)
 {
     SparseHessian& H = hess_of(p);
     const int n = p->n, m1 = p->m - 1;
@@ -369,13 +451,20 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     H.cg_y_buf.ensure(n);
     const int nbc = ceil_div(m1, 4);
     const int nbu = ceil_div(m1, 1024);
-    H.cg_part.ensure(nbc + nbu);
+    H.cg_part.ensure(nbc + 2L * nbu);
     H.cg_scal.ensure(8);
+    if (jacobi) {
+        H.cg_dinv.ensure(m1);
+        hipLaunchKernelGGL(k_delta_diag, dim3(ceil_div(m1, 4)), dim3(256), 0, s, H.colptr.p,
+                           H.rowind.p, H.cval.p, H.h1.p, H.h2.p, shift, m1, H.cg_dinv.p);
+        KERNEL_CHECK();
+    }
+    const double* dinv = jacobi ? H.cg_dinv.p : nullptr;
     H.cg_ctl.ensure(4);
     H.cg_hctl.ensure(16);
     H.hbuf.ensure(8);
 
-    // x = x0 (or 0), r = p = rhs - Delta x
+    // x = x0 (or 0), r = rhs - Delta x, p = z = D^-1 r (or r)
     if (x0) {
         vec_copy(x, x0, m1, s);
         hess_apply_delta(p, x0, shift, H.cg_q);
@@ -384,14 +473,23 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
         launch_fill(x, 0.0, m1, s);
         vec_copy(H.cg_r, rhs, m1, s);
     }
-    vec_copy(H.cg_p, H.cg_r, m1, s);
-    const double* xs[2] = {rhs, H.cg_r.p};
-    vec_dots(xs, xs, m1, 2, H.cg_scal.p + 3, s);
-    HIP_CHECK(hipMemcpyAsync(H.hbuf.p, H.cg_scal.p + 3, 2 * sizeof(double),
+    hipLaunchKernelGGL(k_pcg_start, dim3(nbu), dim3(256), 0, s, H.cg_r.p, dinv, m1, H.cg_p.p,
+                       H.cg_part.p);
+    KERNEL_CHECK();
+    const double* xs[1] = {rhs};
+    vec_dots(xs, xs, m1, 1, H.cg_scal.p + 3, s);
+    std::vector<double> hpart(2L * nbu);
+    HIP_CHECK(hipMemcpyAsync(hpart.data(), H.cg_part.p, 2L * nbu * sizeof(double),
                              hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(H.hbuf.p, H.cg_scal.p + 3, sizeof(double), hipMemcpyDeviceToHost,
+                             s));
     HIP_CHECK(hipStreamSynchronize(s));
     const double rhs_norm2 = H.hbuf.p[0];
-    const double rr0 = H.hbuf.p[1];
+    double rz0 = 0.0, rr0 = 0.0;
+    for (int b = 0; b < nbu; b++) {
+        rz0 += hpart[2 * b];
+        rr0 += hpart[2 * b + 1];
+    }
     if (rhs_norm2 == 0.0) {
         if (x0) launch_fill(x, 0.0, m1, s);
         if (tol_error) *tol_error = 0.0;
@@ -404,8 +502,8 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
         if (tol_error) *tol_error = std::sqrt(rr0 / rhs_norm2);
         return 0;
     }
-    double hs[4] = {rr0, thr, rr0, rhs_norm2};
-    HIP_CHECK(hipMemcpyAsync(H.cg_scal.p, hs, 4 * sizeof(double), hipMemcpyHostToDevice, s));
+    double hs[5] = {rz0, thr, rz0, rhs_norm2, rr0};
+    HIP_CHECK(hipMemcpyAsync(H.cg_scal.p, hs, 5 * sizeof(double), hipMemcpyHostToDevice, s));
     int hc[4] = {0, 0, 0, 0};
     HIP_CHECK(hipMemcpyAsync(H.cg_ctl.p, hc, 4 * sizeof(int), hipMemcpyHostToDevice, s));
     if (max_iter <= 0) {
@@ -436,6 +534,8 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     a.ctl = H.cg_ctl;
     a.max_iter = max_iter;
     a.nbc = nbc;
+    a.nbu = nbu;
+    a.dinv = dinv;
 
     const int K = 16;
     hipEvent_t ev[2];
@@ -446,7 +546,7 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
             hipLaunchKernelGGL(k_cg_rows, dim3(ceil_div(n, 4)), dim3(256), 0, s, a);
             hipLaunchKernelGGL(k_cg_cols, dim3(nbc), dim3(256), 0, s, a);
             hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, a);
-            hipLaunchKernelGGL(k_cg_dir, dim3(nbu), dim3(256), 0, s, a, nbu);
+            hipLaunchKernelGGL(k_cg_dir, dim3(nbu), dim3(256), 0, s, a);
         }
         KERNEL_CHECK();
         HIP_CHECK(hipMemcpyAsync(H.cg_hctl.p + 4 * slot, H.cg_ctl.p, 4 * sizeof(int),
@@ -480,16 +580,16 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     HIP_CHECK(hipEventDestroy(ev[0]));
     HIP_CHECK(hipEventDestroy(ev[1]));
     HIP_CHECK(hipMemcpy(H.cg_hctl.p + 8, H.cg_ctl.p, 4 * sizeof(int), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(H.hbuf.p, H.cg_scal.p, 4 * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(H.hbuf.p, H.cg_scal.p, 5 * sizeof(double), hipMemcpyDeviceToHost));
     iters = H.cg_hctl.p[9];
-    if (tol_error) *tol_error = std::sqrt(H.hbuf.p[2] / rhs_norm2);
+    if (tol_error) *tol_error = std::sqrt(H.hbuf.p[4] / rhs_norm2);
     return iters;
 }
 
 // (Hs + shift I)^{-1} rhs by block elimination (src/sinkhorn_cg.cpp:9-50):
 // v = w / (h1 + s); y = sigma' v; solve Delta r2 = z - y; r1 = (w - sigma r2) / (h1 + s).
 int hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
-            double* out, double* tol_error, const double* x0)
+            double* out, double* tol_error, const double* x0, bool jacobi)
 {
     SparseHessian& H = hess_of(p);
     const int n = p->n, m1 = p->m - 1;
@@ -504,7 +604,7 @@ int hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int m
     launch_spmv_cols(H, H.cg_y_buf, nullptr, 0.0, 0, H.vm2, s);
     vec_axpby(H.vm2, -1.0, H.vm2, 1.0, rhs + n, m1, s);
     const int it = hess_cg_delta(p, H.vm2, shift, tol, max_iter, out + n, tol_error,
-                                 x0 ? x0 + n : nullptr);
+                                 x0 ? x0 + n : nullptr, jacobi);
     launch_spmv_rows(H, out + n, H.cg_y_buf, false, 0.0, s);
     hipLaunchKernelGGL(k_sub_div, dim3(ceil_div(n, 256)), dim3(256), 0, s, rhs, H.cg_y_buf.p,
                        H.h1.p, shift, n, out);

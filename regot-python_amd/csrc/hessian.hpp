@@ -16,12 +16,14 @@ void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, do
 void hess_apply_delta(regot_problem* p, const double* x, double shift, double* out);
 void hess_apply_hs(regot_problem* p, const double* x, double* out);
 // CG on the Schur complement Delta (m-1) from x0 (device, m-1) or zero;
-// relative residual tol against rhs; returns iterations.
+// relative residual tol against rhs; jacobi: preconditioned by diag(Delta);
+// returns iterations.
 int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
-                  double* x, double* tol_error, const double* x0 = nullptr);
+                  double* x, double* tol_error, const double* x0 = nullptr,
+                  bool jacobi = false);
 // (Hs + shift I)^{-1} rhs (n+m-1) by block elimination; x0 (device, n+m-1,
 // only its last m-1 entries are used) warm-starts the CG; returns CG iterations.
 int hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
-            double* out, double* tol_error, const double* x0 = nullptr);
+            double* out, double* tol_error, const double* x0 = nullptr, bool jacobi = false);
 
 }  // namespace regot

@@ -142,17 +142,18 @@ struct Work {
     }
 
     // (Hs + shift I)^{-1} rhs.  method 0 is the reference's CG (relative
-    // residual 1e-8, Eigen's default cap of 2 * HessianCG::cols() = 2m).  The
-    // reference's default method (SimplicialLDLT) is an exact solve; CG stands
-    // in for it on the GPU with a 1e-10 residual and a cap high enough that the
-    // tolerance, not the cap, ends it on ill-conditioned Schur complements.
+    // residual 1e-8, Eigen's default cap of 2 * HessianCG::cols() = 2m, no
+    // preconditioner).  The reference's default method (SimplicialLDLT) is an
+    // exact solve; a Jacobi-preconditioned CG stands in for it on the GPU with a
+    // 1e-10 residual and a cap high enough that the tolerance, not the cap,
+    // ends it on ill-conditioned Schur complements.
     void lin_solve(const regot_opts& o, const double* r, double shift, double* out)
     {
         const double tol = o.cg_tol > 0 ? o.cg_tol : (o.method == 0 ? 1e-8 : 1e-10);
         const int maxit = o.cg_max_iter > 0 ? o.cg_max_iter
                                             : (o.method == 0 ? 2 * m : 10 * m + 1000);
         const auto t0 = Clock::now();
-        cg_iters += hess_cg(p, r, shift, tol, maxit, out, nullptr);
+        cg_iters += hess_cg(p, r, shift, tol, maxit, out, nullptr, nullptr, o.method != 0);
         phase_ms[2] += ms_since(t0);
         phase_n[2] += 1;
     }

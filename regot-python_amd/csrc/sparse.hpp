@@ -54,7 +54,7 @@ struct SparseHessian {
     DBuf<unsigned int> tiebits;
     DBuf<unsigned long long> hist;   // global count histograms of the top-k radix passes
     DBuf<double> vn, vm, vm2;        // alpha, beta (full) of the current build; scratch
-    DBuf<double> cg_r, cg_p, cg_q, cg_y_buf, cg_part, cg_scal;
+    DBuf<double> cg_r, cg_p, cg_q, cg_y_buf, cg_part, cg_scal, cg_dinv;
     DBuf<int> cg_ctl;
     HBuf<int> cg_hctl;
     HBuf<double> hbuf;
