@@ -10,6 +10,8 @@ Prints one JSON line.  `value` is seconds to reach the tolerance (lower is
 better).  `batched` reports the batched workload (512 problems of uneven
 n, m ~ U{64..4096} in one batched BCD solve to mar_err <= 1e-6, solves/s)
 beside the same problems solved one after another by the single-problem path.
+`newton` times SSNS (same recipe, reg 0.01), sparse Newton and SPLR (mixture
+clouds in R^32, reg 0.001, density 1%) at the same n = m, to the same tol.
 `differentiable` reports the differentiable loss between two BF16 clouds of
 4096 points in R^64 (reg 0.05, tol 1e-6): matrix-core cost kernel, forward
 (cost + solve + implicit adjoint) and backward, beside the plain solve.  `roofline` is the dominant streaming kernel's achieved HBM rate from
@@ -186,6 +188,56 @@ def bench_differentiable(n, d, reg, tol, steps):
     return res
 
 
+def c3_problem(n, seed=0):
+    """Mixture-vs-Gaussian clouds in R^32: X ~ 0.5 N(-1, I) + 0.5 N(+1, I),
+    Y ~ N(0, 1.5^2 I), M = sqeuclid / max in FP32, a, b = (1 + 0.2 U(-1, 1))
+    normalised."""
+    rng = np.random.default_rng(seed)
+    lab = rng.random(n) < 0.5
+    X = rng.standard_normal((n, 32)) + np.where(lab, 1.0, -1.0)[:, None]
+    Y = 1.5 * rng.standard_normal((n, 32))
+    M = X @ Y.T
+    M *= -2.0
+    M += (X * X).sum(1)[:, None]
+    M += (Y * Y).sum(1)[None, :]
+    np.maximum(M, 0.0, out=M)
+    M /= M.max()
+    M32 = M.astype(np.float32)
+    del M
+    a = 1 + 0.2 * rng.uniform(-1, 1, n)
+    b = 1 + 0.2 * rng.uniform(-1, 1, n)
+    return M32, a / a.sum(), b / b.sum()
+
+
+def bench_newton(n, tol):
+    """The Newton-type drivers at n = m: SSNS on the BCD workload's recipe
+    (reg 0.01), sparse Newton and SPLR on the mixture recipe (reg 0.001,
+    density 1%); one warmup, one timed solve each, with per-phase times."""
+    import regot
+    from regot import _lib
+    out = {}
+    M32, a, b = make_problem(n, n, 16, 0)
+    cases = [("ssns", M32, a, b, 0.01, _lib.SOLVER_SSNS, {})]
+    del M32
+    M3, a3, b3 = c3_problem(n)
+    cases += [("sparse_newton", M3, a3, b3, 0.001, _lib.SOLVER_SPARSE_NEWTON, {"density": 0.01}),
+              ("splr", M3, a3, b3, 0.001, _lib.SOLVER_SPLR, {"density": 0.01})]
+    for name, M, a_, b_, reg, sid, kw in cases:
+        with regot.Problem(M, a_, b_, reg) as P:
+            P.solve(sid, tol=tol, max_iter=1000, compute_plan=False, **kw)
+            t0 = time.perf_counter()
+            r = P.solve(sid, tol=tol, max_iter=1000, compute_plan=False, verify=True, **kw)
+            dt = time.perf_counter() - t0
+        out[name] = {"reg": reg, "tol": tol, **kw, "s": dt, "niter": r.niter,
+                     "mar_err": r.mar_errs[-1], "mar_err_fp64": r.mar_err_fp64,
+                     "density_last": r.densities[-1] if r.densities else None,
+                     **{k: (float(v) if isinstance(v, (float, np.floating)) else v)
+                        for k, v in r.stats.items()}}
+        print("bench: %s %.3f s, %d iterations" % (name, dt, r.niter), file=sys.stderr,
+              flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,7 +255,9 @@ def main():
     ap.add_argument("--diff-n", type=int, default=4096,
                     help="points per cloud of the differentiable workload (0 = skip)")
     ap.add_argument("--diff-steps", type=int, default=2)
-    ap.add_argument("--only", choices=["batched", "differentiable"], default=None,
+    ap.add_argument("--newton", type=int, default=1,
+                    help="time SSNS, sparse Newton and SPLR at n = m = --n (0 = skip)")
+    ap.add_argument("--only", choices=["batched", "differentiable", "newton"], default=None,
                     help="run one side workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -217,6 +271,9 @@ def main():
     if args.only == "batched":
         print(json.dumps({"batched": bench_batched(args.batch_problems, args.batch_steps, 1e-6,
                                                    sequential=False)}))
+        return
+    if args.only == "newton":
+        print(json.dumps({"newton": bench_newton(args.n, args.tol)}))
         return
     if args.only == "differentiable":
         print(json.dumps({"differentiable": bench_differentiable(args.diff_n, 64, 0.05, 1e-6,
@@ -270,6 +327,11 @@ def main():
                          "(%.3f s/iter), scaled to the GPU's %d iterations"
                          % (args.cpu_iters, n, per_iter, niter)}
 
+    newton = None
+    if args.newton:
+        del P
+        newton = bench_newton(n, args.tol)
+
     batched = None
     if args.batch_problems > 0:
         print("bench: batched workload, %d problems" % args.batch_problems, file=sys.stderr,
@@ -310,6 +372,7 @@ def main():
                      "row_pass_avg_ms": avg_row, "col_pass_avg_ms": avg_col,
                      "bytes_per_launch": bytes_per_pass},
         "cpu_baseline": cpu,
+        "newton": newton,
         "batched": batched,
         "differentiable": diff,
     }
