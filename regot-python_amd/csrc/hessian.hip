@@ -129,19 +129,23 @@ __global__ __launch_bounds__(256) void k_pcg_start(const double* __restrict__ r,
     }
 }
 
-// y = (sigma p) / (h1 + shift); workgroup 0 also moves the next rr into place.
+// y = (sigma p) / (h1 + shift), one wave per row, the waves of the grid
+// striding over the rows (the grid is one resident round); workgroup 0 also
+// moves the next rho into place.
 __global__ __launch_bounds__(256) void k_cg_rows(CgArgs a)
 {
     if (a.ctl[0]) return;
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.ctl[1] > 0) a.scal[0] = a.scal[2];
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= a.n) return;
-    double s = 0.0;
-    for (int k = a.rowptr[i] + lane; k < a.rowptr[i + 1]; k += 64) s += a.rval[k] * a.p[a.colind[k]];
+    const int nw = gridDim.x * 4;
+    for (int i = blockIdx.x * 4 + (threadIdx.x >> 6); i < a.n; i += nw) {
+        double s = 0.0;
+        for (int k = a.rowptr[i] + lane; k < a.rowptr[i + 1]; k += 64)
+            s += a.rval[k] * a.p[a.colind[k]];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) a.y[i] = s / (a.h1[i] + a.shift);
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) a.y[i] = s / (a.h1[i] + a.shift);
+    }
 }
 
 __device__ __forceinline__ double wg_sum4(double v, double* sm)
@@ -155,22 +159,24 @@ __device__ __forceinline__ double wg_sum4(double v, double* sm)
     return r;
 }
 
-// q = (h2 + shift) p - sigma' y; partial p.q per workgroup.
+// q = (h2 + shift) p - sigma' y, one wave per column, the waves of the grid
+// striding over the columns; partial p.q per workgroup (fixed order).
 __global__ __launch_bounds__(256) void k_cg_cols(CgArgs a)
 {
     __shared__ double sm[4];
     if (a.ctl[0]) return;
     const int lane = threadIdx.x & 63;
-    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nw = gridDim.x * 4;
     double pq = 0.0;
-    if (j < a.m1) {
+    for (int j = blockIdx.x * 4 + (threadIdx.x >> 6); j < a.m1; j += nw) {
         double s = 0.0;
-        for (int k = a.colptr[j] + lane; k < a.colptr[j + 1]; k += 64) s += a.cval[k] * a.y[a.rowind[k]];
+        for (int k = a.colptr[j] + lane; k < a.colptr[j + 1]; k += 64)
+            s += a.cval[k] * a.y[a.rowind[k]];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
         const double qj = a.h2[j] * a.p[j] + a.shift * a.p[j] - s;
         if (lane == 0) a.q[j] = qj;
-        pq = a.p[j] * qj;
+        pq += a.p[j] * qj;
     }
     pq = wg_sum4(pq, sm);
     if (threadIdx.x == 0) a.part[blockIdx.x] = pq;
@@ -294,7 +300,8 @@ static ElemCtx make_ctx(regot_problem* p, SparseHessian& H)
     return c;
 }
 
-// Shared tail of both rules: counts -> scans -> CSR and CSC writes.
+// Shared tail of both rules: row counts -> scan -> CSR write -> CSC by a
+// counting sort of the CSR.
 static void build_storage(regot_problem* p, SparseHessian& H, const ElemCtx& c,
                           const KeepRule& rule, bool row_counts_ready)
 {
@@ -315,12 +322,15 @@ static void build_storage(regot_problem* p, SparseHessian& H, const ElemCtx& c,
     H.rowind.ensure(nnz);
     H.cval.ensure(nnz);
     launch_row_write(p->bulk, p->dM, p->ldm, c, rule, H.rowptr, H.colind, H.rval, s);
-    hess_ensure_transpose(p);
+    // CSC: the transpose of the CSR just written (a counting sort of nnz
+    // entries instead of two more passes over M)
+    // at most 512 row blocks: the offset table stays <= 2 KiB per column
+    const int rows_per_block = std::max(32, ceil_div(n, 512));
     H.ccnt.ensure(m);
     H.colptr.ensure(m);
-    launch_col_count(p->bulk, p->dMT, p->ldt, c, rule, H.ccnt, s);
-    launch_scan(H.ccnt, m - 1, H.colptr, H.ltmp.p + 1, s);
-    launch_col_write(p->bulk, p->dMT, p->ldt, c, rule, H.colptr, H.rowind, H.cval, s);
+    H.csc_work.ensure(static_cast<size_t>(ceil_div(n, rows_per_block)) * std::max(m - 1, 1));
+    launch_csr_to_csc(H, H.csc_work, rows_per_block, H.ccnt, H.colptr, H.ltmp.p + 1, H.rowind,
+                      H.cval, s);
 }
 
 static void prepare(regot_problem* p, SparseHessian& H, const double* d_gamma, const double* d_grad)
@@ -397,10 +407,6 @@ void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, do
     H.ltmp.ensure(n + 4);
     launch_scan64(H.rcnt, n, H.ltmp.p + 4, s);
     rule.tie_off = H.ltmp.p + 4;
-    rule.tie_words = ceil_div(std::max(m - 1, 1), 32);
-    H.tiebits.ensure(static_cast<size_t>(n) * rule.tie_words);
-    HIP_CHECK(hipMemsetAsync(H.tiebits.p, 0, sizeof(unsigned int) * n * rule.tie_words, s));
-    rule.tie_bits = H.tiebits;
     H.nnz_zero = 0;
     build_storage(p, H, c, rule, false);
     HIP_CHECK(hipMemcpyAsync(H.hll.p, H.ltmp.p, 2 * sizeof(long long), hipMemcpyDeviceToHost, s));
@@ -449,7 +455,9 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     H.cg_q.ensure(m1);
     H.vn.ensure(n);
     H.cg_y_buf.ensure(n);
-    const int nbc = ceil_div(m1, 4);
+    // one resident round of workgroups for the SpMV kernels (8 per CU)
+    const int nbr = std::min(ceil_div(n, 4), 2048);
+    const int nbc = std::min(ceil_div(m1, 4), 2048);
     const int nbu = ceil_div(m1, 1024);
     H.cg_part.ensure(nbc + 2L * nbu);
     H.cg_scal.ensure(8);
@@ -543,7 +551,7 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     auto launch_batch = [&](int slot) {
         for (int it = 0; it < K; it++) {
-            hipLaunchKernelGGL(k_cg_rows, dim3(ceil_div(n, 4)), dim3(256), 0, s, a);
+            hipLaunchKernelGGL(k_cg_rows, dim3(nbr), dim3(256), 0, s, a);
             hipLaunchKernelGGL(k_cg_cols, dim3(nbc), dim3(256), 0, s, a);
             hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, a);
             hipLaunchKernelGGL(k_cg_dir, dim3(nbu), dim3(256), 0, s, a);

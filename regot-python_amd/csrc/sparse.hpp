@@ -26,8 +26,6 @@ struct KeepRule {
     unsigned long long kstar = 0;
     long long tie_take = 0;
     const long long* tie_off = nullptr;  // per row: ties in earlier rows
-    unsigned int* tie_bits = nullptr;    // row-major bitmask of kept ties
-    int tie_words = 0;                   // words per bitmask row
     int preserve = 1;
 };
 
@@ -49,9 +47,8 @@ struct SparseHessian {
     DBuf<double> rval, cval;
     // work
     DBuf<double> tau, rho;
-    DBuf<int> rcnt, ccnt, zcnt;
+    DBuf<int> rcnt, ccnt, zcnt, csc_work;
     DBuf<long long> ltmp;
-    DBuf<unsigned int> tiebits;
     DBuf<unsigned long long> hist;   // global count histograms of the top-k radix passes
     DBuf<double> vn, vm, vm2;        // alpha, beta (full) of the current build; scratch
     DBuf<double> cg_r, cg_p, cg_q, cg_y_buf, cg_part, cg_scal, cg_dinv;
@@ -90,16 +87,18 @@ void topk_select(int bulk, const void* M, long ldm, const ElemCtx& c, long long 
 void launch_row_ties(int bulk, const void* M, long ldm, const ElemCtx& c,
                      unsigned long long kstar, int* cnt, hipStream_t s);
 
-// Kept-entry counts per row (CSR) / per column (CSC) under a rule.
+// Kept-entry counts per row under a rule.
 void launch_row_count(int bulk, const void* M, long ldm, const ElemCtx& c, const KeepRule& r,
                       int* cnt, hipStream_t s);
-void launch_col_count(int bulk, const void* MT, long ldt, const ElemCtx& c, const KeepRule& r,
-                      int* cnt, hipStream_t s);
-// Write CSR / CSC (values T / reg) at the given pointers.
+// Write the CSR (values T / reg) at the given row pointers.
 void launch_row_write(int bulk, const void* M, long ldm, const ElemCtx& c, const KeepRule& r,
                       const int* rowptr, int* colind, double* val, hipStream_t s);
-void launch_col_write(int bulk, const void* MT, long ldt, const ElemCtx& c, const KeepRule& r,
-                      const int* colptr, int* rowind, double* val, hipStream_t s);
+
+// CSC (colptr, rowind, cval) from the CSR of H by a stable counting sort by
+// column: workgroups of rows_per_block rows, work = (n / rows_per_block) x
+// (m-1) ints, colcnt = m-1 ints; the entries of a column come out in row order.
+void launch_csr_to_csc(const SparseHessian& H, int* work, int rows_per_block, int* colcnt,
+                       int* colptr, long long* total, int* rowind, double* cval, hipStream_t s);
 
 // Exclusive scan of count (int) into out (count + 1 entries, int64 total check).
 void launch_scan(const int* cnt, int count, int* out, long long* total, hipStream_t s);

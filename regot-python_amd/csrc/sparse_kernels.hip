@@ -402,8 +402,7 @@ __global__ __launch_bounds__(256) void k_row_ties(const TM* __restrict__ M, long
 
 // Row kernels walk row i in 256-column chunks; for the top-k rule the rank of
 // a tie is tie_off[i] + ties earlier in the row, so ties are taken in
-// row-major order.  WRITE also records kept ties in the tie bitmask for the
-// column kernels.
+// row-major order.
 template <typename TM, bool WRITE>
 __global__ __launch_bounds__(256) void k_row_kept(const TM* __restrict__ M, long ldm, ElemCtx c,
                                                   KeepRule rule, int* __restrict__ cnt,
@@ -440,12 +439,7 @@ __global__ __launch_bounds__(256) void k_row_kept(const TM* __restrict__ M, long
         if (rule.mode == 1) {
             int ntie;
             const int tp = block_flag_prefix(tie, &ntie);
-            if (tie && toff + tie_seen + tp < rule.tie_take) {
-                if (!kept && WRITE)
-                    atomicOr(rule.tie_bits + static_cast<long>(i) * rule.tie_words + (j >> 5),
-                             1u << (j & 31));
-                kept = true;
-            }
+            if (tie && toff + tie_seen + tp < rule.tie_take) kept = true;
             tie_seen += ntie;
         }
         const bool stored = kept && t != TM(0);
@@ -461,50 +455,74 @@ __global__ __launch_bounds__(256) void k_row_kept(const TM* __restrict__ M, long
     if (!WRITE && threadIdx.x == 0) cnt[i] = static_cast<int>(written);
 }
 
-// Column kernels walk column j (row j of MT) in 256-row chunks with the same
-// element function and rule; top-k ties are looked up in the tie bitmask.
-template <typename TM, bool WRITE>
-__global__ __launch_bounds__(256) void k_col_kept(const TM* __restrict__ MT, long ldt, ElemCtx c,
-                                                  KeepRule rule, int* __restrict__ cnt,
-                                                  const int* __restrict__ colptr,
-                                                  int* __restrict__ rowind,
-                                                  double* __restrict__ val)
+// ---------------------------------------------------------------------------
+// CSC from CSR: a stable counting sort of the stored entries by column
+// ---------------------------------------------------------------------------
+
+// Workgroup b owns rows [b R, b R + R); cnt[b][j] = its entries in column j.
+// Columns within a row are distinct, so the adds of one row never collide;
+// the counts are integers, so they do not depend on the order of the adds.
+__global__ __launch_bounds__(256) void k_csc_count(const int* __restrict__ rowptr,
+                                                   const int* __restrict__ colind, int n,
+                                                   int rows_per_block, int m1,
+                                                   int* __restrict__ cnt)
 {
-    using E = Elem<TM>;
-    const int j = blockIdx.x;
-    const TM* col = MT + static_cast<long>(j) * ldt;
-    const double bj = c.beta[j];
-    const double inv_reg = 1.0 / c.reg;
-    const double tj = (rule.mode == 0) ? rule.tau[j] : 0.0;
-    long long written = 0;
-    for (int i0 = 0; i0 < c.n; i0 += 256) {
-        const int i = i0 + threadIdx.x;
-        bool kept = false;
-        TM t = TM(0);
-        if (i < c.n) {
-            bool z;
-            t = E::val(c.alpha[i] + bj, col[i], c.k, c.reg, z);
-            if (rule.mode == 0) {
-                kept = (static_cast<double>(t) >= fmin(tj, rule.rho[i])) || z;
-            } else {
-                const unsigned long long key = static_cast<unsigned long long>(E::key(t));
-                kept = key > rule.kstar || (rule.preserve && (i == 0 || j == 0));
-                if (!kept && key == rule.kstar)
-                    kept = (rule.tie_bits[static_cast<long>(i) * rule.tie_words + (j >> 5)] >>
-                            (j & 31)) & 1u;
-            }
+    int* mine = cnt + static_cast<long>(blockIdx.x) * m1;
+    const int r0 = blockIdx.x * rows_per_block;
+    const int r1 = (r0 + rows_per_block < n) ? r0 + rows_per_block : n;
+    if (r0 >= r1) return;
+    for (int k = rowptr[r0] + threadIdx.x; k < rowptr[r1]; k += 256) atomicAdd(&mine[colind[k]], 1);
+}
+
+// Per column j: off[b][j] = entries of column j in workgroups before b
+// (exclusive prefix over b, in place), total[j] = all of them.
+__global__ __launch_bounds__(256) void k_csc_prefix(int* __restrict__ cnt, int nb, int m1,
+                                                    int* __restrict__ total)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= m1) return;
+    int run = 0;
+    // 16 loads in flight per pass: the counts are independent, only the
+    // running sum is serial
+    for (int b0 = 0; b0 < nb; b0 += 16) {
+        int c[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            c[q] = (b0 + q < nb) ? cnt[static_cast<long>(b0 + q) * m1 + j] : 0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            if (b0 + q < nb) cnt[static_cast<long>(b0 + q) * m1 + j] = run;
+            run += c[q];
         }
-        const bool stored = kept && t != TM(0);
-        int nst;
-        const int pos = block_flag_prefix(stored, &nst);
-        if (WRITE && stored) {
-            const long q = static_cast<long>(colptr[j]) + written + pos;
-            rowind[q] = i;
-            val[q] = static_cast<double>(t) * inv_reg;
-        }
-        written += nst;
     }
-    if (!WRITE && threadIdx.x == 0) cnt[j] = static_cast<int>(written);
+    total[j] = run;
+}
+
+// Workgroup b scatters its rows in order: entry (i, j) goes to
+// colptr[j] + off[b][j], then off[b][j] += 1; rows are separated by a barrier,
+// so within a column the entries land in row order.
+__global__ __launch_bounds__(256) void k_csc_scatter(const int* __restrict__ rowptr,
+                                                     const int* __restrict__ colind,
+                                                     const double* __restrict__ rval, int n,
+                                                     int rows_per_block, int m1,
+                                                     const int* __restrict__ colptr,
+                                                     int* __restrict__ off,
+                                                     int* __restrict__ rowind,
+                                                     double* __restrict__ cval)
+{
+    int* mine = off + static_cast<long>(blockIdx.x) * m1;
+    const int r0 = blockIdx.x * rows_per_block;
+    const int r1 = (r0 + rows_per_block < n) ? r0 + rows_per_block : n;
+    for (int i = r0; i < r1; i++) {
+        for (int k = rowptr[i] + threadIdx.x; k < rowptr[i + 1]; k += 256) {
+            const int j = colind[k];
+            const int q = colptr[j] + mine[j];
+            mine[j] += 1;
+            rowind[q] = i;
+            cval[q] = rval[k];
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -717,29 +735,21 @@ void launch_row_write(int bulk, const void* M, long ldm, const ElemCtx& c, const
     KERNEL_CHECK();
 }
 
-void launch_col_count(int bulk, const void* MT, long ldt, const ElemCtx& c, const KeepRule& r,
-                      int* cnt, hipStream_t s)
+void launch_csr_to_csc(const SparseHessian& H, int* work, int rows_per_block, int* colcnt,
+                       int* colptr, long long* total, int* rowind, double* cval, hipStream_t s)
 {
-    if (c.m <= 1) return;
-    if (bulk == 0)
-        hipLaunchKernelGGL((k_col_kept<float, false>), dim3(c.m - 1), dim3(256), 0, s,
-                           cM<float>(MT), ldt, c, r, cnt, nullptr, nullptr, nullptr);
-    else
-        hipLaunchKernelGGL((k_col_kept<double, false>), dim3(c.m - 1), dim3(256), 0, s,
-                           cM<double>(MT), ldt, c, r, cnt, nullptr, nullptr, nullptr);
+    const int n = H.n, m1 = H.m - 1;
+    if (m1 <= 0) return;
+    const int nb = ceil_div(n, rows_per_block);
+    HIP_CHECK(hipMemsetAsync(work, 0, sizeof(int) * static_cast<size_t>(nb) * m1, s));
+    hipLaunchKernelGGL(k_csc_count, dim3(nb), dim3(256), 0, s, H.rowptr.p, H.colind.p, n,
+                       rows_per_block, m1, work);
+    hipLaunchKernelGGL(k_csc_prefix, dim3(ceil_div(m1, 256)), dim3(256), 0, s, work, nb, m1,
+                       colcnt);
     KERNEL_CHECK();
-}
-
-void launch_col_write(int bulk, const void* MT, long ldt, const ElemCtx& c, const KeepRule& r,
-                      const int* colptr, int* rowind, double* val, hipStream_t s)
-{
-    if (c.m <= 1) return;
-    if (bulk == 0)
-        hipLaunchKernelGGL((k_col_kept<float, true>), dim3(c.m - 1), dim3(256), 0, s,
-                           cM<float>(MT), ldt, c, r, nullptr, colptr, rowind, val);
-    else
-        hipLaunchKernelGGL((k_col_kept<double, true>), dim3(c.m - 1), dim3(256), 0, s,
-                           cM<double>(MT), ldt, c, r, nullptr, colptr, rowind, val);
+    launch_scan(colcnt, m1, colptr, total, s);
+    hipLaunchKernelGGL(k_csc_scatter, dim3(nb), dim3(256), 0, s, H.rowptr.p, H.colind.p,
+                       H.rval.p, n, rows_per_block, m1, colptr, work, rowind, cval);
     KERNEL_CHECK();
 }
 
