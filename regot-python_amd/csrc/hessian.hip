@@ -39,11 +39,27 @@ __global__ void k_sub_div(const double* w, const double* y, const double* h1, do
 }
 
 // ---- CG on Delta, device-controlled --------------------------------------
-// Conjugate gradients, optionally with the Jacobi preconditioner z = D^-1 r
-// (dinv = 1 / diag(Delta); null: plain CG, z = r).
-// ctl: [0] done, [1] iterations.  scal: [0] rho = r.z (current), [1] stop
-// threshold on r.r, [2] rho (next, published by the direction kernel),
-// [3] rhs norm^2, [4] r.r of the last update.
+// Preconditioned conjugate gradients in the Chronopoulos-Gear form (one
+// reduction phase per iteration; the iterates of the textbook recurrence in
+// exact arithmetic), Jacobi preconditioner u = D^-1 r (dinv = 1 / diag(Delta);
+// null: plain CG, u = r).  Iteration i is three kernels:
+//
+//   update  alpha_i, beta_i from the partials of gamma_i = r.u, delta_i = w.u
+//           (every workgroup reduces them in the same order); p = u + beta p,
+//           s = w + beta s, x += alpha p, r -= alpha s, u = D^-1 r; partials
+//           of r.r;
+//   rows    y = (sigma u) / (h1 + shift);
+//   cols    stop if r.r < threshold (every workgroup decides alike), else
+//           w = (h2 + shift) u - sigma' y and the partials of r.u, w.u,
+//
+// with beta_i = gamma_i / gamma_{i-1}, alpha_i = gamma_i / (delta_i - beta_i
+// gamma_i / alpha_{i-1}); alpha_0, gamma_0 come from the set-up and beta_0 = 0.
+// The iteration index is a launch argument (the host enqueues at most
+// max_iter iterations); alpha and gamma are kept in two slots by parity so
+// no kernel reads a slot that it writes.
+// ctl: [0] done, [1] iterations when done.  scal: [0..1] alpha by parity,
+// [2..3] gamma by parity, [4] r.r of the last update, [5] stop threshold,
+// [6] rhs norm^2.
 struct CgArgs {
     const int* rowptr;
     const int* colind;
@@ -58,14 +74,15 @@ struct CgArgs {
     double* x;
     double* r;
     double* p;
-    double* q;
+    double* s;
+    double* w;
+    double* u;
     double* y;      // n
-    double* part;   // per column-workgroup partials (2 arrays)
+    double* part;   // [2 nbc] r.u, w.u per column workgroup; [nbu] r.r per update workgroup
     double* scal;
     int* ctl;
-    int max_iter;
-    int nbc;        // column workgroups (4 columns each)
-    int nbu;        // update workgroups (1024 entries each)
+    int nbc;        // column workgroups
+    int nbu;        // update workgroups (256 entries each)
     const double* dinv;   // Jacobi preconditioner, or null
 };
 
@@ -97,10 +114,10 @@ __global__ __launch_bounds__(256) void k_delta_diag(const int* __restrict__ colp
     }
 }
 
-// z = dinv .* r (or r), p = z, partials of r.z and r.r per 1024 entries.
+// u = dinv .* r (or r), partials of r.u and r.r per 1024 entries.
 __global__ __launch_bounds__(256) void k_pcg_start(const double* __restrict__ r,
                                                    const double* __restrict__ dinv, int m1,
-                                                   double* __restrict__ p,
+                                                   double* __restrict__ u,
                                                    double* __restrict__ part)
 {
     __shared__ double sm[2][256];
@@ -109,7 +126,7 @@ __global__ __launch_bounds__(256) void k_pcg_start(const double* __restrict__ r,
          j += 256) {
         const double rj = r[j];
         const double zj = dinv ? rj * dinv[j] : rj;
-        p[j] = zj;
+        u[j] = zj;
         rz += rj * zj;
         rr += rj * rj;
     }
@@ -129,140 +146,151 @@ __global__ __launch_bounds__(256) void k_pcg_start(const double* __restrict__ r,
     }
 }
 
-// y = (sigma p) / (h1 + shift), one wave per row, the waves of the grid
-// striding over the rows (the grid is one resident round); workgroup 0 also
-// moves the next rho into place.
-__global__ __launch_bounds__(256) void k_cg_rows(CgArgs a)
+// Sum of v over lanes [0, 64) of wave 0 in a fixed butterfly order,
+// broadcast to the workgroup through sm.
+__device__ __forceinline__ double wave0_sum_bcast(double v, double* sm)
 {
-    if (a.ctl[0]) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && a.ctl[1] > 0) a.scal[0] = a.scal[2];
-    const int lane = threadIdx.x & 63;
-    const int nw = gridDim.x * 4;
-    for (int i = blockIdx.x * 4 + (threadIdx.x >> 6); i < a.n; i += nw) {
-        double s = 0.0;
-        for (int k = a.rowptr[i] + lane; k < a.rowptr[i + 1]; k += 64)
-            s += a.rval[k] * a.p[a.colind[k]];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if (lane == 0) a.y[i] = s / (a.h1[i] + a.shift);
-    }
-}
-
-__device__ __forceinline__ double wg_sum4(double v, double* sm)
-{
-    // lane 0 of each wave holds a value; fixed-order sum of the 4
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) sm[wave] = v;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (threadIdx.x == 0) *sm = v;
     __syncthreads();
-    const double r = ((sm[0] + sm[1]) + sm[2]) + sm[3];
+    const double r = *sm;
     __syncthreads();
     return r;
 }
 
-// q = (h2 + shift) p - sigma' y, one wave per column, the waves of the grid
-// striding over the columns; partial p.q per workgroup (fixed order).
-__global__ __launch_bounds__(256) void k_cg_cols(CgArgs a)
-{
-    __shared__ double sm[4];
-    if (a.ctl[0]) return;
-    const int lane = threadIdx.x & 63;
-    const int nw = gridDim.x * 4;
-    double pq = 0.0;
-    for (int j = blockIdx.x * 4 + (threadIdx.x >> 6); j < a.m1; j += nw) {
-        double s = 0.0;
-        for (int k = a.colptr[j] + lane; k < a.colptr[j + 1]; k += 64)
-            s += a.cval[k] * a.y[a.rowind[k]];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        const double qj = a.h2[j] * a.p[j] + a.shift * a.p[j] - s;
-        if (lane == 0) a.q[j] = qj;
-        pq += a.p[j] * qj;
-    }
-    pq = wg_sum4(pq, sm);
-    if (threadIdx.x == 0) a.part[blockIdx.x] = pq;
-}
-
-// Every workgroup reduces the p.q partials (same order, same value), then
-// x += alpha p, r -= alpha q over its 1024 entries; partials of r.z and r.r.
-__global__ __launch_bounds__(256) void k_cg_update(CgArgs a)
+__global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int it)
 {
     __shared__ double sm[2][256];
     if (a.ctl[0]) return;
-    double v = 0.0;
-    for (int k = threadIdx.x; k < a.nbc; k += 256) v += a.part[k];
-    sm[0][threadIdx.x] = v;
+    double al, be = 0.0;
+    const int cur = it & 1, prev = cur ^ 1;
+    if (it == 0) {
+        al = a.scal[0];
+    } else {
+        double v0 = 0.0, v1 = 0.0;
+        for (int b = threadIdx.x; b < a.nbc; b += 256) {
+            v0 += a.part[2 * b];
+            v1 += a.part[2 * b + 1];
+        }
+        sm[0][threadIdx.x] = v0;
+        sm[1][threadIdx.x] = v1;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (threadIdx.x < o) {
+                sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
+                sm[1][threadIdx.x] += sm[1][threadIdx.x + o];
+            }
+            __syncthreads();
+        }
+        const double g = sm[0][0], d = sm[1][0];
+        const double al_prev = a.scal[prev], g_prev = a.scal[2 + prev];
+        be = g / g_prev;
+        al = g / (d - be * g / al_prev);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.scal[cur] = al;
+            a.scal[2 + cur] = g;
+        }
+    }
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    double rr = 0.0;
+    if (j < a.m1) {
+        const double pj = a.u[j] + be * a.p[j];
+        const double sj = a.w[j] + be * a.s[j];
+        const double rj = a.r[j] - al * sj;
+        a.p[j] = pj;
+        a.s[j] = sj;
+        a.x[j] += al * pj;
+        a.r[j] = rj;
+        a.u[j] = a.dinv ? a.dinv[j] * rj : rj;
+        rr = rj * rj;
+    }
+    // fixed-order workgroup sum
+    __syncthreads();
+    sm[0][threadIdx.x] = rr;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
         __syncthreads();
     }
-    const double pq = sm[0][0];
-    const double alpha = a.scal[0] / pq;
-    __syncthreads();
-    double rr = 0.0, rz = 0.0;
-    for (int j = blockIdx.x * 1024 + threadIdx.x; j < (blockIdx.x + 1) * 1024 && j < a.m1; j += 256) {
-        a.x[j] += alpha * a.p[j];
-        const double rj = a.r[j] - alpha * a.q[j];
-        a.r[j] = rj;
-        rr += rj * rj;
-        rz += a.dinv ? rj * rj * a.dinv[j] : 0.0;
-    }
-    sm[0][threadIdx.x] = rr;
-    sm[1][threadIdx.x] = rz;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
-            sm[1][threadIdx.x] += sm[1][threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        a.part[a.nbc + 2 * blockIdx.x] = sm[0][0];
-        a.part[a.nbc + 2 * blockIdx.x + 1] = sm[1][0];
-    }
+    if (threadIdx.x == 0) a.part[2 * a.nbc + blockIdx.x] = sm[0][0];
 }
 
-// Every workgroup reduces r.r and r.z; stop (rr < threshold, or the
-// iteration cap after this update), else p = z + (rho_new / rho) p.
-__global__ __launch_bounds__(256) void k_cg_dir(CgArgs a)
+// y = (sigma u) / (h1 + shift); LPR lanes per row, 64 / LPR rows per wave.
+template <int LPR>
+__global__ __launch_bounds__(256) void k_cg_rows(CgArgs a)
 {
-    __shared__ double sm[2][256];
     if (a.ctl[0]) return;
-    double v = 0.0, w = 0.0;
-    for (int k = threadIdx.x; k < a.nbu; k += 256) {
-        v += a.part[a.nbc + 2 * k];
-        w += a.part[a.nbc + 2 * k + 1];
+    const int lane = threadIdx.x & 63;
+    const int sl = lane & (LPR - 1);
+    const int i = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / LPR) + lane / LPR;
+    double acc = 0.0;
+    if (i < a.n) {
+        const int e = a.rowptr[i + 1];
+#pragma unroll 4
+        for (int k = a.rowptr[i] + sl; k < e; k += LPR) acc += a.rval[k] * a.u[a.colind[k]];
     }
-    sm[0][threadIdx.x] = v;
-    sm[1][threadIdx.x] = w;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
-            sm[1][threadIdx.x] += sm[1][threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    const double rr_new = sm[0][0];
-    const double rho_new = a.dinv ? sm[1][0] : rr_new;
-    const bool conv = rr_new < a.scal[1];
-    if (!conv) {
-        const double beta = rho_new / a.scal[0];
-        for (int j = blockIdx.x * 1024 + threadIdx.x; j < (blockIdx.x + 1) * 1024 && j < a.m1;
-             j += 256)
-            a.p[j] = (a.dinv ? a.r[j] * a.dinv[j] : a.r[j]) + beta * a.p[j];
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.scal[2] = rho_new;
-        a.scal[4] = rr_new;
-        if (conv) {
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (sl == 0 && i < a.n) a.y[i] = acc / (a.h1[i] + a.shift);
+}
+
+// Stop test on the r.r partials of the update kernel (same order in every
+// workgroup), then w = (h2 + shift) u - sigma' y with LPR lanes per column and
+// the partials of r.u and w.u per workgroup (fixed order).
+template <int LPR>
+__global__ __launch_bounds__(256) void k_cg_cols(CgArgs a, int it)
+{
+    constexpr int CPW = 64 / LPR;   // columns per wave
+    __shared__ double sm[2][4 * CPW];
+    __shared__ double bc;
+    if (a.ctl[0]) return;
+    double v = 0.0;
+    if (threadIdx.x < 64)
+        for (int b = threadIdx.x; b < a.nbu; b += 64) v += a.part[2 * a.nbc + b];
+    const double rr = wave0_sum_bcast(v, &bc);
+    if (rr < a.scal[5]) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.scal[4] = rr;
+            a.ctl[1] = it;   // the converged update is not counted (Eigen's convention)
             a.ctl[0] = 1;
-        } else {
-            const int it = a.ctl[1] + 1;
-            a.ctl[1] = it;
-            if (it >= a.max_iter) a.ctl[0] = 1;
         }
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.scal[4] = rr;
+    const int lane = threadIdx.x & 63;
+    const int sl = lane & (LPR - 1);
+    const int slot = (threadIdx.x >> 6) * CPW + lane / LPR;
+    const int j = blockIdx.x * (4 * CPW) + slot;
+    double t = 0.0;
+    if (j < a.m1) {
+        const int e = a.colptr[j + 1];
+#pragma unroll 4
+        for (int k = a.colptr[j] + sl; k < e; k += LPR) t += a.cval[k] * a.y[a.rowind[k]];
+    }
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if (sl == 0) {
+        double g = 0.0, d = 0.0;
+        if (j < a.m1) {
+            const double uj = a.u[j];
+            const double wj = a.h2[j] * uj + a.shift * uj - t;
+            a.w[j] = wj;
+            g = a.r[j] * uj;
+            d = wj * uj;
+        }
+        sm[0][slot] = g;
+        sm[1][slot] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double g = 0.0, d = 0.0;
+        for (int q = 0; q < 4 * CPW; q++) {
+            g += sm[0][q];
+            d += sm[1][q];
+        }
+        a.part[2 * blockIdx.x] = g;
+        a.part[2 * blockIdx.x + 1] = d;
     }
 }
 
@@ -300,14 +328,13 @@ static ElemCtx make_ctx(regot_problem* p, SparseHessian& H)
     return c;
 }
 
-// Shared tail of both rules: row counts -> scan -> CSR write -> CSC by a
-// counting sort of the CSR.
+// Shared tail of both rules: row counts (in H.rcnt) -> scan -> CSR write ->
+// CSC by a counting sort of the CSR.
 static void build_storage(regot_problem* p, SparseHessian& H, const ElemCtx& c,
-                          const KeepRule& rule, bool row_counts_ready)
+                          const KeepRule& rule)
 {
     const int n = p->n, m = p->m;
     hipStream_t s = p->stream;
-    if (!row_counts_ready) launch_row_count(p->bulk, p->dM, p->ldm, c, rule, H.rcnt, s);
     H.rowptr.ensure(n + 1);
     H.ltmp.ensure(4);
     launch_scan(H.rcnt, n, H.rowptr, H.ltmp.p, s);
@@ -322,15 +349,17 @@ static void build_storage(regot_problem* p, SparseHessian& H, const ElemCtx& c,
     H.rowind.ensure(nnz);
     H.cval.ensure(nnz);
     launch_row_write(p->bulk, p->dM, p->ldm, c, rule, H.rowptr, H.colind, H.rval, s);
-    // CSC: the transpose of the CSR just written (a counting sort of nnz
-    // entries instead of two more passes over M)
-    // at most 512 row blocks: the offset table stays <= 2 KiB per column
+    // CSC: the transpose of the CSR just written.  Below 2% density a
+    // counting sort of the nnz entries; above, one pass over M by column strips
+    // (the sort's scattered writes cost more than re-reading M there).  At most
+    // 512 row blocks: the offset table stays <= 2 KiB per column.
     const int rows_per_block = std::max(32, ceil_div(n, 512));
     H.ccnt.ensure(m);
     H.colptr.ensure(m);
     H.csc_work.ensure(static_cast<size_t>(ceil_div(n, rows_per_block)) * std::max(m - 1, 1));
-    launch_csr_to_csc(H, H.csc_work, rows_per_block, H.ccnt, H.colptr, H.ltmp.p + 1, H.rowind,
-                      H.cval, s);
+    const bool from_M = static_cast<double>(nnz) > 0.02 * n * static_cast<double>(m - 1);
+    launch_build_csc(p->bulk, p->dM, p->ldm, c, rule, H, from_M, H.csc_work, rows_per_block,
+                     H.ccnt, H.colptr, H.ltmp.p + 1, H.rowind, H.cval, s);
 }
 
 static void prepare(regot_problem* p, SparseHessian& H, const double* d_gamma, const double* d_grad)
@@ -365,7 +394,7 @@ void hess_threshold(regot_problem* p, const double* d_gamma, const double* d_gra
     rule.mode = 0;
     rule.tau = H.tau;
     rule.rho = H.rho;
-    build_storage(p, H, c, rule, true);
+    build_storage(p, H, c, rule);
     // FP64-zero kept entries count towards the reported density
     launch_scan(H.zcnt, p->n, H.rcnt, H.ltmp.p + 2, s);
     HIP_CHECK(hipMemcpyAsync(H.hll.p, H.ltmp.p, 3 * sizeof(long long), hipMemcpyDeviceToHost, s));
@@ -402,13 +431,15 @@ void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, do
         rule.kstar = kstar;
         rule.tie_take = kk - above;
     }
-    // tie offsets per row (row-major order)
-    launch_row_ties(p->bulk, p->dM, p->ldm, c, rule.kstar, H.rcnt, s);
+    // per row: ties and the stored entries kept without a tie slot; the tie
+    // offsets (row-major order) then give the stored count of every row
+    launch_row_tally(p->bulk, p->dM, p->ldm, c, rule.kstar, rule.preserve, H.rcnt, H.zcnt, s);
     H.ltmp.ensure(n + 4);
     launch_scan64(H.rcnt, n, H.ltmp.p + 4, s);
     rule.tie_off = H.ltmp.p + 4;
+    launch_topk_count(H.rcnt, H.zcnt, rule, n, H.rcnt, s);
     H.nnz_zero = 0;
-    build_storage(p, H, c, rule, false);
+    build_storage(p, H, c, rule);
     HIP_CHECK(hipMemcpyAsync(H.hll.p, H.ltmp.p, 2 * sizeof(long long), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (H.hll.p[1] != H.nnz)
@@ -440,6 +471,51 @@ void hess_apply_hs(regot_problem* p, const double* x, double* out)
 // CG on Delta (size m-1) from zero; textbook unpreconditioned CG with the
 // stopping rule ||r||^2 < max(tol^2 ||b||^2, DBL_MIN) and the iteration cap
 // counted as Eigen's ConjugateGradient counts it (src/sinkhorn_cg.cpp:29-38).
+// Lanes per row (column) of the CG products: about three entries per lane,
+// and at least 8192 waves over `count` rows (one resident round of the chip;
+// fewer waves leave the load latency exposed).
+static int lanes_for(double avg, int count)
+{
+    int l = 4;
+    while (l < 64 && (3.0 * l < avg || static_cast<long>(count) * l < 8192L * 64)) l <<= 1;
+    return l;
+}
+
+template <int LPR>
+static void launch_cg_rows(const CgArgs& a, hipStream_t s)
+{
+    const int rows_per_wg = 4 * (64 / LPR);
+    hipLaunchKernelGGL(k_cg_rows<LPR>, dim3(ceil_div(a.n, rows_per_wg)), dim3(256), 0, s, a);
+}
+
+template <int LPR>
+static void launch_cg_cols(const CgArgs& a, int it, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cg_cols<LPR>, dim3(a.nbc), dim3(256), 0, s, a, it);
+}
+
+static void cg_rows(int lpr, const CgArgs& a, hipStream_t s)
+{
+    switch (lpr) {
+    case 4: launch_cg_rows<4>(a, s); break;
+    case 8: launch_cg_rows<8>(a, s); break;
+    case 16: launch_cg_rows<16>(a, s); break;
+    case 32: launch_cg_rows<32>(a, s); break;
+    default: launch_cg_rows<64>(a, s); break;
+    }
+}
+
+static void cg_cols(int lpr, const CgArgs& a, int it, hipStream_t s)
+{
+    switch (lpr) {
+    case 4: launch_cg_cols<4>(a, it, s); break;
+    case 8: launch_cg_cols<8>(a, it, s); break;
+    case 16: launch_cg_cols<16>(a, it, s); break;
+    case 32: launch_cg_cols<32>(a, it, s); break;
+    default: launch_cg_cols<64>(a, it, s); break;
+    }
+}
+
 int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
                   double* x, double* tol_error, const double* x0, bool jacobi)
 {
@@ -453,13 +529,16 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     H.cg_r.ensure(m1);
     H.cg_p.ensure(m1);
     H.cg_q.ensure(m1);
+    H.cg_s.ensure(m1);
+    H.cg_u.ensure(m1);
     H.vn.ensure(n);
     H.cg_y_buf.ensure(n);
-    // one resident round of workgroups for the SpMV kernels (8 per CU)
-    const int nbr = std::min(ceil_div(n, 4), 2048);
-    const int nbc = std::min(ceil_div(m1, 4), 2048);
-    const int nbu = ceil_div(m1, 1024);
-    H.cg_part.ensure(nbc + 2L * nbu);
+    const int lpr_r = lanes_for(static_cast<double>(H.nnz) / std::max(n, 1), n);
+    const int lpr_c = lanes_for(static_cast<double>(H.nnz) / m1, m1);
+    const int nbc = ceil_div(m1, 4 * (64 / lpr_c));
+    const int nbu = ceil_div(m1, 256);
+    const int nbs = ceil_div(m1, 1024);   // set-up partials
+    H.cg_part.ensure(std::max(2L * nbc + nbu, 2L * nbs));
     H.cg_scal.ensure(8);
     if (jacobi) {
         H.cg_dinv.ensure(m1);
@@ -472,7 +551,8 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     H.cg_hctl.ensure(16);
     H.hbuf.ensure(8);
 
-    // x = x0 (or 0), r = rhs - Delta x, p = z = D^-1 r (or r)
+    // x = x0 (or 0), r = rhs - Delta x, u = D^-1 r (or r), w = Delta u (in q),
+    // p = s = 0 (beta_0 = 0)
     if (x0) {
         vec_copy(x, x0, m1, s);
         hess_apply_delta(p, x0, shift, H.cg_q);
@@ -481,20 +561,25 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
         launch_fill(x, 0.0, m1, s);
         vec_copy(H.cg_r, rhs, m1, s);
     }
-    hipLaunchKernelGGL(k_pcg_start, dim3(nbu), dim3(256), 0, s, H.cg_r.p, dinv, m1, H.cg_p.p,
+    hipLaunchKernelGGL(k_pcg_start, dim3(nbs), dim3(256), 0, s, H.cg_r.p, dinv, m1, H.cg_u.p,
                        H.cg_part.p);
     KERNEL_CHECK();
-    const double* xs[1] = {rhs};
-    vec_dots(xs, xs, m1, 1, H.cg_scal.p + 3, s);
-    std::vector<double> hpart(2L * nbu);
-    HIP_CHECK(hipMemcpyAsync(hpart.data(), H.cg_part.p, 2L * nbu * sizeof(double),
+    hess_apply_delta(p, H.cg_u, shift, H.cg_q);
+    const double* xs[2] = {rhs, H.cg_q.p};
+    const double* ys[2] = {rhs, H.cg_u.p};
+    vec_dots(xs, ys, m1, 2, H.cg_scal.p + 6, s);
+    std::vector<double> hpart(2L * nbs);
+    HIP_CHECK(hipMemcpyAsync(hpart.data(), H.cg_part.p, 2L * nbs * sizeof(double),
                              hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(H.hbuf.p, H.cg_scal.p + 3, sizeof(double), hipMemcpyDeviceToHost,
-                             s));
+    HIP_CHECK(hipMemcpyAsync(H.hbuf.p, H.cg_scal.p + 6, 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+    launch_fill(H.cg_p, 0.0, m1, s);
+    launch_fill(H.cg_s, 0.0, m1, s);
     HIP_CHECK(hipStreamSynchronize(s));
     const double rhs_norm2 = H.hbuf.p[0];
+    const double delta0 = H.hbuf.p[1];
     double rz0 = 0.0, rr0 = 0.0;
-    for (int b = 0; b < nbu; b++) {
+    for (int b = 0; b < nbs; b++) {
         rz0 += hpart[2 * b];
         rr0 += hpart[2 * b + 1];
     }
@@ -506,19 +591,14 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     double thr = tol * tol * rhs_norm2;
     if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
     // converged at the start (tol >= 1 without a guess, or an exact guess)
-    if (rr0 < thr) {
+    if (rr0 < thr || max_iter <= 0) {
         if (tol_error) *tol_error = std::sqrt(rr0 / rhs_norm2);
         return 0;
     }
-    double hs[5] = {rz0, thr, rz0, rhs_norm2, rr0};
-    HIP_CHECK(hipMemcpyAsync(H.cg_scal.p, hs, 5 * sizeof(double), hipMemcpyHostToDevice, s));
+    double hs[7] = {rz0 / delta0, 0.0, rz0, 0.0, rr0, thr, rhs_norm2};
+    HIP_CHECK(hipMemcpyAsync(H.cg_scal.p, hs, 7 * sizeof(double), hipMemcpyHostToDevice, s));
     int hc[4] = {0, 0, 0, 0};
     HIP_CHECK(hipMemcpyAsync(H.cg_ctl.p, hc, 4 * sizeof(int), hipMemcpyHostToDevice, s));
-    if (max_iter <= 0) {
-        if (tol_error) *tol_error = 1.0;
-        HIP_CHECK(hipStreamSynchronize(s));
-        return 0;
-    }
 
     CgArgs a;
     a.rowptr = H.rowptr;
@@ -535,48 +615,52 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     a.x = x;
     a.r = H.cg_r;
     a.p = H.cg_p;
-    a.q = H.cg_q;
+    a.s = H.cg_s;
+    a.w = H.cg_q;
+    a.u = H.cg_u;
     a.y = H.cg_y_buf;
     a.part = H.cg_part;
     a.scal = H.cg_scal;
     a.ctl = H.cg_ctl;
-    a.max_iter = max_iter;
     a.nbc = nbc;
     a.nbu = nbu;
     a.dinv = dinv;
 
+    // batches of K iterations, two in flight; iteration indices run 0 ..
+    // max_iter - 1 and nothing is enqueued past them
     const int K = 16;
+    int k_enq = 0;
     hipEvent_t ev[2];
     HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     auto launch_batch = [&](int slot) {
-        for (int it = 0; it < K; it++) {
-            hipLaunchKernelGGL(k_cg_rows, dim3(nbr), dim3(256), 0, s, a);
-            hipLaunchKernelGGL(k_cg_cols, dim3(nbc), dim3(256), 0, s, a);
-            hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, a);
-            hipLaunchKernelGGL(k_cg_dir, dim3(nbu), dim3(256), 0, s, a);
+        for (int q = 0; q < K && k_enq < max_iter; q++, k_enq++) {
+            hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, a, k_enq);
+            cg_rows(lpr_r, a, s);
+            cg_cols(lpr_c, a, k_enq, s);
         }
         KERNEL_CHECK();
         HIP_CHECK(hipMemcpyAsync(H.cg_hctl.p + 4 * slot, H.cg_ctl.p, 4 * sizeof(int),
                                  hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipEventRecord(ev[slot], s));
     };
-    int iters = 0;
     try {
         std::deque<int> q;
         launch_batch(0);
         q.push_back(0);
-        launch_batch(1);
-        q.push_back(1);
-        for (;;) {
+        if (k_enq < max_iter) {
+            launch_batch(1);
+            q.push_back(1);
+        }
+        while (!q.empty()) {
             const int slot = q.front();
             q.pop_front();
             HIP_CHECK(hipEventSynchronize(ev[slot]));
-            if (H.cg_hctl.p[4 * slot]) {
-                break;
+            if (H.cg_hctl.p[4 * slot]) break;
+            if (k_enq < max_iter) {
+                launch_batch(slot);
+                q.push_back(slot);
             }
-            launch_batch(slot);
-            q.push_back(slot);
         }
         HIP_CHECK(hipStreamSynchronize(s));
     } catch (...) {
@@ -589,7 +673,7 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     HIP_CHECK(hipEventDestroy(ev[1]));
     HIP_CHECK(hipMemcpy(H.cg_hctl.p + 8, H.cg_ctl.p, 4 * sizeof(int), hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(H.hbuf.p, H.cg_scal.p, 5 * sizeof(double), hipMemcpyDeviceToHost));
-    iters = H.cg_hctl.p[9];
+    const int iters = H.cg_hctl.p[8] ? H.cg_hctl.p[9] : max_iter;
     if (tol_error) *tol_error = std::sqrt(H.hbuf.p[4] / rhs_norm2);
     return iters;
 }

@@ -51,7 +51,7 @@ struct SparseHessian {
     DBuf<long long> ltmp;
     DBuf<unsigned long long> hist;   // global count histograms of the top-k radix passes
     DBuf<double> vn, vm, vm2;        // alpha, beta (full) of the current build; scratch
-    DBuf<double> cg_r, cg_p, cg_q, cg_y_buf, cg_part, cg_scal, cg_dinv;
+    DBuf<double> cg_r, cg_p, cg_q, cg_s, cg_u, cg_y_buf, cg_part, cg_scal, cg_dinv;
     DBuf<int> cg_ctl;
     HBuf<int> cg_hctl;
     HBuf<double> hbuf;
@@ -83,22 +83,28 @@ void launch_row_threshold(int bulk, const void* M, long ldm, const ElemCtx& c,
 void topk_select(int bulk, const void* M, long ldm, const ElemCtx& c, long long k,
                  SparseHessian& H, unsigned long long* kstar, long long* n_greater,
                  hipStream_t s);
-// Per row: ties (key == kstar) into cnt.
-void launch_row_ties(int bulk, const void* M, long ldm, const ElemCtx& c,
-                     unsigned long long kstar, int* cnt, hipStream_t s);
-
-// Kept-entry counts per row under a rule.
-void launch_row_count(int bulk, const void* M, long ldm, const ElemCtx& c, const KeepRule& r,
-                      int* cnt, hipStream_t s);
+// Top-k tallies per row: ties (key == kstar) into tcnt; 2 x (stored entries
+// kept without a tie slot) + (column 0 is a preserved tie of a row > 0) into gcnt.
+void launch_row_tally(int bulk, const void* M, long ldm, const ElemCtx& c,
+                      unsigned long long kstar, int preserve, int* tcnt, int* gcnt,
+                      hipStream_t s);
+// Stored entries per row under the top-k rule from the tallies and r.tie_off.
+void launch_topk_count(const int* tcnt, const int* gcnt, const KeepRule& r, int n, int* cnt,
+                       hipStream_t s);
 // Write the CSR (values T / reg) at the given row pointers.
 void launch_row_write(int bulk, const void* M, long ldm, const ElemCtx& c, const KeepRule& r,
                       const int* rowptr, int* colind, double* val, hipStream_t s);
 
-// CSC (colptr, rowind, cval) from the CSR of H by a stable counting sort by
-// column: workgroups of rows_per_block rows, work = (n / rows_per_block) x
-// (m-1) ints, colcnt = m-1 ints; the entries of a column come out in row order.
-void launch_csr_to_csc(const SparseHessian& H, int* work, int rows_per_block, int* colcnt,
-                       int* colptr, long long* total, int* rowind, double* cval, hipStream_t s);
+// CSC (colptr, rowind, cval) of the kept set whose CSR H holds: column
+// counts per block of rows_per_block rows from the CSR (work = (n /
+// rows_per_block) x (m-1) ints, colcnt = m-1 ints), then either a stable
+// counting sort of the CSR entries (sparse sets) or, with from_M, a pass over
+// M by column strips under the same rule r (dense sets).  The entries of a
+// column come out in row order either way.
+void launch_build_csc(int bulk, const void* M, long ldm, const ElemCtx& c, const KeepRule& r,
+                      const SparseHessian& H, bool from_M, int* work, int rows_per_block,
+                      int* colcnt, int* colptr, long long* total, int* rowind, double* cval,
+                      hipStream_t s);
 
 // Exclusive scan of count (int) into out (count + 1 entries, int64 total check).
 void launch_scan(const int* cnt, int count, int* out, long long* total, hipStream_t s);

@@ -345,9 +345,64 @@ __global__ __launch_bounds__(256) void k_row_threshold(const TM* __restrict__ M,
 // Top-k rule
 // ---------------------------------------------------------------------------
 
+// Four consecutive elements from column c0 (c0 % 4 == 0) of a 16-byte aligned
+// row of `lim` valid entries; past lim the values are 0 and nothing is read.
+__device__ __forceinline__ void load4(const float* __restrict__ row, int c0, int lim, float (&v)[4])
+{
+    if (c0 + 3 < lim) {
+        const float4 u = *reinterpret_cast<const float4*>(row + c0);
+        v[0] = u.x;
+        v[1] = u.y;
+        v[2] = u.z;
+        v[3] = u.w;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = (c0 + q < lim) ? row[c0 + q] : 0.0f;
+    }
+}
+
+__device__ __forceinline__ void load4(const double* __restrict__ row, int c0, int lim,
+                                      double (&v)[4])
+{
+    if (c0 + 3 < lim) {
+        const double2 u0 = *reinterpret_cast<const double2*>(row + c0);
+        const double2 u1 = *reinterpret_cast<const double2*>(row + c0 + 2);
+        v[0] = u0.x;
+        v[1] = u0.y;
+        v[2] = u1.x;
+        v[3] = u1.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = (c0 + q < lim) ? row[c0 + q] : 0.0;
+    }
+}
+
+// Exclusive scan of one int per thread over the 256-thread workgroup, in
+// thread order; *total gets the sum.
+__device__ int block_excl_scan_int(int v, int* total)
+{
+    __shared__ int wt[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(inc, o);
+        if (lane >= o) inc += u;
+    }
+    __syncthreads();
+    if (lane == 63) wt[wave] = inc;
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wave; w++) off += wt[w];
+    *total = wt[0] + wt[1] + wt[2] + wt[3];
+    return off + inc - v;
+}
+
 // Count histogram of the current digit over entries whose higher digits match
-// prefix; workgroups cover row ranges, LDS histogram, then added to the global
-// one (integer adds: the totals do not depend on order).
+// prefix.  Workgroup b covers rows [b R, b R + R); thread t owns the column
+// quads 4t + 1024 s, keeps their beta in registers and walks the rows four at
+// a time (four vector loads in flight).  LDS histogram, then added to the
+// global one (integer adds: the totals do not depend on order).
 template <typename TM>
 __global__ __launch_bounds__(256) void k_topk_hist(const TM* __restrict__ M, long ldm, ElemCtx c,
                                                    int rows_per_block, int shift, int width,
@@ -362,15 +417,29 @@ __global__ __launch_bounds__(256) void k_topk_hist(const TM* __restrict__ M, lon
     const int m1 = c.m - 1;
     const long r0 = static_cast<long>(blockIdx.x) * rows_per_block;
     const long r1 = (r0 + rows_per_block < c.n) ? r0 + rows_per_block : c.n;
-    for (long i = r0; i < r1; i++) {
-        const TM* row = M + i * ldm;
-        const double ai = c.alpha[i];
-        for (int j = threadIdx.x; j < m1; j += 256) {
-            bool z;
-            const TM t = E::val(ai + c.beta[j], row[j], c.k, c.reg, z);
-            const K key = E::key(t);
-            if (first || (static_cast<unsigned long long>(key >> (shift + width)) == prefix))
-                atomicAdd(&hist[(key >> shift) & ((K(1) << width) - 1)], 1u);
+    const K dmask = (K(1) << width) - 1;
+    for (int c0 = threadIdx.x * 4; c0 < m1; c0 += 1024) {
+        double b4[4];
+        load4(c.beta, c0, c.m, b4);
+        for (long i0 = r0; i0 < r1; i0 += 4) {
+            TM v[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (i0 + u < r1) load4(M + (i0 + u) * ldm, c0, c.m, v[u]);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (i0 + u >= r1) break;
+                const double ai = c.alpha[i0 + u];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (c0 + q >= m1) break;
+                    bool z;
+                    const TM t = E::val(ai + b4[q], v[u][q], c.k, c.reg, z);
+                    const K key = E::key(t);
+                    if (first || (static_cast<unsigned long long>(key >> (shift + width)) == prefix))
+                        atomicAdd(&hist[(key >> shift) & dmask], 1u);
+                }
+            }
         }
     }
     __syncthreads();
@@ -378,37 +447,88 @@ __global__ __launch_bounds__(256) void k_topk_hist(const TM* __restrict__ M, lon
         if (hist[q]) atomicAdd(ghist + q, static_cast<unsigned long long>(hist[q]));
 }
 
+// Top-k tally of row i (one workgroup, 4096 columns per step): ties
+// (key == kstar) into tcnt[i]; stored entries kept without a tie slot
+// (key > kstar, or preserved; nonzero) times 2, plus 1 if column 0 is a tie of
+// a row i > 0 under preserve, into gcnt[i].  k_topk_count turns these and the
+// tie offsets into the stored count of the row.
 template <typename TM>
-__global__ __launch_bounds__(256) void k_row_ties(const TM* __restrict__ M, long ldm, ElemCtx c,
-                                                  unsigned long long kstar, int* __restrict__ cnt)
+__global__ __launch_bounds__(256) void k_row_tally(const TM* __restrict__ M, long ldm, ElemCtx c,
+                                                   unsigned long long kstar, int preserve,
+                                                   int* __restrict__ tcnt, int* __restrict__ gcnt)
 {
     using E = Elem<TM>;
     const int i = blockIdx.x;
     const TM* row = M + static_cast<long>(i) * ldm;
     const double ai = c.alpha[i];
-    long long nt = 0;
-    for (int j = threadIdx.x; j < c.m - 1; j += 256) {
-        bool z;
-        const TM t = E::val(ai + c.beta[j], row[j], c.k, c.reg, z);
-        nt += (static_cast<unsigned long long>(E::key(t)) == kstar) ? 1 : 0;
+    const int m1 = c.m - 1;
+    int nt = 0, ng = 0, tie0 = 0;
+    for (int s0 = threadIdx.x * 4; s0 < m1; s0 += 4096) {
+        TM v[4][4];
+        double b4[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int cu = s0 + 1024 * u;
+            if (cu < m1) {
+                load4(row, cu, c.m, v[u]);
+                load4(c.beta, cu, c.m, b4[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int j = s0 + 1024 * u + q;
+                if (j >= m1) break;
+                bool z;
+                const TM t = E::val(ai + b4[u][q], v[u][q], c.k, c.reg, z);
+                const unsigned long long key = static_cast<unsigned long long>(E::key(t));
+                const bool base = key > kstar || (preserve && (i == 0 || j == 0));
+                const bool tie = key == kstar;
+                nt += tie ? 1 : 0;
+                ng += (base && t != TM(0)) ? 1 : 0;
+                if (j == 0 && tie && preserve && i > 0) tie0 = 1;
+            }
+        }
     }
-    nt = block_sum_ll(nt);
-    if (threadIdx.x == 0) cnt[i] = static_cast<int>(nt);
+    int tot_t, tot_g;
+    (void)block_excl_scan_int(nt, &tot_t);
+    (void)block_excl_scan_int(ng, &tot_g);
+    if (threadIdx.x == 0) {
+        tcnt[i] = tot_t;
+        gcnt[i] = 2 * tot_g + tie0;
+    }
 }
 
-// ---------------------------------------------------------------------------
-// Kept-set count and write
-// ---------------------------------------------------------------------------
+// Stored entries of row i under the top-k rule: the base count plus the
+// nonzero ties it takes, the ties of the row having ranks tie_off[i] ..
+// tie_off[i] + ties - 1 (row-major) and the first tie_take ranks being taken.
+// A tie in column 0 of a row i > 0 is preserved, so it is counted once; every
+// tie of row 0 is preserved; ties are nonzero unless kstar is the key of 0.
+__global__ void k_topk_count(const int* tcnt, const int* __restrict__ gcnt,
+                             const long long* __restrict__ tie_off, long long tie_take,
+                             unsigned long long kstar, int preserve, int n, int* cnt)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const long long rem = tie_take - tie_off[i];
+    const int nt = tcnt[i];
+    const long long taken = rem < 0 ? 0 : (rem < nt ? rem : nt);
+    long long extra = 0;
+    if (kstar != 0ull && !(preserve && i == 0))
+        extra = taken - (((gcnt[i] & 1) && taken > 0) ? 1 : 0);
+    cnt[i] = (gcnt[i] >> 1) + static_cast<int>(extra);
+}
 
-// Row kernels walk row i in 256-column chunks; for the top-k rule the rank of
-// a tie is tie_off[i] + ties earlier in the row, so ties are taken in
-// row-major order.
-template <typename TM, bool WRITE>
-__global__ __launch_bounds__(256) void k_row_kept(const TM* __restrict__ M, long ldm, ElemCtx c,
-                                                  KeepRule rule, int* __restrict__ cnt,
-                                                  const int* __restrict__ rowptr,
-                                                  int* __restrict__ colind,
-                                                  double* __restrict__ val)
+// Ordered CSR write of row i under either rule (one workgroup, 1024 columns
+// per step, the next step's loads issued before the current step's scan).
+// Top-k ties are taken in row-major order: a tie of chunk rank r is kept when
+// tie_off[i] + ties of earlier chunks + r < tie_take.
+template <typename TM>
+__global__ __launch_bounds__(256) void k_row_write(const TM* __restrict__ M, long ldm, ElemCtx c,
+                                                   KeepRule rule, const int* __restrict__ rowptr,
+                                                   int* __restrict__ colind,
+                                                   double* __restrict__ val)
 {
     using E = Elem<TM>;
     const int i = blockIdx.x;
@@ -416,43 +536,86 @@ __global__ __launch_bounds__(256) void k_row_kept(const TM* __restrict__ M, long
     const double ai = c.alpha[i];
     const double inv_reg = 1.0 / c.reg;
     const int m1 = c.m - 1;
+    const bool topk = rule.mode == 1;
+    const double r_i = topk ? 0.0 : rule.rho[i];
+    const long long rem0 = topk ? rule.tie_take - rule.tie_off[i] : 0;
     long long tie_seen = 0;
     long long written = 0;
-    const double r_i = (rule.mode == 0) ? rule.rho[i] : 0.0;
-    const long long toff = (rule.mode == 1) ? rule.tie_off[i] : 0;
-    for (int j0 = 0; j0 < m1; j0 += 256) {
-        const int j = j0 + threadIdx.x;
-        bool kept = false;
-        TM t = TM(0);
-        bool tie = false;
-        if (j < m1) {
-            bool z;
-            t = E::val(ai + c.beta[j], row[j], c.k, c.reg, z);
-            if (rule.mode == 0) {
-                kept = (static_cast<double>(t) >= fmin(rule.tau[j], r_i)) || z;
-            } else {
-                const unsigned long long key = static_cast<unsigned long long>(E::key(t));
-                kept = key > rule.kstar || (rule.preserve && (i == 0 || j == 0));
-                tie = (key == rule.kstar);
+    const long base_pos = rowptr[i];
+    const long end_pos = rowptr[i + 1];   // writes never pass the counted end
+    TM v[4] = {TM(0), TM(0), TM(0), TM(0)};
+    double b4[4] = {0.0, 0.0, 0.0, 0.0};
+    double t4[4] = {0.0, 0.0, 0.0, 0.0};
+    int cj = threadIdx.x * 4;
+    if (cj < m1) {
+        load4(row, cj, c.m, v);
+        load4(c.beta, cj, c.m, b4);
+        if (!topk) load4(rule.tau, cj, m1, t4);
+    }
+    for (int s0 = 0; s0 < m1; s0 += 1024, cj += 1024) {
+        TM vn[4] = {TM(0), TM(0), TM(0), TM(0)};
+        double bn[4] = {0.0, 0.0, 0.0, 0.0};
+        double tn[4] = {0.0, 0.0, 0.0, 0.0};
+        if (cj + 1024 < m1) {
+            load4(row, cj + 1024, c.m, vn);
+            load4(c.beta, cj + 1024, c.m, bn);
+            if (!topk) load4(rule.tau, cj + 1024, m1, tn);
+        }
+        TM t[4];
+        bool base[4], tie[4];
+        int ntie = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = cj + q;
+            base[q] = false;
+            tie[q] = false;
+            t[q] = TM(0);
+            if (j < m1) {
+                bool z;
+                t[q] = E::val(ai + b4[q], v[q], c.k, c.reg, z);
+                if (topk) {
+                    const unsigned long long key = static_cast<unsigned long long>(E::key(t[q]));
+                    base[q] = key > rule.kstar || (rule.preserve && (i == 0 || j == 0));
+                    tie[q] = key == rule.kstar;
+                    ntie += tie[q] ? 1 : 0;
+                } else {
+                    base[q] = (static_cast<double>(t[q]) >= fmin(t4[q], r_i)) || z;
+                }
             }
         }
-        if (rule.mode == 1) {
-            int ntie;
-            const int tp = block_flag_prefix(tie, &ntie);
-            if (tie && toff + tie_seen + tp < rule.tie_take) kept = true;
-            tie_seen += ntie;
+        if (topk) {
+            int tot;
+            int r = block_excl_scan_int(ntie, &tot);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (tie[q]) {
+                    if (tie_seen + r < rem0) base[q] = true;
+                    r++;
+                }
+            }
+            tie_seen += tot;
         }
-        const bool stored = kept && t != TM(0);
-        int nst;
-        const int pos = block_flag_prefix(stored, &nst);
-        if (WRITE && stored) {
-            const long q = static_cast<long>(rowptr[i]) + written + pos;
-            colind[q] = j;
-            val[q] = static_cast<double>(t) * inv_reg;
+        int ns = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) ns += (base[q] && t[q] != TM(0)) ? 1 : 0;
+        int tot;
+        long q0 = base_pos + written + block_excl_scan_int(ns, &tot);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (base[q] && t[q] != TM(0) && q0 < end_pos) {
+                colind[q0] = cj + q;
+                val[q0] = static_cast<double>(t[q]) * inv_reg;
+                q0++;
+            }
         }
-        written += nst;
+        written += tot;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            v[q] = vn[q];
+            b4[q] = bn[q];
+            t4[q] = tn[q];
+        }
     }
-    if (!WRITE && threadIdx.x == 0) cnt[i] = static_cast<int>(written);
 }
 
 // ---------------------------------------------------------------------------
@@ -471,7 +634,10 @@ __global__ __launch_bounds__(256) void k_csc_count(const int* __restrict__ rowpt
     const int r0 = blockIdx.x * rows_per_block;
     const int r1 = (r0 + rows_per_block < n) ? r0 + rows_per_block : n;
     if (r0 >= r1) return;
-    for (int k = rowptr[r0] + threadIdx.x; k < rowptr[r1]; k += 256) atomicAdd(&mine[colind[k]], 1);
+    for (int k = rowptr[r0] + threadIdx.x; k < rowptr[r1]; k += 256) {
+        const int j = colind[k];
+        if (static_cast<unsigned>(j) < static_cast<unsigned>(m1)) atomicAdd(&mine[j], 1);
+    }
 }
 
 // Per column j: off[b][j] = entries of column j in workgroups before b
@@ -516,11 +682,135 @@ __global__ __launch_bounds__(256) void k_csc_scatter(const int* __restrict__ row
     for (int i = r0; i < r1; i++) {
         for (int k = rowptr[i] + threadIdx.x; k < rowptr[i + 1]; k += 256) {
             const int j = colind[k];
+            if (static_cast<unsigned>(j) >= static_cast<unsigned>(m1)) continue;
             const int q = colptr[j] + mine[j];
             mine[j] += 1;
             rowind[q] = i;
             cval[q] = rval[k];
         }
+        __syncthreads();
+    }
+}
+
+// True when row i of the CSR stores column j (binary search; the columns of
+// a row are ascending).
+__device__ __forceinline__ bool csr_has(const int* __restrict__ rowptr,
+                                        const int* __restrict__ colind, int i, int j)
+{
+    int lo = rowptr[i], hi = rowptr[i + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int cm = colind[mid];
+        if (cm == j) return true;
+        if (cm < j)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return false;
+}
+
+// CSC straight from M for dense keep sets.  Workgroup (x, y) covers the
+// column strip [256 x, 256 x + 256) and the count blocks [y B, y B + B) of
+// rows, in tiles of 32 rows: thread t evaluates column 256 x + t on the tile
+// (coalesced row segments), keeps a 32-bit mask of its stored rows and their
+// values in LDS; a workgroup scan of the per-column counts then lets
+// consecutive threads write consecutive entries of each column's run, so the
+// stores are coalesced.  Column j's run starts at colptr[j] + off[y B][j].
+// The keep rule is the CSR's; a top-k tie is kept exactly when the CSR
+// stores it.
+template <typename TM>
+__global__ __launch_bounds__(256) void k_csc_strip(const TM* __restrict__ M, long ldm, ElemCtx c,
+                                                   KeepRule rule, const int* __restrict__ rowptr,
+                                                   const int* __restrict__ colind,
+                                                   int rows_per_block, int blocks_per_wg,
+                                                   const int* __restrict__ off,
+                                                   const int* __restrict__ colptr,
+                                                   int* __restrict__ rowind,
+                                                   double* __restrict__ cval)
+{
+    using E = Elem<TM>;
+    constexpr int RT = 32;
+    __shared__ TM tv[RT][256];
+    __shared__ unsigned int msk[256];
+    __shared__ int base[256];
+    __shared__ int posb[256];
+    __shared__ int endb[256];
+    const int m1 = c.m - 1;
+    const int tid = threadIdx.x;
+    const int j = blockIdx.x * 256 + tid;
+    const int b0 = blockIdx.y * blocks_per_wg;
+    const long r0 = static_cast<long>(b0) * rows_per_block;
+    const long re = r0 + static_cast<long>(blocks_per_wg) * rows_per_block;
+    const int r1 = static_cast<int>(re < c.n ? re : c.n);
+    const bool topk = rule.mode == 1;
+    const bool col_ok = j < m1;
+    int pos = col_ok ? colptr[j] + off[static_cast<long>(b0) * m1 + j] : 0;
+    endb[tid] = col_ok ? colptr[j + 1] : 0;
+    const double bj = col_ok ? c.beta[j] : 0.0;
+    const double tj = (col_ok && !topk) ? rule.tau[j] : 0.0;
+    const double inv_reg = 1.0 / c.reg;
+    // every thread runs every tile: the barriers below are workgroup-wide
+    for (int i0 = static_cast<int>(r0); i0 < r1; i0 += RT) {
+        unsigned int mask = 0u;
+        if (col_ok) {
+#pragma unroll
+            for (int u0 = 0; u0 < RT; u0 += 8) {
+                TM v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    v[u] = (i0 + u0 + u < r1) ? M[static_cast<long>(i0 + u0 + u) * ldm + j]
+                                              : TM(0);
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int i = i0 + u0 + u;
+                    if (i >= r1) break;
+                    bool z;
+                    const TM t = E::val(c.alpha[i] + bj, v[u], c.k, c.reg, z);
+                    if (t == TM(0)) continue;
+                    bool kept;
+                    if (topk) {
+                        const unsigned long long key = static_cast<unsigned long long>(E::key(t));
+                        kept = key > rule.kstar || (rule.preserve && (i == 0 || j == 0));
+                        if (!kept && key == rule.kstar) kept = csr_has(rowptr, colind, i, j);
+                    } else {
+                        kept = static_cast<double>(t) >= fmin(tj, rule.rho[i]);
+                    }
+                    if (kept) {
+                        mask |= 1u << (u0 + u);
+                        tv[u0 + u][tid] = t;
+                    }
+                }
+            }
+        }
+        const int cnt = __popc(mask);
+        int tot;
+        const int ex = block_excl_scan_int(cnt, &tot);
+        msk[tid] = mask;
+        base[tid] = ex;
+        posb[tid] = pos;
+        __syncthreads();
+        for (int e = tid; e < tot; e += 256) {
+            // the column: the last one whose run starts at or before e
+            int lo = 0, hi = 255;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (base[mid] <= e)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int k = e - base[lo];
+            unsigned int mm = msk[lo];
+            for (int q = 0; q < k; q++) mm &= mm - 1u;
+            const int rho = __ffs(mm) - 1;
+            const int q = posb[lo] + k;
+            if (q < endb[lo]) {
+                rowind[q] = i0 + rho;
+                cval[q] = static_cast<double>(tv[rho][lo]) * inv_reg;
+            }
+        }
+        pos += cnt;
         __syncthreads();
     }
 }
@@ -699,27 +989,24 @@ void topk_select(int bulk, const void* M, long ldm, const ElemCtx& c, long long 
     *n_greater = above;
 }
 
-void launch_row_ties(int bulk, const void* M, long ldm, const ElemCtx& c,
-                     unsigned long long kstar, int* cnt, hipStream_t s)
+void launch_row_tally(int bulk, const void* M, long ldm, const ElemCtx& c,
+                      unsigned long long kstar, int preserve, int* tcnt, int* gcnt,
+                      hipStream_t s)
 {
     if (bulk == 0)
-        hipLaunchKernelGGL(k_row_ties<float>, dim3(c.n), dim3(256), 0, s, cM<float>(M), ldm, c,
-                           kstar, cnt);
+        hipLaunchKernelGGL(k_row_tally<float>, dim3(c.n), dim3(256), 0, s, cM<float>(M), ldm, c,
+                           kstar, preserve, tcnt, gcnt);
     else
-        hipLaunchKernelGGL(k_row_ties<double>, dim3(c.n), dim3(256), 0, s, cM<double>(M), ldm, c,
-                           kstar, cnt);
+        hipLaunchKernelGGL(k_row_tally<double>, dim3(c.n), dim3(256), 0, s, cM<double>(M), ldm,
+                           c, kstar, preserve, tcnt, gcnt);
     KERNEL_CHECK();
 }
 
-void launch_row_count(int bulk, const void* M, long ldm, const ElemCtx& c, const KeepRule& r,
-                      int* cnt, hipStream_t s)
+void launch_topk_count(const int* tcnt, const int* gcnt, const KeepRule& r, int n, int* cnt,
+                       hipStream_t s)
 {
-    if (bulk == 0)
-        hipLaunchKernelGGL((k_row_kept<float, false>), dim3(c.n), dim3(256), 0, s, cM<float>(M),
-                           ldm, c, r, cnt, nullptr, nullptr, nullptr);
-    else
-        hipLaunchKernelGGL((k_row_kept<double, false>), dim3(c.n), dim3(256), 0, s,
-                           cM<double>(M), ldm, c, r, cnt, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_topk_count, dim3(ceil_div(n, 256)), dim3(256), 0, s, tcnt, gcnt,
+                       r.tie_off, r.tie_take, r.kstar, r.preserve, n, cnt);
     KERNEL_CHECK();
 }
 
@@ -727,16 +1014,18 @@ void launch_row_write(int bulk, const void* M, long ldm, const ElemCtx& c, const
                       const int* rowptr, int* colind, double* val, hipStream_t s)
 {
     if (bulk == 0)
-        hipLaunchKernelGGL((k_row_kept<float, true>), dim3(c.n), dim3(256), 0, s, cM<float>(M),
-                           ldm, c, r, nullptr, rowptr, colind, val);
+        hipLaunchKernelGGL(k_row_write<float>, dim3(c.n), dim3(256), 0, s, cM<float>(M), ldm, c,
+                           r, rowptr, colind, val);
     else
-        hipLaunchKernelGGL((k_row_kept<double, true>), dim3(c.n), dim3(256), 0, s,
-                           cM<double>(M), ldm, c, r, nullptr, rowptr, colind, val);
+        hipLaunchKernelGGL(k_row_write<double>, dim3(c.n), dim3(256), 0, s, cM<double>(M), ldm,
+                           c, r, rowptr, colind, val);
     KERNEL_CHECK();
 }
 
-void launch_csr_to_csc(const SparseHessian& H, int* work, int rows_per_block, int* colcnt,
-                       int* colptr, long long* total, int* rowind, double* cval, hipStream_t s)
+void launch_build_csc(int bulk, const void* M, long ldm, const ElemCtx& c, const KeepRule& r,
+                      const SparseHessian& H, bool from_M, int* work, int rows_per_block,
+                      int* colcnt, int* colptr, long long* total, int* rowind, double* cval,
+                      hipStream_t s)
 {
     const int n = H.n, m1 = H.m - 1;
     if (m1 <= 0) return;
@@ -748,8 +1037,22 @@ void launch_csr_to_csc(const SparseHessian& H, int* work, int rows_per_block, in
                        colcnt);
     KERNEL_CHECK();
     launch_scan(colcnt, m1, colptr, total, s);
-    hipLaunchKernelGGL(k_csc_scatter, dim3(nb), dim3(256), 0, s, H.rowptr.p, H.colind.p,
-                       H.rval.p, n, rows_per_block, m1, colptr, work, rowind, cval);
+    if (!from_M) {
+        hipLaunchKernelGGL(k_csc_scatter, dim3(nb), dim3(256), 0, s, H.rowptr.p, H.colind.p,
+                           H.rval.p, n, rows_per_block, m1, colptr, work, rowind, cval);
+    } else {
+        // about 512 rows per workgroup: long front-to-back runs per column
+        const int bpw = std::max(1, 512 / rows_per_block);
+        const dim3 grid(ceil_div(m1, 256), ceil_div(nb, bpw));
+        if (bulk == 0)
+            hipLaunchKernelGGL(k_csc_strip<float>, grid, dim3(256), 0, s, cM<float>(M), ldm, c, r,
+                               H.rowptr.p, H.colind.p, rows_per_block, bpw, work, colptr, rowind,
+                               cval);
+        else
+            hipLaunchKernelGGL(k_csc_strip<double>, grid, dim3(256), 0, s, cM<double>(M), ldm, c,
+                               r, H.rowptr.p, H.colind.p, rows_per_block, bpw, work, colptr,
+                               rowind, cval);
+    }
     KERNEL_CHECK();
 }
 

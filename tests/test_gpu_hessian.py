@@ -118,6 +118,63 @@ def test_topk_rule(gpu_lib, bulk, density):
     assert np.all(Sg[0, :] != 0) and np.all(Sg[:, 0] != 0)
 
 
+def topk_ties_expected(Mq, reg, density):
+    """Kept set of the top-k rule with ties at the k-th value taken in
+    row-major order (preserve on), for T = exp(-M / reg) at gamma = 0 with M
+    on a few levels, so that whole levels tie exactly."""
+    n, m = Mq.shape
+    B = Mq[:, :-1]
+    k = int(n * (m - 1) * density)
+    kept = np.zeros(B.shape, bool)
+    if k > 0:
+        levels = np.unique(B)            # ascending M = descending T
+        before = 0
+        for lv in levels:
+            cnt = int(np.sum(B == lv))
+            if before + cnt >= k:
+                kept |= B < lv
+                take = k - before
+                tie = np.flatnonzero((B == lv).ravel())[:take]   # row-major ranks
+                kept.ravel()[tie] = True
+                break
+            before += cnt
+    kept[0, :] = True
+    kept[:, 0] = True
+    return kept
+
+
+@pytest.mark.parametrize("bulk", ["fp64", "fp32"])
+@pytest.mark.parametrize("n,m,density", [(150, 121, 0.3), (400, 300, 0.005), (64, 50, 1.0)])
+def test_topk_rule_ties(gpu_lib, bulk, n, m, density):
+    # four cost levels: the k-th value falls inside a class of exact ties;
+    # the sparse case builds the CSC by the counting sort, the dense ones by
+    # column strips that look ties up in the CSR
+    rng = np.random.default_rng(n + m)
+    Mq = 0.25 * rng.integers(0, 4, size=(n, m)).astype(np.float64)
+    a = np.full(n, 1.0 / n)
+    b = np.full(m, 1.0 / m)
+    reg = 0.5
+    Mg = Mq.astype(np.float32) if bulk == "fp32" else Mq
+    gamma = np.zeros(n + m - 1)
+    with gpu_lib.Problem(Mg, a, b, reg) as P:
+        nnz = P.sparsified_hess_with_density(gamma, density)
+        H = P.hess_export()
+    kept = topk_ties_expected(Mq, reg, density)
+    Sg = csc_dense(n, m, H["colptr"], H["rowind"], H["cval"])
+    assert nnz == int(kept.sum())
+    assert np.array_equal(Sg != 0, kept)
+    T = np.exp(-Mq[:, :-1] / reg) / reg
+    np.testing.assert_allclose(Sg[kept], T[kept], rtol=2e-6 if bulk == "fp32" else 1e-13)
+    Sr = np.zeros((n, m - 1))
+    for i in range(n):
+        sl = slice(H["rowptr"][i], H["rowptr"][i + 1])
+        assert np.all(np.diff(H["colind"][sl]) > 0)
+        Sr[i, H["colind"][sl]] = H["rval"][sl]
+    assert np.array_equal(Sr, Sg)
+    for j in range(m - 1):
+        assert np.all(np.diff(H["rowind"][H["colptr"][j]:H["colptr"][j + 1]]) > 0)
+
+
 @pytest.mark.parametrize("bulk", ["fp64", "fp32"])
 def test_hessian_products_and_cg(gpu_lib, bulk):
     n, m, reg = 180, 140, 0.05
