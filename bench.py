@@ -159,30 +159,43 @@ def bench_differentiable(n, d, reg, tol, steps):
         M = sqeuclid(X, Y)
     torch.cuda.synchronize()
     cost_ms = (time.perf_counter() - t0) / 20 * 1e3
-    # differentiable form: loss forward (cost + solve + adjoint) and backward
-    info = {}
-    L = sinkhorn_loss(X, Y, reg=reg, tol=tol, max_iter=200000, info=info)
-    L.backward()
-    torch.cuda.synchronize()
-    fw = bw = 0.0
-    for _ in range(steps):
-        X.grad = None
-        Y.grad = None
-        t0 = time.perf_counter()
-        L = sinkhorn_loss(X, Y, reg=reg, tol=tol, max_iter=200000, info=info)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
+    # differentiable form: loss forward (cost + solve + adjoint) and backward,
+    # with the BCD solver and with SSNS for the forward solve
+    def timed(solver):
+        info = {}
+        kw = dict(reg=reg, tol=tol, max_iter=200000, solver=solver, info=info)
+        L = sinkhorn_loss(X, Y, **kw)
         L.backward()
         torch.cuda.synchronize()
-        fw += t1 - t0
-        bw += time.perf_counter() - t1
+        fw = bw = 0.0
+        for _ in range(steps):
+            X.grad = None
+            Y.grad = None
+            t0 = time.perf_counter()
+            L = sinkhorn_loss(X, Y, **kw)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            L.backward()
+            torch.cuda.synchronize()
+            fw += t1 - t0
+            bw += time.perf_counter() - t1
+        return L, info, fw / steps * 1e3, bw / steps * 1e3
+
+    L, info, fw_ms, bw_ms = timed("bcd")
     res = {"n": n, "m": n, "d": d, "reg": reg, "tol": tol, "niter": info.get("niter"),
            "loss": L.item(), "cost_kernel_ms": cost_ms,
            "cost_kernel_GBps": 4.0 * n * n / (cost_ms * 1e-3) / 1e9,
-           "forward_ms": fw / steps * 1e3, "backward_ms": bw / steps * 1e3,
+           "forward_ms": fw_ms, "backward_ms": bw_ms,
            "plain_solve_ms": plain * 1e3,
            "adjoint_cg_iters": info.get("cg_iters"), "adjoint_residual": info.get("cg_err"),
            "grad_finite": bool(torch.isfinite(X.grad.float()).all().item())}
+    gX_bcd = X.grad.float().clone()
+    L2, info2, fw2, bw2 = timed("ssns")
+    res["ssns"] = {"niter": info2.get("niter"), "mar_err": info2.get("mar_err"),
+                   "loss": L2.item(), "forward_ms": fw2, "backward_ms": bw2,
+                   "adjoint_cg_iters": info2.get("cg_iters"),
+                   "grad_rel_diff_vs_bcd": float((X.grad.float() - gX_bcd).norm().item() /
+                                                 gX_bcd.norm().item())}
     del M
     torch.cuda.empty_cache()
     return res

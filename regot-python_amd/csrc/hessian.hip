@@ -350,16 +350,18 @@ static void build_storage(regot_problem* p, SparseHessian& H, const ElemCtx& c,
     H.cval.ensure(nnz);
     launch_row_write(p->bulk, p->dM, p->ldm, c, rule, H.rowptr, H.colind, H.rval, s);
     // CSC: the transpose of the CSR just written.  Below 2% density a
-    // counting sort of the nnz entries; above, one pass over M by column strips
-    // (the sort's scattered writes cost more than re-reading M there).  At most
-    // 512 row blocks: the offset table stays <= 2 KiB per column.
+    // counting sort of the nnz entries; above, a pass over M' by the ordered
+    // row writer when M' is there (threshold rule), else over M by column
+    // strips (the sort's scattered writes cost more than re-reading M there).
+    // At most 512 row blocks: the offset table stays <= 2 KiB per column.
     const int rows_per_block = std::max(32, ceil_div(n, 512));
     H.ccnt.ensure(m);
     H.colptr.ensure(m);
     H.csc_work.ensure(static_cast<size_t>(ceil_div(n, rows_per_block)) * std::max(m - 1, 1));
-    const bool from_M = static_cast<double>(nnz) > 0.02 * n * static_cast<double>(m - 1);
-    launch_build_csc(p->bulk, p->dM, p->ldm, c, rule, H, from_M, H.csc_work, rows_per_block,
-                     H.ccnt, H.colptr, H.ltmp.p + 1, H.rowind, H.cval, s);
+    const bool dense = static_cast<double>(nnz) > 0.02 * n * static_cast<double>(m - 1);
+    const int how = !dense ? 0 : (rule.mode == 0 && p->dMT) ? 2 : 1;
+    launch_build_csc(p->bulk, p->dM, p->ldm, p->dMT, p->ldt, c, rule, H, how, H.csc_work,
+                     rows_per_block, H.ccnt, H.colptr, H.ltmp.p + 1, H.rowind, H.cval, s);
 }
 
 static void prepare(regot_problem* p, SparseHessian& H, const double* d_gamma, const double* d_grad)

@@ -97,14 +97,15 @@ void launch_row_write(int bulk, const void* M, long ldm, const ElemCtx& c, const
 
 // CSC (colptr, rowind, cval) of the kept set whose CSR H holds: column
 // counts per block of rows_per_block rows from the CSR (work = (n /
-// rows_per_block) x (m-1) ints, colcnt = m-1 ints), then either a stable
-// counting sort of the CSR entries (sparse sets) or, with from_M, a pass over
-// M by column strips under the same rule r (dense sets).  The entries of a
-// column come out in row order either way.
-void launch_build_csc(int bulk, const void* M, long ldm, const ElemCtx& c, const KeepRule& r,
-                      const SparseHessian& H, bool from_M, int* work, int rows_per_block,
-                      int* colcnt, int* colptr, long long* total, int* rowind, double* cval,
-                      hipStream_t s);
+// rows_per_block) x (m-1) ints, colcnt = m-1 ints), then, by `how`: 0 a
+// stable counting sort of the CSR entries (sparse sets); 1 a pass over M by
+// column strips under the same rule r (dense sets); 2 the ordered row writer
+// on M' (threshold rule, dense sets, M' present).  The entries of a column
+// come out in row order every way.
+void launch_build_csc(int bulk, const void* M, long ldm, const void* MT, long ldt,
+                      const ElemCtx& c, const KeepRule& r, const SparseHessian& H, int how,
+                      int* work, int rows_per_block, int* colcnt, int* colptr, long long* total,
+                      int* rowind, double* cval, hipStream_t s);
 
 // Exclusive scan of count (int) into out (count + 1 entries, int64 total check).
 void launch_scan(const int* cnt, int count, int* out, long long* total, hipStream_t s);

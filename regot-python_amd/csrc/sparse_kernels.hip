@@ -418,6 +418,10 @@ __global__ __launch_bounds__(256) void k_topk_hist(const TM* __restrict__ M, lon
     const long r0 = static_cast<long>(blockIdx.x) * rows_per_block;
     const long r1 = (r0 + rows_per_block < c.n) ? r0 + rows_per_block : c.n;
     const K dmask = (K(1) << width) - 1;
+    // run-length aggregation: T is smooth, so a thread's consecutive keys
+    // mostly share a bucket (all of them in the first pass, whose digit is
+    // nearly the exponent); one LDS add per run instead of one per entry
+    unsigned int run_b = 0u, run_c = 0u;
     for (int c0 = threadIdx.x * 4; c0 < m1; c0 += 1024) {
         double b4[4];
         load4(c.beta, c0, c.m, b4);
@@ -436,12 +440,20 @@ __global__ __launch_bounds__(256) void k_topk_hist(const TM* __restrict__ M, lon
                     bool z;
                     const TM t = E::val(ai + b4[q], v[u][q], c.k, c.reg, z);
                     const K key = E::key(t);
-                    if (first || (static_cast<unsigned long long>(key >> (shift + width)) == prefix))
-                        atomicAdd(&hist[(key >> shift) & dmask], 1u);
+                    if (first || (static_cast<unsigned long long>(key >> (shift + width)) == prefix)) {
+                        const unsigned int bk = static_cast<unsigned int>((key >> shift) & dmask);
+                        if (bk != run_b) {
+                            if (run_c) atomicAdd(&hist[run_b], run_c);
+                            run_b = bk;
+                            run_c = 0u;
+                        }
+                        run_c++;
+                    }
                 }
             }
         }
     }
+    if (run_c) atomicAdd(&hist[run_b], run_c);
     __syncthreads();
     for (int q = threadIdx.x; q < (1 << width); q += 256)
         if (hist[q]) atomicAdd(ghist + q, static_cast<unsigned long long>(hist[q]));
@@ -523,7 +535,9 @@ __global__ void k_topk_count(const int* tcnt, const int* __restrict__ gcnt,
 // Ordered CSR write of row i under either rule (one workgroup, 1024 columns
 // per step, the next step's loads issued before the current step's scan).
 // Top-k ties are taken in row-major order: a tie of chunk rank r is kept when
-// tie_off[i] + ties of earlier chunks + r < tie_take.
+// tie_off[i] + ties of earlier chunks + r < tie_take.  Only columns < m - 1
+// are read (of M, beta and tau), so the kernel also runs on M' with the roles
+// of alpha / beta and tau / rho exchanged, writing the CSC.
 template <typename TM>
 __global__ __launch_bounds__(256) void k_row_write(const TM* __restrict__ M, long ldm, ElemCtx c,
                                                    KeepRule rule, const int* __restrict__ rowptr,
@@ -548,8 +562,8 @@ __global__ __launch_bounds__(256) void k_row_write(const TM* __restrict__ M, lon
     double t4[4] = {0.0, 0.0, 0.0, 0.0};
     int cj = threadIdx.x * 4;
     if (cj < m1) {
-        load4(row, cj, c.m, v);
-        load4(c.beta, cj, c.m, b4);
+        load4(row, cj, m1, v);
+        load4(c.beta, cj, m1, b4);
         if (!topk) load4(rule.tau, cj, m1, t4);
     }
     for (int s0 = 0; s0 < m1; s0 += 1024, cj += 1024) {
@@ -557,8 +571,8 @@ __global__ __launch_bounds__(256) void k_row_write(const TM* __restrict__ M, lon
         double bn[4] = {0.0, 0.0, 0.0, 0.0};
         double tn[4] = {0.0, 0.0, 0.0, 0.0};
         if (cj + 1024 < m1) {
-            load4(row, cj + 1024, c.m, vn);
-            load4(c.beta, cj + 1024, c.m, bn);
+            load4(row, cj + 1024, m1, vn);
+            load4(c.beta, cj + 1024, m1, bn);
             if (!topk) load4(rule.tau, cj + 1024, m1, tn);
         }
         TM t[4];
@@ -1022,10 +1036,10 @@ void launch_row_write(int bulk, const void* M, long ldm, const ElemCtx& c, const
     KERNEL_CHECK();
 }
 
-void launch_build_csc(int bulk, const void* M, long ldm, const ElemCtx& c, const KeepRule& r,
-                      const SparseHessian& H, bool from_M, int* work, int rows_per_block,
-                      int* colcnt, int* colptr, long long* total, int* rowind, double* cval,
-                      hipStream_t s)
+void launch_build_csc(int bulk, const void* M, long ldm, const void* MT, long ldt,
+                      const ElemCtx& c, const KeepRule& r, const SparseHessian& H, int how,
+                      int* work, int rows_per_block, int* colcnt, int* colptr, long long* total,
+                      int* rowind, double* cval, hipStream_t s)
 {
     const int n = H.n, m1 = H.m - 1;
     if (m1 <= 0) return;
@@ -1037,9 +1051,26 @@ void launch_build_csc(int bulk, const void* M, long ldm, const ElemCtx& c, const
                        colcnt);
     KERNEL_CHECK();
     launch_scan(colcnt, m1, colptr, total, s);
-    if (!from_M) {
+    if (how == 0) {
         hipLaunchKernelGGL(k_csc_scatter, dim3(nb), dim3(256), 0, s, H.rowptr.p, H.colind.p,
                            H.rval.p, n, rows_per_block, m1, colptr, work, rowind, cval);
+    } else if (how == 2) {
+        // threshold rule on M': row j of M' is column j of M; the rule is
+        // symmetric in (tau_j, rho_i) and alpha_i + beta_j is exact either way
+        ElemCtx ct = c;
+        ct.alpha = c.beta;
+        ct.beta = c.alpha;
+        ct.n = m1;
+        ct.m = n + 1;
+        KeepRule rt = r;
+        rt.tau = r.rho;
+        rt.rho = r.tau;
+        if (bulk == 0)
+            hipLaunchKernelGGL(k_row_write<float>, dim3(m1), dim3(256), 0, s, cM<float>(MT), ldt,
+                               ct, rt, colptr, rowind, cval);
+        else
+            hipLaunchKernelGGL(k_row_write<double>, dim3(m1), dim3(256), 0, s, cM<double>(MT),
+                               ldt, ct, rt, colptr, rowind, cval);
     } else {
         // about 512 rows per workgroup: long front-to-back runs per column
         const int bpw = std::max(1, 512 / rows_per_block);
