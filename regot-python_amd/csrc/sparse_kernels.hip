@@ -532,8 +532,26 @@ __global__ void k_topk_count(const int* tcnt, const int* __restrict__ gcnt,
     cnt[i] = (gcnt[i] >> 1) + static_cast<int>(extra);
 }
 
-// Ordered CSR write of row i under either rule (one workgroup, 1024 columns
-// per step, the next step's loads issued before the current step's scan).
+// Exclusive prefix, over the lanes of a wave, of per-lane flag counts (four
+// flags per lane, lane-major order) by ballots; *total gets the wave's count.
+__device__ __forceinline__ int wave_excl_count4(const bool (&f)[4], int* total)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const unsigned long long b = __ballot(f[q]);
+        pre += __popcll(b & lt);
+        tot += __popcll(b);
+    }
+    *total = tot;
+    return pre;
+}
+
+// Ordered CSR write of row i under either rule: one wave per row, 256
+// columns (4 per lane) per step, positions from ballots (no LDS, no
+// barriers), the next step's loads in flight during the current step.
 // Top-k ties are taken in row-major order: a tie of chunk rank r is kept when
 // tie_off[i] + ties of earlier chunks + r < tie_take.  Only columns < m - 1
 // are read (of M, beta and tau), so the kernel also runs on M' with the roles
@@ -545,7 +563,9 @@ __global__ __launch_bounds__(256) void k_row_write(const TM* __restrict__ M, lon
                                                    double* __restrict__ val)
 {
     using E = Elem<TM>;
-    const int i = blockIdx.x;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= c.n) return;   // wave-uniform
+    const int lane = threadIdx.x & 63;
     const TM* row = M + static_cast<long>(i) * ldm;
     const double ai = c.alpha[i];
     const double inv_reg = 1.0 / c.reg;
@@ -554,30 +574,30 @@ __global__ __launch_bounds__(256) void k_row_write(const TM* __restrict__ M, lon
     const double r_i = topk ? 0.0 : rule.rho[i];
     const long long rem0 = topk ? rule.tie_take - rule.tie_off[i] : 0;
     long long tie_seen = 0;
-    long long written = 0;
+    long written = 0;
     const long base_pos = rowptr[i];
     const long end_pos = rowptr[i + 1];   // writes never pass the counted end
     TM v[4] = {TM(0), TM(0), TM(0), TM(0)};
     double b4[4] = {0.0, 0.0, 0.0, 0.0};
     double t4[4] = {0.0, 0.0, 0.0, 0.0};
-    int cj = threadIdx.x * 4;
+    int cj = lane * 4;
     if (cj < m1) {
         load4(row, cj, m1, v);
         load4(c.beta, cj, m1, b4);
         if (!topk) load4(rule.tau, cj, m1, t4);
     }
-    for (int s0 = 0; s0 < m1; s0 += 1024, cj += 1024) {
+    for (int s0 = 0; s0 < m1; s0 += 256, cj += 256) {
         TM vn[4] = {TM(0), TM(0), TM(0), TM(0)};
         double bn[4] = {0.0, 0.0, 0.0, 0.0};
         double tn[4] = {0.0, 0.0, 0.0, 0.0};
-        if (cj + 1024 < m1) {
-            load4(row, cj + 1024, m1, vn);
-            load4(c.beta, cj + 1024, m1, bn);
-            if (!topk) load4(rule.tau, cj + 1024, m1, tn);
+        if (cj + 256 < m1) {
+            load4(row, cj + 256, m1, vn);
+            load4(c.beta, cj + 256, m1, bn);
+            if (!topk) load4(rule.tau, cj + 256, m1, tn);
         }
         TM t[4];
         bool base[4], tie[4];
-        int ntie = 0;
+        bool any_tie = false;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int j = cj + q;
@@ -591,15 +611,15 @@ __global__ __launch_bounds__(256) void k_row_write(const TM* __restrict__ M, lon
                     const unsigned long long key = static_cast<unsigned long long>(E::key(t[q]));
                     base[q] = key > rule.kstar || (rule.preserve && (i == 0 || j == 0));
                     tie[q] = key == rule.kstar;
-                    ntie += tie[q] ? 1 : 0;
+                    any_tie = any_tie || tie[q];
                 } else {
                     base[q] = (static_cast<double>(t[q]) >= fmin(t4[q], r_i)) || z;
                 }
             }
         }
-        if (topk) {
+        if (topk && __ballot(any_tie) != 0ull) {
             int tot;
-            int r = block_excl_scan_int(ntie, &tot);
+            int r = wave_excl_count4(tie, &tot);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 if (tie[q]) {
@@ -609,18 +629,18 @@ __global__ __launch_bounds__(256) void k_row_write(const TM* __restrict__ M, lon
             }
             tie_seen += tot;
         }
-        int ns = 0;
+        bool st[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) ns += (base[q] && t[q] != TM(0)) ? 1 : 0;
+        for (int q = 0; q < 4; q++) st[q] = base[q] && t[q] != TM(0);
         int tot;
-        long q0 = base_pos + written + block_excl_scan_int(ns, &tot);
+        long q0 = base_pos + written + wave_excl_count4(st, &tot);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            if (base[q] && t[q] != TM(0) && q0 < end_pos) {
+            if (st[q] && q0 < end_pos) {
                 colind[q0] = cj + q;
                 val[q0] = static_cast<double>(t[q]) * inv_reg;
-                q0++;
             }
+            q0 += st[q] ? 1 : 0;
         }
         written += tot;
 #pragma unroll
@@ -1028,11 +1048,11 @@ void launch_row_write(int bulk, const void* M, long ldm, const ElemCtx& c, const
                       const int* rowptr, int* colind, double* val, hipStream_t s)
 {
     if (bulk == 0)
-        hipLaunchKernelGGL(k_row_write<float>, dim3(c.n), dim3(256), 0, s, cM<float>(M), ldm, c,
-                           r, rowptr, colind, val);
+        hipLaunchKernelGGL(k_row_write<float>, dim3(ceil_div(c.n, 4)), dim3(256), 0, s,
+                           cM<float>(M), ldm, c, r, rowptr, colind, val);
     else
-        hipLaunchKernelGGL(k_row_write<double>, dim3(c.n), dim3(256), 0, s, cM<double>(M), ldm,
-                           c, r, rowptr, colind, val);
+        hipLaunchKernelGGL(k_row_write<double>, dim3(ceil_div(c.n, 4)), dim3(256), 0, s,
+                           cM<double>(M), ldm, c, r, rowptr, colind, val);
     KERNEL_CHECK();
 }
 
@@ -1066,11 +1086,11 @@ void launch_build_csc(int bulk, const void* M, long ldm, const void* MT, long ld
         rt.tau = r.rho;
         rt.rho = r.tau;
         if (bulk == 0)
-            hipLaunchKernelGGL(k_row_write<float>, dim3(m1), dim3(256), 0, s, cM<float>(MT), ldt,
-                               ct, rt, colptr, rowind, cval);
+            hipLaunchKernelGGL(k_row_write<float>, dim3(ceil_div(m1, 4)), dim3(256), 0, s,
+                               cM<float>(MT), ldt, ct, rt, colptr, rowind, cval);
         else
-            hipLaunchKernelGGL(k_row_write<double>, dim3(m1), dim3(256), 0, s, cM<double>(MT),
-                               ldt, ct, rt, colptr, rowind, cval);
+            hipLaunchKernelGGL(k_row_write<double>, dim3(ceil_div(m1, 4)), dim3(256), 0, s,
+                               cM<double>(MT), ldt, ct, rt, colptr, rowind, cval);
     } else {
         // about 512 rows per workgroup: long front-to-back runs per column
         const int bpw = std::max(1, 512 / rows_per_block);
