@@ -1137,3 +1137,555 @@ int orc_sinkhorn_splr(int n, int m, const double* M, const double* a, const doub
     return newton_family(1, n, m, M, a, b, reg, tol, max_iter, verbose, opts, dual, obj_vals,
                          mar_errs, run_times, densities, hist_len);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Dense Newton, L-BFGS dual and APDAGD drivers                              */
+/* ------------------------------------------------------------------------ */
+
+/* Dense dual Hessian at gamma (N x N row-major, N = n + m - 1): diagonal
+ * [r; c~] / reg, off-diagonal block T[:, 0:m-1] / reg and its transpose.
+ * Restates Problem::dual_obj_grad_densehess, src/sinkhorn_problem_obj.cpp:396-418
+ * (r = grad + a, c~ = grad + b~ as there). */
+static void dense_hess(const prob_t* P, const double* grad, double* H)
+{
+    const int n = P->n, m = P->m;
+    const long N = n + m - 1;
+    memset(H, 0, sizeof(double) * (size_t)N * N);
+    for (int i = 0; i < n; i++) H[i * N + i] = (grad[i] + P->a[i]) / P->reg;
+    for (int j = 0; j < m - 1; j++)
+        H[(n + j) * N + n + j] = (grad[n + j] + P->b[j]) / P->reg;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m - 1; j++) {
+            const double v = P->T[(long)i * m + j] / P->reg;
+            H[(long)i * N + n + j] = v;
+            H[(long)(n + j) * N + i] = v;
+        }
+}
+
+/* Solve A x = rhs for symmetric positive definite A (row-major, overwritten
+ * by its Cholesky factor).  The reference uses Eigen::LLT
+ * (src/sinkhorn_newton.cpp:87-88).  Returns 0, or -2 if A is not PD. */
+static int chol_solve(long N, double* A, const double* rhs, double* x)
+{
+    for (long j = 0; j < N; j++) {
+        double s = A[j * N + j];
+        for (long k = 0; k < j; k++) s -= A[j * N + k] * A[j * N + k];
+        if (!(s > 0.0)) return -2;
+        const double d = sqrt(s);
+        A[j * N + j] = d;
+        for (long i = j + 1; i < N; i++) {
+            double t = A[i * N + j];
+            for (long k = 0; k < j; k++) t -= A[i * N + k] * A[j * N + k];
+            A[i * N + j] = t / d;
+        }
+    }
+    for (long i = 0; i < N; i++) {
+        double t = rhs[i];
+        for (long k = 0; k < i; k++) t -= A[i * N + k] * x[k];
+        x[i] = t / A[i * N + i];
+    }
+    for (long i = N - 1; i >= 0; i--) {
+        double t = x[i];
+        for (long k = i + 1; k < N; k++) t -= A[k * N + i] * x[k];
+        x[i] = t / A[i * N + i];
+    }
+    return 0;
+}
+
+int orc_newton_direction(int n, int m, const double* M, const double* a, const double* b,
+                         double reg, const double* gamma, double* direc)
+{
+    const long N = n + m - 1;
+    prob_t P = {n, m, M, a, b, reg, (double*)malloc(sizeof(double) * (size_t)n * m)};
+    double* g = (double*)malloc(sizeof(double) * N);
+    double* H = (double*)malloc(sizeof(double) * (size_t)N * N);
+    double* rhs = (double*)malloc(sizeof(double) * N);
+    orc_dual_obj_grad(n, m, M, a, b, reg, gamma, g, P.T, 1);
+    dense_hess(&P, g, H);
+    for (long k = 0; k < N; k++) rhs[k] = -g[k];
+    const int rc = chol_solve(N, H, rhs, direc);
+    free(P.T);
+    free(g);
+    free(H);
+    free(rhs);
+    return rc;
+}
+
+/* Newton's method with the dense Hessian, Cholesky direction and Armijo
+ * backtracking (theta = 0.5, kappa = 0.5, <= 20 trials).  Restates
+ * sinkhorn_newton_internal, src/sinkhorn_newton.cpp:17-129.  Returns niter, or
+ * -2 when the Hessian is not positive definite (the reference's LLT would then
+ * return a meaningless direction; the GPU driver reports an error). */
+int orc_sinkhorn_newton(int n, int m, const double* M, const double* a, const double* b,
+                        double reg, double tol, int max_iter, int verbose,
+                        double* gamma, double* obj_vals, double* mar_errs, double* run_times,
+                        int* hist_len)
+{
+    const long N = n + m - 1;
+    const double theta = 0.5, kappa = 0.5;
+    const int nlinesearch = 20;
+    prob_t P = {n, m, M, a, b, reg, (double*)malloc(sizeof(double) * (size_t)n * m)};
+    double* g = (double*)malloc(sizeof(double) * N);
+    double* H = (double*)malloc(sizeof(double) * (size_t)N * N);
+    double* rhs = (double*)malloc(sizeof(double) * N);
+    double* direc = (double*)malloc(sizeof(double) * N);
+    double* newgamma = (double*)malloc(sizeof(double) * N);
+    initial_gamma(&P, gamma);
+    double t0 = now_ms();
+    double f = orc_dual_obj_grad(n, m, M, a, b, reg, gamma, g, P.T, 1);
+    dense_hess(&P, g, H);
+    double gnorm = norm2(g, N);
+    double t1 = now_ms();
+    obj_vals[0] = f;
+    mar_errs[0] = gnorm;
+    run_times[0] = t1 - t0;
+    int h = 1, i, rc = 0;
+    for (i = 0; i < max_iter; i++) {
+        if (verbose >= 1) printf("iter = %d, objval = %g, ||grad|| = %g\n", i, f, gnorm);
+        t0 = now_ms();
+        if (gnorm < tol || !isfinite(f)) break;
+        for (long k = 0; k < N; k++) rhs[k] = -g[k];
+        if (chol_solve(N, H, rhs, direc) != 0) {
+            rc = -2;
+            break;
+        }
+        double step = 1.0;
+        const double thresh = theta * dot(g, direc, N);
+        for (int k = 0; k < nlinesearch; k++) {
+            for (long q = 0; q < N; q++) newgamma[q] = gamma[q] + step * direc[q];
+            const double newf = orc_dual_obj(n, m, M, a, b, reg, newgamma);
+            if (newf <= f + step * thresh) break;
+            step *= kappa;
+        }
+        memcpy(gamma, newgamma, sizeof(double) * N);
+        f = orc_dual_obj_grad(n, m, M, a, b, reg, gamma, g, P.T, 1);
+        dense_hess(&P, g, H);
+        gnorm = norm2(g, N);
+        t1 = now_ms();
+        obj_vals[h] = f;
+        mar_errs[h] = gnorm;
+        run_times[h] = run_times[h - 1] + (t1 - t0);
+        h++;
+    }
+    free(P.T);
+    free(g);
+    free(H);
+    free(rhs);
+    free(direc);
+    free(newgamma);
+    *hist_len = h;
+    return rc != 0 ? rc : i;
+}
+
+/* Limited-memory BFGS inverse-Hessian operator with mm correction pairs kept
+ * in a ring.  Restates BFGSMat (L-BFGS part), src/LBFGSpp/LBFGSpp/BFGSMat.h:
+ * reset :65-83, add_correction :86-97 (theta = y'y / s'y of the newest pair),
+ * apply_Hv (two-loop recursion, H0 = I / theta) :276-302. */
+typedef struct {
+    int mm, ncorr, ptr;
+    long N;
+    double theta;
+    double *S, *Y, *ys, *al;
+} lbfgs_mat;
+
+static void lbfgs_reset(lbfgs_mat* B, long N, int mm)
+{
+    B->mm = mm;
+    B->N = N;
+    B->theta = 1.0;
+    B->ncorr = 0;
+    B->ptr = mm;
+    B->S = (double*)malloc(sizeof(double) * (size_t)N * mm);
+    B->Y = (double*)malloc(sizeof(double) * (size_t)N * mm);
+    B->ys = (double*)malloc(sizeof(double) * mm);
+    B->al = (double*)malloc(sizeof(double) * mm);
+}
+
+static void lbfgs_free(lbfgs_mat* B)
+{
+    free(B->S);
+    free(B->Y);
+    free(B->ys);
+    free(B->al);
+}
+
+static void lbfgs_add(lbfgs_mat* B, const double* s, const double* y)
+{
+    const int loc = B->ptr % B->mm;
+    double* sl = B->S + (long)loc * B->N;
+    double* yl = B->Y + (long)loc * B->N;
+    memcpy(sl, s, sizeof(double) * B->N);
+    memcpy(yl, y, sizeof(double) * B->N);
+    const double ys = dot(sl, yl, B->N);
+    B->ys[loc] = ys;
+    B->theta = dot(yl, yl, B->N) / ys;
+    if (B->ncorr < B->mm) B->ncorr++;
+    B->ptr = loc + 1;
+}
+
+static void lbfgs_apply_hv(lbfgs_mat* B, const double* v, double a, double* res)
+{
+    const long N = B->N;
+    for (long k = 0; k < N; k++) res[k] = a * v[k];
+    int j = B->ptr % B->mm;
+    for (int i = 0; i < B->ncorr; i++) {
+        j = (j + B->mm - 1) % B->mm;
+        const double* sj = B->S + (long)j * N;
+        const double* yj = B->Y + (long)j * N;
+        B->al[j] = dot(sj, res, N) / B->ys[j];
+        for (long k = 0; k < N; k++) res[k] -= B->al[j] * yj[k];
+    }
+    for (long k = 0; k < N; k++) res[k] /= B->theta;
+    for (int i = 0; i < B->ncorr; i++) {
+        const double* sj = B->S + (long)j * N;
+        const double* yj = B->Y + (long)j * N;
+        const double bet = dot(yj, res, N) / B->ys[j];
+        for (long k = 0; k < N; k++) res[k] += (B->al[j] - bet) * sj[k];
+        j = (j + 1) % B->mm;
+    }
+}
+
+/* Exposed for the tests: the two-loop product H v after adding the pairs
+ * (S[k], Y[k]), k = 0..npairs-1, in order (skipping none), memory mm. */
+void orc_lbfgs_apply(long N, int mm, int npairs, const double* S, const double* Y,
+                     const double* v, double* res)
+{
+    lbfgs_mat B;
+    lbfgs_reset(&B, N, mm);
+    for (int k = 0; k < npairs; k++) lbfgs_add(&B, S + (long)k * N, Y + (long)k * N);
+    lbfgs_apply_hv(&B, v, 1.0, res);
+    lbfgs_free(&B);
+}
+
+/* Step candidate of the zoom phase: minimiser of the quadratic through
+ * (lo, f_lo), (hi, f_hi) with slope dg_lo at lo, replaced by the midpoint
+ * when not finite, outside (lo, hi) or within 1% of the interval of an end.
+ * Restates LineSearchNocedalWright::quad_interp,
+ * src/LBFGSpp/LBFGSpp/LineSearchNocedalWright.h:26-55. */
+static double nw_quad_interp(double lo, double hi, double flo, double fhi, double dglo)
+{
+    const double fdiff = fhi - flo, sdiff = hi - lo, smid = (hi + lo) / 2.0;
+    double c = fdiff * lo - smid * sdiff * dglo;
+    c = c / (fdiff - sdiff * dglo);
+    const double end_dist = fmin(fabs(c - lo), fabs(c - hi));
+    const int near_end = end_dist < 0.01 * fabs(sdiff);
+    const int bisect = !isfinite(c) || c <= fmin(lo, hi) || c >= fmax(lo, hi) || near_end;
+    return bisect ? smid : c;
+}
+
+/* Strong-Wolfe line search (ftol 1e-4, wolfe 0.9, <= 20 evaluations,
+ * expansion x2): bracketing, then zoom by quadratic interpolation.  Restates
+ * LineSearchNocedalWright::LineSearch,
+ * src/LBFGSpp/LBFGSpp/LineSearchNocedalWright.h:58-283.  On entry *fx, grad,
+ * *dg are at xp; on exit x, *fx, grad, *dg, *step describe the accepted point
+ * and *last_f is the last value evaluated.  Returns 0, -3 (direction is not a
+ * descent direction), -4 (insufficient precision) or -5 (no decrease). */
+static int nw_line_search(const prob_t* P, const double* xp, const double* drt, double* step,
+                          double* fx, double* grad, double* dg, double* x, double* last_f,
+                          double* xlo, double* glo)
+{
+    const long N = P->n + P->m - 1;
+    const int max_ls = 20;
+    const double ftol = 1e-4, wolfe = 0.9, expansion = 2.0;
+    const double fx_init = *fx, dg_init = *dg;
+    if (dg_init > 0.0) return -3;
+    const double test_decr = ftol * dg_init, test_curv = -wolfe * dg_init;
+    double step_hi, fx_hi, step_lo = 0.0, fx_lo = fx_init, dg_lo = dg_init;
+    memcpy(xlo, xp, sizeof(double) * N);
+    memcpy(glo, grad, sizeof(double) * N);
+    int iter = 0;
+    for (;;) {
+        for (long k = 0; k < N; k++) x[k] = xp[k] + (*step) * drt[k];
+        *fx = orc_dual_obj_grad(P->n, P->m, P->M, P->a, P->b, P->reg, x, grad, NULL, 1);
+        *last_f = *fx;
+        *dg = dot(grad, drt, N);
+        if (*fx - fx_init > (*step) * test_decr || (0.0 < step_lo && *fx >= fx_lo)) {
+            step_hi = *step;
+            fx_hi = *fx;
+            break;
+        }
+        if (fabs(*dg) <= test_curv) return 0;
+        step_hi = step_lo;
+        fx_hi = fx_lo;
+        step_lo = *step;
+        fx_lo = *fx;
+        dg_lo = *dg;
+        memcpy(xlo, x, sizeof(double) * N);
+        memcpy(glo, grad, sizeof(double) * N);
+        if (*dg >= 0.0) break;
+        iter++;
+        if (iter >= max_ls) {
+            /* every trial decreased f: keep the latest (== the _lo state) */
+            return 0;
+        }
+        *step *= expansion;
+    }
+    for (;;) {
+        *step = nw_quad_interp(step_lo, step_hi, fx_lo, fx_hi, dg_lo);
+        for (long k = 0; k < N; k++) x[k] = xp[k] + (*step) * drt[k];
+        *fx = orc_dual_obj_grad(P->n, P->m, P->M, P->a, P->b, P->reg, x, grad, NULL, 1);
+        *last_f = *fx;
+        *dg = dot(grad, drt, N);
+        if (*fx - fx_init > (*step) * test_decr || *fx >= fx_lo) {
+            if (*step == step_hi) return -4;
+            step_hi = *step;
+            fx_hi = *fx;
+        } else {
+            if (fabs(*dg) <= test_curv) return 0;
+            if (*dg * (step_hi - step_lo) >= 0.0) {
+                step_hi = step_lo;
+                fx_hi = fx_lo;
+            }
+            if (*step == step_lo) return -4;
+            step_lo = *step;
+            fx_lo = *fx;
+            dg_lo = *dg;
+            memcpy(xlo, x, sizeof(double) * N);
+            memcpy(glo, grad, sizeof(double) * N);
+        }
+        iter++;
+        if (iter >= max_ls) {
+            if (step_lo <= 0.0) return -5;
+            *step = step_lo;
+            *fx = fx_lo;
+            *dg = dg_lo;
+            memcpy(x, xlo, sizeof(double) * N);
+            memcpy(grad, glo, sizeof(double) * N);
+            return 0;
+        }
+    }
+}
+
+/* Exposed for the tests: one line search from gamma along direc with the
+ * initial step; returns the status, fills the accepted step, f and gradient. */
+int orc_nw_line_search(int n, int m, const double* M, const double* a, const double* b,
+                       double reg, const double* gamma, const double* direc, double init_step,
+                       double* step, double* f, double* grad, double* last_f)
+{
+    const long N = n + m - 1;
+    prob_t P = {n, m, M, a, b, reg, NULL};
+    double* x = (double*)malloc(sizeof(double) * N);
+    double* xlo = (double*)malloc(sizeof(double) * N);
+    double* glo = (double*)malloc(sizeof(double) * N);
+    *f = orc_dual_obj_grad(n, m, M, a, b, reg, gamma, grad, NULL, 1);
+    double dg = dot(grad, direc, N);
+    *step = init_step;
+    const int rc = nw_line_search(&P, gamma, direc, step, f, grad, &dg, x, last_f, xlo, glo);
+    free(x);
+    free(xlo);
+    free(glo);
+    return rc;
+}
+
+/* L-BFGS on the dual (memory 6, epsilon = tol, epsilon_rel = 0, no
+ * past-delta test, max_iterations = max_iter with 0 meaning unlimited).
+ * Restates sinkhorn_lbfgs_dual_internal, src/sinkhorn_lbfgs_dual.cpp:84-125,
+ * with LBFGSSolver::minimize, src/LBFGSpp/LBFGS.h:64-189 (parameters
+ * src/LBFGSpp/LBFGSpp/Param.h:154-171).  History entries come from the
+ * iterate() callback (:56-76): obj_vals holds the LAST objective evaluated
+ * (after a line search that fell back to an earlier trial this is not the
+ * accepted point's value), mar_errs the accepted point's ||grad||.  Returns
+ * niter as minimize() does (1 when the initial point already passes), or a
+ * negative line-search status.  At most hist_cap history entries are kept. */
+int orc_sinkhorn_lbfgs_dual(int n, int m, const double* M, const double* a, const double* b,
+                            double reg, double tol, int max_iter, int verbose, int hist_cap,
+                            double* x, double* obj_vals, double* mar_errs, double* run_times,
+                            int* hist_len)
+{
+    const long N = n + m - 1;
+    const double eps = 2.220446049250313e-16;
+    prob_t P = {n, m, M, a, b, reg, NULL};
+    double* grad = (double*)malloc(sizeof(double) * N);
+    double* xp = (double*)malloc(sizeof(double) * N);
+    double* gradp = (double*)malloc(sizeof(double) * N);
+    double* drt = (double*)malloc(sizeof(double) * N);
+    double* vs = (double*)malloc(sizeof(double) * N);
+    double* vy = (double*)malloc(sizeof(double) * N);
+    double* xlo = (double*)malloc(sizeof(double) * N);
+    double* glo = (double*)malloc(sizeof(double) * N);
+    lbfgs_mat B;
+    lbfgs_reset(&B, N, 6);
+    initial_gamma(&P, x);
+    int h = 0, k = 0, rc = 0;
+    double t0 = now_ms();
+    double fx = orc_dual_obj_grad(n, m, M, a, b, reg, x, grad, NULL, 1);
+    double last_f = fx;
+    double gnorm = norm2(grad, N);
+#define LBFGS_ITERATE()                                                                        \
+    do {                                                                                       \
+        const double t1_ = now_ms();                                                           \
+        if (h < hist_cap) {                                                                    \
+            obj_vals[h] = last_f;                                                              \
+            mar_errs[h] = gnorm;                                                               \
+            run_times[h] = (h > 0 ? run_times[h - 1] : 0.0) + (t1_ - t0);                      \
+            h++;                                                                               \
+        }                                                                                      \
+        if (verbose >= 1)                                                                      \
+            printf("iter = %d, objval = %g, ||grad|| = %g\n", h - 1, last_f, gnorm);           \
+        t0 = now_ms();                                                                         \
+    } while (0)
+    LBFGS_ITERATE();
+    if (gnorm <= tol || gnorm <= 0.0 * norm2(x, N)) {
+        k = 1;
+        goto done;
+    }
+    for (long q = 0; q < N; q++) drt[q] = -grad[q];
+    double step = 1.0 / norm2(drt, N);
+    k = 1;
+    for (;;) {
+        memcpy(xp, x, sizeof(double) * N);
+        memcpy(gradp, grad, sizeof(double) * N);
+        double dg = dot(grad, drt, N);
+        rc = nw_line_search(&P, xp, drt, &step, &fx, grad, &dg, x, &last_f, xlo, glo);
+        if (rc != 0) break;
+        gnorm = norm2(grad, N);
+        LBFGS_ITERATE();
+        if (gnorm <= tol || gnorm <= 0.0 * norm2(x, N)) break;
+        if (max_iter != 0 && k >= max_iter) break;
+        for (long q = 0; q < N; q++) {
+            vs[q] = x[q] - xp[q];
+            vy[q] = grad[q] - gradp[q];
+        }
+        if (dot(vs, vy, N) > eps * dot(vy, vy, N)) lbfgs_add(&B, vs, vy);
+        lbfgs_apply_hv(&B, grad, -1.0, drt);
+        step = 1.0;
+        k++;
+    }
+#undef LBFGS_ITERATE
+done:
+    lbfgs_free(&B);
+    free(grad);
+    free(xp);
+    free(gradp);
+    free(drt);
+    free(vs);
+    free(vy);
+    free(xlo);
+    free(glo);
+    *hist_len = h;
+    return rc != 0 ? rc : k;
+}
+
+/* Accelerated primal-dual adaptive gradient descent.  Restates
+ * sinkhorn_apdagd_internal, src/sinkhorn_apdagd.cpp:14-183: dual variables
+ * lambda = -gamma, adaptive L (halved per iteration, doubled per failed
+ * test, <= 20 inner trials), primal plan = running combination
+ * (1 - tau) plan + tau T(-lambda~); stop when primal - dual gap <= tol and
+ * the full marginal error (all m columns) <= tol.  entropy = sum P (1 - log P),
+ * so an exact zero in P makes it NaN (0 * inf), as in the reference.
+ * plan (n x m) receives the primal plan; x (N) the final gamma = -lambda~. */
+int orc_sinkhorn_apdagd(int n, int m, const double* M, const double* a, const double* b,
+                        double reg, double tol, int max_iter, int verbose,
+                        double* x, double* plan, double* obj_vals, double* mar_errs,
+                        double* run_times, int* hist_len)
+{
+    const long N = n + m - 1;
+    const double L0 = 1.0;
+    const int max_inner = 20;
+    prob_t P = {n, m, M, a, b, reg, NULL};
+    double* grad = (double*)malloc(sizeof(double) * N);
+    double* du = (double*)malloc(sizeof(double) * N);      /* dual */
+    double* du_new = (double*)malloc(sizeof(double) * N);
+    double* dp = (double*)malloc(sizeof(double) * N);      /* dualp */
+    double* dp_new = (double*)malloc(sizeof(double) * N);
+    double* dt = (double*)malloc(sizeof(double) * N);      /* dualt */
+    double* Tn = (double*)malloc(sizeof(double) * (size_t)n * m);
+    double* rs = (double*)malloc(sizeof(double) * n);
+    double* cs = (double*)malloc(sizeof(double) * m);
+    initial_gamma(&P, x);
+    for (long k = 0; k < N; k++) du[k] = dp[k] = dt[k] = -x[k];
+    orc_get_plan(n, m, M, reg, x, plan);
+    double t0 = now_ms();
+    double f = orc_dual_obj_grad(n, m, M, a, b, reg, x, grad, NULL, 1);
+    double gnorm = norm2(grad, N);
+    double t1 = now_ms();
+    obj_vals[0] = f;
+    mar_errs[0] = gnorm;
+    run_times[0] = t1 - t0;
+    int h = 1, i;
+    double L = L0, pbeta = 0.0;
+    for (i = 0; i < max_iter; i++) {
+        t0 = now_ms();
+        if (verbose >= 2)
+            printf("\n[line search]=================================================\n");
+        L *= 0.5;
+        double ptau = 0.0, phi_new = 0.0;
+        for (int j = 0; j < max_inner; j++) {
+            const double palpha = (1.0 + sqrt(4.0 * L * pbeta + 1.0)) / (2.0 * L);
+            const double pbeta_new = pbeta + palpha;
+            ptau = palpha / pbeta_new;
+            for (long k = 0; k < N; k++) {
+                dt[k] = (1.0 - ptau) * du[k] + ptau * dp[k];
+                x[k] = -dt[k];
+            }
+            f = orc_dual_obj_grad(n, m, M, a, b, reg, x, grad, NULL, 1);
+            const double phit = -f;
+            for (long k = 0; k < N; k++) {
+                dp_new[k] = dp[k] + palpha * grad[k];
+                du_new[k] = (1.0 - ptau) * du[k] + ptau * dp_new[k];
+                x[k] = -du_new[k];
+            }
+            f = orc_dual_obj(n, m, M, a, b, reg, x);
+            phi_new = -f;
+            double gd = 0.0, dd = 0.0;
+            for (long k = 0; k < N; k++) {
+                const double d = du_new[k] - dt[k];
+                gd += grad[k] * d;
+                dd += d * d;
+            }
+            const double rhs = phit + gd - 0.5 * L * dd;
+            if (verbose >= 2) {
+                printf("║ j = %d, palpha = %g, pbeta = %g\n", j, palpha, pbeta_new);
+                printf("║ phit = %g, phi_new = %g, rhs = %g\n", phit, phi_new, rhs);
+                printf("║-------------------------------------------------------------\n");
+            }
+            if (phi_new >= rhs || j >= max_inner - 1) {
+                pbeta = pbeta_new;
+                memcpy(dp, dp_new, sizeof(double) * N);
+                memcpy(du, du_new, sizeof(double) * N);
+                break;
+            }
+            L *= 2.0;
+        }
+        for (long k = 0; k < N; k++) x[k] = -dt[k];
+        orc_get_plan(n, m, M, reg, x, Tn);
+        double ent = 0.0, mp = 0.0;
+        memset(rs, 0, sizeof(double) * n);
+        memset(cs, 0, sizeof(double) * m);
+        for (int r = 0; r < n; r++)
+            for (int c = 0; c < m; c++) {
+                const long q = (long)r * m + c;
+                const double v = (1.0 - ptau) * plan[q] + ptau * Tn[q];
+                plan[q] = v;
+                ent += v * (1.0 - log(v));
+                mp += M[q] * v;
+                rs[r] += v;
+                cs[c] += v;
+            }
+        const double primal_obj = mp - reg * ent;
+        double e2 = 0.0;
+        for (int r = 0; r < n; r++) e2 += (rs[r] - a[r]) * (rs[r] - a[r]);
+        for (int c = 0; c < m; c++) e2 += (cs[c] - b[c]) * (cs[c] - b[c]);
+        const double mar_err = sqrt(e2);
+        t1 = now_ms();
+        obj_vals[h] = f;
+        mar_errs[h] = mar_err;
+        run_times[h] = run_times[h - 1] + (t1 - t0);
+        h++;
+        if (verbose >= 1)
+            printf("iter = %d, primal_obj = %g, phi_new = %g, mar_err = %g\n", i, primal_obj,
+                   phi_new, mar_err);
+        if (primal_obj - phi_new <= tol && mar_err <= tol) break;
+    }
+    free(grad);
+    free(du);
+    free(du_new);
+    free(dp);
+    free(dp_new);
+    free(dt);
+    free(Tn);
+    free(rs);
+    free(cs);
+    *hist_len = h;
+    return i;
+}

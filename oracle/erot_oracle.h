@@ -109,6 +109,33 @@ int orc_sinkhorn_splr(int n, int m, const double* M, const double* a, const doub
                       double* dual, double* obj_vals, double* mar_errs, double* run_times,
                       double* densities, int* hist_len);
 
+/* ---- Dense Newton, L-BFGS dual, APDAGD (src/sinkhorn_{newton,lbfgs_dual,apdagd}.cpp) ---- */
+/* Newton direction -H^{-1} g with the dense Hessian and a Cholesky solve
+ * (src/sinkhorn_problem_obj.cpp:396-418, src/sinkhorn_newton.cpp:87-88). */
+int orc_newton_direction(int n, int m, const double* M, const double* a, const double* b,
+                         double reg, const double* gamma, double* direc);
+int orc_sinkhorn_newton(int n, int m, const double* M, const double* a, const double* b,
+                        double reg, double tol, int max_iter, int verbose,
+                        double* dual, double* obj_vals, double* mar_errs, double* run_times,
+                        int* hist_len);
+/* Two-loop L-BFGS product H v after adding npairs pairs (S[k], Y[k]) in order
+ * (src/LBFGSpp/LBFGSpp/BFGSMat.h:86-97,276-302). */
+void orc_lbfgs_apply(long N, int mm, int npairs, const double* S, const double* Y,
+                     const double* v, double* res);
+/* One strong-Wolfe line search on the dual (LineSearchNocedalWright). */
+int orc_nw_line_search(int n, int m, const double* M, const double* a, const double* b,
+                       double reg, const double* gamma, const double* direc, double init_step,
+                       double* step, double* f, double* grad, double* last_f);
+int orc_sinkhorn_lbfgs_dual(int n, int m, const double* M, const double* a, const double* b,
+                            double reg, double tol, int max_iter, int verbose, int hist_cap,
+                            double* dual, double* obj_vals, double* mar_errs, double* run_times,
+                            int* hist_len);
+/* plan (n x m) receives the primal plan. */
+int orc_sinkhorn_apdagd(int n, int m, const double* M, const double* a, const double* b,
+                        double reg, double tol, int max_iter, int verbose,
+                        double* dual, double* plan, double* obj_vals, double* mar_errs,
+                        double* run_times, int* hist_len);
+
 /* BCD iterations only (no convergence test), for timing a bounded CPU sample. */
 double orc_bcd_iterations(int n, int m, const double* M, const double* a, const double* b,
                           double reg, int iters, double* gamma_out);

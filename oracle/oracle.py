@@ -289,3 +289,93 @@ def bcd_iterations(M, a, b, reg, iters):
     g = np.empty(n + m - 1)
     return lib().orc_bcd_iterations(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg), int(iters),
                                     _p(g))
+
+
+# --------------------------------------------------------------------------
+# Dense Newton, L-BFGS dual, APDAGD
+# --------------------------------------------------------------------------
+_LS_ERRORS = {
+    -2: "the dense Hessian is not positive definite",
+    -3: "the moving direction increases the objective function value",
+    -4: "the line search routine failed, possibly due to insufficient numeric precision",
+    -5: "the line search routine failed, unable to sufficiently decrease the function value",
+}
+
+
+def _check(rc):
+    if rc < 0:
+        raise RuntimeError(_LS_ERRORS.get(rc, "oracle error %d" % rc))
+    return rc
+
+
+def newton_direction(M, a, b, reg, gamma):
+    """-H^{-1} g with the dense Hessian (src/sinkhorn_problem_obj.cpp:396-418)"""
+    M, a, b, gamma = _d(M), _d(a), _d(b), _d(gamma)
+    n, m = M.shape
+    d = np.empty(n + m - 1)
+    _check(lib().orc_newton_direction(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg),
+                                      _p(gamma), _p(d)))
+    return d
+
+
+def sinkhorn_newton(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0):
+    """src/sinkhorn_newton.cpp:17-129"""
+    out = _run(lib().orc_sinkhorn_newton, M, a, b, reg, tol, max_iter, verbose, None, False)
+    _check(out["niter"])
+    return out
+
+
+def lbfgs_apply(S, Y, v, mm=6):
+    """Two-loop product H v after the pairs (S[k], Y[k]) (BFGSMat.h:86-97,276-302)"""
+    S, Y, v = _d(S), _d(Y), _d(v)
+    res = np.empty_like(v)
+    lib().orc_lbfgs_apply(ctypes.c_long(len(v)), int(mm), int(S.shape[0]), _p(S), _p(Y), _p(v),
+                          _p(res))
+    return res
+
+
+def nw_line_search(M, a, b, reg, gamma, direc, init_step):
+    """One LineSearchNocedalWright search -> (status, step, f, grad, last_f)"""
+    M, a, b, gamma, direc = _d(M), _d(a), _d(b), _d(gamma), _d(direc)
+    n, m = M.shape
+    g = np.empty(n + m - 1)
+    step, f, lf = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+    rc = lib().orc_nw_line_search(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg), _p(gamma),
+                                  _p(direc), ctypes.c_double(init_step), ctypes.byref(step),
+                                  ctypes.byref(f), _p(g), ctypes.byref(lf))
+    return rc, step.value, f.value, g, lf.value
+
+
+def sinkhorn_lbfgs_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, hist_cap=None):
+    """src/sinkhorn_lbfgs_dual.cpp:84-125 with LBFGSSolver::minimize (src/LBFGSpp/LBFGS.h)"""
+    M, a, b = _d(M), _d(a), _d(b)
+    n, m = M.shape
+    cap = int(hist_cap if hist_cap else (max_iter + 1 if max_iter > 0 else 100001))
+    dual = np.empty(n + m - 1)
+    h = [np.empty(cap) for _ in range(3)]
+    hl = ctypes.c_int(0)
+    niter = lib().orc_sinkhorn_lbfgs_dual(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg),
+                                          ctypes.c_double(tol), int(max_iter), int(verbose), cap,
+                                          _p(dual), _p(h[0]), _p(h[1]), _p(h[2]),
+                                          ctypes.byref(hl))
+    _check(niter)
+    k = hl.value
+    return dict(niter=niter, dual=dual, obj_vals=list(h[0][:k]), mar_errs=list(h[1][:k]),
+                run_times=list(h[2][:k]), densities=[])
+
+
+def sinkhorn_apdagd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0):
+    """src/sinkhorn_apdagd.cpp:14-183 -> dict with the primal plan under "plan" """
+    M, a, b = _d(M), _d(a), _d(b)
+    n, m = M.shape
+    dual = np.empty(n + m - 1)
+    plan = np.empty((n, m))
+    h = [np.empty(max_iter + 1) for _ in range(3)]
+    hl = ctypes.c_int(0)
+    niter = lib().orc_sinkhorn_apdagd(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg),
+                                      ctypes.c_double(tol), int(max_iter), int(verbose),
+                                      _p(dual), _p(plan), _p(h[0]), _p(h[1]), _p(h[2]),
+                                      ctypes.byref(hl))
+    k = hl.value
+    return dict(niter=niter, dual=dual, plan=plan, obj_vals=list(h[0][:k]),
+                mar_errs=list(h[1][:k]), run_times=list(h[2][:k]), densities=[])

@@ -307,3 +307,128 @@ def test_golden_c1_readme_recipe():
     # the fixture's own marginal error is ~1e-15; 1e-12 pins the dual to ~1e-9
     np.testing.assert_allclose(r["dual"], z["dual"], rtol=0, atol=1e-9)
     assert abs(r["obj_vals"][-1] - float(z["obj"])) < 1e-12
+
+
+# --- dense Newton, L-BFGS dual, APDAGD (src/sinkhorn_{newton,lbfgs_dual,apdagd}.cpp) ---
+
+def test_newton_direction_against_dense_numpy_solve():
+    # H assembled from an independently computed T (numpy exp), then
+    # numpy.linalg.solve: the oracle's Cholesky direction must agree
+    M, a, b, reg = rand_problem(n=9, m=7, seed=11)
+    n, m = M.shape
+    gamma = np.random.default_rng(12).standard_normal(n + m - 1) * 0.05
+    beta = np.append(gamma[n:], 0.0)
+    T = np.exp((gamma[:n, None] + beta[None, :] - M) / reg)
+    g = np.concatenate([T.sum(1) - a, T.sum(0)[:-1] - b[:-1]])
+    H = np.zeros((n + m - 1, n + m - 1))
+    H[:n, :n] = np.diag(T.sum(1))
+    H[n:, n:] = np.diag(T.sum(0)[:-1])
+    H[:n, n:] = T[:, :-1]
+    H[n:, :n] = T[:, :-1].T
+    H /= reg
+    np.testing.assert_allclose(O.newton_direction(M, a, b, reg, gamma),
+                               np.linalg.solve(H, -g), rtol=1e-10, atol=1e-12)
+
+
+def test_lbfgs_two_loop_against_dense_bfgs_recursion():
+    # The two-loop product equals the dense inverse-BFGS recursion
+    # H+ = (I - rho s y')H(I - rho y s') + rho s s' started at H0 = I / theta,
+    # theta = y'y / s'y of the newest pair, over the last mm pairs.
+    rng = np.random.default_rng(21)
+    N, mm = 11, 6
+    A = rng.standard_normal((N, N))
+    A = A @ A.T + N * np.eye(N)
+    for npairs in (1, 4, 9):
+        S = rng.standard_normal((npairs, N))
+        Y = S @ A          # curvature pairs of a convex quadratic: s'y > 0
+        v = rng.standard_normal(N)
+        keep = range(max(0, npairs - mm), npairs)
+        theta = Y[-1] @ Y[-1] / (S[-1] @ Y[-1])
+        H = np.eye(N) / theta
+        I = np.eye(N)
+        for k in keep:
+            rho = 1.0 / (Y[k] @ S[k])
+            H = (I - rho * np.outer(S[k], Y[k])) @ H @ (I - rho * np.outer(Y[k], S[k])) \
+                + rho * np.outer(S[k], S[k])
+        np.testing.assert_allclose(O.lbfgs_apply(S, Y, v, mm), H @ v, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_nocedal_wright_search_returns_a_strong_wolfe_point(seed):
+    M, a, b, reg = rand_problem(n=15, m=12, seed=30 + seed, reg=0.05)
+    n, m = M.shape
+    rng = np.random.default_rng(seed)
+    gamma = rng.standard_normal(n + m - 1) * 0.05
+    f0, g0 = O.dual_obj_grad(M, a, b, reg, gamma)
+    d = -g0 + 0.3 * rng.standard_normal(n + m - 1) * np.abs(g0).mean()
+    if g0 @ d > 0:
+        d = -g0
+    for init in (1e-3, 1.0, 50.0):
+        rc, step, f, g, lf = O.nw_line_search(M, a, b, reg, gamma, d, init)
+        assert rc == 0
+        # accepted point is where the oracle says, with its own f and gradient
+        f_chk, g_chk = O.dual_obj_grad(M, a, b, reg, gamma + step * d)
+        assert abs(f - f_chk) <= 1e-14 * max(1.0, abs(f))
+        np.testing.assert_allclose(g, g_chk, rtol=0, atol=1e-14)
+        # sufficient decrease (ftol 1e-4) and strong curvature (wolfe 0.9)
+        assert f - f0 <= 1e-4 * step * (g0 @ d) + 1e-15
+        assert abs(g @ d) <= 0.9 * abs(g0 @ d) + 1e-15
+        assert lf == f
+    # an ascent direction is refused (LBFGSpp's logic_error)
+    rc = O.nw_line_search(M, a, b, reg, gamma, g0, 1.0)[0]
+    assert rc == -3
+
+
+def test_newton_lbfgs_apdagd_reach_the_unique_optimum():
+    M, a, b = cloud_problem(40, 30, d=4, seed=3)
+    reg = 0.1
+    ref = independent_sinkhorn(M, a, b, reg)
+    plan_ref = O.get_plan(M, reg, ref)
+    nw = O.sinkhorn_newton(M, a, b, reg, tol=1e-11, max_iter=100)
+    lb = O.sinkhorn_lbfgs_dual(M, a, b, reg, tol=1e-8, max_iter=5000)
+    for name, r, bound in (("newton", nw, 1e-11), ("lbfgs", lb, 1e-8)):
+        err = r["mar_errs"][-1]
+        assert err <= bound, name
+        tol = max(1e-8, 1e3 * err)
+        np.testing.assert_allclose(r["dual"], ref, rtol=0, atol=tol, err_msg=name)
+        np.testing.assert_allclose(O.get_plan(M, reg, r["dual"]), plan_ref, atol=10 * tol)
+        assert len(r["obj_vals"]) == r["niter"] + 1
+    # Newton converges quadratically: few iterations, ||g|| squares at the end
+    assert nw["niter"] <= 15
+    e = nw["mar_errs"]
+    assert e[-2] < 1e-3 and e[-1] < 1e3 * e[-2] ** 2
+    # APDAGD: the averaged primal plan converges to T*, with gap and
+    # (full) marginal error below tol at the stop
+    # (first-order: about 4400 iterations to 1e-6 here)
+    ap = O.sinkhorn_apdagd(M, a, b, reg, tol=1e-6, max_iter=20000)
+    assert ap["niter"] < 20000
+    P = ap["plan"]
+    full_err = np.sqrt(((P.sum(1) - a) ** 2).sum() + ((P.sum(0) - b) ** 2).sum())
+    assert abs(full_err - ap["mar_errs"][-1]) < 1e-15 and full_err <= 1e-6
+    np.testing.assert_allclose(P, plan_ref, rtol=0, atol=1e-4)
+    primal = (M * P).sum() - reg * (P * (1 - np.log(P))).sum()
+    assert primal - (-ap["obj_vals"][-1]) <= 1e-6 + 1e-12
+    # the stop test follows the history push (src/sinkhorn_apdagd.cpp:160-174):
+    # a converged run records niter + 2 entries
+    assert len(ap["obj_vals"]) == ap["niter"] + 2
+
+
+def test_separable_cost_newton_and_lbfgs():
+    rng = np.random.default_rng(7)
+    n, m, reg = 12, 9, 0.3
+    u, v = rng.random(n), rng.random(m)
+    M = u[:, None] + v[None, :]
+    a = rng.random(n) + 0.2
+    b = rng.random(m) + 0.2
+    a, b = a / a.sum(), b / b.sum()
+    c = v[-1] + reg * np.log(b[-1])
+    dual = np.concatenate([u + reg * np.log(a) + c, (v + reg * np.log(b) - c)[:-1]])
+    r = O.sinkhorn_newton(M, a, b, reg, tol=1e-13, max_iter=50)
+    np.testing.assert_allclose(r["dual"], dual, rtol=1e-12, atol=1e-12)
+    # L-BFGS: f differences drop below FP64 resolution near ||g|| ~ 5e-9
+    # here, where the line search (like LBFGSpp's) gives up with an error
+    r = O.sinkhorn_lbfgs_dual(M, a, b, reg, tol=1e-8, max_iter=200)
+    assert r["mar_errs"][-1] <= 1e-8
+    np.testing.assert_allclose(O.get_plan(M, reg, r["dual"]), np.outer(a, b), rtol=1e-6)
+    with pytest.raises(RuntimeError, match="unable to sufficiently decrease"):
+        O.sinkhorn_lbfgs_dual(M, a, b, reg, tol=1e-12, max_iter=200)
