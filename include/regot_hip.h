@@ -28,6 +28,9 @@ extern "C" {
 #define REGOT_SOLVER_SSNS 1
 #define REGOT_SOLVER_SPARSE_NEWTON 2
 #define REGOT_SOLVER_SPLR 3
+#define REGOT_SOLVER_NEWTON 4
+#define REGOT_SOLVER_LBFGS_DUAL 5
+#define REGOT_SOLVER_APDAGD 6
 
 typedef struct regot_problem regot_problem;
 typedef struct regot_result regot_result;
@@ -138,12 +141,15 @@ void regot_opts_default(regot_opts* o, int solver);
 
 /*
  * Run a solver on a problem handle.
- * Replaces sinkhorn_{bcd,ssns,sparse_newton,splr}_internal,
+ * Replaces sinkhorn_{bcd,ssns,sparse_newton,splr,newton,lbfgs_dual,apdagd}_internal,
  * src/sinkhorn_solvers.h:40-94 (drivers src/sinkhorn_bcd.cpp:14-124,
  * src/sinkhorn_ssns.cpp:16-234, src/sinkhorn_sparse_newton.cpp:24-170,
- * src/sinkhorn_splr.cpp:24-199).  The plan is not formed here; see
- * regot_result_plan().  verbose >= 1 prints "iter = i, objval = f,
- * ||grad|| = g" lines to stdout, as the reference does.
+ * src/sinkhorn_splr.cpp:24-199, src/sinkhorn_newton.cpp:17-129,
+ * src/sinkhorn_lbfgs_dual.cpp:84-125, src/sinkhorn_apdagd.cpp:14-183).
+ * The plan is not formed here, except APDAGD's primal plan, which the result
+ * keeps on the device; see regot_result_plan().  verbose >= 1 prints the
+ * reference's progress lines to stdout.  A failed L-BFGS line search returns
+ * -6 with LBFGSpp's message.
  */
 int regot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,
                 const regot_opts* opts, regot_result** out);
@@ -156,7 +162,8 @@ int regot_result_dens_len(const regot_result* r);       /* 0 for BCD */
  * densities (dens_len).  Any pointer may be NULL. */
 int regot_result_get(const regot_result* r, double* dual, double* obj_vals, double* mar_errs,
                      double* run_times, double* densities);
-/* FP64 plan of the result's dual, row-major n x m, into host memory. */
+/* FP64 plan, row-major n x m, into host memory: APDAGD's primal plan, else
+ * the plan of the result's dual (SinkhornResult::get_plan). */
 int regot_result_plan(regot_problem* p, const regot_result* r, double* plan);
 /* Marginal error ||grad|| re-evaluated at the result's dual in FP64 arithmetic
  * (exp in FP64 regardless of bulk), plus the FP64 objective. */

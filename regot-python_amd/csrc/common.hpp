@@ -26,6 +26,7 @@ enum Status : int {
     E_NOMEM = -3,     // device allocation failed
     E_STATE = -4,     // operation needs state that is not there (e.g. no Hessian yet)
     E_INTERNAL = -5,
+    E_SOLVER = -6,    // a solver stopped with an error (e.g. a failed line search)
 };
 
 [[noreturn]] inline void fail(int code, const char* fmt, ...)

@@ -17,6 +17,11 @@ struct regot_result {
     double phase_ms[3] = {0.0, 0.0, 0.0};
     long long phase_n[3] = {0, 0, 0};
     long long cg_iters = 0;
+    // APDAGD's primal plan (device, row pitch plan_ld); empty for the others,
+    // whose plan is formed from the dual on request
+    regot::DBuf<double> plan;
+    long plan_ld = 0;
+    int plan_device = 0;
 };
 
 namespace regot {
@@ -26,5 +31,13 @@ void solve_ssns(regot_problem* p, double tol, int max_iter, int verbose, const r
 // splr = false: sparse Newton; true: SPLR.
 void solve_newton_family(regot_problem* p, bool splr, double tol, int max_iter, int verbose,
                          const regot_opts& o, regot_result* res);
+// Newton's method with the full dual Hessian (newton.cpp).
+void solve_newton_dense(regot_problem* p, double tol, int max_iter, int verbose,
+                        const regot_opts& o, regot_result* res);
+// L-BFGS on the dual and APDAGD (first_order.cpp).
+void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
+                      const regot_opts& o, regot_result* res);
+void solve_apdagd(regot_problem* p, double tol, int max_iter, int verbose, const regot_opts& o,
+                  regot_result* res);
 
 }  // namespace regot

@@ -1,0 +1,342 @@
+// first_order.cpp -- host drivers of the first-order EROT solvers: L-BFGS on
+// the dual and APDAGD.  Control flow and the line-search / step-size logic
+// run on the host in FP64 on scalars; every vector lives on the device and
+// every O(n m) evaluation (objective, gradient, the APDAGD plan pass) is a
+// HIP kernel.  Objectives and gradients are evaluated in FP64 sums (precise
+// passes) because both line searches difference objective values.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <utility>
+#include <vector>
+
+#include "driver_work.hpp"
+
+namespace regot {
+namespace {
+
+// Line-search failures; the texts are LBFGSpp's exception messages
+// (src/LBFGSpp/LBFGSpp/LineSearchNocedalWright.h:95-96,216-217,233-234,251-252).
+[[noreturn]] void ls_fail(int which)
+{
+    static const char* msg[] = {
+        "the moving direction increases the objective function value",
+        "the line search routine failed, possibly due to insufficient numeric precision",
+        "the line search routine failed, unable to sufficiently decrease the function value"};
+    fail(E_SOLVER, "%s", msg[which]);
+}
+
+// Candidate of the zoom phase: minimiser of the quadratic through
+// (lo, f_lo), (hi, f_hi) with slope dg_lo at lo; the midpoint when that is not
+// finite, not inside (lo, hi) or within 1% of the interval of an end.
+// LineSearchNocedalWright::quad_interp,
+// src/LBFGSpp/LBFGSpp/LineSearchNocedalWright.h:26-55.
+double nw_quad_interp(double lo, double hi, double flo, double fhi, double dglo)
+{
+    const double fdiff = fhi - flo, sdiff = hi - lo, smid = (hi + lo) / 2.0;
+    double c = fdiff * lo - smid * sdiff * dglo;
+    c = c / (fdiff - sdiff * dglo);
+    const double end_dist = std::min(std::fabs(c - lo), std::fabs(c - hi));
+    const bool near_end = end_dist < 0.01 * std::fabs(sdiff);
+    const bool bisect = !std::isfinite(c) || c <= std::min(lo, hi) || c >= std::max(lo, hi) ||
+                        near_end;
+    return bisect ? smid : c;
+}
+
+// Device state of the L-BFGS iteration.  x / grad and their "_lo" copies are
+// swapped by pointer, as the line search swaps its vectors.
+struct LbfgsState {
+    double* x;
+    double* grad;
+    double* xlo;
+    double* glo;
+    double* xp;
+    double* drt;
+    double last_f = 0.0;   // the most recent objective evaluation
+};
+
+// Strong-Wolfe search (ftol 1e-4, wolfe 0.9, at most 20 evaluations, x2
+// expansion): bracketing, then zoom.  LineSearchNocedalWright::LineSearch,
+// src/LBFGSpp/LBFGSpp/LineSearchNocedalWright.h:58-283.  On entry fx, grad,
+// dg hold the values at xp; on return x, fx, grad, dg and step describe the
+// accepted point.
+void nw_line_search(Work& w, LbfgsState& st, double& step, double& fx, double& dg)
+{
+    const int max_ls = 20;
+    const double ftol = 1e-4, wolfe = 0.9, expansion = 2.0;
+    const double fx_init = fx, dg_init = dg;
+    if (dg_init > 0.0) ls_fail(0);
+    const double test_decr = ftol * dg_init, test_curv = -wolfe * dg_init;
+    double step_hi = 0.0, fx_hi = 0.0;
+    double step_lo = 0.0, fx_lo = fx_init, dg_lo = dg_init;
+    vec_copy(st.xlo, st.xp, w.N, w.s);
+    vec_copy(st.glo, st.grad, w.N, w.s);
+    auto eval = [&]() {
+        w.move(st.x, st.xp, step, st.drt);
+        fx = w.obj_grad(st.x, st.grad, nullptr);
+        st.last_f = fx;
+        dg = w.dot(st.grad, st.drt);
+    };
+    int iter = 0;
+    for (;;) {
+        eval();
+        if (fx - fx_init > step * test_decr || (0.0 < step_lo && fx >= fx_lo)) {
+            step_hi = step;
+            fx_hi = fx;
+            break;
+        }
+        if (std::fabs(dg) <= test_curv) return;
+        step_hi = step_lo;
+        fx_hi = fx_lo;
+        step_lo = step;
+        fx_lo = fx;
+        dg_lo = dg;
+        std::swap(st.xlo, st.x);
+        std::swap(st.glo, st.grad);
+        if (dg >= 0.0) break;
+        iter++;
+        if (iter >= max_ls) {
+            // every trial lowered f: the latest one (now in the _lo slots)
+            std::swap(st.x, st.xlo);
+            std::swap(st.grad, st.glo);
+            return;
+        }
+        step *= expansion;
+    }
+    for (;;) {
+        step = nw_quad_interp(step_lo, step_hi, fx_lo, fx_hi, dg_lo);
+        eval();
+        if (fx - fx_init > step * test_decr || fx >= fx_lo) {
+            if (step == step_hi) ls_fail(1);
+            step_hi = step;
+            fx_hi = fx;
+        } else {
+            if (std::fabs(dg) <= test_curv) return;
+            if (dg * (step_hi - step_lo) >= 0.0) {
+                step_hi = step_lo;
+                fx_hi = fx_lo;
+            }
+            if (step == step_lo) ls_fail(1);
+            step_lo = step;
+            fx_lo = fx;
+            dg_lo = dg;
+            std::swap(st.xlo, st.x);
+            std::swap(st.glo, st.grad);
+        }
+        iter++;
+        if (iter >= max_ls) {
+            if (step_lo <= 0.0) ls_fail(2);
+            step = step_lo;
+            fx = fx_lo;
+            dg = dg_lo;
+            std::swap(st.x, st.xlo);
+            std::swap(st.grad, st.glo);
+            return;
+        }
+    }
+}
+
+}  // namespace
+
+// L-BFGS on the dual: memory 6, epsilon = tol, epsilon_rel = 0, no
+// past-delta test, max_iterations = max_iter (0 = unlimited).  Mirrors
+// sinkhorn_lbfgs_dual_internal, src/sinkhorn_lbfgs_dual.cpp:84-125, with
+// LBFGSSolver::minimize, src/LBFGSpp/LBFGS.h:64-189 (parameters
+// src/LBFGSpp/LBFGSpp/Param.h:154-171) and the iterate() callback
+// (src/sinkhorn_lbfgs_dual.cpp:56-76): obj_vals records the most recent
+// objective evaluation, mar_errs the accepted point's ||grad||.  niter is
+// minimize()'s return value (1 when the initial point already passes).  The
+// two-loop product is one device kernel over the correction ring.
+void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
+                      const regot_opts& o, regot_result* res)
+{
+    const int mm = 6;
+    const double eps = std::numeric_limits<double>::epsilon();
+    PreciseScope ps(p);
+    Work w(p);
+    w.init(o);
+    DBuf<double> S(static_cast<size_t>(mm) * w.N), Y(static_cast<size_t>(mm) * w.N);
+    std::vector<double> ys(mm, 0.0);
+    int ncorr = 0, ptr = mm;
+    double theta = 1.0;
+    LbfgsState st{w.gamma.p, w.g.p, w.lr0.p, w.lr1.p, w.gamma_pre.p, w.direc.p};
+    Timer timer;
+    timer.tic();
+    double gnorm;
+    double fx = w.obj_grad(st.x, st.grad, &gnorm);
+    st.last_f = fx;
+    int k = 0;
+    auto iterate = [&]() {
+        const double dur = timer.toc();
+        res->obj_vals.push_back(st.last_f);
+        res->mar_errs.push_back(gnorm);
+        res->run_times.push_back((res->run_times.empty() ? 0.0 : res->run_times.back()) + dur);
+        if (verbose >= 1) {
+            printf("iter = %d, objval = %g, ||grad|| = %g\n",
+                   static_cast<int>(res->obj_vals.size()) - 1, st.last_f, gnorm);
+            fflush(stdout);
+        }
+        timer.tic();
+    };
+    iterate();
+    // epsilon_rel = 0: the relative test reads ||g|| <= 0
+    if (gnorm <= tol || gnorm <= 0.0) {
+        k = 1;
+    } else {
+        vec_axpby(st.drt, -1.0, st.grad, 0.0, nullptr, w.N, w.s);
+        double step = 1.0 / gnorm;   // 1 / ||drt||, drt = -grad
+        k = 1;
+        for (;;) {
+            vec_copy(st.xp, st.x, w.N, w.s);
+            vec_copy(w.g_pre, st.grad, w.N, w.s);
+            double dg = w.dot(st.grad, st.drt);
+            nw_line_search(w, st, step, fx, dg);
+            gnorm = std::sqrt(w.dot(st.grad, st.grad));
+            iterate();
+            if (gnorm <= tol || gnorm <= 0.0) break;
+            if (max_iter != 0 && k >= max_iter) break;
+            // s = x - xp, y = grad - gradp into the next ring slot; kept only
+            // when s'y > eps y'y (the slot is reused otherwise)
+            const int loc = ptr % mm;
+            double* sl = S.p + static_cast<long>(loc) * w.N;
+            double* yl = Y.p + static_cast<long>(loc) * w.N;
+            vec_axpby(sl, 1.0, st.x, -1.0, st.xp, w.N, w.s);
+            vec_axpby(yl, 1.0, st.grad, -1.0, w.g_pre, w.N, w.s);
+            const double* xs[2] = {sl, yl};
+            const double* zs[2] = {yl, yl};
+            double d[2];
+            w.dots(2, xs, zs, d);
+            if (d[0] > eps * d[1]) {
+                ys[loc] = d[0];
+                theta = d[1] / d[0];
+                if (ncorr < mm) ncorr++;
+                ptr = loc + 1;
+            }
+            vec_lbfgs_apply(S, Y, ys.data(), mm, ncorr, ptr, theta, st.grad, -1.0, st.drt, w.N,
+                            w.s);
+            step = 1.0;
+            k++;
+        }
+    }
+    res->niter = k;
+    w.report(res);
+    res->dual.resize(w.N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), st.x, w.N * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+// Accelerated primal-dual adaptive gradient descent.  Mirrors
+// sinkhorn_apdagd_internal, src/sinkhorn_apdagd.cpp:14-183: dual variables
+// lambda = -gamma, L halved per iteration and doubled per failed test (at most
+// 20 inner trials, the last one accepted), the primal plan the running
+// combination (1 - tau) P + tau T(-lambda~) kept on the device, convergence
+// when the primal-dual gap and the full marginal error (all m columns) are
+// both <= tol, tested after the history entry.  The final dual is
+// -lambda~; the plan stays in the result (device) and is copied out on request.
+void solve_apdagd(regot_problem* p, double tol, int max_iter, int verbose, const regot_opts& o,
+                  regot_result* res)
+{
+    const double L0 = 1.0;
+    const int max_inner = 20;
+    const int n = p->n, m = p->m;
+    PreciseScope ps(p);
+    Work w(p);
+    w.init(o);
+    // lambda (du), lambda' (dp), lambda~ (dt) and their new values
+    double* du = w.direc.p;
+    double* du_new = w.step.p;
+    double* dp = w.Hstep.p;
+    double* dp_new = w.rhs.p;
+    double* dt = w.y.p;
+    double* diff = w.sv.p;
+    double* x = w.gamma.p;
+    double* grad = w.g.p;
+    vec_axpby(du, -1.0, x, 0.0, nullptr, w.N, w.s);
+    vec_copy(dp, du, w.N, w.s);
+    vec_copy(dt, du, w.N, w.s);
+    // plan pass buffers
+    const long ldp = (m + 31) / 32 * 32;
+    res->plan.alloc(static_cast<size_t>(n) * ldp);
+    res->plan_ld = ldp;
+    res->plan_device = p->device;
+    long nrp, ncp, nsc, npt;
+    apd_scratch_sizes(n, m, &nrp, &ncp, &nsc, &npt);
+    DBuf<double> rowpart(nrp), colpart(ncp), scal(nsc), part(npt), out(3);
+    HBuf<double> hout;
+    hout.ensure(3);
+    DBuf<double> alpha(n), beta(m);
+    auto plan_pass = [&](double tau, bool init) {
+        launch_split_gamma(x, n, m, alpha, beta, w.s);
+        launch_apd_plan(p->bulk, p->dM, p->ldm, n, m, alpha, beta, p->a, p->b, p->reg, tau, init,
+                        res->plan, ldp, rowpart, colpart, scal, part, out, w.s);
+        HIP_CHECK(hipMemcpyAsync(hout.p, out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, w.s));
+        HIP_CHECK(hipStreamSynchronize(w.s));
+    };
+    plan_pass(1.0, true);   // result.get_plan(gamma) at the initial point
+    Timer timer;
+    timer.tic();
+    double gnorm;
+    double f = w.obj_grad(x, grad, &gnorm);
+    res->obj_vals.push_back(f);
+    res->mar_errs.push_back(gnorm);
+    res->run_times.push_back(timer.toc());
+    int i;
+    double L = L0, pbeta = 0.0;
+    for (i = 0; i < max_iter; i++) {
+        timer.tic();
+        if (verbose >= 2)
+            printf("\n[line search]=================================================\n");
+        L *= 0.5;
+        double ptau = 0.0, phi_new = 0.0;
+        for (int j = 0; j < max_inner; j++) {
+            const double palpha = (1.0 + std::sqrt(4.0 * L * pbeta + 1.0)) / (2.0 * L);
+            const double pbeta_new = pbeta + palpha;
+            ptau = palpha / pbeta_new;
+            vec_axpby(dt, 1.0 - ptau, du, ptau, dp, w.N, w.s);
+            vec_axpby(x, -1.0, dt, 0.0, nullptr, w.N, w.s);
+            f = w.obj_grad(x, grad, nullptr);
+            const double phit = -f;
+            vec_axpby(dp_new, 1.0, dp, palpha, grad, w.N, w.s);
+            vec_axpby(du_new, 1.0 - ptau, du, ptau, dp_new, w.N, w.s);
+            vec_axpby(x, -1.0, du_new, 0.0, nullptr, w.N, w.s);
+            f = w.obj(x);
+            phi_new = -f;
+            vec_axpby(diff, 1.0, du_new, -1.0, dt, w.N, w.s);
+            const double* xs[2] = {grad, diff};
+            const double* zs[2] = {diff, diff};
+            double d[2];
+            w.dots(2, xs, zs, d);
+            const double rhs = phit + d[0] - 0.5 * L * d[1];
+            if (verbose >= 2) {
+                printf("║ j = %d, palpha = %g, pbeta = %g\n", j, palpha, pbeta_new);
+                printf("║ phit = %g, phi_new = %g, rhs = %g\n", phit, phi_new, rhs);
+                printf("║-------------------------------------------------------------\n");
+            }
+            if (phi_new >= rhs || j >= max_inner - 1) {
+                pbeta = pbeta_new;
+                std::swap(dp, dp_new);
+                std::swap(du, du_new);
+                break;
+            }
+            L *= 2.0;
+        }
+        vec_axpby(x, -1.0, dt, 0.0, nullptr, w.N, w.s);
+        plan_pass(ptau, false);
+        const double primal_obj = hout.p[1] - p->reg * hout.p[2];
+        const double mar_err = std::sqrt(hout.p[0]);
+        res->obj_vals.push_back(f);
+        res->mar_errs.push_back(mar_err);
+        res->run_times.push_back(res->run_times.back() + timer.toc());
+        if (verbose >= 1) {
+            printf("iter = %d, primal_obj = %g, phi_new = %g, mar_err = %g\n", i, primal_obj,
+                   phi_new, mar_err);
+            fflush(stdout);
+        }
+        if (primal_obj - phi_new <= tol && mar_err <= tol) break;
+    }
+    res->niter = i;
+    w.report(res);
+    res->dual.resize(w.N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), x, w.N * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+}  // namespace regot

@@ -1,5 +1,5 @@
 // newton.cpp -- host drivers of the Newton-type EROT solvers (SSNS, sparse
-// Newton, SPLR).  Control flow, schedules and line searches run on the host
+// Newton, SPLR, and Newton with the full Hessian).  Control flow, schedules and line searches run on the host
 // in FP64 and mirror the reference drivers step for step; every vector lives
 // on the device, every O(n m) evaluation and every Hessian product or CG
 // solve is a HIP kernel, and only scalars come back to the host.
@@ -9,175 +9,10 @@
 #include <limits>
 #include <vector>
 
-#include "drivers.hpp"
-#include "hessian.hpp"
+#include "driver_work.hpp"
 
 namespace regot {
 namespace {
-
-using Clock = std::chrono::steady_clock;
-
-double ms_since(Clock::time_point t0)
-{
-    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
-}
-
-// Sets the handle's precise flag for the lifetime of a solve: objectives and
-// gradients the line searches compare are then FP64 sums even in FP32 bulk
-// (the sparsified Hessian keeps the bulk precision).
-struct PreciseScope {
-    regot_problem* p;
-    bool was;
-    explicit PreciseScope(regot_problem* pp) : p(pp), was(pp->precise) { p->precise = true; }
-    ~PreciseScope() { p->precise = was; }
-};
-
-// Device workspace of one Newton-type solve.
-struct Work {
-    regot_problem* p;
-    int n, m;
-    long N;
-    hipStream_t s;
-    DBuf<double> gamma, g, direc, step, Hstep, x, xg, rhs, y, sv, g_pre, gamma_pre, lr0, lr1;
-    DBuf<double> dscal;
-    HBuf<double> hscal;
-    long long cg_iters = 0;
-    // per-phase wall time (each phase ends in a host synchronisation)
-    double phase_ms[3] = {0.0, 0.0, 0.0};
-    long long phase_n[3] = {0, 0, 0};
-
-    explicit Work(regot_problem* pp)
-        : p(pp), n(pp->n), m(pp->m), N(pp->n + pp->m - 1), s(pp->stream)
-    {
-        for (DBuf<double>* b : {&gamma, &g, &direc, &step, &Hstep, &x, &xg, &rhs, &y, &sv, &g_pre,
-                                &gamma_pre, &lr0, &lr1})
-            b->alloc(N);
-        dscal.alloc(8);
-        hscal.ensure(8);
-    }
-
-    // Reference init (e.g. src/sinkhorn_ssns.cpp:53-62): alpha = 0,
-    // beta = optimal_beta(alpha), gamma normalised; or the caller's x0.
-    void init(const regot_opts& o)
-    {
-        if (o.x0) {
-            HIP_CHECK(hipMemcpyAsync(gamma, o.x0, N * sizeof(double), hipMemcpyHostToDevice, s));
-            return;
-        }
-        launch_fill(p->alpha, 0.0, n, s);
-        ColEpi c;
-        c.beta_out = p->beta;
-        p->col_pass(p->alpha, nullptr, c);
-        vec_normalise(p->alpha, p->beta, n, m, gamma, s);
-    }
-
-    double dot(const double* a, const double* b)
-    {
-        const double* xs[1] = {a};
-        const double* ys[1] = {b};
-        vec_dots(xs, ys, N, 1, dscal, s);
-        HIP_CHECK(hipMemcpyAsync(hscal.p, dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        return hscal.p[0];
-    }
-
-    // Several dots in one launch.
-    void dots(int k, const double* const* a, const double* const* b, double* out)
-    {
-        vec_dots(a, b, N, k, dscal, s);
-        HIP_CHECK(hipMemcpyAsync(hscal.p, dscal.p, k * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        for (int q = 0; q < k; q++) out[q] = hscal.p[q];
-    }
-
-    // out = x + t d
-    void move(double* out, const double* x0, double t, const double* d)
-    {
-        vec_axpby(out, 1.0, x0, t, d, N, s);
-    }
-
-    double obj(const double* at)
-    {
-        const auto t0 = Clock::now();
-        const double f = p->dev_obj(at);
-        phase_ms[0] += ms_since(t0);
-        phase_n[0] += 1;
-        return f;
-    }
-
-    double obj_grad(const double* at, double* grad, double* gnorm)
-    {
-        const auto t0 = Clock::now();
-        double gn2 = 0.0;
-        const double f = p->dev_obj_grad(at, grad, &gn2);
-        if (gnorm) *gnorm = std::sqrt(gn2);
-        phase_ms[0] += ms_since(t0);
-        phase_n[0] += 1;
-        return f;
-    }
-
-    void threshold(double delta, double hint)
-    {
-        const auto t0 = Clock::now();
-        hess_threshold(p, gamma, g, delta, hint);
-        phase_ms[1] += ms_since(t0);
-        phase_n[1] += 1;
-    }
-
-    void topk(double density)
-    {
-        const auto t0 = Clock::now();
-        hess_topk(p, gamma, g, density);
-        phase_ms[1] += ms_since(t0);
-        phase_n[1] += 1;
-    }
-
-    void report(regot_result* res) const
-    {
-        res->cg_iters = cg_iters;
-        for (int q = 0; q < 3; q++) {
-            res->phase_ms[q] = phase_ms[q];
-            res->phase_n[q] = phase_n[q];
-        }
-    }
-
-    // (Hs + shift I)^{-1} rhs.  method 0 is the reference's CG (relative
-    // residual 1e-8, Eigen's default cap of 2 * HessianCG::cols() = 2m, no
-    // preconditioner).  The reference's default method (SimplicialLDLT) is an
-    // exact solve; a Jacobi-preconditioned CG stands in for it on the GPU with a
-    // 1e-10 residual and a cap high enough that the tolerance, not the cap,
-    // ends it on ill-conditioned Schur complements.
-    void lin_solve(const regot_opts& o, const double* r, double shift, double* out)
-    {
-        const double tol = o.cg_tol > 0 ? o.cg_tol : (o.method == 0 ? 1e-8 : 1e-10);
-        const int maxit = o.cg_max_iter > 0 ? o.cg_max_iter
-                                            : (o.method == 0 ? 2 * m : 10 * m + 1000);
-        const auto t0 = Clock::now();
-        cg_iters += hess_cg(p, r, shift, tol, maxit, out, nullptr, nullptr, o.method != 0);
-        phase_ms[2] += ms_since(t0);
-        phase_n[2] += 1;
-    }
-
-    // Inverse-BFGS correction of (Hs + shift I)^{-1} applied to r, with the
-    // curvature pair (s, y).  src/sinkhorn_linsolve.cpp:160-225 (two solves, then
-    // res = A^-1 r + ((1 + y'A^-1y / y's) s'r/y's - y'A^-1r / y's) s - (s'r/y's) A^-1 y).
-    void lin_solve_low_rank(const regot_opts& o, const double* r, double shift, const double* yv,
-                            const double* sv_, double* out)
-    {
-        lin_solve(o, r, shift, lr0);
-        lin_solve(o, yv, shift, lr1);
-        const double* xs[4] = {sv_, yv, yv, yv};
-        const double* ys[4] = {r, sv_, lr1.p, lr0.p};
-        double d[4];
-        dots(4, xs, ys, d);
-        const double rs = d[0], ys_ = d[1], yiAy = d[2], yiAr = d[3];
-        const double rs_ys = rs / ys_;
-        const double cs = (1.0 + yiAy / ys_) * rs_ys - yiAr / ys_;
-        // out = lr0 + cs s - rs_ys lr1
-        vec_axpby(out, 1.0, lr0, cs, sv_, N, s);
-        vec_axpy(out, -rs_ys, lr1, N, s);
-    }
-};
 
 // Minimiser of the quadratic through fa, ga (at a) and fb (at b);
 // src/sinkhorn_problem_ls.cpp:72-77.
@@ -365,11 +200,6 @@ double wolfe(Work& w, double init_step, double f0, double dg0, bool* reuse_last,
     return I_lo;
 }
 
-struct Timer {
-    Clock::time_point t0;
-    void tic() { t0 = Clock::now(); }
-    double toc() { return ms_since(t0); }
-};
 
 }  // namespace
 
@@ -540,6 +370,70 @@ void solve_newton_family(regot_problem* p, bool splr, double tol, int max_iter, 
         res->mar_errs.push_back(gnorm);
         res->run_times.push_back(res->run_times.back() + timer.toc());
         res->densities.push_back(density_T2H(density));
+    }
+    res->niter = i;
+    w.report(res);
+    res->dual.resize(w.N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), w.gamma.p, w.N * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+// Newton's method with the full dual Hessian.  Mirrors sinkhorn_newton_internal,
+// src/sinkhorn_newton.cpp:17-129: direction -H^{-1} g, Armijo backtracking on
+// the objective (theta = 0.5, kappa = 0.5, at most 20 trials, the last trial
+// taken when none passes), then f, g and H at the new point; stop on
+// ||g|| < tol or a non-finite f.  The reference forms H densely
+// (src/sinkhorn_problem_obj.cpp:396-418) and factors it with Eigen::LLT; here
+// H is the sparsified Hessian with every entry kept (top-k at density 1) and
+// the direction comes from the Jacobi-preconditioned device CG on its Schur
+// complement, run to a 1e-12 relative residual (cg_tol / cg_max_iter
+// override), which reproduces the exact solve to that accuracy.
+void solve_newton_dense(regot_problem* p, double tol, int max_iter, int verbose,
+                        const regot_opts& o, regot_result* res)
+{
+    const double theta = 0.5, kappa = 0.5;
+    const int nlinesearch = 20;
+    const double cg_tol = o.cg_tol > 0 ? o.cg_tol : 1e-12;
+    const int cg_max = o.cg_max_iter > 0 ? o.cg_max_iter : 10 * p->m + 1000;
+    PreciseScope ps(p);
+    Work w(p);
+    w.init(o);
+    Timer timer;
+    timer.tic();
+    double gnorm;
+    double f = w.obj_grad(w.gamma, w.g, &gnorm);
+    w.topk(1.0);
+    res->obj_vals.push_back(f);
+    res->mar_errs.push_back(gnorm);
+    res->run_times.push_back(timer.toc());
+    int i;
+    for (i = 0; i < max_iter; i++) {
+        if (verbose >= 1) {
+            printf("iter = %d, objval = %g, ||grad|| = %g\n", i, f, gnorm);
+            fflush(stdout);
+        }
+        timer.tic();
+        if (gnorm < tol || !std::isfinite(f)) break;
+        vec_axpby(w.rhs, -1.0, w.g, 0.0, nullptr, w.N, w.s);
+        {
+            const auto t0 = Clock::now();
+            w.cg_iters += hess_cg(p, w.rhs, 0.0, cg_tol, cg_max, w.direc, nullptr, nullptr, true);
+            w.phase_ms[2] += ms_since(t0);
+            w.phase_n[2] += 1;
+        }
+        double step = 1.0;
+        const double thresh = theta * w.dot(w.g, w.direc);
+        for (int k = 0; k < nlinesearch; k++) {
+            w.move(w.x, w.gamma, step, w.direc);
+            const double newf = w.obj(w.x);
+            if (newf <= f + step * thresh) break;
+            step *= kappa;
+        }
+        vec_copy(w.gamma, w.x, w.N, w.s);
+        f = w.obj_grad(w.gamma, w.g, &gnorm);
+        w.topk(1.0);
+        res->obj_vals.push_back(f);
+        res->mar_errs.push_back(gnorm);
+        res->run_times.push_back(res->run_times.back() + timer.toc());
     }
     res->niter = i;
     w.report(res);

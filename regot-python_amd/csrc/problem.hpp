@@ -34,6 +34,23 @@ void vec_objgrad_finish(const double* r, const double* c, const double* a, const
 void vec_obj_finish(const double* r, const double* a, const double* b, const double* gamma,
                     int n, int m, double reg, double* out, hipStream_t s);
 
+// res = H (a v), H the L-BFGS inverse-Hessian operator of ncorr pairs in a
+// ring of mm <= 8 slots (S, Y: mm x count, slot j at j * count; ys host
+// array of s'y per slot; ptr = slot after the newest pair), H0 = I / theta.
+void vec_lbfgs_apply(const double* S, const double* Y, const double* ys, int mm, int ncorr,
+                     int ptr, double theta, const double* v, double a, double* res, long count,
+                     hipStream_t s);
+
+// APDAGD primal-plan pass (apdagd_kernels.hip): P <- (1 - tau) P + tau T
+// (P <- T when init), T_ij = exp((alpha_i + beta_j - M_ij) / reg) in FP64,
+// P row-major with pitch ldp (even, >= m); out[0..2] = squared full marginal
+// error, <M, P>, sum P (1 - log P).  Scratch sizes from apd_scratch_sizes.
+void apd_scratch_sizes(int n, int m, long* rowpart, long* colpart, long* scal, long* part);
+void launch_apd_plan(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                     const double* beta, const double* a, const double* b, double reg,
+                     double tau, bool init, double* P, long ldp, double* rowpart,
+                     double* colpart, double* scal, double* part, double* out, hipStream_t s);
+
 // Buffers of the on-device BCD loop, kept across solves on one handle.
 struct BcdWork {
     DBuf<double> ap0, ap1, beta, gcol, rpart, cpart, bm;

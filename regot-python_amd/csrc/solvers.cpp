@@ -252,6 +252,15 @@ int regot_solve(regot_problem* p, int solver, double tol, int max_iter, int verb
         case REGOT_SOLVER_SPLR:
             solve_newton_family(p, true, tol, max_iter, verbose, o, res.get());
             break;
+        case REGOT_SOLVER_NEWTON:
+            solve_newton_dense(p, tol, max_iter, verbose, o, res.get());
+            break;
+        case REGOT_SOLVER_LBFGS_DUAL:
+            solve_lbfgs_dual(p, tol, max_iter, verbose, o, res.get());
+            break;
+        case REGOT_SOLVER_APDAGD:
+            solve_apdagd(p, tol, max_iter, verbose, o, res.get());
+            break;
         default:
             fail(E_INVALID, "unknown solver id %d", solver);
     }
@@ -292,7 +301,14 @@ int regot_result_plan(regot_problem* p, const regot_result* r, double* plan)
     if (!r) return set_error(E_INVALID, "null result");
     if (!p || p->n != r->n || p->m != r->m)
         return set_error(E_INVALID, "problem and result sizes differ");
-    return regot_get_plan(p, r->dual.data(), plan);
+    if (!r->plan.p) return regot_get_plan(p, r->dual.data(), plan);
+    // a solver that keeps its own primal plan (APDAGD)
+    CAPI_BEGIN
+    if (!plan) fail(E_INVALID, "plan is null");
+    HIP_CHECK(hipSetDevice(r->plan_device));
+    HIP_CHECK(hipMemcpy2D(plan, sizeof(double) * r->m, r->plan.p, sizeof(double) * r->plan_ld,
+                          sizeof(double) * r->m, r->n, hipMemcpyDeviceToHost));
+    CAPI_END
 }
 
 int regot_result_verify(regot_problem* p, const regot_result* r, double* mar_err_fp64,

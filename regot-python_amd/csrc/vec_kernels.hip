@@ -90,6 +90,53 @@ __global__ void k_normalise(const double* alpha, const double* beta, int n, int 
     if (t < m - 1) gamma[n + t] = beta[t] - bm;
 }
 
+// L-BFGS two-loop product res = H (a v) with the inverse-Hessian operator of
+// the last ncorr pairs (s_j, y_j) in a ring of mm slots (slot j at S + j N),
+// H0 = I / theta.  Order of operations as BFGSMat::apply_Hv,
+// src/LBFGSpp/LBFGSpp/BFGSMat.h:276-302.  One 1024-thread workgroup: every
+// thread owns the indices i = t mod 1024, so only the dot products need the
+// workgroup (fixed-order tree, reproducible).
+struct LbfgsArgs {
+    const double* S;
+    const double* Y;
+    long N;
+    int mm, ncorr, ptr;
+    double theta;
+    double ys[8];
+};
+
+__global__ __launch_bounds__(1024) void k_lbfgs_apply(LbfgsArgs L, const double* __restrict__ v,
+                                                      double a, double* __restrict__ res)
+{
+    __shared__ double sm[1024];
+    __shared__ double al[8];
+    const long N = L.N;
+    for (long i = threadIdx.x; i < N; i += 1024) res[i] = a * v[i];
+    int j = L.ptr % L.mm;
+    for (int k = 0; k < L.ncorr; k++) {
+        j = (j + L.mm - 1) % L.mm;
+        const double* sj = L.S + static_cast<long>(j) * N;
+        const double* yj = L.Y + static_cast<long>(j) * N;
+        double d = 0.0;
+        for (long i = threadIdx.x; i < N; i += 1024) d += sj[i] * res[i];
+        const double aj = block_reduce(d, sm) / L.ys[j];
+        if (threadIdx.x == 0) al[j] = aj;
+        for (long i = threadIdx.x; i < N; i += 1024) res[i] -= aj * yj[i];
+    }
+    for (long i = threadIdx.x; i < N; i += 1024) res[i] /= L.theta;
+    __syncthreads();
+    for (int k = 0; k < L.ncorr; k++) {
+        const double* sj = L.S + static_cast<long>(j) * N;
+        const double* yj = L.Y + static_cast<long>(j) * N;
+        double d = 0.0;
+        for (long i = threadIdx.x; i < N; i += 1024) d += yj[i] * res[i];
+        const double bet = block_reduce(d, sm) / L.ys[j];
+        const double c = al[j] - bet;
+        for (long i = threadIdx.x; i < N; i += 1024) res[i] += c * sj[i];
+        j = (j + 1) % L.mm;
+    }
+}
+
 int grid_for(long count)
 {
     long g = (count + 255) / 256;
@@ -158,6 +205,24 @@ void vec_obj_finish(const double* r, const double* a, const double* b, const dou
 {
     hipLaunchKernelGGL(k_objgrad_finish, dim3(1), dim3(1024), 0, s, r, nullptr, a, b, gamma, n,
                        m, reg, nullptr, out);
+    KERNEL_CHECK();
+}
+
+void vec_lbfgs_apply(const double* S, const double* Y, const double* ys, int mm, int ncorr,
+                     int ptr, double theta, const double* v, double a, double* res, long count,
+                     hipStream_t s)
+{
+    if (mm < 1 || mm > 8 || ncorr < 0 || ncorr > mm) fail(E_INTERNAL, "L-BFGS memory %d/%d", ncorr, mm);
+    LbfgsArgs L{};
+    L.S = S;
+    L.Y = Y;
+    L.N = count;
+    L.mm = mm;
+    L.ncorr = ncorr;
+    L.ptr = ptr;
+    L.theta = theta;
+    for (int q = 0; q < mm; q++) L.ys[q] = ys[q];
+    hipLaunchKernelGGL(k_lbfgs_apply, dim3(1), dim3(1024), 0, s, L, v, a, res);
     KERNEL_CHECK();
 }
 

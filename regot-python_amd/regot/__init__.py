@@ -511,3 +511,22 @@ def sinkhorn_sparse_newton(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **k
 def sinkhorn_splr(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
     """Sparse-plus-low-rank quasi-Newton (src/solvers.cpp:443-459, src/sinkhorn_splr.cpp:24-199)."""
     return _run(_lib.SOLVER_SPLR, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def sinkhorn_newton(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Newton's method with the full dual Hessian and Armijo backtracking
+    (src/solvers.cpp:391-405, src/sinkhorn_newton.cpp:17-129)."""
+    return _run(_lib.SOLVER_NEWTON, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def sinkhorn_lbfgs_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """L-BFGS on the dual (src/solvers.cpp:375-389, src/sinkhorn_lbfgs_dual.cpp:84-125).
+    A failed line search raises RuntimeError with LBFGSpp's message."""
+    return _run(_lib.SOLVER_LBFGS_DUAL, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def sinkhorn_apdagd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Accelerated primal-dual adaptive gradient descent; ``plan`` is the
+    averaged primal plan (src/solvers.cpp:343-357, src/sinkhorn_apdagd.cpp:14-183)."""
+    return _run(_lib.SOLVER_APDAGD, M, a, b, reg, tol, max_iter, verbose, kwargs)
+

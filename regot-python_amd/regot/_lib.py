@@ -16,6 +16,9 @@ SOLVER_BCD = 0
 SOLVER_SSNS = 1
 SOLVER_SPARSE_NEWTON = 2
 SOLVER_SPLR = 3
+SOLVER_NEWTON = 4
+SOLVER_LBFGS_DUAL = 5
+SOLVER_APDAGD = 6
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
