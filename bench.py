@@ -11,7 +11,9 @@ better).  `batched` reports the batched workload (512 problems of uneven
 n, m ~ U{64..4096} in one batched BCD solve to mar_err <= 1e-6, solves/s)
 beside the same problems solved one after another by the single-problem path.
 `newton` times SSNS (same recipe, reg 0.01), sparse Newton and SPLR (mixture
-clouds in R^32, reg 0.001, density 1%) at the same n = m, to the same tol.
+clouds in R^32, reg 0.001, density 1%) at the same n = m, to the same tol;
+`first_order` times Newton with the full Hessian and L-BFGS on the headline
+problem, and APDAGD on the same recipe at n = m = 4096 to 1e-5.
 `differentiable` reports the differentiable loss between two BF16 clouds of
 4096 points in R^64 (reg 0.05, tol 1e-6): matrix-core cost kernel, forward
 (cost + solve + implicit adjoint) and backward, beside the plain solve.  `roofline` is the dominant streaming kernel's achieved HBM rate from
@@ -251,6 +253,39 @@ def bench_newton(n, tol):
     return out
 
 
+def bench_first_order(P, n, tol):
+    """Newton with the full Hessian and L-BFGS on the BCD workload's problem
+    handle (n = m, reg 0.01), APDAGD on the same recipe at n = m = 4096 to
+    mar_err <= 1e-5 (a first-order method: thousands of iterations); one timed
+    solve each after a warmup, with per-phase times."""
+    import regot
+    from regot import _lib
+    out = {}
+
+    def run(name, Q, sid, t, mi, nn):
+        try:
+            Q.solve(sid, tol=t, max_iter=mi, compute_plan=False)
+            t0 = time.perf_counter()
+            r = Q.solve(sid, tol=t, max_iter=mi, compute_plan=False)
+            dt = time.perf_counter() - t0
+        except RuntimeError as e:
+            out[name] = {"n": nn, "tol": t, "error": str(e)}
+            return
+        out[name] = {"n": nn, "reg": Q.reg, "tol": t, "s": dt, "niter": r.niter,
+                     "mar_err": r.mar_errs[-1],
+                     **{k: (float(v) if isinstance(v, (float, np.floating)) else v)
+                        for k, v in r.stats.items()}}
+        print("bench: %s %.3f s, %d iterations" % (name, dt, r.niter), file=sys.stderr,
+              flush=True)
+
+    run("newton", P, _lib.SOLVER_NEWTON, tol, 200, n)
+    run("lbfgs_dual", P, _lib.SOLVER_LBFGS_DUAL, tol, 5000, n)
+    M4, a4, b4 = make_problem(4096, 4096, 16, 0)
+    with regot.Problem(M4, a4, b4, 0.01) as Q:
+        run("apdagd", Q, _lib.SOLVER_APDAGD, 1e-5, 5000, 4096)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,7 +305,10 @@ def main():
     ap.add_argument("--diff-steps", type=int, default=2)
     ap.add_argument("--newton", type=int, default=1,
                     help="time SSNS, sparse Newton and SPLR at n = m = --n (0 = skip)")
-    ap.add_argument("--only", choices=["batched", "differentiable", "newton"], default=None,
+    ap.add_argument("--first-order", type=int, default=1,
+                    help="time dense Newton, L-BFGS and APDAGD (0 = skip)")
+    ap.add_argument("--only", choices=["batched", "differentiable", "newton", "first_order"],
+                    default=None,
                     help="run one side workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -287,6 +325,11 @@ def main():
         return
     if args.only == "newton":
         print(json.dumps({"newton": bench_newton(args.n, args.tol)}))
+        return
+    if args.only == "first_order":
+        M32, a, b = make_problem(args.n, args.n, 16, 0)
+        with regot.Problem(M32, a, b, args.reg) as P:
+            print(json.dumps({"first_order": bench_first_order(P, args.n, args.tol)}))
         return
     if args.only == "differentiable":
         print(json.dumps({"differentiable": bench_differentiable(args.diff_n, 64, 0.05, 1e-6,
@@ -340,6 +383,10 @@ def main():
                          "(%.3f s/iter), scaled to the GPU's %d iterations"
                          % (args.cpu_iters, n, per_iter, niter)}
 
+    first_order = None
+    if args.first_order:
+        first_order = bench_first_order(P, n, args.tol)
+
     newton = None
     if args.newton:
         del P
@@ -386,6 +433,7 @@ def main():
                      "bytes_per_launch": bytes_per_pass},
         "cpu_baseline": cpu,
         "newton": newton,
+        "first_order": first_order,
         "batched": batched,
         "differentiable": diff,
     }

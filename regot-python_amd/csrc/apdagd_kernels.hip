@@ -9,7 +9,7 @@
 // sums, column sums, <M, P> and the entropy term, in FP64 throughout (T with
 // FP64 exp regardless of the bulk precision, as the reference forms it).
 //
-// Tiling: a workgroup owns a 256-column strip of a block of APD_ROWS rows;
+// Tiling: a workgroup owns a 256-column strip of a block of rows (16-256);
 // each lane owns 4 adjacent columns (one 16-byte load of FP32 M, two of P per
 // row) and its 4 waves take the block's rows round robin, so every wave load
 // is a coalesced row segment.  Row sums are reduced across the wave per row
@@ -23,7 +23,15 @@
 namespace regot {
 namespace {
 
-constexpr int APD_ROWS = 128;
+// Rows per workgroup: enough workgroups to fill the chip (>= 2048 at 4 waves
+// each), at most 256 rows so the column partials stay small (<= n/16 blocks).
+int apd_rows(int n, int m)
+{
+    const long nstrip = ceil_div(m, 256);
+    long r = static_cast<long>(n) * nstrip / 2048;
+    r = std::max(16L, std::min(256L, r));
+    return static_cast<int>((r + 3) / 4 * 4);
+}
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -73,15 +81,16 @@ __device__ __forceinline__ void st4(double* p, int c0, int m, const double (&v)[
     }
 }
 
-// grid (strips = ceil(m / 256), row blocks = ceil(n / APD_ROWS)), 256 threads.
+// grid (strips = ceil(m / 256), row blocks = ceil(n / rows)), 256 threads.
 // beta has m entries (beta_{m-1} = 0).  init = 1: P = T (P is not read).
 // rowpart[strip * n + i], colpart[block * m + j], scal[2 * wg + {0, 1}] =
 // this workgroup's sum of M P and of P (1 - log P).
 template <typename TM>
 __global__ __launch_bounds__(256) void k_apd_plan(const TM* __restrict__ M, long ldm, int n, int m,
                                                   const double* __restrict__ alpha,
-                                                  const double* __restrict__ beta, double reg,
-                                                  double tau, int init, double* __restrict__ P,
+                                                  const double* __restrict__ beta, int rows,
+                                                  double reg, double tau, int init,
+                                                  double* __restrict__ P,
                                                   long ldp, double* __restrict__ rowpart,
                                                   double* __restrict__ colpart,
                                                   double* __restrict__ scal)
@@ -91,8 +100,8 @@ __global__ __launch_bounds__(256) void k_apd_plan(const TM* __restrict__ M, long
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int strip = blockIdx.x, blk = blockIdx.y;
     const int c0 = strip * 256 + lane * 4;
-    const int r0 = blk * APD_ROWS;
-    const int r1 = min(n, r0 + APD_ROWS);
+    const int r0 = blk * rows;
+    const int r1 = min(n, r0 + rows);
     double b4[4];
     ld4(beta, c0, m, b4);
     double cs[4] = {0.0, 0.0, 0.0, 0.0};
@@ -209,7 +218,7 @@ __global__ __launch_bounds__(256) void k_apd_final(const double* __restrict__ pa
 // Scratch sizes (doubles) for a plan pass on an n x m problem.
 void apd_scratch_sizes(int n, int m, long* rowpart, long* colpart, long* scal, long* part)
 {
-    const long nstrip = ceil_div(m, 256), nblk = ceil_div(n, APD_ROWS);
+    const long nstrip = ceil_div(m, 256), nblk = ceil_div(n, apd_rows(n, m));
     *rowpart = nstrip * n;
     *colpart = nblk * m;
     *scal = 2 * nstrip * nblk;
@@ -224,15 +233,16 @@ void launch_apd_plan(int bulk, const void* M, long ldm, int n, int m, const doub
     if (n <= 0 || m <= 0) fail(E_INVALID, "plan pass on an empty problem");
     if (ldp % 2 != 0 || ldp < m) fail(E_INTERNAL, "plan pitch %ld for m = %d", ldp, m);
     const int nstrip = ceil_div(m, 256);
-    const int nblk = ceil_div(n, APD_ROWS);
+    const int rows = apd_rows(n, m);
+    const int nblk = ceil_div(n, rows);
     const dim3 grid(nstrip, nblk);
     if (bulk == 0)
         hipLaunchKernelGGL(k_apd_plan<float>, grid, dim3(256), 0, s,
-                           static_cast<const float*>(M), ldm, n, m, alpha, beta, reg, tau,
+                           static_cast<const float*>(M), ldm, n, m, alpha, beta, rows, reg, tau,
                            init ? 1 : 0, P, ldp, rowpart, colpart, scal);
     else
         hipLaunchKernelGGL(k_apd_plan<double>, grid, dim3(256), 0, s,
-                           static_cast<const double*>(M), ldm, n, m, alpha, beta, reg, tau,
+                           static_cast<const double*>(M), ldm, n, m, alpha, beta, rows, reg, tau,
                            init ? 1 : 0, P, ldp, rowpart, colpart, scal);
     KERNEL_CHECK();
     const int npart = ceil_div(std::max(n, m), 256);
