@@ -101,18 +101,30 @@ def bench_batched(P, steps, tol, sequential=True):
     ns, ms, regs, As, Bs, dev = batched_workload(P)
     descs = [(t.data_ptr(), t.shape[0], t.shape[1], np.float32) for t in dev]
     B = regot.BatchProblem(descs, As, Bs, regs)
-    out = B.solve_bcd(tol=tol, max_iter=100000)          # warmup
+    # warmup; the first solve on the handle also autotunes the coarse tiles
+    out = B.solve_bcd(tol=tol, max_iter=100000, autotune=True, graph=False)
     t0 = time.perf_counter()
     for _ in range(steps):
-        out = B.solve_bcd(tol=tol, max_iter=100000)
+        out = B.solve_bcd(tol=tol, max_iter=100000, autotune=True, graph=False)
     dt = (time.perf_counter() - t0) / steps
+    # the same rounds as hipGraph replays (8 rounds per launch)
+    B.solve_bcd(tol=tol, max_iter=100000, autotune=True, graph=True)
+    t0 = time.perf_counter()
+    og = B.solve_bcd(tol=tol, max_iter=100000, autotune=True, graph=True)
+    dt_graph = time.perf_counter() - t0
     it = out["niter"].astype(np.float64)
     streamed = float((it * ns * ms).sum()) * 2 * 4
     res = {"problems": P, "elements": int((ns * ms).sum()), "tol": tol,
            "ms_per_batch": dt * 1e3, "solves_per_s": P / dt, "rounds": out["rounds"],
            "niter_median": float(np.median(it)), "niter_max": int(it.max()),
            "mar_err_max": float(out["mar_err"].max()),
-           "streamed_GB": streamed / 1e9, "streamed_GBps": streamed / dt / 1e9}
+           "streamed_GB": streamed / 1e9, "streamed_GBps": streamed / dt / 1e9,
+           "tiles": {"rows_per_wave": out["rows_per_wave"], "block_rows": out["block_rows"],
+                     "autotuned": out["tuned"], "candidate_solve_ms": out["tune_ms"]},
+           "graph": {"ms_per_batch": dt_graph * 1e3,
+                     "builds_per_solve": og["graph_builds"] / 2,
+                     "launches_per_solve": og["graph_launches"] / 2,
+                     "same_niter": bool(np.array_equal(og["niter"], out["niter"]))}}
     B.close()
     if sequential:
         # the plain GPU form: one regot.Problem per problem, solved one after another
@@ -203,6 +215,15 @@ def bench_differentiable(n, d, reg, tol, steps):
     return res
 
 
+def hbm_rate(alg_bytes, seconds):
+    """Whole-solve HBM rate of a Newton-type or first-order solve: algorithmic
+    bytes (one read of M per objective / gradient pass, M plus the CSR and CSC
+    writes per Hessian build, CSR + CSC reads per Hessian product) over the
+    wall time of the solve, against the 8 TB/s peak."""
+    gbps = alg_bytes / seconds / 1e9
+    return {"alg_GB": alg_bytes / 1e9, "alg_GBps": gbps, "hbm_frac": gbps / HBM_PEAK_GBS}
+
+
 def c3_problem(n, seed=0):
     """Mixture-vs-Gaussian clouds in R^32: X ~ 0.5 N(-1, I) + 0.5 N(+1, I),
     Y ~ N(0, 1.5^2 I), M = sqeuclid / max in FP32, a, b = (1 + 0.2 U(-1, 1))
@@ -248,6 +269,7 @@ def bench_newton(n, tol):
                      "density_last": r.densities[-1] if r.densities else None,
                      **{k: (float(v) if isinstance(v, (float, np.floating)) else v)
                         for k, v in r.stats.items()}}
+        out[name].update(hbm_rate(r.stats["alg_bytes"], dt))
         print("bench: %s %.3f s, %d iterations" % (name, dt, r.niter), file=sys.stderr,
               flush=True)
     return out
@@ -275,6 +297,7 @@ def bench_first_order(P, n, tol):
                      "mar_err": r.mar_errs[-1],
                      **{k: (float(v) if isinstance(v, (float, np.floating)) else v)
                         for k, v in r.stats.items()}}
+        out[name].update(hbm_rate(r.stats["alg_bytes"], dt))
         print("bench: %s %.3f s, %d iterations" % (name, dt, r.niter), file=sys.stderr,
               flush=True)
 

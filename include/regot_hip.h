@@ -44,6 +44,11 @@ int regot_version(void);
 /* Number of visible GPUs (0 when the HIP runtime finds none). */
 int regot_device_count(void);
 
+/* y[i] = 2^x[i] by the device FP64 exponential of the one-pass objective /
+ * gradient and APDAGD kernels (host arrays; no reference counterpart: a
+ * self-test of that routine against a correctly rounded exp2). */
+int regot_check_exp2(const double* x, double* y, long long count, int device);
+
 /*
  * Problem handle: M (n x m, row-major, row pitch ldm elements) resident on
  * one GPU in the bulk precision, plus a, b, log a, log b and reg.
@@ -169,9 +174,12 @@ int regot_result_plan(regot_problem* p, const regot_result* r, double* plan);
  * (exp in FP64 regardless of bulk), plus the FP64 objective. */
 int regot_result_verify(regot_problem* p, const regot_result* r, double* mar_err_fp64,
                         double* obj_fp64);
-/* Per-phase statistics of the Newton-type drivers (no reference counterpart;
- * zeros for BCD): out[8] = {objective/gradient ms, count, sparsification ms,
- * count, CG ms, CG solves, CG iterations, 0}.  Host wall time per phase. */
+/* Per-phase statistics of the Newton-type and first-order drivers (no
+ * reference counterpart; zeros for BCD): out[8] = {objective/gradient ms,
+ * count, sparsification ms, count, CG ms, CG solves, CG iterations,
+ * algorithmic HBM bytes}.  Host wall time per phase; the bytes count one
+ * read of M per objective or gradient, one read of M plus the CSR and CSC
+ * writes per Hessian build and the CSR + CSC reads per Hessian product. */
 int regot_result_stats(const regot_result* r, double* out);
 void regot_result_free(regot_result* r);
 
@@ -194,6 +202,27 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
                        const double* const* a, const double* const* b, const double* reg,
                        int device);
 int regot_batch_destroy(regot_batch* batch);
+/* Options of the batched solver (no reference counterpart):
+ * REGOT_BATCH_GRAPH     1: every group of 8 rounds is one hipGraph launch
+ *                       (captured once per tile-table configuration and
+ *                       replayed); 0 (default): the rounds' kernels are
+ *                       launched one by one, which measured slightly faster
+ *                       (the GPU, not the launch path, bounds the rounds, and
+ *                       shrinking tables force re-captures).  Results do not
+ *                       depend on this option.
+ * REGOT_BATCH_AUTOTUNE  1: the next solve first runs one whole solve per
+ *                       coarse-tile candidate (rows per wave {2, 4} FP32 /
+ *                       {1, 2} FP64 x column-block rows {256, 512, 1024}) and
+ *                       keeps the fastest for the handle; 0 (default): fixed
+ *                       rules (4 / 2 rows per wave, 512 rows).  Results depend
+ *                       on the tiles at rounding level. */
+#define REGOT_BATCH_GRAPH 0
+#define REGOT_BATCH_AUTOTUNE 1
+int regot_batch_set_option(regot_batch* batch, int option, long long value);
+/* out[11]: rows per wave, column-block rows, tuned (0/1), graph
+ * instantiations, graph launches, then the six candidates' solve times (ms;
+ * 0 for candidates not run). */
+int regot_batch_stats(const regot_batch* batch, double* out);
 /* Number of problems, total n*m elements, and the packed dual length
  * sum_p (n[p] + m[p] - 1). */
 int regot_batch_info(const regot_batch* batch, int* nprob, long long* total_elems,

@@ -7,7 +7,8 @@
 // marginals and primal value  <M, P> - reg sum P (1 - log P)  (:146-152).
 // One streaming pass does all of it: read M and P, write P, and reduce row
 // sums, column sums, <M, P> and the entropy term, in FP64 throughout (T with
-// FP64 exp regardless of the bulk precision, as the reference forms it).
+// an FP64 exponential regardless of the bulk precision, as the reference
+// forms it: 2^((alpha_i + beta_j - M_ij) log2(e) / reg) by dexp2.hpp).
 //
 // Tiling: a workgroup owns a 256-column strip of a block of rows (16-256);
 // each lane owns 4 adjacent columns (one 16-byte load of FP32 M, two of P per
@@ -18,6 +19,7 @@
 // blocks in a fixed order (no atomics: reproducible).
 #include <algorithm>
 
+#include "dexp2.hpp"
 #include "problem.hpp"
 
 namespace regot {
@@ -89,7 +91,7 @@ template <typename TM>
 __global__ __launch_bounds__(256) void k_apd_plan(const TM* __restrict__ M, long ldm, int n, int m,
                                                   const double* __restrict__ alpha,
                                                   const double* __restrict__ beta, int rows,
-                                                  double reg, double tau, int init,
+                                                  double k, double tau, int init,
                                                   double* __restrict__ P,
                                                   long ldp, double* __restrict__ rowpart,
                                                   double* __restrict__ colpart,
@@ -97,6 +99,9 @@ __global__ __launch_bounds__(256) void k_apd_plan(const TM* __restrict__ M, long
 {
     __shared__ double sm[4][256];
     __shared__ double ss[2][4];
+    __shared__ double tab[128];
+    dev::exp2_tab_load(tab);
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int strip = blockIdx.x, blk = blockIdx.y;
     const int c0 = strip * 256 + lane * 4;
@@ -117,7 +122,7 @@ __global__ __launch_bounds__(256) void k_apd_plan(const TM* __restrict__ M, long
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             if (c0 + q < m) {
-                const double t = exp((ai + b4[q] - mv[q]) / reg);
+                const double t = dev::dexp2((ai + b4[q] - mv[q]) * k, tab);
                 const double v = init ? t : keep * pv[q] + tau * t;
                 pv[q] = v;
                 rs += v;
@@ -236,13 +241,14 @@ void launch_apd_plan(int bulk, const void* M, long ldm, int n, int m, const doub
     const int rows = apd_rows(n, m);
     const int nblk = ceil_div(n, rows);
     const dim3 grid(nstrip, nblk);
+    const double k = 1.4426950408889634074 / reg;
     if (bulk == 0)
         hipLaunchKernelGGL(k_apd_plan<float>, grid, dim3(256), 0, s,
-                           static_cast<const float*>(M), ldm, n, m, alpha, beta, rows, reg, tau,
+                           static_cast<const float*>(M), ldm, n, m, alpha, beta, rows, k, tau,
                            init ? 1 : 0, P, ldp, rowpart, colpart, scal);
     else
         hipLaunchKernelGGL(k_apd_plan<double>, grid, dim3(256), 0, s,
-                           static_cast<const double*>(M), ldm, n, m, alpha, beta, rows, reg, tau,
+                           static_cast<const double*>(M), ldm, n, m, alpha, beta, rows, k, tau,
                            init ? 1 : 0, P, ldp, rowpart, colpart, scal);
     KERNEL_CHECK();
     const int npart = ceil_div(std::max(n, m), 256);

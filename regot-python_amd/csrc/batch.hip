@@ -23,6 +23,7 @@
 // is read through the caches once the active matrices fit the 256 MB
 // Infinity Cache.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -83,12 +84,13 @@ struct Vecs {
 template <typename TM, int R>
 __global__ __launch_bounds__(256) void k_batch_row(const TM* __restrict__ M,
                                                    const Desc* __restrict__ D,
-                                                   const int2* __restrict__ tiles, Vecs v, int k,
-                                                   int nt)
+                                                   const int2* __restrict__ tiles, Vecs v,
+                                                   const int* __restrict__ kp, int nt)
 {
     const int2 tl = tiles[blockIdx.x];
     const int p = tl.x;
     if (v.done[p]) return;
+    const int k = *kp;
     const Desc d = D[p];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -136,12 +138,14 @@ template <typename TM>
 __global__ __launch_bounds__(256) void k_batch_colpart(const TM* __restrict__ M,
                                                        const Desc* __restrict__ D,
                                                        const int4* __restrict__ tiles, Vecs v,
-                                                       int k, int init, int nt)
+                                                       const int* __restrict__ kp, int init,
+                                                       int nt)
 {
     constexpr int TW = 64 * VecOf<TM>::N;
     const int4 tl = tiles[blockIdx.x];
     const int p = tl.x;
     if (!init && v.done[p]) return;
+    const int k = *kp;
     const Desc d = D[p];
     const double* alpha = (((k + 1) & 1) ? v.ap1 : v.ap0) + d.aoff;   // alpha_{k+1}
     const long rb = static_cast<long>(tl.z) * d.rpb;
@@ -176,11 +180,13 @@ __device__ void batch_decide(const Desc& d, const Vecs& v, int p, int k, double 
 
 __global__ __launch_bounds__(256) void k_batch_combine(const Desc* __restrict__ D,
                                                        const int2* __restrict__ tiles, Vecs v,
-                                                       int k, int init, double tol, int max_iter)
+                                                       const int* __restrict__ kp, int init,
+                                                       double tol, int max_iter)
 {
     const int2 tl = tiles[blockIdx.x];
     const int p = tl.x;
     if (!init && v.done[p]) return;
+    const int k = *kp;
     const Desc d = D[p];
     if (!init) {
         double f, gn;
@@ -251,11 +257,19 @@ __global__ void k_pack(const Tin* __restrict__ src, long lds, Tout* __restrict__
     }
 }
 
+// Advances the device round counter read by the round's kernels; the last
+// launch of every round, so one captured sequence of rounds replays as is.
+__global__ void k_round_next(int* kp)
+{
+    if (threadIdx.x == 0) *kp += 1;
+}
+
 __global__ void k_batch_init(double* ap0, const double* a, double* loga, long na,
                              const double* b, double* logb, long nb, int* done, int* niter,
-                             int np)
+                             int np, int* kp)
 {
     const long t = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (t == 0) *kp = -1;   // the init round (beta_0) runs as round -1
     if (t < na) {
         ap0[t] = 0.0;
         loga[t] = log(a[t]);
@@ -282,7 +296,7 @@ struct regot_batch {
     DBuf<unsigned char> M;
     DBuf<Desc> d_desc;
     DBuf<double> a, b, loga, logb, ap0, ap1, beta, rpart, part, cpart, bm, fval, gval;
-    DBuf<int> done, niter;
+    DBuf<int> done, niter, kdev;
     DBuf<int2> t_row, t_comb;
     DBuf<int4> t_col;
     HBuf<int> h_done;
@@ -293,11 +307,26 @@ struct regot_batch {
     hipEvent_t ev[2] = {nullptr, nullptr};
     long long total_elems = 0;
     bool fine = false;   // tile granularity of the current tables
+    // coarse tiles: rows per wave of the row tiles and rows per column-pass
+    // block; fixed rules unless an autotune run picked them
+    int coarse_R = 4;
+    int coarse_rows = kBlockRows;
+    bool autotune = false, tuned = false;
+    bool started_fine = false;   // the last solve began in fine tiles
+    double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool nt = true;      // nontemporal loads of M (active matrices beyond the cache)
+    // K rounds captured once per tile-table configuration (and tol,
+    // max_iter) and replayed
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    int g_key[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};
+    double g_tol = -1.0;
+    long long graph_builds = 0, graph_launches = 0;
 
     ~regot_batch()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -327,11 +356,10 @@ struct regot_batch {
     // Tile granularity of every problem's descriptor (uploaded on change).
     void set_mode(bool f)
     {
-        const int R = bulk == 0 ? 4 : 2;
         for (Desc& d : desc) {
-            d.rpt = f ? 4 : 4 * R;
+            d.rpt = f ? 4 : 4 * coarse_R;
             d.nrt = ceil_div(d.n, d.rpt);
-            d.nblk = ceil_div(d.n, f ? kBlockRowsFine : kBlockRows);
+            d.nblk = ceil_div(d.n, f ? kBlockRowsFine : coarse_rows);
             d.rpb = ceil_div(d.n, d.nblk);
         }
         fine = f;
@@ -344,18 +372,21 @@ struct regot_batch {
 
     // Tile tables of the listed problems into staging slot s; switches to
     // fine tiles when the coarse tables would be small.
-    void build_tables(const std::vector<int>& act, int s, int* nrow, int* ncol, int* ncomb)
+    void build_tables(const std::vector<int>& act, int s, int* nrow, int* ncol, int* ncomb,
+                      bool refresh = false)
     {
-        long coarse_rows = 0;
+        // the coarse / fine switch counts tiles of the default rows per wave,
+        // so it does not move with the tuned tile shape
+        const int R0 = bulk == 0 ? 4 : 2;
+        long ntiles = 0;
         double bytes = 0.0;
-        const int R = bulk == 0 ? 4 : 2;
         for (int p : act) {
-            coarse_rows += ceil_div(desc[p].n, 4 * R);
+            ntiles += ceil_div(desc[p].n, 4 * R0);
             bytes += static_cast<double>(desc[p].n) * desc[p].ldm * (bulk == 0 ? 4 : 8);
         }
         nt = bytes > static_cast<double>(256L << 20);
-        const bool want_fine = coarse_rows < kFineBelow;
-        if (want_fine != fine) set_mode(want_fine);
+        const bool want_fine = ntiles < kFineBelow;
+        if (want_fine != fine || refresh) set_mode(want_fine);
         long cr = 0, cc = 0, cb = 0;
         const int vw = bulk == 0 ? 4 : 2;
         for (int p : act) {
@@ -387,39 +418,202 @@ struct regot_batch {
                                  hipMemcpyHostToDevice, stream));
     }
 
+    // One BCD round (round index from the device counter, advanced at its end).
     template <typename TM>
-    void launch_round(int k, int init, int nrow, int ncol, int ncomb, double tol, int max_iter)
+    void launch_round(int init, int nrow, int ncol, int ncomb, double tol, int max_iter)
     {
         const TM* Mp = reinterpret_cast<const TM*>(M.p);
         const Vecs v = vecs();
         const int ntf = nt ? 1 : 0;
+        const int* kp = kdev.p;
         if (!init && nrow > 0) {
-            if (fine)
+            const int R = fine ? 1 : coarse_R;
+            if (R == 1)
                 hipLaunchKernelGGL((k_batch_row<TM, 1>), dim3(nrow), dim3(256), 0, stream, Mp,
-                                   d_desc.p, t_row.p, v, k, ntf);
+                                   d_desc.p, t_row.p, v, kp, ntf);
+            else if (R == 2)
+                hipLaunchKernelGGL((k_batch_row<TM, 2>), dim3(nrow), dim3(256), 0, stream, Mp,
+                                   d_desc.p, t_row.p, v, kp, ntf);
             else
                 hipLaunchKernelGGL((k_batch_row<TM, kRows<TM>>), dim3(nrow), dim3(256), 0, stream,
-                                   Mp, d_desc.p, t_row.p, v, k, ntf);
+                                   Mp, d_desc.p, t_row.p, v, kp, ntf);
             KERNEL_CHECK();
         }
         if (ncol > 0) {
             hipLaunchKernelGGL(k_batch_colpart<TM>, dim3(ncol), dim3(256), 0, stream, Mp,
-                               d_desc.p, t_col.p, v, k, init, 1);
+                               d_desc.p, t_col.p, v, kp, init, 1);
             KERNEL_CHECK();
         }
         if (ncomb > 0) {
             hipLaunchKernelGGL(k_batch_combine, dim3(ncomb), dim3(256), 0, stream, d_desc.p,
-                               t_comb.p, v, k, init, tol, max_iter);
+                               t_comb.p, v, kp, init, tol, max_iter);
             KERNEL_CHECK();
         }
+        hipLaunchKernelGGL(k_round_next, dim3(1), dim3(64), 0, stream, kdev.p);
+        KERNEL_CHECK();
     }
 
-    void round(int k, int init, int nrow, int ncol, int ncomb, double tol, int max_iter)
+    void round(int init, int nrow, int ncol, int ncomb, double tol, int max_iter)
     {
         if (bulk == 0)
-            launch_round<float>(k, init, nrow, ncol, ncomb, tol, max_iter);
+            launch_round<float>(init, nrow, ncol, ncomb, tol, max_iter);
         else
-            launch_round<double>(k, init, nrow, ncol, ncomb, tol, max_iter);
+            launch_round<double>(init, nrow, ncol, ncomb, tol, max_iter);
+    }
+
+    // Reference initial point of every problem: alpha_0 = 0, flags cleared,
+    // round counter at -1 (the init round computes beta_0).
+    void reset()
+    {
+        const long nmax = std::max<long>(std::max(total_n, total_m), nprob);
+        hipLaunchKernelGGL(k_batch_init, dim3(static_cast<int>(ceil_div(nmax, 256L))), dim3(256),
+                           0, stream, ap0.p, a.p, loga.p, total_n, b.p, logb.p, total_m, done.p,
+                           niter.p, nprob, kdev.p);
+        KERNEL_CHECK();
+    }
+
+    // Autotuner of the coarse tiles (rows per wave of the row tiles, rows per
+    // column-pass block): every candidate runs a whole batched solve of this
+    // handle's problems to the requested tolerance (host wall time, the
+    // solve's own graph / launch mode); the fastest is kept for the handle.
+    // Each solve starts from the reference's initial point, so results depend
+    // only on the tiles finally chosen.  A batch too small for coarse tiles
+    // (it starts in fine tiles) keeps the defaults.
+    void tune(double tol, int max_iter)
+    {
+        const int R32[2] = {2, 4}, R64[2] = {1, 2};
+        const int rows[3] = {256, 512, 1024};
+        double best = 1e300;
+        int bR = coarse_R, bB = coarse_rows, c = 0;
+        for (int ri = 0; ri < 2; ri++) {
+            for (int bi = 0; bi < 3; bi++, c++) {
+                coarse_R = (bulk == 0 ? R32 : R64)[ri];
+                coarse_rows = rows[bi];
+                const auto t0 = std::chrono::steady_clock::now();
+                run(tol, max_iter);
+                const double ms = std::chrono::duration<double, std::milli>(
+                                      std::chrono::steady_clock::now() - t0).count();
+                tune_ms[c] = ms;
+                if (ms < best) {
+                    best = ms;
+                    bR = coarse_R;
+                    bB = coarse_rows;
+                }
+                if (started_fine) break;
+            }
+            if (started_fine) break;
+        }
+        coarse_R = bR;
+        coarse_rows = bB;
+        tuned = true;
+    }
+
+    // One batched solve from the reference's initial point to the stop of
+    // every problem (or max_iter rounds); returns the rounds that did work.
+    long long run(double tol, int max_iter)
+    {
+        hipStream_t s = stream;
+        const int np = nprob;
+        reset();
+
+        std::vector<int> act(np);
+        for (int p = 0; p < np; p++) act[p] = p;
+        int slot = 0, nrow = 0, ncol = 0, ncomb = 0;
+        build_tables(act, slot, &nrow, &ncol, &ncomb, true);
+        started_fine = fine;
+        // beta_0 = optimal_beta(alpha_0 = 0) and its residual (parity 0)
+        round(1, nrow, ncol, ncomb, tol, max_iter);
+
+        // K rounds per batch, two batches in flight.  Every problem is stopped
+        // by round max_iter, so rounds past it (the tail of the last batch)
+        // return at once.
+        const int K = 8;
+        int k = 0;
+        long long nround = 0;
+        auto launch_batch = [&](int e) {
+            rounds(K, nrow, ncol, ncomb, tol, max_iter);
+            const int work = std::min(K, max_iter + 1 - k);
+            nround += work;
+            k += work;
+            HIP_CHECK(hipMemcpyAsync(h_done.p + static_cast<long>(e) * np, done.p,
+                                     sizeof(int) * np, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipEventRecord(ev[e], s));
+        };
+        std::deque<int> q;
+        try {
+            launch_batch(0);
+            q.push_back(0);
+            if (k <= max_iter) {
+                launch_batch(1);
+                q.push_back(1);
+            }
+            while (!q.empty()) {
+                const int e = q.front();
+                q.pop_front();
+                HIP_CHECK(hipEventSynchronize(ev[e]));
+                const int* dn = h_done.p + static_cast<long>(e) * np;
+                std::vector<int> still;
+                still.reserve(act.size());
+                for (int p : act)
+                    if (!dn[p]) still.push_back(p);
+                if (still.empty() || k > max_iter) {
+                    // every problem stopped, or every round up to max_iter is queued
+                    if (still.empty()) break;
+                    continue;
+                }
+                if (still.size() != act.size()) {
+                    act.swap(still);
+                    slot ^= 1;
+                    build_tables(act, slot, &nrow, &ncol, &ncomb);
+                }
+                launch_batch(e);
+                q.push_back(e);
+            }
+            HIP_CHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            throw;
+        }
+        return nround;
+    }
+
+    // K rounds as one graph launch.  The graph is captured from the stream
+    // once per configuration (grid sizes, tile mode, load mode; tables and
+    // descriptors are device buffers refreshed outside it) and replayed;
+    // the round index lives on the device, so replays need no parameter
+    // updates.  The rounds contain kernels only (no memset nodes).
+    void rounds(int K, int nrow, int ncol, int ncomb, double tol, int max_iter)
+    {
+        if (!use_graph) {
+            for (int q = 0; q < K; q++) round(0, nrow, ncol, ncomb, tol, max_iter);
+            return;
+        }
+        const int key[9] = {K,         nrow,     ncol,     ncomb,      fine ? 1 : 0,
+                            nt ? 1 : 0, max_iter, coarse_R, coarse_rows};
+        if (!gexec || !std::equal(key, key + 9, g_key) || tol != g_tol) {
+            if (gexec) {
+                HIP_CHECK(hipGraphExecDestroy(gexec));
+                gexec = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                for (int q = 0; q < K; q++) round(0, nrow, ncol, ncomb, tol, max_iter);
+            } catch (...) {
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIP_CHECK(hipStreamEndCapture(stream, &g));
+            const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIP_CHECK(e);
+            std::copy(key, key + 9, g_key);
+            g_tol = tol;
+            graph_builds++;
+        }
+        HIP_CHECK(hipGraphLaunch(gexec, stream));
+        graph_launches++;
     }
 };
 
@@ -456,6 +650,7 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
     B->device = device;
     B->nprob = nprob;
     B->bulk = bulk;
+    B->coarse_R = bulk == REGOT_F32 ? 4 : 2;
     HIP_CHECK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
     const int esz = bulk == REGOT_F32 ? 4 : 8;
     const int R = bulk == REGOT_F32 ? 4 : 2;
@@ -561,6 +756,7 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
     B->gval.alloc(nprob);
     B->done.alloc(nprob);
     B->niter.alloc(nprob);
+    B->kdev.alloc(1);
     HIP_CHECK(hipMemcpyAsync(B->a.p, ha.data(), sizeof(double) * aoff, hipMemcpyHostToDevice,
                              B->stream));
     HIP_CHECK(hipMemcpyAsync(B->b.p, hb.data(), sizeof(double) * boff, hipMemcpyHostToDevice,
@@ -580,6 +776,41 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
     for (auto& e : B->ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(hipStreamSynchronize(B->stream));
     *out = B.release();
+    CAPI_END
+}
+
+int regot_batch_set_option(regot_batch* B, int option, long long value)
+{
+    CAPI_BEGIN
+    if (!B) fail(E_INVALID, "null batch");
+    switch (option) {
+        case REGOT_BATCH_GRAPH:
+            B->use_graph = value != 0;
+            break;
+        case REGOT_BATCH_AUTOTUNE:
+            B->autotune = value != 0;
+            if (!B->autotune) {
+                B->tuned = false;
+                B->coarse_R = B->bulk == REGOT_F32 ? 4 : 2;
+                B->coarse_rows = kBlockRows;
+            }
+            break;
+        default:
+            fail(E_INVALID, "unknown batch option %d", option);
+    }
+    CAPI_END
+}
+
+int regot_batch_stats(const regot_batch* B, double* out)
+{
+    CAPI_BEGIN
+    if (!B || !out) fail(E_INVALID, "null argument");
+    out[0] = B->coarse_R;
+    out[1] = B->coarse_rows;
+    out[2] = B->tuned ? 1.0 : 0.0;
+    out[3] = static_cast<double>(B->graph_builds);
+    out[4] = static_cast<double>(B->graph_launches);
+    for (int c = 0; c < 6; c++) out[5 + c] = B->tune_ms[c];
     CAPI_END
 }
 
@@ -610,66 +841,8 @@ int regot_batch_solve_bcd(regot_batch* B, double tol, int max_iter, int* niter, 
     HIP_CHECK(hipSetDevice(B->device));
     hipStream_t s = B->stream;
     const int np = B->nprob;
-    const long nmax = std::max<long>(std::max(B->total_n, B->total_m), np);
-    hipLaunchKernelGGL(k_batch_init, dim3(static_cast<int>(ceil_div(nmax, 256L))), dim3(256), 0,
-                       s, B->ap0.p, B->a.p, B->loga.p, B->total_n, B->b.p, B->logb.p,
-                       B->total_m, B->done.p, B->niter.p, np);
-    KERNEL_CHECK();
-
-    std::vector<int> act(np);
-    for (int p = 0; p < np; p++) act[p] = p;
-    int slot = 0, nrow = 0, ncol = 0, ncomb = 0;
-    B->build_tables(act, slot, &nrow, &ncol, &ncomb);
-    // beta_0 = optimal_beta(alpha_0 = 0) and its residual (parity 0)
-    B->round(-1, 1, nrow, ncol, ncomb, tol, max_iter);
-
-    const int K = 8;   // rounds per batch; two batches in flight
-    int k = 0;
-    long long nround = 0;
-    auto launch_batch = [&](int e) {
-        for (int q = 0; q < K && k <= max_iter; q++, k++) {
-            B->round(k, 0, nrow, ncol, ncomb, tol, max_iter);
-            nround++;
-        }
-        HIP_CHECK(hipMemcpyAsync(B->h_done.p + static_cast<long>(e) * np, B->done.p,
-                                 sizeof(int) * np, hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipEventRecord(B->ev[e], s));
-    };
-    std::deque<int> q;
-    try {
-        launch_batch(0);
-        q.push_back(0);
-        if (k <= max_iter) {
-            launch_batch(1);
-            q.push_back(1);
-        }
-        while (!q.empty()) {
-            const int e = q.front();
-            q.pop_front();
-            HIP_CHECK(hipEventSynchronize(B->ev[e]));
-            const int* dn = B->h_done.p + static_cast<long>(e) * np;
-            std::vector<int> still;
-            still.reserve(act.size());
-            for (int p : act)
-                if (!dn[p]) still.push_back(p);
-            if (still.empty() || k > max_iter) {
-                // every problem stopped, or every round up to max_iter is queued
-                if (still.empty()) break;
-                continue;
-            }
-            if (still.size() != act.size()) {
-                act.swap(still);
-                slot ^= 1;
-                B->build_tables(act, slot, &nrow, &ncol, &ncomb);
-            }
-            launch_batch(e);
-            q.push_back(e);
-        }
-        HIP_CHECK(hipStreamSynchronize(s));
-    } catch (...) {
-        (void)hipStreamSynchronize(s);
-        throw;
-    }
+    if (B->autotune && !B->tuned) B->tune(tol, max_iter);
+    const long long nround = B->run(tol, max_iter);
 
     if (niter) HIP_CHECK(hipMemcpyAsync(niter, B->niter.p, sizeof(int) * np, hipMemcpyDeviceToHost, s));
     if (obj) HIP_CHECK(hipMemcpyAsync(obj, B->fval.p, sizeof(double) * np, hipMemcpyDeviceToHost, s));

@@ -40,6 +40,10 @@ struct Work {
     DBuf<double> dscal;
     HBuf<double> hscal;
     long long cg_iters = 0;
+    // algorithmic HBM bytes of the work done: n m s per objective or gradient
+    // evaluation (one logical read of M), n m s + 2 nnz 12 per Hessian build
+    // (CSR + CSC written), 2 nnz 12 per Hessian product (s = 4 or 8 bytes)
+    double alg_bytes = 0.0;
     // per-phase wall time (each phase ends in a host synchronisation)
     double phase_ms[3] = {0.0, 0.0, 0.0};
     long long phase_n[3] = {0, 0, 0};
@@ -98,6 +102,7 @@ struct Work {
     {
         const auto t0 = Clock::now();
         const double f = p->dev_obj(at);
+        alg_bytes += pass_bytes();
         phase_ms[0] += ms_since(t0);
         phase_n[0] += 1;
         return f;
@@ -108,6 +113,7 @@ struct Work {
         const auto t0 = Clock::now();
         double gn2 = 0.0;
         const double f = p->dev_obj_grad(at, grad, &gn2);
+        alg_bytes += pass_bytes();
         if (gnorm) *gnorm = std::sqrt(gn2);
         phase_ms[0] += ms_since(t0);
         phase_n[0] += 1;
@@ -118,6 +124,7 @@ struct Work {
     {
         const auto t0 = Clock::now();
         hess_threshold(p, gamma, g, delta, hint);
+        alg_bytes += pass_bytes() + 2.0 * hess_of(p).nnz * 12.0;
         phase_ms[1] += ms_since(t0);
         phase_n[1] += 1;
     }
@@ -126,13 +133,28 @@ struct Work {
     {
         const auto t0 = Clock::now();
         hess_topk(p, gamma, g, density);
+        alg_bytes += pass_bytes() + 2.0 * hess_of(p).nnz * 12.0;
         phase_ms[1] += ms_since(t0);
         phase_n[1] += 1;
+    }
+
+    double pass_bytes() const
+    {
+        return static_cast<double>(n) * m * (p->bulk == 0 ? 4.0 : 8.0);
+    }
+
+    // CG iterations of one solve: one product with Delta each, plus the two
+    // sigma products of the block elimination
+    void count_cg(int iters)
+    {
+        cg_iters += iters;
+        alg_bytes += (iters + 1) * 2.0 * hess_of(p).nnz * 12.0;
     }
 
     void report(regot_result* res) const
     {
         res->cg_iters = cg_iters;
+        res->alg_bytes = alg_bytes;
         for (int q = 0; q < 3; q++) {
             res->phase_ms[q] = phase_ms[q];
             res->phase_n[q] = phase_n[q];
@@ -151,7 +173,7 @@ struct Work {
         const int maxit = o.cg_max_iter > 0 ? o.cg_max_iter
                                             : (o.method == 0 ? 2 * m : 10 * m + 1000);
         const auto t0 = Clock::now();
-        cg_iters += hess_cg(p, r, shift, tol, maxit, out, nullptr, nullptr, o.method != 0);
+        count_cg(hess_cg(p, r, shift, tol, maxit, out, nullptr, nullptr, o.method != 0));
         phase_ms[2] += ms_since(t0);
         phase_n[2] += 1;
     }

@@ -17,6 +17,7 @@ struct regot_result {
     double phase_ms[3] = {0.0, 0.0, 0.0};
     long long phase_n[3] = {0, 0, 0};
     long long cg_iters = 0;
+    double alg_bytes = 0.0;   // algorithmic HBM bytes of the solve (Newton-type drivers)
     // APDAGD's primal plan (device, row pitch plan_ld); empty for the others,
     // whose plan is formed from the dual on request
     regot::DBuf<double> plan;

@@ -268,6 +268,8 @@ void solve_apdagd(regot_problem* p, double tol, int max_iter, int verbose, const
         launch_split_gamma(x, n, m, alpha, beta, w.s);
         launch_apd_plan(p->bulk, p->dM, p->ldm, n, m, alpha, beta, p->a, p->b, p->reg, tau, init,
                         res->plan, ldp, rowpart, colpart, scal, part, out, w.s);
+        // read M (and P unless init), write P
+        w.alg_bytes += static_cast<double>(n) * m * ((p->bulk == 0 ? 4.0 : 8.0) + (init ? 8 : 16));
         HIP_CHECK(hipMemcpyAsync(hout.p, out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, w.s));
         HIP_CHECK(hipStreamSynchronize(w.s));
     };

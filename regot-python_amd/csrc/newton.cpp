@@ -416,7 +416,7 @@ void solve_newton_dense(regot_problem* p, double tol, int max_iter, int verbose,
         vec_axpby(w.rhs, -1.0, w.g, 0.0, nullptr, w.N, w.s);
         {
             const auto t0 = Clock::now();
-            w.cg_iters += hess_cg(p, w.rhs, 0.0, cg_tol, cg_max, w.direc, nullptr, nullptr, true);
+            w.count_cg(hess_cg(p, w.rhs, 0.0, cg_tol, cg_max, w.direc, nullptr, nullptr, true));
             w.phase_ms[2] += ms_since(t0);
             w.phase_n[2] += 1;
         }

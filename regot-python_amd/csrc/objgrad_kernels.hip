@@ -1,0 +1,261 @@
+// objgrad_kernels.hip -- objective and gradient of the dual in one pass over M.
+//
+// f(gamma) = reg sum_ij T_ij - alpha'a - betat'bt and
+// g = [T 1 - a ; (T' 1 - b)_{0..m-2}],  T_ij = 2^((alpha_i + beta_j - M_ij) k),
+// k = log2(e) / reg, in FP64 throughout (src/sinkhorn_problem_obj.cpp:139-297,
+// 324-393 form the same T with FP64 exp and sum it in FP64; no shift is
+// applied there either, so overflow and underflow behave alike).  2^x is the
+// table-driven FP64 exp2 of dexp2.hpp (about 0.5 ulp), several times cheaper
+// than the library routine, which bounded this pass by FP64 issue rate.
+//
+// The row and column sums come out of the same read of M, so each T_ij is
+// formed once (the FP64 exponential is the expensive part of this pass).
+// Tiling as the APDAGD plan pass: a workgroup owns a 256-column strip of a
+// block of rows; each lane owns 4 adjacent columns and the 4 waves take the
+// block's rows round robin (coalesced row segments).  Row sums are reduced
+// across the wave per row and written per strip, column sums accumulate per
+// lane down the block and are merged through LDS per block; two small
+// kernels fold strips, blocks and the scalars in a fixed order.
+#include <algorithm>
+
+#include "dexp2.hpp"
+#include "problem.hpp"
+
+namespace regot {
+namespace {
+
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ void ldm4(const float* p, int c0, int m, double (&v)[4])
+{
+    if (c0 + 3 < m) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const f4 u = __builtin_nontemporal_load(reinterpret_cast<const f4*>(p + c0));
+        v[0] = u.x;
+        v[1] = u.y;
+        v[2] = u.z;
+        v[3] = u.w;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = (c0 + q < m) ? static_cast<double>(p[c0 + q]) : 0.0;
+    }
+}
+
+__device__ __forceinline__ void ldm4(const double* p, int c0, int m, double (&v)[4])
+{
+    if (c0 + 3 < m) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const d2 u0 = __builtin_nontemporal_load(reinterpret_cast<const d2*>(p + c0));
+        const d2 u1 = __builtin_nontemporal_load(reinterpret_cast<const d2*>(p + c0 + 2));
+        v[0] = u0.x;
+        v[1] = u0.y;
+        v[2] = u1.x;
+        v[3] = u1.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = (c0 + q < m) ? p[c0 + q] : 0.0;
+    }
+}
+
+// grid (strips = ceil(m / 256), row blocks = ceil(n / rows)), 256 threads.
+// beta has m entries (beta_{m-1} = 0).  rowpart[strip * n + i] = sum of row i
+// over the strip; colpart[block * m + j] (when non-null) = sum of column j
+// over the block.
+template <typename TM>
+__global__ __launch_bounds__(256) void k_sums_tile(const TM* __restrict__ M, long ldm, int n, int m,
+                                                   const double* __restrict__ alpha,
+                                                   const double* __restrict__ beta, double k,
+                                                   int rows, double* __restrict__ rowpart,
+                                                   double* __restrict__ colpart)
+{
+    __shared__ double sm[4][256];
+    __shared__ double tab[128];
+    dev::exp2_tab_load(tab);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int strip = blockIdx.x, blk = blockIdx.y;
+    const int c0 = strip * 256 + lane * 4;
+    const int r0 = blk * rows;
+    const int r1 = min(n, r0 + rows);
+    double bk[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) bk[q] = (c0 + q < m) ? beta[c0 + q] : 0.0;
+    double cs[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int r = r0 + wave; r < r1; r += 4) {
+        const double ai = alpha[r];
+        double mv[4];
+        ldm4(M + static_cast<long>(r) * ldm, c0, m, mv);
+        double rs = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (c0 + q < m) {
+                const double t = dev::dexp2((ai + bk[q] - mv[q]) * k, tab);
+                rs += t;
+                cs[q] += t;
+            }
+        }
+        rs = wsum(rs);
+        if (lane == 0) rowpart[static_cast<long>(strip) * n + r] = rs;
+    }
+    if (!colpart) return;   // grid-uniform
+#pragma unroll
+    for (int q = 0; q < 4; q++) sm[wave][lane * 4 + q] = cs[q];
+    __syncthreads();
+    const int j = strip * 256 + threadIdx.x;
+    if (j < m)
+        colpart[static_cast<long>(blk) * m + j] =
+            ((sm[0][threadIdx.x] + sm[1][threadIdx.x]) + sm[2][threadIdx.x]) + sm[3][threadIdx.x];
+}
+
+__device__ __forceinline__ double bsum(double v, double* sm4)
+{
+    v = wsum(v);
+    if ((threadIdx.x & 63) == 0) sm4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double r = ((sm4[0] + sm4[1]) + sm4[2]) + sm4[3];
+    __syncthreads();
+    return r;
+}
+
+// Thread t < n: r_t = sum over strips, grad_t = r_t - a_t; thread t < m - 1:
+// c_t = sum over blocks (when colpart is set), grad_{n+t} = c_t - b_t.
+// part[4 wg + {0..3}] = this workgroup's sum r, sum alpha a, sum betat bt,
+// sum g^2.
+__global__ __launch_bounds__(256) void k_sums_fold(const double* __restrict__ rowpart, int nstrip,
+                                                   const double* __restrict__ colpart, int nblk,
+                                                   const double* __restrict__ alpha,
+                                                   const double* __restrict__ beta,
+                                                   const double* __restrict__ a,
+                                                   const double* __restrict__ b, int n, int m,
+                                                   double* __restrict__ grad,
+                                                   double* __restrict__ part)
+{
+    __shared__ double sm4[4];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    double sr = 0.0, sa = 0.0, sb = 0.0, g2 = 0.0;
+    if (t < n) {
+        double r = 0.0;
+        for (int s = 0; s < nstrip; s++) r += rowpart[static_cast<long>(s) * n + t];
+        const double g = r - a[t];
+        if (grad) grad[t] = g;
+        sr = r;
+        sa = alpha[t] * a[t];
+        g2 = g * g;
+    }
+    if (t < m - 1) {
+        sb = beta[t] * b[t];
+        if (colpart) {
+            double c = 0.0;
+            for (int q = 0; q < nblk; q++) c += colpart[static_cast<long>(q) * m + t];
+            const double g = c - b[t];
+            if (grad) grad[n + t] = g;
+            g2 += g * g;
+        }
+    }
+    sr = bsum(sr, sm4);
+    sa = bsum(sa, sm4);
+    sb = bsum(sb, sm4);
+    g2 = bsum(g2, sm4);
+    if (threadIdx.x == 0) {
+        part[4 * blockIdx.x] = sr;
+        part[4 * blockIdx.x + 1] = sa;
+        part[4 * blockIdx.x + 2] = sb;
+        part[4 * blockIdx.x + 3] = g2;
+    }
+}
+
+// out[0] = reg sum r - alpha'a - betat'bt, out[1] = ||g||^2.
+__global__ __launch_bounds__(256) void k_sums_final(const double* __restrict__ part, int npart,
+                                                    double reg, double* __restrict__ out)
+{
+    __shared__ double sm4[4];
+    double sr = 0.0, sa = 0.0, sb = 0.0, g2 = 0.0;
+    for (int i = threadIdx.x; i < npart; i += 256) {
+        sr += part[4 * i];
+        sa += part[4 * i + 1];
+        sb += part[4 * i + 2];
+        g2 += part[4 * i + 3];
+    }
+    sr = bsum(sr, sm4);
+    sa = bsum(sa, sm4);
+    sb = bsum(sb, sm4);
+    g2 = bsum(g2, sm4);
+    if (threadIdx.x == 0) {
+        out[0] = reg * sr - sa - sb;
+        out[1] = g2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_exp2_check(const double* __restrict__ x,
+                                                    double* __restrict__ y, long count)
+{
+    __shared__ double tab[128];
+    dev::exp2_tab_load(tab);
+    __syncthreads();
+    for (long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; i < count;
+         i += static_cast<long>(gridDim.x) * 256)
+        y[i] = dev::dexp2(x[i], tab);
+}
+
+}  // namespace
+
+void launch_exp2_check(const double* x, double* y, long count, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_exp2_check, dim3(std::min(4096L, (count + 255) / 256)), dim3(256), 0, s,
+                       x, y, count);
+    KERNEL_CHECK();
+}
+
+// Rows per workgroup of the one-pass kernels: >= 2048 workgroups where the
+// problem allows, at most 256 rows so the column partials stay <= n/16 rows.
+int sums_rows(int n, int m)
+{
+    const long nstrip = ceil_div(m, 256);
+    long r = static_cast<long>(n) * nstrip / 2048;
+    r = std::max(16L, std::min(256L, r));
+    return static_cast<int>((r + 3) / 4 * 4);
+}
+
+void sums_scratch_sizes(int n, int m, long* rowpart, long* colpart, long* part)
+{
+    const long nstrip = ceil_div(m, 256), nblk = ceil_div(n, sums_rows(n, m));
+    *rowpart = nstrip * n;
+    *colpart = nblk * m;
+    *part = 4L * ceil_div(std::max(n, m), 256);
+}
+
+void launch_objgrad_fused(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                          const double* beta, const double* a, const double* b, double reg,
+                          bool with_grad, double* grad, double* rowpart, double* colpart,
+                          double* part, double* out, hipStream_t s)
+{
+    if (n <= 0 || m <= 0) fail(E_INVALID, "objective pass on an empty problem");
+    const int nstrip = ceil_div(m, 256);
+    const int rows = sums_rows(n, m);
+    const int nblk = ceil_div(n, rows);
+    const dim3 grid(nstrip, nblk);
+    const double k = 1.4426950408889634074 / reg;
+    double* cp = with_grad ? colpart : nullptr;
+    if (bulk == 0)
+        hipLaunchKernelGGL(k_sums_tile<float>, grid, dim3(256), 0, s, static_cast<const float*>(M),
+                           ldm, n, m, alpha, beta, k, rows, rowpart, cp);
+    else
+        hipLaunchKernelGGL(k_sums_tile<double>, grid, dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, n, m, alpha, beta, k, rows,
+                           rowpart, cp);
+    KERNEL_CHECK();
+    const int nf = ceil_div(std::max(n, m), 256);
+    hipLaunchKernelGGL(k_sums_fold, dim3(nf), dim3(256), 0, s, rowpart, nstrip, cp, nblk, alpha,
+                       beta, a, b, n, m, with_grad ? grad : nullptr, part);
+    KERNEL_CHECK();
+    hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, s, part, nf, reg, out);
+    KERNEL_CHECK();
+}
+
+}  // namespace regot

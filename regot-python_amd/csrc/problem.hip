@@ -253,8 +253,28 @@ void regot_problem::row_pass_bcd(const double* beta_full, const RowEpi& epi, con
     prof_end(0, stop);
 }
 
+// One-pass FP64 objective (and gradient): the precise mode of FP32 bulk and
+// every FP64-bulk evaluation.  Returns f; *gnorm2 = ||g||^2 when with_grad.
+double regot_problem::dev_obj_fused(const double* d_gamma, double* d_grad, double* gnorm2)
+{
+    long nr, nc, np;
+    sums_scratch_sizes(n, m, &nr, &nc, &np);
+    s_row.ensure(nr);
+    s_col.ensure(nc);
+    s_part.ensure(np);
+    launch_split_gamma(d_gamma, n, m, alpha, beta, stream);
+    const bool with_grad = d_grad != nullptr;
+    launch_objgrad_fused(bulk, dM, ldm, n, m, alpha, beta, a, b, reg, with_grad, d_grad, s_row,
+                         s_col, s_part, scal, stream);
+    HIP_CHECK(hipMemcpyAsync(hscal.p, scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    if (gnorm2) *gnorm2 = hscal.p[1];
+    return hscal.p[0];
+}
+
 double regot_problem::dev_obj(const double* d_gamma)
 {
+    if (precise || bulk == 1) return dev_obj_fused(d_gamma, nullptr, nullptr);
     launch_split_gamma(d_gamma, n, m, alpha, beta, stream);
     RowEpi e;
     e.loga = loga;
@@ -269,6 +289,7 @@ double regot_problem::dev_obj(const double* d_gamma)
 
 double regot_problem::dev_obj_grad(const double* d_gamma, double* d_grad, double* gnorm2)
 {
+    if (precise || bulk == 1) return dev_obj_fused(d_gamma, d_grad, gnorm2);
     launch_split_gamma(d_gamma, n, m, alpha, beta, stream);
     RowEpi e;
     e.loga = loga;
@@ -323,6 +344,19 @@ int regot_device_count(void)
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+int regot_check_exp2(const double* x, double* y, long long count, int device)
+{
+    CAPI_BEGIN
+    if (count < 0 || (count > 0 && (!x || !y))) fail(E_INVALID, "bad arguments");
+    if (count == 0) return OK;
+    HIP_CHECK(hipSetDevice(device));
+    DBuf<double> dx(count), dy(count);
+    HIP_CHECK(hipMemcpy(dx.p, x, count * sizeof(double), hipMemcpyHostToDevice));
+    launch_exp2_check(dx, dy, count, nullptr);
+    HIP_CHECK(hipMemcpy(y, dy.p, count * sizeof(double), hipMemcpyDeviceToHost));
+    CAPI_END
 }
 
 int regot_problem_create(regot_problem** out, int n, int m, const void* M, int M_dtype,

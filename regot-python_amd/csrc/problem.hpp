@@ -51,6 +51,19 @@ void launch_apd_plan(int bulk, const void* M, long ldm, int n, int m, const doub
                      double tau, bool init, double* P, long ldp, double* rowpart,
                      double* colpart, double* scal, double* part, double* out, hipStream_t s);
 
+// One-pass FP64 objective / gradient (objgrad_kernels.hip): T formed once per
+// element, row sums per 256-column strip and column sums per row block in
+// scratch (sizes from sums_scratch_sizes); out[0] = f, out[1] = ||g||^2;
+// grad written when with_grad (else only f).
+void sums_scratch_sizes(int n, int m, long* rowpart, long* colpart, long* part);
+void launch_objgrad_fused(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                          const double* beta, const double* a, const double* b, double reg,
+                          bool with_grad, double* grad, double* rowpart, double* colpart,
+                          double* part, double* out, hipStream_t s);
+
+// y = 2^x by the device FP64 exponential of the one-pass kernels (dexp2.hpp).
+void launch_exp2_check(const double* x, double* y, long count, hipStream_t s);
+
 // Buffers of the on-device BCD loop, kept across solves on one handle.
 struct BcdWork {
     DBuf<double> ap0, ap1, beta, gcol, rpart, cpart, bm;
@@ -130,8 +143,11 @@ struct regot_problem {
     void row_pass_bcd(const double* beta_full, const regot::RowEpi& epi,
                       const regot::BcdState& st);
 
-    // f(gamma) for device gamma (n+m-1); row pass only.
+    // f(gamma) for device gamma (n+m-1); row pass only (FP32 bulk), or the
+    // one-pass FP64 kernel (precise mode, FP64 bulk).
     double dev_obj(const double* d_gamma);
+    double dev_obj_fused(const double* d_gamma, double* d_grad, double* gnorm2);
+    regot::DBuf<double> s_row, s_col, s_part;   // scratch of the one-pass kernel
     // f and grad (device, n+m-1) for device gamma; row and column passes.
     // Row sums land in rsum, column sums in csum.  Returns f; *gnorm2 = ||g||^2.
     double dev_obj_grad(const double* d_gamma, double* d_grad, double* gnorm2);

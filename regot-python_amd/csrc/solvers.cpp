@@ -346,7 +346,7 @@ int regot_result_stats(const regot_result* r, double* out)
         out[2 * q + 1] = static_cast<double>(r->phase_n[q]);
     }
     out[6] = static_cast<double>(r->cg_iters);
-    out[7] = 0.0;
+    out[7] = r->alg_bytes;
     return OK;
 }
 

@@ -342,7 +342,8 @@ class Problem:
             check(L.regot_result_stats(r, dptr(st)))
             res.stats = {"objgrad_ms": st[0], "objgrad_calls": int(st[1]),
                          "sparsify_ms": st[2], "sparsify_calls": int(st[3]),
-                         "cg_ms": st[4], "cg_solves": int(st[5]), "cg_iters": int(st[6])}
+                         "cg_ms": st[4], "cg_solves": int(st[5]), "cg_iters": int(st[6]),
+                         "alg_bytes": float(st[7])}
             if compute_plan:
                 plan = np.empty((self.n, self.m))
                 check(L.regot_result_plan(self._h, r, dptr(plan)))
@@ -425,10 +426,17 @@ class BatchProblem:
         check(lib().regot_batch_info(h, None, _lib.ctypes.byref(tot), _lib.ctypes.byref(dl)))
         self.total_elems, self.dual_len = tot.value, dl.value
 
-    def solve_bcd(self, tol=1e-6, max_iter=1000, dual=True):
+    def solve_bcd(self, tol=1e-6, max_iter=1000, dual=True, graph=False, autotune=False):
         """BCD on every problem.  Returns a dict of per-problem arrays:
         ``niter``, ``obj`` and ``mar_err`` (at the final iterate), ``dual``
-        (a list of (n+m-1,) arrays, or None), and ``rounds`` launched."""
+        (a list of (n+m-1,) arrays, or None), ``rounds`` launched, the
+        hipGraph instantiations / launches so far (``graph=True`` replays
+        captured groups of 8 rounds instead of launching the rounds' kernels
+        one by one; results are the same) and the coarse
+        tile configuration (``autotune=True`` times the candidates once per
+        handle and keeps the fastest)."""
+        check(lib().regot_batch_set_option(self._h, 0, 1 if graph else 0))
+        check(lib().regot_batch_set_option(self._h, 1, 1 if autotune else 0))
         P = self.nprob
         it = np.zeros(P, dtype=np.int32)
         ob = np.zeros(P)
@@ -438,7 +446,12 @@ class BatchProblem:
         check(lib().regot_batch_solve_bcd(self._h, float(tol), int(max_iter), _lib.iptr(it),
                                           dptr(ob), dptr(me), dptr(du) if dual else None,
                                           _lib.ctypes.byref(rounds)))
-        out = {"niter": it, "obj": ob, "mar_err": me, "rounds": rounds.value, "dual": None}
+        st = np.zeros(11)
+        check(lib().regot_batch_stats(self._h, dptr(st)))
+        out = {"niter": it, "obj": ob, "mar_err": me, "rounds": rounds.value, "dual": None,
+               "graph_builds": int(st[3]), "graph_launches": int(st[4]),
+               "rows_per_wave": int(st[0]), "block_rows": int(st[1]), "tuned": bool(st[2]),
+               "tune_ms": [float(x) for x in st[5:11]]}
         if dual:
             sizes = self.n.astype(np.int64) + self.m - 1
             out["dual"] = np.split(du, np.cumsum(sizes)[:-1])

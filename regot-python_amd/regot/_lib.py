@@ -79,6 +79,7 @@ def lib():
         ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
         ctypes.c_int, ctypes.c_longlong, _dp, _dp, ctypes.c_double, ctypes.c_int]
     L.regot_problem_destroy.argtypes = [_vp]
+    L.regot_check_exp2.argtypes = [_dp, _dp, ctypes.c_longlong, ctypes.c_int]
     L.regot_problem_dims.argtypes = [_vp, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     for name in ("regot_optimal_alpha", "regot_optimal_beta"):
@@ -114,6 +115,8 @@ def lib():
                                      ctypes.c_int]
     L.regot_batch_destroy.argtypes = [_vp]
     L.regot_batch_info.argtypes = [_vp, _ip, _llp, _llp]
+    L.regot_batch_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_longlong]
+    L.regot_batch_stats.argtypes = [_vp, _dp]
     L.regot_batch_solve_bcd.argtypes = [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp,
                                         _llp]
     L.regot_sqeuclid_bf16.argtypes = [_vp, ctypes.c_longlong, _vp, ctypes.c_longlong,

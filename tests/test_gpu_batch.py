@@ -111,3 +111,48 @@ def test_batch_device_inputs(gpu_lib):
     b = gpu_lib.sinkhorn_bcd_batch(Ms, As, Bs, regs, tol=1e-7, max_iter=3000)
     np.testing.assert_array_equal(a["niter"], b["niter"])
     np.testing.assert_array_equal(np.concatenate(a["dual"]), np.concatenate(b["dual"]))
+
+
+def test_batch_graph_replay_matches_eager_launches(gpu_lib):
+    # The rounds as hipGraph replays (default) and as one-by-one launches
+    # give bitwise the same results: same kernels, same order.  The solve
+    # replays each captured graph many times, rebuilds it when the tables
+    # shrink, and a second solve replays again after eager work in between.
+    Ms, As, Bs, regs = mixed_batch(5, 9, lo=40, hi=260)
+    B = gpu_lib.BatchProblem(Ms, As, Bs, regs, bulk="fp64")
+    e = B.solve_bcd(tol=1e-10, max_iter=2000, graph=False)
+    g = B.solve_bcd(tol=1e-10, max_iter=2000, graph=True)
+    assert g["graph_launches"] >= 2 and g["graph_builds"] >= 2
+    np.testing.assert_array_equal(g["niter"], e["niter"])
+    np.testing.assert_array_equal(g["obj"], e["obj"])
+    np.testing.assert_array_equal(g["mar_err"], e["mar_err"])
+    np.testing.assert_array_equal(np.concatenate(g["dual"]), np.concatenate(e["dual"]))
+    r = gpu_lib.sinkhorn_bcd(Ms[0], As[0], Bs[0], regs[0], tol=1e-10, max_iter=2000)
+    assert abs(r.niter - int(g["niter"][0])) <= 1
+    g2 = B.solve_bcd(tol=1e-10, max_iter=2000, graph=True)
+    np.testing.assert_array_equal(np.concatenate(g2["dual"]), np.concatenate(e["dual"]))
+    assert g2["graph_launches"] > g["graph_launches"]
+    B.close()
+
+
+def test_batch_autotuned_tiles(gpu_lib):
+    # The autotuner times the coarse-tile candidates once per handle and keeps
+    # the fastest; the solve then starts from the reference init, so every
+    # problem still matches the oracle, and later solves reuse the choice.
+    # enough rows for coarse tiles (>= 2048 tiles of 16 rows)
+    Ms, As, Bs, regs = mixed_batch(6, 150, lo=200, hi=900, dtype=np.float32)
+    B = gpu_lib.BatchProblem(Ms, As, Bs, regs)
+    out = B.solve_bcd(tol=1e-7, max_iter=5000, autotune=True)
+    assert out["tuned"] and out["rows_per_wave"] in (2, 4)
+    assert out["block_rows"] in (256, 512, 1024)
+    assert sum(t > 0 for t in out["tune_ms"]) == 6
+    for p in range(0, len(Ms), 15):
+        o = O.sinkhorn_bcd(Ms[p].astype(np.float64), As[p], Bs[p], regs[p], tol=1e-7,
+                           max_iter=5000)
+        assert abs(int(out["niter"][p]) - o["niter"]) <= max(2, 0.05 * o["niter"]), p
+        assert out["mar_err"][p] < 1e-7
+    again = B.solve_bcd(tol=1e-7, max_iter=5000, autotune=True)
+    assert (again["rows_per_wave"], again["block_rows"]) == (out["rows_per_wave"],
+                                                              out["block_rows"])
+    np.testing.assert_array_equal(np.concatenate(again["dual"]), np.concatenate(out["dual"]))
+    B.close()

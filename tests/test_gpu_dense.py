@@ -122,3 +122,29 @@ def test_bcd_edge_cases(gpu_lib):
         o = O.sinkhorn_bcd(Mx, ax, bx, 0.1, tol=1e-10, max_iter=50)
         assert r.niter == o["niter"]
         assert rel_err(r.dual, o["dual"]) < 1e-10
+
+
+def test_device_fp64_exp2_accuracy(gpu_lib):
+    # The table-driven FP64 2^x of the one-pass objective / gradient and
+    # APDAGD kernels, against exp2 in 80-bit extended precision rounded to
+    # double (a near-correctly-rounded reference), on random and special
+    # arguments: at most one ulp, and rarely that.
+    from regot import _lib
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-1080.0, 1030.0, 500000), rng.uniform(-2.0, 2.0, 500000),
+                        np.arange(-1075.0, 1025.0, 0.25),
+                        [0.0, -0.0, 1023.999, 1024.0, -1074.0, -1074.5, -1075.0, -1100.5,
+                         1e300, -1e300, np.inf, -np.inf]])
+    y = np.empty_like(x)
+    _lib.check(_lib.lib().regot_check_exp2(_lib.dptr(x), _lib.dptr(y), len(x), 0))
+    with np.errstate(over="ignore"):
+        ref = np.exp2(x.astype(np.longdouble)).astype(np.float64)
+    fin = np.isfinite(ref) & (ref > 0)
+    ulp = np.abs(y[fin].view(np.int64) - ref[fin].view(np.int64))
+    assert ulp.max() <= 1
+    assert np.mean(ulp > 0) < 0.01
+    assert np.all(y[~fin] == ref[~fin])
+    z = np.array([np.nan])
+    w = np.empty(1)
+    _lib.check(_lib.lib().regot_check_exp2(_lib.dptr(z), _lib.dptr(w), 1, 0))
+    assert np.isnan(w[0])

@@ -48,7 +48,13 @@ def test_newton_dense_readme_and_fp32(gpu_lib):
     print("newton readme 100x80", flush=True)
     r = gpu_lib.sinkhorn_newton(M, a, b, 0.1, tol=1e-8, max_iter=100)
     o = O.sinkhorn_newton(M, a, b, 0.1, tol=1e-8, max_iter=100)
-    assert abs(r.niter - o["niter"]) <= 1 and r.mar_errs[-1] < 1e-8
+    # With theta = 0.5 and the exact Newton direction, a quadratic model meets
+    # the sufficient-decrease test with equality, so close to the optimum the
+    # unit step's acceptance is decided by rounding (in the reference too):
+    # here at ||g|| = 5.7e-7 a rejected unit step costs one or two halvings.
+    assert abs(r.niter - o["niter"]) <= 3 and r.mar_errs[-1] < 1e-8
+    k = 13
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-10)
     assert abs(r.obj_vals[-1] - o["obj_vals"][-1]) < 1e-12 * abs(o["obj_vals"][-1])
     # FP32 bulk: M in FP32, objective/gradient in FP64 sums, Hessian FP32
     M, a, b = cloud_problem(300, 260, seed=8)
