@@ -90,6 +90,16 @@ int regot_dual_obj_grad(regot_problem* p, const double* gamma, double* obj, doub
  * Replaces SinkhornResult::get_plan, src/sinkhorn_result.h:32-50. */
 int regot_get_plan(regot_problem* p, const double* gamma, double* plan);
 
+/* Feasible rounding of plan(gamma) onto the transport polytope of (a, b),
+ * FP64 row-major n x m into host memory.  Replaces approx_proj,
+ * src/approx_proj.h:6-26, applied to T(gamma) as Problem::primal_val does. */
+int regot_round_plan(regot_problem* p, const double* gamma, double* plan);
+
+/* Primal objective <P, M> - reg sum P (1 - log P) of the rounded plan
+ * P = approx_proj(T(gamma)).  Replaces Problem::primal_val,
+ * src/sinkhorn_problem_obj.cpp:300-319. */
+int regot_primal_val(regot_problem* p, const double* gamma, double* value);
+
 /*
  * Sparsified Hessian Hs = [diag(h1) sigma; sigma' diag(h2)] at gamma (host),
  * held on the device in CSR and CSC.  The gradient at gamma is computed here

@@ -1689,3 +1689,71 @@ int orc_sinkhorn_apdagd(int n, int m, const double* M, const double* a, const do
     *hist_len = h;
     return i;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Feasible rounding and the primal value                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Rounding of a nonnegative n x m plan onto the transport polytope of (a, b):
+ * scale rows down to a, then columns down to b, then add the rank-one
+ * correction err_a err_b' / |err_a|_1.  Restates approx_proj,
+ * src/approx_proj.h:6-26 (Altschuler, Weed and Rigollet 2017, Algorithm 2),
+ * including its order of operations: err_b is taken from the column sums of
+ * the scaled matrix, and an exactly feasible input makes |err_a|_1 = 0, so
+ * the correction (and the result) is NaN as in the reference. */
+void orc_approx_proj(int n, int m, const double* plan, const double* a, const double* b,
+                     double* out)
+{
+    double* x = (double*)malloc(sizeof(double) * n);
+    double* y = (double*)malloc(sizeof(double) * m);
+    double* ea = (double*)malloc(sizeof(double) * n);
+    double* eb = (double*)calloc(m, sizeof(double));
+    for (int i = 0; i < n; i++) {
+        double r = 0.0;
+        for (int j = 0; j < m; j++) r += plan[(long)i * m + j];
+        x[i] = fmin(a[i] / r, 1.0);
+    }
+    for (long q = 0; q < (long)n * m; q++) out[q] = plan[q] * x[q / m];
+    for (int j = 0; j < m; j++) y[j] = 0.0;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m; j++) y[j] += out[(long)i * m + j];
+    for (int j = 0; j < m; j++) y[j] = fmin(b[j] / y[j], 1.0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m; j++) out[(long)i * m + j] *= y[j];
+    double l1 = 0.0;
+    for (int i = 0; i < n; i++) {
+        double r = 0.0;
+        for (int j = 0; j < m; j++) r += out[(long)i * m + j];
+        ea[i] = a[i] - r;
+        l1 += fabs(ea[i]);
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m; j++) eb[j] += out[(long)i * m + j];
+    for (int j = 0; j < m; j++) eb[j] = b[j] - eb[j];
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m; j++) out[(long)i * m + j] += ea[i] * eb[j] / l1;
+    free(x);
+    free(y);
+    free(ea);
+    free(eb);
+}
+
+/* Primal objective of the rounded plan P = approx_proj(T(gamma)):
+ * <P, M> - reg sum P (1 - log P).  Restates Problem::primal_val,
+ * src/sinkhorn_problem_obj.cpp:300-319. */
+double orc_primal_val(int n, int m, const double* M, const double* a, const double* b,
+                      double reg, const double* gamma)
+{
+    double* T = (double*)malloc(sizeof(double) * (size_t)n * m);
+    double* P = (double*)malloc(sizeof(double) * (size_t)n * m);
+    orc_get_plan(n, m, M, reg, gamma, T);
+    orc_approx_proj(n, m, T, a, b, P);
+    double ent = 0.0, mp = 0.0;
+    for (long q = 0; q < (long)n * m; q++) {
+        ent += P[q] * (1.0 - log(P[q]));
+        mp += P[q] * M[q];
+    }
+    free(T);
+    free(P);
+    return mp - reg * ent;
+}

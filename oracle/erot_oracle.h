@@ -136,6 +136,12 @@ int orc_sinkhorn_apdagd(int n, int m, const double* M, const double* a, const do
                         double* dual, double* plan, double* obj_vals, double* mar_errs,
                         double* run_times, int* hist_len);
 
+/* ---- Feasible rounding and primal value (src/approx_proj.h, sinkhorn_problem_obj.cpp:300-319) ---- */
+void orc_approx_proj(int n, int m, const double* plan, const double* a, const double* b,
+                     double* out);
+double orc_primal_val(int n, int m, const double* M, const double* a, const double* b,
+                      double reg, const double* gamma);
+
 /* BCD iterations only (no convergence test), for timing a bounded CPU sample. */
 double orc_bcd_iterations(int n, int m, const double* M, const double* a, const double* b,
                           double reg, int iters, double* gamma_out);

@@ -48,7 +48,7 @@ def lib():
             build()
         L = ctypes.CDLL(_SO)
         for name in ("orc_dual_obj", "orc_dual_obj_grad", "orc_compute_T", "orc_bcd_iterations",
-                     "orc_cubic_minimizer", "orc_step_selection"):
+                     "orc_cubic_minimizer", "orc_step_selection", "orc_primal_val"):
             getattr(L, name).restype = ctypes.c_double
         for name in ("orc_sparsify_mat3", "orc_sparsify_density"):
             getattr(L, name).restype = ctypes.c_longlong
@@ -379,3 +379,20 @@ def sinkhorn_apdagd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0):
     k = hl.value
     return dict(niter=niter, dual=dual, plan=plan, obj_vals=list(h[0][:k]),
                 mar_errs=list(h[1][:k]), run_times=list(h[2][:k]), densities=[])
+
+
+def approx_proj(plan, a, b):
+    """Feasible rounding of a plan (src/approx_proj.h:6-26)"""
+    plan, a, b = _d(plan), _d(a), _d(b)
+    n, m = plan.shape
+    out = np.empty((n, m))
+    lib().orc_approx_proj(n, m, _p(plan), _p(a), _p(b), _p(out))
+    return out
+
+
+def primal_val(M, a, b, reg, gamma):
+    """Primal objective of the rounded plan (src/sinkhorn_problem_obj.cpp:300-319)"""
+    M, a, b, gamma = _d(M), _d(a), _d(b), _d(gamma)
+    n, m = M.shape
+    return lib().orc_primal_val(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg), _p(gamma))
+

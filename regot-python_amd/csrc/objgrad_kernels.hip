@@ -63,14 +63,16 @@ __device__ __forceinline__ void ldm4(const double* p, int c0, int m, double (&v)
 }
 
 // grid (strips = ceil(m / 256), row blocks = ceil(n / rows)), 256 threads.
-// beta has m entries (beta_{m-1} = 0).  rowpart[strip * n + i] = sum of row i
-// over the strip; colpart[block * m + j] (when non-null) = sum of column j
-// over the block.
+// beta has m entries (beta_{m-1} = 0).  Sums of w_ij = x_i T_ij y_j (x, y
+// nullable: weight 1): rowpart[strip * n + i] = sum of row i over the strip;
+// colpart[block * m + j] (when non-null) = sum of column j over the block.
 template <typename TM>
 __global__ __launch_bounds__(256) void k_sums_tile(const TM* __restrict__ M, long ldm, int n, int m,
                                                    const double* __restrict__ alpha,
                                                    const double* __restrict__ beta, double k,
-                                                   int rows, double* __restrict__ rowpart,
+                                                   int rows, const double* __restrict__ xw,
+                                                   const double* __restrict__ yw,
+                                                   double* __restrict__ rowpart,
                                                    double* __restrict__ colpart)
 {
     __shared__ double sm[4][256];
@@ -82,19 +84,25 @@ __global__ __launch_bounds__(256) void k_sums_tile(const TM* __restrict__ M, lon
     const int c0 = strip * 256 + lane * 4;
     const int r0 = blk * rows;
     const int r1 = min(n, r0 + rows);
-    double bk[4];
+    double bk[4], yq[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) bk[q] = (c0 + q < m) ? beta[c0 + q] : 0.0;
+    for (int q = 0; q < 4; q++) {
+        bk[q] = (c0 + q < m) ? beta[c0 + q] : 0.0;
+        yq[q] = (yw && c0 + q < m) ? yw[c0 + q] : 1.0;
+    }
     double cs[4] = {0.0, 0.0, 0.0, 0.0};
     for (int r = r0 + wave; r < r1; r += 4) {
         const double ai = alpha[r];
+        const double xi = xw ? xw[r] : 1.0;
         double mv[4];
         ldm4(M + static_cast<long>(r) * ldm, c0, m, mv);
         double rs = 0.0;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             if (c0 + q < m) {
-                const double t = dev::dexp2((ai + bk[q] - mv[q]) * k, tab);
+                double t = dev::dexp2((ai + bk[q] - mv[q]) * k, tab);
+                if (xw) t *= xi;
+                if (yw) t *= yq[q];
                 rs += t;
                 cs[q] += t;
             }
@@ -244,17 +252,35 @@ void launch_objgrad_fused(int bulk, const void* M, long ldm, int n, int m, const
     double* cp = with_grad ? colpart : nullptr;
     if (bulk == 0)
         hipLaunchKernelGGL(k_sums_tile<float>, grid, dim3(256), 0, s, static_cast<const float*>(M),
-                           ldm, n, m, alpha, beta, k, rows, rowpart, cp);
+                           ldm, n, m, alpha, beta, k, rows, nullptr, nullptr, rowpart, cp);
     else
         hipLaunchKernelGGL(k_sums_tile<double>, grid, dim3(256), 0, s,
                            static_cast<const double*>(M), ldm, n, m, alpha, beta, k, rows,
-                           rowpart, cp);
+                           nullptr, nullptr, rowpart, cp);
     KERNEL_CHECK();
     const int nf = ceil_div(std::max(n, m), 256);
     hipLaunchKernelGGL(k_sums_fold, dim3(nf), dim3(256), 0, s, rowpart, nstrip, cp, nblk, alpha,
                        beta, a, b, n, m, with_grad ? grad : nullptr, part);
     KERNEL_CHECK();
     hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, s, part, nf, reg, out);
+    KERNEL_CHECK();
+}
+
+void launch_wsums(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                  const double* beta, double reg, const double* x, const double* y,
+                  double* rowpart, double* colpart, hipStream_t s)
+{
+    if (n <= 0 || m <= 0) fail(E_INVALID, "weighted sums on an empty problem");
+    const int rows = sums_rows(n, m);
+    const dim3 grid(ceil_div(m, 256), ceil_div(n, rows));
+    const double k = 1.4426950408889634074 / reg;
+    if (bulk == 0)
+        hipLaunchKernelGGL(k_sums_tile<float>, grid, dim3(256), 0, s, static_cast<const float*>(M),
+                           ldm, n, m, alpha, beta, k, rows, x, y, rowpart, colpart);
+    else
+        hipLaunchKernelGGL(k_sums_tile<double>, grid, dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, n, m, alpha, beta, k, rows, x, y,
+                           rowpart, colpart);
     KERNEL_CHECK();
 }
 

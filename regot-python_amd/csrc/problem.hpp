@@ -61,6 +61,15 @@ void launch_objgrad_fused(int bulk, const void* M, long ldm, int n, int m, const
                           bool with_grad, double* grad, double* rowpart, double* colpart,
                           double* part, double* out, hipStream_t s);
 
+// Row sums (per 256-column strip: rowpart[strip * n + i]) and, when colpart
+// is set, column sums (per row block: colpart[block * m + j]) of
+// w_ij = x_i T_ij y_j in FP64 (x, y nullable = 1); block sizes as the
+// one-pass objective kernel (sums_rows / sums_scratch_sizes).
+int sums_rows(int n, int m);
+void launch_wsums(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                  const double* beta, double reg, const double* x, const double* y,
+                  double* rowpart, double* colpart, hipStream_t s);
+
 // y = 2^x by the device FP64 exponential of the one-pass kernels (dexp2.hpp).
 void launch_exp2_check(const double* x, double* y, long count, hipStream_t s);
 

@@ -216,6 +216,21 @@ class Problem:
         check(lib().regot_get_plan(self._h, dptr(gamma), dptr(plan)))
         return plan
 
+    def round_plan(self, gamma):
+        """approx_proj of plan(gamma): the nearest-feasible rounding onto the
+        transport polytope (src/approx_proj.h:6-26)."""
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        plan = np.empty((self.n, self.m))
+        check(lib().regot_round_plan(self._h, dptr(g), dptr(plan)))
+        return plan
+
+    def primal_val(self, gamma):
+        """Primal objective of the rounded plan (src/sinkhorn_problem_obj.cpp:300-319)."""
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        v = _lib.ctypes.c_double()
+        check(lib().regot_primal_val(self._h, dptr(g), _lib.ctypes.byref(v)))
+        return v.value
+
     # --- sparsified Hessian ------------------------------------------------
     def sparsified_hess(self, gamma, delta, density_hint=0.001):
         """Threshold rule at gamma (src/sinkhorn_sparsify3.cpp:276-405); returns nnz."""

@@ -87,6 +87,8 @@ def lib():
     L.regot_dual_obj.argtypes = [_vp, _dp, _dp]
     L.regot_dual_obj_grad.argtypes = [_vp, _dp, _dp, _dp]
     L.regot_get_plan.argtypes = [_vp, _dp, _dp]
+    L.regot_round_plan.argtypes = [_vp, _dp, _dp]
+    L.regot_primal_val.argtypes = [_vp, _dp, _dp]
     L.regot_opts_default.argtypes = [ctypes.POINTER(Opts), ctypes.c_int]
     L.regot_opts_default.restype = None
     L.regot_solve.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,

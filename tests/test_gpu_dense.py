@@ -148,3 +148,32 @@ def test_device_fp64_exp2_accuracy(gpu_lib):
     w = np.empty(1)
     _lib.check(_lib.lib().regot_check_exp2(_lib.dptr(z), _lib.dptr(w), 1, 0))
     assert np.isnan(w[0])
+
+
+@pytest.mark.parametrize("bulk", ["fp64", "fp32"])
+def test_round_plan_and_primal_value(gpu_lib, bulk):
+    # approx_proj of T(gamma) and the primal value against the oracle, on a
+    # ragged shape (strips and row blocks not full), at a point away from the
+    # optimum (large rounding correction) and at the optimum (primal = -f*)
+    from conftest import cloud_problem
+    M, a, b = cloud_problem(301, 517, seed=4)
+    if bulk == "fp32":
+        M = M.astype(np.float32).astype(np.float64)
+        Mp = M.astype(np.float32)
+    else:
+        Mp = M
+    reg = 0.05
+    o = O.sinkhorn_bcd(M, a, b, reg, tol=1e-11, max_iter=20000)
+    rng = np.random.default_rng(1)
+    with gpu_lib.Problem(Mp, a, b, reg) as P:
+        for g in (o["dual"], o["dual"] + 0.02 * rng.standard_normal(len(o["dual"]))):
+            R = P.round_plan(g)
+            Ro = O.approx_proj(O.get_plan(M, reg, g), a, b)
+            # the correction err_a err_b' / |err_a|_1 carries the cancellation
+            # of a - rowsum: absolute agreement at 1e-16 of a unit-mass plan
+            np.testing.assert_allclose(R, Ro, rtol=1e-12, atol=1e-16)
+            np.testing.assert_allclose(R.sum(1), a, rtol=1e-12)
+            np.testing.assert_allclose(R.sum(0), b, rtol=1e-12)
+            pv = P.primal_val(g)
+            assert abs(pv - O.primal_val(M, a, b, reg, g)) < 1e-12 * max(1.0, abs(pv))
+        assert abs(P.primal_val(o["dual"]) + o["obj_vals"][-1]) < 1e-9

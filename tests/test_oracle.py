@@ -432,3 +432,40 @@ def test_separable_cost_newton_and_lbfgs():
     np.testing.assert_allclose(O.get_plan(M, reg, r["dual"]), np.outer(a, b), rtol=1e-6)
     with pytest.raises(RuntimeError, match="unable to sufficiently decrease"):
         O.sinkhorn_lbfgs_dual(M, a, b, reg, tol=1e-12, max_iter=200)
+
+
+# --- feasible rounding and primal value (src/approx_proj.h, src/sinkhorn_problem_obj.cpp:300-319) ---
+
+def test_approx_proj_feasibility_and_bound():
+    # Altschuler, Weed and Rigollet (2017), Lemma 7: the rounded plan has
+    # marginals exactly (a, b), is nonnegative, and moves by at most
+    # 2 (|r - a|_1 + |c - b|_1) in l1 from the input.
+    rng = np.random.default_rng(41)
+    for n, m in ((7, 5), (30, 44), (1, 9)):
+        a = rng.random(n) + 0.1
+        b = rng.random(m) + 0.1
+        a, b = a / a.sum(), b / b.sum()
+        T = rng.random((n, m)) * rng.uniform(0.5, 2.0) / (n * m)
+        P = O.approx_proj(T, a, b)
+        np.testing.assert_allclose(P.sum(1), a, rtol=1e-13, atol=1e-16)
+        np.testing.assert_allclose(P.sum(0), b, rtol=1e-13, atol=1e-16)
+        assert P.min() >= -1e-17
+        bound = 2 * (np.abs(T.sum(1) - a).sum() + np.abs(T.sum(0) - b).sum())
+        assert np.abs(P - T).sum() <= bound * (1 + 1e-12)
+    # an exactly feasible input leaves |err_a|_1 = 0: the reference's 0/0
+    T = np.full((2, 2), 0.25)
+    assert np.isnan(O.approx_proj(T, np.array([0.5, 0.5]), np.array([0.5, 0.5]))).all()
+
+
+def test_primal_value_meets_the_dual_at_the_optimum():
+    # strong duality of the entropic problem: at the dual optimum the primal
+    # value of the (nearly feasible) plan equals -f*; away from it the primal
+    # value of a feasible plan bounds -f from above (weak duality)
+    M, a, b = cloud_problem(25, 19, d=3, seed=8)
+    reg = 0.1
+    r = O.sinkhorn_bcd(M, a, b, reg, tol=1e-11, max_iter=100000)
+    pv = O.primal_val(M, a, b, reg, r["dual"])
+    assert abs(pv + r["obj_vals"][-1]) < 1e-9
+    g = r["dual"] + 0.05 * np.random.default_rng(3).standard_normal(len(r["dual"]))
+    f = O.dual_obj(M, a, b, reg, g)
+    assert O.primal_val(M, a, b, reg, g) >= -f - 1e-12
