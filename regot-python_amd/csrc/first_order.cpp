@@ -53,13 +53,14 @@ struct LbfgsState {
     double* xp;
     double* drt;
     double last_f = 0.0;   // the most recent objective evaluation
+    double gnorm = 0.0;    // ||grad|| at x (from the evaluation that formed grad)
 };
 
 // Strong-Wolfe search (ftol 1e-4, wolfe 0.9, at most 20 evaluations, x2
 // expansion): bracketing, then zoom.  LineSearchNocedalWright::LineSearch,
 // src/LBFGSpp/LBFGSpp/LineSearchNocedalWright.h:58-283.  On entry fx, grad,
-// dg hold the values at xp; on return x, fx, grad, dg and step describe the
-// accepted point.
+// dg (and st.gnorm) hold the values at xp; on return x, fx, grad, dg, step
+// and st.gnorm describe the accepted point.
 void nw_line_search(Work& w, LbfgsState& st, double& step, double& fx, double& dg)
 {
     const int max_ls = 20;
@@ -71,9 +72,10 @@ void nw_line_search(Work& w, LbfgsState& st, double& step, double& fx, double& d
     double step_lo = 0.0, fx_lo = fx_init, dg_lo = dg_init;
     vec_copy(st.xlo, st.xp, w.N, w.s);
     vec_copy(st.glo, st.grad, w.N, w.s);
+    double gn = st.gnorm, gn_lo = st.gnorm;
     auto eval = [&]() {
         w.move(st.x, st.xp, step, st.drt);
-        fx = w.obj_grad(st.x, st.grad, nullptr);
+        fx = w.obj_grad(st.x, st.grad, &gn);
         st.last_f = fx;
         dg = w.dot(st.grad, st.drt);
     };
@@ -85,12 +87,16 @@ void nw_line_search(Work& w, LbfgsState& st, double& step, double& fx, double& d
             fx_hi = fx;
             break;
         }
-        if (std::fabs(dg) <= test_curv) return;
+        if (std::fabs(dg) <= test_curv) {
+            st.gnorm = gn;
+            return;
+        }
         step_hi = step_lo;
         fx_hi = fx_lo;
         step_lo = step;
         fx_lo = fx;
         dg_lo = dg;
+        gn_lo = gn;
         std::swap(st.xlo, st.x);
         std::swap(st.glo, st.grad);
         if (dg >= 0.0) break;
@@ -99,6 +105,7 @@ void nw_line_search(Work& w, LbfgsState& st, double& step, double& fx, double& d
             // every trial lowered f: the latest one (now in the _lo slots)
             std::swap(st.x, st.xlo);
             std::swap(st.grad, st.glo);
+            st.gnorm = gn_lo;
             return;
         }
         step *= expansion;
@@ -111,7 +118,10 @@ void nw_line_search(Work& w, LbfgsState& st, double& step, double& fx, double& d
             step_hi = step;
             fx_hi = fx;
         } else {
-            if (std::fabs(dg) <= test_curv) return;
+            if (std::fabs(dg) <= test_curv) {
+                st.gnorm = gn;
+                return;
+            }
             if (dg * (step_hi - step_lo) >= 0.0) {
                 step_hi = step_lo;
                 fx_hi = fx_lo;
@@ -120,6 +130,7 @@ void nw_line_search(Work& w, LbfgsState& st, double& step, double& fx, double& d
             step_lo = step;
             fx_lo = fx;
             dg_lo = dg;
+            gn_lo = gn;
             std::swap(st.xlo, st.x);
             std::swap(st.glo, st.grad);
         }
@@ -131,6 +142,7 @@ void nw_line_search(Work& w, LbfgsState& st, double& step, double& fx, double& d
             dg = dg_lo;
             std::swap(st.x, st.xlo);
             std::swap(st.grad, st.glo);
+            st.gnorm = gn_lo;
             return;
         }
     }
@@ -146,7 +158,12 @@ void nw_line_search(Work& w, LbfgsState& st, double& step, double& fx, double& d
 // (src/sinkhorn_lbfgs_dual.cpp:56-76): obj_vals records the most recent
 // objective evaluation, mar_errs the accepted point's ||grad||.  niter is
 // minimize()'s return value (1 when the initial point already passes).  The
-// two-loop product is one device kernel over the correction ring.
+// two-loop product runs on scalars: one many-workgroup pass forms every dot
+// product it needs (the candidate pair against itself, the kept pairs and the
+// gradient; the kept pairs against the gradient), Gram matrices of the kept
+// pairs carry the rest, and one pass forms the direction as a linear
+// combination of the gradient and the pairs (mathematically the recursion's
+// result; the rounding differs).
 void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
                       const regot_opts& o, regot_result* res)
 {
@@ -155,7 +172,16 @@ void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
     PreciseScope ps(p);
     Work w(p);
     w.init(o);
-    DBuf<double> S(static_cast<size_t>(mm) * w.N), Y(static_cast<size_t>(mm) * w.N);
+    const long N = w.N;
+    DBuf<double> S(static_cast<size_t>(mm) * N), Y(static_cast<size_t>(mm) * N);
+    DBuf<double> dpart(vec_mdots_scratch(N)), dout(32);
+    HBuf<double> hout;
+    hout.ensure(32);
+    auto slot_s = [&](int i) { return S.p + static_cast<long>(i) * N; };
+    auto slot_y = [&](int i) { return Y.p + static_cast<long>(i) * N; };
+    // Gram matrices of the correction pairs (by ring slot): SY[i][j] = s_i'y_j,
+    // YY[i][j] = y_i'y_j; ys[i] = SY[i][i]
+    double SY[6][6] = {}, YY[6][6] = {};
     std::vector<double> ys(mm, 0.0);
     int ncorr = 0, ptr = mm;
     double theta = 1.0;
@@ -165,6 +191,7 @@ void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
     double gnorm;
     double fx = w.obj_grad(st.x, st.grad, &gnorm);
     st.last_f = fx;
+    st.gnorm = gnorm;
     int k = 0;
     auto iterate = [&]() {
         const double dur = timer.toc();
@@ -183,45 +210,131 @@ void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
     if (gnorm <= tol || gnorm <= 0.0) {
         k = 1;
     } else {
-        vec_axpby(st.drt, -1.0, st.grad, 0.0, nullptr, w.N, w.s);
+        vec_axpby(st.drt, -1.0, st.grad, 0.0, nullptr, N, w.s);
         double step = 1.0 / gnorm;   // 1 / ||drt||, drt = -grad
+        double dg = -gnorm * gnorm;  // grad'drt
         k = 1;
         for (;;) {
-            vec_copy(st.xp, st.x, w.N, w.s);
-            vec_copy(w.g_pre, st.grad, w.N, w.s);
-            double dg = w.dot(st.grad, st.drt);
+            vec_copy(st.xp, st.x, N, w.s);
+            vec_copy(w.g_pre, st.grad, N, w.s);
             nw_line_search(w, st, step, fx, dg);
-            gnorm = std::sqrt(w.dot(st.grad, st.grad));
+            gnorm = st.gnorm;
             iterate();
             if (gnorm <= tol || gnorm <= 0.0) break;
             if (max_iter != 0 && k >= max_iter) break;
-            // s = x - xp, y = grad - gradp into the next ring slot; kept only
-            // when s'y > eps y'y (the slot is reused otherwise)
+            // Candidate pair s = x - xp, y = grad - gradp (temporaries: the
+            // ring keeps its slots unless the pair passes s'y > eps y'y), and in
+            // the same pass every dot product the two-loop recursion needs:
+            // the candidate's against itself, the kept pairs and grad, and
+            // the kept pairs' against grad.
+            const double* v = st.grad;
+            vec_axpby(w.sv, 1.0, st.x, -1.0, st.xp, N, w.s);
+            vec_axpby(w.y, 1.0, st.grad, -1.0, w.g_pre, N, w.s);
             const int loc = ptr % mm;
-            double* sl = S.p + static_cast<long>(loc) * w.N;
-            double* yl = Y.p + static_cast<long>(loc) * w.N;
-            vec_axpby(sl, 1.0, st.x, -1.0, st.xp, w.N, w.s);
-            vec_axpby(yl, 1.0, st.grad, -1.0, w.g_pre, w.N, w.s);
-            const double* xs[2] = {sl, yl};
-            const double* zs[2] = {yl, yl};
-            double d[2];
-            w.dots(2, xs, zs, d);
-            if (d[0] > eps * d[1]) {
-                ys[loc] = d[0];
-                theta = d[1] / d[0];
+            std::vector<int> act;
+            for (int c = 0, j = ptr % mm; c < ncorr; c++) {
+                j = (j + mm - 1) % mm;
+                act.push_back(j);
+            }
+            const double* xs[32];
+            const double* zs[32];
+            int np = 0;
+            auto add = [&](const double* x_, const double* z_) {
+                xs[np] = x_;
+                zs[np] = z_;
+                return np++;
+            };
+            const int i_sy = add(w.sv, w.y), i_yy = add(w.y, w.y), i_sv = add(w.sv, v),
+                      i_yv = add(w.y, v), i_vv = add(v, v);
+            int i_sYi[6], i_Siy[6], i_yYi[6], i_Sv[6], i_Yv[6];
+            for (int i : act) {
+                if (i != loc) {
+                    i_sYi[i] = add(w.sv, slot_y(i));
+                    i_Siy[i] = add(slot_s(i), w.y);
+                    i_yYi[i] = add(w.y, slot_y(i));
+                }
+                i_Sv[i] = add(slot_s(i), v);
+                i_Yv[i] = add(slot_y(i), v);
+            }
+            vec_mdots(xs, zs, np, N, dpart, dout, w.s);
+            HIP_CHECK(hipMemcpyAsync(hout.p, dout.p, np * sizeof(double), hipMemcpyDeviceToHost,
+                                     w.s));
+            HIP_CHECK(hipStreamSynchronize(w.s));
+            const double* d = hout.p;
+            double Sv[6] = {}, Yv[6] = {};
+            for (int i : act) {
+                Sv[i] = d[i_Sv[i]];
+                Yv[i] = d[i_Yv[i]];
+            }
+            if (d[i_sy] > eps * d[i_yy]) {
+                // BFGSMat::add_correction: the pair takes slot loc
+                vec_copy(slot_s(loc), w.sv, N, w.s);
+                vec_copy(slot_y(loc), w.y, N, w.s);
+                for (int i : act) {
+                    if (i == loc) continue;
+                    SY[loc][i] = d[i_sYi[i]];
+                    SY[i][loc] = d[i_Siy[i]];
+                    YY[loc][i] = YY[i][loc] = d[i_yYi[i]];
+                }
+                SY[loc][loc] = ys[loc] = d[i_sy];
+                YY[loc][loc] = d[i_yy];
+                Sv[loc] = d[i_sv];
+                Yv[loc] = d[i_yv];
+                theta = d[i_yy] / d[i_sy];
                 if (ncorr < mm) ncorr++;
                 ptr = loc + 1;
             }
-            vec_lbfgs_apply(S, Y, ys.data(), mm, ncorr, ptr, theta, st.grad, -1.0, st.drt, w.N,
-                            w.s);
+            // BFGSMat::apply_Hv(grad, -1, drt) on the scalars: loop 1 (newest
+            // first) alpha_j = s_j'q / ys_j with q = -grad - sum alpha_i y_i;
+            // r = q / theta; loop 2 (oldest first) beta_j = y_j'r / ys_j,
+            // r += (alpha_j - beta_j) s_j.  The inner products of q and r
+            // expand into the dots above and the Gram matrices.
+            const double a = -1.0;
+            double alpha[6] = {}, beta[6] = {};
+            int order[6], nc = 0;
+            for (int c = 0, j = ptr % mm; c < ncorr; c++) {
+                j = (j + mm - 1) % mm;
+                double sq = a * Sv[j];
+                for (int q = 0; q < nc; q++) sq -= alpha[order[q]] * SY[j][order[q]];
+                alpha[j] = sq / ys[j];
+                order[nc++] = j;
+            }
+            double done2_coef[6] = {};
+            int done2[6], nd = 0;
+            for (int c = nc - 1; c >= 0; c--) {
+                const int j = order[c];
+                double yq = a * Yv[j];
+                for (int q = 0; q < nc; q++) yq -= alpha[order[q]] * YY[j][order[q]];
+                double yr = yq / theta;
+                for (int q = 0; q < nd; q++) yr += done2_coef[q] * SY[done2[q]][j];
+                beta[j] = yr / ys[j];
+                done2[nd] = j;
+                done2_coef[nd++] = alpha[j] - beta[j];
+            }
+            // drt = (a / theta) grad - sum (alpha_i / theta) y_i + sum (alpha_i - beta_i) s_i
+            const double* vs[13];
+            double cf[13];
+            int nv = 0;
+            vs[nv] = v;
+            cf[nv++] = a / theta;
+            dg = (a / theta) * d[i_vv];
+            for (int q = 0; q < nc; q++) {
+                const int j = order[q];
+                vs[nv] = slot_y(j);
+                cf[nv++] = -alpha[j] / theta;
+                vs[nv] = slot_s(j);
+                cf[nv++] = alpha[j] - beta[j];
+                dg += (-alpha[j] / theta) * Yv[j] + (alpha[j] - beta[j]) * Sv[j];
+            }
+            vec_lincomb(vs, cf, nv, N, st.drt, w.s);
             step = 1.0;
             k++;
         }
     }
     res->niter = k;
     w.report(res);
-    res->dual.resize(w.N);
-    HIP_CHECK(hipMemcpy(res->dual.data(), st.x, w.N * sizeof(double), hipMemcpyDeviceToHost));
+    res->dual.resize(N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), st.x, N * sizeof(double), hipMemcpyDeviceToHost));
 }
 
 // Accelerated primal-dual adaptive gradient descent.  Mirrors

@@ -34,12 +34,14 @@ void vec_objgrad_finish(const double* r, const double* c, const double* a, const
 void vec_obj_finish(const double* r, const double* a, const double* b, const double* gamma,
                     int n, int m, double reg, double* out, hipStream_t s);
 
-// res = H (a v), H the L-BFGS inverse-Hessian operator of ncorr pairs in a
-// ring of mm <= 8 slots (S, Y: mm x count, slot j at j * count; ys host
-// array of s'y per slot; ptr = slot after the newest pair), H0 = I / theta.
-void vec_lbfgs_apply(const double* S, const double* Y, const double* ys, int mm, int ncorr,
-                     int ptr, double theta, const double* v, double a, double* res, long count,
-                     hipStream_t s);
+// out[q] = x_q . y_q for up to 32 pairs over count entries, one pass of many
+// workgroups (part: vec_mdots_scratch(count) doubles), result on the device.
+void vec_mdots(const double* const* xs, const double* const* ys, int npairs, long count,
+               double* part, double* out, hipStream_t s);
+long vec_mdots_scratch(long count);
+// out = sum_k coef[k] vs[k] (host coefficients, up to 16 device vectors).
+void vec_lincomb(const double* const* vs, const double* coef, int nv, long count, double* out,
+                 hipStream_t s);
 
 // APDAGD primal-plan pass (apdagd_kernels.hip): P <- (1 - tau) P + tau T
 // (P <- T when init), T_ij = exp((alpha_i + beta_j - M_ij) / reg) in FP64,

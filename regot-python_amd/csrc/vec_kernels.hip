@@ -90,50 +90,78 @@ __global__ void k_normalise(const double* alpha, const double* beta, int n, int 
     if (t < m - 1) gamma[n + t] = beta[t] - bm;
 }
 
-// L-BFGS two-loop product res = H (a v) with the inverse-Hessian operator of
-// the last ncorr pairs (s_j, y_j) in a ring of mm slots (slot j at S + j N),
-// H0 = I / theta.  Order of operations as BFGSMat::apply_Hv,
-// src/LBFGSpp/LBFGSpp/BFGSMat.h:276-302.  One 1024-thread workgroup: every
-// thread owns the indices i = t mod 1024, so only the dot products need the
-// workgroup (fixed-order tree, reproducible).
-struct LbfgsArgs {
-    const double* S;
-    const double* Y;
-    long N;
-    int mm, ncorr, ptr;
-    double theta;
-    double ys[8];
+// Many dot products over one index range in one pass: workgroup g sums its
+// contiguous chunk for every pair (all loads of a thread independent, fixed
+// order), then one workgroup folds the chunks.  Used where a single-workgroup reduction chain would be latency
+// bound (the L-BFGS update reads up to 31 dot products at once).
+struct MDotArgs {
+    const double* x[32];
+    const double* y[32];
 };
 
-__global__ __launch_bounds__(1024) void k_lbfgs_apply(LbfgsArgs L, const double* __restrict__ v,
-                                                      double a, double* __restrict__ res)
+__global__ __launch_bounds__(256) void k_mdots(MDotArgs d, int npairs, long count, long chunk,
+                                               double* __restrict__ part)
 {
-    __shared__ double sm[1024];
-    __shared__ double al[8];
-    const long N = L.N;
-    for (long i = threadIdx.x; i < N; i += 1024) res[i] = a * v[i];
-    int j = L.ptr % L.mm;
-    for (int k = 0; k < L.ncorr; k++) {
-        j = (j + L.mm - 1) % L.mm;
-        const double* sj = L.S + static_cast<long>(j) * N;
-        const double* yj = L.Y + static_cast<long>(j) * N;
-        double d = 0.0;
-        for (long i = threadIdx.x; i < N; i += 1024) d += sj[i] * res[i];
-        const double aj = block_reduce(d, sm) / L.ys[j];
-        if (threadIdx.x == 0) al[j] = aj;
-        for (long i = threadIdx.x; i < N; i += 1024) res[i] -= aj * yj[i];
+    __shared__ double sm[4][32];
+    const long i0 = static_cast<long>(blockIdx.x) * chunk;
+    const long i1 = (i0 + chunk < count) ? i0 + chunk : count;
+    double v[32];
+#pragma unroll
+    for (int q = 0; q < 32; q++) v[q] = 0.0;
+    for (long i = i0 + threadIdx.x; i < i1; i += 256) {
+#pragma unroll
+        for (int q = 0; q < 32; q++)
+            if (q < npairs) v[q] += d.x[q][i] * d.y[q][i];
     }
-    for (long i = threadIdx.x; i < N; i += 1024) res[i] /= L.theta;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 32; q++) {
+        if (q < npairs) {
+            double t = v[q];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+            if (lane == 0) sm[wave][q] = t;
+        }
+    }
     __syncthreads();
-    for (int k = 0; k < L.ncorr; k++) {
-        const double* sj = L.S + static_cast<long>(j) * N;
-        const double* yj = L.Y + static_cast<long>(j) * N;
-        double d = 0.0;
-        for (long i = threadIdx.x; i < N; i += 1024) d += yj[i] * res[i];
-        const double bet = block_reduce(d, sm) / L.ys[j];
-        const double c = al[j] - bet;
-        for (long i = threadIdx.x; i < N; i += 1024) res[i] += c * sj[i];
-        j = (j + 1) % L.mm;
+    if (threadIdx.x < npairs) {
+        const int q = threadIdx.x;
+        part[static_cast<long>(blockIdx.x) * 32 + q] = ((sm[0][q] + sm[1][q]) + sm[2][q]) + sm[3][q];
+    }
+}
+
+// One workgroup of 256: 8 threads per pair sum strided chunks, then the 8
+// partials in order.
+__global__ __launch_bounds__(256) void k_mdots_fold(const double* __restrict__ part, int nchunks,
+                                                    int npairs, double* __restrict__ out)
+{
+    __shared__ double sm[256];
+    const int q = threadIdx.x >> 3, sub = threadIdx.x & 7;
+    double v = 0.0;
+    if (q < npairs)
+        for (int g = sub; g < nchunks; g += 8) v += part[static_cast<long>(g) * 32 + q];
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    if (sub == 0 && q < npairs) {
+        double t = 0.0;
+        for (int u = 0; u < 8; u++) t += sm[threadIdx.x + u];
+        out[q] = t;
+    }
+}
+
+// out = sum_k c_k v_k over up to 16 vectors (fixed order per element).
+struct LinCombArgs {
+    const double* v[16];
+    double c[16];
+};
+
+__global__ void k_lincomb(LinCombArgs L, int nv, long count, double* __restrict__ out)
+{
+    for (long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; i < count;
+         i += static_cast<long>(gridDim.x) * 256) {
+        double acc = 0.0;
+        for (int k = 0; k < nv; k++) acc += L.c[k] * L.v[k][i];
+        out[i] = acc;
     }
 }
 
@@ -208,21 +236,47 @@ void vec_obj_finish(const double* r, const double* a, const double* b, const dou
     KERNEL_CHECK();
 }
 
-void vec_lbfgs_apply(const double* S, const double* Y, const double* ys, int mm, int ncorr,
-                     int ptr, double theta, const double* v, double a, double* res, long count,
-                     hipStream_t s)
+// Chunks of at least 256 entries (one per thread), at most 512 workgroups.
+static int mdot_chunks(long count, long* chunk)
 {
-    if (mm < 1 || mm > 8 || ncorr < 0 || ncorr > mm) fail(E_INTERNAL, "L-BFGS memory %d/%d", ncorr, mm);
-    LbfgsArgs L{};
-    L.S = S;
-    L.Y = Y;
-    L.N = count;
-    L.mm = mm;
-    L.ncorr = ncorr;
-    L.ptr = ptr;
-    L.theta = theta;
-    for (int q = 0; q < mm; q++) L.ys[q] = ys[q];
-    hipLaunchKernelGGL(k_lbfgs_apply, dim3(1), dim3(1024), 0, s, L, v, a, res);
+    long g = (count + 255) / 256;
+    g = g < 1 ? 1 : (g > 512 ? 512 : g);
+    *chunk = (count + g - 1) / g;
+    return static_cast<int>(g);
+}
+
+void vec_mdots(const double* const* xs, const double* const* ys, int npairs, long count,
+               double* part, double* out, hipStream_t s)
+{
+    if (npairs < 1 || npairs > 32) fail(E_INTERNAL, "vec_mdots: %d pairs", npairs);
+    MDotArgs d{};
+    for (int q = 0; q < npairs; q++) {
+        d.x[q] = xs[q];
+        d.y[q] = ys[q];
+    }
+    long chunk = 0;
+    const int g = mdot_chunks(count, &chunk);
+    hipLaunchKernelGGL(k_mdots, dim3(g), dim3(256), 0, s, d, npairs, count, chunk, part);
+    hipLaunchKernelGGL(k_mdots_fold, dim3(1), dim3(256), 0, s, part, g, npairs, out);
+    KERNEL_CHECK();
+}
+
+long vec_mdots_scratch(long count)
+{
+    long chunk = 0;
+    return 32L * mdot_chunks(count, &chunk);
+}
+
+void vec_lincomb(const double* const* vs, const double* coef, int nv, long count, double* out,
+                 hipStream_t s)
+{
+    if (nv < 1 || nv > 16) fail(E_INTERNAL, "vec_lincomb: %d vectors", nv);
+    LinCombArgs L{};
+    for (int k = 0; k < nv; k++) {
+        L.v[k] = vs[k];
+        L.c[k] = coef[k];
+    }
+    hipLaunchKernelGGL(k_lincomb, dim3(grid_for(count)), dim3(256), 0, s, L, nv, count, out);
     KERNEL_CHECK();
 }
 
