@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_apd_plan(const TM* __restrict__ M, long
 {
     __shared__ double sm[4][256];
     __shared__ double ss[2][4];
-    __shared__ double tab[128];
+    __shared__ __align__(16) double tab[128];
     dev::exp2_tab_load(tab);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

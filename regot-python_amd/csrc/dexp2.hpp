@@ -11,7 +11,8 @@
 // term is below 3e-20); the result t + fma(t, q, t_lo) is scaled by ldexp,
 // which also rounds subnormal results.  On 2e7 random arguments 0.2% of the
 // results are one ulp from the correctly rounded value, none further.
-// x < -1100 gives 0, x > 1100 gives +inf, NaN propagates.  The tables and the
+// x < -1100 gives 0 and x > 1100 gives +inf (the argument is clamped there),
+// NaN propagates.  The tables and the
 // coefficients were computed with 60-digit decimal arithmetic and rounded to
 // nearest.  Callers stage the 128 table doubles in LDS (exp2_tab_load) and
 // pass that copy.
@@ -63,12 +64,13 @@ static __constant__ double kExp2TabLo[64] = {
     -0x1.e9c23179c2893p-54, 0x1.dc7f486a4b6b0p-54, 0x1.9d3e12dd8a18bp-54, 0x1.74853f3a5931ep-55,
 };
 
-// Copy both tables into a 128-double LDS array (all threads of the workgroup
-// call it; the caller synchronises before use).
+// Copy both tables into a 16-byte aligned 128-double LDS array, interleaved
+// (lead, trail) so one 16-byte LDS read fetches both (all threads of the
+// workgroup call it; the caller synchronises before use).
 __device__ __forceinline__ void exp2_tab_load(double* lds)
 {
     for (int i = threadIdx.x; i < 128; i += blockDim.x)
-        lds[i] = i < 64 ? kExp2Tab[i] : kExp2TabLo[i - 64];
+        lds[i] = (i & 1) ? kExp2TabLo[i >> 1] : kExp2Tab[i >> 1];
 }
 
 __device__ __forceinline__ double dexp2(double x, const double* tab)
@@ -76,14 +78,14 @@ __device__ __forceinline__ double dexp2(double x, const double* tab)
     const double c1 = 0x1.62e42fefa39efp-1, c2 = 0x1.ebfbdff82c58fp-3;
     const double c3 = 0x1.c6b08d704a0c0p-5, c4 = 0x1.3b2ab6fba4e77p-7;
     const double c5 = 0x1.5d87fe78a6731p-10, c6 = 0x1.430912f86c787p-13;
-    const double kx = rint(x * 64.0);
-    const int i = static_cast<int>(fmin(fmax(kx, -80000.0), 80000.0));
-    const double f = fma(kx, -0.015625, x);
+    // clamp: beyond +-1100 the result is already 0 or +inf (NaN restored last)
+    const double xc = fmin(fmax(x, -1100.0), 1100.0);
+    const double kx = rint(xc * 64.0);
+    const int i = static_cast<int>(kx);
+    const double f = fma(kx, -0.015625, xc);
     const double q = f * (c1 + f * (c2 + f * (c3 + f * (c4 + f * (c5 + f * c6)))));
-    const double t = tab[i & 63];
-    double r = ldexp(t + fma(t, q, tab[64 + (i & 63)]), i >> 6);
-    r = (x < -1100.0) ? 0.0 : r;
-    r = (x > 1100.0) ? static_cast<double>(INFINITY) : r;
+    const double2 t = *reinterpret_cast<const double2*>(tab + 2 * (i & 63));
+    const double r = ldexp(t.x + fma(t.x, q, t.y), i >> 6);
     return (x != x) ? x : r;
 }
 

@@ -10,9 +10,9 @@
 //
 // The row and column sums come out of the same read of M, so each T_ij is
 // formed once (the FP64 exponential is the expensive part of this pass).
-// Tiling as the APDAGD plan pass: a workgroup owns a 256-column strip of a
-// block of rows; each lane owns 4 adjacent columns and the 4 waves take the
-// block's rows round robin (coalesced row segments).  Row sums are reduced
+// Tiling: a workgroup owns a 256-column strip of a block of rows; each lane
+// owns 4 adjacent columns and the 4 waves take the block's rows round robin
+// (coalesced row segments).  Row sums are reduced
 // across the wave per row and written per strip, column sums accumulate per
 // lane down the block and are merged through LDS per block; two small
 // kernels fold strips, blocks and the scalars in a fixed order.
@@ -62,11 +62,61 @@ __device__ __forceinline__ void ldm4(const double* p, int c0, int m, double (&v)
     }
 }
 
-// grid (strips = ceil(m / 256), row blocks = ceil(n / rows)), 256 threads.
-// beta has m entries (beta_{m-1} = 0).  Sums of w_ij = x_i T_ij y_j (x, y
-// nullable: weight 1): rowpart[strip * n + i] = sum of row i over the strip;
-// colpart[block * m + j] (when non-null) = sum of column j over the block.
-template <typename TM>
+constexpr int kCPL = 4;              // columns per lane
+constexpr int kStrip = 64 * kCPL;     // columns per workgroup strip
+
+// The rows of one tile: FULL = every column of the strip is inside the
+// matrix (no per-element bounds), WT = weights x, y present.
+template <typename TM, bool FULL, bool WT>
+__device__ __forceinline__ void sums_rows_loop(const TM* __restrict__ M, long ldm, int n, int m,
+                                               const double* __restrict__ alpha, double k,
+                                               const double* __restrict__ xw, int c0, int r, int r1,
+                                               int strip, const double (&bk)[kCPL],
+                                               const double (&yq)[kCPL], const double* tab,
+                                               double (&cs)[kCPL], double* __restrict__ rowpart)
+{
+    const int lane = threadIdx.x & 63;
+    for (; r < r1; r += 4) {
+        double mv[kCPL];
+        if (FULL) {
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            if (sizeof(TM) == 4) {
+                const f4 u = __builtin_nontemporal_load(
+                    reinterpret_cast<const f4*>(reinterpret_cast<const float*>(M) + r * ldm + c0));
+                mv[0] = u.x;
+                mv[1] = u.y;
+                mv[2] = u.z;
+                mv[3] = u.w;
+            } else {
+                ldm4(M + static_cast<long>(r) * ldm, c0, m, mv);
+            }
+        } else {
+            ldm4(M + static_cast<long>(r) * ldm, c0, m, mv);
+        }
+        const double ak = alpha[r] * k;
+        double rs = 0.0;
+#pragma unroll
+        for (int q = 0; q < kCPL; q++) {
+            if (FULL || c0 + q < m) {
+                // (alpha_i + beta_j - M_ij) / reg in log2 units
+                double t = dev::dexp2(fma(-k, mv[q], ak + bk[q]), tab);
+                if (WT) t *= (xw ? xw[r] : 1.0) * yq[q];
+                rs += t;
+                cs[q] += t;
+            }
+        }
+        rs = wsum(rs);
+        if (lane == 0) rowpart[static_cast<long>(strip) * n + r] = rs;
+    }
+}
+
+// grid (strips = ceil(m / kStrip), row blocks = ceil(n / rows)), 256 threads.
+// beta has m entries (beta_{m-1} = 0).  Sums of w_ij = x_i T_ij y_j (WT:
+// weights given, else 1): rowpart[strip * n + i] = sum of row i over the
+// strip; colpart[block * m + j] (when non-null) = sum of column j over the
+// block.  Each lane owns kCPL adjacent columns.  (Loading the next row's M
+// ahead measured 5% slower: more registers, lower occupancy.)
+template <typename TM, bool WT>
 __global__ __launch_bounds__(256) void k_sums_tile(const TM* __restrict__ M, long ldm, int n, int m,
                                                    const double* __restrict__ alpha,
                                                    const double* __restrict__ beta, double k,
@@ -75,49 +125,38 @@ __global__ __launch_bounds__(256) void k_sums_tile(const TM* __restrict__ M, lon
                                                    double* __restrict__ rowpart,
                                                    double* __restrict__ colpart)
 {
-    __shared__ double sm[4][256];
-    __shared__ double tab[128];
+    __shared__ double sm[4][kStrip];
+    __shared__ __align__(16) double tab[128];
     dev::exp2_tab_load(tab);
     __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int strip = blockIdx.x, blk = blockIdx.y;
-    const int c0 = strip * 256 + lane * 4;
+    const int c0 = strip * kStrip + lane * kCPL;
     const int r0 = blk * rows;
     const int r1 = min(n, r0 + rows);
-    double bk[4], yq[4];
+    double bk[kCPL], yq[kCPL], cs[kCPL];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        bk[q] = (c0 + q < m) ? beta[c0 + q] : 0.0;
-        yq[q] = (yw && c0 + q < m) ? yw[c0 + q] : 1.0;
+    for (int q = 0; q < kCPL; q++) {
+        bk[q] = (c0 + q < m) ? beta[c0 + q] * k : 0.0;
+        yq[q] = (WT && yw && c0 + q < m) ? yw[c0 + q] : 1.0;
+        cs[q] = 0.0;
     }
-    double cs[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int r = r0 + wave; r < r1; r += 4) {
-        const double ai = alpha[r];
-        const double xi = xw ? xw[r] : 1.0;
-        double mv[4];
-        ldm4(M + static_cast<long>(r) * ldm, c0, m, mv);
-        double rs = 0.0;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            if (c0 + q < m) {
-                double t = dev::dexp2((ai + bk[q] - mv[q]) * k, tab);
-                if (xw) t *= xi;
-                if (yw) t *= yq[q];
-                rs += t;
-                cs[q] += t;
-            }
-        }
-        rs = wsum(rs);
-        if (lane == 0) rowpart[static_cast<long>(strip) * n + r] = rs;
-    }
+    if ((strip + 1) * kStrip <= m)   // workgroup-uniform
+        sums_rows_loop<TM, true, WT>(M, ldm, n, m, alpha, k, xw, c0, r0 + wave, r1, strip, bk,
+                                     yq, tab, cs, rowpart);
+    else
+        sums_rows_loop<TM, false, WT>(M, ldm, n, m, alpha, k, xw, c0, r0 + wave, r1, strip, bk,
+                                      yq, tab, cs, rowpart);
     if (!colpart) return;   // grid-uniform
 #pragma unroll
-    for (int q = 0; q < 4; q++) sm[wave][lane * 4 + q] = cs[q];
+    for (int q = 0; q < kCPL; q++) sm[wave][lane * kCPL + q] = cs[q];
     __syncthreads();
-    const int j = strip * 256 + threadIdx.x;
-    if (j < m)
-        colpart[static_cast<long>(blk) * m + j] =
-            ((sm[0][threadIdx.x] + sm[1][threadIdx.x]) + sm[2][threadIdx.x]) + sm[3][threadIdx.x];
+    for (int c = threadIdx.x; c < kStrip; c += 256) {
+        const int j = strip * kStrip + c;
+        if (j < m)
+            colpart[static_cast<long>(blk) * m + j] =
+                ((sm[0][c] + sm[1][c]) + sm[2][c]) + sm[3][c];
+    }
 }
 
 __device__ __forceinline__ double bsum(double v, double* sm4)
@@ -202,7 +241,7 @@ __global__ __launch_bounds__(256) void k_sums_final(const double* __restrict__ p
 __global__ __launch_bounds__(256) void k_exp2_check(const double* __restrict__ x,
                                                     double* __restrict__ y, long count)
 {
-    __shared__ double tab[128];
+    __shared__ __align__(16) double tab[128];
     dev::exp2_tab_load(tab);
     __syncthreads();
     for (long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; i < count;
@@ -220,19 +259,20 @@ void launch_exp2_check(const double* x, double* y, long count, hipStream_t s)
     KERNEL_CHECK();
 }
 
-// Rows per workgroup of the one-pass kernels: >= 2048 workgroups where the
-// problem allows, at most 256 rows so the column partials stay <= n/16 rows.
+// Rows per workgroup of the one-pass kernels: >= 1024 workgroups where the
+// problem allows, 32 to 256 rows (the column partials stay <= n/32 rows;
+// at 16384^2, 128- and 256-row blocks measured 7% faster than 64).
 int sums_rows(int n, int m)
 {
-    const long nstrip = ceil_div(m, 256);
-    long r = static_cast<long>(n) * nstrip / 2048;
-    r = std::max(16L, std::min(256L, r));
+    const long nstrip = ceil_div(m, kStrip);
+    long r = static_cast<long>(n) * nstrip / 1024;
+    r = std::max(32L, std::min(256L, r));
     return static_cast<int>((r + 3) / 4 * 4);
 }
 
 void sums_scratch_sizes(int n, int m, long* rowpart, long* colpart, long* part)
 {
-    const long nstrip = ceil_div(m, 256), nblk = ceil_div(n, sums_rows(n, m));
+    const long nstrip = ceil_div(m, kStrip), nblk = ceil_div(n, sums_rows(n, m));
     *rowpart = nstrip * n;
     *colpart = nblk * m;
     *part = 4L * ceil_div(std::max(n, m), 256);
@@ -244,17 +284,18 @@ void launch_objgrad_fused(int bulk, const void* M, long ldm, int n, int m, const
                           double* part, double* out, hipStream_t s)
 {
     if (n <= 0 || m <= 0) fail(E_INVALID, "objective pass on an empty problem");
-    const int nstrip = ceil_div(m, 256);
+    const int nstrip = ceil_div(m, kStrip);
     const int rows = sums_rows(n, m);
     const int nblk = ceil_div(n, rows);
     const dim3 grid(nstrip, nblk);
     const double k = 1.4426950408889634074 / reg;
     double* cp = with_grad ? colpart : nullptr;
     if (bulk == 0)
-        hipLaunchKernelGGL(k_sums_tile<float>, grid, dim3(256), 0, s, static_cast<const float*>(M),
-                           ldm, n, m, alpha, beta, k, rows, nullptr, nullptr, rowpart, cp);
+        hipLaunchKernelGGL((k_sums_tile<float, false>), grid, dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, alpha, beta, k, rows, nullptr,
+                           nullptr, rowpart, cp);
     else
-        hipLaunchKernelGGL(k_sums_tile<double>, grid, dim3(256), 0, s,
+        hipLaunchKernelGGL((k_sums_tile<double, false>), grid, dim3(256), 0, s,
                            static_cast<const double*>(M), ldm, n, m, alpha, beta, k, rows,
                            nullptr, nullptr, rowpart, cp);
     KERNEL_CHECK();
@@ -272,16 +313,28 @@ void launch_wsums(int bulk, const void* M, long ldm, int n, int m, const double*
 {
     if (n <= 0 || m <= 0) fail(E_INVALID, "weighted sums on an empty problem");
     const int rows = sums_rows(n, m);
-    const dim3 grid(ceil_div(m, 256), ceil_div(n, rows));
+    const dim3 grid(ceil_div(m, kStrip), ceil_div(n, rows));
     const double k = 1.4426950408889634074 / reg;
-    if (bulk == 0)
-        hipLaunchKernelGGL(k_sums_tile<float>, grid, dim3(256), 0, s, static_cast<const float*>(M),
-                           ldm, n, m, alpha, beta, k, rows, x, y, rowpart, colpart);
+    const bool wt = x || y;
+    if (bulk == 0 && wt)
+        hipLaunchKernelGGL((k_sums_tile<float, true>), grid, dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, alpha, beta, k, rows, x, y,
+                           rowpart, colpart);
+    else if (bulk == 0)
+        hipLaunchKernelGGL((k_sums_tile<float, false>), grid, dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, alpha, beta, k, rows, x, y,
+                           rowpart, colpart);
+    else if (wt)
+        hipLaunchKernelGGL((k_sums_tile<double, true>), grid, dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, n, m, alpha, beta, k, rows, x, y,
+                           rowpart, colpart);
     else
-        hipLaunchKernelGGL(k_sums_tile<double>, grid, dim3(256), 0, s,
+        hipLaunchKernelGGL((k_sums_tile<double, false>), grid, dim3(256), 0, s,
                            static_cast<const double*>(M), ldm, n, m, alpha, beta, k, rows, x, y,
                            rowpart, colpart);
     KERNEL_CHECK();
 }
+
+int sums_strip_width() { return kStrip; }
 
 }  // namespace regot

@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void k_round_tile(const TM* __restrict__ M, lo
                                                     double* __restrict__ P, long ldp,
                                                     double* __restrict__ scal)
 {
-    __shared__ double tab[128];
+    __shared__ __align__(16) double tab[128];
     __shared__ double ss[2][4];
     dev::exp2_tab_load(tab);
     __syncthreads();
@@ -166,12 +166,13 @@ static void round_plan(regot_problem* p, const double* d_gamma, double* P, long 
 {
     const int n = p->n, m = p->m;
     hipStream_t s = p->stream;
-    const int nstrip = ceil_div(m, 256);
+    const int nstrip = ceil_div(m, sums_strip_width());   // weighted-sum passes
     const int rows = sums_rows(n, m);
     const int nblk = ceil_div(n, rows);
+    const int nstrip4 = ceil_div(m, 256);                  // rounding pass (4 columns per lane)
     DBuf<double> alpha(n), beta(m), rowpart(static_cast<size_t>(nstrip) * n),
         colpart(static_cast<size_t>(nblk) * m), rsum(n), csum(m), x(n), y(m), ea(n), eb(m),
-        l1part(ceil_div(n, 256)), l1(1), scal(2L * nstrip * nblk), tot(2);
+        l1part(ceil_div(n, 256)), l1(1), scal(2L * nstrip4 * nblk), tot(2);
     launch_split_gamma(d_gamma, n, m, alpha, beta, s);
     const dim3 gn(ceil_div(n, 256)), gm(ceil_div(m, 256));
     // x = min(a ./ rowsum(T), 1)
@@ -201,7 +202,7 @@ static void round_plan(regot_problem* p, const double* d_gamma, double* P, long 
     KERNEL_CHECK();
     // P, <P, M>, entropy
     const double k = 1.4426950408889634074 / p->reg;
-    const dim3 grid(nstrip, nblk);
+    const dim3 grid(nstrip4, nblk);
     if (p->bulk == 0)
         hipLaunchKernelGGL(k_round_tile<float>, grid, dim3(256), 0, s,
                            static_cast<const float*>(p->dM), p->ldm, n, m, alpha.p, beta.p, k,
@@ -211,7 +212,7 @@ static void round_plan(regot_problem* p, const double* d_gamma, double* P, long 
                            static_cast<const double*>(p->dM), p->ldm, n, m, alpha.p, beta.p, k,
                            rows, x.p, y.p, ea.p, eb.p, l1.p, P, ldp, scal.p);
     KERNEL_CHECK();
-    hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, s, scal.p, nstrip * nblk, 2, tot.p);
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, s, scal.p, nstrip4 * nblk, 2, tot.p);
     KERNEL_CHECK();
     HIP_CHECK(hipMemcpyAsync(out, tot.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));

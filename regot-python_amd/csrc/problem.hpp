@@ -63,11 +63,13 @@ void launch_objgrad_fused(int bulk, const void* M, long ldm, int n, int m, const
                           bool with_grad, double* grad, double* rowpart, double* colpart,
                           double* part, double* out, hipStream_t s);
 
-// Row sums (per 256-column strip: rowpart[strip * n + i]) and, when colpart
-// is set, column sums (per row block: colpart[block * m + j]) of
-// w_ij = x_i T_ij y_j in FP64 (x, y nullable = 1); block sizes as the
-// one-pass objective kernel (sums_rows / sums_scratch_sizes).
+// Row sums (per column strip of sums_strip_width() columns:
+// rowpart[strip * n + i]) and, when colpart is set, column sums (per row
+// block: colpart[block * m + j]) of w_ij = x_i T_ij y_j in FP64 (x, y
+// nullable = 1); block sizes as the one-pass objective kernel (sums_rows /
+// sums_scratch_sizes).
 int sums_rows(int n, int m);
+int sums_strip_width();
 void launch_wsums(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
                   const double* beta, double reg, const double* x, const double* y,
                   double* rowpart, double* colpart, hipStream_t s);
