@@ -13,7 +13,8 @@ beside the same problems solved one after another by the single-problem path.
 `newton` times SSNS (same recipe, reg 0.01), sparse Newton and SPLR (mixture
 clouds in R^32, reg 0.001, density 1%) at the same n = m, to the same tol;
 `first_order` times Newton with the full Hessian and L-BFGS on the headline
-problem, and APDAGD on the same recipe at n = m = 4096 to 1e-5.
+problem, and APDAGD on the same recipe at n = m = 4096 to 1e-5; `c2` times
+sinkhorn_bcd and sinkhorn_ssns at n = m = 8192 with reg 0.1 and 0.01.
 `differentiable` reports the differentiable loss between two BF16 clouds of
 4096 points in R^64 (reg 0.05, tol 1e-6): matrix-core cost kernel, forward
 (cost + solve + implicit adjoint) and backward, beside the plain solve.  `roofline` is the dominant streaming kernel's achieved HBM rate from
@@ -248,7 +249,8 @@ def c3_problem(n, seed=0):
 def bench_newton(n, tol):
     """The Newton-type drivers at n = m: SSNS on the BCD workload's recipe
     (reg 0.01), sparse Newton and SPLR on the mixture recipe (reg 0.001,
-    density 1%); one warmup, one timed solve each, with per-phase times."""
+    density 1% and 5%); one warmup, one timed solve each, with per-phase
+    times."""
     import regot
     from regot import _lib
     out = {}
@@ -257,7 +259,9 @@ def bench_newton(n, tol):
     del M32
     M3, a3, b3 = c3_problem(n)
     cases += [("sparse_newton", M3, a3, b3, 0.001, _lib.SOLVER_SPARSE_NEWTON, {"density": 0.01}),
-              ("splr", M3, a3, b3, 0.001, _lib.SOLVER_SPLR, {"density": 0.01})]
+              ("splr", M3, a3, b3, 0.001, _lib.SOLVER_SPLR, {"density": 0.01}),
+              ("sparse_newton_d5", M3, a3, b3, 0.001, _lib.SOLVER_SPARSE_NEWTON, {"density": 0.05}),
+              ("splr_d5", M3, a3, b3, 0.001, _lib.SOLVER_SPLR, {"density": 0.05})]
     for name, M, a_, b_, reg, sid, kw in cases:
         with regot.Problem(M, a_, b_, reg) as P:
             P.solve(sid, tol=tol, max_iter=1000, compute_plan=False, **kw)
@@ -272,6 +276,37 @@ def bench_newton(n, tol):
         out[name].update(hbm_rate(r.stats["alg_bytes"], dt))
         print("bench: %s %.3f s, %d iterations" % (name, dt, r.niter), file=sys.stderr,
               flush=True)
+    return out
+
+
+def bench_c2(tol):
+    """sinkhorn_bcd and sinkhorn_ssns at n = m = 8192 (X, Y ~ N(0, I_16),
+    M = sqeuclid / max FP32, a, b = 0.9 Dirichlet + 0.1/n), reg 0.1 and 0.01;
+    one warmup and one timed solve each."""
+    import regot
+    from regot import _lib
+    n = 8192
+    M, a, b = make_problem(n, n, 16, 0)
+    out = {}
+    for reg in (0.1, 0.01):
+        with regot.Problem(M, a, b, reg) as P:
+            for name, sid, mi in (("bcd", _lib.SOLVER_BCD, 20000), ("ssns", _lib.SOLVER_SSNS, 500)):
+                P.solve(sid, tol=tol, max_iter=mi, compute_plan=False)
+                t0 = time.perf_counter()
+                r = P.solve(sid, tol=tol, max_iter=mi, compute_plan=False, verify=True)
+                dt = time.perf_counter() - t0
+                row = {"s": dt, "niter": r.niter, "mar_err": r.mar_errs[-1],
+                       "mar_err_fp64": r.mar_err_fp64}
+                if sid == _lib.SOLVER_BCD:
+                    rt = r.run_times
+                    it_ms = (rt[-1] - rt[0]) / max(r.niter, 1)
+                    row["device_ms_per_iteration"] = it_ms
+                    row["iteration_GBps"] = 2.0 * n * n * 4 / (it_ms * 1e-3) / 1e9
+                else:
+                    row.update(hbm_rate(r.stats["alg_bytes"], dt))
+                out["%s_reg%g" % (name, reg)] = row
+                print("bench: c2 %s reg %g %.3f s, %d iterations" % (name, reg, dt, r.niter),
+                      file=sys.stderr, flush=True)
     return out
 
 
@@ -328,9 +363,11 @@ def main():
     ap.add_argument("--diff-steps", type=int, default=2)
     ap.add_argument("--newton", type=int, default=1,
                     help="time SSNS, sparse Newton and SPLR at n = m = --n (0 = skip)")
+    ap.add_argument("--c2", type=int, default=1,
+                    help="time sinkhorn_bcd and sinkhorn_ssns at n = m = 8192, reg 0.1 and 0.01")
     ap.add_argument("--first-order", type=int, default=1,
                     help="time dense Newton, L-BFGS and APDAGD (0 = skip)")
-    ap.add_argument("--only", choices=["batched", "differentiable", "newton", "first_order"],
+    ap.add_argument("--only", choices=["batched", "differentiable", "newton", "first_order", "c2"],
                     default=None,
                     help="run one side workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
@@ -348,6 +385,9 @@ def main():
         return
     if args.only == "newton":
         print(json.dumps({"newton": bench_newton(args.n, args.tol)}))
+        return
+    if args.only == "c2":
+        print(json.dumps({"c2": bench_c2(args.tol)}))
         return
     if args.only == "first_order":
         M32, a, b = make_problem(args.n, args.n, 16, 0)
@@ -410,6 +450,10 @@ def main():
     if args.first_order:
         first_order = bench_first_order(P, n, args.tol)
 
+    c2 = None
+    if args.c2:
+        c2 = bench_c2(args.tol)
+
     newton = None
     if args.newton:
         del P
@@ -455,6 +499,7 @@ def main():
                      "row_pass_avg_ms": avg_row, "col_pass_avg_ms": avg_col,
                      "bytes_per_launch": bytes_per_pass},
         "cpu_baseline": cpu,
+        "c2": c2,
         "newton": newton,
         "first_order": first_order,
         "batched": batched,
