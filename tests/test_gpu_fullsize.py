@@ -46,8 +46,10 @@ def test_c2_bcd_and_ssns_8192(gpu_lib, reg):
     for r in (rb, rs):
         assert r.mar_errs[-1] < 1e-8 and len(r.obj_vals) == r.niter + 1
         assert all(e >= 1e-8 for e in r.mar_errs[:-1])
+        # the FP32-bulk in-loop ||g|| and its FP64 re-evaluation differ by the
+        # FP32 summation error (~1e-11 here)
         err = cpu_marginal_error(M, a, b, reg, r.dual)
-        assert err < 1e-8, err
+        assert err < 1.01e-8, err
     # the unique optimum: same objective, and duals that agree where the
     # marginals pin them
     assert abs(rb.obj_vals[-1] - rs.obj_vals[-1]) < 1e-10 * max(1.0, abs(rb.obj_vals[-1]))
