@@ -150,6 +150,11 @@ typedef struct {
     int cg_max_iter;     /* <= 0: driver default */
     const double* x0;    /* optional warm start (n+m-1), host; NULL = reference init */
     int bcd_batch;       /* BCD iterations enqueued per batch (two in flight); <= 0: 8 */
+    /* BCD iteration form.  0: two passes over M per iteration (row pass,
+     * column pass); nonzero (default 1): one pass per iteration when M is
+     * FP32, 1024 < m <= 16384, at most 1024 rows fall to each compute unit
+     * and min a min b >= 2^-60 sum a sum b; the two-pass form otherwise. */
+    int bcd_fused;
 } regot_opts;
 
 void regot_opts_default(regot_opts* o, int solver);
@@ -272,8 +277,9 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
                         double* w_out, int* cg_iters, double* cg_err);
 
 /* Device-side timing of the hot kernels (HIP events on the handle's stream).
- * kind: 0 = row pass, 1 = column pass (partial + combine).  Enabled by
- * regot_profile_enable(p, 1); totals accumulate until reset. */
+ * kind: 0 = row pass, 1 = column pass (partial + combine), 2 = single-read
+ * BCD pass, 3 = its column-sum combine.  Enabled by regot_profile_enable(p, 1);
+ * totals accumulate until reset. */
 int regot_profile_enable(regot_problem* p, int on);
 int regot_profile_get(regot_problem* p, int kind, long long* launches, double* total_ms);
 int regot_profile_reset(regot_problem* p);

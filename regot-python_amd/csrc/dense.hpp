@@ -97,6 +97,21 @@ void launch_row_pass_bcd(int bulk, const void* M, long ldm, int n, int m, const 
                          double k, const RowEpi& epi, const BcdState& st, hipStream_t s);
 int row_pass_blocks(int bulk, int n);
 
+// Single-read BCD iteration (FP32 bulk; bcd_fused.hip): per workgroup of whole
+// rows, alpha_out = optimal_alpha(beta) for its rows, the row partials of
+// BcdState (g^2, r, alpha'a at iterate k) and the FP64 column sums of
+// exp((alpha_out_i + beta_j - M_ij) / reg) over its rows into part[blk * m + j].
+// launch_col_combine_sums then finishes the iteration like the two-pass
+// combine (epi.colpot = beta).
+bool bcd_fused_shape_ok(int n, int m, int ncu);
+int bcd_fused_blocks(int n, int ncu);
+void launch_bcd_fused(const void* M, long ldm, int n, int m, const double* beta, double k,
+                      double reg, const double* loga, const double* a, const double* alpha,
+                      double* alpha_out, int nblk, double* part, double* rpart, double* bm,
+                      const int* ctl, hipStream_t s);
+void launch_col_combine_sums(const double* part, int nblk, int m, double k, const ColEpi& epi,
+                             const BcdState& st, hipStream_t s);
+
 void launch_fill(double* p, double v, long count, hipStream_t s);
 void launch_fill_int(int* p, int v, long count, hipStream_t s);
 void launch_stamp(long long* dst, hipStream_t s);

@@ -138,7 +138,10 @@ __device__ void bcd_decide(const BcdState& st, int kk, double* f_out, double* g_
 // result), otherwise beta <- beta_{k+1} and the tile sums of the new column
 // residual go to the other parity half of cpart.  Workgroup 0 records the
 // history entry and advances or stops the loop.
-template <bool BCD>
+// SUMS: the partials are plain FP64 column sums of exp((alpha_i + colpot_j -
+// M_ij) / reg) (the single-read BCD pass), so log2 sum_i 2^((alpha_i - M_ij) k)
+// = log2(sum) - colpot_j k.
+template <bool BCD, bool SUMS>
 __global__ __launch_bounds__(256) void k_col_combine(const double* __restrict__ part, int nblk,
                                                      int m, double k, ColEpi epi, BcdState st,
                                                      const int* __restrict__ ctl)
@@ -164,7 +167,28 @@ __global__ __launch_bounds__(256) void k_col_combine(const double* __restrict__ 
     const int j = blockIdx.x * kCombineCols + (threadIdx.x >> 2);
     double r2 = 0.0, bb = 0.0;
     if (!done) {
-        const double L = combine_partials(part, nblk, m, j, j < m, sub);
+        double L;
+        if (SUMS) {
+            // 16 loads in flight per thread, then added in order p = sub, sub + 4, ...
+            double cs = 0.0;
+            if (j < m) {
+                int p = sub;
+                for (; p + 60 < nblk; p += 64) {
+                    double x[16];
+#pragma unroll
+                    for (int u = 0; u < 16; u++) x[u] = part[static_cast<long>(p + 4 * u) * m + j];
+#pragma unroll
+                    for (int u = 0; u < 16; u++) cs += x[u];
+                }
+                for (; p < nblk; p += 4) cs += part[static_cast<long>(p) * m + j];
+            }
+            cs += __shfl_xor(cs, 1);
+            cs += __shfl_xor(cs, 2);
+            // colpot_j is read before sub 0 of the same column writes beta_out_j
+            L = (j < m && cs > 0.0) ? log2(cs) - epi.colpot[j] * k : -INFINITY;
+        } else {
+            L = combine_partials(part, nblk, m, j, j < m, sub);
+        }
         if (sub == 0 && j < m) {
             if (epi.lse2) epi.lse2[j] = L;
             double cs = 0.0;
@@ -366,11 +390,20 @@ void launch_col_pass(int bulk, const void* M, long ldm, int n, int m, const doub
                            k, rpb, part, ctl, nt);
     KERNEL_CHECK();
     if (st != nullptr)
-        hipLaunchKernelGGL(k_col_combine<true>, dim3(col_combine_blocks(m)), dim3(256), 0, s,
-                           part, nblk, m, k, epi, *st, ctl);
+        hipLaunchKernelGGL((k_col_combine<true, false>), dim3(col_combine_blocks(m)), dim3(256),
+                           0, s, part, nblk, m, k, epi, *st, ctl);
     else
-        hipLaunchKernelGGL(k_col_combine<false>, dim3(col_combine_blocks(m)), dim3(256), 0, s,
-                           part, nblk, m, k, epi, BcdState{}, ctl);
+        hipLaunchKernelGGL((k_col_combine<false, false>), dim3(col_combine_blocks(m)),
+                           dim3(256), 0, s, part, nblk, m, k, epi, BcdState{}, ctl);
+    KERNEL_CHECK();
+}
+
+void launch_col_combine_sums(const double* part, int nblk, int m, double k, const ColEpi& epi,
+                             const BcdState& st, hipStream_t s)
+{
+    if (epi.colpot == nullptr) fail(E_INTERNAL, "column-sum combine needs the column potential");
+    hipLaunchKernelGGL((k_col_combine<true, true>), dim3(col_combine_blocks(m)), dim3(256), 0, s,
+                       part, nblk, m, k, epi, st, st.ctl);
     KERNEL_CHECK();
 }
 

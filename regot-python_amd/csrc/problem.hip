@@ -1,5 +1,6 @@
 // problem.hip -- the EROT problem handle: M resident in HBM, the two streaming
 // passes, and the operator-level C-ABI entry points.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -43,8 +44,8 @@ regot_problem::~regot_problem()
     bcd.reset();
     for (auto& slot : prof_slot) {
         for (auto& e : slot.pending) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
+            (void)hipEventDestroy(e.start);
+            (void)hipEventDestroy(e.stop);
         }
         for (auto& e : slot.pool) {
             (void)hipEventDestroy(e.first);
@@ -149,6 +150,12 @@ void regot_problem::setup(int n_, int m_, const void* M, int M_dtype, int bulk_,
     sum_b1 = 0.0;
     for (int i = 0; i < n; i++) sum_a += a_h[i];
     for (int j = 0; j < m - 1; j++) sum_b1 += b_h[j];
+    sum_b = sum_b1 + b_h[m - 1];
+    min_a = a_h[0];
+    min_b = b_h[0];
+    for (int i = 1; i < n; i++) min_a = std::min(min_a, a_h[i]);
+    for (int j = 1; j < m; j++) min_b = std::min(min_b, b_h[j]);
+    HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
     HIP_CHECK(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device));
 
     a.alloc(n);
@@ -193,7 +200,7 @@ void regot_problem::prof_begin(int kind, hipEvent_t* stop_out)
         HIP_CHECK(hipEventCreate(&ev.second));
     }
     HIP_CHECK(hipEventRecord(ev.first, stream));
-    slot.pending.push_back(ev);
+    slot.pending.push_back({ev.first, ev.second, prof_tag});
     *stop_out = ev.second;
 }
 
@@ -201,21 +208,38 @@ void regot_problem::prof_end(int kind, hipEvent_t stop)
 {
     if (!prof || !stop) return;
     HIP_CHECK(hipEventRecord(stop, stream));
-    if (prof_slot[kind].pending.size() >= 256) prof_collect();
+    if (prof_slot[kind].pending.size() >= 2048) prof_collect();
 }
 
 void regot_problem::prof_collect()
 {
     for (auto& slot : prof_slot) {
         for (auto& e : slot.pending) {
-            HIP_CHECK(hipEventSynchronize(e.second));
+            HIP_CHECK(hipEventSynchronize(e.stop));
             float ms = 0.f;
-            HIP_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
-            slot.total_ms += ms;
-            slot.launches += 1;
-            slot.pool.push_back(e);
+            HIP_CHECK(hipEventElapsedTime(&ms, e.start, e.stop));
+            if (e.tag < 0) {
+                slot.total_ms += ms;
+                slot.launches += 1;
+            } else {
+                slot.tagged.emplace_back(e.tag, static_cast<double>(ms));
+            }
+            slot.pool.emplace_back(e.start, e.stop);
         }
         slot.pending.clear();
+    }
+}
+
+void regot_problem::prof_commit(long last)
+{
+    prof_collect();
+    for (auto& slot : prof_slot) {
+        for (const auto& t : slot.tagged) {
+            if (t.first > last) continue;
+            slot.total_ms += t.second;
+            slot.launches += 1;
+        }
+        slot.tagged.clear();
     }
 }
 
@@ -251,6 +275,36 @@ void regot_problem::row_pass_bcd(const double* beta_full, const RowEpi& epi, con
     e.reg = reg;
     launch_row_pass_bcd(bulk, dM, ldm, n, m, beta_full, k, e, st, stream);
     prof_end(0, stop);
+}
+
+// The single-read pass forms column sums of the plan at (alpha_{k+1}, beta_k)
+// from row-shifted FP32 terms; they keep full relative accuracy when every
+// column sum is far above the 2^-106 sum a that flushing can drop, which holds
+// when beta_k came out of an exact beta update and the marginals are bounded
+// away from zero as below (see bcd_fused.hip).
+bool regot_problem::bcd_fused_ok() const
+{
+    return bulk == REGOT_F32 && bcd_fused_shape_ok(n, m, ncu) && min_a > 0.0 && min_b > 0.0 &&
+           min_a * min_b >= 0x1p-60 * sum_a * sum_b;
+}
+
+void regot_problem::bcd_fused_iteration(const double* alpha_k, double* alpha_out, double* beta_v,
+                                        const ColEpi& epi, const BcdState& st, int nblk_f,
+                                        double* fpart)
+{
+    hipEvent_t stop;
+    prof_begin(2, &stop);
+    launch_bcd_fused(dM, ldm, n, m, beta_v, k, reg, loga, a, alpha_k, alpha_out, nblk_f, fpart,
+                     st.rpart, st.bm, st.ctl, stream);
+    prof_end(2, stop);
+    prof_begin(3, &stop);
+    ColEpi e = epi;
+    e.reg = reg;
+    e.logb = logb;
+    e.b = b;
+    e.colpot = beta_v;
+    launch_col_combine_sums(fpart, nblk_f, m, k, e, st, stream);
+    prof_end(3, stop);
 }
 
 // One-pass FP64 objective (and gradient): the precise mode of FP32 bulk and
@@ -487,7 +541,7 @@ int regot_profile_get(regot_problem* p, int kind, long long* launches, double* t
 {
     CAPI_BEGIN
     check_handle(p);
-    if (kind < 0 || kind > 1) fail(E_INVALID, "profile kind %d", kind);
+    if (kind < 0 || kind > 3) fail(E_INVALID, "profile kind %d", kind);
     p->prof_collect();
     if (launches) *launches = p->prof_slot[kind].launches;
     if (total_ms) *total_ms = p->prof_slot[kind].total_ms;

@@ -80,6 +80,7 @@ void launch_exp2_check(const double* x, double* y, long count, hipStream_t s);
 // Buffers of the on-device BCD loop, kept across solves on one handle.
 struct BcdWork {
     DBuf<double> ap0, ap1, beta, gcol, rpart, cpart, bm;
+    DBuf<double> fpart;         // column sums of the single-read pass (blocks x m)
     DBuf<int> ctl;
     MappedRing<double> hf, hg;
     MappedRing<long long> hts;
@@ -92,9 +93,17 @@ struct BcdWork {
     }
 };
 
+// Event pairs of one kind of pass.  A pair recorded with a tag >= 0 (the BCD
+// iteration it belongs to) is held in `tagged` until the driver knows which
+// iterations did work (prof_commit); untagged pairs count at once.
+struct ProfilePair {
+    hipEvent_t start, stop;
+    long tag;
+};
 struct ProfileSlot {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<ProfilePair> pending;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    std::vector<std::pair<long, double>> tagged;
     long long launches = 0;
     double total_ms = 0.0;
 };
@@ -115,6 +124,9 @@ struct regot_problem {
     double k = 1.0;             // log2(e) / reg
     double sum_a = 0.0;         // sum_i a_i
     double sum_b1 = 0.0;        // sum_{j < m-1} b_j
+    double sum_b = 0.0;         // sum_j b_j
+    double min_a = 0.0, min_b = 0.0;
+    int ncu = 0;                // compute units of the device
     int wall_khz = 0;           // device wall clock rate (timestamps)
     regot::DBuf<double> a, b, loga, logb;
 
@@ -134,9 +146,11 @@ struct regot_problem {
     // (objective and gradient evaluations of the Newton drivers, verification).
     bool precise = false;
 
-    // profiling of the hot passes: [0] row pass, [1] column pass
+    // profiling of the hot passes: [0] row pass, [1] column pass (partial +
+    // combine), [2] single-read BCD pass, [3] its column-sum combine
     bool prof = false;
-    regot::ProfileSlot prof_slot[2];
+    long prof_tag = -1;         // tag of the passes being enqueued (-1: none)
+    regot::ProfileSlot prof_slot[4];
 
     regot_problem() = default;
     regot_problem(const regot_problem&) = delete;
@@ -155,6 +169,13 @@ struct regot_problem {
     // Row pass with the BCD bookkeeping fused (see BcdState).
     void row_pass_bcd(const double* beta_full, const regot::RowEpi& epi,
                       const regot::BcdState& st);
+    // Single-read BCD iteration k (bcd_fused.hip): alpha_k -> alpha_out and the
+    // column sums into the BCD workspace, then the stop decision and beta_{k+1}
+    // (beta is both the input beta_k and the output).
+    bool bcd_fused_ok() const;
+    void bcd_fused_iteration(const double* alpha_k, double* alpha_out, double* beta,
+                             const regot::ColEpi& epi, const regot::BcdState& st, int nblk_f,
+                             double* fpart);
 
     // f(gamma) for device gamma (n+m-1); row pass only (FP32 bulk), or the
     // one-pass FP64 kernel (precise mode, FP64 bulk).
@@ -169,4 +190,7 @@ struct regot_problem {
     void prof_begin(int kind, hipEvent_t* stop_out);
     void prof_end(int kind, hipEvent_t stop);
     void prof_collect();
+    // Count the tagged pairs with tag <= last, drop the others (launches
+    // enqueued past a stop decision return at once and are not passes).
+    void prof_commit(long last);
 };

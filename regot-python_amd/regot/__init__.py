@@ -299,7 +299,8 @@ class Problem:
         check(lib().regot_profile_enable(self._h, int(bool(on))))
 
     def profile_stats(self, kind):
-        """(launches, total ms) of kind 0 = row pass, 1 = column pass."""
+        """(launches, total ms) of kind 0 = row pass, 1 = column pass, 2 = single-read
+        BCD pass, 3 = its column-sum combine."""
         cnt = _lib.ctypes.c_longlong()
         ms = _lib.ctypes.c_double()
         check(lib().regot_profile_get(self._h, int(kind), _lib.ctypes.byref(cnt),
@@ -328,6 +329,8 @@ class Problem:
             opts.cg_max_iter = int(kwargs["cg_max_iter"])
         if "bcd_batch" in kwargs:
             opts.bcd_batch = int(kwargs["bcd_batch"])
+        if "bcd_fused" in kwargs:
+            opts.bcd_fused = int(kwargs["bcd_fused"])
         x0 = kwargs.get("x0")
         if x0 is not None:
             x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)

@@ -39,6 +39,7 @@ class Opts(ctypes.Structure):
         ("cg_max_iter", ctypes.c_int),
         ("x0", _dp),
         ("bcd_batch", ctypes.c_int),
+        ("bcd_fused", ctypes.c_int),
     ]
 
 
