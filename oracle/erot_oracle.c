@@ -1380,11 +1380,17 @@ static double nw_quad_interp(double lo, double hi, double flo, double fhi, doubl
  * *dg are at xp; on exit x, *fx, grad, *dg, *step describe the accepted point
  * and *last_f is the last value evaluated.  Returns 0, -3 (direction is not a
  * descent direction), -4 (insufficient precision) or -5 (no decrease). */
-static int nw_line_search(const prob_t* P, const double* xp, const double* drt, double* step,
-                          double* fx, double* grad, double* dg, double* x, double* last_f,
-                          double* xlo, double* glo)
+/* f and grad of the EROT dual at x (the L-BFGS objective). */
+static double erot_objgrad(void* ctx, const double* x, double* grad)
 {
-    const long N = P->n + P->m - 1;
+    const prob_t* P = (const prob_t*)ctx;
+    return orc_dual_obj_grad(P->n, P->m, P->M, P->a, P->b, P->reg, x, grad, NULL, 1);
+}
+
+static int nw_line_search(long N, orc_objgrad_fn fn, void* ctx, const double* xp,
+                          const double* drt, double* step, double* fx, double* grad, double* dg,
+                          double* x, double* last_f, double* xlo, double* glo)
+{
     const int max_ls = 20;
     const double ftol = 1e-4, wolfe = 0.9, expansion = 2.0;
     const double fx_init = *fx, dg_init = *dg;
@@ -1396,7 +1402,7 @@ static int nw_line_search(const prob_t* P, const double* xp, const double* drt, 
     int iter = 0;
     for (;;) {
         for (long k = 0; k < N; k++) x[k] = xp[k] + (*step) * drt[k];
-        *fx = orc_dual_obj_grad(P->n, P->m, P->M, P->a, P->b, P->reg, x, grad, NULL, 1);
+        *fx = fn(ctx, x, grad);
         *last_f = *fx;
         *dg = dot(grad, drt, N);
         if (*fx - fx_init > (*step) * test_decr || (0.0 < step_lo && *fx >= fx_lo)) {
@@ -1423,7 +1429,7 @@ static int nw_line_search(const prob_t* P, const double* xp, const double* drt, 
     for (;;) {
         *step = nw_quad_interp(step_lo, step_hi, fx_lo, fx_hi, dg_lo);
         for (long k = 0; k < N; k++) x[k] = xp[k] + (*step) * drt[k];
-        *fx = orc_dual_obj_grad(P->n, P->m, P->M, P->a, P->b, P->reg, x, grad, NULL, 1);
+        *fx = fn(ctx, x, grad);
         *last_f = *fx;
         *dg = dot(grad, drt, N);
         if (*fx - fx_init > (*step) * test_decr || *fx >= fx_lo) {
@@ -1470,31 +1476,27 @@ int orc_nw_line_search(int n, int m, const double* M, const double* a, const dou
     *f = orc_dual_obj_grad(n, m, M, a, b, reg, gamma, grad, NULL, 1);
     double dg = dot(grad, direc, N);
     *step = init_step;
-    const int rc = nw_line_search(&P, gamma, direc, step, f, grad, &dg, x, last_f, xlo, glo);
+    const int rc = nw_line_search(N, erot_objgrad, &P, gamma, direc, step, f, grad, &dg, x, last_f,
+                                  xlo, glo);
     free(x);
     free(xlo);
     free(glo);
     return rc;
 }
 
-/* L-BFGS on the dual (memory 6, epsilon = tol, epsilon_rel = 0, no
- * past-delta test, max_iterations = max_iter with 0 meaning unlimited).
- * Restates sinkhorn_lbfgs_dual_internal, src/sinkhorn_lbfgs_dual.cpp:84-125,
- * with LBFGSSolver::minimize, src/LBFGSpp/LBFGS.h:64-189 (parameters
- * src/LBFGSpp/LBFGSpp/Param.h:154-171).  History entries come from the
- * iterate() callback (:56-76): obj_vals holds the LAST objective evaluated
- * (after a line search that fell back to an earlier trial this is not the
- * accepted point's value), mar_errs the accepted point's ||grad||.  Returns
- * niter as minimize() does (1 when the initial point already passes), or a
- * negative line-search status.  At most hist_cap history entries are kept. */
-int orc_sinkhorn_lbfgs_dual(int n, int m, const double* M, const double* a, const double* b,
-                            double reg, double tol, int max_iter, int verbose, int hist_cap,
-                            double* x, double* obj_vals, double* mar_errs, double* run_times,
-                            int* hist_len)
+/* LBFGSSolver::minimize, src/LBFGSpp/LBFGS.h:64-189 (memory 6, parameters
+ * src/LBFGSpp/LBFGSpp/Param.h:154-171, no past-delta test) on the objective
+ * fn over N variables from x: stops when ||grad|| <= eps or ||grad|| <=
+ * eps_rel ||x||, or after max_iter iterations (0 = unlimited).  it(ictx, x,
+ * last_f, ||grad||) is the iterate() callback, called with the initial point
+ * and after every line search; last_f is the most recent objective evaluated
+ * (after a search that fell back to an earlier trial, not the accepted
+ * point's value).  Returns minimize()'s niter (1 when the initial point
+ * passes) or a negative line-search status. */
+int orc_lbfgs_minimize(long N, orc_objgrad_fn fn, void* fctx, double eps_abs, double eps_rel,
+                       int max_iter, orc_iter_fn it, void* ictx, double* x)
 {
-    const long N = n + m - 1;
     const double eps = 2.220446049250313e-16;
-    prob_t P = {n, m, M, a, b, reg, NULL};
     double* grad = (double*)malloc(sizeof(double) * N);
     double* xp = (double*)malloc(sizeof(double) * N);
     double* gradp = (double*)malloc(sizeof(double) * N);
@@ -1505,27 +1507,12 @@ int orc_sinkhorn_lbfgs_dual(int n, int m, const double* M, const double* a, cons
     double* glo = (double*)malloc(sizeof(double) * N);
     lbfgs_mat B;
     lbfgs_reset(&B, N, 6);
-    initial_gamma(&P, x);
-    int h = 0, k = 0, rc = 0;
-    double t0 = now_ms();
-    double fx = orc_dual_obj_grad(n, m, M, a, b, reg, x, grad, NULL, 1);
+    int k = 0, rc = 0;
+    double fx = fn(fctx, x, grad);
     double last_f = fx;
     double gnorm = norm2(grad, N);
-#define LBFGS_ITERATE()                                                                        \
-    do {                                                                                       \
-        const double t1_ = now_ms();                                                           \
-        if (h < hist_cap) {                                                                    \
-            obj_vals[h] = last_f;                                                              \
-            mar_errs[h] = gnorm;                                                               \
-            run_times[h] = (h > 0 ? run_times[h - 1] : 0.0) + (t1_ - t0);                      \
-            h++;                                                                               \
-        }                                                                                      \
-        if (verbose >= 1)                                                                      \
-            printf("iter = %d, objval = %g, ||grad|| = %g\n", h - 1, last_f, gnorm);           \
-        t0 = now_ms();                                                                         \
-    } while (0)
-    LBFGS_ITERATE();
-    if (gnorm <= tol || gnorm <= 0.0 * norm2(x, N)) {
+    it(ictx, x, last_f, gnorm);
+    if (gnorm <= eps_abs || gnorm <= eps_rel * norm2(x, N)) {
         k = 1;
         goto done;
     }
@@ -1536,11 +1523,11 @@ int orc_sinkhorn_lbfgs_dual(int n, int m, const double* M, const double* a, cons
         memcpy(xp, x, sizeof(double) * N);
         memcpy(gradp, grad, sizeof(double) * N);
         double dg = dot(grad, drt, N);
-        rc = nw_line_search(&P, xp, drt, &step, &fx, grad, &dg, x, &last_f, xlo, glo);
+        rc = nw_line_search(N, fn, fctx, xp, drt, &step, &fx, grad, &dg, x, &last_f, xlo, glo);
         if (rc != 0) break;
         gnorm = norm2(grad, N);
-        LBFGS_ITERATE();
-        if (gnorm <= tol || gnorm <= 0.0 * norm2(x, N)) break;
+        it(ictx, x, last_f, gnorm);
+        if (gnorm <= eps_abs || gnorm <= eps_rel * norm2(x, N)) break;
         if (max_iter != 0 && k >= max_iter) break;
         for (long q = 0; q < N; q++) {
             vs[q] = x[q] - xp[q];
@@ -1551,7 +1538,6 @@ int orc_sinkhorn_lbfgs_dual(int n, int m, const double* M, const double* a, cons
         step = 1.0;
         k++;
     }
-#undef LBFGS_ITERATE
 done:
     lbfgs_free(&B);
     free(grad);
@@ -1562,8 +1548,48 @@ done:
     free(vy);
     free(xlo);
     free(glo);
-    *hist_len = h;
     return rc != 0 ? rc : k;
+}
+
+typedef struct {
+    int h, hist_cap, verbose;
+    double t0;
+    double *obj_vals, *mar_errs, *run_times;
+} erot_hist;
+
+/* iterate() of src/sinkhorn_lbfgs_dual.cpp:56-76 */
+static void erot_lbfgs_iter(void* ctx, const double* x, double last_f, double gnorm)
+{
+    (void)x;
+    erot_hist* H = (erot_hist*)ctx;
+    const double t1 = now_ms();
+    if (H->h < H->hist_cap) {
+        H->obj_vals[H->h] = last_f;
+        H->mar_errs[H->h] = gnorm;
+        H->run_times[H->h] = (H->h > 0 ? H->run_times[H->h - 1] : 0.0) + (t1 - H->t0);
+        H->h++;
+    }
+    if (H->verbose >= 1) printf("iter = %d, objval = %g, ||grad|| = %g\n", H->h - 1, last_f, gnorm);
+    H->t0 = now_ms();
+}
+
+/* L-BFGS on the dual (epsilon = tol, epsilon_rel = 0, max_iterations =
+ * max_iter).  Restates sinkhorn_lbfgs_dual_internal,
+ * src/sinkhorn_lbfgs_dual.cpp:84-125: obj_vals holds the last objective
+ * evaluated, mar_errs the accepted point's ||grad||.  At most hist_cap
+ * history entries are kept. */
+int orc_sinkhorn_lbfgs_dual(int n, int m, const double* M, const double* a, const double* b,
+                            double reg, double tol, int max_iter, int verbose, int hist_cap,
+                            double* x, double* obj_vals, double* mar_errs, double* run_times,
+                            int* hist_len)
+{
+    prob_t P = {n, m, M, a, b, reg, NULL};
+    initial_gamma(&P, x);
+    erot_hist H = {0, hist_cap, verbose, now_ms(), obj_vals, mar_errs, run_times};
+    const int rc = orc_lbfgs_minimize(n + m - 1, erot_objgrad, &P, tol, 0.0, max_iter,
+                                      erot_lbfgs_iter, &H, x);
+    *hist_len = H.h;
+    return rc;
 }
 
 /* Accelerated primal-dual adaptive gradient descent.  Restates

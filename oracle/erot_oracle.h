@@ -142,6 +142,30 @@ void orc_approx_proj(int n, int m, const double* plan, const double* a, const do
 double orc_primal_val(int n, int m, const double* M, const double* a, const double* b,
                       double reg, const double* gamma);
 
+/* ---- Generic L-BFGS (LBFGSSolver::minimize with the Nocedal-Wright search) ---- */
+typedef double (*orc_objgrad_fn)(void* ctx, const double* x, double* grad);
+typedef void (*orc_iter_fn)(void* ctx, const double* x, double last_f, double gnorm);
+int orc_lbfgs_minimize(long N, orc_objgrad_fn fn, void* fctx, double eps_abs, double eps_rel,
+                       int max_iter, orc_iter_fn it, void* ictx, double* x);
+
+/* ---- QROT (qrot_oracle.c): gamma = [alpha (n); beta (m)] ---- */
+double orc_qrot_dual_obj_grad(int n, int m, const double* M, const double* a, const double* b,
+                              double reg, double tau, const double* gamma, double* grad);
+double orc_qrot_pl_root(int len, const double* y, double c);
+void orc_qrot_optimal_beta(int n, int m, const double* M, const double* b, double reg,
+                           const double* alpha, double* beta);
+void orc_qrot_optimal_alpha(int n, int m, const double* M, const double* a, double reg,
+                            const double* beta, double* alpha);
+void orc_qrot_get_plan(int n, int m, const double* M, double reg, const double* gamma,
+                       double* plan);
+double orc_qrot_primal_val(int n, int m, const double* M, const double* a, const double* b,
+                           double reg, const double* gamma);
+/* solver: 0 = BCD, 1 = gradient descent, 2 = L-BFGS on the dual */
+int orc_qrot_solve(int solver, int n, int m, const double* M, const double* a, const double* b,
+                   double reg, double tau, double tol, int max_iter, int verbose, int hist_cap,
+                   const double* x0, double* x, double* obj_vals, double* prim_vals,
+                   double* mar_errs, double* run_times, int* hist_len);
+
 /* BCD iterations only (no convergence test), for timing a bounded CPU sample. */
 double orc_bcd_iterations(int n, int m, const double* M, const double* a, const double* b,
                           double reg, int iters, double* gamma_out);

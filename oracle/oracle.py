@@ -48,7 +48,8 @@ def lib():
             build()
         L = ctypes.CDLL(_SO)
         for name in ("orc_dual_obj", "orc_dual_obj_grad", "orc_compute_T", "orc_bcd_iterations",
-                     "orc_cubic_minimizer", "orc_step_selection", "orc_primal_val"):
+                     "orc_cubic_minimizer", "orc_step_selection", "orc_primal_val",
+                     "orc_qrot_dual_obj_grad", "orc_qrot_pl_root", "orc_qrot_primal_val"):
             getattr(L, name).restype = ctypes.c_double
         for name in ("orc_sparsify_mat3", "orc_sparsify_density"):
             getattr(L, name).restype = ctypes.c_longlong
@@ -396,3 +397,90 @@ def primal_val(M, a, b, reg, gamma):
     n, m = M.shape
     return lib().orc_primal_val(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg), _p(gamma))
 
+
+
+# --------------------------------------------------------------------------
+# QROT (quadratically regularised OT): gamma = [alpha (n); beta (m)]
+# --------------------------------------------------------------------------
+def qrot_dual_obj_grad(M, a, b, reg, gamma, tau=0.0):
+    """(f, grad): src/qrot_problem.cpp:37-96, 226-258"""
+    M, a, b, gamma = _d(M), _d(a), _d(b), _d(gamma)
+    n, m = M.shape
+    g = np.empty(n + m)
+    f = lib().orc_qrot_dual_obj_grad(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg),
+                                     ctypes.c_double(tau), _p(gamma), _p(g))
+    return f, g
+
+
+def qrot_pl_root(y, c):
+    """Root of sum_k (x - y_k)_+ = c (src/qrot_problem.cpp:365-394)"""
+    y = _d(y)
+    return lib().orc_qrot_pl_root(y.shape[0], _p(y), ctypes.c_double(c))
+
+
+def qrot_optimal_beta(M, b, reg, alpha):
+    """src/qrot_problem.cpp:397-406"""
+    M, b, alpha = _d(M), _d(b), _d(alpha)
+    n, m = M.shape
+    beta = np.empty(m)
+    lib().orc_qrot_optimal_beta(n, m, _p(M), _p(b), ctypes.c_double(reg), _p(alpha), _p(beta))
+    return beta
+
+
+def qrot_optimal_alpha(M, a, reg, beta):
+    """src/qrot_problem.cpp:409-418"""
+    M, a, beta = _d(M), _d(a), _d(beta)
+    n, m = M.shape
+    alpha = np.empty(n)
+    lib().orc_qrot_optimal_alpha(n, m, _p(M), _p(a), ctypes.c_double(reg), _p(beta), _p(alpha))
+    return alpha
+
+
+def qrot_get_plan(M, reg, gamma):
+    """(alpha_i + beta_j - M_ij)_+ / reg (src/qrot_result.h:35-49)"""
+    M, gamma = _d(M), _d(gamma)
+    n, m = M.shape
+    P = np.empty((n, m))
+    lib().orc_qrot_get_plan(n, m, _p(M), ctypes.c_double(reg), _p(gamma), _p(P))
+    return P
+
+
+def qrot_primal_val(M, a, b, reg, gamma):
+    """<P, M> + reg/2 ||P||^2 of the rounded plan (src/qrot_problem.cpp:10-26)"""
+    M, a, b, gamma = _d(M), _d(a), _d(b), _d(gamma)
+    n, m = M.shape
+    return lib().orc_qrot_primal_val(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg), _p(gamma))
+
+
+def _qrot_solve(solver, M, a, b, reg, tol, max_iter, verbose, tau=0.0, x0=None):
+    M, a, b = _d(M), _d(a), _d(b)
+    n, m = M.shape
+    cap = max_iter + 2
+    dual = np.empty(n + m)
+    h = [np.empty(cap) for _ in range(4)]
+    hl = ctypes.c_int(0)
+    x0p = _p(_d(x0)) if x0 is not None else None
+    niter = lib().orc_qrot_solve(int(solver), n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg),
+                                 ctypes.c_double(tau), ctypes.c_double(tol), int(max_iter),
+                                 int(verbose), cap, x0p, _p(dual), _p(h[0]), _p(h[1]), _p(h[2]),
+                                 _p(h[3]), ctypes.byref(hl))
+    _check(niter)
+    k = hl.value
+    return dict(niter=niter, dual=dual, plan=qrot_get_plan(M, reg, dual),
+                obj_vals=list(h[0][:k]), prim_vals=list(h[1][:k]), mar_errs=list(h[2][:k]),
+                run_times=list(h[3][:k]))
+
+
+def qrot_bcd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, x0=None):
+    """src/qrot_bcd.cpp:15-101"""
+    return _qrot_solve(0, M, a, b, reg, tol, max_iter, verbose, 0.0, x0)
+
+
+def qrot_gd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, tau=0.0, x0=None):
+    """src/qrot_gd.cpp:15-121"""
+    return _qrot_solve(1, M, a, b, reg, tol, max_iter, verbose, tau, x0)
+
+
+def qrot_lbfgs_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, tau=0.0, x0=None):
+    """src/qrot_lbfgs_dual.cpp:85-126 with LBFGSSolver::minimize"""
+    return _qrot_solve(2, M, a, b, reg, tol, max_iter, verbose, tau, x0)

@@ -169,16 +169,16 @@ __global__ __launch_bounds__(256) void k_col_combine(const double* __restrict__ 
     if (!done) {
         double L;
         if (SUMS) {
-            // 16 loads in flight per thread, then added in order p = sub, sub + 4, ...
+            // 32 loads in flight per thread, then added in order p = sub, sub + 4, ...
             double cs = 0.0;
             if (j < m) {
                 int p = sub;
-                for (; p + 60 < nblk; p += 64) {
-                    double x[16];
+                for (; p + 124 < nblk; p += 128) {
+                    double x[32];
 #pragma unroll
-                    for (int u = 0; u < 16; u++) x[u] = part[static_cast<long>(p + 4 * u) * m + j];
+                    for (int u = 0; u < 32; u++) x[u] = part[static_cast<long>(p + 4 * u) * m + j];
 #pragma unroll
-                    for (int u = 0; u < 16; u++) cs += x[u];
+                    for (int u = 0; u < 32; u++) cs += x[u];
                 }
                 for (; p < nblk; p += 4) cs += part[static_cast<long>(p) * m + j];
             }

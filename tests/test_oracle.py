@@ -469,3 +469,99 @@ def test_primal_value_meets_the_dual_at_the_optimum():
     g = r["dual"] + 0.05 * np.random.default_rng(3).standard_normal(len(r["dual"]))
     f = O.dual_obj(M, a, b, reg, g)
     assert O.primal_val(M, a, b, reg, g) >= -f - 1e-12
+
+
+# --------------------------------------------------------------------------
+# QROT (quadratically regularised OT)
+# --------------------------------------------------------------------------
+def test_qrot_obj_grad_brute_force_and_fd():
+    M, a, b = cloud_problem(6, 5, d=2, seed=31)
+    rng = np.random.default_rng(3)
+    gamma = 0.3 * rng.standard_normal(11)
+    for tau in (0.0, 0.7):
+        f, g = O.qrot_dual_obj_grad(M, a, b, 0.2, gamma, tau)
+        al, be = gamma[:6], gamma[6:]
+        D = np.maximum(al[:, None] + be[None, :] - M, 0.0)
+        d = al.sum() - be.sum()
+        f_ref = 0.5 * (D * D).sum() - 0.2 * (al @ a + be @ b) + 0.5 * tau * d * d
+        g_ref = np.concatenate([D.sum(1) - 0.2 * a + tau * d, D.sum(0) - 0.2 * b - tau * d])
+        assert abs(f - f_ref) < 1e-14 and np.max(np.abs(g - g_ref)) < 1e-14
+        h = 1e-6
+        for k in range(11):
+            e = np.zeros(11)
+            e[k] = h
+            fd = (O.qrot_dual_obj_grad(M, a, b, 0.2, gamma + e, tau)[0]
+                  - O.qrot_dual_obj_grad(M, a, b, 0.2, gamma - e, tau)[0]) / (2 * h)
+            assert abs(fd - g[k]) < 1e-7
+
+
+def test_qrot_piecewise_root():
+    rng = np.random.default_rng(4)
+    for trial in range(30):
+        y = rng.standard_normal(rng.integers(1, 40))
+        c = float(rng.exponential(0.5 if trial % 3 else 50.0))
+        x = O.qrot_pl_root(y, c)
+        assert abs(np.maximum(x - y, 0.0).sum() - c) < 1e-12 * max(1.0, c)
+        lo, hi = y.min(), y.max() + c          # independent bisection
+        for _ in range(200):
+            mid = 0.5 * (lo + hi)
+            lo, hi = (mid, hi) if np.maximum(mid - y, 0.0).sum() < c else (lo, mid)
+        assert abs(x - 0.5 * (lo + hi)) < 1e-12 * max(1.0, abs(x))
+    # every point active: x = (c + sum y) / len
+    y = np.array([0.0, 0.1, 0.2])
+    assert abs(O.qrot_pl_root(y, 10.0) - (10.0 + 0.3) / 3) < 1e-15
+
+
+def test_qrot_optimal_alpha_beta_zero_their_gradient_blocks():
+    M, a, b = cloud_problem(12, 9, d=3, seed=32)
+    reg = 0.05
+    alpha = 0.1 * np.random.default_rng(5).standard_normal(12)
+    beta = O.qrot_optimal_beta(M, b, reg, alpha)
+    _, g = O.qrot_dual_obj_grad(M, a, b, reg, np.concatenate([alpha, beta]))
+    assert np.max(np.abs(g[12:])) < 1e-15
+    alpha2 = O.qrot_optimal_alpha(M, a, reg, beta)
+    _, g = O.qrot_dual_obj_grad(M, a, b, reg, np.concatenate([alpha2, beta]))
+    assert np.max(np.abs(g[:12])) < 1e-15
+
+
+def test_qrot_drivers_reach_the_qp_optimum():
+    # The primal min <P, M> + reg/2 ||P||^2 over the transport polytope,
+    # solved independently by scipy's SLSQP on a 5 x 4 problem.
+    from scipy.optimize import minimize
+    M, a, b = cloud_problem(5, 4, d=2, seed=33)
+    reg = 0.1
+    n, m = M.shape
+    cons = [{"type": "eq", "fun": lambda p: p.reshape(n, m).sum(1) - a},
+            {"type": "eq", "fun": lambda p: p.reshape(n, m).sum(0)[:-1] - b[:-1]}]
+    qp = minimize(lambda p: p @ M.ravel() + 0.5 * reg * p @ p, np.outer(a, b).ravel(),
+                  jac=lambda p: M.ravel() + reg * p, bounds=[(0, None)] * (n * m),
+                  constraints=cons, method="SLSQP", options={"ftol": 1e-15, "maxiter": 1000})
+    P_qp = qp.x.reshape(n, m)
+    # (L-BFGS at tol 1e-10 ends in LBFGS++'s line-search error: f no longer
+    # decreases at FP64 resolution)
+    for solve, tol in ((O.qrot_bcd, 1e-10), (O.qrot_gd, 1e-10), (O.qrot_lbfgs_dual, 1e-9)):
+        r = solve(M, a, b, reg, tol=tol, max_iter=100000)
+        assert np.max(np.abs(r["plan"] - P_qp)) < 1e-6, solve.__name__
+        # strong duality: primal value = -f / reg at the optimum
+        assert abs(r["prim_vals"][-1] + r["obj_vals"][-1] / reg) < 1e-8
+        assert abs(r["prim_vals"][-1] - qp.fun) < 1e-8
+
+
+def test_qrot_driver_histories():
+    M, a, b = cloud_problem(20, 15, d=3, seed=34)
+    reg, tol = 0.05, 1e-7
+    for solve in (O.qrot_bcd, O.qrot_gd):
+        r = solve(M, a, b, reg, tol=tol, max_iter=20000)
+        assert len(r["obj_vals"]) == len(r["prim_vals"]) == r["niter"] + 1
+        g = np.array(r["mar_errs"]) * reg          # mar_errs = ||grad|| / reg
+        assert g[-1] < tol and np.all(g[:-1] >= tol)
+        _, grad = O.qrot_dual_obj_grad(M, a, b, reg, r["dual"])
+        assert abs(np.linalg.norm(grad) - g[-1]) < 1e-14
+        # weak duality along the way: primal value of the rounded plan >= -f / reg
+        assert np.all(np.array(r["prim_vals"]) >= -np.array(r["obj_vals"]) / reg - 1e-12)
+    assert O.qrot_bcd(M, a, b, reg, tol=1e9)["niter"] == 0
+    r = O.qrot_gd(M, a, b, reg, tol=0.0, max_iter=7)
+    assert r["niter"] == 7 and len(r["obj_vals"]) == 8
+    # BCD's objective decreases monotonically (exact block minimisation)
+    r = O.qrot_bcd(M, a, b, reg, tol=tol, max_iter=20000)
+    assert np.all(np.diff(r["obj_vals"]) <= 1e-15)
