@@ -155,6 +155,8 @@ typedef struct {
      * FP32, 1024 < m <= 16384, at most 1024 rows fall to each compute unit
      * and min a min b >= 2^-60 sum a sum b; the two-pass form otherwise. */
     int bcd_fused;
+    double tau;          /* QROT gradient descent / L-BFGS: extra term tau/2 (1'alpha - 1'beta)^2
+                            (QROTSolverOpts::tau, src/qrot_solvers.h:15-30); default 0 */
 } regot_opts;
 
 void regot_opts_default(regot_opts* o, int solver);
@@ -275,6 +277,36 @@ int regot_sqeuclid_bf16(const void* X, long long ldx, const void* Y, long long l
 int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double cg_tol,
                         int cg_max_iter, void* G, int G_dtype, long long ldg, double* loss_value,
                         double* w_out, int* cg_iters, double* cg_err);
+
+/*
+ * Quadratically regularised OT (QROT) on the same problem handle: the dual
+ * variable is gamma = [alpha (n); beta (m)] and
+ *   f = 0.5 ||(alpha (+) beta - M)_+||^2 - reg (a'alpha + b'beta) + tau/2 (1'alpha - 1'beta)^2.
+ * regot_qrot_solve replaces qrot_bcd_internal, qrot_gd_internal and
+ * qrot_lbfgs_dual_internal (src/qrot_bcd.cpp:15-101, src/qrot_gd.cpp:15-121,
+ * src/qrot_lbfgs_dual.cpp:85-126; bindings src/solvers.cpp:85-148).  Its result
+ * carries prim_vals (regot_result_prim_vals, one per history entry) and a
+ * dual of n + m entries (regot_result_dual_len); mar_errs are ||grad|| / reg;
+ * regot_result_plan gives (alpha (+) beta - M)_+ / reg.  opts may be NULL;
+ * opts->x0 (n + m) and opts->tau are used (BCD forces tau = 0).
+ */
+#define REGOT_QROT_BCD 0
+#define REGOT_QROT_GD 1
+#define REGOT_QROT_LBFGS_DUAL 2
+int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,
+                     const regot_opts* opts, regot_result** out);
+int regot_result_prim_vals(const regot_result* r, double* out);
+int regot_result_dual_len(const regot_result* r);
+/* QROT Problem::dual_obj_grad (src/qrot_problem.cpp:226-258): f and grad (n + m,
+ * may be NULL) at host gamma (n + m). */
+int regot_qrot_obj_grad(regot_problem* p, const double* gamma, double tau, double* f,
+                        double* grad);
+/* QROT Problem::optimal_beta / optimal_alpha (src/qrot_problem.cpp:397-418). */
+int regot_qrot_optimal_beta(regot_problem* p, const double* alpha, double* beta);
+int regot_qrot_optimal_alpha(regot_problem* p, const double* beta, double* alpha);
+/* QROT Problem::primal_val (src/qrot_problem.cpp:10-26): <P, M> + reg/2 ||P||^2 of
+ * P = approx_proj((alpha (+) beta - M)_+ / reg) at host gamma (n + m). */
+int regot_qrot_primal_val(regot_problem* p, const double* gamma, double* value);
 
 /* Device-side timing of the hot kernels (HIP events on the handle's stream).
  * kind: 0 = row pass, 1 = column pass (partial + combine), 2 = single-read

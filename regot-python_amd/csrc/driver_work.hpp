@@ -6,6 +6,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <limits>
 
 #include "drivers.hpp"
@@ -47,9 +48,13 @@ struct Work {
     // per-phase wall time (each phase ends in a host synchronisation)
     double phase_ms[3] = {0.0, 0.0, 0.0};
     long long phase_n[3] = {0, 0, 0};
+    // Objective other than the EROT dual (e.g. QROT): og(x, grad, &||grad||^2)
+    // returns f; grad and the norm pointer may be null.  Vectors then have
+    // N = nvar entries.
+    std::function<double(const double*, double*, double*)> og;
 
-    explicit Work(regot_problem* pp)
-        : p(pp), n(pp->n), m(pp->m), N(pp->n + pp->m - 1), s(pp->stream)
+    explicit Work(regot_problem* pp, long nvar = -1)
+        : p(pp), n(pp->n), m(pp->m), N(nvar > 0 ? nvar : pp->n + pp->m - 1), s(pp->stream)
     {
         for (DBuf<double>* b : {&gamma, &g, &direc, &step, &Hstep, &x, &xg, &rhs, &y, &sv, &g_pre,
                                 &gamma_pre, &lr0, &lr1})
@@ -101,7 +106,7 @@ struct Work {
     double obj(const double* at)
     {
         const auto t0 = Clock::now();
-        const double f = p->dev_obj(at);
+        const double f = og ? og(at, nullptr, nullptr) : p->dev_obj(at);
         alg_bytes += pass_bytes();
         phase_ms[0] += ms_since(t0);
         phase_n[0] += 1;
@@ -112,7 +117,7 @@ struct Work {
     {
         const auto t0 = Clock::now();
         double gn2 = 0.0;
-        const double f = p->dev_obj_grad(at, grad, &gn2);
+        const double f = og ? og(at, grad, &gn2) : p->dev_obj_grad(at, grad, &gn2);
         alg_bytes += pass_bytes();
         if (gnorm) *gnorm = std::sqrt(gn2);
         phase_ms[0] += ms_since(t0);
@@ -204,5 +209,10 @@ struct Timer {
     void tic() { t0 = Clock::now(); }
     double toc() { return ms_since(t0); }
 };
+
+// L-BFGS (first_order.cpp) on w's objective from w.gamma; see there.
+int lbfgs_minimize(Work& w, double eps_abs, double eps_rel, int max_iter,
+                   const std::function<void(const double*, double, double)>& iterate,
+                   const double** xout);
 
 }  // namespace regot

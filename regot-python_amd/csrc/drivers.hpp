@@ -23,6 +23,10 @@ struct regot_result {
     regot::DBuf<double> plan;
     long plan_ld = 0;
     int plan_device = 0;
+    // QROT results (regot_qrot_solve): dual = [alpha; beta] (n + m), the primal
+    // value of the rounded plan per history entry, plan (alpha (+) beta - M)+ / reg
+    bool qrot = false;
+    std::vector<double> prim_vals;
 };
 
 namespace regot {
@@ -40,5 +44,9 @@ void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
                       const regot_opts& o, regot_result* res);
 void solve_apdagd(regot_problem* p, double tol, int max_iter, int verbose, const regot_opts& o,
                   regot_result* res);
+
+// QROT drivers (qrot.cpp): solver 0 = BCD, 1 = gradient descent, 2 = L-BFGS.
+void solve_qrot(regot_problem* p, int solver, double tol, int max_iter, int verbose,
+                const regot_opts& o, regot_result* res);
 
 }  // namespace regot

@@ -150,28 +150,25 @@ void nw_line_search(Work& w, LbfgsState& st, double& step, double& fx, double& d
 
 }  // namespace
 
-// L-BFGS on the dual: memory 6, epsilon = tol, epsilon_rel = 0, no
-// past-delta test, max_iterations = max_iter (0 = unlimited).  Mirrors
-// sinkhorn_lbfgs_dual_internal, src/sinkhorn_lbfgs_dual.cpp:84-125, with
-// LBFGSSolver::minimize, src/LBFGSpp/LBFGS.h:64-189 (parameters
-// src/LBFGSpp/LBFGSpp/Param.h:154-171) and the iterate() callback
-// (src/sinkhorn_lbfgs_dual.cpp:56-76): obj_vals records the most recent
-// objective evaluation, mar_errs the accepted point's ||grad||.  niter is
-// minimize()'s return value (1 when the initial point already passes).  The
-// two-loop product runs on scalars: one many-workgroup pass forms every dot
-// product it needs (the candidate pair against itself, the kept pairs and the
-// gradient; the kept pairs against the gradient), Gram matrices of the kept
-// pairs carry the rest, and one pass forms the direction as a linear
-// combination of the gradient and the pairs (mathematically the recursion's
-// result; the rounding differs).
-void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
-                      const regot_opts& o, regot_result* res)
+// LBFGSSolver::minimize, src/LBFGSpp/LBFGS.h:64-189 (memory 6, parameters
+// src/LBFGSpp/LBFGSpp/Param.h:154-171, no past-delta test) on w's objective
+// from w.gamma: stops when ||grad|| <= eps_abs or ||grad|| <= eps_rel ||x||,
+// or after max_iter iterations (0 = unlimited).  iterate(x, last_f, ||grad||)
+// is the iterate() callback (the initial point, then every accepted point);
+// last_f is the most recent objective evaluated.  Returns minimize()'s niter
+// (1 when the initial point passes); *xout is the device vector holding the
+// final point.  The two-loop product runs on scalars: one many-workgroup pass
+// forms every dot product it needs (the candidate pair against itself, the
+// kept pairs and the gradient; the kept pairs against the gradient), Gram
+// matrices of the kept pairs carry the rest, and one pass forms the direction
+// as a linear combination of the gradient and the pairs (mathematically the
+// recursion's result; the rounding differs).
+int lbfgs_minimize(Work& w, double eps_abs, double eps_rel, int max_iter,
+                   const std::function<void(const double*, double, double)>& iterate,
+                   const double** xout)
 {
     const int mm = 6;
     const double eps = std::numeric_limits<double>::epsilon();
-    PreciseScope ps(p);
-    Work w(p);
-    w.init(o);
     const long N = w.N;
     DBuf<double> S(static_cast<size_t>(mm) * N), Y(static_cast<size_t>(mm) * N);
     DBuf<double> dpart(vec_mdots_scratch(N)), dout(32);
@@ -186,28 +183,18 @@ void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
     int ncorr = 0, ptr = mm;
     double theta = 1.0;
     LbfgsState st{w.gamma.p, w.g.p, w.lr0.p, w.lr1.p, w.gamma_pre.p, w.direc.p};
-    Timer timer;
-    timer.tic();
     double gnorm;
     double fx = w.obj_grad(st.x, st.grad, &gnorm);
     st.last_f = fx;
     st.gnorm = gnorm;
-    int k = 0;
-    auto iterate = [&]() {
-        const double dur = timer.toc();
-        res->obj_vals.push_back(st.last_f);
-        res->mar_errs.push_back(gnorm);
-        res->run_times.push_back((res->run_times.empty() ? 0.0 : res->run_times.back()) + dur);
-        if (verbose >= 1) {
-            printf("iter = %d, objval = %g, ||grad|| = %g\n",
-                   static_cast<int>(res->obj_vals.size()) - 1, st.last_f, gnorm);
-            fflush(stdout);
-        }
-        timer.tic();
+    auto converged = [&]() {
+        if (gnorm <= eps_abs) return true;
+        if (eps_rel > 0.0) return gnorm <= eps_rel * std::sqrt(w.dot(st.x, st.x));
+        return gnorm <= 0.0;
     };
-    iterate();
-    // epsilon_rel = 0: the relative test reads ||g|| <= 0
-    if (gnorm <= tol || gnorm <= 0.0) {
+    int k = 0;
+    iterate(st.x, st.last_f, gnorm);
+    if (converged()) {
         k = 1;
     } else {
         vec_axpby(st.drt, -1.0, st.grad, 0.0, nullptr, N, w.s);
@@ -219,8 +206,8 @@ void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
             vec_copy(w.g_pre, st.grad, N, w.s);
             nw_line_search(w, st, step, fx, dg);
             gnorm = st.gnorm;
-            iterate();
-            if (gnorm <= tol || gnorm <= 0.0) break;
+            iterate(st.x, st.last_f, gnorm);
+            if (converged()) break;
             if (max_iter != 0 && k >= max_iter) break;
             // Candidate pair s = x - xp, y = grad - gradp (temporaries: the
             // ring keeps its slots unless the pair passes s'y > eps y'y), and in
@@ -331,10 +318,40 @@ void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
             k++;
         }
     }
-    res->niter = k;
+    *xout = st.x;
+    return k;
+}
+
+// L-BFGS on the dual: epsilon = tol, epsilon_rel = 0, max_iterations =
+// max_iter.  Mirrors sinkhorn_lbfgs_dual_internal,
+// src/sinkhorn_lbfgs_dual.cpp:84-125, with the iterate() callback
+// (src/sinkhorn_lbfgs_dual.cpp:56-76): obj_vals records the most recent
+// objective evaluation, mar_errs the accepted point's ||grad||.
+void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
+                      const regot_opts& o, regot_result* res)
+{
+    PreciseScope ps(p);
+    Work w(p);
+    w.init(o);
+    Timer timer;
+    timer.tic();
+    auto iterate = [&](const double*, double last_f, double gnorm) {
+        const double dur = timer.toc();
+        res->obj_vals.push_back(last_f);
+        res->mar_errs.push_back(gnorm);
+        res->run_times.push_back((res->run_times.empty() ? 0.0 : res->run_times.back()) + dur);
+        if (verbose >= 1) {
+            printf("iter = %d, objval = %g, ||grad|| = %g\n",
+                   static_cast<int>(res->obj_vals.size()) - 1, last_f, gnorm);
+            fflush(stdout);
+        }
+        timer.tic();
+    };
+    const double* x = nullptr;
+    res->niter = lbfgs_minimize(w, tol, 0.0, max_iter, iterate, &x);
     w.report(res);
-    res->dual.resize(N);
-    HIP_CHECK(hipMemcpy(res->dual.data(), st.x, N * sizeof(double), hipMemcpyDeviceToHost));
+    res->dual.resize(w.N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), x, w.N * sizeof(double), hipMemcpyDeviceToHost));
 }
 
 // Accelerated primal-dual adaptive gradient descent.  Mirrors

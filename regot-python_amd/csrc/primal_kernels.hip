@@ -18,6 +18,7 @@
 
 #include "dexp2.hpp"
 #include "problem.hpp"
+#include "qrot.hpp"
 #include "regot_hip.h"
 
 namespace regot {
@@ -216,6 +217,55 @@ static void round_plan(regot_problem* p, const double* d_gamma, double* P, long 
     KERNEL_CHECK();
     HIP_CHECK(hipMemcpyAsync(out, tot.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// QROT: the same rounding of plan_ij = (alpha_i + beta_j - M_ij)_+ / reg
+// (Problem::primal_val, src/qrot_problem.cpp:10-26): three weighted-sum passes
+// and the rounding pass, <P, M> + reg/2 ||P||^2.
+double qrot_round_plan(regot_problem* p, const double* alpha, const double* beta, double* P,
+                       long ldp)
+{
+    const int n = p->n, m = p->m;
+    hipStream_t s = p->stream;
+    const int rows = qrot_rows(n, m);
+    const int nstrip = ceil_div(m, 256);
+    const int nblk = ceil_div(n, rows);
+    DBuf<double> rowpart(static_cast<size_t>(nstrip) * n), colpart(static_cast<size_t>(nblk) * m),
+        rsum(n), csum(m), x(n), y(m), ea(n), eb(m), l1part(ceil_div(n, 256)), l1(1),
+        scal(2L * nstrip * nblk), tot(2);
+    const dim3 gn(ceil_div(n, 256)), gm(ceil_div(m, 256));
+    const double reg = p->reg;
+    launch_qrot_tile(p->bulk, p->dM, p->ldm, n, m, alpha, beta, reg, nullptr, nullptr, rowpart,
+                     nullptr, nullptr, s);
+    hipLaunchKernelGGL(k_fold, gn, dim3(256), 0, s, rowpart.p, nstrip, static_cast<long>(n), n,
+                       rsum.p);
+    hipLaunchKernelGGL(k_scale, gn, dim3(256), 0, s, p->a.p, rsum.p, n, x.p);
+    KERNEL_CHECK();
+    launch_qrot_tile(p->bulk, p->dM, p->ldm, n, m, alpha, beta, reg, x, nullptr, nullptr,
+                     colpart, nullptr, s);
+    hipLaunchKernelGGL(k_fold, gm, dim3(256), 0, s, colpart.p, nblk, static_cast<long>(m), m,
+                       csum.p);
+    hipLaunchKernelGGL(k_scale, gm, dim3(256), 0, s, p->b.p, csum.p, m, y.p);
+    KERNEL_CHECK();
+    launch_qrot_tile(p->bulk, p->dM, p->ldm, n, m, alpha, beta, reg, x, y, rowpart, colpart,
+                     nullptr, s);
+    hipLaunchKernelGGL(k_fold, gn, dim3(256), 0, s, rowpart.p, nstrip, static_cast<long>(n), n,
+                       rsum.p);
+    hipLaunchKernelGGL(k_fold, gm, dim3(256), 0, s, colpart.p, nblk, static_cast<long>(m), m,
+                       csum.p);
+    hipLaunchKernelGGL(k_err, gn, dim3(256), 0, s, p->a.p, rsum.p, n, ea.p, l1part.p);
+    hipLaunchKernelGGL(k_err, gm, dim3(256), 0, s, p->b.p, csum.p, m, eb.p,
+                       static_cast<double*>(nullptr));
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, s, l1part.p, ceil_div(n, 256), 1, l1.p);
+    KERNEL_CHECK();
+    launch_qrot_round(p->bulk, p->dM, p->ldm, n, m, alpha, beta, reg, x, y, ea, eb, l1, P, ldp,
+                      scal, s);
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, s, scal.p, nstrip * nblk, 2, tot.p);
+    KERNEL_CHECK();
+    double out[2];
+    HIP_CHECK(hipMemcpyAsync(out, tot.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return out[0] + 0.5 * reg * out[1];
 }
 
 }  // namespace regot

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "problem.hpp"
+#include "qrot.hpp"
 #include "regot_hip.h"
 #include "sparse.hpp"
 
@@ -42,6 +43,7 @@ regot_problem::~regot_problem()
 {
     hess.reset();
     bcd.reset();
+    qrot.reset();
     for (auto& slot : prof_slot) {
         for (auto& e : slot.pending) {
             (void)hipEventDestroy(e.start);

@@ -96,6 +96,8 @@ struct BcdWork {
 // Event pairs of one kind of pass.  A pair recorded with a tag >= 0 (the BCD
 // iteration it belongs to) is held in `tagged` until the driver knows which
 // iterations did work (prof_commit); untagged pairs count at once.
+struct QrotScratch;
+
 struct ProfilePair {
     hipEvent_t start, stop;
     long tag;
@@ -141,6 +143,8 @@ struct regot_problem {
 
     // sparsified Hessian of the Newton-type drivers (sparse.hpp)
     std::unique_ptr<regot::SparseHessian> hess;
+    // scratch of the QROT passes (qrot.hpp)
+    std::unique_ptr<regot::QrotScratch> qrot;
 
     // FP32 bulk: run the general row / column passes with FP64 exp and sums
     // (objective and gradient evaluations of the Newton drivers, verification).

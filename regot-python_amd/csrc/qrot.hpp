@@ -1,0 +1,89 @@
+// qrot.hpp -- quadratically regularised OT (QROT) on the device: the dual
+// passes (qrot_kernels.hip), the rounded primal value (primal_kernels.hip)
+// and the host drivers (qrot.cpp).  The QROT dual variable is
+// gamma = [alpha (n); beta (m)]: no elimination of beta_m.
+#pragma once
+
+#include "problem.hpp"
+
+namespace regot {
+
+int qrot_rows(int n, int m);
+
+// Device scratch of the QROT passes and root iterations, sized for a shape;
+// owned by the problem handle.
+struct QrotScratch {
+    DBuf<double> rowpart, colpart, sqpart, rsum, csum, sq, out, spart;
+    DBuf<int> kpart, kprev, done, open;
+    HBuf<double> hout;
+    HBuf<int> hopen;
+    QrotScratch(int n, int m)
+    {
+        const int rows = qrot_rows(n, m);
+        const long nstrip = ceil_div(m, 256), nblk = ceil_div(n, rows);
+        rowpart.alloc(static_cast<size_t>(nstrip) * n);
+        colpart.alloc(static_cast<size_t>(nblk) * m);
+        sqpart.alloc(static_cast<size_t>(nstrip) * nblk);
+        const size_t a = static_cast<size_t>(nblk) * m, b = static_cast<size_t>(nstrip) * n;
+        spart.alloc(a > b ? a : b);
+        kpart.alloc(a > b ? a : b);
+        rsum.alloc(n);
+        csum.alloc(m);
+        sq.alloc(1);
+        out.alloc(2);
+        const int c = n > m ? n : m;
+        kprev.alloc(c);
+        done.alloc(c);
+        open.alloc(1);
+        hout.ensure(2);
+        hopen.ensure(1);
+    }
+};
+// Row sums (per 256-column strip: rowpart[strip * n + i]), column sums (per
+// row block: colpart[blk * m + j]) and sums of squares (sqpart[blk * strips +
+// strip]) of ((D / div) x_i) y_j, D = (alpha_i + beta_j - M_ij)_+; x, y and
+// any output may be null.
+void launch_qrot_tile(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                      const double* beta, double div, const double* x, const double* y,
+                      double* rowpart, double* colpart, double* sqpart, hipStream_t s);
+// P_ij = ((D_ij / reg) x_i) y_j + ea_i eb_j / l1; scal[2 wg] = sum P M,
+// scal[2 wg + 1] = sum P^2 per workgroup; P written when non-null.
+void launch_qrot_round(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                       const double* beta, double reg, const double* x, const double* y,
+                       const double* ea, const double* eb, const double* l1, double* P,
+                       long ldp, double* scal, hipStream_t s);
+void launch_qrot_pl_init(double* xc, int* kprev, int* done, int count, int* open, hipStream_t s);
+// One Newton step of the per-column (cols) or per-row roots; *open counts the
+// entries still moving.
+void launch_qrot_pl_step(int bulk, const void* M, long ldm, int n, int m, bool cols,
+                         const double* pot, const double* w, double reg, double* xc,
+                         int* kprev, int* done, double* spart, int* kpart, int* open,
+                         hipStream_t s);
+void launch_qrot_plan(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                      const double* beta, double reg, double* plan, long r0, int nr,
+                      hipStream_t s);
+void launch_qrot_finish(const double* alpha, const double* beta, const double* a,
+                        const double* b, int n, int m, double reg, double tau, const double* sq,
+                        const double* rsum, const double* csum, double* grad, double* out,
+                        hipStream_t s);
+
+void launch_qrot_fold(const double* part, int nparts, long stride, int count, double* out,
+                      hipStream_t s);
+void launch_qrot_sum1(const double* part, int count, double* out, hipStream_t s);
+
+// f (and grad, ||grad||^2 when grad is non-null) of the QROT dual at device
+// gamma (n + m), src/qrot_problem.cpp:226-258.
+double qrot_obj_grad(regot_problem* p, const double* gamma, double* grad, double tau,
+                     double* gnorm2);
+// optimal_beta (cols = true: out = beta (m) given alpha = pot) or
+// optimal_alpha (cols = false: out = alpha (n) given beta = pot), device.
+void qrot_optimal(regot_problem* p, bool cols, const double* pot, double* out);
+
+// <P, M> + reg/2 ||P||^2 of P = approx_proj((alpha (+) beta - M)_+ / reg)
+// for device alpha (n), beta (m); P (device, pitch ldp) written when non-null.
+double qrot_round_plan(regot_problem* p, const double* alpha, const double* beta, double* P,
+                       long ldp);
+// The QROT plan (alpha (+) beta - M)+ / reg of a host dual (n + m) into host memory.
+void qrot_plan_host(regot_problem* p, const double* dual, double* plan);
+
+}  // namespace regot

@@ -1,0 +1,512 @@
+// qrot_kernels.hip -- streaming passes of the quadratically regularised OT
+// (QROT) dual over the cost matrix M (FP32 or FP64 storage, FP64 arithmetic).
+//
+// With D_ij = (alpha_i + beta_j - M_ij)_+ (src/qrot_problem.cpp:37-96):
+//   k_qrot_tile       row sums, column sums and sum of squares of
+//                     W_ij = ((D_ij / div) x_i) y_j (x, y optional weights;
+//                     div = 1 for the dual, reg for the plan D / reg);
+//   k_qrot_round_tile the rounded plan P_ij = W_ij + ea_i eb_j / l1 of
+//                     approx_proj (src/approx_proj.h:6-26) with <P, M> and
+//                     ||P||^2, P optionally written;
+//   k_qrot_pl_tile    one Newton step of the piecewise-linear roots behind
+//                     optimal_beta / optimal_alpha (src/qrot_problem.cpp:365-418):
+//                     per column (or row) the sum and count of y = M_ij - alpha_i
+//                     (or M_ij - beta_j) below the current iterate;
+//   k_qrot_pl_update  the step x <- (c + S) / K per column or row.
+// Tiles are 256-column strips (64 lanes x 4 adjacent columns) over blocks of
+// `rows` rows, one row per wave at a time; every reduction has a fixed order.
+#include <cmath>
+
+#include "qrot.hpp"
+
+namespace regot {
+namespace {
+
+__device__ __forceinline__ double wsum64(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+template <typename TM>
+__device__ __forceinline__ void load4(const TM* row, int c0, int m, double* v)
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = (c0 + q < m) ? static_cast<double>(row[c0 + q]) : 0.0;
+}
+
+template <typename TM, bool WT>
+__global__ __launch_bounds__(256) void k_qrot_tile(const TM* __restrict__ M, long ldm, int n,
+                                                   int m, const double* __restrict__ alpha,
+                                                   const double* __restrict__ beta, double div,
+                                                   const double* __restrict__ x,
+                                                   const double* __restrict__ y, int rows,
+                                                   double* __restrict__ rowpart,
+                                                   double* __restrict__ colpart,
+                                                   double* __restrict__ sqpart)
+{
+    __shared__ double red[4][257];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int strip = blockIdx.x, blk = blockIdx.y;
+    const int c0 = strip * 256 + lane * 4;
+    const int r0 = blk * rows;
+    const int r1 = min(n, r0 + rows);
+    double bq[4], yq[4], cs[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        bq[q] = (c0 + q < m) ? beta[c0 + q] : 0.0;
+        yq[q] = (WT && y != nullptr && c0 + q < m) ? y[c0 + q] : 1.0;
+    }
+    double sq = 0.0;
+    for (int r = r0 + wave; r < r1; r += 4) {
+        const double ai = alpha[r];
+        const double xi = (WT && x != nullptr) ? x[r] : 1.0;
+        double mv[4];
+        load4(M + static_cast<long>(r) * ldm, c0, m, mv);
+        double rs = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (c0 + q < m) {
+                const double c = (ai + bq[q]) - mv[q];
+                double d = (c > 0.0 ? c : 0.0) / div;
+                if (WT) {
+                    if (x != nullptr) d = d * xi;
+                    if (y != nullptr) d = d * yq[q];
+                }
+                rs += d;
+                cs[q] += d;
+                sq += d * d;
+            }
+        }
+        rs = wsum64(rs);
+        if (lane == 0 && rowpart) rowpart[static_cast<long>(strip) * n + r] = rs;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) red[wave][lane * 4 + q] = cs[q];
+    sq = wsum64(sq);
+    if (lane == 0) red[wave][256] = sq;
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = c0 + q;
+            const double v = ((red[0][lane * 4 + q] + red[1][lane * 4 + q]) +
+                              red[2][lane * 4 + q]) + red[3][lane * 4 + q];
+            if (j < m && colpart) colpart[static_cast<long>(blk) * m + j] = v;
+        }
+        if (lane == 0 && sqpart)
+            sqpart[static_cast<long>(blk) * gridDim.x + strip] =
+                ((red[0][256] + red[1][256]) + red[2][256]) + red[3][256];
+    }
+}
+
+template <typename TM>
+__global__ __launch_bounds__(256) void k_qrot_round_tile(
+    const TM* __restrict__ M, long ldm, int n, int m, const double* __restrict__ alpha,
+    const double* __restrict__ beta, double reg, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ ea, const double* __restrict__ eb,
+    const double* __restrict__ l1p, int rows, double* __restrict__ P, long ldp,
+    double* __restrict__ scal)
+{
+    __shared__ double ss[2][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int strip = blockIdx.x, blk = blockIdx.y;
+    const int c0 = strip * 256 + lane * 4;
+    const int r0 = blk * rows;
+    const int r1 = min(n, r0 + rows);
+    const double l1 = *l1p;
+    double bq[4], yq[4], eq[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const bool in = c0 + q < m;
+        bq[q] = in ? beta[c0 + q] : 0.0;
+        yq[q] = in ? y[c0 + q] : 0.0;
+        eq[q] = in ? eb[c0 + q] : 0.0;
+    }
+    double mp = 0.0, sq = 0.0;
+    for (int r = r0 + wave; r < r1; r += 4) {
+        const double ai = alpha[r], xi = x[r], ei = ea[r];
+        double mv[4];
+        load4(M + static_cast<long>(r) * ldm, c0, m, mv);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = c0 + q;
+            if (j < m) {
+                const double c = (ai + bq[q]) - mv[q];
+                const double v = (((c > 0.0 ? c : 0.0) / reg) * xi) * yq[q] + ei * eq[q] / l1;
+                if (P) P[static_cast<long>(r) * ldp + j] = v;
+                mp += v * mv[q];
+                sq += v * v;
+            }
+        }
+    }
+    mp = wsum64(mp);
+    sq = wsum64(sq);
+    if (lane == 0) {
+        ss[0][wave] = mp;
+        ss[1][wave] = sq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long wg = static_cast<long>(blk) * gridDim.x + strip;
+        scal[2 * wg] = ((ss[0][0] + ss[0][1]) + ss[0][2]) + ss[0][3];
+        scal[2 * wg + 1] = ((ss[1][0] + ss[1][1]) + ss[1][2]) + ss[1][3];
+    }
+}
+
+// COLS: per column j, y_i = M_ij - pot_i over the block's rows; else per row
+// i, y_j = M_ij - pot_j over the strip's columns.  Sum and count of y < xc.
+template <typename TM, bool COLS>
+__global__ __launch_bounds__(256) void k_qrot_pl_tile(const TM* __restrict__ M, long ldm, int n,
+                                                      int m, const double* __restrict__ pot,
+                                                      const double* __restrict__ xc,
+                                                      const int* __restrict__ done, int rows,
+                                                      double* __restrict__ spart,
+                                                      int* __restrict__ kpart)
+{
+    __shared__ double rs_[4][256];
+    __shared__ int rk_[4][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int strip = blockIdx.x, blk = blockIdx.y;
+    const int c0 = strip * 256 + lane * 4;
+    const int r0 = blk * rows;
+    const int r1 = min(n, r0 + rows);
+    double pc[4], thr[4], S[4] = {0.0, 0.0, 0.0, 0.0};
+    int K[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const bool in = c0 + q < m;
+        pc[q] = (!COLS && in) ? pot[c0 + q] : 0.0;
+        thr[q] = (COLS && in) ? xc[c0 + q] : 0.0;
+    }
+    for (int r = r0 + wave; r < r1; r += 4) {
+        double mv[4];
+        load4(M + static_cast<long>(r) * ldm, c0, m, mv);
+        if (COLS) {
+            const double pr = pot[r];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double yv = mv[q] - pr;
+                if (c0 + q < m && yv < thr[q]) {
+                    S[q] += yv;
+                    K[q] += 1;
+                }
+            }
+        } else {
+            if (done[r]) continue;   // wave-uniform: r is the same on every lane
+            const double xr = xc[r];
+            double s = 0.0;
+            int kk = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double yv = mv[q] - pc[q];
+                if (c0 + q < m && yv < xr) {
+                    s += yv;
+                    kk += 1;
+                }
+            }
+            s = wsum64(s);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) kk += __shfl_xor(kk, o);
+            if (lane == 0) {
+                spart[static_cast<long>(strip) * n + r] = s;
+                kpart[static_cast<long>(strip) * n + r] = kk;
+            }
+        }
+    }
+    if (COLS) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            rs_[wave][lane * 4 + q] = S[q];
+            rk_[wave][lane * 4 + q] = K[q];
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int j = c0 + q;
+                if (j >= m) continue;
+                const int t = lane * 4 + q;
+                spart[static_cast<long>(blk) * m + j] =
+                    ((rs_[0][t] + rs_[1][t]) + rs_[2][t]) + rs_[3][t];
+                kpart[static_cast<long>(blk) * m + j] =
+                    ((rk_[0][t] + rk_[1][t]) + rk_[2][t]) + rk_[3][t];
+            }
+        }
+    }
+}
+
+// Per entry t < count (a column or a row): S = sum of the nparts partials
+// (fixed order), K likewise; x_t <- (reg w_t + S) / K.  The iteration stops
+// for t once K no longer shrinks (the active set is final; from the right
+// Newton's iterates only shrink it), counted in *open.
+__global__ void k_qrot_pl_update(const double* __restrict__ spart, const int* __restrict__ kpart,
+                                 int nparts, long stride, int count,
+                                 const double* __restrict__ w, double reg,
+                                 double* __restrict__ xc, int* __restrict__ kprev,
+                                 int* __restrict__ done, int* __restrict__ open)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= count || done[t]) return;
+    double S = 0.0;
+    int K = 0;
+    for (int s = 0; s < nparts; s++) {
+        S += spart[static_cast<long>(s) * stride + t];
+        K += kpart[static_cast<long>(s) * stride + t];
+    }
+    if (kprev[t] >= 0 && K >= kprev[t]) {
+        done[t] = 1;
+        return;
+    }
+    if (K > 0) xc[t] = (reg * w[t] + S) / K;
+    kprev[t] = K;
+    atomicAdd(open, 1);
+}
+
+__global__ void k_qrot_pl_init(double* xc, int* kprev, int* done, int count, int* open)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t == 0) open[0] = 0;
+    if (t >= count) return;
+    xc[t] = INFINITY;
+    kprev[t] = -1;
+    done[t] = 0;
+}
+
+__global__ void k_qrot_zero_int(int* p) { p[0] = 0; }
+
+// out[t] = sum over s of part[s * stride + t] (fixed order), t < count.
+__global__ void k_qrot_fold(const double* __restrict__ part, int nparts, long stride, int count,
+                            double* __restrict__ out)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= count) return;
+    double v = 0.0;
+    for (int s = 0; s < nparts; s++) v += part[static_cast<long>(s) * stride + t];
+    out[t] = v;
+}
+
+// One workgroup: out[0] = sum of part[0..count) (fixed order).
+__global__ __launch_bounds__(256) void k_qrot_sum1(const double* __restrict__ part, int count,
+                                                   double* __restrict__ out)
+{
+    __shared__ double sm[4];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < count; i += 256) v += part[i];
+    v = wsum64(v);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = ((sm[0] + sm[1]) + sm[2]) + sm[3];
+}
+
+template <typename TM>
+__global__ __launch_bounds__(256) void k_qrot_plan(const TM* __restrict__ M, long ldm, int m,
+                                                   const double* __restrict__ alpha,
+                                                   const double* __restrict__ beta, double reg,
+                                                   double* __restrict__ plan, long r0)
+{
+    const long i = r0 + blockIdx.y;
+    const double ai = alpha[i];
+    const TM* Mi = M + i * ldm;
+    double* Pi = plan + static_cast<long>(blockIdx.y) * m;
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) {
+        const double c = (ai + beta[j]) - static_cast<double>(Mi[j]);
+        Pi[j] = (c > 0.0 ? c : 0.0) / reg;
+    }
+}
+
+// One workgroup: with S_a = alpha'a, S_b = beta'b, d = 1'alpha - 1'beta,
+// f = 0.5 sq - reg (S_a + S_b) + tau/2 d^2 and (if grad) grad = [rsum - reg a
+// + tau d; csum - reg b - tau d]; out[0] = f, out[1] = ||grad||^2.
+__global__ __launch_bounds__(256) void k_qrot_finish(
+    const double* __restrict__ alpha, const double* __restrict__ beta,
+    const double* __restrict__ a, const double* __restrict__ b, int n, int m, double reg,
+    double tau, const double* __restrict__ sq, const double* __restrict__ rsum,
+    const double* __restrict__ csum, double* __restrict__ grad, double* __restrict__ out)
+{
+    __shared__ double sm[4][5];
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += 256) {
+        v[0] += alpha[i] * a[i];
+        v[2] += alpha[i];
+    }
+    for (int j = threadIdx.x; j < m; j += 256) {
+        v[1] += beta[j] * b[j];
+        v[3] += beta[j];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        v[q] = wsum64(v[q]);
+        if (lane == 0) sm[wave][q] = v[q];
+    }
+    __syncthreads();
+    double tot[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) tot[q] = ((sm[0][q] + sm[1][q]) + sm[2][q]) + sm[3][q];
+    const double d = tot[2] - tot[3];
+    const double td = tau > 0.0 ? tau * d : 0.0;
+    double g2 = 0.0;
+    if (grad) {
+        for (int i = threadIdx.x; i < n; i += 256) {
+            const double g = (rsum[i] - reg * a[i]) + td;
+            grad[i] = g;
+            g2 += g * g;
+        }
+        for (int j = threadIdx.x; j < m; j += 256) {
+            const double g = (csum[j] - reg * b[j]) - td;
+            grad[n + j] = g;
+            g2 += g * g;
+        }
+    }
+    g2 = wsum64(g2);
+    __syncthreads();
+    if (lane == 0) sm[wave][4] = g2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double f = 0.5 * sq[0] - reg * (tot[0] + tot[1]);
+        if (tau > 0.0) f += 0.5 * tau * d * d;
+        out[0] = f;
+        out[1] = ((sm[0][4] + sm[1][4]) + sm[2][4]) + sm[3][4];
+    }
+}
+
+}  // namespace
+
+int qrot_rows(int n, int m)
+{
+    // >= 1024 workgroups where the matrix allows, blocks of 16..256 rows
+    const int strips = ceil_div(m, 256);
+    int rows = 16;
+    while (rows < 256 && static_cast<long>(strips) * ceil_div(n, rows * 2) >= 1024) rows *= 2;
+    return rows;
+}
+
+void launch_qrot_tile(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                      const double* beta, double div, const double* x, const double* y,
+                      double* rowpart, double* colpart, double* sqpart, hipStream_t s)
+{
+    const int rows = qrot_rows(n, m);
+    const dim3 grid(ceil_div(m, 256), ceil_div(n, rows));
+    const bool wt = x != nullptr || y != nullptr;
+    if (bulk == 0) {
+        const float* Mf = static_cast<const float*>(M);
+        if (wt)
+            hipLaunchKernelGGL((k_qrot_tile<float, true>), grid, dim3(256), 0, s, Mf, ldm, n, m,
+                               alpha, beta, div, x, y, rows, rowpart, colpart, sqpart);
+        else
+            hipLaunchKernelGGL((k_qrot_tile<float, false>), grid, dim3(256), 0, s, Mf, ldm, n,
+                               m, alpha, beta, div, x, y, rows, rowpart, colpart, sqpart);
+    } else {
+        const double* Md = static_cast<const double*>(M);
+        if (wt)
+            hipLaunchKernelGGL((k_qrot_tile<double, true>), grid, dim3(256), 0, s, Md, ldm, n,
+                               m, alpha, beta, div, x, y, rows, rowpart, colpart, sqpart);
+        else
+            hipLaunchKernelGGL((k_qrot_tile<double, false>), grid, dim3(256), 0, s, Md, ldm, n,
+                               m, alpha, beta, div, x, y, rows, rowpart, colpart, sqpart);
+    }
+    KERNEL_CHECK();
+}
+
+void launch_qrot_round(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                       const double* beta, double reg, const double* x, const double* y,
+                       const double* ea, const double* eb, const double* l1, double* P,
+                       long ldp, double* scal, hipStream_t s)
+{
+    const int rows = qrot_rows(n, m);
+    const dim3 grid(ceil_div(m, 256), ceil_div(n, rows));
+    if (bulk == 0)
+        hipLaunchKernelGGL(k_qrot_round_tile<float>, grid, dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, alpha, beta, reg, x, y, ea,
+                           eb, l1, rows, P, ldp, scal);
+    else
+        hipLaunchKernelGGL(k_qrot_round_tile<double>, grid, dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, n, m, alpha, beta, reg, x, y, ea,
+                           eb, l1, rows, P, ldp, scal);
+    KERNEL_CHECK();
+}
+
+void launch_qrot_pl_init(double* xc, int* kprev, int* done, int count, int* open, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qrot_pl_init, dim3(ceil_div(count, 256)), dim3(256), 0, s, xc, kprev,
+                       done, count, open);
+    KERNEL_CHECK();
+}
+
+void launch_qrot_pl_step(int bulk, const void* M, long ldm, int n, int m, bool cols,
+                         const double* pot, const double* w, double reg, double* xc,
+                         int* kprev, int* done, double* spart, int* kpart, int* open,
+                         hipStream_t s)
+{
+    const int rows = qrot_rows(n, m);
+    const dim3 grid(ceil_div(m, 256), ceil_div(n, rows));
+    hipLaunchKernelGGL(k_qrot_zero_int, dim3(1), dim3(1), 0, s, open);
+    if (bulk == 0) {
+        const float* Mf = static_cast<const float*>(M);
+        if (cols)
+            hipLaunchKernelGGL((k_qrot_pl_tile<float, true>), grid, dim3(256), 0, s, Mf, ldm, n,
+                               m, pot, xc, done, rows, spart, kpart);
+        else
+            hipLaunchKernelGGL((k_qrot_pl_tile<float, false>), grid, dim3(256), 0, s, Mf, ldm,
+                               n, m, pot, xc, done, rows, spart, kpart);
+    } else {
+        const double* Md = static_cast<const double*>(M);
+        if (cols)
+            hipLaunchKernelGGL((k_qrot_pl_tile<double, true>), grid, dim3(256), 0, s, Md, ldm,
+                               n, m, pot, xc, done, rows, spart, kpart);
+        else
+            hipLaunchKernelGGL((k_qrot_pl_tile<double, false>), grid, dim3(256), 0, s, Md, ldm,
+                               n, m, pot, xc, done, rows, spart, kpart);
+    }
+    KERNEL_CHECK();
+    const int count = cols ? m : n;
+    const int nparts = cols ? ceil_div(n, rows) : ceil_div(m, 256);
+    hipLaunchKernelGGL(k_qrot_pl_update, dim3(ceil_div(count, 256)), dim3(256), 0, s, spart,
+                       kpart, nparts, static_cast<long>(count), count, w, reg, xc, kprev, done,
+                       open);
+    KERNEL_CHECK();
+}
+
+void launch_qrot_plan(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                      const double* beta, double reg, double* plan, long r0, int nr,
+                      hipStream_t s)
+{
+    (void)n;
+    if (nr <= 0 || m <= 0) return;
+    const int gx = ceil_div(m, 256) < 8 ? ceil_div(m, 256) : 8;
+    if (bulk == 0)
+        hipLaunchKernelGGL(k_qrot_plan<float>, dim3(gx, nr), dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, m, alpha, beta, reg, plan, r0);
+    else
+        hipLaunchKernelGGL(k_qrot_plan<double>, dim3(gx, nr), dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, m, alpha, beta, reg, plan, r0);
+    KERNEL_CHECK();
+}
+
+void launch_qrot_fold(const double* part, int nparts, long stride, int count, double* out,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qrot_fold, dim3(ceil_div(count, 256)), dim3(256), 0, s, part, nparts,
+                       stride, count, out);
+    KERNEL_CHECK();
+}
+
+void launch_qrot_sum1(const double* part, int count, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qrot_sum1, dim3(1), dim3(256), 0, s, part, count, out);
+    KERNEL_CHECK();
+}
+
+void launch_qrot_finish(const double* alpha, const double* beta, const double* a,
+                        const double* b, int n, int m, double reg, double tau, const double* sq,
+                        const double* rsum, const double* csum, double* grad, double* out,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qrot_finish, dim3(1), dim3(256), 0, s, alpha, beta, a, b, n, m, reg,
+                       tau, sq, rsum, csum, grad, out);
+    KERNEL_CHECK();
+}
+
+}  // namespace regot

@@ -9,6 +9,7 @@
 
 #include "drivers.hpp"
 #include "problem.hpp"
+#include "qrot.hpp"
 #include "regot_hip.h"
 #include "sparse.hpp"
 
@@ -238,6 +239,7 @@ void regot_opts_default(regot_opts* o, int solver)
     o->x0 = nullptr;
     o->bcd_batch = 0;
     o->bcd_fused = 1;
+    o->tau = 0.0;
 }
 
 int regot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,
@@ -320,6 +322,13 @@ int regot_result_plan(regot_problem* p, const regot_result* r, double* plan)
     if (!r) return set_error(E_INVALID, "null result");
     if (!p || p->n != r->n || p->m != r->m)
         return set_error(E_INVALID, "problem and result sizes differ");
+    if (r->qrot) {
+        CAPI_BEGIN
+        if (!plan) fail(E_INVALID, "plan is null");
+        HIP_CHECK(hipSetDevice(p->device));
+        qrot_plan_host(p, r->dual.data(), plan);
+        CAPI_END
+    }
     if (!r->plan.p) return regot_get_plan(p, r->dual.data(), plan);
     // a solver that keeps its own primal plan (APDAGD)
     CAPI_BEGIN
@@ -336,6 +345,7 @@ int regot_result_verify(regot_problem* p, const regot_result* r, double* mar_err
     CAPI_BEGIN
     if (!p || !r) fail(E_INVALID, "null handle");
     if (p->n != r->n || p->m != r->m) fail(E_INVALID, "problem and result sizes differ");
+    if (r->qrot) fail(E_INVALID, "verify applies to EROT results");
     HIP_CHECK(hipSetDevice(p->device));
     const int n = p->n, m = p->m;
     hipStream_t s = p->stream;

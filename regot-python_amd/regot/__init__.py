@@ -28,7 +28,9 @@ from ._lib import RegotError, check, dptr, lib
 
 __all__ = [
     "sinkhorn_bcd", "sinkhorn_ssns", "sinkhorn_sparse_newton", "sinkhorn_splr",
-    "sinkhorn_result", "Problem", "RegotError",
+    "sinkhorn_newton", "sinkhorn_lbfgs_dual", "sinkhorn_apdagd", "sinkhorn_result",
+    "qrot_bcd", "qrot_gd", "qrot_lbfgs_dual", "qrot_result",
+    "Problem", "BatchProblem", "RegotError",
 ]
 
 __version__ = "0.1.0"
@@ -90,6 +92,42 @@ class sinkhorn_result:
 
     def __repr__(self):
         return "<regot.sinkhorn_result niter=%d>" % self._niter
+
+
+class qrot_result:
+    """Result object with the fields of the reference's ``qrot_result``
+    (src/solvers.cpp:531-538): niter, dual (n+m,), plan, obj_vals, prim_vals,
+    mar_errs, run_times; each read returns a fresh copy."""
+
+    __slots__ = ("_niter", "_dual", "_plan", "_obj_vals", "_prim_vals", "_mar_errs",
+                 "_run_times")
+
+    def __init__(self):
+        self._niter = 0
+        self._dual = np.zeros(0)
+        self._plan = np.zeros((0, 0))
+        self._obj_vals = []
+        self._prim_vals = []
+        self._mar_errs = []
+        self._run_times = []
+
+    niter = sinkhorn_result.niter
+    dual = sinkhorn_result.dual
+    plan = sinkhorn_result.plan
+    obj_vals = sinkhorn_result.obj_vals
+    mar_errs = sinkhorn_result.mar_errs
+    run_times = sinkhorn_result.run_times
+
+    @property
+    def prim_vals(self):
+        return list(self._prim_vals)
+
+    @prim_vals.setter
+    def prim_vals(self, v):
+        self._prim_vals = [float(x) for x in v]
+
+    def __repr__(self):
+        return "<regot.qrot_result niter=%d>" % self._niter
 
 
 def _as_bulk(M, bulk):
@@ -230,6 +268,80 @@ class Problem:
         v = _lib.ctypes.c_double()
         check(lib().regot_primal_val(self._h, dptr(g), _lib.ctypes.byref(v)))
         return v.value
+
+    # --- QROT ---------------------------------------------------------------
+    def qrot_obj_grad(self, gamma, tau=0.0):
+        """QROT dual (f, grad) at gamma = [alpha; beta] (n+m)
+        (src/qrot_problem.cpp:226-258)."""
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        if g.shape[0] != self.n + self.m:
+            raise ValueError("gamma must have n+m entries")
+        f = _lib.ctypes.c_double()
+        grad = np.empty(self.n + self.m)
+        check(lib().regot_qrot_obj_grad(self._h, dptr(g), float(tau), _lib.ctypes.byref(f),
+                                        dptr(grad)))
+        return f.value, grad
+
+    def qrot_optimal_beta(self, alpha):
+        """QROT optimal_beta (src/qrot_problem.cpp:397-406)."""
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        beta = np.empty(self.m)
+        check(lib().regot_qrot_optimal_beta(self._h, dptr(a), dptr(beta)))
+        return beta
+
+    def qrot_optimal_alpha(self, beta):
+        """QROT optimal_alpha (src/qrot_problem.cpp:409-418)."""
+        b = np.ascontiguousarray(beta, dtype=np.float64)
+        alpha = np.empty(self.n)
+        check(lib().regot_qrot_optimal_alpha(self._h, dptr(b), dptr(alpha)))
+        return alpha
+
+    def qrot_primal_val(self, gamma):
+        """QROT primal value of the rounded plan (src/qrot_problem.cpp:10-26)."""
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        v = _lib.ctypes.c_double()
+        check(lib().regot_qrot_primal_val(self._h, dptr(g), _lib.ctypes.byref(v)))
+        return v.value
+
+    def qrot_solve(self, solver, tol=1e-6, max_iter=1000, verbose=0, compute_plan=True,
+                   **kwargs):
+        """QROT solver (_lib.QROT_BCD / QROT_GD / QROT_LBFGS_DUAL) -> qrot_result.
+        Keywords: x0 (n+m warm start), tau (extra dual term; the reference's
+        Python bindings never set it, src/solvers.cpp:33-51)."""
+        opts = _lib.Opts()
+        lib().regot_opts_default(_lib.ctypes.byref(opts), 0)
+        if "tau" in kwargs:
+            opts.tau = float(kwargs["tau"])
+        x0 = kwargs.get("x0")
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)
+            if x0.shape[0] != self.n + self.m:
+                raise ValueError("x0 must have n+m = %d entries" % (self.n + self.m))
+            opts.x0 = dptr(x0)
+        r = _lib._vp()
+        check(lib().regot_qrot_solve(self._h, int(solver), float(tol), int(max_iter),
+                                     int(verbose), _lib.ctypes.byref(opts), _lib.ctypes.byref(r)))
+        try:
+            L = lib()
+            res = qrot_result()
+            hl = L.regot_result_hist_len(r)
+            dual = np.empty(L.regot_result_dual_len(r))
+            h = [np.empty(max(hl, 1)) for _ in range(4)]
+            check(L.regot_result_get(r, dptr(dual), dptr(h[0]), dptr(h[1]), dptr(h[2]), None))
+            check(L.regot_result_prim_vals(r, dptr(h[3])))
+            res._niter = int(L.regot_result_niter(r))
+            res._dual = dual
+            res._obj_vals = [float(x) for x in h[0][:hl]]
+            res._mar_errs = [float(x) for x in h[1][:hl]]
+            res._run_times = [float(x) for x in h[2][:hl]]
+            res._prim_vals = [float(x) for x in h[3][:hl]]
+            if compute_plan:
+                plan = np.empty((self.n, self.m))
+                check(L.regot_result_plan(self._h, r, dptr(plan)))
+                res._plan = plan
+        finally:
+            lib().regot_result_free(r)
+        return res
 
     # --- sparsified Hessian ------------------------------------------------
     def sparsified_hess(self, gamma, delta, density_hint=0.001):
@@ -561,3 +673,27 @@ def sinkhorn_apdagd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
     averaged primal plan (src/solvers.cpp:343-357, src/sinkhorn_apdagd.cpp:14-183)."""
     return _run(_lib.SOLVER_APDAGD, M, a, b, reg, tol, max_iter, verbose, kwargs)
 
+
+
+def _run_qrot(solver, M, a, b, reg, tol, max_iter, verbose, kwargs):
+    # the reference parses only method and shift for QROT (src/solvers.cpp:33-51);
+    # other keywords are ignored, like there
+    with Problem(M, a, b, reg, bulk=kwargs.get("bulk"), device=kwargs.get("device", 0)) as p:
+        return p.qrot_solve(solver, tol=tol, max_iter=max_iter, verbose=verbose,
+                            compute_plan=kwargs.get("compute_plan", True))
+
+
+def qrot_bcd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Block coordinate descent on the QROT dual (src/solvers.cpp:85-99,
+    src/qrot_bcd.cpp:15-101)."""
+    return _run_qrot(_lib.QROT_BCD, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def qrot_gd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Gradient descent on the QROT dual (src/solvers.cpp:101-115, src/qrot_gd.cpp:15-121)."""
+    return _run_qrot(_lib.QROT_GD, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def qrot_lbfgs_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """L-BFGS on the QROT dual (src/solvers.cpp:134-148, src/qrot_lbfgs_dual.cpp:85-126)."""
+    return _run_qrot(_lib.QROT_LBFGS_DUAL, M, a, b, reg, tol, max_iter, verbose, kwargs)

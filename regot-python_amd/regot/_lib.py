@@ -19,6 +19,9 @@ SOLVER_SPLR = 3
 SOLVER_NEWTON = 4
 SOLVER_LBFGS_DUAL = 5
 SOLVER_APDAGD = 6
+QROT_BCD = 0
+QROT_GD = 1
+QROT_LBFGS_DUAL = 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -40,6 +43,7 @@ class Opts(ctypes.Structure):
         ("x0", _dp),
         ("bcd_batch", ctypes.c_int),
         ("bcd_fused", ctypes.c_int),
+        ("tau", ctypes.c_double),
     ]
 
 
@@ -90,6 +94,14 @@ def lib():
     L.regot_get_plan.argtypes = [_vp, _dp, _dp]
     L.regot_round_plan.argtypes = [_vp, _dp, _dp]
     L.regot_primal_val.argtypes = [_vp, _dp, _dp]
+    L.regot_qrot_solve.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_int, ctypes.POINTER(Opts), ctypes.POINTER(_vp)]
+    L.regot_result_prim_vals.argtypes = [_vp, _dp]
+    L.regot_result_dual_len.argtypes = [_vp]
+    L.regot_qrot_obj_grad.argtypes = [_vp, _dp, ctypes.c_double, _dp, _dp]
+    L.regot_qrot_optimal_beta.argtypes = [_vp, _dp, _dp]
+    L.regot_qrot_optimal_alpha.argtypes = [_vp, _dp, _dp]
+    L.regot_qrot_primal_val.argtypes = [_vp, _dp, _dp]
     L.regot_opts_default.argtypes = [ctypes.POINTER(Opts), ctypes.c_int]
     L.regot_opts_default.restype = None
     L.regot_solve.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
