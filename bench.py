@@ -20,8 +20,12 @@ sinkhorn_bcd and sinkhorn_ssns at n = m = 8192 with reg 0.1 and 0.01.
 `differentiable` reports the differentiable loss between two BF16 clouds of
 4096 points in R^64 (reg 0.05, tol 1e-6): matrix-core cost kernel, forward
 (cost + solve + implicit adjoint) and backward, beside the plain solve.  `roofline` is the dominant streaming kernel's achieved HBM rate from
-HIP events on the library's stream (algorithmic bytes n*m*4 per launch).
-`cpu_baseline` times the FP64 CPU oracle (one thread) on a bounded sample of
+HIP events on the library's stream (algorithmic bytes n*m*4 per launch),
+recorded around every pass of `--steps` solves run right after the timed
+ones (events between kernels add a few us per pass, so `value` comes from
+solves without them; `config.ms_per_step_with_events` shows the difference).
+`qrot` times the QROT drivers (L-BFGS, BCD, gradient descent) at n = m =
+8192.  `cpu_baseline` times the FP64 CPU oracle (one thread) on a bounded sample of
 BCD iterations of the same problem and scales to the GPU's iteration count.
 """
 import argparse
@@ -325,6 +329,32 @@ def bench_c2(tol):
     return out
 
 
+def bench_qrot(n):
+    """QROT on the C2 recipe at n = m (FP32 M, reg 0.1): L-BFGS on the dual to
+    mar_err <= 1e-6 (at most 500 iterations), and 20 iterations each of BCD
+    (two root solves per iteration) and gradient descent; every iteration
+    includes the primal value of the rounded plan, as the reference's
+    histories do.  One warmup and one timed solve each."""
+    import regot
+    from regot import _lib
+    M, a, b = make_problem(n, n, 16, 0)
+    out = {"n": n, "reg": 0.1}
+    with regot.Problem(M, a, b, 0.1) as P:
+        for name, sid, tol, mi in (("lbfgs_dual", _lib.QROT_LBFGS_DUAL, 1e-6, 500),
+                                   ("bcd", _lib.QROT_BCD, 1e-6, 20),
+                                   ("gd", _lib.QROT_GD, 1e-6, 20)):
+            P.qrot_solve(sid, tol=tol, max_iter=mi, compute_plan=False)
+            t0 = time.perf_counter()
+            r = P.qrot_solve(sid, tol=tol, max_iter=mi, compute_plan=False)
+            dt = time.perf_counter() - t0
+            out[name] = {"s": dt, "niter": r.niter, "ms_per_iteration": dt * 1e3 / max(r.niter, 1),
+                         "mar_err": r.mar_errs[-1], "obj": r.obj_vals[-1],
+                         "prim": r.prim_vals[-1]}
+            print("bench: qrot %s %.3f s, %d iterations" % (name, dt, r.niter), file=sys.stderr,
+                  flush=True)
+    return out
+
+
 def bench_first_order(P, n, tol):
     """Newton with the full Hessian and L-BFGS on the BCD workload's problem
     handle (n = m, reg 0.01), APDAGD on the same recipe at n = m = 4096 to
@@ -382,7 +412,10 @@ def main():
                     help="time sinkhorn_bcd and sinkhorn_ssns at n = m = 8192, reg 0.1 and 0.01")
     ap.add_argument("--first-order", type=int, default=1,
                     help="time dense Newton, L-BFGS and APDAGD (0 = skip)")
-    ap.add_argument("--only", choices=["batched", "differentiable", "newton", "first_order", "c2"],
+    ap.add_argument("--qrot-n", type=int, default=8192,
+                    help="n = m of the QROT section (0 = skip)")
+    ap.add_argument("--only", choices=["batched", "differentiable", "newton", "first_order", "c2",
+                                       "qrot"],
                     default=None,
                     help="run one side workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
@@ -404,6 +437,9 @@ def main():
     if args.only == "c2":
         print(json.dumps({"c2": bench_c2(args.tol)}))
         return
+    if args.only == "qrot":
+        print(json.dumps({"qrot": bench_qrot(args.qrot_n)}))
+        return
     if args.only == "first_order":
         M32, a, b = make_problem(args.n, args.n, 16, 0)
         with regot.Problem(M32, a, b, args.reg) as P:
@@ -420,13 +456,14 @@ def main():
     P = regot.Problem(M32, a, b, args.reg)
     solve = lambda **kw: P.solve(_lib.SOLVER_BCD, tol=args.tol, max_iter=args.max_iter,
                                  compute_plan=False, **kw)
-    def timed(steps, **kw):
-        """`steps` solves with HIP events recorded on the library's stream
-        around every pass (launches past a stop decision are not counted):
+    def timed(steps, events, **kw):
+        """`steps` solves; with `events`, HIP events are recorded on the
+        library's stream around every pass (the iterations are enqueued as
+        without them; launches past a stop decision are not counted):
         (seconds per solve, last result, {kind: (launches, avg ms)}) for 0 row
         pass, 1 column pass (partial + combine), 2 single-read pass, 3 its
         column-sum combine."""
-        P.profile(True)
+        P.profile(events)
         P.profile_reset()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -442,7 +479,11 @@ def main():
     for _ in range(args.warmup):
         solve()
     print("bench: warmup done", file=sys.stderr, flush=True)
-    sec, res, st = timed(args.steps)
+    # `value`: the timed solves without events (an event pair between kernels
+    # adds a few us per pass); the kernel averages: the same number of solves
+    # with events, right after
+    sec, res, _ = timed(args.steps, False)
+    sec_ev, _, st = timed(args.steps, True)
     ms_per_step = sec * 1e3
     niter = res.niter
     print("bench: %d steps, %.2f ms/solve, niter %d" % (args.steps, ms_per_step, niter),
@@ -468,7 +509,8 @@ def main():
     two_pass = None
     if fused:
         solve(bcd_fused=0)
-        t2, r2, st2 = timed(args.steps, bcd_fused=0)
+        t2, r2, _ = timed(args.steps, False, bcd_fused=0)
+        _, _, st2 = timed(args.steps, True, bcd_fused=0)
         r2p = solve(verify=True, bcd_fused=0)
         rt2 = r2.run_times
         two_pass = {"s": t2, "niter": r2.niter, "mar_err_fp64": r2p.mar_err_fp64,
@@ -509,6 +551,11 @@ def main():
               flush=True)
         batched = bench_batched(args.batch_problems, args.batch_steps, 1e-6)
 
+    qrot = None
+    if args.qrot_n > 0:
+        print("bench: qrot workload, n=m=%d" % args.qrot_n, file=sys.stderr, flush=True)
+        qrot = bench_qrot(args.qrot_n)
+
     diff = None
     if args.diff_n > 0:
         print("bench: differentiable workload, n=m=%d" % args.diff_n, file=sys.stderr,
@@ -532,6 +579,7 @@ def main():
                                "a,b=0.9 Dirichlet+0.1/n" % (n, args.reg, args.tol),
                    "n": n, "m": m, "reg": args.reg, "tol": args.tol, "niter": niter,
                    "ms_per_iteration": iter_ms,
+                   "ms_per_step_with_events": sec_ev * 1e3,
                    "device_ms_per_iteration": dev_iter_ms,
                    "iteration_form": "single-read" if fused else "two-pass",
                    "iteration_GBps": passes_per_iter * bytes_per_pass / (dev_iter_ms * 1e-3)
@@ -552,6 +600,7 @@ def main():
         "first_order": first_order,
         "batched": batched,
         "differentiable": diff,
+        "qrot": qrot,
     }
     print(json.dumps(out))
 

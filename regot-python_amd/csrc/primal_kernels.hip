@@ -230,9 +230,11 @@ double qrot_round_plan(regot_problem* p, const double* alpha, const double* beta
     const int rows = qrot_rows(n, m);
     const int nstrip = ceil_div(m, 256);
     const int nblk = ceil_div(n, rows);
-    DBuf<double> rowpart(static_cast<size_t>(nstrip) * n), colpart(static_cast<size_t>(nblk) * m),
-        rsum(n), csum(m), x(n), y(m), ea(n), eb(m), l1part(ceil_div(n, 256)), l1(1),
-        scal(2L * nstrip * nblk), tot(2);
+    if (!p->qrot) p->qrot = std::make_unique<QrotScratch>(n, m);
+    QrotScratch& w = *p->qrot;
+    DBuf<double>&rowpart = w.rowpart, &colpart = w.colpart, &rsum = w.rsum, &csum = w.csum,
+    &x = w.x, &y = w.y, &ea = w.ea, &eb = w.eb, &l1part = w.l1part, &l1 = w.l1, &scal = w.scal,
+    &tot = w.tot;
     const dim3 gn(ceil_div(n, 256)), gm(ceil_div(m, 256));
     const double reg = p->reg;
     launch_qrot_tile(p->bulk, p->dM, p->ldm, n, m, alpha, beta, reg, nullptr, nullptr, rowpart,

@@ -14,6 +14,8 @@ int qrot_rows(int n, int m);
 // owned by the problem handle.
 struct QrotScratch {
     DBuf<double> rowpart, colpart, sqpart, rsum, csum, sq, out, spart;
+    // rounding (qrot_round_plan): row / column scales, errors, |err_a|_1, totals
+    DBuf<double> x, y, ea, eb, l1part, l1, scal, tot;
     DBuf<int> kpart, kprev, done, open;
     HBuf<double> hout;
     HBuf<int> hopen;
@@ -37,6 +39,14 @@ struct QrotScratch {
         open.alloc(1);
         hout.ensure(2);
         hopen.ensure(1);
+        x.alloc(n);
+        y.alloc(m);
+        ea.alloc(n);
+        eb.alloc(m);
+        l1part.alloc(ceil_div(n, 256));
+        l1.alloc(1);
+        scal.alloc(2 * static_cast<size_t>(nstrip) * nblk);
+        tot.alloc(2);
     }
 };
 // Row sums (per 256-column strip: rowpart[strip * n + i]), column sums (per
