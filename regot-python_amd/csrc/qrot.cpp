@@ -43,7 +43,7 @@ double qrot_obj_grad(regot_problem* p, const double* gamma, double* grad, double
     }
     launch_qrot_sum1(w.sqpart, nstrip * nblk, w.sq, s);
     launch_qrot_finish(alpha, beta, p->a, p->b, n, m, p->reg, tau, w.sq, w.rsum, w.csum, grad,
-                       w.out, s);
+                       w.fpart, w.gpart, w.out, s);
     HIP_CHECK(hipMemcpyAsync(w.hout.p, w.out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (gnorm2) *gnorm2 = w.hout.p[1];

@@ -13,7 +13,7 @@ int qrot_rows(int n, int m);
 // Device scratch of the QROT passes and root iterations, sized for a shape;
 // owned by the problem handle.
 struct QrotScratch {
-    DBuf<double> rowpart, colpart, sqpart, rsum, csum, sq, out, spart;
+    DBuf<double> rowpart, colpart, sqpart, rsum, csum, sq, out, spart, fpart, gpart;
     // rounding (qrot_round_plan): row / column scales, errors, |err_a|_1, totals
     DBuf<double> x, y, ea, eb, l1part, l1, scal, tot;
     DBuf<int> kpart, kprev, done, open;
@@ -33,6 +33,9 @@ struct QrotScratch {
         csum.alloc(m);
         sq.alloc(1);
         out.alloc(2);
+        const int nw = ceil_div(n > m ? n : m, 256);
+        fpart.alloc(4 * static_cast<size_t>(nw));
+        gpart.alloc(nw);
         const int c = n > m ? n : m;
         kprev.alloc(c);
         done.alloc(c);
@@ -72,10 +75,12 @@ void launch_qrot_pl_step(int bulk, const void* M, long ldm, int n, int m, bool c
 void launch_qrot_plan(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
                       const double* beta, double reg, double* plan, long r0, int nr,
                       hipStream_t s);
+// f into out[0] and (with grad) grad and ||grad||^2 into out[1]; part holds
+// 4 ceil(max(n, m) / 256) doubles, gpart ceil(max(n, m) / 256).
 void launch_qrot_finish(const double* alpha, const double* beta, const double* a,
                         const double* b, int n, int m, double reg, double tau, const double* sq,
-                        const double* rsum, const double* csum, double* grad, double* out,
-                        hipStream_t s);
+                        const double* rsum, const double* csum, double* grad, double* part,
+                        double* gpart, double* out, hipStream_t s);
 
 void launch_qrot_fold(const double* part, int nparts, long stride, int count, double* out,
                       hipStream_t s);

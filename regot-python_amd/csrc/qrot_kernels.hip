@@ -29,11 +29,36 @@ __device__ __forceinline__ double wsum64(double v)
     return v;
 }
 
-template <typename TM>
-__device__ __forceinline__ void load4(const TM* row, int c0, int m, double* v)
+// The lane's 4 adjacent entries of a row: one 16-byte load (FP32) or two
+// (FP64) when all 4 are in range (rows are 16-byte aligned and c0 is a
+// multiple of 4), else element by element.
+__device__ __forceinline__ void load4(const float* row, int c0, int m, double* v)
 {
+    if (c0 + 3 < m) {
+        const float4 x = *reinterpret_cast<const float4*>(row + c0);
+        v[0] = x.x;
+        v[1] = x.y;
+        v[2] = x.z;
+        v[3] = x.w;
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < 4; q++) v[q] = (c0 + q < m) ? static_cast<double>(row[c0 + q]) : 0.0;
+}
+
+__device__ __forceinline__ void load4(const double* row, int c0, int m, double* v)
+{
+    if (c0 + 3 < m) {
+        const double2 x = *reinterpret_cast<const double2*>(row + c0);
+        const double2 y = *reinterpret_cast<const double2*>(row + c0 + 2);
+        v[0] = x.x;
+        v[1] = x.y;
+        v[2] = y.x;
+        v[3] = y.y;
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = (c0 + q < m) ? row[c0 + q] : 0.0;
 }
 
 template <typename TM, bool WT>
@@ -316,24 +341,23 @@ __global__ __launch_bounds__(256) void k_qrot_plan(const TM* __restrict__ M, lon
     }
 }
 
-// One workgroup: with S_a = alpha'a, S_b = beta'b, d = 1'alpha - 1'beta,
-// f = 0.5 sq - reg (S_a + S_b) + tau/2 d^2 and (if grad) grad = [rsum - reg a
-// + tau d; csum - reg b - tau d]; out[0] = f, out[1] = ||grad||^2.
-__global__ __launch_bounds__(256) void k_qrot_finish(
+// Workgroup w of the finish: partial sums over its 256-entry slices of
+// alpha'a, beta'b, 1'alpha, 1'beta (fixed order) into part[4 w ..].
+__global__ __launch_bounds__(256) void k_qrot_fin_part(
     const double* __restrict__ alpha, const double* __restrict__ beta,
-    const double* __restrict__ a, const double* __restrict__ b, int n, int m, double reg,
-    double tau, const double* __restrict__ sq, const double* __restrict__ rsum,
-    const double* __restrict__ csum, double* __restrict__ grad, double* __restrict__ out)
+    const double* __restrict__ a, const double* __restrict__ b, int n, int m,
+    double* __restrict__ part)
 {
-    __shared__ double sm[4][5];
-    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < n; i += 256) {
-        v[0] += alpha[i] * a[i];
-        v[2] += alpha[i];
+    __shared__ double sm[4][4];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (t < n) {
+        v[0] = alpha[t] * a[t];
+        v[2] = alpha[t];
     }
-    for (int j = threadIdx.x; j < m; j += 256) {
-        v[1] += beta[j] * b[j];
-        v[3] += beta[j];
+    if (t < m) {
+        v[1] = beta[t] * b[t];
+        v[3] = beta[t];
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -342,34 +366,65 @@ __global__ __launch_bounds__(256) void k_qrot_finish(
         if (lane == 0) sm[wave][q] = v[q];
     }
     __syncthreads();
-    double tot[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) tot[q] = ((sm[0][q] + sm[1][q]) + sm[2][q]) + sm[3][q];
-    const double d = tot[2] - tot[3];
+    if (threadIdx.x < 4) {
+        const int q = threadIdx.x;
+        part[4 * blockIdx.x + q] = ((sm[0][q] + sm[1][q]) + sm[2][q]) + sm[3][q];
+    }
+}
+
+// Gradient (if requested) and its squared norm per workgroup; every
+// workgroup folds the nw partials of k_qrot_fin_part in the same order, so d
+// is the same everywhere.  gpart[w] = sum of g^2 over the slice.
+__global__ __launch_bounds__(256) void k_qrot_fin_grad(
+    const double* __restrict__ part, int nw, const double* __restrict__ a,
+    const double* __restrict__ b, int n, int m, double reg, double tau,
+    const double* __restrict__ rsum, const double* __restrict__ csum,
+    double* __restrict__ grad, double* __restrict__ gpart)
+{
+    __shared__ double sm[4];
+    double sa = 0.0, sb = 0.0;
+    for (int w = 0; w < nw; w++) {
+        sa += part[4 * w + 2];
+        sb += part[4 * w + 3];
+    }
+    const double d = sa - sb;
     const double td = tau > 0.0 ? tau * d : 0.0;
+    const int t = blockIdx.x * 256 + threadIdx.x;
     double g2 = 0.0;
-    if (grad) {
-        for (int i = threadIdx.x; i < n; i += 256) {
-            const double g = (rsum[i] - reg * a[i]) + td;
-            grad[i] = g;
-            g2 += g * g;
-        }
-        for (int j = threadIdx.x; j < m; j += 256) {
-            const double g = (csum[j] - reg * b[j]) - td;
-            grad[n + j] = g;
-            g2 += g * g;
-        }
+    if (t < n) {
+        const double g = (rsum[t] - reg * a[t]) + td;
+        grad[t] = g;
+        g2 += g * g;
+    }
+    if (t < m) {
+        const double g = (csum[t] - reg * b[t]) - td;
+        grad[n + t] = g;
+        g2 += g * g;
     }
     g2 = wsum64(g2);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = g2;
     __syncthreads();
-    if (lane == 0) sm[wave][4] = g2;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double f = 0.5 * sq[0] - reg * (tot[0] + tot[1]);
-        if (tau > 0.0) f += 0.5 * tau * d * d;
-        out[0] = f;
-        out[1] = ((sm[0][4] + sm[1][4]) + sm[2][4]) + sm[3][4];
-    }
+    if (threadIdx.x == 0) gpart[blockIdx.x] = ((sm[0] + sm[1]) + sm[2]) + sm[3];
+}
+
+// out[0] = f = 0.5 sq - reg (alpha'a + beta'b) + tau/2 d^2, out[1] = ||g||^2.
+__global__ __launch_bounds__(64) void k_qrot_fin_total(const double* __restrict__ part, int nw,
+                                                       const double* __restrict__ gpart,
+                                                       int ngw, double reg, double tau,
+                                                       const double* __restrict__ sq,
+                                                       double* __restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double tot[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int w = 0; w < nw; w++)
+        for (int q = 0; q < 4; q++) tot[q] += part[4 * w + q];
+    const double d = tot[2] - tot[3];
+    double f = 0.5 * sq[0] - reg * (tot[0] + tot[1]);
+    if (tau > 0.0) f += 0.5 * tau * d * d;
+    double g2 = 0.0;
+    for (int w = 0; w < ngw; w++) g2 += gpart[w];
+    out[0] = f;
+    out[1] = g2;
 }
 
 }  // namespace
@@ -501,11 +556,19 @@ void launch_qrot_sum1(const double* part, int count, double* out, hipStream_t s)
 
 void launch_qrot_finish(const double* alpha, const double* beta, const double* a,
                         const double* b, int n, int m, double reg, double tau, const double* sq,
-                        const double* rsum, const double* csum, double* grad, double* out,
-                        hipStream_t s)
+                        const double* rsum, const double* csum, double* grad, double* part,
+                        double* gpart, double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_qrot_finish, dim3(1), dim3(256), 0, s, alpha, beta, a, b, n, m, reg,
-                       tau, sq, rsum, csum, grad, out);
+    const int nw = ceil_div(n > m ? n : m, 256);
+    hipLaunchKernelGGL(k_qrot_fin_part, dim3(nw), dim3(256), 0, s, alpha, beta, a, b, n, m, part);
+    KERNEL_CHECK();
+    if (grad) {
+        hipLaunchKernelGGL(k_qrot_fin_grad, dim3(nw), dim3(256), 0, s, part, nw, a, b, n, m, reg,
+                           tau, rsum, csum, grad, gpart);
+        KERNEL_CHECK();
+    }
+    hipLaunchKernelGGL(k_qrot_fin_total, dim3(1), dim3(64), 0, s, part, nw, gpart,
+                       grad ? nw : 0, reg, tau, sq, out);
     KERNEL_CHECK();
 }
 
