@@ -282,17 +282,21 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
  * Quadratically regularised OT (QROT) on the same problem handle: the dual
  * variable is gamma = [alpha (n); beta (m)] and
  *   f = 0.5 ||(alpha (+) beta - M)_+||^2 - reg (a'alpha + b'beta) + tau/2 (1'alpha - 1'beta)^2.
- * regot_qrot_solve replaces qrot_bcd_internal, qrot_gd_internal and
- * qrot_lbfgs_dual_internal (src/qrot_bcd.cpp:15-101, src/qrot_gd.cpp:15-121,
- * src/qrot_lbfgs_dual.cpp:85-126; bindings src/solvers.cpp:85-148).  Its result
+ * regot_qrot_solve replaces qrot_bcd_internal, qrot_gd_internal,
+ * qrot_lbfgs_dual_internal and qrot_lbfgs_semi_dual_internal
+ * (src/qrot_bcd.cpp:15-101, src/qrot_gd.cpp:15-121,
+ * src/qrot_lbfgs_dual.cpp:85-126, src/qrot_lbfgs_semi_dual.cpp:78-118;
+ * bindings src/solvers.cpp:85-164).  Its result
  * carries prim_vals (regot_result_prim_vals, one per history entry) and a
  * dual of n + m entries (regot_result_dual_len); mar_errs are ||grad|| / reg;
  * regot_result_plan gives (alpha (+) beta - M)_+ / reg.  opts may be NULL;
- * opts->x0 (n + m) and opts->tau are used (BCD forces tau = 0).
+ * opts->x0 (n + m; the semi-dual uses its first n) and opts->tau are used
+ * (BCD and the semi-dual force tau = 0).
  */
 #define REGOT_QROT_BCD 0
 #define REGOT_QROT_GD 1
 #define REGOT_QROT_LBFGS_DUAL 2
+#define REGOT_QROT_LBFGS_SEMI_DUAL 3   /* src/qrot_lbfgs_semi_dual.cpp:78-118 */
 int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,
                      const regot_opts* opts, regot_result** out);
 int regot_result_prim_vals(const regot_result* r, double* out);

@@ -484,3 +484,8 @@ def qrot_gd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, tau=0.0, x0=None):
 def qrot_lbfgs_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, tau=0.0, x0=None):
     """src/qrot_lbfgs_dual.cpp:85-126 with LBFGSSolver::minimize"""
     return _qrot_solve(2, M, a, b, reg, tol, max_iter, verbose, tau, x0)
+
+
+def qrot_lbfgs_semi_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, x0=None):
+    """src/qrot_lbfgs_semi_dual.cpp:78-118 with LBFGSSolver::minimize"""
+    return _qrot_solve(3, M, a, b, reg, tol, max_iter, verbose, 0.0, x0)

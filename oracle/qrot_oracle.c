@@ -203,8 +203,41 @@ static void qrot_lbfgs_iter(void* ctx, const double* x, double last_f, double gn
         printf("iter = %d, objval = %g, ||grad|| = %g\n", Q->h - 1, last_f, gnorm);
 }
 
+/* Semi-dual objective at alpha (n): beta = optimal_beta(alpha), f = the dual
+ * objective at [alpha; beta] (tau = 0), grad = D 1 - reg a.  Restates
+ * Problem::semi_dual_obj_grad, src/qrot_problem.cpp:421-441. */
+static double qrot_semi_objgrad(void* ctx, const double* x, double* grad)
+{
+    const qrot_ctx* Q = (const qrot_ctx*)ctx;
+    const int n = Q->n, m = Q->m;
+    double* g = (double*)malloc(sizeof(double) * (size_t)(n + m));
+    double* gm = (double*)malloc(sizeof(double) * (size_t)(n + m));
+    memcpy(g, x, sizeof(double) * n);
+    orc_qrot_optimal_beta(n, m, Q->M, Q->b, Q->reg, x, g + n);
+    const double f = orc_qrot_dual_obj_grad(n, m, Q->M, Q->a, Q->b, Q->reg, 0.0, g,
+                                            grad ? gm : NULL);
+    if (grad) memcpy(grad, gm, sizeof(double) * n);
+    free(g);
+    free(gm);
+    return f;
+}
+
+/* iterate() of src/qrot_lbfgs_semi_dual.cpp:51-75: the primal value at
+ * [alpha; optimal_beta(alpha)]. */
+static void qrot_semi_iter(void* ctx, const double* x, double last_f, double gnorm)
+{
+    qrot_ctx* Q = (qrot_ctx*)ctx;
+    double* g = (double*)malloc(sizeof(double) * (size_t)(Q->n + Q->m));
+    memcpy(g, x, sizeof(double) * Q->n);
+    orc_qrot_optimal_beta(Q->n, Q->m, Q->M, Q->b, Q->reg, x, g + Q->n);
+    qrot_record(Q, g, last_f, gnorm);
+    free(g);
+    if (Q->verbose >= 1)
+        printf("iter = %d, objval = %g, ||grad|| = %g\n", Q->h - 1, last_f, gnorm);
+}
+
 /* QROT drivers (solver 0 = BCD, 1 = gradient descent, 2 = L-BFGS on the
- * dual).  Initial point alpha = 0, beta = optimal_beta(alpha), or x0 (n+m)
+ * dual, 3 = L-BFGS on the semi-dual).  Initial point alpha = 0, beta = optimal_beta(alpha), or x0 (n+m)
  * when non-NULL.  x receives the final dual; histories hold at most
  * hist_cap entries (niter + 1 for BCD and GD).  Returns niter, or a negative
  * line-search status (L-BFGS).
@@ -212,15 +245,30 @@ static void qrot_lbfgs_iter(void* ctx, const double* x, double last_f, double gn
  *        alpha = optimal_alpha(beta), beta = optimal_beta(alpha)).
  *   GD:  src/qrot_gd.cpp:15-121 (step 1 halved up to 20 times until
  *        f(gamma - step g) <= f - step 0.5 ||g||^2; the last trial is taken).
- *   L-BFGS: src/qrot_lbfgs_dual.cpp:85-126 (epsilon = epsilon_rel = tol). */
+ *   L-BFGS: src/qrot_lbfgs_dual.cpp:85-126 (epsilon = epsilon_rel = tol).
+ *   Semi-dual L-BFGS: src/qrot_lbfgs_semi_dual.cpp:78-118 (over alpha, from
+ *        alpha = 0 or x0's head; tau = 0; the result is [alpha;
+ *        optimal_beta(alpha)]). */
 int orc_qrot_solve(int solver, int n, int m, const double* M, const double* a, const double* b,
                    double reg, double tau, double tol, int max_iter, int verbose, int hist_cap,
                    const double* x0, double* x, double* obj_vals, double* prim_vals,
                    double* mar_errs, double* run_times, int* hist_len)
 {
     const long N = (long)n + m;
-    qrot_ctx Q = {n, m, M, a, b, reg, solver == 0 ? 0.0 : tau, 0, hist_cap, verbose, 0.0,
-                  obj_vals, prim_vals, mar_errs, run_times};
+    qrot_ctx Q = {n, m, M, a, b, reg, (solver == 0 || solver == 3) ? 0.0 : tau, 0, hist_cap,
+                  verbose, 0.0, obj_vals, prim_vals, mar_errs, run_times};
+    if (solver == 3) {
+        if (x0)
+            memcpy(x, x0, sizeof(double) * n);
+        else
+            memset(x, 0, sizeof(double) * n);
+        Q.t0 = q_now_ms();
+        const int rc = orc_lbfgs_minimize(n, qrot_semi_objgrad, &Q, tol, tol, max_iter,
+                                          qrot_semi_iter, &Q, x);
+        orc_qrot_optimal_beta(n, m, M, b, reg, x, x + n);
+        *hist_len = Q.h;
+        return rc;
+    }
     if (x0) {
         memcpy(x, x0, sizeof(double) * N);
     } else {

@@ -45,7 +45,8 @@ void solve_lbfgs_dual(regot_problem* p, double tol, int max_iter, int verbose,
 void solve_apdagd(regot_problem* p, double tol, int max_iter, int verbose, const regot_opts& o,
                   regot_result* res);
 
-// QROT drivers (qrot.cpp): solver 0 = BCD, 1 = gradient descent, 2 = L-BFGS.
+// QROT drivers (qrot.cpp): solver 0 = BCD, 1 = gradient descent, 2 = L-BFGS,
+// 3 = L-BFGS on the semi-dual.
 void solve_qrot(regot_problem* p, int solver, double tol, int max_iter, int verbose,
                 const regot_opts& o, regot_result* res);
 

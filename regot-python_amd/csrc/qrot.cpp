@@ -80,9 +80,69 @@ void qrot_optimal(regot_problem* p, bool cols, const double* pot, double* out)
 // the rounded plan, mar_errs ||grad|| / reg, run_times cumulative ms; niter + 1
 // entries for BCD and GD (test ||grad|| < tol before each update), one per
 // iterate() call for L-BFGS.
+// L-BFGS on the semi-dual (src/qrot_lbfgs_semi_dual.cpp:78-118): over alpha
+// alone, from alpha = 0 or x0's head; every evaluation finds beta =
+// optimal_beta(alpha) and takes the dual objective there (tau = 0) with the
+// gradient's alpha block; every iterate records the primal value at [alpha;
+// optimal_beta(alpha)]; the result is [alpha; optimal_beta(alpha)].
+static void solve_qrot_semi_dual(regot_problem* p, double tol, int max_iter, int verbose,
+                                 const regot_opts& o, regot_result* res)
+{
+    const int n = p->n, m = p->m;
+    const long N = static_cast<long>(n) + m;
+    hipStream_t s = p->stream;
+    const double reg = p->reg;
+    Work w(p, n);
+    DBuf<double> gfull(N), gtmp(N), gpr(N);
+    w.og = [&](const double* x, double* g, double* gn2) {
+        vec_copy(gfull, x, n, s);
+        qrot_optimal(p, true, gfull, gfull.p + n);
+        const double f = qrot_obj_grad(p, gfull, g ? gtmp.p : nullptr, 0.0, nullptr);
+        if (g) {
+            vec_copy(g, gtmp, n, s);
+            if (gn2) *gn2 = w.dot(g, g);
+        }
+        return f;
+    };
+    if (o.x0)
+        HIP_CHECK(hipMemcpyAsync(w.gamma, o.x0, n * sizeof(double), hipMemcpyHostToDevice, s));
+    else
+        launch_fill(w.gamma, 0.0, n, s);
+    res->qrot = true;
+    Timer timer;
+    timer.tic();
+    auto iterate = [&](const double* x, double last_f, double gnorm) {
+        const double dur = timer.toc();
+        vec_copy(gpr, x, n, s);
+        qrot_optimal(p, true, gpr, gpr.p + n);
+        const double prim = qrot_round_plan(p, gpr, gpr.p + n, nullptr, 0);
+        res->obj_vals.push_back(last_f);
+        res->prim_vals.push_back(prim);
+        res->mar_errs.push_back(gnorm / reg);
+        res->run_times.push_back((res->run_times.empty() ? 0.0 : res->run_times.back()) + dur);
+        if (verbose >= 1) {
+            printf("iter = %d, objval = %g, ||grad|| = %g\n",
+                   static_cast<int>(res->obj_vals.size()) - 1, last_f, gnorm);
+            fflush(stdout);
+        }
+        timer.tic();
+    };
+    const double* xfinal = w.gamma.p;
+    res->niter = lbfgs_minimize(w, tol, tol, max_iter, iterate, &xfinal);
+    vec_copy(gfull, xfinal, n, s);
+    qrot_optimal(p, true, gfull, gfull.p + n);
+    w.report(res);
+    res->dual.resize(N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), gfull.p, N * sizeof(double), hipMemcpyDeviceToHost));
+}
+
 void solve_qrot(regot_problem* p, int solver, double tol, int max_iter, int verbose,
                 const regot_opts& o, regot_result* res)
 {
+    if (solver == 3) {
+        solve_qrot_semi_dual(p, tol, max_iter, verbose, o, res);
+        return;
+    }
     const int n = p->n, m = p->m;
     const long N = static_cast<long>(n) + m;
     hipStream_t s = p->stream;
@@ -199,7 +259,7 @@ int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int
     CAPI_BEGIN
     if (!p || !out) fail(E_INVALID, "null argument");
     *out = nullptr;
-    if (solver < 0 || solver > 2) fail(E_INVALID, "unknown QROT solver %d", solver);
+    if (solver < 0 || solver > 3) fail(E_INVALID, "unknown QROT solver %d", solver);
     if (max_iter < 0) fail(E_INVALID, "max_iter must be >= 0");
     HIP_CHECK(hipSetDevice(p->device));
     regot_opts o;

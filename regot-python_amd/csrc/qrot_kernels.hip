@@ -61,7 +61,9 @@ __device__ __forceinline__ void load4(const double* row, int c0, int m, double* 
     for (int q = 0; q < 4; q++) v[q] = (c0 + q < m) ? row[c0 + q] : 0.0;
 }
 
-template <typename TM, bool WT>
+// DIV: divide D by div (the plan D / reg of the rounding passes, in the
+// reference's order of operations); without it div is 1 and skipped.
+template <typename TM, bool WT, bool DIV>
 __global__ __launch_bounds__(256) void k_qrot_tile(const TM* __restrict__ M, long ldm, int n,
                                                    int m, const double* __restrict__ alpha,
                                                    const double* __restrict__ beta, double div,
@@ -94,7 +96,8 @@ __global__ __launch_bounds__(256) void k_qrot_tile(const TM* __restrict__ M, lon
         for (int q = 0; q < 4; q++) {
             if (c0 + q < m) {
                 const double c = (ai + bq[q]) - mv[q];
-                double d = (c > 0.0 ? c : 0.0) / div;
+                double d = c > 0.0 ? c : 0.0;
+                if (DIV) d = d / div;
                 if (WT) {
                     if (x != nullptr) d = d * xi;
                     if (y != nullptr) d = d * yq[q];
@@ -438,31 +441,36 @@ int qrot_rows(int n, int m)
     return rows;
 }
 
+template <typename TM>
+void qrot_tile_launch(dim3 grid, int rows, hipStream_t s, const TM* M, long ldm, int n, int m,
+                      const double* alpha, const double* beta, double div, const double* x,
+                      const double* y, double* rowpart, double* colpart, double* sqpart)
+{
+    const bool wt = x != nullptr || y != nullptr;
+    if (wt)
+        hipLaunchKernelGGL((k_qrot_tile<TM, true, true>), grid, dim3(256), 0, s, M, ldm, n, m,
+                           alpha, beta, div, x, y, rows, rowpart, colpart, sqpart);
+    else if (div != 1.0)
+        hipLaunchKernelGGL((k_qrot_tile<TM, false, true>), grid, dim3(256), 0, s, M, ldm, n, m,
+                           alpha, beta, div, x, y, rows, rowpart, colpart, sqpart);
+    else
+        hipLaunchKernelGGL((k_qrot_tile<TM, false, false>), grid, dim3(256), 0, s, M, ldm, n, m,
+                           alpha, beta, div, x, y, rows, rowpart, colpart, sqpart);
+    KERNEL_CHECK();
+}
+
 void launch_qrot_tile(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
                       const double* beta, double div, const double* x, const double* y,
                       double* rowpart, double* colpart, double* sqpart, hipStream_t s)
 {
     const int rows = qrot_rows(n, m);
     const dim3 grid(ceil_div(m, 256), ceil_div(n, rows));
-    const bool wt = x != nullptr || y != nullptr;
-    if (bulk == 0) {
-        const float* Mf = static_cast<const float*>(M);
-        if (wt)
-            hipLaunchKernelGGL((k_qrot_tile<float, true>), grid, dim3(256), 0, s, Mf, ldm, n, m,
-                               alpha, beta, div, x, y, rows, rowpart, colpart, sqpart);
-        else
-            hipLaunchKernelGGL((k_qrot_tile<float, false>), grid, dim3(256), 0, s, Mf, ldm, n,
-                               m, alpha, beta, div, x, y, rows, rowpart, colpart, sqpart);
-    } else {
-        const double* Md = static_cast<const double*>(M);
-        if (wt)
-            hipLaunchKernelGGL((k_qrot_tile<double, true>), grid, dim3(256), 0, s, Md, ldm, n,
-                               m, alpha, beta, div, x, y, rows, rowpart, colpart, sqpart);
-        else
-            hipLaunchKernelGGL((k_qrot_tile<double, false>), grid, dim3(256), 0, s, Md, ldm, n,
-                               m, alpha, beta, div, x, y, rows, rowpart, colpart, sqpart);
-    }
-    KERNEL_CHECK();
+    if (bulk == 0)
+        qrot_tile_launch(grid, rows, s, static_cast<const float*>(M), ldm, n, m, alpha, beta, div,
+                         x, y, rowpart, colpart, sqpart);
+    else
+        qrot_tile_launch(grid, rows, s, static_cast<const double*>(M), ldm, n, m, alpha, beta,
+                         div, x, y, rowpart, colpart, sqpart);
 }
 
 void launch_qrot_round(int bulk, const void* M, long ldm, int n, int m, const double* alpha,

@@ -29,7 +29,7 @@ from ._lib import RegotError, check, dptr, lib
 __all__ = [
     "sinkhorn_bcd", "sinkhorn_ssns", "sinkhorn_sparse_newton", "sinkhorn_splr",
     "sinkhorn_newton", "sinkhorn_lbfgs_dual", "sinkhorn_apdagd", "sinkhorn_result",
-    "qrot_bcd", "qrot_gd", "qrot_lbfgs_dual", "qrot_result",
+    "qrot_bcd", "qrot_gd", "qrot_lbfgs_dual", "qrot_lbfgs_semi_dual", "qrot_result",
     "Problem", "BatchProblem", "RegotError",
 ]
 
@@ -305,7 +305,8 @@ class Problem:
 
     def qrot_solve(self, solver, tol=1e-6, max_iter=1000, verbose=0, compute_plan=True,
                    **kwargs):
-        """QROT solver (_lib.QROT_BCD / QROT_GD / QROT_LBFGS_DUAL) -> qrot_result.
+        """QROT solver (_lib.QROT_BCD / QROT_GD / QROT_LBFGS_DUAL /
+        QROT_LBFGS_SEMI_DUAL) -> qrot_result.
         Keywords: x0 (n+m warm start), tau (extra dual term; the reference's
         Python bindings never set it, src/solvers.cpp:33-51)."""
         opts = _lib.Opts()
@@ -697,3 +698,9 @@ def qrot_gd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
 def qrot_lbfgs_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
     """L-BFGS on the QROT dual (src/solvers.cpp:134-148, src/qrot_lbfgs_dual.cpp:85-126)."""
     return _run_qrot(_lib.QROT_LBFGS_DUAL, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def qrot_lbfgs_semi_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """L-BFGS on the QROT semi-dual (src/solvers.cpp:150-164,
+    src/qrot_lbfgs_semi_dual.cpp:78-118)."""
+    return _run_qrot(_lib.QROT_LBFGS_SEMI_DUAL, M, a, b, reg, tol, max_iter, verbose, kwargs)

@@ -22,6 +22,7 @@ SOLVER_APDAGD = 6
 QROT_BCD = 0
 QROT_GD = 1
 QROT_LBFGS_DUAL = 2
+QROT_LBFGS_SEMI_DUAL = 3
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p

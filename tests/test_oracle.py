@@ -539,7 +539,8 @@ def test_qrot_drivers_reach_the_qp_optimum():
     P_qp = qp.x.reshape(n, m)
     # (L-BFGS at tol 1e-10 ends in LBFGS++'s line-search error: f no longer
     # decreases at FP64 resolution)
-    for solve, tol in ((O.qrot_bcd, 1e-10), (O.qrot_gd, 1e-10), (O.qrot_lbfgs_dual, 1e-9)):
+    for solve, tol in ((O.qrot_bcd, 1e-10), (O.qrot_gd, 1e-10), (O.qrot_lbfgs_dual, 1e-9),
+                       (O.qrot_lbfgs_semi_dual, 1e-9)):
         r = solve(M, a, b, reg, tol=tol, max_iter=100000)
         assert np.max(np.abs(r["plan"] - P_qp)) < 1e-6, solve.__name__
         # strong duality: primal value = -f / reg at the optimum
@@ -565,3 +566,29 @@ def test_qrot_driver_histories():
     # BCD's objective decreases monotonically (exact block minimisation)
     r = O.qrot_bcd(M, a, b, reg, tol=tol, max_iter=20000)
     assert np.all(np.diff(r["obj_vals"]) <= 1e-15)
+
+
+def test_qrot_semi_dual_gradient_is_the_envelope():
+    # f(alpha) = dual(alpha, beta*(alpha)); its gradient is D 1 - reg a (the
+    # beta block vanishes at beta*): central differences of the semi-dual
+    # objective against the gradient L-BFGS sees at its first iterate
+    M, a, b = cloud_problem(9, 7, d=2, seed=35)
+    reg = 0.1
+    rng = np.random.default_rng(6)
+    alpha = 0.05 * rng.standard_normal(9)
+
+    def semi(al):
+        be = O.qrot_optimal_beta(M, b, reg, al)
+        return O.qrot_dual_obj_grad(M, a, b, reg, np.concatenate([al, be]))
+
+    f, g = semi(alpha)
+    h = 1e-7
+    for k in range(9):
+        e = np.zeros(9)
+        e[k] = h
+        fd = (semi(alpha + e)[0] - semi(alpha - e)[0]) / (2 * h)
+        assert abs(fd - g[k]) < 1e-6
+    r = O.qrot_lbfgs_semi_dual(M, a, b, reg, tol=1e-9, max_iter=5000)
+    # the result's beta is optimal for its alpha, and the histories line up
+    assert np.max(np.abs(r["dual"][9:] - O.qrot_optimal_beta(M, b, reg, r["dual"][:9]))) == 0.0
+    assert len(r["obj_vals"]) == len(r["prim_vals"]) == len(r["mar_errs"])
