@@ -283,7 +283,8 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
  * variable is gamma = [alpha (n); beta (m)] and
  *   f = 0.5 ||(alpha (+) beta - M)_+||^2 - reg (a'alpha + b'beta) + tau/2 (1'alpha - 1'beta)^2.
  * regot_qrot_solve replaces qrot_bcd_internal, qrot_gd_internal,
- * qrot_lbfgs_dual_internal and qrot_lbfgs_semi_dual_internal
+ * qrot_lbfgs_dual_internal, qrot_lbfgs_semi_dual_internal and
+ * qrot_apdagd_internal (src/qrot_apdagd.cpp:14-181)
  * (src/qrot_bcd.cpp:15-101, src/qrot_gd.cpp:15-121,
  * src/qrot_lbfgs_dual.cpp:85-126, src/qrot_lbfgs_semi_dual.cpp:78-118;
  * bindings src/solvers.cpp:85-164).  Its result
@@ -297,6 +298,8 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
 #define REGOT_QROT_GD 1
 #define REGOT_QROT_LBFGS_DUAL 2
 #define REGOT_QROT_LBFGS_SEMI_DUAL 3   /* src/qrot_lbfgs_semi_dual.cpp:78-118 */
+#define REGOT_QROT_APDAGD 4            /* src/qrot_apdagd.cpp:14-181; the result keeps its
+                                          averaged primal plan (regot_result_plan) */
 int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,
                      const regot_opts* opts, regot_result** out);
 int regot_result_prim_vals(const regot_result* r, double* out);

@@ -160,11 +160,13 @@ void orc_qrot_get_plan(int n, int m, const double* M, double reg, const double* 
                        double* plan);
 double orc_qrot_primal_val(int n, int m, const double* M, const double* a, const double* b,
                            double reg, const double* gamma);
-/* solver: 0 = BCD, 1 = gradient descent, 2 = L-BFGS on the dual, 3 = L-BFGS on the semi-dual */
+/* solver: 0 = BCD, 1 = gradient descent, 2 = L-BFGS on the dual, 3 = L-BFGS on
+ * the semi-dual, 4 = APDAGD (plan, n x m, receives its averaged primal plan;
+ * may be NULL for the others) */
 int orc_qrot_solve(int solver, int n, int m, const double* M, const double* a, const double* b,
                    double reg, double tau, double tol, int max_iter, int verbose, int hist_cap,
                    const double* x0, double* x, double* obj_vals, double* prim_vals,
-                   double* mar_errs, double* run_times, int* hist_len);
+                   double* mar_errs, double* run_times, int* hist_len, double* plan);
 
 /* BCD iterations only (no convergence test), for timing a bounded CPU sample. */
 double orc_bcd_iterations(int n, int m, const double* M, const double* a, const double* b,

@@ -460,13 +460,16 @@ def _qrot_solve(solver, M, a, b, reg, tol, max_iter, verbose, tau=0.0, x0=None):
     h = [np.empty(cap) for _ in range(4)]
     hl = ctypes.c_int(0)
     x0p = _p(_d(x0)) if x0 is not None else None
+    plan = np.empty((n, m))
     niter = lib().orc_qrot_solve(int(solver), n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg),
                                  ctypes.c_double(tau), ctypes.c_double(tol), int(max_iter),
                                  int(verbose), cap, x0p, _p(dual), _p(h[0]), _p(h[1]), _p(h[2]),
-                                 _p(h[3]), ctypes.byref(hl))
+                                 _p(h[3]), ctypes.byref(hl), _p(plan))
     _check(niter)
     k = hl.value
-    return dict(niter=niter, dual=dual, plan=qrot_get_plan(M, reg, dual),
+    if solver != 4:
+        plan = qrot_get_plan(M, reg, dual)
+    return dict(niter=niter, dual=dual, plan=plan,
                 obj_vals=list(h[0][:k]), prim_vals=list(h[1][:k]), mar_errs=list(h[2][:k]),
                 run_times=list(h[3][:k]))
 
@@ -489,3 +492,8 @@ def qrot_lbfgs_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, tau=0.0, x
 def qrot_lbfgs_semi_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, x0=None):
     """src/qrot_lbfgs_semi_dual.cpp:78-118 with LBFGSSolver::minimize"""
     return _qrot_solve(3, M, a, b, reg, tol, max_iter, verbose, 0.0, x0)
+
+
+def qrot_apdagd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, x0=None):
+    """src/qrot_apdagd.cpp:14-181 -> dict with the averaged primal plan under "plan" """
+    return _qrot_solve(4, M, a, b, reg, tol, max_iter, verbose, 0.0, x0)

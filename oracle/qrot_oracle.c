@@ -236,8 +236,105 @@ static void qrot_semi_iter(void* ctx, const double* x, double last_f, double gno
         printf("iter = %d, objval = %g, ||grad|| = %g\n", Q->h - 1, last_f, gnorm);
 }
 
+/* APDAGD on the QROT dual (src/qrot_apdagd.cpp:14-181): dual variables
+ * lambda = -gamma, L halved per iteration and doubled per failed test (at
+ * most 20 inner trials, the last one accepted), the primal plan the running
+ * combination (1 - tau) P + tau plan(-lambda~) starting from plan(gamma_0);
+ * histories: f of the last objective evaluated, the primal value of the
+ * rounded plan at -lambda~, the full marginal error of P (all m columns);
+ * stop when <M, P> + reg/2 ||P||^2 - phi <= tol and the marginal error <=
+ * tol.  x receives -lambda~ of the last iteration, plan the averaged P. */
+static int qrot_apdagd(qrot_ctx* Q, double tol, int max_iter, double* x, double* plan)
+{
+    const int n = Q->n, m = Q->m;
+    const long N = (long)n + m;
+    const double reg = Q->reg;
+    double* grad = (double*)malloc(sizeof(double) * N);
+    double* dual = (double*)malloc(sizeof(double) * N);
+    double* dualp = (double*)malloc(sizeof(double) * N);
+    double* dualt = (double*)malloc(sizeof(double) * N);
+    double* dual_new = (double*)malloc(sizeof(double) * N);
+    double* dualp_new = (double*)malloc(sizeof(double) * N);
+    double* T = (double*)malloc(sizeof(double) * (size_t)n * m);
+    for (long q = 0; q < N; q++) dual[q] = dualp[q] = dualt[q] = -x[q];
+    orc_qrot_get_plan(n, m, Q->M, reg, x, plan);
+    double f = qrot_objgrad(Q, x, grad);
+    qrot_record(Q, x, f, q_norm2(grad, N));
+    double L = 1.0, pbeta = 0.0;
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        L *= 0.5;
+        double ptau = 0.0, phi_new = 0.0;
+        for (int j = 0; j < 20; j++) {
+            const double palpha = (1.0 + sqrt(4.0 * L * pbeta + 1.0)) / (2.0 * L);
+            const double pbeta_new = pbeta + palpha;
+            ptau = palpha / pbeta_new;
+            for (long q = 0; q < N; q++) {
+                dualt[q] = (1.0 - ptau) * dual[q] + ptau * dualp[q];
+                x[q] = -dualt[q];
+            }
+            f = qrot_objgrad(Q, x, grad);
+            const double phit = -f / reg;
+            double gd = 0.0, dd = 0.0;
+            for (long q = 0; q < N; q++) {
+                dualp_new[q] = dualp[q] + (palpha / reg) * grad[q];
+                dual_new[q] = (1.0 - ptau) * dual[q] + ptau * dualp_new[q];
+                x[q] = -dual_new[q];
+                const double df = dual_new[q] - dualt[q];
+                gd += grad[q] * df;
+                dd += df * df;
+            }
+            f = qrot_objgrad(Q, x, NULL);
+            phi_new = -f / reg;
+            const double rhs = phit + gd / reg - 0.5 * L * dd;
+            if (phi_new >= rhs || j >= 19) {
+                pbeta = pbeta_new;
+                memcpy(dualp, dualp_new, sizeof(double) * N);
+                memcpy(dual, dual_new, sizeof(double) * N);
+                break;
+            }
+            L *= 2.0;
+        }
+        for (long q = 0; q < N; q++) x[q] = -dualt[q];
+        orc_qrot_get_plan(n, m, Q->M, reg, x, T);
+        double mp = 0.0, sq = 0.0;
+        for (long q = 0; q < (long)n * m; q++) {
+            plan[q] = (1.0 - ptau) * plan[q] + ptau * T[q];
+            mp += Q->M[q] * plan[q];
+            sq += plan[q] * plan[q];
+        }
+        const double primal_obj = mp + 0.5 * reg * sq;
+        double ra = 0.0, rb = 0.0;
+        for (int i = 0; i < n; i++) {
+            double r = 0.0;
+            for (int jj = 0; jj < m; jj++) r += plan[(long)i * m + jj];
+            ra += (r - Q->a[i]) * (r - Q->a[i]);
+        }
+        for (int jj = 0; jj < m; jj++) {
+            double c = 0.0;
+            for (int i = 0; i < n; i++) c += plan[(long)i * m + jj];
+            rb += (c - Q->b[jj]) * (c - Q->b[jj]);
+        }
+        const double mar_err = sqrt(ra + rb);
+        qrot_record(Q, x, f, mar_err * reg);   /* qrot_record divides by reg */
+        if (Q->verbose >= 1)
+            printf("iter = %d, primal_obj = %g, phi_new = %g, mar_err = %g\n", it, primal_obj,
+                   phi_new, mar_err);
+        if (primal_obj - phi_new <= tol && mar_err <= tol) break;
+    }
+    free(grad);
+    free(dual);
+    free(dualp);
+    free(dualt);
+    free(dual_new);
+    free(dualp_new);
+    free(T);
+    return it;
+}
+
 /* QROT drivers (solver 0 = BCD, 1 = gradient descent, 2 = L-BFGS on the
- * dual, 3 = L-BFGS on the semi-dual).  Initial point alpha = 0, beta = optimal_beta(alpha), or x0 (n+m)
+ * dual, 3 = L-BFGS on the semi-dual, 4 = APDAGD; plan (n x m, solver 4 only)
+ * receives the averaged primal plan).  Initial point alpha = 0, beta = optimal_beta(alpha), or x0 (n+m)
  * when non-NULL.  x receives the final dual; histories hold at most
  * hist_cap entries (niter + 1 for BCD and GD).  Returns niter, or a negative
  * line-search status (L-BFGS).
@@ -252,7 +349,7 @@ static void qrot_semi_iter(void* ctx, const double* x, double last_f, double gno
 int orc_qrot_solve(int solver, int n, int m, const double* M, const double* a, const double* b,
                    double reg, double tau, double tol, int max_iter, int verbose, int hist_cap,
                    const double* x0, double* x, double* obj_vals, double* prim_vals,
-                   double* mar_errs, double* run_times, int* hist_len)
+                   double* mar_errs, double* run_times, int* hist_len, double* plan)
 {
     const long N = (long)n + m;
     qrot_ctx Q = {n, m, M, a, b, reg, (solver == 0 || solver == 3) ? 0.0 : tau, 0, hist_cap,
@@ -276,6 +373,12 @@ int orc_qrot_solve(int solver, int n, int m, const double* M, const double* a, c
         orc_qrot_optimal_beta(n, m, M, b, reg, x, x + n);
     }
     Q.t0 = q_now_ms();
+    if (solver == 4) {
+        Q.tau = 0.0;
+        const int it = qrot_apdagd(&Q, tol, max_iter, x, plan);
+        *hist_len = Q.h;
+        return it;
+    }
     if (solver == 2) {
         const int rc = orc_lbfgs_minimize(N, qrot_objgrad, &Q, tol, tol, max_iter,
                                           qrot_lbfgs_iter, &Q, x);

@@ -46,7 +46,7 @@ void solve_apdagd(regot_problem* p, double tol, int max_iter, int verbose, const
                   regot_result* res);
 
 // QROT drivers (qrot.cpp): solver 0 = BCD, 1 = gradient descent, 2 = L-BFGS,
-// 3 = L-BFGS on the semi-dual.
+// 3 = L-BFGS on the semi-dual, 4 = APDAGD (keeps its primal plan in res->plan).
 void solve_qrot(regot_problem* p, int solver, double tol, int max_iter, int verbose,
                 const regot_opts& o, regot_result* res);
 

@@ -136,11 +136,136 @@ static void solve_qrot_semi_dual(regot_problem* p, double tol, int max_iter, int
     HIP_CHECK(hipMemcpy(res->dual.data(), gfull.p, N * sizeof(double), hipMemcpyDeviceToHost));
 }
 
+// APDAGD on the QROT dual (src/qrot_apdagd.cpp:14-181): dual variables
+// lambda = -gamma, L halved per iteration and doubled per failed test (at most
+// 20 inner trials, the last one accepted); the primal plan P (device, n x m
+// FP64, kept in the result) starts at plan(gamma_0) and becomes (1 - tau) P +
+// tau plan(-lambda~) in one fused pass with its sums; stop when <M, P> +
+// reg/2 ||P||^2 - phi <= tol and the full marginal error <= tol.  History:
+// f of the last objective evaluated, the primal value of the rounded plan at
+// -lambda~, mar_errs the full marginal error of P (the first entry ||g|| /
+// reg); niter + 2 entries after a stop.  The dual is -lambda~.
+static void solve_qrot_apdagd(regot_problem* p, double tol, int max_iter, int verbose,
+                              const regot_opts& o, regot_result* res)
+{
+    const int n = p->n, m = p->m;
+    const long N = static_cast<long>(n) + m;
+    hipStream_t s = p->stream;
+    const double reg = p->reg;
+    Work w(p, N);
+    w.og = [&](const double* x, double* g, double* gn2) {
+        return qrot_obj_grad(p, x, g, 0.0, gn2);
+    };
+    if (o.x0) {
+        HIP_CHECK(hipMemcpyAsync(w.gamma, o.x0, N * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+        launch_fill(w.gamma, 0.0, n, s);
+        qrot_optimal(p, true, w.gamma, w.gamma.p + n);
+    }
+    double* dual = w.x.p;
+    double* dualp = w.xg.p;
+    double* dualt = w.rhs.p;
+    double* dual_new = w.y.p;
+    double* dualp_new = w.sv.p;
+    double* gamma = w.gamma.p;
+    double* grad = w.g.p;
+    double* diff = w.step.p;
+    vec_axpby(dual, -1.0, gamma, 0.0, nullptr, N, s);
+    vec_copy(dualp, dual, N, s);
+    vec_copy(dualt, dual, N, s);
+
+    QrotScratch& sc = scratch(p);
+    const int rows = qrot_rows(n, m);
+    const long nstrip = ceil_div(m, 256), nblk = ceil_div(n, rows);
+    DBuf<double> wpart(2 * nstrip * nblk + ceil_div(n > m ? n : m, 256)), pout(3);
+    HBuf<double> hp;
+    hp.ensure(3);
+    res->plan.alloc(static_cast<size_t>(n) * m);
+    res->plan_ld = m;
+    res->plan_device = p->device;
+    auto plan_pass = [&](double tau, bool init) {
+        launch_qrot_apd_plan(p->bulk, p->dM, p->ldm, n, m, gamma, gamma + n, reg, tau, init,
+                             res->plan.p, m, p->a, p->b, sc.rowpart, sc.colpart, sc.rsum, sc.csum,
+                             wpart, pout, s);
+        HIP_CHECK(hipMemcpyAsync(hp.p, pout.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        w.alg_bytes += w.pass_bytes() + 16.0 * n * m;
+    };
+    plan_pass(1.0, true);
+    res->qrot = true;
+    Timer timer;
+    timer.tic();
+    double gnorm;
+    double f = w.obj_grad(gamma, grad, &gnorm);
+    double dur = timer.toc();
+    res->obj_vals.push_back(f);
+    res->prim_vals.push_back(qrot_round_plan(p, gamma, gamma + n, nullptr, 0));
+    res->mar_errs.push_back(gnorm / reg);
+    res->run_times.push_back(dur);
+    double L = 1.0, pbeta = 0.0;
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        timer.tic();
+        L *= 0.5;
+        double ptau = 0.0, phi_new = 0.0;
+        for (int j = 0; j < 20; j++) {
+            const double palpha = (1.0 + std::sqrt(4.0 * L * pbeta + 1.0)) / (2.0 * L);
+            const double pbeta_new = pbeta + palpha;
+            ptau = palpha / pbeta_new;
+            vec_axpby(dualt, 1.0 - ptau, dual, ptau, dualp, N, s);
+            vec_axpby(gamma, -1.0, dualt, 0.0, nullptr, N, s);
+            f = w.obj_grad(gamma, grad, nullptr);
+            const double phit = -f / reg;
+            vec_axpby(dualp_new, 1.0, dualp, palpha / reg, grad, N, s);
+            vec_axpby(dual_new, 1.0 - ptau, dual, ptau, dualp_new, N, s);
+            vec_axpby(gamma, -1.0, dual_new, 0.0, nullptr, N, s);
+            f = w.obj(gamma);
+            phi_new = -f / reg;
+            vec_axpby(diff, 1.0, dual_new, -1.0, dualt, N, s);
+            const double* xs[2] = {grad, diff};
+            const double* ys[2] = {diff, diff};
+            double d[2];
+            w.dots(2, xs, ys, d);
+            const double rhs = phit + d[0] / reg - 0.5 * L * d[1];
+            if (phi_new >= rhs || j >= 19) {
+                pbeta = pbeta_new;
+                std::swap(dualp, dualp_new);
+                std::swap(dual, dual_new);
+                break;
+            }
+            L *= 2.0;
+        }
+        vec_axpby(gamma, -1.0, dualt, 0.0, nullptr, N, s);
+        plan_pass(ptau, false);
+        const double primal_obj = hp.p[0] + 0.5 * reg * hp.p[1];
+        const double mar_err = std::sqrt(hp.p[2]);
+        dur = timer.toc();
+        res->obj_vals.push_back(f);
+        res->prim_vals.push_back(qrot_round_plan(p, gamma, gamma + n, nullptr, 0));
+        res->mar_errs.push_back(mar_err);
+        res->run_times.push_back(res->run_times.back() + dur);
+        if (verbose >= 1) {
+            printf("iter = %d, primal_obj = %g, phi_new = %g, mar_err = %g\n", it, primal_obj,
+                   phi_new, mar_err);
+            fflush(stdout);
+        }
+        if (primal_obj - phi_new <= tol && mar_err <= tol) break;
+    }
+    res->niter = it;
+    w.report(res);
+    res->dual.resize(N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), gamma, N * sizeof(double), hipMemcpyDeviceToHost));
+}
+
 void solve_qrot(regot_problem* p, int solver, double tol, int max_iter, int verbose,
                 const regot_opts& o, regot_result* res)
 {
     if (solver == 3) {
         solve_qrot_semi_dual(p, tol, max_iter, verbose, o, res);
+        return;
+    }
+    if (solver == 4) {
+        solve_qrot_apdagd(p, tol, max_iter, verbose, o, res);
         return;
     }
     const int n = p->n, m = p->m;
@@ -259,7 +384,7 @@ int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int
     CAPI_BEGIN
     if (!p || !out) fail(E_INVALID, "null argument");
     *out = nullptr;
-    if (solver < 0 || solver > 3) fail(E_INVALID, "unknown QROT solver %d", solver);
+    if (solver < 0 || solver > 4) fail(E_INVALID, "unknown QROT solver %d", solver);
     if (max_iter < 0) fail(E_INVALID, "max_iter must be >= 0");
     HIP_CHECK(hipSetDevice(p->device));
     regot_opts o;

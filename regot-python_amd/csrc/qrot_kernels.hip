@@ -302,6 +302,95 @@ __global__ void k_qrot_pl_init(double* xc, int* kprev, int* done, int count, int
     done[t] = 0;
 }
 
+// APDAGD's primal plan: P = (1 - tau) P + tau (alpha_i + beta_j - M_ij)_+ / reg
+// (INIT: P = the plan itself, P not read), written back, with row sums per
+// strip, column sums per block and <M, P>, ||P||^2 per workgroup
+// (mpart, spart[blk * strips + strip]).
+template <typename TM, bool INIT>
+__global__ __launch_bounds__(256) void k_qrot_apd_plan(
+    const TM* __restrict__ M, long ldm, int n, int m, const double* __restrict__ alpha,
+    const double* __restrict__ beta, double reg, double tau, double* __restrict__ P, long ldp,
+    int rows, double* __restrict__ rowpart, double* __restrict__ colpart,
+    double* __restrict__ mpart, double* __restrict__ spart)
+{
+    __shared__ double red[4][258];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int strip = blockIdx.x, blk = blockIdx.y;
+    const int c0 = strip * 256 + lane * 4;
+    const int r0 = blk * rows;
+    const int r1 = min(n, r0 + rows);
+    double bq[4], cs[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) bq[q] = (c0 + q < m) ? beta[c0 + q] : 0.0;
+    double mp = 0.0, sq = 0.0;
+    for (int r = r0 + wave; r < r1; r += 4) {
+        const double ai = alpha[r];
+        double mv[4];
+        load4(M + static_cast<long>(r) * ldm, c0, m, mv);
+        double* Pr = P + static_cast<long>(r) * ldp;
+        double rs = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = c0 + q;
+            if (j < m) {
+                const double c = (ai + bq[q]) - mv[q];
+                const double t = (c > 0.0 ? c : 0.0) / reg;
+                const double v = INIT ? t : (1.0 - tau) * Pr[j] + tau * t;
+                Pr[j] = v;
+                rs += v;
+                cs[q] += v;
+                mp += mv[q] * v;
+                sq += v * v;
+            }
+        }
+        rs = wsum64(rs);
+        if (lane == 0) rowpart[static_cast<long>(strip) * n + r] = rs;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) red[wave][lane * 4 + q] = cs[q];
+    mp = wsum64(mp);
+    sq = wsum64(sq);
+    if (lane == 0) {
+        red[wave][256] = mp;
+        red[wave][257] = sq;
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = c0 + q;
+            const int t = lane * 4 + q;
+            if (j < m)
+                colpart[static_cast<long>(blk) * m + j] =
+                    ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+        }
+        if (lane == 0) {
+            const long wg = static_cast<long>(blk) * gridDim.x + strip;
+            mpart[wg] = ((red[0][256] + red[1][256]) + red[2][256]) + red[3][256];
+            spart[wg] = ((red[0][257] + red[1][257]) + red[2][257]) + red[3][257];
+        }
+    }
+}
+
+// part[w] = sum over the workgroup's slice of (rs - a)^2 (t < n) and
+// (cs - b)^2 (t < m), fixed order.
+__global__ __launch_bounds__(256) void k_qrot_resid2(const double* __restrict__ rs,
+                                                     const double* __restrict__ a, int n,
+                                                     const double* __restrict__ cs,
+                                                     const double* __restrict__ b, int m,
+                                                     double* __restrict__ part)
+{
+    __shared__ double sm[4];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (t < n) v += (rs[t] - a[t]) * (rs[t] - a[t]);
+    if (t < m) v += (cs[t] - b[t]) * (cs[t] - b[t]);
+    v = wsum64(v);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((sm[0] + sm[1]) + sm[2]) + sm[3];
+}
+
 __global__ void k_qrot_zero_int(int* p) { p[0] = 0; }
 
 // out[t] = sum over s of part[s * stride + t] (fixed order), t < count.
@@ -546,6 +635,54 @@ void launch_qrot_plan(int bulk, const void* M, long ldm, int n, int m, const dou
         hipLaunchKernelGGL(k_qrot_plan<double>, dim3(gx, nr), dim3(256), 0, s,
                            static_cast<const double*>(M), ldm, m, alpha, beta, reg, plan, r0);
     KERNEL_CHECK();
+}
+
+// APDAGD plan pass and the marginal residual: out[0] = <M, P>, out[1] =
+// ||P||^2, out[2] = ||P1 - a||^2 + ||P'1 - b||^2.  Scratch: rowpart (strips
+// x n), colpart (blocks x m), rsum (n), csum (m), wpart (2 strips blocks +
+// ceil(max(n, m) / 256)).
+void launch_qrot_apd_plan(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                          const double* beta, double reg, double tau, bool init, double* P,
+                          long ldp, const double* a, const double* b, double* rowpart,
+                          double* colpart, double* rsum, double* csum, double* wpart,
+                          double* out, hipStream_t s)
+{
+    const int rows = qrot_rows(n, m);
+    const int nstrip = ceil_div(m, 256), nblk = ceil_div(n, rows);
+    const dim3 grid(nstrip, nblk);
+    double* mpart = wpart;
+    double* spart = wpart + static_cast<long>(nstrip) * nblk;
+    double* rpart = spart + static_cast<long>(nstrip) * nblk;
+    if (bulk == 0) {
+        const float* Mf = static_cast<const float*>(M);
+        if (init)
+            hipLaunchKernelGGL((k_qrot_apd_plan<float, true>), grid, dim3(256), 0, s, Mf, ldm, n,
+                               m, alpha, beta, reg, tau, P, ldp, rows, rowpart, colpart, mpart,
+                               spart);
+        else
+            hipLaunchKernelGGL((k_qrot_apd_plan<float, false>), grid, dim3(256), 0, s, Mf, ldm,
+                               n, m, alpha, beta, reg, tau, P, ldp, rows, rowpart, colpart, mpart,
+                               spart);
+    } else {
+        const double* Md = static_cast<const double*>(M);
+        if (init)
+            hipLaunchKernelGGL((k_qrot_apd_plan<double, true>), grid, dim3(256), 0, s, Md, ldm,
+                               n, m, alpha, beta, reg, tau, P, ldp, rows, rowpart, colpart, mpart,
+                               spart);
+        else
+            hipLaunchKernelGGL((k_qrot_apd_plan<double, false>), grid, dim3(256), 0, s, Md, ldm,
+                               n, m, alpha, beta, reg, tau, P, ldp, rows, rowpart, colpart,
+                               mpart, spart);
+    }
+    KERNEL_CHECK();
+    launch_qrot_fold(rowpart, nstrip, n, n, rsum, s);
+    launch_qrot_fold(colpart, nblk, m, m, csum, s);
+    const int nw = ceil_div(n > m ? n : m, 256);
+    hipLaunchKernelGGL(k_qrot_resid2, dim3(nw), dim3(256), 0, s, rsum, a, n, csum, b, m, rpart);
+    KERNEL_CHECK();
+    launch_qrot_sum1(mpart, nstrip * nblk, out, s);
+    launch_qrot_sum1(spart, nstrip * nblk, out + 1, s);
+    launch_qrot_sum1(rpart, nw, out + 2, s);
 }
 
 void launch_qrot_fold(const double* part, int nparts, long stride, int count, double* out,

@@ -322,7 +322,7 @@ int regot_result_plan(regot_problem* p, const regot_result* r, double* plan)
     if (!r) return set_error(E_INVALID, "null result");
     if (!p || p->n != r->n || p->m != r->m)
         return set_error(E_INVALID, "problem and result sizes differ");
-    if (r->qrot) {
+    if (r->qrot && !r->plan.p) {
         CAPI_BEGIN
         if (!plan) fail(E_INVALID, "plan is null");
         HIP_CHECK(hipSetDevice(p->device));
@@ -330,7 +330,7 @@ int regot_result_plan(regot_problem* p, const regot_result* r, double* plan)
         CAPI_END
     }
     if (!r->plan.p) return regot_get_plan(p, r->dual.data(), plan);
-    // a solver that keeps its own primal plan (APDAGD)
+    // a solver that keeps its own primal plan (EROT and QROT APDAGD)
     CAPI_BEGIN
     if (!plan) fail(E_INVALID, "plan is null");
     HIP_CHECK(hipSetDevice(r->plan_device));

@@ -29,7 +29,8 @@ from ._lib import RegotError, check, dptr, lib
 __all__ = [
     "sinkhorn_bcd", "sinkhorn_ssns", "sinkhorn_sparse_newton", "sinkhorn_splr",
     "sinkhorn_newton", "sinkhorn_lbfgs_dual", "sinkhorn_apdagd", "sinkhorn_result",
-    "qrot_bcd", "qrot_gd", "qrot_lbfgs_dual", "qrot_lbfgs_semi_dual", "qrot_result",
+    "qrot_bcd", "qrot_gd", "qrot_lbfgs_dual", "qrot_lbfgs_semi_dual", "qrot_apdagd",
+    "qrot_result",
     "Problem", "BatchProblem", "RegotError",
 ]
 
@@ -306,7 +307,7 @@ class Problem:
     def qrot_solve(self, solver, tol=1e-6, max_iter=1000, verbose=0, compute_plan=True,
                    **kwargs):
         """QROT solver (_lib.QROT_BCD / QROT_GD / QROT_LBFGS_DUAL /
-        QROT_LBFGS_SEMI_DUAL) -> qrot_result.
+        QROT_LBFGS_SEMI_DUAL / QROT_APDAGD) -> qrot_result.
         Keywords: x0 (n+m warm start), tau (extra dual term; the reference's
         Python bindings never set it, src/solvers.cpp:33-51)."""
         opts = _lib.Opts()
@@ -704,3 +705,9 @@ def qrot_lbfgs_semi_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwa
     """L-BFGS on the QROT semi-dual (src/solvers.cpp:150-164,
     src/qrot_lbfgs_semi_dual.cpp:78-118)."""
     return _run_qrot(_lib.QROT_LBFGS_SEMI_DUAL, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def qrot_apdagd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """APDAGD on the QROT dual (src/solvers.cpp:53-67, src/qrot_apdagd.cpp:14-181);
+    the result's plan is the averaged primal plan."""
+    return _run_qrot(_lib.QROT_APDAGD, M, a, b, reg, tol, max_iter, verbose, kwargs)

@@ -23,6 +23,7 @@ QROT_BCD = 0
 QROT_GD = 1
 QROT_LBFGS_DUAL = 2
 QROT_LBFGS_SEMI_DUAL = 3
+QROT_APDAGD = 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p

@@ -139,3 +139,26 @@ def test_fused_many_rows_per_workgroup(gpu_lib):
     assert abs(rf.niter - rt.niter) <= 1
     assert rel_err(rf.dual, rt.dual) < 1e-6
     assert rf.mar_err_fp64 < 1.05e-7
+
+
+def test_fused_edge_shapes(gpu_lib):
+    # one row; three rows over the full 16384 columns (one row per workgroup);
+    # max_iter = 0 and an immediately met tolerance: the single-read pass
+    # still runs the stop test of iteration 0.  (Rows carry a_i ~ 1 / n here:
+    # FP32-bulk row sums resolve their marginal to ~1e-7, hence tol 1e-6.)
+    for n, m in ((1, 3000), (3, 16384)):
+        M, a, b = cloud_problem(n, m, seed=n)
+        M32 = M.astype(np.float32)
+        print("fused edge %dx%d" % (n, m), flush=True)
+        r, nf = solve(gpu_lib, M32, a, b, 0.05, 1e-6, 200, fused=1)
+        o = O.sinkhorn_bcd(M32.astype(np.float64), a, b, 0.05, tol=1e-6, max_iter=200)
+        assert nf == r.niter + 1 and abs(r.niter - o["niter"]) <= 1
+        assert rel_err(r.dual, o["dual"]) < 1e-6
+    M, a, b = cloud_problem(50, 2500, seed=3)
+    M32 = M.astype(np.float32)
+    for tol, max_iter in ((1e-8, 0), (1e9, 100)):
+        r, nf = solve(gpu_lib, M32, a, b, 0.05, tol, max_iter, fused=1)
+        o = O.sinkhorn_bcd(M32.astype(np.float64), a, b, 0.05, tol=tol, max_iter=max_iter)
+        assert r.niter == o["niter"] == 0 and nf == 1 and len(r.obj_vals) == 1
+        np.testing.assert_allclose(r.obj_vals, o["obj_vals"], rtol=1e-9)
+        assert rel_err(r.dual, o["dual"]) < 1e-6

@@ -126,3 +126,25 @@ def test_qrot_dropin_contract(gpu_lib, capfd):
     o = O.qrot_gd(M, a, b, 0.1, tol=1e-7, max_iter=3000, tau=0.3, x0=x0)
     assert abs(r.niter - o["niter"]) <= 1
     np.testing.assert_allclose(r.obj_vals[:10], o["obj_vals"][:10], rtol=1e-10)
+
+
+def test_qrot_apdagd(gpu_lib):
+    M, a, b = cloud_problem(40, 30, d=4, seed=3)
+    reg, tol = 0.1, 1e-5
+    print("qrot apdagd 40x30", flush=True)
+    r = gpu_lib.qrot_apdagd(M, a, b, reg, tol=tol, max_iter=20000)
+    o = O.qrot_apdagd(M, a, b, reg, tol=tol, max_iter=20000)
+    assert r.niter < 20000 and abs(r.niter - o["niter"]) <= max(2, 0.02 * o["niter"])
+    assert len(r.obj_vals) == len(r.prim_vals) == r.niter + 2
+    k = min(50, len(r.obj_vals), len(o["obj_vals"]))
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-10)
+    np.testing.assert_allclose(r.prim_vals[:k], o["prim_vals"][:k], rtol=1e-9)
+    np.testing.assert_allclose(r.mar_errs[:k], o["mar_errs"][:k], rtol=1e-8)
+    # the result's plan is the averaged primal plan: its full marginal error
+    # is the last history entry and within tol
+    P = r.plan
+    full = np.sqrt(((P.sum(1) - a) ** 2).sum() + ((P.sum(0) - b) ** 2).sum())
+    assert full <= tol and abs(full - r.mar_errs[-1]) < 1e-12
+    # entries near the ReLU kink switch on and off across thousands of
+    # averaging steps and carry their rounding (~1e-9 absolute here)
+    np.testing.assert_allclose(P, o["plan"], rtol=1e-5, atol=1e-8)

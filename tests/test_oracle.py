@@ -592,3 +592,25 @@ def test_qrot_semi_dual_gradient_is_the_envelope():
     # the result's beta is optimal for its alpha, and the histories line up
     assert np.max(np.abs(r["dual"][9:] - O.qrot_optimal_beta(M, b, reg, r["dual"][:9]))) == 0.0
     assert len(r["obj_vals"]) == len(r["prim_vals"]) == len(r["mar_errs"])
+
+
+def test_qrot_apdagd_reaches_the_qp_optimum():
+    # the averaged primal plan of APDAGD against scipy's SLSQP solution of the
+    # primal QP; the stop rule (gap and full marginal error <= tol) holds
+    from scipy.optimize import minimize
+    M, a, b = cloud_problem(5, 4, d=2, seed=33)
+    reg, tol = 0.1, 1e-8
+    n, m = M.shape
+    cons = [{"type": "eq", "fun": lambda p: p.reshape(n, m).sum(1) - a},
+            {"type": "eq", "fun": lambda p: p.reshape(n, m).sum(0)[:-1] - b[:-1]}]
+    qp = minimize(lambda p: p @ M.ravel() + 0.5 * reg * p @ p, np.outer(a, b).ravel(),
+                  jac=lambda p: M.ravel() + reg * p, bounds=[(0, None)] * (n * m),
+                  constraints=cons, method="SLSQP", options={"ftol": 1e-15, "maxiter": 1000})
+    r = O.qrot_apdagd(M, a, b, reg, tol=tol, max_iter=2000000)
+    P = r["plan"]
+    assert len(r["obj_vals"]) == r["niter"] + 2          # initial entry + every iteration
+    full = np.sqrt(((P.sum(1) - a) ** 2).sum() + ((P.sum(0) - b) ** 2).sum())
+    assert full <= tol and abs(full - r["mar_errs"][-1]) < 1e-15
+    assert np.max(np.abs(P - qp.x.reshape(n, m))) < 1e-5
+    primal = (P * M).sum() + 0.5 * reg * (P * P).sum()
+    assert abs(primal - qp.fun) < 1e-7
