@@ -334,7 +334,9 @@ def bench_qrot(n):
     mar_err <= 1e-6 (at most 500 iterations), and 20 iterations each of BCD
     (two root solves per iteration) and gradient descent; every iteration
     includes the primal value of the rounded plan, as the reference's
-    histories do.  One warmup and one timed solve each."""
+    histories do; 5 GRSSN (semismooth Newton) iterations with the reference's
+    defaults (Schur-complement CG to 1e-8, shift 1e-3) on the bitmask
+    Hessian.  One warmup and one timed solve each."""
     import regot
     from regot import _lib
     M, a, b = make_problem(n, n, 16, 0)
@@ -342,7 +344,8 @@ def bench_qrot(n):
     with regot.Problem(M, a, b, 0.1) as P:
         for name, sid, tol, mi in (("lbfgs_dual", _lib.QROT_LBFGS_DUAL, 1e-6, 500),
                                    ("bcd", _lib.QROT_BCD, 1e-6, 20),
-                                   ("gd", _lib.QROT_GD, 1e-6, 20)):
+                                   ("gd", _lib.QROT_GD, 1e-6, 20),
+                                   ("grssn", _lib.QROT_GRSSN, 1e-6, 5)):
             P.qrot_solve(sid, tol=tol, max_iter=mi, compute_plan=False)
             t0 = time.perf_counter()
             r = P.qrot_solve(sid, tol=tol, max_iter=mi, compute_plan=False)

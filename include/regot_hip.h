@@ -284,10 +284,11 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
  *   f = 0.5 ||(alpha (+) beta - M)_+||^2 - reg (a'alpha + b'beta) + tau/2 (1'alpha - 1'beta)^2.
  * regot_qrot_solve replaces qrot_bcd_internal, qrot_gd_internal,
  * qrot_lbfgs_dual_internal, qrot_lbfgs_semi_dual_internal and
- * qrot_apdagd_internal (src/qrot_apdagd.cpp:14-181)
+ * qrot_apdagd_internal (src/qrot_apdagd.cpp:14-181) and qrot_grssn_internal
+ * (src/qrot_grssn.cpp:16-133)
  * (src/qrot_bcd.cpp:15-101, src/qrot_gd.cpp:15-121,
  * src/qrot_lbfgs_dual.cpp:85-126, src/qrot_lbfgs_semi_dual.cpp:78-118;
- * bindings src/solvers.cpp:85-164).  Its result
+ * bindings src/solvers.cpp:53-164).  Its result
  * carries prim_vals (regot_result_prim_vals, one per history entry) and a
  * dual of n + m entries (regot_result_dual_len); mar_errs are ||grad|| / reg;
  * regot_result_plan gives (alpha (+) beta - M)_+ / reg.  opts may be NULL;
@@ -300,9 +301,23 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
 #define REGOT_QROT_LBFGS_SEMI_DUAL 3   /* src/qrot_lbfgs_semi_dual.cpp:78-118 */
 #define REGOT_QROT_APDAGD 4            /* src/qrot_apdagd.cpp:14-181; the result keeps its
                                           averaged primal plan (regot_result_plan) */
+#define REGOT_QROT_GRSSN 5             /* src/qrot_grssn.cpp:16-133; reads opts->shift
+                                          (default 1e-3), opts->method (0, the default
+                                          with NULL opts: CG to 1e-8; other values name
+                                          the reference's direct factorisations, run as
+                                          CG to 1e-12), opts->cg_tol / cg_max_iter */
 int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,
                      const regot_opts* opts, regot_result** out);
 int regot_result_prim_vals(const regot_result* r, double* out);
+/* QROT generalised Hessian solve at host gamma (n + m): out = (H + shift I +
+ * tau vv')^-1 rhs, v = [1 (n); -1 (m)], H = [diag(sigma 1) sigma; sigma'
+ * diag(sigma' 1)], sigma_ij = [alpha_i + beta_j - M_ij > 0]: the operator
+ * behind QROT::Hessian::compute_D_hess and hess_cg (src/qrot_hess.cpp:42-96,
+ * src/qrot_cg.cpp:9-51; CG to tol with at most max_iter steps, <= 0: 2m);
+ * *cg_iters (may be NULL) receives the CG steps. */
+int regot_qrot_hess_solve(regot_problem* p, const double* gamma, const double* rhs,
+                          double shift, double tau, double tol, int max_iter, double* out,
+                          int* cg_iters);
 int regot_result_dual_len(const regot_result* r);
 /* QROT Problem::dual_obj_grad (src/qrot_problem.cpp:226-258): f and grad (n + m,
  * may be NULL) at host gamma (n + m). */

@@ -167,6 +167,15 @@ int orc_qrot_solve(int solver, int n, int m, const double* M, const double* a, c
                    double reg, double tau, double tol, int max_iter, int verbose, int hist_cap,
                    const double* x0, double* x, double* obj_vals, double* prim_vals,
                    double* mar_errs, double* run_times, int* hist_len, double* plan);
+/* (H + shift I + tau vv')^-1 rhs with the generalised Hessian at gamma;
+ * returns the CG steps. */
+int orc_qrot_hess_solve(int n, int m, const double* M, const double* gamma, const double* rhs,
+                        double shift, double tau, double tol, int max_iter, double* out);
+int orc_qrot_grssn(int n, int m, const double* M, const double* a, const double* b, double reg,
+                   double tau, double shift, int method, double tol, int max_iter, int verbose,
+                   int hist_cap, const double* x0, double* x, double* obj_vals,
+                   double* prim_vals, double* mar_errs, double* run_times, int* hist_len,
+                   int* cg_iters);
 
 /* BCD iterations only (no convergence test), for timing a bounded CPU sample. */
 double orc_bcd_iterations(int n, int m, const double* M, const double* a, const double* b,

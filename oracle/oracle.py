@@ -497,3 +497,38 @@ def qrot_lbfgs_semi_dual(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, x0=No
 def qrot_apdagd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, x0=None):
     """src/qrot_apdagd.cpp:14-181 -> dict with the averaged primal plan under "plan" """
     return _qrot_solve(4, M, a, b, reg, tol, max_iter, verbose, 0.0, x0)
+
+
+def qrot_hess_solve(M, gamma, rhs, shift, tau=0.0, tol=1e-8, max_iter=None):
+    """(H + shift I + tau vv')^-1 rhs, H the generalised Hessian at gamma
+    (src/qrot_hess.cpp:42-96, 130-187; src/qrot_cg.cpp:9-51) -> (x, cg steps)"""
+    M, gamma, rhs = _d(M), _d(gamma), _d(rhs)
+    n, m = M.shape
+    out = np.empty(n + m)
+    it = lib().orc_qrot_hess_solve(n, m, _p(M), _p(gamma), _p(rhs), ctypes.c_double(shift),
+                                   ctypes.c_double(tau), ctypes.c_double(tol),
+                                   int(2 * m if max_iter is None else max_iter), _p(out))
+    return out, it
+
+
+def qrot_grssn(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, shift=1e-3, method=0,
+               tau=0.0, x0=None):
+    """src/qrot_grssn.cpp:16-133 -> dict (+ "cg_iters" per Newton step)"""
+    M, a, b = _d(M), _d(a), _d(b)
+    n, m = M.shape
+    cap = max_iter + 2
+    dual = np.empty(n + m)
+    h = [np.empty(cap) for _ in range(4)]
+    cg = np.zeros(cap, dtype=np.int32)
+    hl = ctypes.c_int(0)
+    x0p = _p(_d(x0)) if x0 is not None else None
+    niter = lib().orc_qrot_grssn(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg),
+                                 ctypes.c_double(tau), ctypes.c_double(shift), int(method),
+                                 ctypes.c_double(tol), int(max_iter), int(verbose), cap, x0p,
+                                 _p(dual), _p(h[0]), _p(h[1]), _p(h[2]), _p(h[3]),
+                                 ctypes.byref(hl), cg.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    _check(niter)
+    k = hl.value
+    return dict(niter=niter, dual=dual, plan=qrot_get_plan(M, reg, dual),
+                obj_vals=list(h[0][:k]), prim_vals=list(h[1][:k]), mar_errs=list(h[2][:k]),
+                run_times=list(h[3][:k]), cg_iters=list(cg[:niter]))

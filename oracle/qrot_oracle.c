@@ -417,3 +417,252 @@ int orc_qrot_solve(int solver, int n, int m, const double* M, const double* a, c
     *hist_len = Q.h;
     return it;
 }
+
+/* ---- Generalised Hessian and GRSSN ----
+ * H = [diag(h1) sigma; sigma' diag(h2)], sigma_ij = [alpha_i + beta_j - M_ij
+ * > 0], h1 = sigma 1, h2 = sigma' 1 (src/qrot_hess.h:11-36,
+ * src/qrot_hess.cpp:42-96).  (H + shift I + tau vv') x = rhs, v = [1; -1], is
+ * solved by block elimination with CG on the Schur complement
+ * Delta = diag(h2) + shift I - C A^-1 B over beta (src/qrot_hess.cpp:130-187,
+ * src/qrot_cg.cpp:9-51). */
+typedef struct {
+    int n, m;
+    unsigned char* sig; /* n x m, row-major */
+    double *h1, *h2, *tn, *tm, *tn2;
+    double shift, tau;
+} qrot_hess;
+
+static void qh_build(qrot_hess* H, int n, int m, const double* M, const double* gamma)
+{
+    H->n = n;
+    H->m = m;
+    H->sig = (unsigned char*)malloc((size_t)n * m);
+    H->h1 = (double*)calloc((size_t)n, sizeof(double));
+    H->h2 = (double*)calloc((size_t)m, sizeof(double));
+    H->tn = (double*)malloc(sizeof(double) * (size_t)n);
+    H->tn2 = (double*)malloc(sizeof(double) * (size_t)n);
+    H->tm = (double*)malloc(sizeof(double) * (size_t)m);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m; j++) {
+            const int on = gamma[i] + gamma[n + j] - M[(long)i * m + j] > 0.0;
+            H->sig[(long)i * m + j] = (unsigned char)on;
+            H->h1[i] += on;
+            H->h2[j] += on;
+        }
+}
+
+static void qh_free(qrot_hess* H)
+{
+    free(H->sig);
+    free(H->h1);
+    free(H->h2);
+    free(H->tn);
+    free(H->tn2);
+    free(H->tm);
+}
+
+static double q_sum(const double* x, long n)
+{
+    double s = 0.0;
+    for (long i = 0; i < n; i++) s += x[i];
+    return s;
+}
+
+/* B x = sigma x - tau (1'x) 1 (n); x has m entries */
+static void qh_Bx(const qrot_hess* H, const double* x, double* res)
+{
+    const double c = H->tau > 0.0 ? H->tau * q_sum(x, H->m) : 0.0;
+    for (int i = 0; i < H->n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < H->m; j++)
+            if (H->sig[(long)i * H->m + j]) s += x[j];
+        res[i] = s - c;
+    }
+}
+
+/* C x = sigma' x - tau (1'x) 1 (m); x has n entries */
+static void qh_Cx(const qrot_hess* H, const double* x, double* res)
+{
+    const double c = H->tau > 0.0 ? H->tau * q_sum(x, H->n) : 0.0;
+    for (int j = 0; j < H->m; j++) res[j] = 0.0;
+    for (int i = 0; i < H->n; i++)
+        for (int j = 0; j < H->m; j++)
+            if (H->sig[(long)i * H->m + j]) res[j] += x[i];
+    for (int j = 0; j < H->m; j++) res[j] -= c;
+}
+
+/* A^-1 x with A = diag(h1 + shift) + tau 11' (Sherman-Morrison when tau > 0) */
+static void qh_solve_A(const qrot_hess* H, const double* x, double* res)
+{
+    const int n = H->n;
+    if (H->tau > 0.0) {
+        double c1 = 1.0 / H->tau, c2 = 0.0;
+        for (int i = 0; i < n; i++) {
+            const double h = H->h1[i] + H->shift;
+            res[i] = x[i] / h;
+            c1 += 1.0 / h;
+            c2 += res[i];
+        }
+        for (int i = 0; i < n; i++) res[i] -= (c2 / c1) / (H->h1[i] + H->shift);
+    } else {
+        for (int i = 0; i < n; i++) res[i] = x[i] / (H->h1[i] + H->shift);
+    }
+}
+
+/* Delta x = h2 .* x + shift x - C A^-1 B x + tau (1'x) 1 */
+static void qh_delta(const qrot_hess* H, const double* x, double* res)
+{
+    qh_Bx(H, x, H->tn);
+    qh_solve_A(H, H->tn, H->tn2);
+    qh_Cx(H, H->tn2, H->tm);
+    const double c = H->tau > 0.0 ? H->tau * q_sum(x, H->m) : 0.0;
+    for (int j = 0; j < H->m; j++) res[j] = H->h2[j] * x[j] + H->shift * x[j] - H->tm[j] + c;
+}
+
+/* Conjugate gradient with an identity preconditioner from x = 0, stopping at
+ * ||r||^2 < max(tol^2 ||rhs||^2, DBL_MIN) or after max_iter steps (the rules
+ * of Eigen::ConjugateGradient that src/qrot_cg.cpp:31-38 calls; max_iter
+ * defaults to 2m there).  Returns the number of steps. */
+static int qh_cg(qrot_hess* H, const double* rhs, double tol, int max_iter, double* x)
+{
+    const int m = H->m;
+    double* r = (double*)malloc(sizeof(double) * (size_t)m);
+    double* p = (double*)malloc(sizeof(double) * (size_t)m);
+    double* q = (double*)malloc(sizeof(double) * (size_t)m);
+    double rhs2 = 0.0;
+    for (int j = 0; j < m; j++) {
+        x[j] = 0.0;
+        r[j] = rhs[j];
+        p[j] = rhs[j];
+        rhs2 += rhs[j] * rhs[j];
+    }
+    int it = 0;
+    if (rhs2 > 0.0) {
+        double thr = tol * tol * rhs2;
+        if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+        double rr = rhs2;
+        if (rr >= thr) {
+            double abs_new = rr;
+            while (it < max_iter) {
+                qh_delta(H, p, q);
+                double pq = 0.0;
+                for (int j = 0; j < m; j++) pq += p[j] * q[j];
+                const double al = abs_new / pq;
+                rr = 0.0;
+                for (int j = 0; j < m; j++) {
+                    x[j] += al * p[j];
+                    r[j] -= al * q[j];
+                    rr += r[j] * r[j];
+                }
+                if (rr < thr) break;
+                const double bt = rr / abs_new;
+                abs_new = rr;
+                for (int j = 0; j < m; j++) p[j] = r[j] + bt * p[j];
+                it++;
+            }
+        }
+    }
+    free(r);
+    free(p);
+    free(q);
+    return it;
+}
+
+/* (H + shift I + tau vv')^-1 rhs by block elimination (src/qrot_cg.cpp:9-51):
+ * v = A^-1 w, r2 = Delta^-1 (z - C v) by CG, r1 = A^-1 (w - B r2). */
+static int qh_solve(qrot_hess* H, const double* rhs, double tol, int max_iter, double* out)
+{
+    const int n = H->n, m = H->m;
+    double* v = (double*)malloc(sizeof(double) * (size_t)n);
+    double* zy = (double*)malloc(sizeof(double) * (size_t)m);
+    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    qh_solve_A(H, rhs, v);
+    qh_Cx(H, v, zy);
+    for (int j = 0; j < m; j++) zy[j] = rhs[n + j] - zy[j];
+    const int it = qh_cg(H, zy, tol, max_iter, out + n);
+    qh_Bx(H, out + n, w);
+    for (int i = 0; i < n; i++) w[i] = rhs[i] - w[i];
+    qh_solve_A(H, w, out);
+    free(v);
+    free(zy);
+    free(w);
+    return it;
+}
+
+int orc_qrot_hess_solve(int n, int m, const double* M, const double* gamma, const double* rhs,
+                        double shift, double tau, double tol, int max_iter, double* out)
+{
+    qrot_hess H;
+    qh_build(&H, n, m, M, gamma);
+    H.shift = shift;
+    H.tau = tau;
+    const int it = qh_solve(&H, rhs, tol, max_iter, out);
+    qh_free(&H);
+    return it;
+}
+
+/* GRSSN (src/qrot_grssn.cpp:16-133): direction (H + shift I + tau vv')^-1
+ * (-g) by the CG above (tol 1e-8, at most 2m steps; method != 0 asks the
+ * reference for a direct factorisation, an exact solve, here CG to 1e-12 with
+ * at most 10m + 1000 steps), Armijo backtracking from step 1 (theta = 0.5,
+ * kappa = 0.5, 20 trials, the last trial taken), histories niter + 1 long;
+ * cg_iters[k] receives the CG steps of iteration k. */
+int orc_qrot_grssn(int n, int m, const double* M, const double* a, const double* b, double reg,
+                   double tau, double shift, int method, double tol, int max_iter, int verbose,
+                   int hist_cap, const double* x0, double* x, double* obj_vals,
+                   double* prim_vals, double* mar_errs, double* run_times, int* hist_len,
+                   int* cg_iters)
+{
+    const long N = (long)n + m;
+    qrot_ctx Q = {n, m, M, a, b, reg, tau, 0, hist_cap, verbose, 0.0,
+                  obj_vals, prim_vals, mar_errs, run_times};
+    if (x0) {
+        memcpy(x, x0, sizeof(double) * N);
+    } else {
+        memset(x, 0, sizeof(double) * n);
+        orc_qrot_optimal_beta(n, m, M, b, reg, x, x + n);
+    }
+    const double cg_tol = method == 0 ? 1e-8 : 1e-12;
+    const int cg_max = method == 0 ? 2 * m : 10 * m + 1000;
+    double* grad = (double*)malloc(sizeof(double) * N);
+    double* rhs = (double*)malloc(sizeof(double) * N);
+    double* dir = (double*)malloc(sizeof(double) * N);
+    double* xn = (double*)malloc(sizeof(double) * N);
+    Q.t0 = q_now_ms();
+    double f = qrot_objgrad(&Q, x, grad);
+    double gnorm = q_norm2(grad, N);
+    qrot_record(&Q, x, f, gnorm);
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        if (verbose >= 1) printf("iter = %d, objval = %g, ||grad|| = %g\n", it, f, gnorm);
+        if (gnorm < tol) break;
+        qrot_hess H;
+        qh_build(&H, n, m, M, x);
+        H.shift = shift;
+        H.tau = tau;
+        for (long q = 0; q < N; q++) rhs[q] = -grad[q];
+        const int cgi = qh_solve(&H, rhs, cg_tol, cg_max, dir);
+        qh_free(&H);
+        if (cg_iters && it < hist_cap) cg_iters[it] = cgi;
+        double gd = 0.0;
+        for (long q = 0; q < N; q++) gd += grad[q] * dir[q];
+        const double thresh = 0.5 * gd;
+        double step = 1.0;
+        for (int k = 0; k < 20; k++) {
+            for (long q = 0; q < N; q++) xn[q] = x[q] + step * dir[q];
+            const double fn = qrot_objgrad(&Q, xn, NULL);
+            if (fn <= f + step * thresh) break;
+            step *= 0.5;
+        }
+        memcpy(x, xn, sizeof(double) * N);
+        f = qrot_objgrad(&Q, x, grad);
+        gnorm = q_norm2(grad, N);
+        qrot_record(&Q, x, f, gnorm);
+    }
+    free(grad);
+    free(rhs);
+    free(dir);
+    free(xn);
+    *hist_len = Q.h;
+    return it;
+}

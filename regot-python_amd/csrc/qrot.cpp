@@ -74,6 +74,233 @@ void qrot_optimal(regot_problem* p, bool cols, const double* pot, double* out)
     fail(E_INTERNAL, "QROT root iteration did not settle");
 }
 
+// ---- Generalised Hessian (src/qrot_hess.cpp, src/qrot_cg.cpp) ----
+
+namespace {
+
+// k dots of length len on the stream, results on the host
+void qdots(QrotHess& H, hipStream_t s, long len, int k, const double* const* a,
+           const double* const* b, double* out)
+{
+    vec_dots(a, b, len, k, H.dsc, s);
+    HIP_CHECK(hipMemcpyAsync(H.hs.p, H.dsc.p, k * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (int q = 0; q < k; q++) out[q] = H.hs.p[q];
+}
+
+double qsum(QrotHess& H, hipStream_t s, long len, const double* x)
+{
+    const double* a[1] = {x};
+    const double* b[1] = {H.ones.p};
+    double r;
+    qdots(H, s, len, 1, a, b, &r);
+    return r;
+}
+
+// out (n) = B x = sigma x - tau (1'x) 1, x (m)
+void apply_B(regot_problem* p, QrotHess& H, const double* x, double tau, double* out)
+{
+    const int n = p->n, m = p->m;
+    QrotScratch& w = scratch(p);
+    launch_qrot_sig(H.mask, n, m, true, false, nullptr, x, w.rowpart, nullptr, p->stream);
+    launch_qrot_fold(w.rowpart, ceil_div(m, 256), n, n, out, p->stream);
+    if (tau > 0.0) {
+        const double c = tau * qsum(H, p->stream, m, x);
+        vec_axpby(out, 1.0, out, -c, H.ones, n, p->stream);
+    }
+}
+
+// out (m) = C x = sigma' x - tau (1'x) 1, x (n)
+void apply_C(regot_problem* p, QrotHess& H, const double* x, double tau, double* out)
+{
+    const int n = p->n, m = p->m;
+    QrotScratch& w = scratch(p);
+    launch_qrot_sig(H.mask, n, m, false, true, x, nullptr, nullptr, w.colpart, p->stream);
+    launch_qrot_fold(w.colpart, ceil_div(n, qrot_rows(n, m)), m, m, out, p->stream);
+    if (tau > 0.0) {
+        const double c = tau * qsum(H, p->stream, n, x);
+        vec_axpby(out, 1.0, out, -c, H.ones, m, p->stream);
+    }
+}
+
+// out (n) = A^-1 x, A = diag(h1 + shift) + tau 11' (src/qrot_hess.cpp:130-146)
+void solve_A(regot_problem* p, QrotHess& H, const double* x, double shift, double tau,
+             double* out)
+{
+    const int n = p->n;
+    hipStream_t s = p->stream;
+    double c = 0.0;
+    if (tau > 0.0) {
+        if (H.hinv_shift != shift) {
+            launch_qrot_recip(H.h1, shift, H.hinv, n, s);
+            H.hinv_shift = shift;
+        }
+        const double* a[2] = {H.hinv.p, x};
+        const double* b[2] = {H.ones.p, H.hinv.p};
+        double d[2];
+        qdots(H, s, n, 2, a, b, d);
+        c = d[1] / (1.0 / tau + d[0]);
+    }
+    launch_qrot_ainv(x, H.h1, shift, c, out, n, s);
+}
+
+// out (m) = Delta x = h2 .* x + shift x - C A^-1 B x + tau (1'x) 1
+void apply_Delta(regot_problem* p, QrotHess& H, const double* x, double shift, double tau,
+                 double* out)
+{
+    const int m = p->m;
+    apply_B(p, H, x, tau, H.tn);
+    solve_A(p, H, H.tn, shift, tau, H.tn2);
+    apply_C(p, H, H.tn2, tau, H.tm2);
+    const double c = tau > 0.0 ? tau * qsum(H, p->stream, m, x) : 0.0;
+    launch_qrot_delta_fin(x, H.h2, shift, H.tm2, c, out, m, p->stream);
+}
+
+}  // namespace
+
+QrotHess& qrot_hess_build(regot_problem* p, const double* gamma)
+{
+    const int n = p->n, m = p->m;
+    hipStream_t s = p->stream;
+    QrotScratch& w = scratch(p);
+    if (!w.hess) {
+        w.hess = std::make_unique<QrotHess>(n, m);
+        launch_fill(w.hess->ones, 1.0, n > m ? n : m, s);
+    }
+    QrotHess& H = *w.hess;
+    launch_qrot_mask(p->bulk, p->dM, p->ldm, n, m, gamma, gamma + n, H.mask, s);
+    launch_qrot_sig(H.mask, n, m, true, true, nullptr, nullptr, w.rowpart, w.colpart, s);
+    launch_qrot_fold(w.rowpart, ceil_div(m, 256), n, n, H.h1, s);
+    launch_qrot_fold(w.colpart, ceil_div(n, qrot_rows(n, m)), m, m, H.h2, s);
+    H.hinv_shift = -1.0;
+    return H;
+}
+
+// Block elimination of src/qrot_cg.cpp:9-51 and Eigen's ConjugateGradient
+// (identity preconditioner) on Delta: x = 0, r = p = rhs; stop when ||r||^2 <
+// max(tol^2 ||rhs||^2, DBL_MIN) or after max_iter steps.
+int qrot_hess_solve(regot_problem* p, QrotHess& H, const double* rhs, double shift, double tau,
+                    double tol, int max_iter, double* out)
+{
+    const int n = p->n, m = p->m;
+    hipStream_t s = p->stream;
+    double* x = out + n;
+    solve_A(p, H, rhs, shift, tau, H.v);
+    apply_C(p, H, H.v, tau, H.tm);
+    vec_axpby(H.r, 1.0, rhs + n, -1.0, H.tm, m, s);     // r = z - C v (the CG rhs)
+    vec_copy(H.pv, H.r, m, s);
+    launch_fill(x, 0.0, m, s);
+    double rhs2;
+    {
+        const double* a[1] = {H.r.p};
+        qdots(H, s, m, 1, a, a, &rhs2);
+    }
+    int it = 0;
+    if (rhs2 > 0.0) {
+        const double thr = std::max(tol * tol * rhs2, 2.2250738585072014e-308);
+        double abs_new = rhs2;
+        if (rhs2 >= thr) {
+            while (it < max_iter) {
+                apply_Delta(p, H, H.pv, shift, tau, H.q);
+                double pq;
+                {
+                    const double* a[1] = {H.pv.p};
+                    const double* b[1] = {H.q.p};
+                    qdots(H, s, m, 1, a, b, &pq);
+                }
+                const double al = abs_new / pq;
+                vec_axpby(x, 1.0, x, al, H.pv, m, s);
+                vec_axpby(H.r, 1.0, H.r, -al, H.q, m, s);
+                double rr;
+                {
+                    const double* a[1] = {H.r.p};
+                    qdots(H, s, m, 1, a, a, &rr);
+                }
+                if (rr < thr) break;
+                const double bt = rr / abs_new;
+                abs_new = rr;
+                vec_axpby(H.pv, 1.0, H.r, bt, H.pv, m, s);
+                it++;
+            }
+        }
+    }
+    // r1 = A^-1 (w - B r2)
+    apply_B(p, H, x, tau, H.tn);
+    vec_axpby(H.tn, 1.0, rhs, -1.0, H.tn, n, s);
+    solve_A(p, H, H.tn, shift, tau, out);
+    return it;
+}
+
+// GRSSN (src/qrot_grssn.cpp:16-133): direction (H + shift I + tau vv')^-1 (-g)
+// by qrot_hess_solve (method 0: tol 1e-8 and at most 2m CG steps, the
+// reference's CG; other methods name its direct factorisations, an exact
+// solve, run here as CG to 1e-12 with at most 10m + 1000 steps; cg_tol /
+// cg_max_iter override), Armijo backtracking from 1 (theta = kappa = 0.5, 20
+// trials, the last one taken); test ||g|| < tol before each step.
+static void solve_qrot_grssn(regot_problem* p, double tol, int max_iter, int verbose,
+                             const regot_opts& o, regot_result* res)
+{
+    const int n = p->n, m = p->m;
+    const long N = static_cast<long>(n) + m;
+    hipStream_t s = p->stream;
+    const double reg = p->reg, tau = o.tau, shift = o.shift;
+    const double cg_tol = o.cg_tol > 0.0 ? o.cg_tol : (o.method == 0 ? 1e-8 : 1e-12);
+    const int cg_max = o.cg_max_iter > 0 ? o.cg_max_iter : (o.method == 0 ? 2 * m : 10 * m + 1000);
+    Work w(p, N);
+    w.og = [&](const double* x, double* g, double* gn2) {
+        return qrot_obj_grad(p, x, g, tau, gn2);
+    };
+    if (o.x0) {
+        HIP_CHECK(hipMemcpyAsync(w.gamma, o.x0, N * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+        launch_fill(w.gamma, 0.0, n, s);
+        qrot_optimal(p, true, w.gamma, w.gamma.p + n);
+    }
+    res->qrot = true;
+    Timer timer;
+    timer.tic();
+    auto record = [&](const double* x, double f, double gnorm) {
+        const double prim = qrot_round_plan(p, x, x + n, nullptr, 0);
+        const double dur = timer.toc();
+        res->obj_vals.push_back(f);
+        res->prim_vals.push_back(prim);
+        res->mar_errs.push_back(gnorm / reg);
+        res->run_times.push_back((res->run_times.empty() ? 0.0 : res->run_times.back()) + dur);
+        timer.tic();
+    };
+    double gnorm;
+    double f = w.obj_grad(w.gamma, w.g, &gnorm);
+    record(w.gamma, f, gnorm);
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        if (verbose >= 1) {
+            printf("iter = %d, objval = %g, ||grad|| = %g\n", it, f, gnorm);
+            fflush(stdout);
+        }
+        if (gnorm < tol) break;
+        QrotHess& H = qrot_hess_build(p, w.gamma);
+        vec_axpby(w.rhs, -1.0, w.g, 0.0, nullptr, N, s);
+        const int cgi = qrot_hess_solve(p, H, w.rhs, shift, tau, cg_tol, cg_max, w.direc);
+        w.cg_iters += cgi;
+        w.alg_bytes += w.pass_bytes() + (2.0 * (cgi + 2)) * (n * (m / 8.0));
+        const double thresh = 0.5 * w.dot(w.g, w.direc);
+        double step = 1.0;
+        for (int k = 0; k < 20; k++) {
+            w.move(w.x, w.gamma, step, w.direc);
+            const double fn = w.obj(w.x);
+            if (fn <= f + step * thresh) break;
+            step *= 0.5;
+        }
+        std::swap(w.gamma, w.x);
+        f = w.obj_grad(w.gamma, w.g, &gnorm);
+        record(w.gamma, f, gnorm);
+    }
+    res->niter = it;
+    w.report(res);
+    res->dual.resize(N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), w.gamma.p, N * sizeof(double), hipMemcpyDeviceToHost));
+}
+
 // Drivers.  Semantics: src/qrot_bcd.cpp:15-101 (tau forced to 0),
 // src/qrot_gd.cpp:15-121, src/qrot_lbfgs_dual.cpp:85-126 (epsilon =
 // epsilon_rel = tol).  Histories: obj_vals f, prim_vals the primal value of
@@ -268,6 +495,10 @@ void solve_qrot(regot_problem* p, int solver, double tol, int max_iter, int verb
         solve_qrot_apdagd(p, tol, max_iter, verbose, o, res);
         return;
     }
+    if (solver == 5) {
+        solve_qrot_grssn(p, tol, max_iter, verbose, o, res);
+        return;
+    }
     const int n = p->n, m = p->m;
     const long N = static_cast<long>(n) + m;
     hipStream_t s = p->stream;
@@ -384,7 +615,7 @@ int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int
     CAPI_BEGIN
     if (!p || !out) fail(E_INVALID, "null argument");
     *out = nullptr;
-    if (solver < 0 || solver > 4) fail(E_INVALID, "unknown QROT solver %d", solver);
+    if (solver < 0 || solver > 5) fail(E_INVALID, "unknown QROT solver %d", solver);
     if (max_iter < 0) fail(E_INVALID, "max_iter must be >= 0");
     HIP_CHECK(hipSetDevice(p->device));
     regot_opts o;
@@ -393,7 +624,9 @@ int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int
     } else {
         o = regot_opts{};
         regot_opts_default(&o, 0);
+        o.method = 0;   // QROTSolverOpts' default linear solver: CG
     }
+    if (solver == 5 && !(o.shift >= 0.0)) fail(E_INVALID, "shift must be >= 0");
     auto res = std::make_unique<regot_result>();
     res->solver = 100 + solver;
     res->n = p->n;
@@ -450,6 +683,26 @@ int regot_qrot_primal_val(regot_problem* p, const double* gamma, double* value)
     DBuf<double> g(N);
     HIP_CHECK(hipMemcpyAsync(g.p, gamma, N * sizeof(double), hipMemcpyHostToDevice, p->stream));
     *value = qrot_round_plan(p, g.p, g.p + p->n, nullptr, 0);
+    CAPI_END
+}
+
+int regot_qrot_hess_solve(regot_problem* p, const double* gamma, const double* rhs,
+                          double shift, double tau, double tol, int max_iter, double* out,
+                          int* cg_iters)
+{
+    CAPI_BEGIN
+    if (!p || !gamma || !rhs || !out) fail(E_INVALID, "null argument");
+    if (!(shift >= 0.0) || !(tau >= 0.0)) fail(E_INVALID, "shift and tau must be >= 0");
+    HIP_CHECK(hipSetDevice(p->device));
+    const long N = static_cast<long>(p->n) + p->m;
+    DBuf<double> g(N), r(N), x(N);
+    HIP_CHECK(hipMemcpyAsync(g.p, gamma, N * sizeof(double), hipMemcpyHostToDevice, p->stream));
+    HIP_CHECK(hipMemcpyAsync(r.p, rhs, N * sizeof(double), hipMemcpyHostToDevice, p->stream));
+    QrotHess& H = qrot_hess_build(p, g.p);
+    const int it = qrot_hess_solve(p, H, r.p, shift, tau, tol,
+                                   max_iter > 0 ? max_iter : 2 * p->m, x.p);
+    HIP_CHECK(hipMemcpy(out, x.p, N * sizeof(double), hipMemcpyDeviceToHost));
+    if (cg_iters) *cg_iters = it;
     CAPI_END
 }
 

@@ -4,11 +4,42 @@
 // gamma = [alpha (n); beta (m)]: no elimination of beta_m.
 #pragma once
 
+#include <memory>
+
 #include "problem.hpp"
 
 namespace regot {
 
 int qrot_rows(int n, int m);
+
+// The generalised Hessian H = [diag(h1) sigma; sigma' diag(h2)] at one gamma
+// (src/qrot_hess.h:11-36): sigma as a bitmask (launch_qrot_mask), h1 = sigma 1,
+// h2 = sigma' 1, plus the vectors of its block solve.
+struct QrotHess {
+    DBuf<unsigned long long> mask;
+    DBuf<double> h1, h2, hinv, tn, tn2, tm, tm2, ones, v, r, pv, q, dsc;
+    HBuf<double> hs;
+    double hinv_shift = -1.0;
+    QrotHess(int n, int m)
+    {
+        mask.alloc(static_cast<size_t>(n) * ceil_div(m, 256) * 4);
+        const int c = n > m ? n : m;
+        h1.alloc(n);
+        h2.alloc(m);
+        hinv.alloc(n);
+        tn.alloc(n);
+        tn2.alloc(n);
+        v.alloc(n);
+        tm.alloc(m);
+        tm2.alloc(m);
+        r.alloc(m);
+        pv.alloc(m);
+        q.alloc(m);
+        ones.alloc(c);
+        dsc.alloc(8);
+        hs.ensure(8);
+    }
+};
 
 // Device scratch of the QROT passes and root iterations, sized for a shape;
 // owned by the problem handle.
@@ -19,6 +50,7 @@ struct QrotScratch {
     DBuf<int> kpart, kprev, done, open;
     HBuf<double> hout;
     HBuf<int> hopen;
+    std::unique_ptr<QrotHess> hess;
     QrotScratch(int n, int m)
     {
         const int rows = qrot_rows(n, m);
@@ -93,6 +125,28 @@ void launch_qrot_apd_plan(int bulk, const void* M, long ldm, int n, int m, const
                           double* colpart, double* rsum, double* csum, double* wpart,
                           double* out, hipStream_t s);
 void launch_qrot_sum1(const double* part, int count, double* out, hipStream_t s);
+
+// sigma (bitmask, see QrotHess) at alpha, beta
+void launch_qrot_mask(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                      const double* beta, unsigned long long* mask, hipStream_t s);
+// rowpart / colpart as launch_qrot_tile, of sigma_ij y_j (rows_out) and
+// sigma_ij x_i (cols_out); null x or y count ones.
+void launch_qrot_sig(const unsigned long long* mask, int n, int m, bool rows_out, bool cols_out,
+                     const double* x, const double* y, double* rowpart, double* colpart,
+                     hipStream_t s);
+void launch_qrot_ainv(const double* x, const double* h, double shift, double c, double* out,
+                      int count, hipStream_t s);
+void launch_qrot_delta_fin(const double* x, const double* h, double shift, const double* t,
+                           double c, double* out, int count, hipStream_t s);
+void launch_qrot_recip(const double* h, double shift, double* out, int count, hipStream_t s);
+
+// The generalised Hessian at device gamma (n + m) into the problem's QrotHess.
+QrotHess& qrot_hess_build(regot_problem* p, const double* gamma);
+// out (device, n + m) = (H + shift I + tau vv')^-1 rhs, v = [1; -1], by block
+// elimination with CG on the Schur complement (Eigen's rules: from zero,
+// ||r||^2 < tol^2 ||rhs_beta||^2); returns the CG steps.
+int qrot_hess_solve(regot_problem* p, QrotHess& H, const double* rhs, double shift, double tau,
+                    double tol, int max_iter, double* out);
 
 // f (and grad, ||grad||^2 when grad is non-null) of the QROT dual at device
 // gamma (n + m), src/qrot_problem.cpp:226-258.

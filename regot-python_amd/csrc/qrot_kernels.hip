@@ -13,6 +13,11 @@
 //                     per column (or row) the sum and count of y = M_ij - alpha_i
 //                     (or M_ij - beta_j) below the current iterate;
 //   k_qrot_pl_update  the step x <- (c + S) / K per column or row.
+//   k_qrot_mask       the generalised Hessian's 0/1 pattern sigma_ij =
+//                     [alpha_i + beta_j - M_ij > 0] (src/qrot_hess.cpp:42-96)
+//                     as a bitmask, one bit per entry (n m / 8 bytes);
+//   k_qrot_sig        sigma y (row sums) and sigma' x (column sums) read from
+//                     the bitmask (src/qrot_hess.cpp:99-125).
 // Tiles are 256-column strips (64 lanes x 4 adjacent columns) over blocks of
 // `rows` rows, one row per wave at a time; every reduction has a fixed order.
 #include <cmath>
@@ -519,6 +524,123 @@ __global__ __launch_bounds__(64) void k_qrot_fin_total(const double* __restrict_
     out[1] = g2;
 }
 
+// The bitmask shares the tiles' lane layout: for row r and 256-column strip
+// st, word q of mask[(r * nstrip + st) * 4 + q] holds in bit l the entry of
+// column st * 256 + 4 l + q (the lane-l, q-th column of the strip), so a wave
+// writes a row's strip with four ballots and reads it back as four
+// wave-uniform words.  Columns past m read as 0.
+template <typename TM>
+__global__ __launch_bounds__(256) void k_qrot_mask(const TM* __restrict__ M, long ldm, int n,
+                                                   int m, const double* __restrict__ alpha,
+                                                   const double* __restrict__ beta, int rows,
+                                                   unsigned long long* __restrict__ mask)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int strip = blockIdx.x, nstrip = gridDim.x;
+    const int c0 = strip * 256 + lane * 4;
+    const int r0 = blockIdx.y * rows;
+    const int r1 = min(n, r0 + rows);
+    double bq[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) bq[q] = (c0 + q < m) ? beta[c0 + q] : 0.0;
+    for (int r = r0 + wave; r < r1; r += 4) {
+        const double ai = alpha[r];
+        double mv[4];
+        load4(M + static_cast<long>(r) * ldm, c0, m, mv);
+        unsigned long long w[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) w[q] = __ballot(c0 + q < m && (ai + bq[q]) - mv[q] > 0.0);
+        if (lane < 4) {
+            const unsigned long long v = lane == 0 ? w[0] : lane == 1 ? w[1] : lane == 2 ? w[2] : w[3];
+            mask[(static_cast<long>(r) * nstrip + strip) * 4 + lane] = v;
+        }
+    }
+}
+
+// ROWS: rowpart[strip * n + r] = sum_j sigma_rj y_j over the strip (y null:
+// y = 1); COLS: colpart[blk * m + j] = sum_r sigma_rj x_r over the row block
+// (x null: x = 1).  Same grid and fixed reduction order as k_qrot_tile.
+template <bool ROWS, bool COLS>
+__global__ __launch_bounds__(256) void k_qrot_sig(const unsigned long long* __restrict__ mask,
+                                                  int n, int m, const double* __restrict__ x,
+                                                  const double* __restrict__ y, int rows,
+                                                  double* __restrict__ rowpart,
+                                                  double* __restrict__ colpart)
+{
+    __shared__ double red[4][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int strip = blockIdx.x, blk = blockIdx.y, nstrip = gridDim.x;
+    const int c0 = strip * 256 + lane * 4;
+    const int r0 = blk * rows;
+    const int r1 = min(n, r0 + rows);
+    double yq[4], cs[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) yq[q] = (ROWS && c0 + q < m) ? (y ? y[c0 + q] : 1.0) : 0.0;
+    for (int r = r0 + wave; r < r1; r += 4) {
+        const unsigned long long* wr = mask + (static_cast<long>(r) * nstrip + strip) * 4;
+        bool on[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) on[q] = (wr[q] >> lane) & 1ull;
+        if (ROWS) {
+            double rs = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) rs += on[q] ? yq[q] : 0.0;
+            rs = wsum64(rs);
+            if (lane == 0) rowpart[static_cast<long>(strip) * n + r] = rs;
+        }
+        if (COLS) {
+            const double xr = x ? x[r] : 1.0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) cs[q] += on[q] ? xr : 0.0;
+        }
+    }
+    if (COLS) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) red[wave][lane * 4 + q] = cs[q];
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int j = c0 + q;
+                const double v = ((red[0][lane * 4 + q] + red[1][lane * 4 + q]) +
+                                  red[2][lane * 4 + q]) + red[3][lane * 4 + q];
+                if (j < m) colpart[static_cast<long>(blk) * m + j] = v;
+            }
+        }
+    }
+}
+
+// out_i = x_i / (h_i + shift) - c / (h_i + shift)
+__global__ __launch_bounds__(256) void k_qrot_ainv(const double* __restrict__ x,
+                                                   const double* __restrict__ h, double shift,
+                                                   double c, double* __restrict__ out, int count)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count) {
+        const double hs = h[i] + shift;
+        out[i] = c != 0.0 ? x[i] / hs - c / hs : x[i] / hs;
+    }
+}
+
+// out_i = h_i x_i + shift x_i - t_i + c
+__global__ __launch_bounds__(256) void k_qrot_delta_fin(const double* __restrict__ x,
+                                                        const double* __restrict__ h,
+                                                        double shift,
+                                                        const double* __restrict__ t, double c,
+                                                        double* __restrict__ out, int count)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count) out[i] = ((h[i] * x[i] + shift * x[i]) - t[i]) + c;
+}
+
+// out_i = 1 / (h_i + shift)
+__global__ __launch_bounds__(256) void k_qrot_recip(const double* __restrict__ h, double shift,
+                                                    double* __restrict__ out, int count)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count) out[i] = 1.0 / (h[i] + shift);
+}
+
 }  // namespace
 
 int qrot_rows(int n, int m)
@@ -714,6 +836,65 @@ void launch_qrot_finish(const double* alpha, const double* beta, const double* a
     }
     hipLaunchKernelGGL(k_qrot_fin_total, dim3(1), dim3(64), 0, s, part, nw, gpart,
                        grad ? nw : 0, reg, tau, sq, out);
+    KERNEL_CHECK();
+}
+
+}  // namespace regot
+
+namespace regot {
+
+void launch_qrot_mask(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
+                      const double* beta, unsigned long long* mask, hipStream_t s)
+{
+    const int rows = qrot_rows(n, m);
+    const dim3 grid(ceil_div(m, 256), ceil_div(n, rows));
+    if (bulk == 0)
+        hipLaunchKernelGGL(k_qrot_mask<float>, grid, dim3(256), 0, s,
+                           static_cast<const float*>(M), ldm, n, m, alpha, beta, rows, mask);
+    else
+        hipLaunchKernelGGL(k_qrot_mask<double>, grid, dim3(256), 0, s,
+                           static_cast<const double*>(M), ldm, n, m, alpha, beta, rows, mask);
+    KERNEL_CHECK();
+}
+
+void launch_qrot_sig(const unsigned long long* mask, int n, int m, bool rows_out, bool cols_out,
+                     const double* x, const double* y, double* rowpart, double* colpart,
+                     hipStream_t s)
+{
+    const int rows = qrot_rows(n, m);
+    const dim3 grid(ceil_div(m, 256), ceil_div(n, rows));
+    if (rows_out && cols_out)
+        hipLaunchKernelGGL((k_qrot_sig<true, true>), grid, dim3(256), 0, s, mask, n, m, x, y,
+                           rows, rowpart, colpart);
+    else if (rows_out)
+        hipLaunchKernelGGL((k_qrot_sig<true, false>), grid, dim3(256), 0, s, mask, n, m, x, y,
+                           rows, rowpart, colpart);
+    else
+        hipLaunchKernelGGL((k_qrot_sig<false, true>), grid, dim3(256), 0, s, mask, n, m, x, y,
+                           rows, rowpart, colpart);
+    KERNEL_CHECK();
+}
+
+void launch_qrot_ainv(const double* x, const double* h, double shift, double c, double* out,
+                      int count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qrot_ainv, dim3(ceil_div(count, 256)), dim3(256), 0, s, x, h, shift, c,
+                       out, count);
+    KERNEL_CHECK();
+}
+
+void launch_qrot_delta_fin(const double* x, const double* h, double shift, const double* t,
+                           double c, double* out, int count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qrot_delta_fin, dim3(ceil_div(count, 256)), dim3(256), 0, s, x, h,
+                       shift, t, c, out, count);
+    KERNEL_CHECK();
+}
+
+void launch_qrot_recip(const double* h, double shift, double* out, int count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qrot_recip, dim3(ceil_div(count, 256)), dim3(256), 0, s, h, shift, out,
+                       count);
     KERNEL_CHECK();
 }
 

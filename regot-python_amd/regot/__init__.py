@@ -30,6 +30,7 @@ __all__ = [
     "sinkhorn_bcd", "sinkhorn_ssns", "sinkhorn_sparse_newton", "sinkhorn_splr",
     "sinkhorn_newton", "sinkhorn_lbfgs_dual", "sinkhorn_apdagd", "sinkhorn_result",
     "qrot_bcd", "qrot_gd", "qrot_lbfgs_dual", "qrot_lbfgs_semi_dual", "qrot_apdagd",
+    "qrot_grssn",
     "qrot_result",
     "Problem", "BatchProblem", "RegotError",
 ]
@@ -304,14 +305,35 @@ class Problem:
         check(lib().regot_qrot_primal_val(self._h, dptr(g), _lib.ctypes.byref(v)))
         return v.value
 
+    def qrot_hess_solve(self, gamma, rhs, shift, tau=0.0, tol=1e-8, max_iter=0):
+        """(H + shift I + tau vv')^-1 rhs with the QROT generalised Hessian at
+        gamma (src/qrot_hess.cpp:42-96, src/qrot_cg.cpp:9-51) -> (x, CG steps)."""
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        r = np.ascontiguousarray(rhs, dtype=np.float64)
+        if g.shape[0] != self.n + self.m or r.shape[0] != self.n + self.m:
+            raise ValueError("gamma and rhs must have n+m = %d entries" % (self.n + self.m))
+        out = np.empty(self.n + self.m)
+        it = _lib.ctypes.c_int(0)
+        check(lib().regot_qrot_hess_solve(self._h, dptr(g), dptr(r), float(shift), float(tau),
+                                          float(tol), int(max_iter), dptr(out),
+                                          _lib.ctypes.byref(it)))
+        return out, it.value
+
     def qrot_solve(self, solver, tol=1e-6, max_iter=1000, verbose=0, compute_plan=True,
                    **kwargs):
         """QROT solver (_lib.QROT_BCD / QROT_GD / QROT_LBFGS_DUAL /
-        QROT_LBFGS_SEMI_DUAL / QROT_APDAGD) -> qrot_result.
+        QROT_LBFGS_SEMI_DUAL / QROT_APDAGD / QROT_GRSSN) -> qrot_result.
         Keywords: x0 (n+m warm start), tau (extra dual term; the reference's
-        Python bindings never set it, src/solvers.cpp:33-51)."""
+        Python bindings never set it, src/solvers.cpp:33-51); GRSSN: method
+        (default 0, CG), shift (default 1e-3), cg_tol, cg_max_iter."""
         opts = _lib.Opts()
         lib().regot_opts_default(_lib.ctypes.byref(opts), 0)
+        opts.method = int(kwargs.get("method", 0))
+        opts.shift = float(kwargs.get("shift", 1e-3))
+        if "cg_tol" in kwargs:
+            opts.cg_tol = float(kwargs["cg_tol"])
+        if "cg_max_iter" in kwargs:
+            opts.cg_max_iter = int(kwargs["cg_max_iter"])
         if "tau" in kwargs:
             opts.tau = float(kwargs["tau"])
         x0 = kwargs.get("x0")
@@ -681,8 +703,9 @@ def _run_qrot(solver, M, a, b, reg, tol, max_iter, verbose, kwargs):
     # the reference parses only method and shift for QROT (src/solvers.cpp:33-51);
     # other keywords are ignored, like there
     with Problem(M, a, b, reg, bulk=kwargs.get("bulk"), device=kwargs.get("device", 0)) as p:
+        kw = {k: kwargs[k] for k in ("method", "shift") if k in kwargs}
         return p.qrot_solve(solver, tol=tol, max_iter=max_iter, verbose=verbose,
-                            compute_plan=kwargs.get("compute_plan", True))
+                            compute_plan=kwargs.get("compute_plan", True), **kw)
 
 
 def qrot_bcd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
@@ -711,3 +734,10 @@ def qrot_apdagd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
     """APDAGD on the QROT dual (src/solvers.cpp:53-67, src/qrot_apdagd.cpp:14-181);
     the result's plan is the averaged primal plan."""
     return _run_qrot(_lib.QROT_APDAGD, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def qrot_grssn(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Globalised regularised semismooth Newton on the QROT dual
+    (src/solvers.cpp:117-131 binding, src/qrot_grssn.cpp:16-133); keywords
+    method (default 0) and shift (default 1e-3)."""
+    return _run_qrot(_lib.QROT_GRSSN, M, a, b, reg, tol, max_iter, verbose, kwargs)

@@ -24,6 +24,7 @@ QROT_GD = 1
 QROT_LBFGS_DUAL = 2
 QROT_LBFGS_SEMI_DUAL = 3
 QROT_APDAGD = 4
+QROT_GRSSN = 5
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -104,6 +105,9 @@ def lib():
     L.regot_qrot_optimal_beta.argtypes = [_vp, _dp, _dp]
     L.regot_qrot_optimal_alpha.argtypes = [_vp, _dp, _dp]
     L.regot_qrot_primal_val.argtypes = [_vp, _dp, _dp]
+    L.regot_qrot_hess_solve.argtypes = [_vp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int, _dp,
+                                        ctypes.POINTER(ctypes.c_int)]
     L.regot_opts_default.argtypes = [ctypes.POINTER(Opts), ctypes.c_int]
     L.regot_opts_default.restype = None
     L.regot_solve.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,

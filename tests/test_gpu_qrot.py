@@ -148,3 +148,76 @@ def test_qrot_apdagd(gpu_lib):
     # entries near the ReLU kink switch on and off across thousands of
     # averaging steps and carry their rounding (~1e-9 absolute here)
     np.testing.assert_allclose(P, o["plan"], rtol=1e-5, atol=1e-8)
+
+
+@pytest.mark.parametrize("bulk", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape", [(7, 5), (300, 517), (1030, 260)])
+def test_qrot_hess_solve(gpu_lib, bulk, shape):
+    # the bitmask Hessian and its Schur-complement CG against the oracle's
+    # dense-mask restatement: tight tolerance (both near the exact solve) and
+    # the reference's 1e-8 (same CG step count within one)
+    n, m = shape
+    M, a, b = cloud_problem(n, m, d=5, seed=n + 1)
+    if bulk == "fp32":
+        M = M.astype(np.float32).astype(np.float64)
+    Mg = M.astype(np.float32) if bulk == "fp32" else M
+    reg = 0.05
+    rng = np.random.default_rng(2)
+    alpha = 0.02 * rng.standard_normal(n)
+    gamma = np.concatenate([alpha, O.qrot_optimal_beta(M, b, reg, alpha)])
+    gamma += 0.002 * rng.standard_normal(n + m)
+    rhs = rng.standard_normal(n + m)
+    print("qrot hess solve %s %dx%d" % (bulk, n, m), flush=True)
+    with gpu_lib.Problem(Mg, a, b, reg, bulk=bulk) as P:
+        # With tau > 0 the Schur complement of src/qrot_hess.cpp:130-187 cancels
+        # tau 11' against its Sherman-Morrison correction (sum 1 / (h1 +
+        # shift) ~ 1e5 when most rows are inactive): the computed Delta, and
+        # so both solves, carry ~1e-6 relative error (the oracle's own
+        # residual against the dense system is 3e-6 here), hence 1e-5; the CG
+        # step counts at 1e-8 then follow rounding (153 vs 172 seen) and are
+        # compared only for tau = 0.
+        for shift, tau in ((1e-3, 0.0), (0.1, 0.0), (1e-3, 0.5)):
+            x, it = P.qrot_hess_solve(gamma, rhs, shift, tau, tol=1e-13, max_iter=20 * m)
+            xo, ito = O.qrot_hess_solve(M, gamma, rhs, shift, tau, tol=1e-13, max_iter=20 * m)
+            assert rel(x, xo) < (1e-9 if tau == 0.0 else 1e-5)
+            x, it = P.qrot_hess_solve(gamma, rhs, shift, tau)
+            xo, ito = O.qrot_hess_solve(M, gamma, rhs, shift, tau)
+            assert (abs(it - ito) <= 1 or tau > 0.0) and it <= 2 * m
+            assert rel(x, xo) < (1e-6 if tau == 0.0 else 1e-5)
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_qrot_grssn_matches_oracle(gpu_lib, method):
+    M, a, b = cloud_problem(60, 45, d=4, seed=7)
+    reg, tol = 0.05, 1e-9
+    print("qrot grssn 60x45 method %d" % method, flush=True)
+    r = gpu_lib.qrot_grssn(M, a, b, reg, tol=tol, max_iter=500, method=method)
+    o = O.qrot_grssn(M, a, b, reg, tol=tol, max_iter=500, method=method)
+    assert type(r).__name__ == "qrot_result"
+    assert abs(r.niter - o["niter"]) <= 1 and len(r.obj_vals) == r.niter + 1
+    k = min(len(r.obj_vals), len(o["obj_vals"]))
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-9)
+    np.testing.assert_allclose(r.prim_vals[:k], o["prim_vals"][:k], rtol=1e-8)
+    assert r.mar_errs[-1] * reg < tol
+    assert rel(r.dual, o["dual"]) < 1e-8
+    np.testing.assert_allclose(r.plan, o["plan"], atol=1e-6 * np.max(o["plan"]))
+
+
+def test_qrot_grssn_fp32_larger(gpu_lib, capfd):
+    # FP32 M, 150 x 600 (ragged strips, several row blocks), defaults of the
+    # drop-in (method 0, shift 1e-3); the verbose line format.  ~140 Newton
+    # steps of ~270 CG steps each: the CG counts follow rounding by +-1, so
+    # the iteration count may drift by a few while the start of the history
+    # and the end point agree.
+    M, a, b = cloud_problem(150, 600, d=6, seed=12)
+    M32 = M.astype(np.float32)
+    reg = 0.05
+    print("qrot grssn fp32 150x600", flush=True)
+    r = gpu_lib.qrot_grssn(M32, a, b, reg, tol=1e-6, max_iter=300, verbose=1)
+    out = capfd.readouterr().out
+    assert "iter = 0, objval = " in out and ", ||grad|| = " in out
+    o = O.qrot_grssn(M32.astype(np.float64), a, b, reg, tol=1e-6, max_iter=300)
+    assert o["niter"] < 300 and abs(r.niter - o["niter"]) <= max(2, 0.05 * o["niter"])
+    np.testing.assert_allclose(r.obj_vals[:20], o["obj_vals"][:20], rtol=1e-8)
+    assert r.mar_errs[-1] * reg < 1e-6
+    assert abs(r.obj_vals[-1] - o["obj_vals"][-1]) < 1e-6 * abs(o["obj_vals"][-1])

@@ -614,3 +614,84 @@ def test_qrot_apdagd_reaches_the_qp_optimum():
     assert np.max(np.abs(P - qp.x.reshape(n, m))) < 1e-5
     primal = (P * M).sum() + 0.5 * reg * (P * P).sum()
     assert abs(primal - qp.fun) < 1e-7
+
+
+def _qrot_dense_hess(M, gamma, shift, tau):
+    n, m = M.shape
+    sig = (gamma[:n, None] + gamma[None, n:] - M > 0).astype(float)
+    H = np.zeros((n + m, n + m))
+    H[:n, :n] = np.diag(sig.sum(1))
+    H[n:, n:] = np.diag(sig.sum(0))
+    H[:n, n:] = sig
+    H[n:, :n] = sig.T
+    v = np.concatenate([np.ones(n), -np.ones(m)])
+    return H + shift * np.eye(n + m) + tau * np.outer(v, v)
+
+
+def test_qrot_hessian_is_the_gradient_jacobian_and_solves():
+    # Away from the kinks the QROT dual is piecewise quadratic: central
+    # differences of the gradient give H + tau vv' exactly (to rounding).
+    # The block-elimination solve (Schur complement CG) against numpy's dense
+    # solve, with and without tau, and the CG step count against the bound m.
+    M, a, b = cloud_problem(8, 6, d=2, seed=41)
+    reg = 0.1
+    rng = np.random.default_rng(8)
+    alpha = 0.02 * rng.standard_normal(8)
+    gamma = np.concatenate([alpha, O.qrot_optimal_beta(M, b, reg, alpha)])
+    gamma += 0.01 * rng.standard_normal(14)
+    for tau in (0.0, 0.3):
+        J = np.zeros((14, 14))
+        h = 1e-7
+        for k in range(14):
+            e = np.zeros(14)
+            e[k] = h
+            J[:, k] = (O.qrot_dual_obj_grad(M, a, b, reg, gamma + e, tau)[1] -
+                       O.qrot_dual_obj_grad(M, a, b, reg, gamma - e, tau)[1]) / (2 * h)
+        Hd = _qrot_dense_hess(M, gamma, 0.0, tau)
+        assert np.max(np.abs(J - Hd)) < 1e-6
+        nact = np.sum(_qrot_dense_hess(M, gamma, 0.0, 0.0)[:8, 8:])
+        assert 0 < nact < 48                         # some, not all, entries active
+        rhs = rng.standard_normal(14)
+        for shift in (1e-3, 0.5):
+            x, it = O.qrot_hess_solve(M, gamma, rhs, shift, tau, tol=1e-14, max_iter=500)
+            ref = np.linalg.solve(_qrot_dense_hess(M, gamma, shift, tau), rhs)
+            assert np.max(np.abs(x - ref)) < 1e-9 * max(1.0, np.max(np.abs(ref)))
+            assert 0 < it <= 6 + 1                   # CG on a 6 x 6 SPD system
+    # zero right-hand side: no CG step, zero solution
+    x, it = O.qrot_hess_solve(M, gamma, np.zeros(14), 1e-3)
+    assert it == 0 and not np.any(x)
+
+
+def test_qrot_grssn_reaches_the_qp_optimum():
+    from scipy.optimize import minimize
+    M, a, b = cloud_problem(5, 4, d=2, seed=33)
+    reg = 0.1
+    n, m = M.shape
+    cons = [{"type": "eq", "fun": lambda p: p.reshape(n, m).sum(1) - a},
+            {"type": "eq", "fun": lambda p: p.reshape(n, m).sum(0)[:-1] - b[:-1]}]
+    qp = minimize(lambda p: p @ M.ravel() + 0.5 * reg * p @ p, np.outer(a, b).ravel(),
+                  jac=lambda p: M.ravel() + reg * p, bounds=[(0, None)] * (n * m),
+                  constraints=cons, method="SLSQP", options={"ftol": 1e-15, "maxiter": 1000})
+    for method in (0, 1):
+        r = O.qrot_grssn(M, a, b, reg, tol=1e-11, max_iter=500, method=method)
+        assert r["niter"] < 100 and len(r["obj_vals"]) == r["niter"] + 1
+        assert np.max(np.abs(r["plan"] - qp.x.reshape(n, m))) < 1e-6
+        assert abs(r["prim_vals"][-1] - qp.fun) < 1e-8
+        # Armijo steps along a descent direction: f never increases
+        assert np.all(np.diff(r["obj_vals"]) <= 1e-15)
+        assert r["mar_errs"][-1] * reg < 1e-11 and len(r["cg_iters"]) == r["niter"]
+
+
+def test_qrot_grssn_history_rules():
+    M, a, b = cloud_problem(30, 20, d=3, seed=42)
+    reg = 0.05
+    r = O.qrot_grssn(M, a, b, reg, tol=1e-9, max_iter=200)
+    g = np.array(r["mar_errs"]) * reg
+    assert g[-1] < 1e-9 and np.all(g[:-1] >= 1e-9)
+    _, grad = O.qrot_dual_obj_grad(M, a, b, reg, r["dual"])
+    assert abs(np.linalg.norm(grad) - g[-1]) < 1e-14
+    # the semismooth Newton method ends superlinearly: far fewer steps than GD
+    assert r["niter"] < 60
+    assert O.qrot_grssn(M, a, b, reg, tol=1e9)["niter"] == 0
+    r = O.qrot_grssn(M, a, b, reg, tol=0.0, max_iter=3)
+    assert r["niter"] == 3 and len(r["obj_vals"]) == 4
