@@ -101,9 +101,8 @@ double qsum(QrotHess& H, hipStream_t s, long len, const double* x)
 void apply_B(regot_problem* p, QrotHess& H, const double* x, double tau, double* out)
 {
     const int n = p->n, m = p->m;
-    QrotScratch& w = scratch(p);
-    launch_qrot_sig(H.mask, n, m, true, false, nullptr, x, w.rowpart, nullptr, p->stream);
-    launch_qrot_fold(w.rowpart, ceil_div(m, 256), n, n, out, p->stream);
+    launch_qrot_sig(H.mask, n, m, true, false, nullptr, x, H.rowpart4, nullptr, p->stream);
+    launch_qrot_fold(H.rowpart4, 4 * ceil_div(m, 256), n, n, out, p->stream);
     if (tau > 0.0) {
         const double c = tau * qsum(H, p->stream, m, x);
         vec_axpby(out, 1.0, out, -c, H.ones, n, p->stream);
@@ -169,8 +168,8 @@ QrotHess& qrot_hess_build(regot_problem* p, const double* gamma)
     }
     QrotHess& H = *w.hess;
     launch_qrot_mask(p->bulk, p->dM, p->ldm, n, m, gamma, gamma + n, H.mask, s);
-    launch_qrot_sig(H.mask, n, m, true, true, nullptr, nullptr, w.rowpart, w.colpart, s);
-    launch_qrot_fold(w.rowpart, ceil_div(m, 256), n, n, H.h1, s);
+    launch_qrot_sig(H.mask, n, m, true, true, nullptr, nullptr, H.rowpart4, w.colpart, s);
+    launch_qrot_fold(H.rowpart4, 4 * ceil_div(m, 256), n, n, H.h1, s);
     launch_qrot_fold(w.colpart, ceil_div(n, qrot_rows(n, m)), m, m, H.h2, s);
     H.hinv_shift = -1.0;
     return H;

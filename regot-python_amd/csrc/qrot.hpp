@@ -17,12 +17,13 @@ int qrot_rows(int n, int m);
 // h2 = sigma' 1, plus the vectors of its block solve.
 struct QrotHess {
     DBuf<unsigned long long> mask;
-    DBuf<double> h1, h2, hinv, tn, tn2, tm, tm2, ones, v, r, pv, q, dsc;
+    DBuf<double> h1, h2, hinv, tn, tn2, tm, tm2, ones, v, r, pv, q, dsc, rowpart4;
     HBuf<double> hs;
     double hinv_shift = -1.0;
     QrotHess(int n, int m)
     {
         mask.alloc(static_cast<size_t>(n) * ceil_div(m, 256) * 4);
+        rowpart4.alloc(static_cast<size_t>(n) * ceil_div(m, 256) * 4);
         const int c = n > m ? n : m;
         h1.alloc(n);
         h2.alloc(m);
@@ -129,8 +130,10 @@ void launch_qrot_sum1(const double* part, int count, double* out, hipStream_t s)
 // sigma (bitmask, see QrotHess) at alpha, beta
 void launch_qrot_mask(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
                       const double* beta, unsigned long long* mask, hipStream_t s);
-// rowpart / colpart as launch_qrot_tile, of sigma_ij y_j (rows_out) and
-// sigma_ij x_i (cols_out); null x or y count ones.
+// Row sums of sigma_ij y_j (rows_out) into rowpart[(strip * 4 + k) * n + i]
+// (4 partials per 256-column strip: fold 4 ceil(m / 256) of them) and column
+// sums of sigma_ij x_i (cols_out) into colpart as launch_qrot_tile; null x or
+// y count ones.
 void launch_qrot_sig(const unsigned long long* mask, int n, int m, bool rows_out, bool cols_out,
                      const double* x, const double* y, double* rowpart, double* colpart,
                      hipStream_t s);

@@ -34,6 +34,27 @@ __device__ __forceinline__ double wsum64(double v)
     return v;
 }
 
+// One DPP step on a double (both halves moved with the same control).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over each 16-lane row of the wave, left in every lane of the row:
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror
+// (fixed order, no LDS).
+__device__ __forceinline__ double rsum16(double v)
+{
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x140>(v);
+    return v;
+}
+
 // The lane's 4 adjacent entries of a row: one 16-byte load (FP32) or two
 // (FP64) when all 4 are in range (rows are 16-byte aligned and c0 is a
 // multiple of 4), else element by element.
@@ -557,8 +578,8 @@ __global__ __launch_bounds__(256) void k_qrot_mask(const TM* __restrict__ M, lon
     }
 }
 
-// ROWS: rowpart[strip * n + r] = sum_j sigma_rj y_j over the strip (y null:
-// y = 1); COLS: colpart[blk * m + j] = sum_r sigma_rj x_r over the row block
+// ROWS: rowpart[(strip * 4 + k) * n + r] = sum_j sigma_rj y_j over the k-th
+// 64 columns of the strip (16 lanes; y null: y = 1); COLS: colpart[blk * m + j] = sum_r sigma_rj x_r over the row block
 // (x null: x = 1).  Same grid and fixed reduction order as k_qrot_tile.
 template <bool ROWS, bool COLS>
 __global__ __launch_bounds__(256) void k_qrot_sig(const unsigned long long* __restrict__ mask,
@@ -568,7 +589,10 @@ __global__ __launch_bounds__(256) void k_qrot_sig(const unsigned long long* __re
                                                   double* __restrict__ colpart)
 {
     __shared__ double red[4][256];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index as a wave-uniform value: the mask words of a row are
+    // then one uniform address (scalar loads instead of 64-lane broadcasts)
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int strip = blockIdx.x, blk = blockIdx.y, nstrip = gridDim.x;
     const int c0 = strip * 256 + lane * 4;
     const int r0 = blk * rows;
@@ -576,22 +600,45 @@ __global__ __launch_bounds__(256) void k_qrot_sig(const unsigned long long* __re
     double yq[4], cs[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 4; q++) yq[q] = (ROWS && c0 + q < m) ? (y ? y[c0 + q] : 1.0) : 0.0;
-    for (int r = r0 + wave; r < r1; r += 4) {
-        const unsigned long long* wr = mask + (static_cast<long>(r) * nstrip + strip) * 4;
-        bool on[4];
+    // four rows per step (rows r, r + 4, r + 8, r + 12 of this wave): their
+    // words (two 16-byte loads each) and weights are loaded before any is used
+    constexpr int U = 4;
+    for (int rb = r0 + wave; rb < r1; rb += 4 * U) {
+        ulonglong2 w[U][2];
+        double xr[U];
 #pragma unroll
-        for (int q = 0; q < 4; q++) on[q] = (wr[q] >> lane) & 1ull;
-        if (ROWS) {
-            double rs = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; q++) rs += on[q] ? yq[q] : 0.0;
-            rs = wsum64(rs);
-            if (lane == 0) rowpart[static_cast<long>(strip) * n + r] = rs;
+        for (int u = 0; u < U; u++) {
+            const int r = rb + 4 * u;
+            if (r < r1) {
+                const ulonglong2* wr = reinterpret_cast<const ulonglong2*>(
+                    mask + (static_cast<long>(r) * nstrip + strip) * 4);
+                w[u][0] = wr[0];
+                w[u][1] = wr[1];
+                xr[u] = (COLS && x) ? x[r] : 1.0;
+            } else {
+                w[u][0] = make_ulonglong2(0ull, 0ull);
+                w[u][1] = make_ulonglong2(0ull, 0ull);
+                xr[u] = 0.0;
+            }
         }
-        if (COLS) {
-            const double xr = x ? x[r] : 1.0;
 #pragma unroll
-            for (int q = 0; q < 4; q++) cs[q] += on[q] ? xr : 0.0;
+        for (int u = 0; u < U; u++) {
+            const int r = rb + 4 * u;
+            if (r >= r1) break;
+            const bool on[4] = {((w[u][0].x >> lane) & 1ull) != 0, ((w[u][0].y >> lane) & 1ull) != 0,
+                                ((w[u][1].x >> lane) & 1ull) != 0, ((w[u][1].y >> lane) & 1ull) != 0};
+            if (ROWS) {
+                double rs = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) rs += on[q] ? yq[q] : 0.0;
+                rs = rsum16(rs);
+                if ((lane & 15) == 0)
+                    rowpart[(static_cast<long>(strip) * 4 + (lane >> 4)) * n + r] = rs;
+            }
+            if (COLS) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) cs[q] += on[q] ? xr[u] : 0.0;
+            }
         }
     }
     if (COLS) {

@@ -175,14 +175,15 @@ def test_qrot_hess_solve(gpu_lib, bulk, shape):
         # so both solves, carry ~1e-6 relative error (the oracle's own
         # residual against the dense system is 3e-6 here), hence 1e-5; the CG
         # step counts at 1e-8 then follow rounding (153 vs 172 seen) and are
-        # compared only for tau = 0.
+        # compared only for tau = 0, where summation order still moves them
+        # by a few on these systems (condition ~4e4: 74 vs 76 seen).
         for shift, tau in ((1e-3, 0.0), (0.1, 0.0), (1e-3, 0.5)):
             x, it = P.qrot_hess_solve(gamma, rhs, shift, tau, tol=1e-13, max_iter=20 * m)
             xo, ito = O.qrot_hess_solve(M, gamma, rhs, shift, tau, tol=1e-13, max_iter=20 * m)
             assert rel(x, xo) < (1e-9 if tau == 0.0 else 1e-5)
             x, it = P.qrot_hess_solve(gamma, rhs, shift, tau)
             xo, ito = O.qrot_hess_solve(M, gamma, rhs, shift, tau)
-            assert (abs(it - ito) <= 1 or tau > 0.0) and it <= 2 * m
+            assert (abs(it - ito) <= max(2, 0.05 * ito) or tau > 0.0) and it <= 2 * m
             assert rel(x, xo) < (1e-6 if tau == 0.0 else 1e-5)
 
 
