@@ -284,8 +284,8 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
  *   f = 0.5 ||(alpha (+) beta - M)_+||^2 - reg (a'alpha + b'beta) + tau/2 (1'alpha - 1'beta)^2.
  * regot_qrot_solve replaces qrot_bcd_internal, qrot_gd_internal,
  * qrot_lbfgs_dual_internal, qrot_lbfgs_semi_dual_internal and
- * qrot_apdagd_internal (src/qrot_apdagd.cpp:14-181) and qrot_grssn_internal
- * (src/qrot_grssn.cpp:16-133)
+ * qrot_apdagd_internal (src/qrot_apdagd.cpp:14-181), qrot_grssn_internal
+ * (src/qrot_grssn.cpp:16-133) and qrot_assn_internal (src/qrot_assn.cpp:16-170)
  * (src/qrot_bcd.cpp:15-101, src/qrot_gd.cpp:15-121,
  * src/qrot_lbfgs_dual.cpp:85-126, src/qrot_lbfgs_semi_dual.cpp:78-118;
  * bindings src/solvers.cpp:53-164).  Its result
@@ -306,6 +306,9 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
                                           with NULL opts: CG to 1e-8; other values name
                                           the reference's direct factorisations, run as
                                           CG to 1e-12), opts->cg_tol / cg_max_iter */
+#define REGOT_QROT_ASSN 6              /* src/qrot_assn.cpp:16-170; reads opts->method
+                                          and cg_tol / cg_max_iter as GRSSN (its shift
+                                          is adaptive) */
 int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,
                      const regot_opts* opts, regot_result** out);
 int regot_result_prim_vals(const regot_result* r, double* out);

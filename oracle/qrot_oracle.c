@@ -666,3 +666,97 @@ int orc_qrot_grssn(int n, int m, const double* M, const double* a, const double*
     *hist_len = Q.h;
     return it;
 }
+
+/* ASSN (src/qrot_assn.cpp:16-170): Newton direction with shift mu = lam ||g||
+ * (CG as in GRSSN); u = gamma + d is taken when ||g(u)|| <= nu xi (xi the
+ * last accepted gradient norm, ||g_0|| at the start); otherwise rho =
+ * -g(u)'d / ||d||^2 picks the projection point v = gamma + (g(u)'d /
+ * ||g(u)||^2) g(u) (rho >= eta1 and ||g(v)|| <= ||g||) or a gradient step
+ * gamma - 0.5 g, and lam moves to (lam_min + lam) / 2, 2 lam or 4 lam for rho
+ * >= eta2, >= eta1 or below.  nu = 0.99, eta1 = 0.25, eta2 = 0.75,
+ * lam_min = 0.01, lam_0 = 1. */
+int orc_qrot_assn(int n, int m, const double* M, const double* a, const double* b, double reg,
+                  double tau, int method, double tol, int max_iter, int verbose, int hist_cap,
+                  const double* x0, double* x, double* obj_vals, double* prim_vals,
+                  double* mar_errs, double* run_times, int* hist_len, int* cg_iters)
+{
+    const long N = (long)n + m;
+    qrot_ctx Q = {n, m, M, a, b, reg, tau, 0, hist_cap, verbose, 0.0,
+                  obj_vals, prim_vals, mar_errs, run_times};
+    if (x0) {
+        memcpy(x, x0, sizeof(double) * N);
+    } else {
+        memset(x, 0, sizeof(double) * n);
+        orc_qrot_optimal_beta(n, m, M, b, reg, x, x + n);
+    }
+    const double nu = 0.99, eta1 = 0.25, eta2 = 0.75, lam_min = 0.01;
+    const double cg_tol = method == 0 ? 1e-8 : 1e-12;
+    const int cg_max = method == 0 ? 2 * m : 10 * m + 1000;
+    double xi = 0.0, lam = 1.0;
+    double* g = (double*)malloc(sizeof(double) * N);
+    double* rhs = (double*)malloc(sizeof(double) * N);
+    double* d = (double*)malloc(sizeof(double) * N);
+    double* u = (double*)malloc(sizeof(double) * N);
+    double* gu = (double*)malloc(sizeof(double) * N);
+    double* v = (double*)malloc(sizeof(double) * N);
+    double* gv = (double*)malloc(sizeof(double) * N);
+    Q.t0 = q_now_ms();
+    double f = qrot_objgrad(&Q, x, g);
+    double gnorm = q_norm2(g, N);
+    qrot_record(&Q, x, f, gnorm);
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        if (verbose >= 1) printf("iter = %d, objval = %g, ||grad|| = %g\n", it, f, gnorm);
+        if (gnorm < tol) break;
+        if (it == 0) xi = gnorm;
+        qrot_hess H;
+        qh_build(&H, n, m, M, x);
+        H.shift = lam * gnorm;
+        H.tau = tau;
+        for (long q = 0; q < N; q++) rhs[q] = -g[q];
+        const int cgi = qh_solve(&H, rhs, cg_tol, cg_max, d);
+        qh_free(&H);
+        if (cg_iters && it < hist_cap) cg_iters[it] = cgi;
+        for (long q = 0; q < N; q++) u[q] = x[q] + d[q];
+        qrot_objgrad(&Q, u, gu);
+        const double gunorm = q_norm2(gu, N);
+        if (gunorm <= nu * xi) {
+            memcpy(x, u, sizeof(double) * N);
+            xi = gunorm;
+        } else {
+            double numer = 0.0, denom = 0.0;
+            for (long q = 0; q < N; q++) {
+                numer -= gu[q] * d[q];
+                denom += d[q] * d[q];
+            }
+            const double rho = numer / denom;
+            const double c = (-numer) / (gunorm * gunorm);
+            for (long q = 0; q < N; q++) v[q] = x[q] + c * gu[q];
+            qrot_objgrad(&Q, v, gv);
+            const double gvnorm = q_norm2(gv, N);
+            if (rho >= eta1 && gvnorm <= gnorm) {
+                memcpy(x, v, sizeof(double) * N);
+            } else {
+                for (long q = 0; q < N; q++) x[q] -= 0.5 * g[q];
+            }
+            if (rho >= eta2)
+                lam = 0.5 * (lam_min + lam);
+            else if (rho >= eta1)
+                lam *= 2.0;
+            else
+                lam *= 4.0;
+        }
+        f = qrot_objgrad(&Q, x, g);
+        gnorm = q_norm2(g, N);
+        qrot_record(&Q, x, f, gnorm);
+    }
+    free(g);
+    free(rhs);
+    free(d);
+    free(u);
+    free(gu);
+    free(v);
+    free(gv);
+    *hist_len = Q.h;
+    return it;
+}

@@ -300,6 +300,99 @@ static void solve_qrot_grssn(regot_problem* p, double tol, int max_iter, int ver
     HIP_CHECK(hipMemcpy(res->dual.data(), w.gamma.p, N * sizeof(double), hipMemcpyDeviceToHost));
 }
 
+// ASSN (src/qrot_assn.cpp:16-170): Newton direction with shift mu = lam ||g||
+// (CG as GRSSN), u = gamma + d taken when ||g(u)|| <= nu xi (xi: the last
+// accepted gradient norm, ||g_0|| at the start); otherwise rho = -g(u)'d /
+// ||d||^2 chooses the projection point v = gamma + (g(u)'d / ||g(u)||^2) g(u)
+// (rho >= eta1 and ||g(v)|| <= ||g||) or the gradient step gamma - 0.5 g, and
+// lam becomes (lam_min + lam) / 2, 2 lam or 4 lam (rho >= eta2, >= eta1,
+// below).  nu = 0.99, eta1 = 0.25, eta2 = 0.75, lam_min = 0.01, lam_0 = 1.
+static void solve_qrot_assn(regot_problem* p, double tol, int max_iter, int verbose,
+                            const regot_opts& o, regot_result* res)
+{
+    const int n = p->n, m = p->m;
+    const long N = static_cast<long>(n) + m;
+    hipStream_t s = p->stream;
+    const double reg = p->reg, tau = o.tau;
+    const double nu = 0.99, eta1 = 0.25, eta2 = 0.75, lam_min = 0.01;
+    const double cg_tol = o.cg_tol > 0.0 ? o.cg_tol : (o.method == 0 ? 1e-8 : 1e-12);
+    const int cg_max = o.cg_max_iter > 0 ? o.cg_max_iter : (o.method == 0 ? 2 * m : 10 * m + 1000);
+    Work w(p, N);
+    w.og = [&](const double* x, double* g, double* gn2) {
+        return qrot_obj_grad(p, x, g, tau, gn2);
+    };
+    if (o.x0) {
+        HIP_CHECK(hipMemcpyAsync(w.gamma, o.x0, N * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+        launch_fill(w.gamma, 0.0, n, s);
+        qrot_optimal(p, true, w.gamma, w.gamma.p + n);
+    }
+    res->qrot = true;
+    Timer timer;
+    timer.tic();
+    auto record = [&](const double* x, double f, double gnorm) {
+        const double prim = qrot_round_plan(p, x, x + n, nullptr, 0);
+        const double dur = timer.toc();
+        res->obj_vals.push_back(f);
+        res->prim_vals.push_back(prim);
+        res->mar_errs.push_back(gnorm / reg);
+        res->run_times.push_back((res->run_times.empty() ? 0.0 : res->run_times.back()) + dur);
+        timer.tic();
+    };
+    // u = w.x, g(u) = w.xg, v = w.y, g(v) = w.sv
+    double gnorm;
+    double f = w.obj_grad(w.gamma, w.g, &gnorm);
+    record(w.gamma, f, gnorm);
+    double xi = 0.0, lam = 1.0;
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        if (verbose >= 1) {
+            printf("iter = %d, objval = %g, ||grad|| = %g\n", it, f, gnorm);
+            fflush(stdout);
+        }
+        if (gnorm < tol) break;
+        if (it == 0) xi = gnorm;
+        QrotHess& H = qrot_hess_build(p, w.gamma);
+        vec_axpby(w.rhs, -1.0, w.g, 0.0, nullptr, N, s);
+        const int cgi = qrot_hess_solve(p, H, w.rhs, lam * gnorm, tau, cg_tol, cg_max, w.direc);
+        w.cg_iters += cgi;
+        w.alg_bytes += w.pass_bytes() + (2.0 * (cgi + 2)) * (n * (m / 8.0));
+        w.move(w.x, w.gamma, 1.0, w.direc);
+        double gunorm;
+        w.obj_grad(w.x, w.xg, &gunorm);
+        if (gunorm <= nu * xi) {
+            std::swap(w.gamma, w.x);
+            xi = gunorm;
+        } else {
+            const double* xs[2] = {w.xg.p, w.direc.p};
+            const double* ys[2] = {w.direc.p, w.direc.p};
+            double d[2];
+            w.dots(2, xs, ys, d);
+            const double numer = -d[0];
+            const double rho = numer / d[1];
+            w.move(w.y, w.gamma, (-numer) / (gunorm * gunorm), w.xg);
+            double gvnorm;
+            w.obj_grad(w.y, w.sv, &gvnorm);
+            if (rho >= eta1 && gvnorm <= gnorm)
+                std::swap(w.gamma, w.y);
+            else
+                w.move(w.gamma, w.gamma, -0.5, w.g);
+            if (rho >= eta2)
+                lam = 0.5 * (lam_min + lam);
+            else if (rho >= eta1)
+                lam *= 2.0;
+            else
+                lam *= 4.0;
+        }
+        f = w.obj_grad(w.gamma, w.g, &gnorm);
+        record(w.gamma, f, gnorm);
+    }
+    res->niter = it;
+    w.report(res);
+    res->dual.resize(N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), w.gamma.p, N * sizeof(double), hipMemcpyDeviceToHost));
+}
+
 // Drivers.  Semantics: src/qrot_bcd.cpp:15-101 (tau forced to 0),
 // src/qrot_gd.cpp:15-121, src/qrot_lbfgs_dual.cpp:85-126 (epsilon =
 // epsilon_rel = tol).  Histories: obj_vals f, prim_vals the primal value of
@@ -498,6 +591,10 @@ void solve_qrot(regot_problem* p, int solver, double tol, int max_iter, int verb
         solve_qrot_grssn(p, tol, max_iter, verbose, o, res);
         return;
     }
+    if (solver == 6) {
+        solve_qrot_assn(p, tol, max_iter, verbose, o, res);
+        return;
+    }
     const int n = p->n, m = p->m;
     const long N = static_cast<long>(n) + m;
     hipStream_t s = p->stream;
@@ -614,7 +711,7 @@ int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int
     CAPI_BEGIN
     if (!p || !out) fail(E_INVALID, "null argument");
     *out = nullptr;
-    if (solver < 0 || solver > 5) fail(E_INVALID, "unknown QROT solver %d", solver);
+    if (solver < 0 || solver > 6) fail(E_INVALID, "unknown QROT solver %d", solver);
     if (max_iter < 0) fail(E_INVALID, "max_iter must be >= 0");
     HIP_CHECK(hipSetDevice(p->device));
     regot_opts o;

@@ -30,7 +30,7 @@ __all__ = [
     "sinkhorn_bcd", "sinkhorn_ssns", "sinkhorn_sparse_newton", "sinkhorn_splr",
     "sinkhorn_newton", "sinkhorn_lbfgs_dual", "sinkhorn_apdagd", "sinkhorn_result",
     "qrot_bcd", "qrot_gd", "qrot_lbfgs_dual", "qrot_lbfgs_semi_dual", "qrot_apdagd",
-    "qrot_grssn",
+    "qrot_grssn", "qrot_assn",
     "qrot_result",
     "Problem", "BatchProblem", "RegotError",
 ]
@@ -322,10 +322,11 @@ class Problem:
     def qrot_solve(self, solver, tol=1e-6, max_iter=1000, verbose=0, compute_plan=True,
                    **kwargs):
         """QROT solver (_lib.QROT_BCD / QROT_GD / QROT_LBFGS_DUAL /
-        QROT_LBFGS_SEMI_DUAL / QROT_APDAGD / QROT_GRSSN) -> qrot_result.
+        QROT_LBFGS_SEMI_DUAL / QROT_APDAGD / QROT_GRSSN / QROT_ASSN) -> qrot_result.
         Keywords: x0 (n+m warm start), tau (extra dual term; the reference's
         Python bindings never set it, src/solvers.cpp:33-51); GRSSN: method
-        (default 0, CG), shift (default 1e-3), cg_tol, cg_max_iter."""
+        (default 0, CG), shift (default 1e-3), cg_tol, cg_max_iter; ASSN:
+        method, cg_tol, cg_max_iter."""
         opts = _lib.Opts()
         lib().regot_opts_default(_lib.ctypes.byref(opts), 0)
         opts.method = int(kwargs.get("method", 0))
@@ -741,3 +742,9 @@ def qrot_grssn(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
     (src/solvers.cpp:117-131 binding, src/qrot_grssn.cpp:16-133); keywords
     method (default 0) and shift (default 1e-3)."""
     return _run_qrot(_lib.QROT_GRSSN, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def qrot_assn(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Adaptive semismooth Newton on the QROT dual (src/solvers.cpp:69-83
+    binding, src/qrot_assn.cpp:16-170); keyword method (default 0)."""
+    return _run_qrot(_lib.QROT_ASSN, M, a, b, reg, tol, max_iter, verbose, kwargs)

@@ -25,6 +25,7 @@ QROT_LBFGS_DUAL = 2
 QROT_LBFGS_SEMI_DUAL = 3
 QROT_APDAGD = 4
 QROT_GRSSN = 5
+QROT_ASSN = 6
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p

@@ -695,3 +695,29 @@ def test_qrot_grssn_history_rules():
     assert O.qrot_grssn(M, a, b, reg, tol=1e9)["niter"] == 0
     r = O.qrot_grssn(M, a, b, reg, tol=0.0, max_iter=3)
     assert r["niter"] == 3 and len(r["obj_vals"]) == 4
+
+
+def test_qrot_assn_reaches_the_qp_optimum_and_history_rules():
+    from scipy.optimize import minimize
+    M, a, b = cloud_problem(5, 4, d=2, seed=33)
+    reg = 0.1
+    n, m = M.shape
+    cons = [{"type": "eq", "fun": lambda p: p.reshape(n, m).sum(1) - a},
+            {"type": "eq", "fun": lambda p: p.reshape(n, m).sum(0)[:-1] - b[:-1]}]
+    qp = minimize(lambda p: p @ M.ravel() + 0.5 * reg * p @ p, np.outer(a, b).ravel(),
+                  jac=lambda p: M.ravel() + reg * p, bounds=[(0, None)] * (n * m),
+                  constraints=cons, method="SLSQP", options={"ftol": 1e-15, "maxiter": 1000})
+    for method in (0, 1):
+        r = O.qrot_assn(M, a, b, reg, tol=1e-11, max_iter=5000, method=method)
+        assert r["niter"] < 5000 and len(r["obj_vals"]) == r["niter"] + 1
+        assert np.max(np.abs(r["plan"] - qp.x.reshape(n, m))) < 1e-6
+        assert abs(r["prim_vals"][-1] - qp.fun) < 1e-8
+    # the same optimum as GRSSN on a larger problem (plans: the dual is
+    # unique only up to alpha + c, beta - c when sum a = sum b)
+    M, a, b = cloud_problem(30, 20, d=3, seed=42)
+    r = O.qrot_assn(M, a, b, 0.1, tol=1e-9, max_iter=2000)
+    g = np.array(r["mar_errs"]) * 0.1
+    assert g[-1] < 1e-9 and np.all(g[:-1] >= 1e-9)
+    q = O.qrot_grssn(M, a, b, 0.1, tol=1e-9, max_iter=2000)
+    assert np.max(np.abs(r["plan"] - q["plan"])) < 1e-7
+    assert O.qrot_assn(M, a, b, 0.1, tol=1e9)["niter"] == 0
