@@ -285,7 +285,8 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
  * regot_qrot_solve replaces qrot_bcd_internal, qrot_gd_internal,
  * qrot_lbfgs_dual_internal, qrot_lbfgs_semi_dual_internal and
  * qrot_apdagd_internal (src/qrot_apdagd.cpp:14-181), qrot_grssn_internal
- * (src/qrot_grssn.cpp:16-133) and qrot_assn_internal (src/qrot_assn.cpp:16-170)
+ * (src/qrot_grssn.cpp:16-133), qrot_assn_internal (src/qrot_assn.cpp:16-170)
+ * and qrot_s5n_internal (src/qrot_s5n.cpp:16-196)
  * (src/qrot_bcd.cpp:15-101, src/qrot_gd.cpp:15-121,
  * src/qrot_lbfgs_dual.cpp:85-126, src/qrot_lbfgs_semi_dual.cpp:78-118;
  * bindings src/solvers.cpp:53-164).  Its result
@@ -309,6 +310,9 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
 #define REGOT_QROT_ASSN 6              /* src/qrot_assn.cpp:16-170; reads opts->method
                                           and cg_tol / cg_max_iter as GRSSN (its shift
                                           is adaptive) */
+#define REGOT_QROT_S5N 7               /* src/qrot_s5n.cpp:16-196; reads opts->method,
+                                          opts->mu0 (default 1; with NULL opts 1) and
+                                          cg_tol / cg_max_iter */
 int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,
                      const regot_opts* opts, regot_result** out);
 int regot_result_prim_vals(const regot_result* r, double* out);

@@ -180,6 +180,10 @@ int orc_qrot_assn(int n, int m, const double* M, const double* a, const double* 
                   double tau, int method, double tol, int max_iter, int verbose, int hist_cap,
                   const double* x0, double* x, double* obj_vals, double* prim_vals,
                   double* mar_errs, double* run_times, int* hist_len, int* cg_iters);
+int orc_qrot_s5n(int n, int m, const double* M, const double* a, const double* b, double reg,
+                 double tau, int method, double mu0, double tol, int max_iter, int verbose,
+                 int hist_cap, const double* x0, double* x, double* obj_vals, double* prim_vals,
+                 double* mar_errs, double* run_times, int* hist_len, int* cg_iters);
 
 /* BCD iterations only (no convergence test), for timing a bounded CPU sample. */
 double orc_bcd_iterations(int n, int m, const double* M, const double* a, const double* b,

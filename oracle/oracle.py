@@ -554,3 +554,26 @@ def qrot_assn(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, method=0, tau=0.
     return dict(niter=niter, dual=dual, plan=qrot_get_plan(M, reg, dual),
                 obj_vals=list(h[0][:k]), prim_vals=list(h[1][:k]), mar_errs=list(h[2][:k]),
                 run_times=list(h[3][:k]), cg_iters=list(cg[:niter]))
+
+
+def qrot_s5n(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, method=0, mu0=1.0, tau=0.0,
+             x0=None):
+    """src/qrot_s5n.cpp:16-196 -> dict (+ "cg_iters" per Newton step)"""
+    M, a, b = _d(M), _d(a), _d(b)
+    n, m = M.shape
+    cap = max_iter + 2
+    dual = np.empty(n + m)
+    h = [np.empty(cap) for _ in range(4)]
+    cg = np.zeros(cap, dtype=np.int32)
+    hl = ctypes.c_int(0)
+    x0p = _p(_d(x0)) if x0 is not None else None
+    niter = lib().orc_qrot_s5n(n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg),
+                               ctypes.c_double(tau), int(method), ctypes.c_double(mu0),
+                               ctypes.c_double(tol), int(max_iter), int(verbose), cap, x0p,
+                               _p(dual), _p(h[0]), _p(h[1]), _p(h[2]), _p(h[3]),
+                               ctypes.byref(hl), cg.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    _check(niter)
+    k = hl.value
+    return dict(niter=niter, dual=dual, plan=qrot_get_plan(M, reg, dual),
+                obj_vals=list(h[0][:k]), prim_vals=list(h[1][:k]), mar_errs=list(h[2][:k]),
+                run_times=list(h[3][:k]), cg_iters=list(cg[:niter]))

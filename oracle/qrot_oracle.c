@@ -519,28 +519,43 @@ static void qh_delta(const qrot_hess* H, const double* x, double* res)
     for (int j = 0; j < H->m; j++) res[j] = H->h2[j] * x[j] + H->shift * x[j] - H->tm[j] + c;
 }
 
-/* Conjugate gradient with an identity preconditioner from x = 0, stopping at
- * ||r||^2 < max(tol^2 ||rhs||^2, DBL_MIN) or after max_iter steps (the rules
- * of Eigen::ConjugateGradient that src/qrot_cg.cpp:31-38 calls; max_iter
- * defaults to 2m there).  Returns the number of steps. */
-static int qh_cg(qrot_hess* H, const double* rhs, double tol, int max_iter, double* x)
+/* Conjugate gradient with an identity preconditioner from x = guess (or 0),
+ * stopping at ||r||^2 < max(tol^2 ||rhs||^2, DBL_MIN) or after max_iter steps
+ * (the rules of Eigen::ConjugateGradient that src/qrot_cg.cpp:31-38 calls;
+ * max_iter defaults to 2m there; a zero rhs gives x = 0).  Returns the number
+ * of steps. */
+static int qh_cg(qrot_hess* H, const double* rhs, double tol, int max_iter, const double* guess,
+                 double* x)
 {
     const int m = H->m;
     double* r = (double*)malloc(sizeof(double) * (size_t)m);
     double* p = (double*)malloc(sizeof(double) * (size_t)m);
     double* q = (double*)malloc(sizeof(double) * (size_t)m);
     double rhs2 = 0.0;
-    for (int j = 0; j < m; j++) {
-        x[j] = 0.0;
-        r[j] = rhs[j];
-        p[j] = rhs[j];
-        rhs2 += rhs[j] * rhs[j];
+    for (int j = 0; j < m; j++) rhs2 += rhs[j] * rhs[j];
+    if (guess) {
+        qh_delta(H, guess, q);
+        for (int j = 0; j < m; j++) {
+            x[j] = guess[j];
+            r[j] = rhs[j] - q[j];
+        }
+    } else {
+        for (int j = 0; j < m; j++) {
+            x[j] = 0.0;
+            r[j] = rhs[j];
+        }
     }
     int it = 0;
-    if (rhs2 > 0.0) {
+    if (rhs2 == 0.0) {
+        for (int j = 0; j < m; j++) x[j] = 0.0;
+    } else {
         double thr = tol * tol * rhs2;
         if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
-        double rr = rhs2;
+        double rr = 0.0;
+        for (int j = 0; j < m; j++) {
+            rr += r[j] * r[j];
+            p[j] = r[j];
+        }
         if (rr >= thr) {
             double abs_new = rr;
             while (it < max_iter) {
@@ -570,7 +585,8 @@ static int qh_cg(qrot_hess* H, const double* rhs, double tol, int max_iter, doub
 
 /* (H + shift I + tau vv')^-1 rhs by block elimination (src/qrot_cg.cpp:9-51):
  * v = A^-1 w, r2 = Delta^-1 (z - C v) by CG, r1 = A^-1 (w - B r2). */
-static int qh_solve(qrot_hess* H, const double* rhs, double tol, int max_iter, double* out)
+static int qh_solve(qrot_hess* H, const double* rhs, double tol, int max_iter,
+                    const double* guess, double* out)
 {
     const int n = H->n, m = H->m;
     double* v = (double*)malloc(sizeof(double) * (size_t)n);
@@ -579,7 +595,7 @@ static int qh_solve(qrot_hess* H, const double* rhs, double tol, int max_iter, d
     qh_solve_A(H, rhs, v);
     qh_Cx(H, v, zy);
     for (int j = 0; j < m; j++) zy[j] = rhs[n + j] - zy[j];
-    const int it = qh_cg(H, zy, tol, max_iter, out + n);
+    const int it = qh_cg(H, zy, tol, max_iter, guess, out + n);
     qh_Bx(H, out + n, w);
     for (int i = 0; i < n; i++) w[i] = rhs[i] - w[i];
     qh_solve_A(H, w, out);
@@ -596,7 +612,7 @@ int orc_qrot_hess_solve(int n, int m, const double* M, const double* gamma, cons
     qh_build(&H, n, m, M, gamma);
     H.shift = shift;
     H.tau = tau;
-    const int it = qh_solve(&H, rhs, tol, max_iter, out);
+    const int it = qh_solve(&H, rhs, tol, max_iter, NULL, out);
     qh_free(&H);
     return it;
 }
@@ -641,7 +657,7 @@ int orc_qrot_grssn(int n, int m, const double* M, const double* a, const double*
         H.shift = shift;
         H.tau = tau;
         for (long q = 0; q < N; q++) rhs[q] = -grad[q];
-        const int cgi = qh_solve(&H, rhs, cg_tol, cg_max, dir);
+        const int cgi = qh_solve(&H, rhs, cg_tol, cg_max, NULL, dir);
         qh_free(&H);
         if (cg_iters && it < hist_cap) cg_iters[it] = cgi;
         double gd = 0.0;
@@ -714,7 +730,7 @@ int orc_qrot_assn(int n, int m, const double* M, const double* a, const double* 
         H.shift = lam * gnorm;
         H.tau = tau;
         for (long q = 0; q < N; q++) rhs[q] = -g[q];
-        const int cgi = qh_solve(&H, rhs, cg_tol, cg_max, d);
+        const int cgi = qh_solve(&H, rhs, cg_tol, cg_max, NULL, d);
         qh_free(&H);
         if (cg_iters && it < hist_cap) cg_iters[it] = cgi;
         for (long q = 0; q < N; q++) u[q] = x[q] + d[q];
@@ -757,6 +773,148 @@ int orc_qrot_assn(int n, int m, const double* M, const double* a, const double* 
     free(gu);
     free(v);
     free(gv);
+    *hist_len = Q.h;
+    return it;
+}
+
+/* (H + shift I + tau vv') x: [h1 .* w + sigma z; h2 .* z + sigma' w] + shift x
+ * + tau (1'w - 1'z) v (src/qrot_hess.cpp:193-214). */
+static void qh_apply_H(const qrot_hess* H, const double* x, double shift, double tau, double* res)
+{
+    const int n = H->n, m = H->m;
+    for (int i = 0; i < n; i++) {
+        double sz = 0.0;
+        for (int j = 0; j < m; j++)
+            if (H->sig[(long)i * m + j]) sz += x[n + j];
+        res[i] = sz;
+    }
+    for (int j = 0; j < m; j++) res[n + j] = 0.0;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < m; j++)
+            if (H->sig[(long)i * m + j]) res[n + j] += x[i];
+    for (int i = 0; i < n; i++) res[i] += H->h1[i] * x[i];
+    for (int j = 0; j < m; j++) res[n + j] += H->h2[j] * x[n + j];
+    if (shift != 0.0)
+        for (long q = 0; q < (long)n + m; q++) res[q] += shift * x[q];
+    if (tau > 0.0) {
+        const double c = q_sum(x, n) - q_sum(x + n, m);
+        for (int i = 0; i < n; i++) res[i] += tau * c;
+        for (int j = 0; j < m; j++) res[n + j] -= tau * c;
+    }
+}
+
+/* Problem::line_selection2 (src/qrot_problem.cpp:337-361): the first candidate
+ * whose objective is below cur, else the best one; *objval its objective. */
+static double q_line_selection2(qrot_ctx* Q, const double* cand, int nc, const double* x,
+                                const double* d, double cur, double* objval, double* work)
+{
+    const long N = (long)Q->n + Q->m;
+    double best = 1.0;
+    *objval = 1.0 / 0.0;
+    for (int k = 0; k < nc; k++) {
+        for (long q = 0; q < N; q++) work[q] = x[q] + cand[k] * d[q];
+        const double fk = qrot_objgrad(Q, work, NULL);
+        if (fk < *objval) {
+            best = cand[k];
+            *objval = fk;
+        }
+        if (*objval < cur) return best;
+    }
+    return best;
+}
+
+/* S5N (src/qrot_s5n.cpp:16-196): shift = mu (primal value + f / reg)^1 (the
+ * duality gap of the rounded plan), direction by the CG solve (warm-started
+ * from the last direction's beta block after a rejected step), step from
+ * line_selection2 over {1, 0.5, 0.1}, then {0.05, 0.01, 1e-3, 1e-4} on
+ * alternate failures; rho = (f - f_new) / (-g's - s'Hs / 2) accepts the step
+ * when > 0 and moves mu by x4 (rho < 0.25) or x0.5 down to 1e-3 (rho > 0.75);
+ * H, f and g are recomputed only after an accepted step; every iteration
+ * records f, the primal value, ||g|| / reg. */
+int orc_qrot_s5n(int n, int m, const double* M, const double* a, const double* b, double reg,
+                 double tau, int method, double mu0, double tol, int max_iter, int verbose,
+                 int hist_cap, const double* x0, double* x, double* obj_vals, double* prim_vals,
+                 double* mar_errs, double* run_times, int* hist_len, int* cg_iters)
+{
+    const long N = (long)n + m;
+    qrot_ctx Q = {n, m, M, a, b, reg, tau, 0, hist_cap, verbose, 0.0,
+                  obj_vals, prim_vals, mar_errs, run_times};
+    if (x0) {
+        memcpy(x, x0, sizeof(double) * N);
+    } else {
+        memset(x, 0, sizeof(double) * n);
+        orc_qrot_optimal_beta(n, m, M, b, reg, x, x + n);
+    }
+    const double cg_tol = method == 0 ? 1e-8 : 1e-12;
+    const int cg_max = method == 0 ? 2 * m : 10 * m + 1000;
+    const double alphas[3] = {1.0, 0.5, 0.1};
+    const double alphas_small[4] = {0.05, 0.01, 0.001, 1e-4};
+    double mu = mu0, rho = 1.0;
+    int use_small = 1;
+    double* g = (double*)malloc(sizeof(double) * N);
+    double* rhs = (double*)malloc(sizeof(double) * N);
+    double* d = (double*)calloc((size_t)N, sizeof(double));
+    double* st = (double*)malloc(sizeof(double) * N);
+    double* hs = (double*)malloc(sizeof(double) * N);
+    double* work = (double*)malloc(sizeof(double) * N);
+    Q.t0 = q_now_ms();
+    double f = qrot_objgrad(&Q, x, g);
+    double gnorm = q_norm2(g, N);
+    qrot_hess H;
+    qh_build(&H, n, m, M, x);
+    H.tau = tau;
+    qrot_record(&Q, x, f, gnorm);
+    double prim = Q.prim_vals[Q.h - 1];
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        const double dgap = prim + f / reg;
+        if (verbose >= 1)
+            printf("iter = %d, objval = %g, ||grad|| = %g, ldgap = %g\n", it, f, gnorm,
+                   log10(dgap));
+        if (gnorm < tol) break;
+        H.shift = mu * pow(dgap, 1.0);
+        for (long q = 0; q < N; q++) rhs[q] = -g[q];
+        const int cgi = qh_solve(&H, rhs, cg_tol, cg_max, rho <= 0.0 ? d + n : NULL, d);
+        if (cg_iters && it < hist_cap) cg_iters[it] = cgi;
+        double newf;
+        double alpha = q_line_selection2(&Q, alphas, 3, x, d, f, &newf, work);
+        if (newf >= f) {
+            if (use_small) alpha = q_line_selection2(&Q, alphas_small, 4, x, d, f, &newf, work);
+            use_small = !use_small;
+        } else {
+            use_small = 1;
+        }
+        for (long q = 0; q < N; q++) st[q] = alpha * d[q];
+        qh_apply_H(&H, st, 0.0, tau, hs);
+        double gs = 0.0, shs = 0.0;
+        for (long q = 0; q < N; q++) {
+            gs += g[q] * st[q];
+            shs += st[q] * hs[q];
+        }
+        rho = (f - newf) / (-gs - 0.5 * shs);
+        if (rho > 0.0)
+            for (long q = 0; q < N; q++) x[q] += st[q];
+        if (rho < 0.25)
+            mu *= 4.0;
+        else if (rho > 0.75)
+            mu = fmax(0.5 * mu, 0.001);
+        if (rho > 0.0) {
+            f = qrot_objgrad(&Q, x, g);
+            gnorm = q_norm2(g, N);
+            qh_free(&H);
+            qh_build(&H, n, m, M, x);
+            H.tau = tau;
+        }
+        qrot_record(&Q, x, f, gnorm);
+        prim = Q.prim_vals[Q.h - 1 < hist_cap ? Q.h - 1 : hist_cap - 1];
+    }
+    qh_free(&H);
+    free(g);
+    free(rhs);
+    free(d);
+    free(st);
+    free(hs);
+    free(work);
     *hist_len = Q.h;
     return it;
 }

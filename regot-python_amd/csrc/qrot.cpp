@@ -155,7 +155,29 @@ void apply_Delta(regot_problem* p, QrotHess& H, const double* x, double shift, d
     launch_qrot_delta_fin(x, H.h2, shift, H.tm2, c, out, m, p->stream);
 }
 
+// out (n + m) = (H + tau vv') x: [sigma z + h1 .* w; sigma' w + h2 .* z] +
+// tau (1'w - 1'z) v, x = [w; z] (src/qrot_hess.cpp:193-214 with shift 0)
+void apply_H(regot_problem* p, QrotHess& H, const double* x, double tau, double* out)
+{
+    const int n = p->n, m = p->m;
+    hipStream_t s = p->stream;
+    QrotScratch& w = scratch(p);
+    launch_qrot_sig(H.mask, n, m, true, false, nullptr, x + n, H.rowpart4, nullptr, s);
+    launch_qrot_fold(H.rowpart4, 4 * ceil_div(m, 256), n, n, H.tn, s);
+    launch_qrot_sig(H.mask, n, m, false, true, x, nullptr, nullptr, w.colpart, s);
+    launch_qrot_fold(w.colpart, ceil_div(n, qrot_rows(n, m)), m, m, H.tm2, s);
+    double c = 0.0;
+    if (tau > 0.0) c = tau * (qsum(H, s, n, x) - qsum(H, s, m, x + n));
+    launch_qrot_hfin(x, H.h1, H.tn, c, out, n, s);
+    launch_qrot_hfin(x + n, H.h2, H.tm2, -c, out + n, m, s);
+}
+
 }  // namespace
+
+void apply_qrot_hess(regot_problem* p, QrotHess& H, const double* x, double tau, double* out)
+{
+    apply_H(p, H, x, tau, out);
+}
 
 QrotHess& qrot_hess_build(regot_problem* p, const double* gamma)
 {
@@ -179,7 +201,7 @@ QrotHess& qrot_hess_build(regot_problem* p, const double* gamma)
 // (identity preconditioner) on Delta: x = 0, r = p = rhs; stop when ||r||^2 <
 // max(tol^2 ||rhs||^2, DBL_MIN) or after max_iter steps.
 int qrot_hess_solve(regot_problem* p, QrotHess& H, const double* rhs, double shift, double tau,
-                    double tol, int max_iter, double* out)
+                    double tol, int max_iter, double* out, const double* x0)
 {
     const int n = p->n, m = p->m;
     hipStream_t s = p->stream;
@@ -187,18 +209,28 @@ int qrot_hess_solve(regot_problem* p, QrotHess& H, const double* rhs, double shi
     solve_A(p, H, rhs, shift, tau, H.v);
     apply_C(p, H, H.v, tau, H.tm);
     vec_axpby(H.r, 1.0, rhs + n, -1.0, H.tm, m, s);     // r = z - C v (the CG rhs)
-    vec_copy(H.pv, H.r, m, s);
-    launch_fill(x, 0.0, m, s);
     double rhs2;
     {
         const double* a[1] = {H.r.p};
         qdots(H, s, m, 1, a, a, &rhs2);
     }
+    // from the guess (which may be x itself): r = rhs - Delta x0
+    double rr = rhs2;
+    if (x0 && rhs2 > 0.0) {
+        if (x0 != x) vec_copy(x, x0, m, s);
+        apply_Delta(p, H, x, shift, tau, H.q);
+        vec_axpby(H.r, 1.0, H.r, -1.0, H.q, m, s);
+        const double* a[1] = {H.r.p};
+        qdots(H, s, m, 1, a, a, &rr);
+    } else {
+        launch_fill(x, 0.0, m, s);
+    }
+    vec_copy(H.pv, H.r, m, s);
     int it = 0;
     if (rhs2 > 0.0) {
         const double thr = std::max(tol * tol * rhs2, 2.2250738585072014e-308);
-        double abs_new = rhs2;
-        if (rhs2 >= thr) {
+        double abs_new = rr;
+        if (rr >= thr) {
             while (it < max_iter) {
                 apply_Delta(p, H, H.pv, shift, tau, H.q);
                 double pq;
@@ -385,6 +417,117 @@ static void solve_qrot_assn(regot_problem* p, double tol, int max_iter, int verb
                 lam *= 4.0;
         }
         f = w.obj_grad(w.gamma, w.g, &gnorm);
+        record(w.gamma, f, gnorm);
+    }
+    res->niter = it;
+    w.report(res);
+    res->dual.resize(N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), w.gamma.p, N * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+// S5N (src/qrot_s5n.cpp:16-196): shift = mu (primal value + f / reg), the
+// duality gap of the rounded plan (delta = 1); direction by the CG solve,
+// warm-started from the last direction's beta block after a rejected step;
+// step from line_selection2 (src/qrot_problem.cpp:337-361) over {1, 0.5,
+// 0.1}, then {0.05, 0.01, 1e-3, 1e-4} on alternate failures; rho = (f -
+// f_new) / (-g's - s'Hs / 2) accepts the step when > 0 and moves mu by x4
+// (rho < 0.25) or x0.5 down to 1e-3 (rho > 0.75); H, f and g are recomputed
+// only after an accepted step; every iteration records f, the primal value
+// and ||g|| / reg.
+static void solve_qrot_s5n(regot_problem* p, double tol, int max_iter, int verbose,
+                           const regot_opts& o, regot_result* res)
+{
+    const int n = p->n, m = p->m;
+    const long N = static_cast<long>(n) + m;
+    hipStream_t s = p->stream;
+    const double reg = p->reg, tau = o.tau;
+    const double cg_tol = o.cg_tol > 0.0 ? o.cg_tol : (o.method == 0 ? 1e-8 : 1e-12);
+    const int cg_max = o.cg_max_iter > 0 ? o.cg_max_iter : (o.method == 0 ? 2 * m : 10 * m + 1000);
+    const double alphas[3] = {1.0, 0.5, 0.1};
+    const double alphas_small[4] = {0.05, 0.01, 0.001, 1e-4};
+    Work w(p, N);
+    w.og = [&](const double* x, double* g, double* gn2) {
+        return qrot_obj_grad(p, x, g, tau, gn2);
+    };
+    if (o.x0) {
+        HIP_CHECK(hipMemcpyAsync(w.gamma, o.x0, N * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+        launch_fill(w.gamma, 0.0, n, s);
+        qrot_optimal(p, true, w.gamma, w.gamma.p + n);
+    }
+    launch_fill(w.direc, 0.0, N, s);
+    res->qrot = true;
+    Timer timer;
+    timer.tic();
+    auto record = [&](const double* x, double f, double gnorm) {
+        const double prim = qrot_round_plan(p, x, x + n, nullptr, 0);
+        const double dur = timer.toc();
+        res->obj_vals.push_back(f);
+        res->prim_vals.push_back(prim);
+        res->mar_errs.push_back(gnorm / reg);
+        res->run_times.push_back((res->run_times.empty() ? 0.0 : res->run_times.back()) + dur);
+        timer.tic();
+    };
+    // line_selection2: the first candidate below cur, else the best one
+    auto select = [&](const double* cand, int nc, double cur, double* objval) {
+        double best = 1.0;
+        *objval = HUGE_VAL;
+        for (int k = 0; k < nc; k++) {
+            w.move(w.x, w.gamma, cand[k], w.direc);
+            const double fk = w.obj(w.x);
+            if (fk < *objval) {
+                best = cand[k];
+                *objval = fk;
+            }
+            if (*objval < cur) break;
+        }
+        return best;
+    };
+    double gnorm;
+    double f = w.obj_grad(w.gamma, w.g, &gnorm);
+    QrotHess* H = &qrot_hess_build(p, w.gamma);
+    record(w.gamma, f, gnorm);
+    double mu = o.mu0, rho = 1.0;
+    bool use_small = true;
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        const double dgap = res->prim_vals.back() + f / reg;
+        if (verbose >= 1) {
+            printf("iter = %d, objval = %g, ||grad|| = %g, ldgap = %g\n", it, f, gnorm,
+                   std::log10(dgap));
+            fflush(stdout);
+        }
+        if (gnorm < tol) break;
+        const double shift = mu * std::pow(dgap, 1.0);
+        vec_axpby(w.rhs, -1.0, w.g, 0.0, nullptr, N, s);
+        const int cgi = qrot_hess_solve(p, *H, w.rhs, shift, tau, cg_tol, cg_max, w.direc,
+                                        rho <= 0.0 ? w.direc.p + n : nullptr);
+        w.cg_iters += cgi;
+        w.alg_bytes += (2.0 * (cgi + 2)) * (n * (m / 8.0));
+        double newf;
+        double alpha = select(alphas, 3, f, &newf);
+        if (newf >= f) {
+            if (use_small) alpha = select(alphas_small, 4, f, &newf);
+            use_small = !use_small;
+        } else {
+            use_small = true;
+        }
+        vec_axpby(w.step, alpha, w.direc, 0.0, nullptr, N, s);
+        apply_qrot_hess(p, *H, w.step, tau, w.Hstep);
+        const double* xs[2] = {w.g.p, w.step.p};
+        const double* ys[2] = {w.step.p, w.Hstep.p};
+        double d[2];
+        w.dots(2, xs, ys, d);
+        rho = (f - newf) / (-d[0] - 0.5 * d[1]);
+        if (rho > 0.0) vec_axpby(w.gamma, 1.0, w.gamma, 1.0, w.step, N, s);
+        if (rho < 0.25)
+            mu *= 4.0;
+        else if (rho > 0.75)
+            mu = std::max(0.5 * mu, 0.001);
+        if (rho > 0.0) {
+            f = w.obj_grad(w.gamma, w.g, &gnorm);
+            H = &qrot_hess_build(p, w.gamma);
+        }
         record(w.gamma, f, gnorm);
     }
     res->niter = it;
@@ -595,6 +738,10 @@ void solve_qrot(regot_problem* p, int solver, double tol, int max_iter, int verb
         solve_qrot_assn(p, tol, max_iter, verbose, o, res);
         return;
     }
+    if (solver == 7) {
+        solve_qrot_s5n(p, tol, max_iter, verbose, o, res);
+        return;
+    }
     const int n = p->n, m = p->m;
     const long N = static_cast<long>(n) + m;
     hipStream_t s = p->stream;
@@ -711,7 +858,7 @@ int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int
     CAPI_BEGIN
     if (!p || !out) fail(E_INVALID, "null argument");
     *out = nullptr;
-    if (solver < 0 || solver > 6) fail(E_INVALID, "unknown QROT solver %d", solver);
+    if (solver < 0 || solver > 7) fail(E_INVALID, "unknown QROT solver %d", solver);
     if (max_iter < 0) fail(E_INVALID, "max_iter must be >= 0");
     HIP_CHECK(hipSetDevice(p->device));
     regot_opts o;
@@ -720,8 +867,10 @@ int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int
     } else {
         o = regot_opts{};
         regot_opts_default(&o, 0);
-        o.method = 0;   // QROTSolverOpts' default linear solver: CG
+        o.method = 0;   // QROTSolverOpts' defaults: CG, mu0 = 1
+        o.mu0 = 1.0;
     }
+    if (solver == 7 && !(o.mu0 > 0.0)) fail(E_INVALID, "mu0 must be > 0");
     if (solver == 5 && !(o.shift >= 0.0)) fail(E_INVALID, "shift must be >= 0");
     auto res = std::make_unique<regot_result>();
     res->solver = 100 + solver;

@@ -142,14 +142,20 @@ void launch_qrot_ainv(const double* x, const double* h, double shift, double c, 
 void launch_qrot_delta_fin(const double* x, const double* h, double shift, const double* t,
                            double c, double* out, int count, hipStream_t s);
 void launch_qrot_recip(const double* h, double shift, double* out, int count, hipStream_t s);
+// out_i = (t_i + h_i x_i) + c
+void launch_qrot_hfin(const double* x, const double* h, const double* t, double c, double* out,
+                      int count, hipStream_t s);
 
 // The generalised Hessian at device gamma (n + m) into the problem's QrotHess.
 QrotHess& qrot_hess_build(regot_problem* p, const double* gamma);
 // out (device, n + m) = (H + shift I + tau vv')^-1 rhs, v = [1; -1], by block
-// elimination with CG on the Schur complement (Eigen's rules: from zero,
-// ||r||^2 < tol^2 ||rhs_beta||^2); returns the CG steps.
+// elimination with CG on the Schur complement (Eigen's rules: from x0
+// (device, m; may alias out + n) or zero, ||r||^2 < tol^2 ||rhs_beta||^2);
+// returns the CG steps.
 int qrot_hess_solve(regot_problem* p, QrotHess& H, const double* rhs, double shift, double tau,
-                    double tol, int max_iter, double* out);
+                    double tol, int max_iter, double* out, const double* x0 = nullptr);
+// out (device, n + m) = (H + tau vv') x (src/qrot_hess.cpp:193-214, shift 0).
+void apply_qrot_hess(regot_problem* p, QrotHess& H, const double* x, double tau, double* out);
 
 // f (and grad, ||grad||^2 when grad is non-null) of the QROT dual at device
 // gamma (n + m), src/qrot_problem.cpp:226-258.

@@ -680,6 +680,16 @@ __global__ __launch_bounds__(256) void k_qrot_delta_fin(const double* __restrict
     if (i < count) out[i] = ((h[i] * x[i] + shift * x[i]) - t[i]) + c;
 }
 
+// out_i = (t_i + h_i x_i) + c
+__global__ __launch_bounds__(256) void k_qrot_hfin(const double* __restrict__ x,
+                                                   const double* __restrict__ h,
+                                                   const double* __restrict__ t, double c,
+                                                   double* __restrict__ out, int count)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count) out[i] = (t[i] + h[i] * x[i]) + c;
+}
+
 // out_i = 1 / (h_i + shift)
 __global__ __launch_bounds__(256) void k_qrot_recip(const double* __restrict__ h, double shift,
                                                     double* __restrict__ out, int count)
@@ -941,6 +951,14 @@ void launch_qrot_delta_fin(const double* x, const double* h, double shift, const
 void launch_qrot_recip(const double* h, double shift, double* out, int count, hipStream_t s)
 {
     hipLaunchKernelGGL(k_qrot_recip, dim3(ceil_div(count, 256)), dim3(256), 0, s, h, shift, out,
+                       count);
+    KERNEL_CHECK();
+}
+
+void launch_qrot_hfin(const double* x, const double* h, const double* t, double c, double* out,
+                      int count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qrot_hfin, dim3(ceil_div(count, 256)), dim3(256), 0, s, x, h, t, c, out,
                        count);
     KERNEL_CHECK();
 }

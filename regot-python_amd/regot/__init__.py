@@ -30,7 +30,7 @@ __all__ = [
     "sinkhorn_bcd", "sinkhorn_ssns", "sinkhorn_sparse_newton", "sinkhorn_splr",
     "sinkhorn_newton", "sinkhorn_lbfgs_dual", "sinkhorn_apdagd", "sinkhorn_result",
     "qrot_bcd", "qrot_gd", "qrot_lbfgs_dual", "qrot_lbfgs_semi_dual", "qrot_apdagd",
-    "qrot_grssn", "qrot_assn",
+    "qrot_grssn", "qrot_assn", "qrot_s5n",
     "qrot_result",
     "Problem", "BatchProblem", "RegotError",
 ]
@@ -322,15 +322,16 @@ class Problem:
     def qrot_solve(self, solver, tol=1e-6, max_iter=1000, verbose=0, compute_plan=True,
                    **kwargs):
         """QROT solver (_lib.QROT_BCD / QROT_GD / QROT_LBFGS_DUAL /
-        QROT_LBFGS_SEMI_DUAL / QROT_APDAGD / QROT_GRSSN / QROT_ASSN) -> qrot_result.
+        QROT_LBFGS_SEMI_DUAL / QROT_APDAGD / QROT_GRSSN / QROT_ASSN / QROT_S5N) -> qrot_result.
         Keywords: x0 (n+m warm start), tau (extra dual term; the reference's
         Python bindings never set it, src/solvers.cpp:33-51); GRSSN: method
         (default 0, CG), shift (default 1e-3), cg_tol, cg_max_iter; ASSN:
-        method, cg_tol, cg_max_iter."""
+        method, cg_tol, cg_max_iter; S5N: those and mu0 (default 1)."""
         opts = _lib.Opts()
         lib().regot_opts_default(_lib.ctypes.byref(opts), 0)
         opts.method = int(kwargs.get("method", 0))
         opts.shift = float(kwargs.get("shift", 1e-3))
+        opts.mu0 = float(kwargs.get("mu0", 1.0))
         if "cg_tol" in kwargs:
             opts.cg_tol = float(kwargs["cg_tol"])
         if "cg_max_iter" in kwargs:
@@ -748,3 +749,9 @@ def qrot_assn(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
     """Adaptive semismooth Newton on the QROT dual (src/solvers.cpp:69-83
     binding, src/qrot_assn.cpp:16-170); keyword method (default 0)."""
     return _run_qrot(_lib.QROT_ASSN, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def qrot_s5n(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Safeguarded semismooth Newton on the QROT dual (src/solvers.cpp:183-197
+    binding, src/qrot_s5n.cpp:16-196); keyword method (default 0)."""
+    return _run_qrot(_lib.QROT_S5N, M, a, b, reg, tol, max_iter, verbose, kwargs)

@@ -721,3 +721,31 @@ def test_qrot_assn_reaches_the_qp_optimum_and_history_rules():
     q = O.qrot_grssn(M, a, b, 0.1, tol=1e-9, max_iter=2000)
     assert np.max(np.abs(r["plan"] - q["plan"])) < 1e-7
     assert O.qrot_assn(M, a, b, 0.1, tol=1e9)["niter"] == 0
+
+
+def test_qrot_s5n_reaches_the_qp_optimum_and_history_rules():
+    from scipy.optimize import minimize
+    M, a, b = cloud_problem(5, 4, d=2, seed=33)
+    reg = 0.1
+    n, m = M.shape
+    cons = [{"type": "eq", "fun": lambda p: p.reshape(n, m).sum(1) - a},
+            {"type": "eq", "fun": lambda p: p.reshape(n, m).sum(0)[:-1] - b[:-1]}]
+    qp = minimize(lambda p: p @ M.ravel() + 0.5 * reg * p @ p, np.outer(a, b).ravel(),
+                  jac=lambda p: M.ravel() + reg * p, bounds=[(0, None)] * (n * m),
+                  constraints=cons, method="SLSQP", options={"ftol": 1e-15, "maxiter": 1000})
+    for method in (0, 1):
+        r = O.qrot_s5n(M, a, b, reg, tol=1e-11, max_iter=500, method=method)
+        assert r["niter"] < 100 and len(r["obj_vals"]) == r["niter"] + 1
+        assert np.max(np.abs(r["plan"] - qp.x.reshape(n, m))) < 1e-6
+        assert abs(r["prim_vals"][-1] - qp.fun) < 1e-8
+    # steps are taken only when the model ratio is positive: f never
+    # increases; the primal value of the rounded plan never drops below -f /
+    # reg (the gap that sets the shift stays >= 0); same optimum as GRSSN
+    M, a, b = cloud_problem(60, 45, d=4, seed=7)
+    r = O.qrot_s5n(M, a, b, 0.05, tol=1e-9, max_iter=500)
+    assert np.all(np.diff(r["obj_vals"]) <= 1e-15)
+    assert np.all(np.array(r["prim_vals"]) >= -np.array(r["obj_vals"]) / 0.05 - 1e-12)
+    g = np.array(r["mar_errs"]) * 0.05
+    assert g[-1] < 1e-9 and np.all(g[:-1] >= 1e-9)
+    q = O.qrot_grssn(M, a, b, 0.05, tol=1e-9, max_iter=500)
+    assert np.max(np.abs(r["plan"] - q["plan"])) < 1e-6
