@@ -286,7 +286,8 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
  * qrot_lbfgs_dual_internal, qrot_lbfgs_semi_dual_internal and
  * qrot_apdagd_internal (src/qrot_apdagd.cpp:14-181), qrot_grssn_internal
  * (src/qrot_grssn.cpp:16-133), qrot_assn_internal (src/qrot_assn.cpp:16-170)
- * and qrot_s5n_internal (src/qrot_s5n.cpp:16-196)
+ * qrot_s5n_internal (src/qrot_s5n.cpp:16-196) and qrot_pdaam_internal
+ * (src/qrot_pdaam.cpp:14-188)
  * (src/qrot_bcd.cpp:15-101, src/qrot_gd.cpp:15-121,
  * src/qrot_lbfgs_dual.cpp:85-126, src/qrot_lbfgs_semi_dual.cpp:78-118;
  * bindings src/solvers.cpp:53-164).  Its result
@@ -313,6 +314,8 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
 #define REGOT_QROT_S5N 7               /* src/qrot_s5n.cpp:16-196; reads opts->method,
                                           opts->mu0 (default 1; with NULL opts 1) and
                                           cg_tol / cg_max_iter */
+#define REGOT_QROT_PDAAM 8             /* src/qrot_pdaam.cpp:14-188; the result keeps its
+                                          averaged primal plan (regot_result_plan) */
 int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int verbose,
                      const regot_opts* opts, regot_result** out);
 int regot_result_prim_vals(const regot_result* r, double* out);

@@ -467,7 +467,7 @@ def _qrot_solve(solver, M, a, b, reg, tol, max_iter, verbose, tau=0.0, x0=None):
                                  _p(h[3]), ctypes.byref(hl), _p(plan))
     _check(niter)
     k = hl.value
-    if solver != 4:
+    if solver not in (4, 8):
         plan = qrot_get_plan(M, reg, dual)
     return dict(niter=niter, dual=dual, plan=plan,
                 obj_vals=list(h[0][:k]), prim_vals=list(h[1][:k]), mar_errs=list(h[2][:k]),
@@ -577,3 +577,8 @@ def qrot_s5n(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, method=0, mu0=1.0
     return dict(niter=niter, dual=dual, plan=qrot_get_plan(M, reg, dual),
                 obj_vals=list(h[0][:k]), prim_vals=list(h[1][:k]), mar_errs=list(h[2][:k]),
                 run_times=list(h[3][:k]), cg_iters=list(cg[:niter]))
+
+
+def qrot_pdaam(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, x0=None):
+    """src/qrot_pdaam.cpp:14-188 -> dict with the averaged primal plan under "plan" """
+    return _qrot_solve(8, M, a, b, reg, tol, max_iter, verbose, 0.0, x0)

@@ -332,6 +332,105 @@ static int qrot_apdagd(qrot_ctx* Q, double tol, int max_iter, double* x, double*
     return it;
 }
 
+/* PDAAM on the QROT dual (src/qrot_pdaam.cpp:14-188): lambda = -gamma; per
+ * iteration L_new = L / 2, doubled per failed test (at most 20 trials, the
+ * last one accepted); alpha_new = (1 + sqrt(4 alpha^2 L L_new + 1)) / (2
+ * L_new), tau = 1 / (alpha_new L_new), lambda~ = (1 - tau) lambda + tau
+ * lambda'; at gamma = -lambda~ the block with the larger gradient norm is
+ * minimised exactly (optimal_alpha or optimal_beta), accepted when phi_new >=
+ * phi~ + ||g||^2 / (2 L_new reg^2); then lambda' += (alpha / reg) g and the
+ * plan becomes (alpha_old / alpha)^2 (L_old / L) P + tau plan(-lambda~).
+ * Histories as APDAGD: f of the last objective, the primal value at
+ * -lambda~, the full marginal error of P; stop when <M, P> + reg/2 ||P||^2 -
+ * phi_new <= tol and the marginal error <= tol.  x receives -lambda~, plan
+ * the averaged P. */
+static int qrot_pdaam(qrot_ctx* Q, double tol, int max_iter, double* x, double* plan)
+{
+    const int n = Q->n, m = Q->m;
+    const long N = (long)n + m;
+    const double reg = Q->reg;
+    double* grad = (double*)malloc(sizeof(double) * N);
+    double* dual = (double*)malloc(sizeof(double) * N);
+    double* dualp = (double*)malloc(sizeof(double) * N);
+    double* dualt = (double*)malloc(sizeof(double) * N);
+    double* T = (double*)malloc(sizeof(double) * (size_t)n * m);
+    for (long q = 0; q < N; q++) dual[q] = dualp[q] = dualt[q] = -x[q];
+    orc_qrot_get_plan(n, m, Q->M, reg, x, plan);
+    double f = qrot_objgrad(Q, x, grad);
+    qrot_record(Q, x, f, q_norm2(grad, N));
+    double L = 1.0, palpha = 0.0;
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        double L_new = 0.5 * L, phi_new = 0.0, c_old = 0.0, c_new = 0.0;
+        for (int j = 0; j < 20; j++) {
+            const double palpha_new =
+                (1.0 + sqrt(4.0 * palpha * palpha * L * L_new + 1.0)) / (2.0 * L_new);
+            const double ptau = 1.0 / (palpha_new * L_new);
+            for (long q = 0; q < N; q++) {
+                dualt[q] = (1.0 - ptau) * dual[q] + ptau * dualp[q];
+                x[q] = -dualt[q];
+            }
+            f = qrot_objgrad(Q, x, grad);
+            const double phit = -f / reg;
+            double lr = 0.0, mr = 0.0;
+            for (int i = 0; i < n; i++) lr += grad[i] * grad[i];
+            for (int jj = 0; jj < m; jj++) mr += grad[n + jj] * grad[n + jj];
+            if (lr > mr)
+                orc_qrot_optimal_alpha(n, m, Q->M, Q->a, reg, x + n, x);
+            else
+                orc_qrot_optimal_beta(n, m, Q->M, Q->b, reg, x, x + n);
+            f = qrot_objgrad(Q, x, NULL);
+            phi_new = -f / reg;
+            const double rhs = phit + (lr + mr) / (2.0 * L_new * reg * reg);
+            if (phi_new >= rhs || j >= 19) {
+                const double ar = palpha / palpha_new, lrat = L / L_new;
+                c_old = ar * ar * lrat;
+                c_new = ptau;
+                palpha = palpha_new;
+                L = L_new;
+                for (long q = 0; q < N; q++) dual[q] = -x[q];
+                break;
+            }
+            L_new *= 2.0;
+        }
+        for (long q = 0; q < N; q++) {
+            dualp[q] += (palpha / reg) * grad[q];
+            x[q] = -dualt[q];
+        }
+        orc_qrot_get_plan(n, m, Q->M, reg, x, T);
+        double mp = 0.0, sq = 0.0;
+        for (long q = 0; q < (long)n * m; q++) {
+            plan[q] = c_old * plan[q] + c_new * T[q];
+            mp += Q->M[q] * plan[q];
+            sq += plan[q] * plan[q];
+        }
+        const double primal_obj = mp + 0.5 * reg * sq;
+        double ra = 0.0, rb = 0.0;
+        for (int i = 0; i < n; i++) {
+            double r = 0.0;
+            for (int jj = 0; jj < m; jj++) r += plan[(long)i * m + jj];
+            ra += (r - Q->a[i]) * (r - Q->a[i]);
+        }
+        for (int jj = 0; jj < m; jj++) {
+            double c = 0.0;
+            for (int i = 0; i < n; i++) c += plan[(long)i * m + jj];
+            rb += (c - Q->b[jj]) * (c - Q->b[jj]);
+        }
+        const double mar_err = sqrt(ra + rb);
+        qrot_record(Q, x, f, mar_err * reg);   /* qrot_record divides by reg */
+        if (Q->verbose >= 1)
+            printf("iter = %d, primal_obj = %g, phi_new = %g, mar_err = %g\n", it, primal_obj,
+                   phi_new, mar_err);
+        if (primal_obj - phi_new <= tol && mar_err <= tol) break;
+    }
+    free(grad);
+    free(dual);
+    free(dualp);
+    free(dualt);
+    free(T);
+    return it;
+}
+
 /* QROT drivers (solver 0 = BCD, 1 = gradient descent, 2 = L-BFGS on the
  * dual, 3 = L-BFGS on the semi-dual, 4 = APDAGD; plan (n x m, solver 4 only)
  * receives the averaged primal plan).  Initial point alpha = 0, beta = optimal_beta(alpha), or x0 (n+m)
@@ -376,6 +475,12 @@ int orc_qrot_solve(int solver, int n, int m, const double* M, const double* a, c
     if (solver == 4) {
         Q.tau = 0.0;
         const int it = qrot_apdagd(&Q, tol, max_iter, x, plan);
+        *hist_len = Q.h;
+        return it;
+    }
+    if (solver == 8) {
+        Q.tau = 0.0;
+        const int it = qrot_pdaam(&Q, tol, max_iter, x, plan);
         *hist_len = Q.h;
         return it;
     }

@@ -536,6 +536,137 @@ static void solve_qrot_s5n(regot_problem* p, double tol, int max_iter, int verbo
     HIP_CHECK(hipMemcpy(res->dual.data(), w.gamma.p, N * sizeof(double), hipMemcpyDeviceToHost));
 }
 
+// PDAAM on the QROT dual (src/qrot_pdaam.cpp:14-188): lambda = -gamma; per
+// iteration L_new = L / 2, doubled per failed test (at most 20 trials, the
+// last one accepted); alpha_new = (1 + sqrt(4 alpha^2 L L_new + 1)) / (2
+// L_new), tau = 1 / (alpha_new L_new), lambda~ = (1 - tau) lambda + tau
+// lambda'; at gamma = -lambda~ the block with the larger gradient norm is
+// minimised exactly (optimal_alpha or optimal_beta) and accepted when phi_new
+// >= phi~ + ||g||^2 / (2 L_new reg^2); then lambda' += (alpha / reg) g and
+// the device plan P (kept in the result) becomes (alpha_old / alpha)^2 (L_old
+// / L) P + tau plan(-lambda~) in one pass with its sums.  Histories and the
+// stop test as APDAGD; the dual is -lambda~.
+static void solve_qrot_pdaam(regot_problem* p, double tol, int max_iter, int verbose,
+                             const regot_opts& o, regot_result* res)
+{
+    const int n = p->n, m = p->m;
+    const long N = static_cast<long>(n) + m;
+    hipStream_t s = p->stream;
+    const double reg = p->reg;
+    Work w(p, N);
+    w.og = [&](const double* x, double* g, double* gn2) {
+        return qrot_obj_grad(p, x, g, 0.0, gn2);
+    };
+    if (o.x0) {
+        HIP_CHECK(hipMemcpyAsync(w.gamma, o.x0, N * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+        launch_fill(w.gamma, 0.0, n, s);
+        qrot_optimal(p, true, w.gamma, w.gamma.p + n);
+    }
+    double* dual = w.x.p;
+    double* dualp = w.xg.p;
+    double* dualt = w.rhs.p;
+    double* gamma = w.gamma.p;
+    double* grad = w.g.p;
+    vec_axpby(dual, -1.0, gamma, 0.0, nullptr, N, s);
+    vec_copy(dualp, dual, N, s);
+    vec_copy(dualt, dual, N, s);
+
+    QrotScratch& sc = scratch(p);
+    const int rows = qrot_rows(n, m);
+    const long nstrip = ceil_div(m, 256), nblk = ceil_div(n, rows);
+    DBuf<double> wpart(2 * nstrip * nblk + ceil_div(n > m ? n : m, 256)), pout(3);
+    HBuf<double> hp;
+    hp.ensure(3);
+    res->plan.alloc(static_cast<size_t>(n) * m);
+    res->plan_ld = m;
+    res->plan_device = p->device;
+    auto plan_pass = [&](double cold, double tau, bool init) {
+        launch_qrot_apd_plan(p->bulk, p->dM, p->ldm, n, m, gamma, gamma + n, reg, cold, tau, init,
+                             res->plan.p, m, p->a, p->b, sc.rowpart, sc.colpart, sc.rsum, sc.csum,
+                             wpart, pout, s);
+        HIP_CHECK(hipMemcpyAsync(hp.p, pout.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        w.alg_bytes += w.pass_bytes() + 16.0 * n * m;
+    };
+    // ||grad_alpha||^2 and ||grad_beta||^2
+    auto block_norms = [&](double* lr, double* mr) {
+        const double* xs[1] = {grad};
+        vec_dots(xs, xs, n, 1, w.dscal.p, s);
+        const double* xt[1] = {grad + n};
+        vec_dots(xt, xt, m, 1, w.dscal.p + 1, s);
+        HIP_CHECK(hipMemcpyAsync(w.hscal.p, w.dscal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        *lr = w.hscal.p[0];
+        *mr = w.hscal.p[1];
+    };
+    plan_pass(0.0, 1.0, true);
+    res->qrot = true;
+    Timer timer;
+    timer.tic();
+    double gnorm;
+    double f = w.obj_grad(gamma, grad, &gnorm);
+    double dur = timer.toc();
+    res->obj_vals.push_back(f);
+    res->prim_vals.push_back(qrot_round_plan(p, gamma, gamma + n, nullptr, 0));
+    res->mar_errs.push_back(gnorm / reg);
+    res->run_times.push_back(dur);
+    double L = 1.0, palpha = 0.0;
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        timer.tic();
+        double L_new = 0.5 * L, phi_new = 0.0, c_old = 0.0, c_new = 0.0;
+        for (int j = 0; j < 20; j++) {
+            const double palpha_new =
+                (1.0 + std::sqrt(4.0 * palpha * palpha * L * L_new + 1.0)) / (2.0 * L_new);
+            const double ptau = 1.0 / (palpha_new * L_new);
+            vec_axpby(dualt, 1.0 - ptau, dual, ptau, dualp, N, s);
+            vec_axpby(gamma, -1.0, dualt, 0.0, nullptr, N, s);
+            f = w.obj_grad(gamma, grad, nullptr);
+            const double phit = -f / reg;
+            double lr, mr;
+            block_norms(&lr, &mr);
+            if (lr > mr)
+                qrot_optimal(p, false, gamma + n, gamma);
+            else
+                qrot_optimal(p, true, gamma, gamma + n);
+            f = w.obj(gamma);
+            phi_new = -f / reg;
+            const double rhs = phit + (lr + mr) / (2.0 * L_new * reg * reg);
+            if (phi_new >= rhs || j >= 19) {
+                const double ar = palpha / palpha_new, lrat = L / L_new;
+                c_old = ar * ar * lrat;
+                c_new = ptau;
+                palpha = palpha_new;
+                L = L_new;
+                vec_axpby(dual, -1.0, gamma, 0.0, nullptr, N, s);
+                break;
+            }
+            L_new *= 2.0;
+        }
+        vec_axpby(dualp, 1.0, dualp, palpha / reg, grad, N, s);
+        vec_axpby(gamma, -1.0, dualt, 0.0, nullptr, N, s);
+        plan_pass(c_old, c_new, false);
+        const double primal_obj = hp.p[0] + 0.5 * reg * hp.p[1];
+        const double mar_err = std::sqrt(hp.p[2]);
+        dur = timer.toc();
+        res->obj_vals.push_back(f);
+        res->prim_vals.push_back(qrot_round_plan(p, gamma, gamma + n, nullptr, 0));
+        res->mar_errs.push_back(mar_err);
+        res->run_times.push_back(res->run_times.back() + dur);
+        if (verbose >= 1) {
+            printf("iter = %d, primal_obj = %g, phi_new = %g, mar_err = %g\n", it, primal_obj,
+                   phi_new, mar_err);
+            fflush(stdout);
+        }
+        if (primal_obj - phi_new <= tol && mar_err <= tol) break;
+    }
+    res->niter = it;
+    w.report(res);
+    res->dual.resize(N);
+    HIP_CHECK(hipMemcpy(res->dual.data(), gamma, N * sizeof(double), hipMemcpyDeviceToHost));
+}
+
 // Drivers.  Semantics: src/qrot_bcd.cpp:15-101 (tau forced to 0),
 // src/qrot_gd.cpp:15-121, src/qrot_lbfgs_dual.cpp:85-126 (epsilon =
 // epsilon_rel = tol).  Histories: obj_vals f, prim_vals the primal value of
@@ -646,7 +777,8 @@ static void solve_qrot_apdagd(regot_problem* p, double tol, int max_iter, int ve
     res->plan_ld = m;
     res->plan_device = p->device;
     auto plan_pass = [&](double tau, bool init) {
-        launch_qrot_apd_plan(p->bulk, p->dM, p->ldm, n, m, gamma, gamma + n, reg, tau, init,
+        launch_qrot_apd_plan(p->bulk, p->dM, p->ldm, n, m, gamma, gamma + n, reg, 1.0 - tau,
+                             tau, init,
                              res->plan.p, m, p->a, p->b, sc.rowpart, sc.colpart, sc.rsum, sc.csum,
                              wpart, pout, s);
         HIP_CHECK(hipMemcpyAsync(hp.p, pout.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -740,6 +872,10 @@ void solve_qrot(regot_problem* p, int solver, double tol, int max_iter, int verb
     }
     if (solver == 7) {
         solve_qrot_s5n(p, tol, max_iter, verbose, o, res);
+        return;
+    }
+    if (solver == 8) {
+        solve_qrot_pdaam(p, tol, max_iter, verbose, o, res);
         return;
     }
     const int n = p->n, m = p->m;
@@ -858,7 +994,7 @@ int regot_qrot_solve(regot_problem* p, int solver, double tol, int max_iter, int
     CAPI_BEGIN
     if (!p || !out) fail(E_INVALID, "null argument");
     *out = nullptr;
-    if (solver < 0 || solver > 7) fail(E_INVALID, "unknown QROT solver %d", solver);
+    if (solver < 0 || solver > 8) fail(E_INVALID, "unknown QROT solver %d", solver);
     if (max_iter < 0) fail(E_INVALID, "max_iter must be >= 0");
     HIP_CHECK(hipSetDevice(p->device));
     regot_opts o;

@@ -117,11 +117,12 @@ void launch_qrot_finish(const double* alpha, const double* beta, const double* a
 
 void launch_qrot_fold(const double* part, int nparts, long stride, int count, double* out,
                       hipStream_t s);
-// APDAGD's plan pass P = (1 - tau) P + tau plan(alpha, beta) (init: P = plan),
+// APDAGD's / PDAAM's plan pass P = cold P + tau plan(alpha, beta) (init: P = plan),
 // then out[0] = <M, P>, out[1] = ||P||^2, out[2] = ||P1 - a||^2 + ||P'1 - b||^2
 // (device); scratch as documented at the definition.
 void launch_qrot_apd_plan(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
-                          const double* beta, double reg, double tau, bool init, double* P,
+                          const double* beta, double reg, double cold, double tau, bool init,
+                          double* P,
                           long ldp, const double* a, const double* b, double* rowpart,
                           double* colpart, double* rsum, double* csum, double* wpart,
                           double* out, hipStream_t s);

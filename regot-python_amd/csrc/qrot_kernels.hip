@@ -328,14 +328,15 @@ __global__ void k_qrot_pl_init(double* xc, int* kprev, int* done, int count, int
     done[t] = 0;
 }
 
-// APDAGD's primal plan: P = (1 - tau) P + tau (alpha_i + beta_j - M_ij)_+ / reg
+// APDAGD's / PDAAM's primal plan: P = cold P + tau (alpha_i + beta_j - M_ij)_+ / reg
 // (INIT: P = the plan itself, P not read), written back, with row sums per
 // strip, column sums per block and <M, P>, ||P||^2 per workgroup
 // (mpart, spart[blk * strips + strip]).
 template <typename TM, bool INIT>
 __global__ __launch_bounds__(256) void k_qrot_apd_plan(
     const TM* __restrict__ M, long ldm, int n, int m, const double* __restrict__ alpha,
-    const double* __restrict__ beta, double reg, double tau, double* __restrict__ P, long ldp,
+    const double* __restrict__ beta, double reg, double cold, double tau,
+    double* __restrict__ P, long ldp,
     int rows, double* __restrict__ rowpart, double* __restrict__ colpart,
     double* __restrict__ mpart, double* __restrict__ spart)
 {
@@ -361,7 +362,7 @@ __global__ __launch_bounds__(256) void k_qrot_apd_plan(
             if (j < m) {
                 const double c = (ai + bq[q]) - mv[q];
                 const double t = (c > 0.0 ? c : 0.0) / reg;
-                const double v = INIT ? t : (1.0 - tau) * Pr[j] + tau * t;
+                const double v = INIT ? t : cold * Pr[j] + tau * t;
                 Pr[j] = v;
                 rs += v;
                 cs[q] += v;
@@ -821,7 +822,8 @@ void launch_qrot_plan(int bulk, const void* M, long ldm, int n, int m, const dou
 // x n), colpart (blocks x m), rsum (n), csum (m), wpart (2 strips blocks +
 // ceil(max(n, m) / 256)).
 void launch_qrot_apd_plan(int bulk, const void* M, long ldm, int n, int m, const double* alpha,
-                          const double* beta, double reg, double tau, bool init, double* P,
+                          const double* beta, double reg, double cold, double tau, bool init,
+                          double* P,
                           long ldp, const double* a, const double* b, double* rowpart,
                           double* colpart, double* rsum, double* csum, double* wpart,
                           double* out, hipStream_t s)
@@ -836,21 +838,21 @@ void launch_qrot_apd_plan(int bulk, const void* M, long ldm, int n, int m, const
         const float* Mf = static_cast<const float*>(M);
         if (init)
             hipLaunchKernelGGL((k_qrot_apd_plan<float, true>), grid, dim3(256), 0, s, Mf, ldm, n,
-                               m, alpha, beta, reg, tau, P, ldp, rows, rowpart, colpart, mpart,
+                               m, alpha, beta, reg, cold, tau, P, ldp, rows, rowpart, colpart, mpart,
                                spart);
         else
             hipLaunchKernelGGL((k_qrot_apd_plan<float, false>), grid, dim3(256), 0, s, Mf, ldm,
-                               n, m, alpha, beta, reg, tau, P, ldp, rows, rowpart, colpart, mpart,
+                               n, m, alpha, beta, reg, cold, tau, P, ldp, rows, rowpart, colpart, mpart,
                                spart);
     } else {
         const double* Md = static_cast<const double*>(M);
         if (init)
             hipLaunchKernelGGL((k_qrot_apd_plan<double, true>), grid, dim3(256), 0, s, Md, ldm,
-                               n, m, alpha, beta, reg, tau, P, ldp, rows, rowpart, colpart, mpart,
+                               n, m, alpha, beta, reg, cold, tau, P, ldp, rows, rowpart, colpart, mpart,
                                spart);
         else
             hipLaunchKernelGGL((k_qrot_apd_plan<double, false>), grid, dim3(256), 0, s, Md, ldm,
-                               n, m, alpha, beta, reg, tau, P, ldp, rows, rowpart, colpart,
+                               n, m, alpha, beta, reg, cold, tau, P, ldp, rows, rowpart, colpart,
                                mpart, spart);
     }
     KERNEL_CHECK();

@@ -30,7 +30,7 @@ __all__ = [
     "sinkhorn_bcd", "sinkhorn_ssns", "sinkhorn_sparse_newton", "sinkhorn_splr",
     "sinkhorn_newton", "sinkhorn_lbfgs_dual", "sinkhorn_apdagd", "sinkhorn_result",
     "qrot_bcd", "qrot_gd", "qrot_lbfgs_dual", "qrot_lbfgs_semi_dual", "qrot_apdagd",
-    "qrot_grssn", "qrot_assn", "qrot_s5n",
+    "qrot_grssn", "qrot_assn", "qrot_s5n", "qrot_pdaam",
     "qrot_result",
     "Problem", "BatchProblem", "RegotError",
 ]
@@ -322,7 +322,8 @@ class Problem:
     def qrot_solve(self, solver, tol=1e-6, max_iter=1000, verbose=0, compute_plan=True,
                    **kwargs):
         """QROT solver (_lib.QROT_BCD / QROT_GD / QROT_LBFGS_DUAL /
-        QROT_LBFGS_SEMI_DUAL / QROT_APDAGD / QROT_GRSSN / QROT_ASSN / QROT_S5N) -> qrot_result.
+        QROT_LBFGS_SEMI_DUAL / QROT_APDAGD / QROT_GRSSN / QROT_ASSN / QROT_S5N /
+        QROT_PDAAM) -> qrot_result.
         Keywords: x0 (n+m warm start), tau (extra dual term; the reference's
         Python bindings never set it, src/solvers.cpp:33-51); GRSSN: method
         (default 0, CG), shift (default 1e-3), cg_tol, cg_max_iter; ASSN:
@@ -755,3 +756,10 @@ def qrot_s5n(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
     """Safeguarded semismooth Newton on the QROT dual (src/solvers.cpp:183-197
     binding, src/qrot_s5n.cpp:16-196); keyword method (default 0)."""
     return _run_qrot(_lib.QROT_S5N, M, a, b, reg, tol, max_iter, verbose, kwargs)
+
+
+def qrot_pdaam(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kwargs):
+    """Primal-dual accelerated alternating minimisation on the QROT dual
+    (src/solvers.cpp:166-180 binding, src/qrot_pdaam.cpp:14-188); the result's plan
+    is the averaged primal plan."""
+    return _run_qrot(_lib.QROT_PDAAM, M, a, b, reg, tol, max_iter, verbose, kwargs)

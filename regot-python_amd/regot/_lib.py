@@ -27,6 +27,7 @@ QROT_APDAGD = 4
 QROT_GRSSN = 5
 QROT_ASSN = 6
 QROT_S5N = 7
+QROT_PDAAM = 8
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p

@@ -256,3 +256,22 @@ def test_qrot_s5n_matches_oracle(gpu_lib, shape, reg):
     np.testing.assert_allclose(r.prim_vals[:k], o["prim_vals"][:k], rtol=1e-8)
     assert r.mar_errs[-1] * reg < tol
     np.testing.assert_allclose(r.plan, o["plan"], atol=1e-6 * np.max(o["plan"]))
+
+
+def test_qrot_pdaam(gpu_lib):
+    M, a, b = cloud_problem(40, 30, d=4, seed=3)
+    reg, tol = 0.1, 1e-5
+    print("qrot pdaam 40x30", flush=True)
+    r = gpu_lib.qrot_pdaam(M, a, b, reg, tol=tol, max_iter=20000)
+    o = O.qrot_pdaam(M, a, b, reg, tol=tol, max_iter=20000)
+    assert r.niter < 20000 and abs(r.niter - o["niter"]) <= max(2, 0.02 * o["niter"])
+    assert len(r.obj_vals) == len(r.prim_vals) == r.niter + 2
+    k = min(50, len(r.obj_vals), len(o["obj_vals"]))
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-10)
+    np.testing.assert_allclose(r.prim_vals[:k], o["prim_vals"][:k], rtol=1e-9)
+    np.testing.assert_allclose(r.mar_errs[:k], o["mar_errs"][:k], rtol=1e-8)
+    # the result's plan is the averaged primal plan, within tol at the stop
+    P = r.plan
+    full = np.sqrt(((P.sum(1) - a) ** 2).sum() + ((P.sum(0) - b) ** 2).sum())
+    assert full <= tol and abs(full - r.mar_errs[-1]) < 1e-12
+    np.testing.assert_allclose(P, o["plan"], rtol=1e-5, atol=1e-8)

@@ -749,3 +749,30 @@ def test_qrot_s5n_reaches_the_qp_optimum_and_history_rules():
     assert g[-1] < 1e-9 and np.all(g[:-1] >= 1e-9)
     q = O.qrot_grssn(M, a, b, 0.05, tol=1e-9, max_iter=500)
     assert np.max(np.abs(r["plan"] - q["plan"])) < 1e-6
+
+
+def test_qrot_pdaam_reaches_the_qp_optimum():
+    # the averaged primal plan of PDAAM against scipy's SLSQP solution of the
+    # primal QP; the stop rule (gap and full marginal error <= tol) holds; the
+    # same plan as APDAGD on a larger problem
+    from scipy.optimize import minimize
+    M, a, b = cloud_problem(5, 4, d=2, seed=33)
+    reg, tol = 0.1, 1e-8
+    n, m = M.shape
+    cons = [{"type": "eq", "fun": lambda p: p.reshape(n, m).sum(1) - a},
+            {"type": "eq", "fun": lambda p: p.reshape(n, m).sum(0)[:-1] - b[:-1]}]
+    qp = minimize(lambda p: p @ M.ravel() + 0.5 * reg * p @ p, np.outer(a, b).ravel(),
+                  jac=lambda p: M.ravel() + reg * p, bounds=[(0, None)] * (n * m),
+                  constraints=cons, method="SLSQP", options={"ftol": 1e-15, "maxiter": 1000})
+    r = O.qrot_pdaam(M, a, b, reg, tol=tol, max_iter=2000000)
+    P = r["plan"]
+    assert len(r["obj_vals"]) == r["niter"] + 2
+    full = np.sqrt(((P.sum(1) - a) ** 2).sum() + ((P.sum(0) - b) ** 2).sum())
+    assert full <= tol and abs(full - r["mar_errs"][-1]) < 1e-15
+    assert np.max(np.abs(P - qp.x.reshape(n, m))) < 1e-5
+    primal = (P * M).sum() + 0.5 * reg * (P * P).sum()
+    assert abs(primal - qp.fun) < 1e-7
+    M, a, b = cloud_problem(40, 30, d=4, seed=3)
+    r = O.qrot_pdaam(M, a, b, 0.1, tol=1e-5, max_iter=200000)
+    q = O.qrot_apdagd(M, a, b, 0.1, tol=1e-5, max_iter=200000)
+    assert np.max(np.abs(r["plan"] - q["plan"])) < 1e-4 * np.max(q["plan"])
