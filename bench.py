@@ -336,7 +336,9 @@ def bench_qrot(n):
     includes the primal value of the rounded plan, as the reference's
     histories do; 5 GRSSN (semismooth Newton) iterations with the reference's
     defaults (Schur-complement CG to 1e-8, shift 1e-3) on the bitmask
-    Hessian.  One warmup and one timed solve each."""
+    Hessian, 5 each of ASSN and S5N, 20 each of APDAGD and PDAAM (averaged
+    n x m FP64 plan on the device) and of L-BFGS on the semi-dual.  One
+    warmup and one timed solve each."""
     import regot
     from regot import _lib
     M, a, b = make_problem(n, n, 16, 0)
@@ -345,7 +347,12 @@ def bench_qrot(n):
         for name, sid, tol, mi in (("lbfgs_dual", _lib.QROT_LBFGS_DUAL, 1e-6, 500),
                                    ("bcd", _lib.QROT_BCD, 1e-6, 20),
                                    ("gd", _lib.QROT_GD, 1e-6, 20),
-                                   ("grssn", _lib.QROT_GRSSN, 1e-6, 5)):
+                                   ("grssn", _lib.QROT_GRSSN, 1e-6, 5),
+                                   ("assn", _lib.QROT_ASSN, 1e-6, 5),
+                                   ("s5n", _lib.QROT_S5N, 1e-6, 5),
+                                   ("apdagd", _lib.QROT_APDAGD, 1e-6, 20),
+                                   ("pdaam", _lib.QROT_PDAAM, 1e-6, 20),
+                                   ("lbfgs_semi_dual", _lib.QROT_LBFGS_SEMI_DUAL, 1e-6, 20)):
             P.qrot_solve(sid, tol=tol, max_iter=mi, compute_plan=False)
             t0 = time.perf_counter()
             r = P.qrot_solve(sid, tol=tol, max_iter=mi, compute_plan=False)
