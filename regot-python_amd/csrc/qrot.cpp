@@ -53,18 +53,22 @@ double qrot_obj_grad(regot_problem* p, const double* gamma, double* grad, double
 // Newton's method from the right on sum_k (x - y_k)_+ = reg w per column
 // (cols) or row: the first step takes every y (x = mean + reg w / count), each
 // later one the y below the current x, until the active set stops shrinking.
-// The iterates decrease monotonically, so at most count + 1 steps run; the
-// root is (reg w + sum of the active y) / (their count), the formula of
-// piecewise_linear_equation, src/qrot_problem.cpp:365-394.
+// The iterates decrease monotonically and every step but the last drops at
+// least one breakpoint from the active set, so an equation with `nbreak`
+// breakpoints (n per column equation, m per row equation) settles within
+// nbreak + 1 steps; the root is (reg w + sum of the active y) / (their
+// count), the formula of piecewise_linear_equation,
+// src/qrot_problem.cpp:365-394.
 void qrot_optimal(regot_problem* p, bool cols, const double* pot, double* out)
 {
     const int n = p->n, m = p->m;
     hipStream_t s = p->stream;
     QrotScratch& w = scratch(p);
     const int count = cols ? m : n;
+    const int nbreak = cols ? n : m;
     launch_qrot_pl_init(out, w.kprev, w.done, count, w.open, s);
     const double* wt = cols ? p->b.p : p->a.p;
-    for (int step = 0; step <= count + 1; step++) {
+    for (int step = 0; step <= nbreak + 1; step++) {
         launch_qrot_pl_step(p->bulk, p->dM, p->ldm, n, m, cols, pot, wt, p->reg, out, w.kprev,
                             w.done, w.spart, w.kpart, w.open, s);
         HIP_CHECK(hipMemcpyAsync(w.hopen.p, w.open.p, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -144,6 +144,15 @@ def _as_bulk(M, bulk):
     return np.ascontiguousarray(M, dtype=dt), bulk
 
 
+def _vec(x, length, name):
+    """FP64 contiguous copy of a 1-D input, checked against the length the C
+    side reads (a short host buffer would be read past its end)."""
+    v = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    if v.shape[0] != length:
+        raise ValueError("%s must have %d entries (got %d)" % (name, length, v.shape[0]))
+    return v
+
+
 class Problem:
     """One EROT problem resident on a GPU (``regot_problem`` handle).
 
@@ -170,8 +179,8 @@ class Problem:
     def from_device(cls, ptr, n, m, dtype, a, b, reg, bulk=None, ldm=0, device=0):
         """Borrow (or convert) a device-resident M given by raw pointer."""
         self = cls.__new__(cls)
-        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
-        b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+        a = _vec(a, n, "a")
+        b = _vec(b, m, "b")
         dcode = _lib.REGOT_F32 if np.dtype(dtype) == np.float32 else _lib.REGOT_F64
         bulk = bulk or ("fp32" if dcode == _lib.REGOT_F32 else "fp64")
         bcode = _lib.REGOT_F32 if bulk == "fp32" else _lib.REGOT_F64
@@ -203,28 +212,28 @@ class Problem:
     # --- operators -------------------------------------------------------
     def optimal_alpha(self, beta):
         """src/sinkhorn_problem_bcd.cpp:177-201"""
-        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        beta = _vec(beta, self.m, "beta")
         out = np.empty(self.n)
         check(lib().regot_optimal_alpha(self._h, dptr(beta), dptr(out)))
         return out
 
     def optimal_beta(self, alpha):
         """src/sinkhorn_problem_bcd.cpp:149-174"""
-        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        alpha = _vec(alpha, self.n, "alpha")
         out = np.empty(self.m)
         check(lib().regot_optimal_beta(self._h, dptr(alpha), dptr(out)))
         return out
 
     def dual_obj(self, gamma):
         """src/sinkhorn_problem_obj.cpp:324-342"""
-        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        gamma = _vec(gamma, self.n + self.m - 1, "gamma")
         f = _lib.ctypes.c_double()
         check(lib().regot_dual_obj(self._h, dptr(gamma), _lib.ctypes.byref(f)))
         return f.value
 
     def dual_obj_grad(self, gamma):
         """src/sinkhorn_problem_obj.cpp:363-393 -> (f, grad)"""
-        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        gamma = _vec(gamma, self.n + self.m - 1, "gamma")
         f = _lib.ctypes.c_double()
         g = np.empty(self.n + self.m - 1)
         check(lib().regot_dual_obj_grad(self._h, dptr(gamma), _lib.ctypes.byref(f), dptr(g)))
@@ -236,7 +245,7 @@ class Problem:
         written to the device array at ``G_ptr`` (n x m, row pitch ``ldg``).
         ``loss`` is "entropic" (<T,M> - reg h(T)) or "sharp" (<T,M>).
         Returns (loss value, w, CG iterations, CG relative residual)."""
-        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        gamma = _vec(gamma, self.n + self.m - 1, "gamma")
         code = {"entropic": 0, "sharp": 1}[loss]
         gcode = _lib.REGOT_F32 if np.dtype(G_dtype) == np.float32 else _lib.REGOT_F64
         lv = _lib.ctypes.c_double()
@@ -251,7 +260,7 @@ class Problem:
 
     def get_plan(self, gamma):
         """src/sinkhorn_result.h:32-50"""
-        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        gamma = _vec(gamma, self.n + self.m - 1, "gamma")
         plan = np.empty((self.n, self.m))
         check(lib().regot_get_plan(self._h, dptr(gamma), dptr(plan)))
         return plan
@@ -259,14 +268,14 @@ class Problem:
     def round_plan(self, gamma):
         """approx_proj of plan(gamma): the nearest-feasible rounding onto the
         transport polytope (src/approx_proj.h:6-26)."""
-        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        g = _vec(gamma, self.n + self.m - 1, "gamma")
         plan = np.empty((self.n, self.m))
         check(lib().regot_round_plan(self._h, dptr(g), dptr(plan)))
         return plan
 
     def primal_val(self, gamma):
         """Primal objective of the rounded plan (src/sinkhorn_problem_obj.cpp:300-319)."""
-        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        g = _vec(gamma, self.n + self.m - 1, "gamma")
         v = _lib.ctypes.c_double()
         check(lib().regot_primal_val(self._h, dptr(g), _lib.ctypes.byref(v)))
         return v.value
@@ -275,9 +284,7 @@ class Problem:
     def qrot_obj_grad(self, gamma, tau=0.0):
         """QROT dual (f, grad) at gamma = [alpha; beta] (n+m)
         (src/qrot_problem.cpp:226-258)."""
-        g = np.ascontiguousarray(gamma, dtype=np.float64)
-        if g.shape[0] != self.n + self.m:
-            raise ValueError("gamma must have n+m entries")
+        g = _vec(gamma, self.n + self.m, "gamma")
         f = _lib.ctypes.c_double()
         grad = np.empty(self.n + self.m)
         check(lib().regot_qrot_obj_grad(self._h, dptr(g), float(tau), _lib.ctypes.byref(f),
@@ -286,21 +293,21 @@ class Problem:
 
     def qrot_optimal_beta(self, alpha):
         """QROT optimal_beta (src/qrot_problem.cpp:397-406)."""
-        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        a = _vec(alpha, self.n, "alpha")
         beta = np.empty(self.m)
         check(lib().regot_qrot_optimal_beta(self._h, dptr(a), dptr(beta)))
         return beta
 
     def qrot_optimal_alpha(self, beta):
         """QROT optimal_alpha (src/qrot_problem.cpp:409-418)."""
-        b = np.ascontiguousarray(beta, dtype=np.float64)
+        b = _vec(beta, self.m, "beta")
         alpha = np.empty(self.n)
         check(lib().regot_qrot_optimal_alpha(self._h, dptr(b), dptr(alpha)))
         return alpha
 
     def qrot_primal_val(self, gamma):
         """QROT primal value of the rounded plan (src/qrot_problem.cpp:10-26)."""
-        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        g = _vec(gamma, self.n + self.m, "gamma")
         v = _lib.ctypes.c_double()
         check(lib().regot_qrot_primal_val(self._h, dptr(g), _lib.ctypes.byref(v)))
         return v.value
@@ -308,10 +315,8 @@ class Problem:
     def qrot_hess_solve(self, gamma, rhs, shift, tau=0.0, tol=1e-8, max_iter=0):
         """(H + shift I + tau vv')^-1 rhs with the QROT generalised Hessian at
         gamma (src/qrot_hess.cpp:42-96, src/qrot_cg.cpp:9-51) -> (x, CG steps)."""
-        g = np.ascontiguousarray(gamma, dtype=np.float64)
-        r = np.ascontiguousarray(rhs, dtype=np.float64)
-        if g.shape[0] != self.n + self.m or r.shape[0] != self.n + self.m:
-            raise ValueError("gamma and rhs must have n+m = %d entries" % (self.n + self.m))
+        g = _vec(gamma, self.n + self.m, "gamma")
+        r = _vec(rhs, self.n + self.m, "rhs")
         out = np.empty(self.n + self.m)
         it = _lib.ctypes.c_int(0)
         check(lib().regot_qrot_hess_solve(self._h, dptr(g), dptr(r), float(shift), float(tau),
@@ -373,7 +378,7 @@ class Problem:
     # --- sparsified Hessian ------------------------------------------------
     def sparsified_hess(self, gamma, delta, density_hint=0.001):
         """Threshold rule at gamma (src/sinkhorn_sparsify3.cpp:276-405); returns nnz."""
-        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        gamma = _vec(gamma, self.n + self.m - 1, "gamma")
         nnz = _lib.ctypes.c_longlong()
         check(lib().regot_sparsified_hess(self._h, dptr(gamma), float(delta), float(density_hint),
                                           _lib.ctypes.byref(nnz)))
@@ -381,7 +386,7 @@ class Problem:
 
     def sparsified_hess_with_density(self, gamma, density):
         """Top-k rule at gamma (src/sinkhorn_sparsify4.cpp:77-174); returns stored nnz."""
-        gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+        gamma = _vec(gamma, self.n + self.m - 1, "gamma")
         nnz = _lib.ctypes.c_longlong()
         check(lib().regot_sparsified_hess_with_density(self._h, dptr(gamma), float(density),
                                                        _lib.ctypes.byref(nnz)))
@@ -413,20 +418,20 @@ class Problem:
         return out
 
     def hess_apply_delta(self, x, shift):
-        x = np.ascontiguousarray(x, dtype=np.float64)
+        x = _vec(x, self.m - 1, "x")
         out = np.empty(self.m - 1)
         check(lib().regot_hess_apply_delta(self._h, dptr(x), float(shift), dptr(out)))
         return out
 
     def hess_apply_hs(self, x):
-        x = np.ascontiguousarray(x, dtype=np.float64)
+        x = _vec(x, self.n + self.m - 1, "x")
         out = np.empty(self.n + self.m - 1)
         check(lib().regot_hess_apply_hs(self._h, dptr(x), dptr(out)))
         return out
 
     def hess_cg(self, rhs, shift, tol=1e-8, max_iter=0):
         """(Hs + shift I)^{-1} rhs -> (x, iterations, relative residual)"""
-        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        rhs = _vec(rhs, self.n + self.m - 1, "rhs")
         out = np.empty(self.n + self.m - 1)
         it = _lib.ctypes.c_int()
         te = _lib.ctypes.c_double()

@@ -66,3 +66,27 @@ def test_result_object_copies_like_def_readwrite():
     assert r.obj_vals == [1.0, 2.0] and isinstance(r.obj_vals[0], float)
     r.niter = 5.0
     assert r.niter == 5 and isinstance(r.niter, int)
+
+
+def test_operator_vectors_checked_against_the_handle():
+    # every Problem operator hands a host pointer to C, which reads n, m,
+    # n+m-1 or n+m doubles from it: a short vector must raise before the call
+    import regot
+    P = regot.Problem.__new__(regot.Problem)
+    P.n, P.m, P.reg, P.bulk, P._h = 5, 4, 0.1, "fp64", None
+    bad = np.zeros(3)
+    for call in (lambda: P.optimal_alpha(bad), lambda: P.optimal_beta(bad),
+                 lambda: P.dual_obj(bad), lambda: P.dual_obj_grad(bad),
+                 lambda: P.get_plan(bad), lambda: P.round_plan(bad),
+                 lambda: P.primal_val(bad), lambda: P.sparsified_hess(bad, 0.1),
+                 lambda: P.sparsified_hess_with_density(bad, 0.1),
+                 lambda: P.hess_apply_delta(np.zeros(4), 0.1), lambda: P.hess_apply_hs(bad),
+                 lambda: P.hess_cg(bad, 0.1), lambda: P.qrot_obj_grad(np.zeros(8)),
+                 lambda: P.qrot_optimal_alpha(bad), lambda: P.qrot_optimal_beta(bad),
+                 lambda: P.qrot_primal_val(np.zeros(8)),
+                 lambda: P.qrot_hess_solve(np.zeros(9), bad, 0.1),
+                 lambda: P.implicit_grad(bad, 0, np.float32)):
+        with pytest.raises(ValueError):
+            call()
+    with pytest.raises(ValueError):
+        regot.Problem.from_device(0, 5, 4, np.float32, np.ones(4), np.ones(4), 0.1)

@@ -44,6 +44,31 @@ def test_qrot_passes(gpu_lib, bulk, shape):
         assert abs(pv - O.qrot_primal_val(M, a, b, reg, gamma)) < 1e-12 * max(1.0, abs(pv))
 
 
+@pytest.mark.parametrize("shape", [(2000, 3), (3, 2000), (5000, 4)])
+def test_qrot_tall_and_wide(gpu_lib, shape):
+    # Each column equation has n breakpoints and each row equation m, so a
+    # root search bounded by the other dimension gives up on tall (wide)
+    # inputs; both half-steps and BCD must match the oracle here.
+    n, m = shape
+    M, a, b = cloud_problem(n, m, d=3, seed=n + 7 * m)
+    reg = 0.05
+    rng = np.random.default_rng(2)
+    alpha = 0.05 * rng.standard_normal(n)
+    print("qrot tall/wide %dx%d" % (n, m), flush=True)
+    with gpu_lib.Problem(M, a, b, reg, bulk="fp64") as P:
+        beta = P.qrot_optimal_beta(alpha)
+        beta_o = O.qrot_optimal_beta(M, b, reg, alpha)
+        assert rel(beta, beta_o) < 1e-13
+        alpha2 = P.qrot_optimal_alpha(beta_o)
+        assert rel(alpha2, O.qrot_optimal_alpha(M, a, reg, beta_o)) < 1e-13
+    r = gpu_lib.qrot_bcd(M, a, b, reg, tol=1e-9, max_iter=300)
+    o = O.qrot_bcd(M, a, b, reg, tol=1e-9, max_iter=300)
+    assert abs(r.niter - o["niter"]) <= 1
+    k = min(len(r.obj_vals), len(o["obj_vals"]))
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-11)
+    assert rel(r.dual, o["dual"]) < 1e-9
+
+
 SOLVERS = [("qrot_bcd", 0), ("qrot_gd", 1), ("qrot_lbfgs_dual", 2), ("qrot_lbfgs_semi_dual", 3)]
 
 
