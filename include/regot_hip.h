@@ -157,6 +157,9 @@ typedef struct {
     int bcd_fused;
     double tau;          /* QROT gradient descent / L-BFGS: extra term tau/2 (1'alpha - 1'beta)^2
                             (QROTSolverOpts::tau, src/qrot_solvers.h:15-30); default 0 */
+    int ls_max_iter;     /* strong-Wolfe trials of sparse Newton / SPLR (the max_iter argument
+                            of Problem::line_search_wolfe, src/sinkhorn_problem.h:136-144);
+                            <= 0: the reference's 20 */
 } regot_opts;
 
 void regot_opts_default(regot_opts* o, int solver);
@@ -198,6 +201,12 @@ int regot_result_verify(regot_problem* p, const regot_result* r, double* mar_err
  * read of M per objective or gradient, one read of M plus the CSR and CSC
  * writes per Hessian build and the CSR + CSC reads per Hessian product. */
 int regot_result_stats(const regot_result* r, double* out);
+/* BCD in FP32 bulk (no reference counterpart; the reference runs FP64
+ * throughout): the iterate from which the loop ran with FP64 exp and sums
+ * over the FP32 M -- 0 when tol lies below the FP32 resolution of the
+ * marginals from the start, k when ||g|| stalled above tol at iterate k --
+ * or -1 when every iteration ran in FP32. */
+int regot_result_precise_from(const regot_result* r);
 void regot_result_free(regot_result* r);
 
 /*

@@ -622,18 +622,20 @@ typedef struct {
     double* T; /* n x m */
 } prob_t;
 
-/* First candidate step that lowers f, else the best one; T is left at the
- * last evaluated candidate.  src/sinkhorn_problem_ls.cpp:13-41 */
-static double line_selection(const prob_t* P, const double* cand, int nc, const double* gamma,
-                             const double* direc, double curobj, double* objval, double* work)
+/* First candidate step that lowers f, else the best one (the first of equal
+ * values, since only a strictly smaller value replaces it); *last_cand is the
+ * last candidate evaluated, whose T the caller holds afterwards.
+ * src/sinkhorn_problem_ls.cpp:13-41 */
+double orc_line_selection(long N, orc_obj_fn fobj, void* ctx, const double* cand, int nc,
+                          const double* gamma, const double* direc, double curobj,
+                          double* objval, double* work, double* last_cand)
 {
-    const long N = P->n + P->m - 1;
     double best = 1.0;
     *objval = INFINITY;
     for (int c = 0; c < nc; c++) {
         for (long i = 0; i < N; i++) work[i] = gamma[i] + cand[c] * direc[i];
-        double Tsum = orc_compute_T(P->n, P->m, P->M, P->reg, work, P->T);
-        double f = P->reg * Tsum - dot(work, P->a, P->n) - dot(work + P->n, P->b, P->m - 1);
+        const double f = fobj(ctx, work);
+        if (last_cand) *last_cand = cand[c];
         if (f < *objval) {
             best = cand[c];
             *objval = f;
@@ -643,17 +645,23 @@ static double line_selection(const prob_t* P, const double* cand, int nc, const 
     return best;
 }
 
-/* More-Thuente-style strong-Wolfe search, c1 = 1e-4, c2 = 0.9, <= 20 trials,
- * steps in [1e-10, 2].  Restates Problem::line_search_wolfe,
- * src/sinkhorn_problem_ls.cpp:226-470 (defaults at src/sinkhorn_problem.h:136-144). */
-static double line_search_wolfe(const prob_t* P, double init_step, const double* gamma,
-                                const double* direc, double curobj, const double* curgrad,
-                                int* recompute_T, double* x, double* grad)
+/* More-Thuente-style strong-Wolfe search, c1 = 1e-4, c2 = 0.9, at most
+ * max_iter trials (the reference's default is 20), steps in [1e-10, 2].
+ * Restates Problem::line_search_wolfe, src/sinkhorn_problem_ls.cpp:226-470
+ * (defaults at src/sinkhorn_problem.h:136-144) over a generic objective:
+ * fg(ctx, x, grad) returns f(x) and writes its gradient.  On return x and
+ * grad hold the last trial evaluated (*last_step; *ntrials trials), which is
+ * the returned step unless *recompute_T is set or the trials ran out. */
+double orc_line_search_wolfe(long N, orc_objgrad_fn fg, void* ctx, double init_step,
+                             const double* gamma, const double* direc, double curobj,
+                             const double* curgrad, int max_iter, int* recompute_T, double* x,
+                             double* grad, double* last_step, int* ntrials)
 {
     const double c1 = 1e-4, c2 = 0.9;
-    const int max_iter = 20;
-    const long N = P->n + P->m - 1;
+    if (max_iter <= 0) max_iter = 20;
     *recompute_T = 0;
+    if (ntrials) *ntrials = 0;
+    if (last_step) *last_step = 0.0;
     const double step_min = 1e-10, step_max = 2.0;
     if (init_step < step_min) init_step = step_min;
     if (init_step > step_max) init_step = step_max;
@@ -676,7 +684,9 @@ static double line_search_wolfe(const prob_t* P, double init_step, const double*
     const double delta_max = 1.1, delta_min = 7.0 / 12, shrink = 0.66;
     for (int iter = 0; iter < max_iter; iter++) {
         for (long i = 0; i < N; i++) x[i] = gamma[i] + step * direc[i];
-        fx = orc_dual_obj_grad(P->n, P->m, P->M, P->a, P->b, P->reg, x, grad, P->T, 1);
+        fx = fg(ctx, x, grad);
+        if (ntrials) *ntrials = iter + 1;
+        if (last_step) *last_step = step;
         dg = dot(grad, direc, N);
         const double psit = fx - fx_init - step * test_decr;
         const double dpsit = dg - test_decr;
@@ -737,6 +747,11 @@ static double line_search_wolfe(const prob_t* P, double init_step, const double*
         }
         step = new_step;
     }
+    /* Trials exhausted (:448-469): the step of the last update is kept when
+     * psi formed from the last trial's f at that step is no worse than
+     * psi(I_lo) -- T, f and the gradient are then the last trial's, as the
+     * reference leaves recompute_T false -- else I_lo, or the initial step
+     * when I_lo is not positive, both with recompute_T set. */
     const double psit = fx - fx_init - step * test_decr;
     if (psit <= psiI_lo) return step;
     if (I_lo <= 0.0) {
@@ -745,6 +760,74 @@ static double line_search_wolfe(const prob_t* P, double init_step, const double*
     }
     *recompute_T = 1;
     return I_lo;
+}
+
+/* The EROT dual as a line-search objective: T is recomputed into P->T. */
+static double erot_fg(void* ctx, const double* x, double* grad)
+{
+    const prob_t* P = (const prob_t*)ctx;
+    return orc_dual_obj_grad(P->n, P->m, P->M, P->a, P->b, P->reg, x, grad, P->T, 1);
+}
+
+static double erot_f(void* ctx, const double* x)
+{
+    const prob_t* P = (const prob_t*)ctx;
+    const double Tsum = orc_compute_T(P->n, P->m, P->M, P->reg, x, P->T);
+    return P->reg * Tsum - dot(x, P->a, P->n) - dot(x + P->n, P->b, P->m - 1);
+}
+
+/* line_search_wolfe on the EROT dual of (M, a, b, reg); see above. */
+double orc_erot_line_search_wolfe(int n, int m, const double* M, const double* a,
+                                  const double* b, double reg, double init_step,
+                                  const double* gamma, const double* direc, double curobj,
+                                  const double* curgrad, int max_iter, int* recompute_T,
+                                  double* x, double* grad, double* last_step, int* ntrials)
+{
+    prob_t P = {n, m, M, a, b, reg, (double*)malloc(sizeof(double) * (size_t)n * m)};
+    const double st = orc_line_search_wolfe(n + m - 1, erot_fg, &P, init_step, gamma, direc,
+                                            curobj, curgrad, max_iter, recompute_T, x, grad,
+                                            last_step, ntrials);
+    free(P.T);
+    return st;
+}
+
+/* ---- Driver trace (test infrastructure) --------------------------------- *
+ * When a buffer is set, the SSNS, sparse Newton and SPLR drivers append one
+ * record per iteration: ORC_TRACE_HDR scalars, then five vectors of length
+ * n + m - 1: gamma before the step, the point where the T (hence the
+ * gradient and the sparsified Hessian) in use was formed, the direction, and
+ * SPLR's y and s (zeros otherwise).  The scalars are listed in erot_oracle.h. */
+static double* g_trace = NULL;
+static long g_trace_cap = 0, g_trace_used = 0;
+
+void orc_trace_set(double* buf, long cap)
+{
+    g_trace = buf;
+    g_trace_cap = cap;
+    g_trace_used = 0;
+}
+
+long orc_trace_used(void) { return g_trace_used; }
+
+static void trace_put(const double* v, long k)
+{
+    if (!g_trace) return;
+    for (long i = 0; i < k; i++) {
+        if (g_trace_used < g_trace_cap) g_trace[g_trace_used] = v ? v[i] : 0.0;
+        g_trace_used++;
+    }
+}
+
+static void trace_record(const double* hdr, long N, const double* gamma, const double* tpt,
+                         const double* direc, const double* y, const double* s)
+{
+    if (!g_trace) return;
+    trace_put(hdr, ORC_TRACE_HDR);
+    trace_put(gamma, N);
+    trace_put(tpt, N);
+    trace_put(direc, N);
+    trace_put(y, N);
+    trace_put(s, N);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -945,7 +1028,9 @@ double orc_bcd_iterations(int n, int m, const double* M, const double* a, const 
 
 /* Safe and sparse Newton (SSNS).  Restates sinkhorn_ssns_internal,
  * src/sinkhorn_ssns.cpp:16-234, including the reuse of T from the last
- * evaluated line-selection candidate (:185-187). */
+ * evaluated line-selection candidate (:185-187): after an accepted step the
+ * gradient and the sparsified Hessian come from that candidate's T, and f
+ * from that T with the accepted gamma. */
 int orc_sinkhorn_ssns(int n, int m, const double* M, const double* a, const double* b,
                       double reg, double tol, int max_iter, int verbose, const orc_opts* opts,
                       double* gamma, double* obj_vals, double* mar_errs, double* run_times,
@@ -963,11 +1048,13 @@ int orc_sinkhorn_ssns(int n, int m, const double* M, const double* a, const doub
     double* g = (double*)malloc(sizeof(double) * N);
     double* rhs = (double*)malloc(sizeof(double) * N);
     double* work = (double*)malloc(sizeof(double) * N);
+    double* tpt = (double*)malloc(sizeof(double) * N);
     hess_t H;
     memset(&H, 0, sizeof(H));
     initial_gamma(&P, gamma);
     double t0 = now_ms();
     double f = orc_dual_obj_grad(n, m, M, a, b, reg, gamma, g, P.T, 1);
+    memcpy(tpt, gamma, sizeof(double) * N);
     double gnorm = norm2(g, N);
     double delta = nu0 * gnorm;
     hess_compute(&P, g, 1, delta, 0.001, &H);
@@ -984,21 +1071,30 @@ int orc_sinkhorn_ssns(int n, int m, const double* M, const double* a, const doub
         const double shift = mu * gnorm;
         for (long k = 0; k < N; k++) rhs[k] = -g[k];
         lin_solve(&P, &H, opts, rhs, shift, direc);
-        double newf;
-        double alpha = line_selection(&P, cand, 4, gamma, direc, f, &newf, work);
+        double newf, c_last = 0.0;
+        double alpha = orc_line_selection(N, erot_f, &P, cand, 4, gamma, direc, f, &newf, work,
+                                          &c_last);
         for (long k = 0; k < N; k++) step[k] = alpha * direc[k];
         orc_apply_hsx(n, m, H.h1, H.h2, H.colptr, H.rowind, H.val, step, Hstep);
         const double numer = f - newf;
         const double denom = -dot(g, step, N) - 0.5 * dot(step, Hstep, N);
         rho = numer / denom;
+        double hdr[ORC_TRACE_HDR] = {1, i, f, gnorm, shift, mu, delta, alpha, newf, rho, numer,
+                                     denom, c_last, hess_density(&P, &H), 0, 0};
+        if (g_trace) {
+            memcpy(work, gamma, sizeof(double) * N);
+        }
         if (rho > 0.0)
             for (long k = 0; k < N; k++) gamma[k] += step[k];
         if (rho < rho_t1)
             mu = mu_s1 * mu;
         else if (rho > rho_t2)
             mu = (mu_s2 * mu > kappa) ? mu_s2 * mu : kappa;
+        hdr[14] = mu;
+        trace_record(hdr, N, work, tpt, direc, NULL, NULL);
         if (rho > 0.0) {
             f = orc_dual_obj_grad(n, m, M, a, b, reg, gamma, g, P.T, 0);
+            for (long k = 0; k < N; k++) tpt[k] = gamma[k] + (c_last - alpha) * direc[k];
             gnorm = norm2(g, N);
             delta = nu0 * gnorm;
             hess_compute(&P, g, 1, delta, hess_density(&P, &H), &H);
@@ -1018,6 +1114,7 @@ int orc_sinkhorn_ssns(int n, int m, const double* M, const double* a, const doub
     free(g);
     free(rhs);
     free(work);
+    free(tpt);
     *hist_len = h;
     return i;
 }
@@ -1044,11 +1141,14 @@ static int newton_family(int splr, int n, int m, const double* M, const double* 
     double* rhs = (double*)malloc(sizeof(double) * N);
     double* x = (double*)malloc(sizeof(double) * N);
     double* xg = (double*)malloc(sizeof(double) * N);
+    double* tpt = (double*)malloc(sizeof(double) * N);
+    double* gamma_old = (double*)malloc(sizeof(double) * N);
     hess_t H;
     memset(&H, 0, sizeof(H));
     initial_gamma(&P, gamma);
     double t0 = now_ms();
     double f = orc_dual_obj_grad(n, m, M, a, b, reg, gamma, g, P.T, 1);
+    memcpy(tpt, gamma, sizeof(double) * N);
     double gnorm = norm2(g, N);
     const double gnorm_init = gnorm;
     hess_compute(&P, g, 0, 0.0, density, &H);
@@ -1078,13 +1178,23 @@ static int newton_family(int splr, int n, int m, const double* M, const double* 
             lin_solve_low_rank(&P, &H, opts, rhs, shift, y, s, direc);
         else
             lin_solve(&P, &H, opts, rhs, shift, direc);
-        int recompute_T = 1;
+        int recompute_T = 1, ntrials = 0;
+        double last_step = 0.0;
         double init = 1.5 * alpha < 1.0 ? 1.5 * alpha : 1.0;
-        alpha = line_search_wolfe(&P, init, gamma, direc, f, g, &recompute_T, x, xg);
+        alpha = orc_line_search_wolfe(N, erot_fg, &P, init, gamma, direc, f, g, opts->ls_max_iter,
+                                      &recompute_T, x, xg, &last_step, &ntrials);
+        double hdr[ORC_TRACE_HDR] = {splr ? 3 : 2, i, f, gnorm, shift, density, init, alpha,
+                                     recompute_T, low_rank, ntrials, last_step, 0, 0, 0, 0};
+        if (g_trace) memcpy(gamma_old, gamma, sizeof(double) * N);
         if (splr) memcpy(gamma_pre, gamma, sizeof(double) * N);
         for (long k = 0; k < N; k++) gamma[k] += alpha * direc[k];
         if (splr) memcpy(g_pre, g, sizeof(double) * N);
+        const double* tnew = recompute_T ? gamma : x;
+        if (g_trace) {
+            trace_record(hdr, N, gamma_old, tpt, direc, low_rank ? y : NULL, low_rank ? s : NULL);
+        }
         f = orc_dual_obj_grad(n, m, M, a, b, reg, gamma, g, P.T, recompute_T);
+        memcpy(tpt, tnew, sizeof(double) * N);
         const double gnorm_pre = gnorm;
         gnorm = norm2(g, N);
         if (splr) {
@@ -1115,6 +1225,8 @@ static int newton_family(int splr, int n, int m, const double* M, const double* 
     free(rhs);
     free(x);
     free(xg);
+    free(tpt);
+    free(gamma_old);
     *hist_len = h;
     return i;
 }

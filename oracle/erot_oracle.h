@@ -19,6 +19,11 @@
 extern "C" {
 #endif
 
+/* Objectives for the generic line searches and L-BFGS: f(x), and f(x) with
+ * its gradient written to grad. */
+typedef double (*orc_objgrad_fn)(void* ctx, const double* x, double* grad);
+typedef double (*orc_obj_fn)(void* ctx, const double* x);
+
 /* ---- Problem dense passes (src/sinkhorn_problem_bcd.cpp, _obj.cpp) ---- */
 void orc_optimal_beta(int n, int m, const double* M, const double* b, double reg,
                       const double* alpha, double* beta);
@@ -89,7 +94,36 @@ typedef struct {
     int cg_max_iter;      /* <= 0: Eigen's default 2 * HessianCG::cols() = 2m */
     int lowrank_cg;       /* SPLR: 1 = CG for the two low-rank solves instead of direct */
     int dense_T;          /* 1: keep the n x m T like the reference (needed by Newton drivers) */
+    int ls_max_iter;      /* Wolfe trials of sparse Newton / SPLR; <= 0: the reference's 20 */
 } orc_opts;
+
+/* ---- Line searches (src/sinkhorn_problem_ls.cpp) over generic objectives ---- */
+double orc_line_selection(long N, orc_obj_fn fobj, void* ctx, const double* cand, int nc,
+                          const double* gamma, const double* direc, double curobj,
+                          double* objval, double* work, double* last_cand);
+double orc_line_search_wolfe(long N, orc_objgrad_fn fg, void* ctx, double init_step,
+                             const double* gamma, const double* direc, double curobj,
+                             const double* curgrad, int max_iter, int* recompute_T, double* x,
+                             double* grad, double* last_step, int* ntrials);
+double orc_erot_line_search_wolfe(int n, int m, const double* M, const double* a,
+                                  const double* b, double reg, double init_step,
+                                  const double* gamma, const double* direc, double curobj,
+                                  const double* curgrad, int max_iter, int* recompute_T,
+                                  double* x, double* grad, double* last_step, int* ntrials);
+
+/* ---- Driver trace: one record per SSNS / sparse Newton / SPLR iteration ----
+ * ORC_TRACE_HDR scalars, then gamma (before the step), the point whose T is
+ * in use, the direction, y and s (SPLR's curvature pair, zeros otherwise),
+ * each n + m - 1 long.
+ * SSNS scalars:   kind 1, iter, f, ||g||, shift, mu, delta, step, f at the
+ *                 step, rho, its numerator, its denominator, last candidate
+ *                 evaluated, H density, mu after the update, 0.
+ * sparse Newton (kind 2) / SPLR (kind 3): kind, iter, f, ||g||, shift,
+ *                 density of H, initial step, step, recompute_T, low-rank
+ *                 flag, Wolfe trials, last trial step, 0, 0, 0, 0. */
+#define ORC_TRACE_HDR 16
+void orc_trace_set(double* buf, long cap);
+long orc_trace_used(void);
 
 int orc_sinkhorn_bcd(int n, int m, const double* M, const double* a, const double* b,
                      double reg, double tol, int max_iter, int verbose,
@@ -143,7 +177,6 @@ double orc_primal_val(int n, int m, const double* M, const double* a, const doub
                       double reg, const double* gamma);
 
 /* ---- Generic L-BFGS (LBFGSSolver::minimize with the Nocedal-Wright search) ---- */
-typedef double (*orc_objgrad_fn)(void* ctx, const double* x, double* grad);
 typedef void (*orc_iter_fn)(void* ctx, const double* x, double last_f, double gnorm);
 int orc_lbfgs_minimize(long N, orc_objgrad_fn fn, void* fctx, double eps_abs, double eps_rel,
                        int max_iter, orc_iter_fn it, void* ictx, double* x);

@@ -30,6 +30,7 @@ class _Opts(ctypes.Structure):
         ("cg_max_iter", ctypes.c_int),
         ("lowrank_cg", ctypes.c_int),
         ("dense_T", ctypes.c_int),
+        ("ls_max_iter", ctypes.c_int),
     ]
 
 
@@ -48,11 +49,13 @@ def lib():
             build()
         L = ctypes.CDLL(_SO)
         for name in ("orc_dual_obj", "orc_dual_obj_grad", "orc_compute_T", "orc_bcd_iterations",
+                     "orc_line_selection", "orc_line_search_wolfe", "orc_erot_line_search_wolfe",
                      "orc_cubic_minimizer", "orc_step_selection", "orc_primal_val",
                      "orc_qrot_dual_obj_grad", "orc_qrot_pl_root", "orc_qrot_primal_val"):
             getattr(L, name).restype = ctypes.c_double
-        for name in ("orc_sparsify_mat3", "orc_sparsify_density"):
+        for name in ("orc_sparsify_mat3", "orc_sparsify_density", "orc_trace_used"):
             getattr(L, name).restype = ctypes.c_longlong
+        L.orc_trace_set.argtypes = [ctypes.c_void_p, ctypes.c_long]
         _lib = L
     return _lib
 
@@ -257,9 +260,9 @@ def _run(fn, M, a, b, reg, tol, max_iter, verbose, opts, with_dens):
 
 
 def _opts(method=1, mu0=1.0, shift=1e-3, density=0.01, cg_tol=0.0, cg_max_iter=0,
-          lowrank_cg=False):
+          lowrank_cg=False, ls_max_iter=0):
     return _Opts(int(method), float(mu0), float(shift), float(density), float(cg_tol),
-                 int(cg_max_iter), int(lowrank_cg), 1)
+                 int(cg_max_iter), int(lowrank_cg), 1, int(ls_max_iter))
 
 
 def sinkhorn_bcd(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0):
@@ -281,6 +284,98 @@ def sinkhorn_sparse_newton(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **k
 def sinkhorn_splr(M, a, b, reg, tol=1e-6, max_iter=1000, verbose=0, **kw):
     """src/sinkhorn_splr.cpp:24-199"""
     return _run(lib().orc_sinkhorn_splr, M, a, b, reg, tol, max_iter, verbose, _opts(**kw), True)
+
+
+# --------------------------------------------------------------------------
+# Line searches over generic objectives, and the driver trace
+# --------------------------------------------------------------------------
+_OBJ_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, _dp)
+_OBJGRAD_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, _dp, _dp)
+
+
+def line_selection(fobj, gamma, direc, curobj, cand=(1.0, 0.5, 0.25, 0.1)):
+    """src/sinkhorn_problem_ls.cpp:13-41 on a Python objective f(x) ->
+    (step, f at the step, last candidate evaluated)."""
+    gamma, direc = _d(gamma), _d(direc)
+    N = gamma.shape[0]
+    cb = _OBJ_FN(lambda ctx, x: float(fobj(np.ctypeslib.as_array(x, (N,)).copy())))
+    c = _d(cand)
+    fv = ctypes.c_double()
+    last = ctypes.c_double()
+    work = np.empty(N)
+    st = lib().orc_line_selection(ctypes.c_long(N), cb, None, _p(c), len(c), _p(gamma),
+                                  _p(direc), ctypes.c_double(curobj), ctypes.byref(fv),
+                                  _p(work), ctypes.byref(last))
+    return st, fv.value, last.value
+
+
+def _wolfe_out(st, rec, last, nt, x, g):
+    return dict(step=st, recompute_T=bool(rec.value), last_step=last.value,
+                trials=nt.value, x=x, grad=g)
+
+
+def line_search_wolfe(fg, gamma, direc, curobj, curgrad, init_step=1.0, max_iter=20):
+    """src/sinkhorn_problem_ls.cpp:226-470 on a Python objective
+    fg(x) -> (f, grad).  Returns a dict: step, recompute_T, last_step,
+    trials, and x / grad of the last trial."""
+    gamma, direc, curgrad = _d(gamma), _d(direc), _d(curgrad)
+    N = gamma.shape[0]
+
+    def call(ctx, x, g):
+        f, gr = fg(np.ctypeslib.as_array(x, (N,)).copy())
+        np.ctypeslib.as_array(g, (N,))[:] = gr
+        return float(f)
+
+    cb = _OBJGRAD_FN(call)
+    rec, nt, last = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    x, g = np.zeros(N), np.zeros(N)
+    st = lib().orc_line_search_wolfe(ctypes.c_long(N), cb, None, ctypes.c_double(init_step),
+                                      _p(gamma), _p(direc), ctypes.c_double(curobj), _p(curgrad),
+                                      int(max_iter), ctypes.byref(rec), _p(x), _p(g),
+                                      ctypes.byref(last), ctypes.byref(nt))
+    return _wolfe_out(st, rec, last, nt, x, g)
+
+
+def erot_line_search_wolfe(M, a, b, reg, gamma, direc, init_step=1.0, max_iter=20):
+    """line_search_wolfe on the EROT dual with the oracle's own f and gradient."""
+    M, a, b, gamma, direc = _d(M), _d(a), _d(b), _d(gamma), _d(direc)
+    n, m = M.shape
+    f, g0 = dual_obj_grad(M, a, b, reg, gamma)
+    rec, nt, last = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    x, g = np.zeros(n + m - 1), np.zeros(n + m - 1)
+    st = lib().orc_erot_line_search_wolfe(
+        n, m, _p(M), _p(a), _p(b), ctypes.c_double(reg), ctypes.c_double(init_step), _p(gamma),
+        _p(direc), ctypes.c_double(f), _p(g0), int(max_iter), ctypes.byref(rec), _p(x), _p(g),
+        ctypes.byref(last), ctypes.byref(nt))
+    return _wolfe_out(st, rec, last, nt, x, g)
+
+
+TRACE_HDR = 16
+
+
+def traced(solver, M, a, b, reg, tol=1e-6, max_iter=1000, **kw):
+    """Run sinkhorn_ssns / sinkhorn_sparse_newton / sinkhorn_splr with the
+    driver trace on -> (result dict, list of per-iteration records).  Each
+    record: ``hdr`` (the scalars listed in erot_oracle.h) and the vectors
+    ``gamma``, ``tpoint``, ``direc``, ``y``, ``s``."""
+    n, m = np.shape(M)
+    N = n + m - 1
+    rec_len = TRACE_HDR + 5 * N
+    buf = np.zeros(rec_len * (max_iter + 1))
+    lib().orc_trace_set(buf.ctypes.data, buf.shape[0])
+    try:
+        res = globals()[solver](M, a, b, reg, tol=tol, max_iter=max_iter, **kw)
+        used = lib().orc_trace_used()
+    finally:
+        lib().orc_trace_set(None, 0)
+    assert used <= buf.shape[0] and used % rec_len == 0
+    recs = []
+    for k in range(used // rec_len):
+        r = buf[k * rec_len:(k + 1) * rec_len]
+        v = r[TRACE_HDR:].reshape(5, N)
+        recs.append(dict(hdr=r[:TRACE_HDR].copy(), gamma=v[0].copy(), tpoint=v[1].copy(),
+                         direc=v[2].copy(), y=v[3].copy(), s=v[4].copy()))
+    return res, recs
 
 
 def bcd_iterations(M, a, b, reg, iters):

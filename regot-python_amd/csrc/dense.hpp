@@ -91,10 +91,19 @@ struct BcdState {
     double reg;
     int max_iter;
     int n, m;
+    // Stall test (stall_window > 0): the loop also stops, with ctl[0] = 2,
+    // when ||g|| has not reached a new minimum for stall_window iterations
+    // while still >= tol -- the FP32-bulk sums have hit their resolution.
+    // stall holds two slots {min ||g|| so far, its iteration} by the parity
+    // of k: iteration k reads slot k & 1, workgroup 0 writes the other.
+    double* stall;
+    int stall_window;
 };
-// Row pass publishing the per-workgroup BCD sums (FIN variant).
+// Row pass publishing the per-workgroup BCD sums (FIN variant); precise (FP32
+// bulk): FP64 exp2 and sums.
 void launch_row_pass_bcd(int bulk, const void* M, long ldm, int n, int m, const double* beta,
-                         double k, const RowEpi& epi, const BcdState& st, hipStream_t s);
+                         double k, const RowEpi& epi, const BcdState& st, hipStream_t s,
+                         bool precise = false);
 int row_pass_blocks(int bulk, int n);
 
 // Single-read BCD iteration (FP32 bulk; bcd_fused.hip): per workgroup of whole

@@ -109,7 +109,7 @@ constexpr int kCombineCols = 64;   // columns per combine workgroup (4 threads e
 // f_k = reg sum r - (alpha_k + beta_{k,m-1})'a - (beta_k - beta_{k,m-1})'b over
 // j < m-1.  The test order is that of src/sinkhorn_bcd.cpp:65-79.
 __device__ void bcd_decide(const BcdState& st, int kk, double* f_out, double* g_out,
-                           bool* done_out)
+                           bool* done_out, bool* stall_out)
 {
     const double* cp = st.cpart + 2L * st.ntiles * (kk & 1);
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -130,6 +130,11 @@ __device__ void bcd_decide(const BcdState& st, int kk, double* f_out, double* g_
     *f_out = st.reg * v[1] - da - db;
     *g_out = gnorm;
     *done_out = (gnorm < st.tol) || (kk >= st.max_iter);
+    *stall_out = false;
+    if (!*done_out && st.stall_window > 0) {
+        const double* sv = st.stall + 2 * (kk & 1);
+        *stall_out = !(gnorm < sv[0]) && (kk - static_cast<int>(sv[1]) >= st.stall_window);
+    }
 }
 
 // Combine the nblk partials of each column (4 threads per column, fixed
@@ -148,17 +153,26 @@ __global__ __launch_bounds__(256) void k_col_combine(const double* __restrict__ 
 {
     int kk = 0;
     double f = 0.0, gnorm = 0.0;
-    bool done = false;
+    bool done = false, stall = false;
     if (BCD) {
         if (st.ctl[0] != 0) return;
         kk = st.k;
-        bcd_decide(st, kk, &f, &gnorm, &done);
+        bcd_decide(st, kk, &f, &gnorm, &done, &stall);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             const int h = kk % st.hist_cap;
             st.hist_f[h] = f;
             st.hist_g[h] = gnorm;
             st.hist_ts[h] = static_cast<long long>(wall_clock64());
+            if (st.stall_window > 0) {
+                // running minimum for the next iteration (other parity slot)
+                const double* sv = st.stall + 2 * (kk & 1);
+                double* nx = st.stall + 2 * ((kk + 1) & 1);
+                const bool better = gnorm < sv[0];
+                nx[0] = better ? gnorm : sv[0];
+                nx[1] = better ? static_cast<double>(kk) : sv[1];
+            }
         }
+        done = done || stall;
     } else if (ctl != nullptr && ctl[0] != 0) {
         return;
     }
@@ -226,7 +240,7 @@ __global__ __launch_bounds__(256) void k_col_combine(const double* __restrict__ 
     if (BCD && blockIdx.x == 0 && threadIdx.x == 0) {
         if (done) {
             st.ctl[2] = kk;
-            st.ctl[0] = 1;
+            st.ctl[0] = stall ? 2 : 1;
         } else {
             st.ctl[1] = kk + 1;
         }
@@ -341,10 +355,13 @@ void launch_row_pass(int bulk, const void* M, long ldm, int n, int m, const doub
 }
 
 void launch_row_pass_bcd(int bulk, const void* M, long ldm, int n, int m, const double* beta,
-                         double k, const RowEpi& epi, const BcdState& st, hipStream_t s)
+                         double k, const RowEpi& epi, const BcdState& st, hipStream_t s,
+                         bool precise)
 {
-    if (bulk == 0)
+    if (bulk == 0 && !precise)
         row_pass_launch<float, float, true>(M, ldm, n, m, beta, k, epi, st, st.ctl, s, bulk);
+    else if (bulk == 0)
+        row_pass_launch<float, double, true>(M, ldm, n, m, beta, k, epi, st, st.ctl, s, bulk);
     else
         row_pass_launch<double, double, true>(M, ldm, n, m, beta, k, epi, st, st.ctl, s, bulk);
 }

@@ -97,6 +97,19 @@ struct Work {
         for (int q = 0; q < k; q++) out[q] = hscal.p[q];
     }
 
+    // [a; b_0..m-2] . v, for objectives formed from one point's T with
+    // another point's potentials (f = reg sum T - gamma'[a; b~])
+    double dot_ab(const double* v)
+    {
+        const double* xs[2] = {v, v + n};
+        const double* ys[2] = {p->a.p, p->b.p};
+        vec_dots(xs, ys, n, 1, dscal, s);
+        vec_dots(xs + 1, ys + 1, m - 1, 1, dscal.p + 1, s);
+        HIP_CHECK(hipMemcpyAsync(hscal.p, dscal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        return hscal.p[0] + hscal.p[1];
+    }
+
     // out = x + t d
     void move(double* out, const double* x0, double t, const double* d)
     {
@@ -125,19 +138,21 @@ struct Work {
         return f;
     }
 
-    void threshold(double delta, double hint)
+    // The sparsified Hessian of the T formed at `tpt` (gamma unless given),
+    // with the diagonals from the gradient g, which is that T's.
+    void threshold(double delta, double hint, const double* tpt = nullptr)
     {
         const auto t0 = Clock::now();
-        hess_threshold(p, gamma, g, delta, hint);
+        hess_threshold(p, tpt ? tpt : gamma.p, g, delta, hint);
         alg_bytes += pass_bytes() + 2.0 * hess_of(p).nnz * 12.0;
         phase_ms[1] += ms_since(t0);
         phase_n[1] += 1;
     }
 
-    void topk(double density)
+    void topk(double density, const double* tpt = nullptr)
     {
         const auto t0 = Clock::now();
-        hess_topk(p, gamma, g, density);
+        hess_topk(p, tpt ? tpt : gamma.p, g, density);
         alg_bytes += pass_bytes() + 2.0 * hess_of(p).nnz * 12.0;
         phase_ms[1] += ms_since(t0);
         phase_n[1] += 1;
@@ -208,6 +223,15 @@ struct Timer {
     Clock::time_point t0;
     void tic() { t0 = Clock::now(); }
     double toc() { return ms_since(t0); }
+    // ms since the last tic or lap, then restart (the reference's
+    // Timer::toc(name), src/timer.h:38-45)
+    double lap()
+    {
+        const auto t = Clock::now();
+        const double d = std::chrono::duration<double, std::milli>(t - t0).count();
+        t0 = t;
+        return d;
+    }
 };
 
 // L-BFGS (first_order.cpp) on w's objective from w.gamma; see there.

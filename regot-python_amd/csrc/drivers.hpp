@@ -18,6 +18,9 @@ struct regot_result {
     long long phase_n[3] = {0, 0, 0};
     long long cg_iters = 0;
     double alg_bytes = 0.0;   // algorithmic HBM bytes of the solve (Newton-type drivers)
+    // BCD in FP32 bulk: the iterate from which the loop ran in FP64 arithmetic
+    // (0: from the start, -1: never)
+    int precise_from = -1;
     // APDAGD's primal plan (device, row pitch plan_ld); empty for the others,
     // whose plan is formed from the dual on request
     regot::DBuf<double> plan;

@@ -90,18 +90,19 @@ double step_select(double al, double au, double at, double fl, double fu, double
     return (at > al) ? std::min(lim, ae) : std::max(lim, ae);
 }
 
-// Strong-Wolfe search (c1 = 1e-4, c2 = 0.9, <= 20 trials, steps in
-// [1e-10, 2]), the More-Thuente variant of Problem::line_search_wolfe,
+// Strong-Wolfe search (c1 = 1e-4, c2 = 0.9, at most max_iter trials -- the
+// reference's 20 when max_iter <= 0 -- steps in [1e-10, 2]), the
+// More-Thuente variant of Problem::line_search_wolfe,
 // src/sinkhorn_problem_ls.cpp:226-470.  Each trial is one dual_obj_grad pass
-// at gamma + step d into w.x / w.xg.  Returns the step; *reuse_last is true
-// when the returned step is the last trial evaluated, so its f and gradient
-// (left in *f_last and w.xg) are those of the new point (the reference's
-// recompute_T == false case).
-double wolfe(Work& w, double init_step, double f0, double dg0, bool* reuse_last, double* f_last,
-             double* gn_last)
+// at gamma + step d into w.x / w.xg.  Returns the step; *reuse_last is the
+// reference's recompute_T == false: the caller keeps the last trial's T (its
+// f in *f_last, gradient in w.xg, norm in *gn_last, taken at *step_last),
+// which is the returned step's except when the trials ran out.
+double wolfe(Work& w, double init_step, double f0, double dg0, int max_iter, bool* reuse_last,
+             double* f_last, double* gn_last, double* step_last)
 {
     const double c1 = 1e-4, c2 = 0.9;
-    const int max_iter = 20;
+    if (max_iter <= 0) max_iter = 20;
     const double step_min = 1e-10, step_max = 2.0;
     *reuse_last = false;
     init_step = std::min(std::max(init_step, step_min), step_max);
@@ -116,12 +117,11 @@ double wolfe(Work& w, double init_step, double f0, double dg0, bool* reuse_last,
     double I_width = inf, I_width_prev = inf;
     int shrink_fail = 0;
     const double delta_max = 1.1, delta_min = 7.0 / 12, shrink = 0.66;
-    double last_step = -1.0;
     for (int it = 0; it < max_iter; it++) {
         w.move(w.x, w.gamma, step, w.direc);
         fx = w.obj_grad(w.x, w.xg, gn_last);
         const double dg = w.dot(w.xg, w.direc);
-        last_step = step;
+        *step_last = step;
         *f_last = fx;
         const double psit = fx - f0 - step * test_decr;
         const double dpsit = dg - test_decr;
@@ -185,12 +185,11 @@ double wolfe(Work& w, double init_step, double f0, double dg0, bool* reuse_last,
         }
         step = new_step;
     }
-    // Trials exhausted (:683-705): the step the last update produced if
+    // Trials exhausted (:448-469): the step the last update produced if
     // psi(last evaluated f, that step) <= psi(I_lo), else I_lo (or the initial
     // step when I_lo is not positive).  As in the reference, the first answer
-    // keeps the last trial's f and gradient (recompute_T stays false there)
-    // even though the returned step was not itself evaluated.
-    (void)last_step;
+    // keeps the last trial's T (recompute_T stays false there) even though the
+    // returned step was not itself evaluated.
     const double psit = fx - f0 - step * test_decr;
     if (psit <= psiI_lo) {
         *reuse_last = true;
@@ -207,9 +206,12 @@ double wolfe(Work& w, double init_step, double f0, double dg0, bool* reuse_last,
 // trust-region-like mu control (rho thresholds 0.25 / 0.75, mu x4 / x0.5,
 // floor 1e-3), shift = mu ||g||, step candidates {1, 0.5, 0.25, 0.1},
 // delta = 0.01 ||g||, threshold sparsification with the previous density as
-// the hint.  The gradient after an accepted step is evaluated at the new
-// point (the reference reuses T from the last evaluated candidate, which is
-// the same point whenever a candidate lowered f).
+// the hint.  After an accepted step the gradient and the sparsified Hessian
+// come from the T of the last candidate evaluated, and f from that T with the
+// accepted potentials, as the reference's dual_obj_grad(gamma, g, T, false)
+// after line_selection (src/sinkhorn_ssns.cpp:185-187).  That candidate is
+// the accepted one whenever a candidate lowered f; otherwise the last is 0.1,
+// which a convex dual makes the best of the four up to exact ties.
 void solve_ssns(regot_problem* p, double tol, int max_iter, int verbose, const regot_opts& o,
                 regot_result* res)
 {
@@ -239,37 +241,82 @@ void solve_ssns(regot_problem* p, double tol, int max_iter, int verbose, const r
         timer.tic();
         if (gnorm < tol) break;
         const double shift = mu * gnorm;
+        if (verbose >= 2) {
+            printf("[params]===================================================\n"
+                   "║ rho = %g, mu = %g, shift = %g\n"
+                   "║----------------------------------------------------------\n",
+                   rho, mu, shift);
+        }
+        Timer inner;
+        inner.tic();
+        double t_lin = 0.0, t_ls = 0.0, t_rho = 0.0, t_grad = 0.0, t_hess = 0.0;
         vec_axpby(w.rhs, -1.0, w.g, 0.0, nullptr, w.N, w.s);
         w.lin_solve(o, w.rhs, shift, w.direc);
+        t_lin = inner.lap();
         // line selection, src/sinkhorn_problem_ls.cpp:13-41
-        double best = 1.0, newf = std::numeric_limits<double>::infinity();
+        double best = 1.0, newf = std::numeric_limits<double>::infinity(), c_last = 1.0;
         for (double a : cand) {
             w.move(w.x, w.gamma, a, w.direc);
             const double fa = w.obj(w.x);
+            c_last = a;
             if (fa < newf) {
                 best = a;
                 newf = fa;
             }
             if (newf < f) break;
         }
+        t_ls = inner.lap();
         vec_axpby(w.step, best, w.direc, 0.0, nullptr, w.N, w.s);
         hess_apply_hs(p, w.step, w.Hstep);
-        const double* xs[2] = {w.g.p, w.step.p};
-        const double* ys[2] = {w.step.p, w.Hstep.p};
-        double d[2];
-        w.dots(2, xs, ys, d);
+        const double* xs[3] = {w.g.p, w.step.p, w.step.p};
+        const double* ys[3] = {w.step.p, w.Hstep.p, w.step.p};
+        double d[3];
+        w.dots(verbose >= 2 ? 3 : 2, xs, ys, d);
+        t_rho = inner.lap();
         const double numer = f - newf;
         const double denom = -d[0] - 0.5 * d[1];
         rho = numer / denom;
+        if (verbose >= 2) {
+            printf("[update]---------------------------------------------------\n"
+                   "║ alpha = %g, ||step|| = %g\n"
+                   "║ rho_numer = %g, rho_denom = %g\n",
+                   best, std::sqrt(d[2]), numer, denom);
+        }
         if (rho > 0.0) vec_axpy(w.gamma, 1.0, w.step, w.N, w.s);
         if (rho < rho_t1)
             mu = mu_s1 * mu;
         else if (rho > rho_t2)
             mu = std::max(mu_s2 * mu, kappa);
         if (rho > 0.0) {
-            f = w.obj_grad(w.gamma, w.g, &gnorm);
-            delta = nu0 * gnorm;
-            w.threshold(delta, hess_of(p).density());
+            const double hint = hess_of(p).density();
+            if (c_last == best) {
+                f = w.obj_grad(w.gamma, w.g, &gnorm);
+                t_grad = inner.lap();
+                delta = nu0 * gnorm;
+                w.threshold(delta, hint);
+            } else {
+                // w.x still holds gamma_old + c_last d, the last candidate
+                const double fx = w.obj_grad(w.x, w.g, &gnorm);
+                f = fx - (best - c_last) * w.dot_ab(w.direc);
+                t_grad = inner.lap();
+                delta = nu0 * gnorm;
+                w.threshold(delta, hint, w.x);
+            }
+            t_hess = inner.lap();
+            if (verbose >= 2) {
+                printf("║----------------------------------------------------------\n"
+                       "[sparse]---------------------------------------------------\n"
+                       "║ delta = %g, den = %g, den_new = %g\n"
+                       "║----------------------------------------------------------\n"
+                       "[timing]---------------------------------------------------\n"
+                       "║ lin_solve = %g, grad = %g, hess = %g\n"
+                       "║ line_search = %g, rho = %g\n",
+                       delta, hint, hess_of(p).density(), t_lin, t_grad, t_hess, t_ls, t_rho);
+            }
+        }
+        if (verbose >= 2) {
+            printf("===========================================================\n\n");
+            fflush(stdout);
         }
         res->obj_vals.push_back(f);
         res->mar_errs.push_back(gnorm);
@@ -326,9 +373,12 @@ void solve_newton_family(regot_problem* p, bool splr, double tol, int max_iter, 
         }
         timer.tic();
         if (gnorm < tol || !std::isfinite(f)) break;
+        Timer inner;
+        inner.tic();
         const double shift = std::min(gnorm, shift_max);
         vec_axpby(w.rhs, -1.0, w.g, 0.0, nullptr, w.N, w.s);
         bool low_rank = false;
+        double ys_ = 0.0, yy_ = 0.0;
         if (splr) {
             vec_axpby(w.y, 1.0, w.g, -1.0, w.g_pre, w.N, w.s);
             vec_axpby(w.sv, 1.0, w.gamma, -1.0, w.gamma_pre, w.N, w.s);
@@ -336,27 +386,40 @@ void solve_newton_family(regot_problem* p, bool splr, double tol, int max_iter, 
             const double* ys[2] = {w.sv.p, w.y.p};
             double d[2];
             w.dots(2, xs, ys, d);
+            ys_ = d[0];
+            yy_ = d[1];
             low_rank = (i > 0) && (d[0] > eps * d[1]);
         }
         if (low_rank)
             w.lin_solve_low_rank(o, w.rhs, shift, w.y, w.sv, w.direc);
         else
             w.lin_solve(o, w.rhs, shift, w.direc);
+        const double t_lin = inner.lap();
         const double dg0 = w.dot(w.g, w.direc);
         bool reuse = false;
-        double f_last = 0.0, gn_last = 0.0;
-        alpha = wolfe(w, std::min(1.0, 1.5 * alpha), f, dg0, &reuse, &f_last, &gn_last);
+        double f_last = 0.0, gn_last = 0.0, step_last = 0.0;
+        alpha = wolfe(w, std::min(1.0, 1.5 * alpha), f, dg0, o.ls_max_iter, &reuse, &f_last,
+                      &gn_last, &step_last);
         if (splr) vec_copy(w.gamma_pre, w.gamma, w.N, w.s);
         vec_axpy(w.gamma, alpha, w.direc, w.N, w.s);
         if (splr) vec_copy(w.g_pre, w.g, w.N, w.s);
+        const double t_ls = inner.lap();
         const double gnorm_pre = gnorm;
+        const double* tpt = nullptr;      // T of the new gamma
         if (reuse) {
+            // the last trial's T (src/sinkhorn_sparse_newton.cpp:124-127):
+            // gradient and Hessian from it, f from it with the new potentials
             vec_copy(w.g, w.xg, w.N, w.s);
             f = f_last;
             gnorm = gn_last;
+            if (step_last != alpha) {
+                f = f_last - (alpha - step_last) * w.dot_ab(w.direc);
+                tpt = w.x;                 // gamma_old + step_last d
+            }
         } else {
             f = w.obj_grad(w.gamma, w.g, &gnorm);
         }
+        const double t_grad = inner.lap();
         if (splr) {
             const bool bad = (gnorm_pre < gnorm_init) && (gnorm > gnorm_pre);
             density *= bad ? 1.1 : 0.99;
@@ -365,7 +428,23 @@ void solve_newton_family(regot_problem* p, bool splr, double tol, int max_iter, 
             density *= bad ? 2.0 : 0.9;
         }
         density = std::min(density_max, std::max(density_min, density));
-        w.topk(density);
+        w.topk(density, tpt);
+        const double t_hess = inner.lap();
+        if (verbose >= 2) {
+            // src/sinkhorn_splr.cpp:167-180, src/sinkhorn_sparse_newton.cpp:142-150
+            if (splr)
+                printf("[lowrank]---------------------------------------------------\n"
+                       "║ ys = %g, yy = %g\n"
+                       "║ low_rank = %d\n"
+                       "===========================================================\n",
+                       ys_, yy_, low_rank ? 1 : 0);
+            printf("[timing]---------------------------------------------------\n"
+                   "║ lin_solve = %g, line_search = %g\n"
+                   "║ grad = %g, hess = %g\n"
+                   "===========================================================\n\n",
+                   t_lin, t_ls, t_grad, t_hess);
+            fflush(stdout);
+        }
         res->obj_vals.push_back(f);
         res->mar_errs.push_back(gnorm);
         res->run_times.push_back(res->run_times.back() + timer.toc());

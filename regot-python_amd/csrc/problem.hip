@@ -157,6 +157,10 @@ void regot_problem::setup(int n_, int m_, const void* M, int M_dtype, int bulk_,
     min_b = b_h[0];
     for (int i = 1; i < n; i++) min_a = std::min(min_a, a_h[i]);
     for (int j = 1; j < m; j++) min_b = std::min(min_b, b_h[j]);
+    sumsq_a = 0.0;
+    sumsq_b = 0.0;
+    for (int i = 0; i < n; i++) sumsq_a += a_h[i] * a_h[i];
+    for (int j = 0; j < m; j++) sumsq_b += b_h[j] * b_h[j];
     HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
     HIP_CHECK(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device));
 
@@ -265,7 +269,7 @@ void regot_problem::col_pass(const double* alpha_v, const double* colpot_guess,
     e.logb = logb;
     e.b = b;
     launch_col_pass(bulk, dM, ldm, n, m, alpha_v, colpot_guess, logb, k, part, nblk, e, st,
-                    ctl_p, stream, precise && st == nullptr);
+                    ctl_p, stream, st == nullptr ? precise : bcd_precise);
     prof_end(1, stop);
 }
 
@@ -275,7 +279,7 @@ void regot_problem::row_pass_bcd(const double* beta_full, const RowEpi& epi, con
     prof_begin(0, &stop);
     RowEpi e = epi;
     e.reg = reg;
-    launch_row_pass_bcd(bulk, dM, ldm, n, m, beta_full, k, e, st, stream);
+    launch_row_pass_bcd(bulk, dM, ldm, n, m, beta_full, k, e, st, stream, bcd_precise);
     prof_end(0, stop);
 }
 

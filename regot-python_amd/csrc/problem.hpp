@@ -79,7 +79,7 @@ void launch_exp2_check(const double* x, double* y, long count, hipStream_t s);
 
 // Buffers of the on-device BCD loop, kept across solves on one handle.
 struct BcdWork {
-    DBuf<double> ap0, ap1, beta, gcol, rpart, cpart, bm;
+    DBuf<double> ap0, ap1, beta, gcol, rpart, cpart, bm, stall;
     DBuf<double> fpart;         // column sums of the single-read pass (blocks x m)
     DBuf<int> ctl;
     MappedRing<double> hf, hg;
@@ -128,6 +128,7 @@ struct regot_problem {
     double sum_b1 = 0.0;        // sum_{j < m-1} b_j
     double sum_b = 0.0;         // sum_j b_j
     double min_a = 0.0, min_b = 0.0;
+    double sumsq_a = 0.0, sumsq_b = 0.0;   // sum a_i^2, sum b_j^2
     int ncu = 0;                // compute units of the device
     int wall_khz = 0;           // device wall clock rate (timestamps)
     regot::DBuf<double> a, b, loga, logb;
@@ -149,6 +150,9 @@ struct regot_problem {
     // FP32 bulk: run the general row / column passes with FP64 exp and sums
     // (objective and gradient evaluations of the Newton drivers, verification).
     bool precise = false;
+    // FP32 bulk: the BCD loop's row and column passes with FP64 exp and sums
+    // (the precise finish of solve_bcd).
+    bool bcd_precise = false;
 
     // profiling of the hot passes: [0] row pass, [1] column pass (partial +
     // combine), [2] single-read BCD pass, [3] its column-sum combine

@@ -17,6 +17,8 @@ Extensions (all optional):
   runs everything in FP64.
 * ``x0``: warm start of length n+m-1 (the reference's ``SinkhornSolverOpts::x0``).
 * ``cg_tol``, ``cg_max_iter``: CG controls of the Newton-type solvers.
+* ``ls_max_iter``: strong-Wolfe trials of sparse Newton and SPLR (default 20,
+  the reference's fixed value).
 * ``device``: GPU ordinal; ``compute_plan``: form ``plan`` (default True).
 
 Every computation runs in libregot_hip.so on the GPU; there is no CPU path.
@@ -475,6 +477,8 @@ class Problem:
             opts.bcd_batch = int(kwargs["bcd_batch"])
         if "bcd_fused" in kwargs:
             opts.bcd_fused = int(kwargs["bcd_fused"])
+        if "ls_max_iter" in kwargs:
+            opts.ls_max_iter = int(kwargs["ls_max_iter"])
         x0 = kwargs.get("x0")
         if x0 is not None:
             x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)
@@ -505,7 +509,8 @@ class Problem:
             res.stats = {"objgrad_ms": st[0], "objgrad_calls": int(st[1]),
                          "sparsify_ms": st[2], "sparsify_calls": int(st[3]),
                          "cg_ms": st[4], "cg_solves": int(st[5]), "cg_iters": int(st[6]),
-                         "alg_bytes": float(st[7])}
+                         "alg_bytes": float(st[7]),
+                         "precise_from": int(L.regot_result_precise_from(r))}
             if compute_plan:
                 plan = np.empty((self.n, self.m))
                 check(L.regot_result_plan(self._h, r, dptr(plan)))

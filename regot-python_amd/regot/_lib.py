@@ -50,6 +50,7 @@ class Opts(ctypes.Structure):
         ("bcd_batch", ctypes.c_int),
         ("bcd_fused", ctypes.c_int),
         ("tau", ctypes.c_double),
+        ("ls_max_iter", ctypes.c_int),
     ]
 
 
@@ -121,6 +122,7 @@ def lib():
     L.regot_result_plan.argtypes = [_vp, _vp, _dp]
     L.regot_result_verify.argtypes = [_vp, _vp, _dp, _dp]
     L.regot_result_stats.argtypes = [_vp, _dp]
+    L.regot_result_precise_from.argtypes = [_vp]
     L.regot_result_free.argtypes = [_vp]
     L.regot_result_free.restype = None
     _llp = ctypes.POINTER(ctypes.c_longlong)

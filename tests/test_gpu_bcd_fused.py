@@ -20,6 +20,13 @@ def rel_err(x, ref):
     return np.max(np.abs(np.asarray(x) - ref)) / max(1.0, np.max(np.abs(ref)))
 
 
+def fused_expected(r):
+    """Single-read passes a solve runs: one per iteration up to the iterate
+    where an FP32 solve hands over to its FP64 finish, that one included."""
+    pf = r.stats["precise_from"]
+    return (pf if pf >= 0 else r.niter) + 1
+
+
 def solve(gpu_lib, M, a, b, reg, tol, max_iter, fused, x0=None):
     """BCD through a problem handle with kernel profiling on; returns the
     result and the number of single-read passes that ran."""
@@ -45,7 +52,7 @@ def test_fused_bcd_matches_oracle(gpu_lib, n, m, reg):
     print("fused bcd %dx%d reg=%g" % (n, m, reg), flush=True)
     r, nf = solve(gpu_lib, M32, a, b, reg, 1e-8, 5000, fused=1)
     o = O.sinkhorn_bcd(M32.astype(np.float64), a, b, reg, tol=1e-8, max_iter=5000)
-    assert nf == r.niter + 1          # every launched iteration used the single read
+    assert nf == fused_expected(r)    # every FP32 iteration used the single read
     assert r.niter < 5000 and abs(r.niter - o["niter"]) <= max(2, 0.02 * o["niter"])
     assert len(r.obj_vals) == r.niter + 1
     assert r.mar_errs[-1] < 1e-8 and all(e >= 1e-8 for e in r.mar_errs[:-1])
@@ -162,3 +169,29 @@ def test_fused_edge_shapes(gpu_lib):
         assert r.niter == o["niter"] == 0 and nf == 1 and len(r.obj_vals) == 1
         np.testing.assert_allclose(r.obj_vals, o["obj_vals"], rtol=1e-9)
         assert rel_err(r.dual, o["dual"]) < 1e-6
+
+
+@pytest.mark.parametrize("n,m,reg,tol,uniform", [(1, 3000, 0.05, 1e-9, False),
+                                                 (3, 16384, 0.05, 1e-9, False),
+                                                 (50, 2500, 0.05, 1e-10, False),
+                                                 (1200, 1100, 0.01, 1e-10, True)])
+def test_fp32_bulk_finishes_below_its_resolution(gpu_lib, n, m, reg, tol, uniform):
+    # FP32 sums resolve each marginal to ~2^-23 of its size; below that the
+    # FP32 loop hands over to FP64 passes over the same FP32 M and goes on to
+    # tol, so the iteration count and the dual match the FP64 oracle's.
+    M, a, b = cloud_problem(n, m, d=4 if uniform else 16, seed=n + 3)
+    if uniform:
+        a, b = np.full(n, 1.0 / n), np.full(m, 1.0 / m)
+    M32 = M.astype(np.float32)
+    print("fp32 finish %dx%d tol %g" % (n, m, tol), flush=True)
+    r, nf = solve(gpu_lib, M32, a, b, reg, tol, 5000, fused=1)
+    o = O.sinkhorn_bcd(M32.astype(np.float64), a, b, reg, tol=tol, max_iter=5000)
+    pf = r.stats["precise_from"]
+    assert 0 <= pf <= r.niter and nf == fused_expected(r)
+    assert abs(r.niter - o["niter"]) <= 1
+    assert r.mar_errs[-1] < tol and r.mar_err_fp64 < 1.05 * tol
+    assert len(r.obj_vals) == len(r.mar_errs) == len(r.run_times) == r.niter + 1
+    assert all(e >= tol for e in r.mar_errs[:-1])
+    k = min(len(r.obj_vals), len(o["obj_vals"]))
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-9)
+    assert rel_err(r.dual, o["dual"]) < 1e-8

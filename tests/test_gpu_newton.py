@@ -95,3 +95,61 @@ def test_dropin_contract(gpu_lib, capfd):
     assert r.niter == 4 and len(r.mar_errs) == 5
     r = gpu_lib.sinkhorn_sparse_newton(M, a, b, 0.1, tol=1e9)
     assert r.niter == 0 and len(r.obj_vals) == 1
+
+
+def test_ssns_accepts_a_step_that_raises_f(gpu_lib):
+    # mu0 < 0 makes Hs + mu ||g|| I negative definite, so the first direction
+    # ascends: no candidate lowers f, both halves of rho are negative, the
+    # step is accepted, and the next gradient and Hessian come from the last
+    # candidate's T (src/sinkhorn_ssns.cpp:185-187).  Plain CG on both sides.
+    M, a, b = cloud_problem(28, 21, d=4, seed=6)
+    reg = 0.05
+    print("ssns mu0=-200 28x21", flush=True)
+    r = gpu_lib.sinkhorn_ssns(M, a, b, reg, tol=1e-8, max_iter=200, method=0, mu0=-200.0)
+    o = O.sinkhorn_ssns(M, a, b, reg, tol=1e-8, max_iter=200, method=0, mu0=-200.0)
+    assert r.obj_vals[1] > r.obj_vals[0] and o["obj_vals"][1] > o["obj_vals"][0]
+    assert abs(r.niter - o["niter"]) <= 1 and r.mar_errs[-1] < 1e-8
+    k = min(6, len(r.obj_vals), len(o["obj_vals"]))
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-9)
+    np.testing.assert_allclose(r.mar_errs[:k], o["mar_errs"][:k], rtol=1e-6)
+    np.testing.assert_allclose(r.densities[:k], o["densities"][:k], rtol=1e-12)
+    assert rel(r.dual, o["dual"]) < 1e-6
+
+
+@pytest.mark.parametrize("solver,ls_max_iter", [("sinkhorn_sparse_newton", 2),
+                                                ("sinkhorn_splr", 2),
+                                                ("sinkhorn_sparse_newton", 1)])
+def test_newton_family_exhausted_wolfe_trials(gpu_lib, solver, ls_max_iter):
+    # One or two Wolfe trials: the first search runs out of trials and keeps
+    # the unevaluated step of its last update with the last trial's T (f from
+    # that T and the new potentials), or falls back to I_lo and recomputes.
+    # The direct solve of the oracle against a 1e-13 PCG on the GPU.
+    M, a, b = cloud_problem(28, 21, d=4, seed=9)
+    reg = 0.05
+    kw = dict(density=0.3, shift=1e-3, ls_max_iter=ls_max_iter)
+    print("%s ls_max_iter=%d 28x21" % (solver, ls_max_iter), flush=True)
+    r = getattr(gpu_lib, solver)(M, a, b, reg, tol=1e-8, max_iter=80, cg_tol=1e-13, **kw)
+    o = getattr(O, solver)(M, a, b, reg, tol=1e-8, max_iter=80, method=1, **kw)
+    k = min(12, len(r.obj_vals), len(o["obj_vals"]))
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-11)
+    np.testing.assert_allclose(r.mar_errs[:k], o["mar_errs"][:k], rtol=1e-7)
+    np.testing.assert_allclose(r.densities[:k], o["densities"][:k], rtol=1e-13)
+    if o["niter"] < 80:
+        assert abs(r.niter - o["niter"]) <= max(2, 0.1 * o["niter"])
+        assert abs(r.obj_vals[-1] - o["obj_vals"][-1]) < 1e-10 * abs(o["obj_vals"][-1])
+
+
+def test_splr_one_wolfe_trial_stops_on_a_nonfinite_objective(gpu_lib):
+    # With a single trial SPLR's low-rank steps run away on this problem and
+    # the objective overflows; both stop on the non-finite f at the same
+    # iteration (src/sinkhorn_splr.cpp:101).
+    M, a, b = cloud_problem(28, 21, d=4, seed=9)
+    reg = 0.05
+    kw = dict(density=0.3, shift=1e-3, ls_max_iter=1)
+    print("splr ls_max_iter=1 28x21", flush=True)
+    r = gpu_lib.sinkhorn_splr(M, a, b, reg, tol=1e-8, max_iter=80, cg_tol=1e-13, **kw)
+    o = O.sinkhorn_splr(M, a, b, reg, tol=1e-8, max_iter=80, method=1, **kw)
+    assert o["niter"] == r.niter < 10
+    assert not np.isfinite(o["obj_vals"][-1]) and not np.isfinite(r.obj_vals[-1])
+    k = len(o["obj_vals"]) - 1
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-10)

@@ -39,3 +39,14 @@ def test_implicit_grad_matches_finite_differences_in_M(loss):
             vals.append(loss_value(Mp, reg, gp, loss))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(fd - (G * E).sum()) < 1e-6 * max(1.0, abs(fd)), (loss, fd, (G * E).sum())
+
+
+def test_dense_newton_refinement_reaches_the_unique_optimum():
+    from implicit_ref import refine_dual
+    M, a, b = cloud_problem(60, 45, d=5, seed=3)
+    reg = 0.05
+    rough = O.sinkhorn_bcd(M, a, b, reg, tol=1e-5, max_iter=100000)["dual"]
+    tight = O.sinkhorn_bcd(M, a, b, reg, tol=1e-14, max_iter=1000000)["dual"]
+    g, gn, it = refine_dual(M, a, b, reg, rough, tol=1e-14)
+    assert gn < 1e-14 and 0 < it < 15
+    np.testing.assert_allclose(g, tight, rtol=0, atol=1e-11)
