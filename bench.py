@@ -26,7 +26,10 @@ ones (events between kernels add a few us per pass, so `value` comes from
 solves without them; `config.ms_per_step_with_events` shows the difference).
 `qrot` times the QROT drivers (L-BFGS, BCD, gradient descent) at n = m =
 8192.  `cpu_baseline` times the FP64 CPU oracle (one thread) on a bounded sample of
-BCD iterations of the same problem and scales to the GPU's iteration count.
+BCD iterations of the same problem and scales to the GPU's iteration count;
+with `c2` on it also times one whole oracle solve of the n = m = 8192, reg 0.01
+problem beside the GPU's, and it records the host (CPU model, cores, threads,
+memory).
 """
 import argparse
 import json
@@ -59,6 +62,26 @@ def measured_traffic(kernel, n):
     if not all(x in k for x in names):
         return None
     return sum(k[x]["traffic_bytes"] for x in names)
+
+
+def host_info():
+    """CPU model, logical cores, OMP_NUM_THREADS and memory of the host."""
+    model, mem_gb = None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal"):
+                    mem_gb = int(line.split()[1]) / 2 ** 20
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "ram_gib": mem_gb}
 
 
 def make_problem(n, m, d, seed):
@@ -408,7 +431,7 @@ def main():
     ap.add_argument("--reg", type=float, default=0.01)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iter", type=int, default=20000)
-    ap.add_argument("--cpu-iters", type=int, default=2,
+    ap.add_argument("--cpu-iters", type=int, default=6,
                     help="BCD iterations of the CPU oracle sample (0 = skip)")
     ap.add_argument("--batch-problems", type=int, default=512,
                     help="problems of the batched workload (0 = skip)")
@@ -533,14 +556,20 @@ def main():
     cpu = None
     if args.cpu_iters > 0:
         from oracle import oracle as O
+        print("bench: CPU oracle sample, %d iterations" % args.cpu_iters, file=sys.stderr,
+              flush=True)
         M64 = M32.astype(np.float64)
         secs = O.bcd_iterations(M64, a, b, args.reg, args.cpu_iters)
         del M64
         per_iter = secs / args.cpu_iters
         cpu = {"value": per_iter * niter, "unit": "s", "cores": 1, "kind": "port",
-               "sample": "%d BCD iterations of the FP64 oracle on the same n=m=%d problem "
-                         "(%.3f s/iter), scaled to the GPU's %d iterations"
-                         % (args.cpu_iters, n, per_iter, niter)}
+               "sample": "%d BCD iterations of the FP64 oracle (one thread) on the same n=m=%d "
+                         "problem (%.3f s/iter), scaled to the GPU's %d iterations"
+                         % (args.cpu_iters, n, per_iter, niter),
+               "exp": "scalar libm exp per element; the reference evaluates compute_T and "
+                      "the column log-sum-exp with Eigen's vectorised pexp, so on those loops "
+                      "it would run up to its SIMD width faster than this port",
+               "host": host_info()}
 
     first_order = None
     if args.first_order:
@@ -549,6 +578,22 @@ def main():
     c2 = None
     if args.c2:
         c2 = bench_c2(args.tol)
+        if cpu is not None:
+            # one whole FP64 oracle solve of the n = m = 8192, reg 0.01 problem
+            from oracle import oracle as O
+            print("bench: CPU oracle whole solve, n=m=8192 reg 0.01", file=sys.stderr,
+                  flush=True)
+            M8, a8, b8 = make_problem(8192, 8192, 16, 0)
+            M8 = M8.astype(np.float64)
+            t0 = time.perf_counter()
+            o = O.sinkhorn_bcd(M8, a8, b8, 0.01, tol=args.tol, max_iter=1000)
+            cpu_s = time.perf_counter() - t0
+            del M8
+            gpu = c2.get("bcd_reg0.01", {})
+            cpu["whole_solve_n8192_reg0.01"] = {
+                "cpu_s": cpu_s, "niter": o["niter"], "mar_err": o["mar_errs"][-1],
+                "gpu_s": gpu.get("s"), "gpu_niter": gpu.get("niter"),
+                "speedup": cpu_s / gpu["s"] if gpu.get("s") else None}
 
     newton = None
     if args.newton:

