@@ -236,6 +236,17 @@ void regot_problem::prof_collect()
     }
 }
 
+void regot_problem::prof_drop_after(long last)
+{
+    prof_collect();
+    for (auto& slot : prof_slot) {
+        auto& t = slot.tagged;
+        t.erase(std::remove_if(t.begin(), t.end(),
+                               [last](const std::pair<long, double>& e) { return e.first > last; }),
+                t.end());
+    }
+}
+
 void regot_problem::prof_commit(long last)
 {
     prof_collect();

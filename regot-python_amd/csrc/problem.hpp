@@ -201,4 +201,8 @@ struct regot_problem {
     // Count the tagged pairs with tag <= last, drop the others (launches
     // enqueued past a stop decision return at once and are not passes).
     void prof_commit(long last);
+    // Drop the tagged pairs with tag > last (launches of a loop that stopped
+    // at iterate `last`, which returned at once), keeping the others pending
+    // a later commit.
+    void prof_drop_after(long last);
 };

@@ -289,9 +289,11 @@ void solve_bcd(regot_problem* p, double tol, int max_iter, int verbose, const re
             k0 = niter;
             precise = true;
             res->precise_from = k0;
-            if (verbose >= 2) {
-                p->prof_collect();
-                for (int q = 0; q < 4; q++) cur[q] = p->prof_slot[q].tagged.size();
+            if (p->prof) {
+                // the FP32 launches past iterate k0 returned at once
+                p->prof_drop_after(k0);
+                for (int q = 0; q < 4; q++)
+                    cur[q] = std::min(cur[q], p->prof_slot[q].tagged.size());
             }
             FlagScope fs(p->precise, true);
             ColEpi c;
