@@ -86,6 +86,23 @@ int regot_dual_obj(regot_problem* p, const double* gamma, double* obj);
  * Replaces Problem::dual_obj_grad, src/sinkhorn_problem_obj.cpp:363-393. */
 int regot_dual_obj_grad(regot_problem* p, const double* gamma, double* obj, double* grad);
 
+/* The same f and gradient on DEVICE buffers, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops) and the variant of
+ * regot_dual_obj_grad with the same kernels and results.  M (n x m, row pitch
+ * ldm elements, 0 = m; pointer and pitch 16-byte aligned) is FP32 (bulk =
+ * REGOT_F32: the FP32 row and column passes) or FP64 (REGOT_F64: the one-pass
+ * FP64 kernel); d_a (n), d_b (m), d_gamma (n+m-1) are FP64.  Outputs:
+ * d_out[0] = f, d_out[1] = ||grad||^2, d_grad (n+m-1).  Work is enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream) and not waited for.  The
+ * kernels' scratch is a fixed 32 MiB device array per GPU (held until the
+ * process ends): sizes needing more (beyond about n = m = 16384 in FP64 bulk)
+ * are rejected.  Calls on one GPU share that array and must not overlap. */
+int regot_dual_obj_grad_noalloc(const void* M, int bulk, long long ldm, int n, int m,
+                                const double* d_a, const double* d_b, double reg,
+                                const double* d_gamma, double* d_out, double* d_grad,
+                                void* stream, int device);
+
 /* plan = exp((alpha (+) beta - M)/reg), FP64 row-major n x m, beta_{m-1} = 0.
  * Replaces SinkhornResult::get_plan, src/sinkhorn_result.h:32-50. */
 int regot_get_plan(regot_problem* p, const double* gamma, double* plan);

@@ -98,6 +98,9 @@ def lib():
         getattr(L, name).argtypes = [_vp, _dp, _dp]
     L.regot_dual_obj.argtypes = [_vp, _dp, _dp]
     L.regot_dual_obj_grad.argtypes = [_vp, _dp, _dp, _dp]
+    L.regot_dual_obj_grad_noalloc.argtypes = [
+        _vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, _vp, _vp,
+        ctypes.c_double, _vp, _vp, _vp, _vp, ctypes.c_int]
     L.regot_get_plan.argtypes = [_vp, _dp, _dp]
     L.regot_round_plan.argtypes = [_vp, _dp, _dp]
     L.regot_primal_val.argtypes = [_vp, _dp, _dp]
