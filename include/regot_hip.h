@@ -137,6 +137,18 @@ int regot_sparsified_hess_with_density(regot_problem* p, const double* gamma, do
  * src/sinkhorn_hess.h:75-80. */
 int regot_hess_info(regot_problem* p, long long* nnz_stored, long long* nnz_zero,
                     double* density);
+/* How the handle holds the sparsified Hessian (no reference counterpart; the
+ * reference always stores it, src/sinkhorn_hess.h:22-60):
+ *   -1 (default)  the solver drivers keep a dense kept set (threshold rule,
+ *                 or every entry kept) as a bitmask and recompute sigma from
+ *                 M in every product (FP32 bulk, m - 1 <= 16384);
+ *                 regot_sparsified_hess* store CSR + CSC;
+ *    0            always CSR + CSC;
+ *    1            matrix-free wherever the bulk and shape allow, including
+ *                 regot_sparsified_hess* (regot_hess_export then fails).
+ * Products, CG and the drivers' iterates agree with the stored form to
+ * rounding (the sums run in another order). */
+int regot_set_hess_storage(regot_problem* p, int mode);
 /* Copy h1 (n), h2 (m-1), CSC (colptr m, rowind nnz, cval nnz) and CSR
  * (rowptr n+1, colind nnz, rval nnz) to host; NULL skips an array.
  * (Hessian::to_spmat, src/sinkhorn_hess.cpp:57-96, builds the same matrix.) */

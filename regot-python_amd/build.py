@@ -20,7 +20,8 @@ BUILD = os.path.join(HERE, "_build")
 SOURCES = ["dense_kernels.hip", "vec_kernels.hip", "sparse_kernels.hip", "hessian.hip",
            "problem.hip", "batch.hip", "cost_kernels.hip", "grad.hip", "solvers.cpp",
            "newton.cpp", "first_order.cpp", "apdagd_kernels.hip", "objgrad_kernels.hip",
-           "primal_kernels.hip", "bcd_fused.hip", "qrot_kernels.hip", "qrot.cpp", "noalloc.hip"]
+           "primal_kernels.hip", "bcd_fused.hip", "qrot_kernels.hip", "qrot.cpp", "noalloc.hip",
+           "hess_mf.hip"]
 ARCH = os.environ.get("REGOT_ARCH", "gfx950")
 
 

@@ -143,8 +143,8 @@ struct Work {
     void threshold(double delta, double hint, const double* tpt = nullptr)
     {
         const auto t0 = Clock::now();
-        hess_threshold(p, tpt ? tpt : gamma.p, g, delta, hint);
-        alg_bytes += pass_bytes() + 2.0 * hess_of(p).nnz * 12.0;
+        hess_threshold(p, tpt ? tpt : gamma.p, g, delta, hint, true);
+        alg_bytes += pass_bytes() + build_bytes();
         phase_ms[1] += ms_since(t0);
         phase_n[1] += 1;
     }
@@ -152,8 +152,8 @@ struct Work {
     void topk(double density, const double* tpt = nullptr)
     {
         const auto t0 = Clock::now();
-        hess_topk(p, tpt ? tpt : gamma.p, g, density);
-        alg_bytes += pass_bytes() + 2.0 * hess_of(p).nnz * 12.0;
+        hess_topk(p, tpt ? tpt : gamma.p, g, density, true);
+        alg_bytes += pass_bytes() + build_bytes();
         phase_ms[1] += ms_since(t0);
         phase_n[1] += 1;
     }
@@ -163,12 +163,30 @@ struct Work {
         return static_cast<double>(n) * m * (p->bulk == 0 ? 4.0 : 8.0);
     }
 
+    // what a Hessian build writes: CSR + CSC (12 bytes per entry each), or
+    // the kept-set bitmask of a matrix-free build (one bit per entry)
+    double build_bytes()
+    {
+        const SparseHessian& H = hess_of(p);
+        if (H.mf) return static_cast<double>(n) * (m - 1) / 8.0;
+        return 2.0 * H.nnz * 12.0;
+    }
+
+    // one product with sigma and sigma': the CSR + CSC reads, or a matrix-free
+    // pass (M's first m - 1 columns in FP32 plus the bitmask)
+    double product_bytes()
+    {
+        const SparseHessian& H = hess_of(p);
+        if (H.mf) return static_cast<double>(n) * (m - 1) * (4.0 + 1.0 / 8.0);
+        return 2.0 * H.nnz * 12.0;
+    }
+
     // CG iterations of one solve: one product with Delta each, plus the two
     // sigma products of the block elimination
     void count_cg(int iters)
     {
         cg_iters += iters;
-        alg_bytes += (iters + 1) * 2.0 * hess_of(p).nnz * 12.0;
+        alg_bytes += (iters + 1) * product_bytes();
     }
 
     void report(regot_result* res) const

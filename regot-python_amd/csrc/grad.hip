@@ -179,7 +179,7 @@ int regot_implicit_grad(regot_problem* p, const double* gamma, int loss, double 
             throw;
         }
         p->precise = was;
-        hess_topk(p, p->gamma, p->grad, 1.0);
+        hess_topk(p, p->gamma, p->grad, 1.0, true);
         DBuf<double> alpha(n), beta(m), rhs(N), w(N), R(n), C(m), P(n), Q(m);
         launch_split_gamma(p->gamma, n, m, alpha, beta, s);
         double lv = 0.0;

@@ -1,0 +1,379 @@
+// hess_mf.hip -- products with the sparsified Hessian of the threshold rule
+// straight from M (FP32 bulk), without its CSR / CSC copies.
+//
+// The threshold rule keeps entry (i, j), j < m-1, iff T_ij >= min(tau_j,
+// rho_i) (or T_ij underflows, which adds nothing to a product), and sigma_ij
+// = T_ij / reg.  T_ij is recomputed from M with the sparsifier's own element
+// arithmetic (log2 T formed in FP64, exp2 in FP32), so the kept set and the
+// values are the stored copies' bit for bit.  At the densities SSNS reaches
+// (tens of percent) the two stored copies cost 2 x 12 bytes per kept entry
+// and product, more than one read of M (4 bytes per entry), and the
+// Schur-complement product Delta u = (h2 + s) u - sigma' ((sigma u) / (h1 +
+// s)) needs both halves: here one pass does both, since a workgroup that
+// holds whole rows knows y_i = (sigma u)_i / (h1_i + s) as soon as row i is
+// summed, and adds sigma_ij y_i into column accumulators from the same
+// registers.
+//
+// Layout: one workgroup of 512 threads per compute unit holds a contiguous
+// block of rows; thread t owns the 4-column vectors at 4t + 2048c (c < NC,
+// m - 1 <= 2048 NC <= 16384), with beta_j (FP64), tau_j (FP32) and its FP64
+// column accumulators in registers; the row input vector is staged in LDS
+// (FP64).  Products and sums are FP64.  Each workgroup writes one FP64 column
+// partial per column; mf_fold sums them in a fixed order.
+#include "dense_device.hpp"
+#include "hess_mf.hpp"
+
+#include <cmath>
+
+namespace regot {
+namespace {
+
+using namespace dev;
+
+constexpr int kMaxCols = 16384;          // m - 1 <= 16384
+
+template <int W>
+__device__ __forceinline__ double sum_waves(const double* v, int lane)
+{
+    double x = v[lane & (W - 1)];
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, W);
+    return __shfl(x, 0);
+}
+
+// T_ij of the FP32-bulk sparsifier (Elem<float>::val in sparse_kernels.hip).
+__device__ __forceinline__ float tval(double ab, float mij, double k)
+{
+    return exp2f(static_cast<float>(fma(-k, static_cast<double>(mij), ab * k)));
+}
+
+template <int NC, bool FULL, int SPAN>
+__device__ __forceinline__ void load_row(const float* __restrict__ Mi, int t, int vlast,
+                                         float4* out, bool nt)
+{
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        if (FULL || c < NC - 1 || vlast > 0)
+            out[c] = ld_m(Mi + 4 * t + c * SPAN, nt);
+        else
+            out[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// Kept-set bitmask of the threshold rule in the pass layout: word (i, t)
+// holds bit 4c + q for column 4t + 2048c + q of row i (kept <=> T_ij >=
+// min(tau_j, rho_i) and T_ij != 0 in FP32, the entries a product sees; with
+// tau and rho null every nonzero entry, the full Hessian).
+template <int NC, bool FULL>
+__global__ __launch_bounds__(512) void k_mf_mask(MfArgs a, unsigned int* __restrict__ mask)
+{
+    constexpr int kSpan = 2048;
+    const int t = threadIdx.x;
+    const int m1 = a.m1;
+    const long r0 = static_cast<long>(blockIdx.x) * a.rpw;
+    const long r1 = (r0 + a.rpw < a.n) ? r0 + a.rpw : a.n;
+    int vlast = m1 - (4 * t + (NC - 1) * kSpan);
+    vlast = FULL ? 4 : (vlast < 0 ? 0 : (vlast > 4 ? 4 : vlast));
+    auto ok = [vlast](int c, int q) { return FULL || c < NC - 1 || q < vlast; };
+    double bj[NC][4];
+    float tau[NC][4];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int j0 = 4 * t + c * kSpan;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            bj[c][q] = ok(c, q) ? a.beta[j0 + q] : 0.0;
+            tau[c][q] = !ok(c, q) ? INFINITY : a.tau ? static_cast<float>(a.tau[j0 + q]) : 0.f;
+        }
+    }
+    float4 pf[NC];
+    if (r0 < r1) load_row<NC, FULL, kSpan>(a.M + r0 * a.ldm, t, vlast, pf, a.nt != 0);
+#pragma unroll 1
+    for (long i = r0; i < r1; i++) {
+        float e[NC][4];
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) e[c][q] = vget<float4, float>(pf[c], q);
+        if (i + 1 < r1) load_row<NC, FULL, kSpan>(a.M + (i + 1) * a.ldm, t, vlast, pf, a.nt != 0);
+        const double ai = a.alpha[i];
+        const float ri = a.rho ? static_cast<float>(a.rho[i]) : 0.f;
+        unsigned int bits = 0u;
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (ok(c, q)) {
+                    const float tv = tval(ai + bj[c][q], e[c][q], a.k);
+                    if (tv >= fminf(tau[c][q], ri) && tv != 0.f) bits |= 1u << (4 * c + q);
+                }
+        mask[i * 512 + t] = bits;
+    }
+}
+
+// MODE (see hess_mf.hpp): which halves run and what weights the columns.
+template <int MODE, int NC, bool FULL, int TH>
+__global__ __launch_bounds__(TH) void k_mf_pass(MfArgs a)
+{
+    constexpr bool ROW = MODE == MF_CG || MODE == MF_HS || MODE == MF_ROWS;
+    constexpr bool COL = MODE != MF_ROWS;
+    constexpr int kThreads = TH;
+    constexpr int kWaves = TH / 64;
+    constexpr int kSpan = 4 * TH;
+    __shared__ double sx[ROW ? NC * kSpan : 1];
+    __shared__ double red[2][kWaves];
+    if (a.ctl != nullptr && a.ctl[0] != 0) return;
+
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    if (MODE == MF_CG && a.rr_part != nullptr) {
+        // every workgroup sums the partials in the same order, so all decide
+        // alike; only workgroup 0 writes, after its own test
+        if (wave == 0) {
+            double v = 0.0;
+            for (int b = lane; b < a.rr_n; b += 64) v += a.rr_part[b];
+            v = wave_sum0(v);
+            if (lane == 0) red[0][0] = v;
+        }
+        __syncthreads();
+        const double rr = red[0][0];
+        const bool stop = rr < a.scal[5];
+        if (blockIdx.x == 0 && t == 0) {
+            a.scal[4] = rr;
+            if (stop) {
+                a.ctl[1] = a.it;   // the converged update is not counted
+                a.ctl[0] = 1;
+            }
+        }
+        if (stop) return;
+        __syncthreads();   // red[0][0] is reused by the row reduction
+    }
+    const int m1 = a.m1;
+    const int r0 = blockIdx.x * a.rpw;
+    const int r1 = (r0 + a.rpw < a.n) ? r0 + a.rpw : a.n;
+
+    int vlast = m1 - (4 * t + (NC - 1) * kSpan);
+    vlast = FULL ? 4 : (vlast < 0 ? 0 : (vlast > 4 ? 4 : vlast));
+    auto ok = [vlast](int c, int q) { return FULL || c < NC - 1 || q < vlast; };
+
+    if (ROW)
+        for (int j = t; j < m1; j += kThreads) sx[j] = a.xc[j];
+    double bj[NC][4];
+    double acc[NC][4];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int j0 = 4 * t + c * kSpan;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            bj[c][q] = ok(c, q) ? a.beta[j0 + q] : 0.0;
+            acc[c][q] = 0.0;
+        }
+    }
+    if (ROW) __syncthreads();
+    const unsigned int* mk = a.mask + t;
+
+    float4 pf[NC];
+    unsigned int pbits = 0u;
+    if (r0 < r1) {
+        load_row<NC, FULL, kSpan>(a.M + static_cast<long>(r0) * a.ldm, t, vlast, pf, a.nt != 0);
+        pbits = mk[static_cast<long>(r0) * kThreads];
+    }
+#pragma unroll 1
+    for (int i = r0; i < r1; i++) {
+        float e[NC][4];
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) e[c][q] = vget<float4, float>(pf[c], q);
+        const unsigned int bits = pbits;
+        if (i + 1 < r1) {
+            load_row<NC, FULL, kSpan>(a.M + static_cast<long>(i + 1) * a.ldm, t, vlast, pf,
+                                      a.nt != 0);
+            pbits = mk[static_cast<long>(i + 1) * kThreads];
+        }
+        const double ai = a.alpha[i];
+        double racc = 0.0;
+        // T is formed for every element and then masked: a branch per element
+        // keeps each exp's operands live across it and spills at 8 chunks
+        const double* vx = sx;
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float tx = tval(ai + bj[c][q], e[c][q], a.k);
+                const float tv = ((bits >> (4 * c + q)) & 1u) ? tx : 0.f;
+                e[c][q] = tv;
+                if (ROW && ok(c, q))
+                    racc = fma(static_cast<double>(tv), vx[4 * t + c * kSpan + q], racc);
+            }
+        // keep the row's terms as FP32 across the row reduction (the FP64
+        // copies the row half converted would otherwise double their registers)
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) asm volatile("" : "+v"(e[c][q]));
+        double cw = 0.0;
+        if (ROW) {
+            const int slot = (i - r0) & 1;
+            const double S = wave_sum0(racc);
+            if (lane == 0) red[slot][wave] = S;
+            __syncthreads();
+            const double rv = sum_waves<kWaves>(red[slot], lane) * a.inv_reg;   // (sigma xc)_i
+            if (MODE == MF_CG) {
+                cw = rv / (a.h1[i] + a.shift) * a.inv_reg;
+            } else {
+                if (t == 0) a.rowout[i] = rv;
+                cw = (MODE == MF_HS) ? a.xr[i] * a.inv_reg : 0.0;
+            }
+        } else if (MODE == MF_COLS) {
+            cw = a.xr[i] * a.inv_reg;
+        } else {   // MF_DIAG: sum_i sigma_ij^2 / (h1_i + s)
+            cw = a.inv_reg * a.inv_reg / (a.h1[i] + a.shift);
+        }
+        if (COL) {
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const double tv = static_cast<double>(e[c][q]);
+                    acc[c][q] = fma(MODE == MF_DIAG ? tv * tv : tv, cw, acc[c][q]);
+                }
+        }
+    }
+    if (COL) {
+        double* pw = a.part + static_cast<long>(blockIdx.x) * m1;
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (ok(c, q)) pw[4 * t + c * kSpan + q] = acc[c][q];
+    }
+}
+
+template <int MODE, int NC, int TH>
+void pass_launch(const MfArgs& a, int nwg, bool full, hipStream_t s)
+{
+    if (full)
+        hipLaunchKernelGGL((k_mf_pass<MODE, NC, true, TH>), dim3(nwg), dim3(TH), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_mf_pass<MODE, NC, false, TH>), dim3(nwg), dim3(TH), 0, s, a);
+}
+
+template <int MODE>
+void pass_nc(const MfArgs& a, int nwg, hipStream_t s)
+{
+    const int NC = ceil_div(a.m1, 2048);
+    const bool full = a.m1 == NC * 2048;
+    switch (NC) {
+    case 1: pass_launch<MODE, 1, 512>(a, nwg, full, s); break;
+    case 2: pass_launch<MODE, 2, 512>(a, nwg, full, s); break;
+    case 3: pass_launch<MODE, 3, 512>(a, nwg, full, s); break;
+    case 4: pass_launch<MODE, 4, 512>(a, nwg, full, s); break;
+    case 5: pass_launch<MODE, 5, 512>(a, nwg, full, s); break;
+    case 6: pass_launch<MODE, 6, 512>(a, nwg, full, s); break;
+    case 7: pass_launch<MODE, 7, 512>(a, nwg, full, s); break;
+    default: pass_launch<MODE, 8, 512>(a, nwg, full, s); break;
+    }
+}
+
+// out_j = sum_w part[w][j] (fixed order), then by epi: 0 out = sum; 1 out =
+// z_j - sum (sigma' applied and subtracted); 2 out = (h2_j + shift) z_j -
+// sum (Delta form); 3 out = h2_j z_j + sum (lower block of Hs);
+// 4 out = 1 / max(h2_j + shift - sum, floor) (Jacobi diagonal of Delta).
+__global__ __launch_bounds__(256) void k_mf_fold(const double* __restrict__ part, int nwg, int m1,
+                                                 int epi, const double* __restrict__ z,
+                                                 const double* __restrict__ h2, double shift,
+                                                 double* __restrict__ out)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= m1) return;
+    double s = 0.0;
+    int w = 0;
+    for (; w + 8 <= nwg; w += 8) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = part[static_cast<long>(w + u) * m1 + j];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += x[u];
+    }
+    for (; w < nwg; w++) s += part[static_cast<long>(w) * m1 + j];
+    switch (epi) {
+    case 0: out[j] = s; break;
+    case 1: out[j] = z[j] - s; break;
+    case 2: out[j] = h2[j] * z[j] + shift * z[j] - s; break;
+    case 3: out[j] = h2[j] * z[j] + s; break;
+    default: {
+        double d = h2[j] + shift - s;
+        const double floor = 1e-12 * (h2[j] + shift);
+        if (!(d > floor)) d = (h2[j] + shift > 0.0) ? h2[j] + shift : 1.0;
+        out[j] = 1.0 / d;
+    }
+    }
+}
+
+}  // namespace
+
+bool mf_shape_ok(int n, int m) { return n >= 1 && m >= 2 && m - 1 <= kMaxCols; }
+
+int mf_blocks(int n, int ncu)
+{
+    int g = ncu > 0 ? ncu : 256;
+    if (g > n) g = n;
+    const int rpw = ceil_div(n, g);
+    return ceil_div(n, rpw);
+}
+
+template <int NC>
+void mask_launch(const MfArgs& a, int nwg, bool full, unsigned int* mask, hipStream_t s)
+{
+    if (full)
+        hipLaunchKernelGGL((k_mf_mask<NC, true>), dim3(nwg), dim3(512), 0, s, a, mask);
+    else
+        hipLaunchKernelGGL((k_mf_mask<NC, false>), dim3(nwg), dim3(512), 0, s, a, mask);
+}
+
+void launch_mf_mask(MfArgs a, int nwg, unsigned int* mask, hipStream_t s)
+{
+    if (!mf_shape_ok(a.n, a.m1 + 1)) fail(E_INTERNAL, "matrix-free mask: %d x %d", a.n, a.m1 + 1);
+    a.rpw = ceil_div(a.n, nwg);
+    a.nt = 0;
+    const int NC = ceil_div(a.m1, 2048);
+    const bool full = a.m1 == NC * 2048;
+    switch (NC) {
+    case 1: mask_launch<1>(a, nwg, full, mask, s); break;
+    case 2: mask_launch<2>(a, nwg, full, mask, s); break;
+    case 3: mask_launch<3>(a, nwg, full, mask, s); break;
+    case 4: mask_launch<4>(a, nwg, full, mask, s); break;
+    case 5: mask_launch<5>(a, nwg, full, mask, s); break;
+    case 6: mask_launch<6>(a, nwg, full, mask, s); break;
+    case 7: mask_launch<7>(a, nwg, full, mask, s); break;
+    default: mask_launch<8>(a, nwg, full, mask, s); break;
+    }
+    KERNEL_CHECK();
+}
+
+void launch_mf_pass(int mode, MfArgs a, int nwg, hipStream_t s)
+{
+    if (!mf_shape_ok(a.n, a.m1 + 1)) fail(E_INTERNAL, "matrix-free pass: %d x %d", a.n, a.m1 + 1);
+    a.rpw = ceil_div(a.n, nwg);
+    a.nt = static_cast<long>(a.n) * a.ldm * 4 > (256L << 20) ? 1 : 0;
+    switch (mode) {
+    case MF_CG: pass_nc<MF_CG>(a, nwg, s); break;
+    case MF_HS: pass_nc<MF_HS>(a, nwg, s); break;
+    case MF_ROWS: pass_nc<MF_ROWS>(a, nwg, s); break;
+    case MF_COLS: pass_nc<MF_COLS>(a, nwg, s); break;
+    default: pass_nc<MF_DIAG>(a, nwg, s); break;
+    }
+    KERNEL_CHECK();
+}
+
+void launch_mf_fold(const double* part, int nwg, int m1, int epi, const double* z,
+                    const double* h2, double shift, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mf_fold, dim3(ceil_div(m1, 256)), dim3(256), 0, s, part, nwg, m1, epi,
+                       z, h2, shift, out);
+    KERNEL_CHECK();
+}
+
+}  // namespace regot

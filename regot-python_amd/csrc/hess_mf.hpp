@@ -1,0 +1,57 @@
+// hess_mf.hpp -- matrix-free products with the threshold-rule sparsified
+// Hessian (FP32 bulk; hess_mf.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace regot {
+
+// What one pass over M computes (sigma = kept T / reg, j < m-1):
+//   MF_CG    y_i = (sigma xc)_i / (h1_i + shift); column partials of sigma' y
+//   MF_HS    rowout = sigma xc; column partials of sigma' xr
+//   MF_ROWS  rowout = sigma xc
+//   MF_COLS  column partials of sigma' xr
+//   MF_DIAG  column partials of sum_i sigma_ij^2 / (h1_i + shift)
+enum { MF_CG = 0, MF_HS = 1, MF_ROWS = 2, MF_COLS = 3, MF_DIAG = 4 };
+
+struct MfArgs {
+    const float* M = nullptr;
+    long ldm = 0;
+    int n = 0, m1 = 0;             // rows, columns of sigma (m - 1)
+    const double* alpha = nullptr;  // n
+    const double* beta = nullptr;   // m (beta[m-1] = 0)
+    double k = 0.0;                 // log2(e) / reg
+    double inv_reg = 0.0;
+    const double* tau = nullptr;    // per column (mask pass)
+    const double* rho = nullptr;    // per row (mask pass)
+    const unsigned int* mask = nullptr;   // kept-set bitmask, n x 512 words (products)
+    const double* xc = nullptr;     // m1: input of the row half
+    const double* xr = nullptr;     // n: input of the column half (HS, COLS)
+    const double* h1 = nullptr;
+    double shift = 0.0;
+    double* rowout = nullptr;       // n (HS, ROWS)
+    double* part = nullptr;         // nwg x m1 column partials
+    int* ctl = nullptr;             // nullable: ctl[0] != 0 -> return at once
+    // MF_CG inside the device-controlled CG (nullable): stop test on the sum
+    // of rr_n partials of r.r against scal[5]; on stop workgroup 0 records
+    // scal[4] = r.r, ctl[1] = it, ctl[0] = 1 and no workgroup does the pass
+    const double* rr_part = nullptr;
+    int rr_n = 0;
+    double* scal = nullptr;
+    int it = 0;
+    int rpw = 0;                    // rows per workgroup (set by the launcher)
+    int nt = 0;                     // nontemporal loads of M (set by the launcher)
+};
+
+bool mf_shape_ok(int n, int m);
+int mf_blocks(int n, int ncu);
+// Kept-set bitmask of the threshold rule (tau, rho; both null: every nonzero
+// entry) into mask (n x 512 words).
+void launch_mf_mask(MfArgs a, int nwg, unsigned int* mask, hipStream_t s);
+void launch_mf_pass(int mode, MfArgs a, int nwg, hipStream_t s);
+// out (m1) from the nwg x m1 partials: epi 0 sum; 1 z - sum; 2 (h2 + shift) z -
+// sum; 3 h2 z + sum; 4 the Jacobi inverse diagonal 1 / (h2 + shift - sum).
+void launch_mf_fold(const double* part, int nwg, int m1, int epi, const double* z,
+                    const double* h2, double shift, double* out, hipStream_t s);
+
+}  // namespace regot

@@ -5,6 +5,7 @@
 #include <deque>
 #include <vector>
 
+#include "hess_mf.hpp"
 #include "hessian.hpp"
 #include "regot_hip.h"
 
@@ -294,6 +295,51 @@ __global__ __launch_bounds__(256) void k_cg_cols(CgArgs a, int it)
     }
 }
 
+// Matrix-free form of k_cg_cols: sigma' y arrives as nwg FP64 partials per
+// column (mpart, nwg x m1) from the pass over M, which also made the stop
+// test; w = (h2 + shift) u - sigma' y and the partials of r.u, w.u per
+// workgroup of 256 columns.
+__global__ __launch_bounds__(256) void k_cg_cols_mf(CgArgs a, const double* __restrict__ mpart,
+                                                    int nwg)
+{
+    __shared__ double sm[2][256];
+    if (a.ctl[0]) return;
+    const int m1 = a.m1;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    double g = 0.0, d = 0.0;
+    if (j < m1) {
+        double t = 0.0;
+        int w = 0;
+        for (; w + 8 <= nwg; w += 8) {
+            double x[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) x[q] = mpart[static_cast<long>(w + q) * m1 + j];
+#pragma unroll
+            for (int q = 0; q < 8; q++) t += x[q];
+        }
+        for (; w < nwg; w++) t += mpart[static_cast<long>(w) * m1 + j];
+        const double uj = a.u[j];
+        const double wj = a.h2[j] * uj + a.shift * uj - t;
+        a.w[j] = wj;
+        g = a.r[j] * uj;
+        d = wj * uj;
+    }
+    sm[0][threadIdx.x] = g;
+    sm[1][threadIdx.x] = d;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.part[2 * blockIdx.x] = sm[0][0];
+        a.part[2 * blockIdx.x + 1] = sm[1][0];
+    }
+}
+
 }  // namespace
 
 void hess_ensure_transpose(regot_problem* p)
@@ -328,12 +374,11 @@ static ElemCtx make_ctx(regot_problem* p, SparseHessian& H)
     return c;
 }
 
-// Shared tail of both rules: row counts (in H.rcnt) -> scan -> CSR write ->
-// CSC by a counting sort of the CSR.
-static void build_storage(regot_problem* p, SparseHessian& H, const ElemCtx& c,
-                          const KeepRule& rule)
+// Stored entries from the row counts in H.rcnt: scan into H.rowptr, total
+// into H.nnz (a host synchronisation).
+static long long count_stored(regot_problem* p, SparseHessian& H)
 {
-    const int n = p->n, m = p->m;
+    const int n = p->n;
     hipStream_t s = p->stream;
     H.rowptr.ensure(n + 1);
     H.ltmp.ensure(4);
@@ -341,9 +386,58 @@ static void build_storage(regot_problem* p, SparseHessian& H, const ElemCtx& c,
     H.hll.ensure(4);
     HIP_CHECK(hipMemcpyAsync(H.hll.p, H.ltmp.p, sizeof(long long), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    const long long nnz = H.hll.p[0];
+    H.nnz = H.hll.p[0];
+    return H.nnz;
+}
+
+static MfArgs mf_args(regot_problem* p, const SparseHessian& H)
+{
+    MfArgs a;
+    a.M = static_cast<const float*>(p->dM);
+    a.ldm = p->ldm;
+    a.n = p->n;
+    a.m1 = p->m - 1;
+    a.alpha = H.vn;
+    a.beta = H.vm;
+    a.k = p->k;
+    a.inv_reg = 1.0 / p->reg;
+    a.mask = H.mf_mask;
+    a.h1 = H.h1;
+    return a;
+}
+
+// Whether a build may go matrix-free: FP32 bulk, m - 1 <= 16384, and the
+// handle's storage mode (driver: the caller is a solver driver, which needs
+// only products).
+static bool mf_allowed(const regot_problem* p, bool driver)
+{
+    if (p->bulk != 0 || !mf_shape_ok(p->n, p->m) || p->hess_storage == 0) return false;
+    return p->hess_storage == 1 || driver;
+}
+
+// Matrix-free build: the kept-set bitmask of the threshold rule (tau, rho)
+// or, with both null, every entry kept (the full Hessian).
+static void build_mf(regot_problem* p, SparseHessian& H, const double* tau, const double* rho)
+{
+    H.mf_nwg = mf_blocks(p->n, p->ncu);
+    H.mf_mask.ensure(static_cast<size_t>(p->n) * 512);
+    H.mf_part.ensure(static_cast<size_t>(H.mf_nwg) * std::max(p->m - 1, 1));
+    MfArgs a = mf_args(p, H);
+    a.tau = tau;
+    a.rho = rho;
+    launch_mf_mask(a, H.mf_nwg, H.mf_mask, p->stream);
+    H.mf = true;
+}
+
+// Shared tail of both rules: row counts (in H.rcnt) -> scan -> CSR write ->
+// CSC by a counting sort of the CSR.
+static void build_storage(regot_problem* p, SparseHessian& H, const ElemCtx& c,
+                          const KeepRule& rule, bool counted = false)
+{
+    const int n = p->n, m = p->m;
+    hipStream_t s = p->stream;
+    const long long nnz = counted ? H.nnz : count_stored(p, H);
     if (nnz > 0x7fffffffLL) fail(E_INVALID, "sparsified Hessian has %lld entries (> 2^31)", nnz);
-    H.nnz = nnz;
     H.colind.ensure(nnz);
     H.rval.ensure(nnz);
     H.rowind.ensure(nnz);
@@ -374,6 +468,7 @@ static void prepare(regot_problem* p, SparseHessian& H, const double* d_gamma, c
     H.h2.ensure(m);
     H.rcnt.ensure(n + 1);   // also receives an (n+1)-entry scan
     H.zcnt.ensure(n);
+    H.mf = false;
     launch_split_gamma(d_gamma, n, m, H.vn, H.vm, s);
     hipLaunchKernelGGL(k_hess_diag, dim3(ceil_div(std::max(n, m), 256)), dim3(256), 0, s, d_grad,
                        p->a.p, p->b.p, n, m, p->reg, H.h1.p, H.h2.p);
@@ -381,7 +476,7 @@ static void prepare(regot_problem* p, SparseHessian& H, const double* d_gamma, c
 }
 
 void hess_threshold(regot_problem* p, const double* d_gamma, const double* d_grad, double delta,
-                    double density_hint)
+                    double density_hint, bool driver)
 {
     SparseHessian& H = hess_of(p);
     prepare(p, H, d_gamma, d_grad);
@@ -396,17 +491,31 @@ void hess_threshold(regot_problem* p, const double* d_gamma, const double* d_gra
     rule.mode = 0;
     rule.tau = H.tau;
     rule.rho = H.rho;
-    build_storage(p, H, c, rule);
+    bool mf = false;
+    if (mf_allowed(p, driver)) {
+        // matrix-free once one product over the stored copies (2 x 12 bytes
+        // per entry) would move more than half a read of M (4 bytes per entry)
+        const long long nnz = count_stored(p, H);
+        mf = p->hess_storage == 1 || 24.0 * static_cast<double>(nnz) >
+                                         2.0 * static_cast<double>(p->n) * (p->m - 1);
+        if (mf)
+            build_mf(p, H, H.tau, H.rho);
+        else
+            build_storage(p, H, c, rule, true);
+    } else {
+        build_storage(p, H, c, rule);
+    }
     // FP64-zero kept entries count towards the reported density
     launch_scan(H.zcnt, p->n, H.rcnt, H.ltmp.p + 2, s);
     HIP_CHECK(hipMemcpyAsync(H.hll.p, H.ltmp.p, 3 * sizeof(long long), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     H.nnz_zero = H.hll.p[2];
-    if (H.hll.p[1] != H.nnz)
+    if (!mf && H.hll.p[1] != H.nnz)
         fail(E_INTERNAL, "CSR and CSC counts differ (%lld vs %lld)", H.nnz, H.hll.p[1]);
 }
 
-void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, double density)
+void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, double density,
+               bool driver)
 {
     SparseHessian& H = hess_of(p);
     prepare(p, H, d_gamma, d_grad);
@@ -416,6 +525,13 @@ void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, do
     density = std::min(1.0, std::max(0.0, density));
     const long long len = static_cast<long long>(n) * (m - 1);
     const long long kk = static_cast<long long>(static_cast<double>(len) * density);
+    if (kk >= len && mf_allowed(p, driver)) {
+        // every nonzero entry kept: products straight from M
+        build_mf(p, H, nullptr, nullptr);
+        H.nnz = len;
+        H.nnz_zero = 0;
+        return;
+    }
     KeepRule rule;
     rule.mode = 1;
     rule.preserve = 1;
@@ -448,10 +564,28 @@ void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, do
         fail(E_INTERNAL, "CSR and CSC counts differ (%lld vs %lld)", H.nnz, H.hll.p[1]);
 }
 
+// One matrix-free pass (see hess_mf.hpp for the modes).
+static void mf_pass(regot_problem* p, SparseHessian& H, int mode, const double* xc,
+                    const double* xr, double shift, double* rowout)
+{
+    MfArgs a = mf_args(p, H);
+    a.xc = xc;
+    a.xr = xr;
+    a.shift = shift;
+    a.rowout = rowout;
+    a.part = H.mf_part;
+    launch_mf_pass(mode, a, H.mf_nwg, p->stream);
+}
+
 // Delta x = h2 .* x + shift x - sigma' ((sigma x) ./ (h1 + shift)); x, out: m-1.
 void hess_apply_delta(regot_problem* p, const double* x, double shift, double* out)
 {
     SparseHessian& H = hess_of(p);
+    if (H.mf) {
+        mf_pass(p, H, MF_CG, x, nullptr, shift, nullptr);
+        launch_mf_fold(H.mf_part, H.mf_nwg, p->m - 1, 2, x, H.h2, shift, out, p->stream);
+        return;
+    }
     H.cg_y_buf.ensure(p->n);
     launch_spmv_rows(H, x, H.cg_y_buf, true, shift, p->stream);
     launch_spmv_cols(H, H.cg_y_buf, x, shift, 1, out, p->stream);
@@ -462,12 +596,17 @@ void hess_apply_hs(regot_problem* p, const double* x, double* out)
 {
     SparseHessian& H = hess_of(p);
     const int n = p->n, m = p->m;
-    launch_spmv_rows(H, x + n, out, false, 0.0, p->stream);
+    if (H.mf)
+        mf_pass(p, H, MF_HS, x + n, x, 0.0, out);
+    else
+        launch_spmv_rows(H, x + n, out, false, 0.0, p->stream);
     hipLaunchKernelGGL(k_diag_axpy, dim3(ceil_div(n, 256)), dim3(256), 0, p->stream, out, H.h1.p,
                        x, 0.0, n, 0);
-    launch_spmv_cols(H, x, x + n, 0.0, 2, out + n, p->stream);
-    (void)m;
     KERNEL_CHECK();
+    if (H.mf)
+        launch_mf_fold(H.mf_part, H.mf_nwg, m - 1, 3, x + n, H.h2, 0.0, out + n, p->stream);
+    else
+        launch_spmv_cols(H, x, x + n, 0.0, 2, out + n, p->stream);
 }
 
 // CG on Delta (size m-1) from zero; textbook unpreconditioned CG with the
@@ -537,16 +676,22 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     H.cg_y_buf.ensure(n);
     const int lpr_r = lanes_for(static_cast<double>(H.nnz) / std::max(n, 1), n);
     const int lpr_c = lanes_for(static_cast<double>(H.nnz) / m1, m1);
-    const int nbc = ceil_div(m1, 4 * (64 / lpr_c));
+    // column workgroups: 256 columns each when the products are passes over M
+    const int nbc = H.mf ? ceil_div(m1, 256) : ceil_div(m1, 4 * (64 / lpr_c));
     const int nbu = ceil_div(m1, 256);
     const int nbs = ceil_div(m1, 1024);   // set-up partials
     H.cg_part.ensure(std::max(2L * nbc + nbu, 2L * nbs));
     H.cg_scal.ensure(8);
     if (jacobi) {
         H.cg_dinv.ensure(m1);
-        hipLaunchKernelGGL(k_delta_diag, dim3(ceil_div(m1, 4)), dim3(256), 0, s, H.colptr.p,
-                           H.rowind.p, H.cval.p, H.h1.p, H.h2.p, shift, m1, H.cg_dinv.p);
-        KERNEL_CHECK();
+        if (H.mf) {
+            mf_pass(p, H, MF_DIAG, nullptr, nullptr, shift, nullptr);
+            launch_mf_fold(H.mf_part, H.mf_nwg, m1, 4, nullptr, H.h2, shift, H.cg_dinv, s);
+        } else {
+            hipLaunchKernelGGL(k_delta_diag, dim3(ceil_div(m1, 4)), dim3(256), 0, s, H.colptr.p,
+                               H.rowind.p, H.cval.p, H.h1.p, H.h2.p, shift, m1, H.cg_dinv.p);
+            KERNEL_CHECK();
+        }
     }
     const double* dinv = jacobi ? H.cg_dinv.p : nullptr;
     H.cg_ctl.ensure(4);
@@ -635,11 +780,31 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     hipEvent_t ev[2];
     HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    // matrix-free: the pass over M makes the stop test and y = (sigma u) /
+    // (h1 + shift) and sigma' y in one read; k_cg_cols_mf folds its partials
+    MfArgs ma;
+    if (H.mf) {
+        ma = mf_args(p, H);
+        ma.xc = H.cg_u;
+        ma.shift = shift;
+        ma.part = H.mf_part;
+        ma.ctl = H.cg_ctl;
+        ma.rr_part = H.cg_part.p + 2 * nbc;
+        ma.rr_n = nbu;
+        ma.scal = H.cg_scal;
+    }
     auto launch_batch = [&](int slot) {
         for (int q = 0; q < K && k_enq < max_iter; q++, k_enq++) {
             hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, a, k_enq);
-            cg_rows(lpr_r, a, s);
-            cg_cols(lpr_c, a, k_enq, s);
+            if (H.mf) {
+                ma.it = k_enq;
+                launch_mf_pass(MF_CG, ma, H.mf_nwg, s);
+                hipLaunchKernelGGL(k_cg_cols_mf, dim3(nbc), dim3(256), 0, s, a, H.mf_part.p,
+                                   H.mf_nwg);
+            } else {
+                cg_rows(lpr_r, a, s);
+                cg_cols(lpr_c, a, k_enq, s);
+            }
         }
         KERNEL_CHECK();
         HIP_CHECK(hipMemcpyAsync(H.cg_hctl.p + 4 * slot, H.cg_ctl.p, 4 * sizeof(int),
@@ -695,11 +860,19 @@ int hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int m
     hipLaunchKernelGGL(k_diag_axpy, dim3(ceil_div(n, 256)), dim3(256), 0, s, H.cg_y_buf.p, H.h1.p,
                        rhs, shift, n, 1);
     KERNEL_CHECK();
-    launch_spmv_cols(H, H.cg_y_buf, nullptr, 0.0, 0, H.vm2, s);
-    vec_axpby(H.vm2, -1.0, H.vm2, 1.0, rhs + n, m1, s);
+    if (H.mf) {
+        mf_pass(p, H, MF_COLS, nullptr, H.cg_y_buf, 0.0, nullptr);
+        launch_mf_fold(H.mf_part, H.mf_nwg, m1, 1, rhs + n, nullptr, 0.0, H.vm2, s);
+    } else {
+        launch_spmv_cols(H, H.cg_y_buf, nullptr, 0.0, 0, H.vm2, s);
+        vec_axpby(H.vm2, -1.0, H.vm2, 1.0, rhs + n, m1, s);
+    }
     const int it = hess_cg_delta(p, H.vm2, shift, tol, max_iter, out + n, tol_error,
                                  x0 ? x0 + n : nullptr, jacobi);
-    launch_spmv_rows(H, out + n, H.cg_y_buf, false, 0.0, s);
+    if (H.mf)
+        mf_pass(p, H, MF_ROWS, out + n, nullptr, 0.0, H.cg_y_buf);
+    else
+        launch_spmv_rows(H, out + n, H.cg_y_buf, false, 0.0, s);
     hipLaunchKernelGGL(k_sub_div, dim3(ceil_div(n, 256)), dim3(256), 0, s, rhs, H.cg_y_buf.p,
                        H.h1.p, shift, n, out);
     KERNEL_CHECK();
@@ -730,7 +903,7 @@ static void hess_check(regot_problem* p, bool need_built)
 {
     if (!p) fail(E_INVALID, "null problem handle");
     HIP_CHECK(hipSetDevice(p->device));
-    if (need_built && (!p->hess || !p->hess->rowptr.p))
+    if (need_built && (!p->hess || (!p->hess->mf && !p->hess->rowptr.p)))
         fail(E_STATE, "no sparsified Hessian: call regot_sparsified_hess* first");
 }
 
@@ -769,6 +942,15 @@ int regot_sparsified_hess_with_density(regot_problem* p, const double* gamma, do
     CAPI_END
 }
 
+int regot_set_hess_storage(regot_problem* p, int mode)
+{
+    CAPI_BEGIN
+    if (!p) fail(E_INVALID, "null problem handle");
+    if (mode < -1 || mode > 1) fail(E_INVALID, "storage mode must be -1, 0 or 1 (got %d)", mode);
+    p->hess_storage = mode;
+    CAPI_END
+}
+
 int regot_hess_info(regot_problem* p, long long* nnz_stored, long long* nnz_zero, double* density)
 {
     CAPI_BEGIN
@@ -785,6 +967,9 @@ int regot_hess_export(regot_problem* p, double* h1, double* h2, int* colptr, int
     CAPI_BEGIN
     hess_check(p, true);
     SparseHessian& H = *p->hess;
+    if (H.mf)
+        fail(E_STATE, "the Hessian was built matrix-free and holds no CSR / CSC "
+                      "(regot_set_hess_storage(p, 0) keeps them)");
     const int n = p->n, m = p->m;
     const long long z = H.nnz;
     auto cp = [&](void* dst, const void* src, size_t bytes) {

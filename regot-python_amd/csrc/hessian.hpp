@@ -9,10 +9,15 @@ namespace regot {
 SparseHessian& hess_of(regot_problem* p);
 void hess_ensure_transpose(regot_problem* p);
 // Threshold rule (SSNS): sparsify_mat3 semantics at (gamma, grad), device vectors.
+// driver: the caller needs only products, so under the handle's default
+// storage mode a dense kept set (FP32 bulk, m - 1 <= 16384) is held as a
+// bitmask and every product is a pass over M (SparseHessian::mf).
 void hess_threshold(regot_problem* p, const double* d_gamma, const double* d_grad, double delta,
-                    double density_hint);
+                    double density_hint, bool driver = false);
 // Top-k rule (sparse Newton, SPLR): sparsify_mat_with_density semantics, preserve = true.
-void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, double density);
+// At density 1 (every entry kept) the build may go matrix-free as above.
+void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, double density,
+               bool driver = false);
 void hess_apply_delta(regot_problem* p, const double* x, double shift, double* out);
 void hess_apply_hs(regot_problem* p, const double* x, double* out);
 // CG on the Schur complement Delta (m-1) from x0 (device, m-1) or zero;

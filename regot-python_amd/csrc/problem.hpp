@@ -153,6 +153,11 @@ struct regot_problem {
     // FP32 bulk: the BCD loop's row and column passes with FP64 exp and sums
     // (the precise finish of solve_bcd).
     bool bcd_precise = false;
+    // Storage of the threshold-rule Hessian: -1 the Newton drivers go
+    // matrix-free (products straight from M, FP32 bulk) when the kept set is
+    // dense enough, the Hessian C-ABI stores CSR / CSC; 0 always stored; 1
+    // matrix-free wherever the bulk and shape allow.
+    int hess_storage = -1;
 
     // profiling of the hot passes: [0] row pass, [1] column pass (partial +
     // combine), [2] single-read BCD pass, [3] its column-sum combine

@@ -53,6 +53,13 @@ struct SparseHessian {
     DBuf<double> vn, vm, vm2;        // alpha, beta (full) of the current build; scratch
     DBuf<double> cg_r, cg_p, cg_q, cg_s, cg_u, cg_y_buf, cg_part, cg_scal, cg_dinv;
     DBuf<int> cg_ctl;
+    // Matrix-free build (threshold rule, FP32 bulk): no CSR / CSC; the kept
+    // set is the bitmask mf_mask (n x 512 words) and every product is a pass
+    // over M by mf_nwg workgroups, whose column partials go to mf_part.
+    bool mf = false;
+    int mf_nwg = 0;
+    DBuf<unsigned int> mf_mask;
+    DBuf<double> mf_part;
     HBuf<int> cg_hctl;
     HBuf<double> hbuf;
     HBuf<long long> hll;

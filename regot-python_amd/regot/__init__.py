@@ -20,6 +20,10 @@ Extensions (all optional):
 * ``ls_max_iter``: strong-Wolfe trials of sparse Newton and SPLR (default 20,
   the reference's fixed value).
 * ``device``: GPU ordinal; ``compute_plan``: form ``plan`` (default True).
+* ``hess_storage``: how the Newton-type solvers hold the sparsified Hessian:
+  -1 (default) matrix-free (products recomputed from M, FP32 bulk, m <= 16385)
+  when the kept set is dense, else CSR + CSC; 0 always CSR + CSC; 1
+  matrix-free wherever it applies.
 
 Every computation runs in libregot_hip.so on the GPU; there is no CPU path.
 """
@@ -394,6 +398,12 @@ class Problem:
                                                        _lib.ctypes.byref(nnz)))
         return nnz.value
 
+    def set_hess_storage(self, mode):
+        """-1: the solver drivers go matrix-free when the kept set is dense
+        (FP32 bulk, m - 1 <= 16384), sparsified_hess* store CSR + CSC; 0: always
+        stored; 1: matrix-free wherever it applies (hess_export then fails)."""
+        check(lib().regot_set_hess_storage(self._h, int(mode)))
+
     def hess_info(self):
         """(stored nnz, FP64-zero kept entries, Hessian density)"""
         a = _lib.ctypes.c_longlong()
@@ -658,12 +668,14 @@ def sinkhorn_bcd_batch(Ms, as_, bs, regs, tol=1e-6, max_iter=1000, bulk=None, de
         return B.solve_bcd(tol=tol, max_iter=max_iter)
 
 
-_HANDLE_KW = ("bulk", "device", "compute_plan", "verify")
+_HANDLE_KW = ("bulk", "device", "compute_plan", "verify", "hess_storage")
 
 
 def _run(solver, M, a, b, reg, tol, max_iter, verbose, kwargs):
     rest = {k: v for k, v in kwargs.items() if k not in _HANDLE_KW}
     with Problem(M, a, b, reg, bulk=kwargs.get("bulk"), device=kwargs.get("device", 0)) as p:
+        if "hess_storage" in kwargs:
+            p.set_hess_storage(kwargs["hess_storage"])
         return p.solve(solver, tol=tol, max_iter=max_iter, verbose=verbose,
                        compute_plan=kwargs.get("compute_plan", True),
                        verify=kwargs.get("verify", False), **rest)

@@ -133,6 +133,7 @@ def lib():
     L.regot_sparsified_hess.argtypes = [_vp, _dp, ctypes.c_double, ctypes.c_double, _llp]
     L.regot_sparsified_hess_with_density.argtypes = [_vp, _dp, ctypes.c_double, _llp]
     L.regot_hess_info.argtypes = [_vp, _llp, _llp, _dp]
+    L.regot_set_hess_storage.argtypes = [_vp, ctypes.c_int]
     L.regot_hess_export.argtypes = [_vp, _dp, _dp, _ip, _ip, _dp, _ip, _ip, _dp]
     L.regot_hess_apply_delta.argtypes = [_vp, _dp, ctypes.c_double, _dp]
     L.regot_hess_apply_hs.argtypes = [_vp, _dp, _dp]
