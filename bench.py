@@ -282,7 +282,7 @@ def c3_problem(n, seed=0):
     return M32, a / a.sum(), b / b.sum()
 
 
-def bench_newton(n, tol):
+def bench_newton(n, tol, cases_wanted=None):
     """The Newton-type drivers at n = m: SSNS on the BCD workload's recipe
     (reg 0.01), sparse Newton and SPLR on the mixture recipe (reg 0.001,
     density 1% and 5%); one warmup, one timed solve each, with per-phase
@@ -290,15 +290,22 @@ def bench_newton(n, tol):
     import regot
     from regot import _lib
     out = {}
-    M32, a, b = make_problem(n, n, 16, 0)
-    cases = [("ssns", M32, a, b, 0.01, _lib.SOLVER_SSNS, {})]
-    del M32
-    M3, a3, b3 = c3_problem(n)
+    cases = []
+    if not cases_wanted or "ssns" in cases_wanted:
+        M32, a, b = make_problem(n, n, 16, 0)
+        cases = [("ssns", M32, a, b, 0.01, _lib.SOLVER_SSNS, {})]
+        del M32
+    if cases_wanted and cases_wanted <= {"ssns"}:
+        M3 = a3 = b3 = None
+    else:
+        M3, a3, b3 = c3_problem(n)
     cases += [("sparse_newton", M3, a3, b3, 0.001, _lib.SOLVER_SPARSE_NEWTON, {"density": 0.01}),
               ("splr", M3, a3, b3, 0.001, _lib.SOLVER_SPLR, {"density": 0.01}),
               ("sparse_newton_d5", M3, a3, b3, 0.001, _lib.SOLVER_SPARSE_NEWTON, {"density": 0.05}),
               ("splr_d5", M3, a3, b3, 0.001, _lib.SOLVER_SPLR, {"density": 0.05})]
     for name, M, a_, b_, reg, sid, kw in cases:
+        if cases_wanted and name not in cases_wanted:
+            continue
         with regot.Problem(M, a_, b_, reg) as P:
             P.solve(sid, tol=tol, max_iter=1000, compute_plan=False, **kw)
             t0 = time.perf_counter()
@@ -441,6 +448,9 @@ def main():
     ap.add_argument("--diff-steps", type=int, default=2)
     ap.add_argument("--newton", type=int, default=1,
                     help="time SSNS, sparse Newton and SPLR at n = m = --n (0 = skip)")
+    ap.add_argument("--newton-cases", default="",
+                    help="with --only newton: comma-separated subset of ssns, sparse_newton, "
+                         "splr, sparse_newton_d5, splr_d5")
     ap.add_argument("--c2", type=int, default=1,
                     help="time sinkhorn_bcd and sinkhorn_ssns at n = m = 8192, reg 0.1 and 0.01")
     ap.add_argument("--first-order", type=int, default=1,
@@ -465,7 +475,8 @@ def main():
                                                    sequential=False)}))
         return
     if args.only == "newton":
-        print(json.dumps({"newton": bench_newton(args.n, args.tol)}))
+        wanted = set(args.newton_cases.split(",")) if args.newton_cases else None
+        print(json.dumps({"newton": bench_newton(args.n, args.tol, wanted)}))
         return
     if args.only == "c2":
         print(json.dumps({"c2": bench_c2(args.tol)}))

@@ -32,19 +32,26 @@ using namespace dev;
 
 constexpr int kMaxCols = 16384;          // m - 1 <= 16384
 
-template <int W>
-__device__ __forceinline__ double sum_waves(const double* v, int lane)
-{
-    double x = v[lane & (W - 1)];
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, W);
-    return __shfl(x, 0);
-}
-
 // T_ij of the FP32-bulk sparsifier (Elem<float>::val in sparse_kernels.hip).
 __device__ __forceinline__ float tval(double ab, float mij, double k)
 {
     return exp2f(static_cast<float>(fma(-k, static_cast<double>(mij), ab * k)));
+}
+
+// T_ij in the products: the same FP64 argument, the hardware exp2.  It equals
+// exp2f wherever T_ij >= 2^-126; below, where exp2f returns a subnormal, it
+// returns 0, which moves a product by less than 2^-126 max|x| per entry.
+__device__ __forceinline__ float tval_pass(double ab, float mij, double k)
+{
+    return __builtin_amdgcn_exp2f(static_cast<float>(fma(-k, static_cast<double>(mij), ab * k)));
+}
+
+// Butterfly sum over the wave; lane 0's value is the one used (fixed order).
+__device__ __forceinline__ double wave_total(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
 }
 
 template <int NC, bool FULL, int SPAN>
@@ -86,8 +93,15 @@ __global__ __launch_bounds__(512) void k_mf_mask(MfArgs a, unsigned int* __restr
             tau[c][q] = !ok(c, q) ? INFINITY : a.tau ? static_cast<float>(a.tau[j0 + q]) : 0.f;
         }
     }
+    // the row in flight and its alpha_i, rho_i load together (see k_mf_pass)
     float4 pf[NC];
-    if (r0 < r1) load_row<NC, FULL, kSpan>(a.M + r0 * a.ldm, t, vlast, pf, a.nt != 0);
+    double pa = 0.0, pr = 0.0;
+    auto prefetch = [&](long r) {
+        load_row<NC, FULL, kSpan>(a.M + r * a.ldm, t, vlast, pf, a.nt != 0);
+        pa = a.alpha[r];
+        if (a.rho) pr = a.rho[r];
+    };
+    if (r0 < r1) prefetch(r0);
 #pragma unroll 1
     for (long i = r0; i < r1; i++) {
         float e[NC][4];
@@ -95,9 +109,9 @@ __global__ __launch_bounds__(512) void k_mf_mask(MfArgs a, unsigned int* __restr
         for (int c = 0; c < NC; c++)
 #pragma unroll
             for (int q = 0; q < 4; q++) e[c][q] = vget<float4, float>(pf[c], q);
-        if (i + 1 < r1) load_row<NC, FULL, kSpan>(a.M + (i + 1) * a.ldm, t, vlast, pf, a.nt != 0);
-        const double ai = a.alpha[i];
-        const float ri = a.rho ? static_cast<float>(a.rho[i]) : 0.f;
+        const double ai = pa;
+        const float ri = static_cast<float>(pr);
+        if (i + 1 < r1) prefetch(i + 1);
         unsigned int bits = 0u;
 #pragma unroll
         for (int c = 0; c < NC; c++)
@@ -108,6 +122,242 @@ __global__ __launch_bounds__(512) void k_mf_mask(MfArgs a, unsigned int* __restr
                     if (tv >= fminf(tau[c][q], ri) && tv != 0.f) bits |= 1u << (4 * c + q);
                 }
         mask[i * 512 + t] = bits;
+    }
+}
+
+// Workgroup-wide exclusive scan of one double per thread (512 threads), in
+// thread order; *total gets the sum.  Two barriers.
+__device__ __forceinline__ double scan512(double v, double* wt, double* total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double u = __shfl_up(inc, o);
+        if (lane >= o) inc += u;
+    }
+    if (lane == 63) wt[wave] = inc;
+    __syncthreads();
+    double off = 0.0, tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+        if (w < wave) off += wt[w];
+        tot += wt[w];
+    }
+    *total = tot;
+    __syncthreads();
+    return off + inc - v;
+}
+
+// Descending bucket search over hist[0..nb) (nb <= 2048, 512 threads, four
+// buckets each): the largest b with above + sum(hist[b..nb-1]) >= thr;
+// *before = above + sum(hist[b+1..]).  -1 when none qualifies.
+__device__ int select_desc512(const double* hist, int nb, double above, double thr,
+                              double* before, double* wt, int* claim, double* claim_before)
+{
+    const int t = threadIdx.x;
+    if (t == 0) *claim = -1;
+    const int u = 511 - t;   // thread t owns buckets [4u, 4u + 4), top buckets first
+    double own = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (4 * u + q < nb) own += hist[4 * u + q];
+    double total;
+    double cum = above + scan512(own, wt, &total);
+    int found = -1;
+    double fb = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int b = 4 * u + 3 - q;
+        if (b >= nb) continue;
+        const double nxt = cum + hist[b];
+        if (nxt >= thr && found < 0) {
+            found = b;
+            fb = cum;
+        }
+        cum = nxt;
+    }
+    if (found >= 0) {
+        claim_before[u] = fb;
+        atomicMax(claim, found);
+    }
+    __syncthreads();
+    const int b = *claim;
+    int res = -1;
+    if (b >= 0) {
+        res = b;
+        *before = claim_before[b / 4];
+    }
+    __syncthreads();
+    return res;
+}
+
+// Row side of the threshold rule in the pass layout (FP32 bulk, m - 1 <=
+// 16384), one read of M: per row the dropped mass D_i = sum_j [T_ij < tau_j]
+// T_ij; rho_i = +inf when D_i <= 2 delta, else the radix select of
+// k_row_threshold (the largest dropped value v with sum_{d >= v} d >= D_i -
+// 2 delta: 11-, 11- and 10-bit digits, FP64 masses) over the row held in
+// registers; the stored count (T_ij >= min(tau_j, rho_i), T_ij != 0), the
+// FP64-zero count (exp2 argument < -1075) and, when mask is not null, the
+// kept bits.  The element arithmetic is Elem<float>::val's.
+template <int NC, bool FULL>
+__global__ __launch_bounds__(512) void k_mf_rule(MfArgs a, double delta2,
+                                                 double* __restrict__ rho, int* __restrict__ cnt,
+                                                 int* __restrict__ zcnt,
+                                                 unsigned int* __restrict__ mask)
+{
+    constexpr int kSpan = 2048;
+    constexpr int kWaves = 8;
+    // tau'_j = max(tau_j, 2^-149) (FP32 values; +inf past m - 1): T_ij < tau'_j
+    // is the dropped test and its negation the kept test (T_ij >= tau_j and
+    // T_ij != 0), since the only value below 2^-149 is 0
+    __shared__ float4 stau[NC * kSpan / 4];
+    __shared__ double hist[2048];
+    __shared__ double rd[2][kWaves];
+    __shared__ int rn[2][kWaves], rn2[kWaves];
+    __shared__ double wt[kWaves];
+    __shared__ double claim_before[512];
+    __shared__ int claim;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int m1 = a.m1;
+    const long r0 = static_cast<long>(blockIdx.x) * a.rpw;
+    const long r1 = (r0 + a.rpw < a.n) ? r0 + a.rpw : a.n;
+    int vlast = m1 - (4 * t + (NC - 1) * kSpan);
+    vlast = FULL ? 4 : (vlast < 0 ? 0 : (vlast > 4 ? 4 : vlast));
+    auto ok = [vlast](int c, int q) { return FULL || c < NC - 1 || q < vlast; };
+    double bj[NC][4];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int j0 = 4 * t + c * kSpan;
+        float tq[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            bj[c][q] = ok(c, q) ? a.beta[j0 + q] : 0.0;
+            tq[q] = ok(c, q) ? fmaxf(static_cast<float>(a.tau[j0 + q]), 0x1p-149f) : INFINITY;
+        }
+        stau[t + c * (kSpan / 4)] = make_float4(tq[0], tq[1], tq[2], tq[3]);
+    }
+    __syncthreads();
+    // tau of this thread's columns (c, q)
+    auto tau_of = [&](int c, int q) { return vget<float4, float>(stau[t + c * (kSpan / 4)], q); };
+    float4 pf[NC];
+    double pa = 0.0;
+    auto prefetch = [&](long r) {
+        load_row<NC, FULL, kSpan>(a.M + r * a.ldm, t, vlast, pf, a.nt != 0);
+        pa = a.alpha[r];
+    };
+    if (r0 < r1) prefetch(r0);
+#pragma unroll 1
+    for (long i = r0; i < r1; i++) {
+        float e[NC][4];
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) e[c][q] = vget<float4, float>(pf[c], q);
+        const double ai = pa;
+        if (i + 1 < r1) prefetch(i + 1);
+        // kept bits per element (the stored count is their popcount)
+        unsigned int bits = 0u;
+        int nz = 0;
+        double dsum = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                double x = fma(-a.k, static_cast<double>(e[c][q]), (ai + bj[c][q]) * a.k);
+                float tv = exp2f(static_cast<float>(x));
+                if (!ok(c, q)) {   // past m - 1 (last chunk only): no mass, not kept
+                    x = 0.0;
+                    tv = 0.f;
+                }
+                e[c][q] = tv;
+                const bool dropped = tv < tau_of(c, q);
+                dsum += static_cast<double>(dropped ? tv : 0.f);
+                bits |= dropped ? 0u : (1u << (4 * c + q));
+                nz += x < -1075.0 ? 1 : 0;
+            }
+        const int slot = static_cast<int>((i - r0) & 1);
+        {
+            double dw = dsum;
+            int cw = __popc(bits) | (nz << 16);   // both <= 2^14 per wave
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                dw += __shfl_xor(dw, o);
+                cw += __shfl_xor(cw, o);
+            }
+            if (lane == 0) {
+                rd[slot][wave] = dw;
+                rn[slot][wave] = cw;
+            }
+        }
+        __syncthreads();
+        double D = 0.0;
+        int NS = 0, NZ = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) {
+            D += rd[slot][w];
+            NS += rn[slot][w] & 0xffff;
+            NZ += rn[slot][w] >> 16;
+        }
+        double r_i = INFINITY;
+        if (D > delta2) {   // uniform: D is the same in every lane
+            const double target = D - delta2;
+            unsigned int prefix = 0u;
+            double above = 0.0;
+#pragma unroll 1
+            for (int p = 0; p < 3; p++) {
+                const int shift = p == 0 ? 21 : (p == 1 ? 10 : 0);
+                const int width = p == 2 ? 10 : 11;
+                for (int q = t; q < 2048; q += 512) hist[q] = 0.0;
+                __syncthreads();
+#pragma unroll
+                for (int c = 0; c < NC; c++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        if (!ok(c, q)) continue;
+                        const float d = e[c][q] < tau_of(c, q) ? e[c][q] : 0.f;
+                        const unsigned int key = __float_as_uint(d);
+                        if (p == 0 || (key >> (shift + width)) == prefix)
+                            atomicAdd(&hist[(key >> shift) & ((1u << width) - 1u)],
+                                      static_cast<double>(d));
+                    }
+                __syncthreads();
+                double before = 0.0;
+                const int b = select_desc512(hist, 1 << width, above, target, &before, wt,
+                                             &claim, claim_before);
+                if (b < 0) break;   // cannot happen: the dropped mass exceeds target
+                above = before;
+                prefix = (prefix << width) | static_cast<unsigned int>(b);
+            }
+            r_i = static_cast<double>(__uint_as_float(prefix));
+            const float rf = __uint_as_float(prefix);
+            bits = 0u;
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (!ok(c, q)) continue;
+                    const bool kept = e[c][q] >= fminf(tau_of(c, q), rf) && e[c][q] != 0.f;
+                    bits |= kept ? (1u << (4 * c + q)) : 0u;
+                }
+            int nw = __popc(bits);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) nw += __shfl_xor(nw, o);
+            if (lane == 0) rn2[wave] = nw;
+            __syncthreads();
+            NS = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; w++) NS += rn2[w];
+            __syncthreads();
+        }
+        if (t == 0) {
+            rho[i] = r_i;
+            cnt[i] = NS;
+            zcnt[i] = NZ;
+        }
+        if (mask != nullptr) mask[i * 512 + t] = bits;
     }
 }
 
@@ -173,12 +423,23 @@ __global__ __launch_bounds__(TH) void k_mf_pass(MfArgs a)
     if (ROW) __syncthreads();
     const unsigned int* mk = a.mask + t;
 
+    // One row in flight: its M values, kept bits, alpha_i and the row input
+    // of the column half (h1_i or xr_i) are loaded together while the row
+    // before is computed.  (Every per-row operand goes through this group: a
+    // load of its own inside the loop would wait for the whole prefetch.)
+    constexpr bool WH1 = MODE == MF_CG || MODE == MF_DIAG;
+    constexpr bool WXR = MODE == MF_HS || MODE == MF_COLS;
     float4 pf[NC];
     unsigned int pbits = 0u;
-    if (r0 < r1) {
-        load_row<NC, FULL, kSpan>(a.M + static_cast<long>(r0) * a.ldm, t, vlast, pf, a.nt != 0);
-        pbits = mk[static_cast<long>(r0) * kThreads];
-    }
+    double pa = 0.0, pw = 0.0;
+    auto prefetch = [&](int r) {
+        load_row<NC, FULL, kSpan>(a.M + static_cast<long>(r) * a.ldm, t, vlast, pf, a.nt != 0);
+        pbits = mk[static_cast<long>(r) * kThreads];
+        pa = a.alpha[r];
+        if (WH1) pw = a.h1[r];
+        if (WXR) pw = a.xr[r];
+    };
+    if (r0 < r1) prefetch(r0);
 #pragma unroll 1
     for (int i = r0; i < r1; i++) {
         float e[NC][4];
@@ -187,12 +448,9 @@ __global__ __launch_bounds__(TH) void k_mf_pass(MfArgs a)
 #pragma unroll
             for (int q = 0; q < 4; q++) e[c][q] = vget<float4, float>(pf[c], q);
         const unsigned int bits = pbits;
-        if (i + 1 < r1) {
-            load_row<NC, FULL, kSpan>(a.M + static_cast<long>(i + 1) * a.ldm, t, vlast, pf,
-                                      a.nt != 0);
-            pbits = mk[static_cast<long>(i + 1) * kThreads];
-        }
-        const double ai = a.alpha[i];
+        const double ai = pa;
+        const double wi = pw;
+        if (i + 1 < r1) prefetch(i + 1);
         double racc = 0.0;
         // T is formed for every element and then masked: a branch per element
         // keeps each exp's operands live across it and spills at 8 chunks
@@ -201,7 +459,7 @@ __global__ __launch_bounds__(TH) void k_mf_pass(MfArgs a)
         for (int c = 0; c < NC; c++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const float tx = tval(ai + bj[c][q], e[c][q], a.k);
+                const float tx = tval_pass(ai + bj[c][q], e[c][q], a.k);
                 const float tv = ((bits >> (4 * c + q)) & 1u) ? tx : 0.f;
                 e[c][q] = tv;
                 if (ROW && ok(c, q))
@@ -216,20 +474,24 @@ __global__ __launch_bounds__(TH) void k_mf_pass(MfArgs a)
         double cw = 0.0;
         if (ROW) {
             const int slot = (i - r0) & 1;
-            const double S = wave_sum0(racc);
+            const double S = wave_total(racc);
             if (lane == 0) red[slot][wave] = S;
             __syncthreads();
-            const double rv = sum_waves<kWaves>(red[slot], lane) * a.inv_reg;   // (sigma xc)_i
+            // every lane sums the wave totals in the same order
+            double rv = 0.0;
+#pragma unroll
+            for (int w = 0; w < kWaves; w++) rv += red[slot][w];
+            rv *= a.inv_reg;   // (sigma xc)_i
             if (MODE == MF_CG) {
-                cw = rv / (a.h1[i] + a.shift) * a.inv_reg;
+                cw = rv / (wi + a.shift) * a.inv_reg;
             } else {
                 if (t == 0) a.rowout[i] = rv;
-                cw = (MODE == MF_HS) ? a.xr[i] * a.inv_reg : 0.0;
+                cw = (MODE == MF_HS) ? wi * a.inv_reg : 0.0;
             }
         } else if (MODE == MF_COLS) {
-            cw = a.xr[i] * a.inv_reg;
+            cw = wi * a.inv_reg;
         } else {   // MF_DIAG: sum_i sigma_ij^2 / (h1_i + s)
-            cw = a.inv_reg * a.inv_reg / (a.h1[i] + a.shift);
+            cw = a.inv_reg * a.inv_reg / (wi + a.shift);
         }
         if (COL) {
 #pragma unroll
@@ -349,6 +611,39 @@ void launch_mf_mask(MfArgs a, int nwg, unsigned int* mask, hipStream_t s)
     case 6: mask_launch<6>(a, nwg, full, mask, s); break;
     case 7: mask_launch<7>(a, nwg, full, mask, s); break;
     default: mask_launch<8>(a, nwg, full, mask, s); break;
+    }
+    KERNEL_CHECK();
+}
+
+template <int NC>
+void rule_launch(const MfArgs& a, int nwg, bool full, double delta2, double* rho, int* cnt,
+                 int* zcnt, unsigned int* mask, hipStream_t s)
+{
+    if (full)
+        hipLaunchKernelGGL((k_mf_rule<NC, true>), dim3(nwg), dim3(512), 0, s, a, delta2, rho, cnt,
+                           zcnt, mask);
+    else
+        hipLaunchKernelGGL((k_mf_rule<NC, false>), dim3(nwg), dim3(512), 0, s, a, delta2, rho,
+                           cnt, zcnt, mask);
+}
+
+void launch_mf_rule(MfArgs a, int nwg, double delta2, double* rho, int* cnt, int* zcnt,
+                    unsigned int* mask, hipStream_t s)
+{
+    if (!mf_shape_ok(a.n, a.m1 + 1)) fail(E_INTERNAL, "threshold rows: %d x %d", a.n, a.m1 + 1);
+    a.rpw = ceil_div(a.n, nwg);
+    a.nt = 0;
+    const int NC = ceil_div(a.m1, 2048);
+    const bool full = a.m1 == NC * 2048;
+    switch (NC) {
+    case 1: rule_launch<1>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
+    case 2: rule_launch<2>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
+    case 3: rule_launch<3>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
+    case 4: rule_launch<4>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
+    case 5: rule_launch<5>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
+    case 6: rule_launch<6>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
+    case 7: rule_launch<7>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
+    default: rule_launch<8>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
     }
     KERNEL_CHECK();
 }

@@ -48,6 +48,11 @@ int mf_blocks(int n, int ncu);
 // Kept-set bitmask of the threshold rule (tau, rho; both null: every nonzero
 // entry) into mask (n x 512 words).
 void launch_mf_mask(MfArgs a, int nwg, unsigned int* mask, hipStream_t s);
+// Row side of the threshold rule from tau (a.tau) in one read of M: rho,
+// stored and FP64-zero counts per row (k_row_threshold's outputs) and, when
+// mask is not null, the kept-set bitmask.
+void launch_mf_rule(MfArgs a, int nwg, double delta2, double* rho, int* cnt, int* zcnt,
+                    unsigned int* mask, hipStream_t s);
 void launch_mf_pass(int mode, MfArgs a, int nwg, hipStream_t s);
 // out (m1) from the nwg x m1 partials: epi 0 sum; 1 z - sum; 2 (h2 + shift) z -
 // sum; 3 h2 z + sum; 4 the Jacobi inverse diagonal 1 / (h2 + shift - sum).

@@ -415,16 +415,20 @@ static bool mf_allowed(const regot_problem* p, bool driver)
     return p->hess_storage == 1 || driver;
 }
 
-// Matrix-free build: the kept-set bitmask of the threshold rule (tau, rho)
-// or, with both null, every entry kept (the full Hessian).
-static void build_mf(regot_problem* p, SparseHessian& H, const double* tau, const double* rho)
+// Buffers of a matrix-free build: the kept-set bitmask (n x 512 words) and
+// the column partials of the passes.
+static void mf_buffers(regot_problem* p, SparseHessian& H)
 {
     H.mf_nwg = mf_blocks(p->n, p->ncu);
     H.mf_mask.ensure(static_cast<size_t>(p->n) * 512);
     H.mf_part.ensure(static_cast<size_t>(H.mf_nwg) * std::max(p->m - 1, 1));
+}
+
+// Matrix-free build with every nonzero entry kept (the full Hessian).
+static void build_mf_full(regot_problem* p, SparseHessian& H)
+{
+    mf_buffers(p, H);
     MfArgs a = mf_args(p, H);
-    a.tau = tau;
-    a.rho = rho;
     launch_mf_mask(a, H.mf_nwg, H.mf_mask, p->stream);
     H.mf = true;
 }
@@ -443,16 +447,19 @@ static void build_storage(regot_problem* p, SparseHessian& H, const ElemCtx& c,
     H.rowind.ensure(nnz);
     H.cval.ensure(nnz);
     launch_row_write(p->bulk, p->dM, p->ldm, c, rule, H.rowptr, H.colind, H.rval, s);
-    // CSC: the transpose of the CSR just written.  Below 2% density a
-    // counting sort of the nnz entries; above, a pass over M' by the ordered
-    // row writer when M' is there (threshold rule), else over M by column
-    // strips (the sort's scattered writes cost more than re-reading M there).
-    // At most 512 row blocks: the offset table stays <= 2 KiB per column.
+    // CSC: the transpose of the CSR just written.  Sparse sets by a counting
+    // sort of the nnz entries; dense ones by a pass over M' with the ordered
+    // row writer when M' is there (threshold rule, FP64 bulk), else over M by
+    // column strips.  At most 512 row blocks: the offset table stays <= 2 KiB
+    // per column.
     const int rows_per_block = std::max(32, ceil_div(n, 512));
     H.ccnt.ensure(m);
     H.colptr.ensure(m);
     H.csc_work.ensure(static_cast<size_t>(ceil_div(n, rows_per_block)) * std::max(m - 1, 1));
-    const bool dense = static_cast<double>(nnz) > 0.02 * n * static_cast<double>(m - 1);
+    // Without M' (FP32 bulk) the strip pass recomputes T column-wise and
+    // costs more than the sort up to the matrix-free densities (1/12).
+    const double sort_max = p->dMT ? 0.02 : 0.09;
+    const bool dense = static_cast<double>(nnz) > sort_max * n * static_cast<double>(m - 1);
     const int how = !dense ? 0 : (rule.mode == 0 && p->dMT) ? 2 : 1;
     launch_build_csc(p->bulk, p->dM, p->ldm, p->dMT, p->ldt, c, rule, H, how, H.csc_work,
                      rows_per_block, H.ccnt, H.colptr, H.ltmp.p + 1, H.rowind, H.cval, s);
@@ -484,22 +491,44 @@ void hess_threshold(regot_problem* p, const double* d_gamma, const double* d_gra
     hipStream_t s = p->stream;
     H.tau.ensure(p->m);
     H.rho.ensure(p->n);
-    hess_ensure_transpose(p);
-    launch_col_threshold(p->bulk, p->dMT, p->ldt, c, delta, density_hint > 0.2 ? 1 : 0, H.tau, s);
-    launch_row_threshold(p->bulk, p->dM, p->ldm, c, H.tau, 2.0 * delta, H.rho, H.rcnt, H.zcnt, s);
+    const int find_small = density_hint > 0.2 ? 1 : 0;
+    const bool may_mf = mf_allowed(p, driver);
+    if (p->bulk == 0) {
+        // FP32 bulk: tau from row-major M (no M'), then the row side in one
+        // read of M, with the kept-set bitmask when the build may go
+        // matrix-free
+        threshold_tau_rows(p->dM, p->ldm, c, delta, find_small, H, H.tau, s);
+        if (mf_shape_ok(p->n, p->m)) {
+            if (may_mf) mf_buffers(p, H);
+            MfArgs a = mf_args(p, H);
+            a.tau = H.tau;
+            launch_mf_rule(a, mf_blocks(p->n, p->ncu), 2.0 * delta, H.rho, H.rcnt, H.zcnt,
+                           may_mf ? H.mf_mask.p : nullptr, s);
+        } else {
+            launch_row_threshold(p->bulk, p->dM, p->ldm, c, H.tau, 2.0 * delta, H.rho, H.rcnt,
+                                 H.zcnt, s);
+        }
+    } else {
+        hess_ensure_transpose(p);
+        launch_col_threshold(p->bulk, p->dMT, p->ldt, c, delta, find_small, H.tau, s);
+        launch_row_threshold(p->bulk, p->dM, p->ldm, c, H.tau, 2.0 * delta, H.rho, H.rcnt,
+                             H.zcnt, s);
+    }
     KeepRule rule;
     rule.mode = 0;
     rule.tau = H.tau;
     rule.rho = H.rho;
     bool mf = false;
-    if (mf_allowed(p, driver)) {
+    if (may_mf) {
         // matrix-free once one product over the stored copies (2 x 12 bytes
-        // per entry) would move more than half a read of M (4 bytes per entry)
+        // per entry) would move more than half a read of M (4 bytes per
+        // entry): the stored builds' CSR + CSC writes and the CG products'
+        // gathers make them dearer per byte than the pass over M
         const long long nnz = count_stored(p, H);
         mf = p->hess_storage == 1 || 24.0 * static_cast<double>(nnz) >
                                          2.0 * static_cast<double>(p->n) * (p->m - 1);
         if (mf)
-            build_mf(p, H, H.tau, H.rho);
+            H.mf = true;
         else
             build_storage(p, H, c, rule, true);
     } else {
@@ -527,7 +556,7 @@ void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, do
     const long long kk = static_cast<long long>(static_cast<double>(len) * density);
     if (kk >= len && mf_allowed(p, driver)) {
         // every nonzero entry kept: products straight from M
-        build_mf(p, H, nullptr, nullptr);
+        build_mf_full(p, H);
         H.nnz = len;
         H.nnz_zero = 0;
         return;

@@ -51,6 +51,12 @@ struct SparseHessian {
     DBuf<long long> ltmp;
     DBuf<unsigned long long> hist;   // global count histograms of the top-k radix passes
     DBuf<double> vn, vm, vm2;        // alpha, beta (full) of the current build; scratch
+    // tau from row-major M (FP32 bulk): level-1 partials, boundary buckets,
+    // candidate offsets and the candidates
+    DBuf<double> tau_pm, tau_base;
+    DBuf<unsigned int> tau_pc, tau_px;
+    DBuf<int> tau_bkt, tau_cnt, tau_off;
+    DBuf<float> tau_cand;
     DBuf<double> cg_r, cg_p, cg_q, cg_s, cg_u, cg_y_buf, cg_part, cg_scal, cg_dinv;
     DBuf<int> cg_ctl;
     // Matrix-free build (threshold rule, FP32 bulk): no CSR / CSC; the kept
@@ -79,6 +85,12 @@ void launch_transpose(int bulk, const void* M, long ldm, int n, int m, void* MT,
 // nothing (tau = +inf); otherwise they keep their largest entry.
 void launch_col_threshold(int bulk, const void* MT, long ldt, const ElemCtx& c, double delta,
                           int find_small, double* tau, hipStream_t s);
+// Threshold rule, step 1 for FP32 bulk from row-major M (no M'): the same
+// tau_j by a histogram pass over 64 exponent buckets per column, a gather of
+// each column's boundary bucket and radix passes over those candidates.
+// Synchronises the stream once (the candidate count).
+void threshold_tau_rows(const void* M, long ldm, const ElemCtx& c, double delta, int find_small,
+                        SparseHessian& H, double* tau, hipStream_t s);
 // Threshold rule, step 2: rho_i per row (dropped-mass correction to 2 delta),
 // stored-entry count per row and FP64-zero kept count per row.
 void launch_row_threshold(int bulk, const void* M, long ldm, const ElemCtx& c,

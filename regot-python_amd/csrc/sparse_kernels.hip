@@ -7,6 +7,7 @@
 // Compaction is count -> exclusive scan -> ordered write; the write position
 // of each kept entry comes from wave ballots, so no global atomics decide
 // placement and the output is identical run to run.
+#include <algorithm>
 #include <cmath>
 
 #include "sparse.hpp"
@@ -338,6 +339,249 @@ __global__ __launch_bounds__(256) void k_row_threshold(const TM* __restrict__ M,
         rho[i] = r_i;
         cnt[i] = static_cast<int>(ns);
         zcnt[i] = static_cast<int>(nz);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Threshold rule, tau_j from row-major M (FP32 bulk; no transpose)
+//
+// tau_j is the smallest key K of column j with sum_{key <= K} T_ij > delta.
+// Level 1 (k_tau_hist): per column, FP64 mass and count of T over 64 buckets
+// of the key's top seven bits (T >= 0: the sign and six exponent bits, four
+// binades per bucket) and the largest key, by strips of 64 columns (lane =
+// column) over row blocks.  k_tau_fold sums the row blocks in a fixed order
+// and finds each column's boundary bucket (or settles tau_j when the column's
+// mass never passes delta); k_tau_gather collects the entries of the boundary
+// bucket; k_tau_select ends the selection on those candidates with the
+// radix passes of k_col_threshold over the remaining 25 key bits.
+// ---------------------------------------------------------------------------
+
+constexpr int TB = 64;    // level-1 buckets (key >> 25)
+constexpr int TC = 64;    // columns per strip workgroup
+constexpr int TU = 16;    // rows per wave step (loads in flight)
+
+__global__ __launch_bounds__(256) void k_tau_hist(const float* __restrict__ M, long ldm, ElemCtx c,
+                                                  int rows_per_blk, double* __restrict__ pmass,
+                                                  unsigned int* __restrict__ pcnt,
+                                                  unsigned int* __restrict__ pmax)
+{
+    constexpr int P = TC + 1;   // padded rows: the bucket-fastest copy-out is conflict-free
+    __shared__ double hm[TB * P];
+    __shared__ unsigned int hc[TB * P];
+    __shared__ unsigned int hx[TC];
+    // rows are interleaved over the four waves (wave w takes i0 + w + 4u);
+    // alpha_i comes through vector loads (a scalar load would share its wait
+    // counter with the LDS atomics)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m1 = c.m - 1;
+    const int j = blockIdx.x * TC + lane;
+    for (int q = threadIdx.x; q < TB * P; q += 256) {
+        hm[q] = 0.0;
+        hc[q] = 0u;
+    }
+    if (threadIdx.x < TC) hx[threadIdx.x] = 0u;
+    __syncthreads();
+    const bool ok = j < m1;
+    const double bj = ok ? c.beta[j] : 0.0;
+    const long i0 = static_cast<long>(blockIdx.y) * rows_per_blk;
+    const long i1 = (i0 + rows_per_blk < c.n) ? i0 + rows_per_blk : c.n;
+    unsigned int kmax = 0u;
+    if (ok) {
+        const float* Mj = M + j;
+        for (long ib = i0 + wave; ib < i1; ib += 4 * TU) {
+            float mv[TU];
+            double av[TU];
+#pragma unroll
+            for (int u = 0; u < TU; u++) {
+                const long i = ib + 4L * u;
+                mv[u] = i < i1 ? Mj[i * ldm] : 0.f;
+                av[u] = i < i1 ? c.alpha[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < TU; u++) {
+                if (ib + 4L * u >= i1) break;
+                bool z;
+                const float t = Elem<float>::val(av[u] + bj, mv[u], c.k, c.reg, z);
+                const unsigned int key = __float_as_uint(t);
+                const int b = static_cast<int>(key >> 25) & (TB - 1);
+                atomicAdd(&hm[b * P + lane], static_cast<double>(t));
+                atomicAdd(&hc[b * P + lane], 1u);
+                kmax = key > kmax ? key : kmax;
+            }
+        }
+        atomicMax(&hx[lane], kmax);
+    }
+    __syncthreads();
+    // partials [row block][column][bucket]
+    const long rb = blockIdx.y;
+    for (int q = threadIdx.x; q < TB * TC; q += 256) {
+        const int b = q % TB, l = q / TB;
+        const int jj = blockIdx.x * TC + l;
+        if (jj < m1) {
+            pmass[(rb * m1 + jj) * TB + b] = hm[b * P + l];
+            pcnt[(rb * m1 + jj) * TB + b] = hc[b * P + l];
+        }
+    }
+    if (threadIdx.x < TC && blockIdx.x * TC + static_cast<int>(threadIdx.x) < m1)
+        pmax[rb * m1 + blockIdx.x * TC + threadIdx.x] = hx[threadIdx.x];
+}
+
+// Per column (one wave, lane = bucket): the boundary bucket (smallest b with
+// sum_{b' <= b} mass_b' > delta), the mass below it and its entry count; or,
+// when the column's whole mass stays <= delta, tau_j itself (+inf with
+// find_small, else the largest entry).  The row blocks are summed in order,
+// the buckets by a fixed-order wave scan.
+__global__ __launch_bounds__(256) void k_tau_fold(const double* __restrict__ pmass,
+                                                  const unsigned int* __restrict__ pcnt,
+                                                  const unsigned int* __restrict__ pmax, int nrb,
+                                                  int m1, double delta, int find_small,
+                                                  int* __restrict__ bucket,
+                                                  double* __restrict__ base,
+                                                  int* __restrict__ count,
+                                                  double* __restrict__ tau)
+{
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= m1) return;
+    double mb = 0.0;
+    unsigned int cb = 0u, mx = 0u;
+    for (int r = 0; r < nrb; r++) {
+        const long o = (static_cast<long>(r) * m1 + j) * TB + lane;
+        mb += pmass[o];
+        cb += pcnt[o];
+        const unsigned int v = pmax[static_cast<long>(r) * m1 + j];
+        mx = v > mx ? v : mx;
+    }
+    const double inc = wave_incl_scan(mb);
+    const unsigned long long hit = __ballot(inc > delta);
+    if (hit == 0ull) {
+        if (lane == 0) {
+            bucket[j] = -1;
+            count[j] = 0;
+            tau[j] = find_small ? INFINITY : static_cast<double>(__uint_as_float(mx));
+        }
+        return;
+    }
+    const int B = __ffsll(static_cast<long long>(hit)) - 1;
+    if (lane == B) {
+        bucket[j] = B;
+        base[j] = inc - mb;
+        count[j] = static_cast<int>(cb);
+    }
+}
+
+// The entries of each column's boundary bucket, into cand[off_j ..): row
+// block rb writes after the blocks before it (their level-1 counts of the
+// bucket), the four waves of a workgroup claim slots by LDS counters.
+__global__ __launch_bounds__(256) void k_tau_gather(const float* __restrict__ M, long ldm,
+                                                    ElemCtx c, int rows_per_blk,
+                                                    const int* __restrict__ bucket,
+                                                    const int* __restrict__ off,
+                                                    const unsigned int* __restrict__ pcnt,
+                                                    float* __restrict__ cand)
+{
+    __shared__ int any;
+    __shared__ int lfill[TC];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m1 = c.m - 1;
+    const int j = blockIdx.x * TC + lane;
+    const int B = j < m1 ? bucket[j] : -1;
+    if (threadIdx.x == 0) any = 0;
+    if (threadIdx.x < TC) lfill[threadIdx.x] = 0;
+    __syncthreads();
+    if (B >= 0) any = 1;
+    __syncthreads();
+    if (!any || B < 0) return;
+    const long rb = blockIdx.y;
+    int o = off[j];
+    for (long r = 0; r < rb; r++) o += static_cast<int>(pcnt[(r * m1 + j) * TB + B]);
+    const int cap = static_cast<int>(pcnt[(rb * m1 + j) * TB + B]);
+    if (cap == 0) return;
+    const double bj = c.beta[j];
+    const long i0 = rb * rows_per_blk;
+    const long i1 = (i0 + rows_per_blk < c.n) ? i0 + rows_per_blk : c.n;
+    const float* Mj = M + j;
+    for (long ib = i0 + wave; ib < i1; ib += 4 * TU) {
+        float mv[TU];
+        double av[TU];
+#pragma unroll
+        for (int u = 0; u < TU; u++) {
+            const long i = ib + 4L * u;
+            mv[u] = i < i1 ? Mj[i * ldm] : 0.f;
+            av[u] = i < i1 ? c.alpha[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < TU; u++) {
+            if (ib + 4L * u >= i1) break;
+            bool z;
+            const float t = Elem<float>::val(av[u] + bj, mv[u], c.k, c.reg, z);
+            if ((static_cast<int>(__float_as_uint(t) >> 25) & (TB - 1)) == B) {
+                const int pos = atomicAdd(&lfill[lane], 1);
+                if (pos < cap) cand[o + pos] = t;
+            }
+        }
+    }
+}
+
+// Radix passes of k_col_threshold over the candidates of a column's boundary
+// bucket (its key bits 24..0: 9, 8 and 8 bits), from the bucket's base mass.
+// If rounding in the re-summed masses leaves no bucket passing delta, the
+// largest candidate key is taken (the entry whose mass makes the sum pass).
+__global__ __launch_bounds__(256) void k_tau_select(const int* __restrict__ bucket,
+                                                    const double* __restrict__ base_in,
+                                                    const int* __restrict__ count,
+                                                    const int* __restrict__ off,
+                                                    const float* __restrict__ cand, double delta,
+                                                    double* __restrict__ tau)
+{
+    __shared__ double hist[512];
+    __shared__ unsigned int kmax_s[4];
+    const int j = blockIdx.x;
+    const int B = bucket[j];
+    if (B < 0) return;
+    const float* cj = cand + off[j];
+    const int cn = count[j];
+    unsigned int prefix = static_cast<unsigned int>(B);
+    double base = base_in[j];
+    const int shifts[3] = {16, 8, 0};
+    const int widths[3] = {9, 8, 8};
+    bool none = false;
+    unsigned int kmax = 0u;
+    for (int p = 0; p < 3; p++) {
+        const int shift = shifts[p], width = widths[p];
+        for (int q = threadIdx.x; q < 512; q += 256) hist[q] = 0.0;
+        __syncthreads();
+        for (int e = threadIdx.x; e < cn; e += 256) {
+            const float t = cj[e];
+            const unsigned int key = __float_as_uint(t) & 0x7fffffffu;
+            if (p == 0) kmax = key > kmax ? key : kmax;
+            if ((key >> (shift + width)) == prefix)
+                atomicAdd(&hist[(key >> shift) & ((1u << width) - 1u)], static_cast<double>(t));
+        }
+        __syncthreads();
+        double before = 0.0;
+        const int b = select_bucket<true>(hist, 1 << width, base, delta, &before);
+        if (b < 0) {
+            none = true;
+            break;
+        }
+        base = before;
+        prefix = (prefix << width) | static_cast<unsigned int>(b);
+    }
+    if (none) {
+        unsigned int v = kmax;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned int w = static_cast<unsigned int>(__shfl_xor(static_cast<int>(v), o));
+            v = w > v ? w : v;
+        }
+        if ((threadIdx.x & 63) == 0) kmax_s[threadIdx.x >> 6] = v;
+        __syncthreads();
+        unsigned int mx = kmax_s[0];
+        for (int w = 1; w < 4; w++) mx = kmax_s[w] > mx ? kmax_s[w] : mx;
+        if (threadIdx.x == 0) tau[j] = static_cast<double>(__uint_as_float(mx));
+    } else if (threadIdx.x == 0) {
+        tau[j] = static_cast<double>(__uint_as_float(prefix));
     }
 }
 
@@ -966,6 +1210,50 @@ void launch_col_threshold(int bulk, const void* MT, long ldt, const ElemCtx& c, 
         hipLaunchKernelGGL(k_col_threshold<double>, dim3(c.m - 1), dim3(256), 0, s,
                            cM<double>(MT), ldt, c, delta, find_small, tau);
     KERNEL_CHECK();
+}
+
+void threshold_tau_rows(const void* M, long ldm, const ElemCtx& c, double delta, int find_small,
+                        SparseHessian& H, double* tau, hipStream_t s)
+{
+    const int m1 = c.m - 1, n = c.n;
+    if (m1 <= 0) return;
+    const int strips = ceil_div(m1, TC);
+    // about 2048 workgroups, row blocks of at least 256 rows
+    int nrb = std::max(1, std::min(ceil_div(2048, strips), ceil_div(n, 256)));
+    const int rpb = ceil_div(n, nrb);
+    nrb = ceil_div(n, rpb);
+    const size_t ph = static_cast<size_t>(nrb) * TB * m1;
+    H.tau_pm.ensure(ph);
+    H.tau_pc.ensure(ph);
+    H.tau_px.ensure(static_cast<size_t>(nrb) * m1);
+    H.tau_bkt.ensure(m1);
+    H.tau_base.ensure(m1);
+    H.tau_cnt.ensure(m1 + 1);
+    H.tau_off.ensure(m1 + 1);
+    H.ltmp.ensure(4);
+    H.hll.ensure(4);
+    const float* Mf = static_cast<const float*>(M);
+    hipLaunchKernelGGL(k_tau_hist, dim3(strips, nrb), dim3(256), 0, s, Mf, ldm, c, rpb,
+                       H.tau_pm.p, H.tau_pc.p, H.tau_px.p);
+    hipLaunchKernelGGL(k_tau_fold, dim3(ceil_div(m1, 4)), dim3(256), 0, s, H.tau_pm.p,
+                       H.tau_pc.p, H.tau_px.p, nrb, m1, delta, find_small, H.tau_bkt.p,
+                       H.tau_base.p, H.tau_cnt.p, tau);
+    KERNEL_CHECK();
+    launch_scan(H.tau_cnt, m1, H.tau_off, H.ltmp.p + 3, s);
+    HIP_CHECK(hipMemcpyAsync(H.hll.p + 3, H.ltmp.p + 3, sizeof(long long), hipMemcpyDeviceToHost,
+                             s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    const long long total = H.hll.p[3];
+    if (total > 0x7fffffffLL)
+        fail(E_INVALID, "threshold rule: %lld boundary candidates (> 2^31)", total);
+    if (total > 0) {
+        H.tau_cand.ensure(static_cast<size_t>(total));
+        hipLaunchKernelGGL(k_tau_gather, dim3(strips, nrb), dim3(256), 0, s, Mf, ldm, c, rpb,
+                           H.tau_bkt.p, H.tau_off.p, H.tau_pc.p, H.tau_cand.p);
+        hipLaunchKernelGGL(k_tau_select, dim3(m1), dim3(256), 0, s, H.tau_bkt.p, H.tau_base.p,
+                           H.tau_cnt.p, H.tau_off.p, H.tau_cand.p, delta, tau);
+        KERNEL_CHECK();
+    }
 }
 
 void launch_row_threshold(int bulk, const void* M, long ldm, const ElemCtx& c,

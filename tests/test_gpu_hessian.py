@@ -90,6 +90,61 @@ def test_threshold_rule_fp32_invariants(gpu_lib):
     assert np.max(np.abs(S[kept] - T[:, :-1][kept]) / T[:, :-1][kept]) < 2e-6
 
 
+def column_tau(S, delta):
+    """Per column the smallest value v with sum_{S_ij <= v} S_ij > delta
+    (+inf when the column's mass never passes delta)."""
+    St = np.sort(S, axis=0)
+    C = np.cumsum(St, axis=0)
+    hit = C > delta
+    idx = hit.argmax(axis=0)
+    return np.where(hit.any(axis=0), St[idx, np.arange(S.shape[1])], np.inf)
+
+
+@pytest.mark.parametrize("n,m,side", [(300, 257, "le"), (200, 2100, "gt"), (64, 12001, "gt"),
+                                      (40, 16500, "gt")])
+@pytest.mark.parametrize("hint", [0.001, 0.5])
+def test_threshold_rule_fp32_exact_on_device_values(gpu_lib, n, m, side, hint):
+    # FP32 bulk builds tau from row-major M and the row side in one pass (m -
+    # 1 <= 16384; wider: the per-row kernel).  The rule is checked exactly on
+    # the device's own T: every nonzero sigma_ij = T_ij / reg comes from the
+    # full Hessian (top-k at density 1), and the oracle's threshold rule runs
+    # on that matrix with delta / reg.
+    reg = 0.02
+    M, a, b = cloud_problem(n, m, seed=n + m)
+    M32 = M.astype(np.float32)
+    gamma = point(M32.astype(np.float64), a, b, reg, iters=2, seed=n)
+    print(f"fp32 threshold {n}x{m} hint {hint}", flush=True)
+    with gpu_lib.Problem(M32, a, b, reg) as P:
+        _, g = P.dual_obj_grad(gamma)
+        delta = 0.01 * np.linalg.norm(g)
+        P.sparsified_hess_with_density(gamma, 1.0)
+        F = P.hess_export()
+        Sf = np.zeros((n, m - 1))
+        for i in range(n):
+            sl = slice(F["rowptr"][i], F["rowptr"][i + 1])
+            Sf[i, F["colind"][sl]] = F["rval"][sl]
+        nnz = P.sparsified_hess(gamma, delta, hint)
+        stored, zeros, _ = P.hess_info()
+        H = P.hess_export()
+    Sg = csc_dense(n, m, H["colptr"], H["rowind"], H["cval"])
+    Sp = np.concatenate([Sf, np.zeros((n, 1))], axis=1)
+    cp, ri, va = O.sparsify_threshold(Sp, delta / reg, hint)
+    kept_o = np.zeros((n, m - 1), bool)
+    for j in range(m - 1):
+        kept_o[ri[cp[j]:cp[j + 1]], j] = True
+    kept_o &= Sf != 0
+    assert np.array_equal(Sg != 0, kept_o)
+    assert np.array_equal(Sg[kept_o], Sf[kept_o])   # same values, bit for bit
+    assert stored == int(kept_o.sum()) and nnz == stored + zeros
+    # the cases cover both sides of the row rule: rows whose dropped mass
+    # under the column thresholds alone stays <= 2 delta (rho_i = +inf), and
+    # rows above it (rho_i from the radix select)
+    tau = column_tau(Sf, delta / reg)
+    D = np.where(Sf < tau[None, :], Sf, 0.0).sum(1)
+    if hint < 0.2:
+        assert np.any(D <= 2 * delta / reg) if side == "le" else np.any(D > 2 * delta / reg)
+
+
 @pytest.mark.parametrize("bulk", ["fp64", "fp32"])
 @pytest.mark.parametrize("density", [0.0, 0.01, 0.1, 1.0])
 def test_topk_rule(gpu_lib, bulk, density):
