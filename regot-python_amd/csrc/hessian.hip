@@ -572,15 +572,19 @@ void hess_topk(regot_problem* p, const double* d_gamma, const double* d_grad, do
         rule.kstar = 0ull;    // every key > 0 kept; ties at key 0 (exact zeros) all taken
         rule.tie_take = len;
     } else {
+        // the k-th key and, in the same reads of M, the per-row tallies
         unsigned long long kstar = 0;
         long long above = 0;
-        topk_select(p->bulk, p->dM, p->ldm, c, kk, H, &kstar, &above, s);
+        topk_select_rows(p->bulk, p->dM, p->ldm, c, kk, rule.preserve, H, &kstar, &above,
+                         H.rcnt, H.zcnt, s);
         rule.kstar = kstar;
         rule.tie_take = kk - above;
     }
     // per row: ties and the stored entries kept without a tie slot; the tie
     // offsets (row-major order) then give the stored count of every row
-    launch_row_tally(p->bulk, p->dM, p->ldm, c, rule.kstar, rule.preserve, H.rcnt, H.zcnt, s);
+    if (kk <= 0 || kk >= len)
+        launch_row_tally(p->bulk, p->dM, p->ldm, c, rule.kstar, rule.preserve, H.rcnt, H.zcnt,
+                         s);
     H.ltmp.ensure(n + 4);
     launch_scan64(H.rcnt, n, H.ltmp.p + 4, s);
     rule.tie_off = H.ltmp.p + 4;

@@ -38,6 +38,13 @@ struct ElemCtx {
     int n = 0, m = 0;
 };
 
+// An entry of the top-k boundary bucket: its key (the bit pattern of T) and
+// position.
+struct TopkCand {
+    unsigned long long key;
+    int i, j;
+};
+
 struct SparseHessian {
     int n = 0, m = 0;
     long long nnz = 0;        // stored entries of sigma
@@ -50,6 +57,8 @@ struct SparseHessian {
     DBuf<int> rcnt, ccnt, zcnt, csc_work;
     DBuf<long long> ltmp;
     DBuf<unsigned long long> hist;   // global count histograms of the top-k radix passes
+    DBuf<TopkCand> topk_cand;        // top-k: entries of the boundary bucket
+    DBuf<unsigned long long> topk_ncand;
     DBuf<double> vn, vm, vm2;        // alpha, beta (full) of the current build; scratch
     // tau from row-major M (FP32 bulk): level-1 partials, boundary buckets,
     // candidate offsets and the candidates
@@ -102,6 +111,13 @@ void launch_row_threshold(int bulk, const void* M, long ldm, const ElemCtx& c,
 void topk_select(int bulk, const void* M, long ldm, const ElemCtx& c, long long k,
                  SparseHessian& H, unsigned long long* kstar, long long* n_greater,
                  hipStream_t s);
+// The same selection with two reads of M (the top digit's histogram, then a
+// pass that gathers the boundary bucket's entries and tallies every row
+// outside it), the remaining digits over those candidates; also fills the
+// row tallies of launch_row_tally (tcnt, gcnt) for 0 < k < n (m-1).
+void topk_select_rows(int bulk, const void* M, long ldm, const ElemCtx& c, long long k,
+                      int preserve, SparseHessian& H, unsigned long long* kstar,
+                      long long* n_greater, int* tcnt, int* gcnt, hipStream_t s);
 // Top-k tallies per row: ties (key == kstar) into tcnt; 2 x (stored entries
 // kept without a tie slot) + (column 0 is a preserved tie of a row > 0) into gcnt.
 void launch_row_tally(int bulk, const void* M, long ldm, const ElemCtx& c,

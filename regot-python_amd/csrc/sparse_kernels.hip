@@ -703,6 +703,176 @@ __global__ __launch_bounds__(256) void k_topk_hist(const TM* __restrict__ M, lon
         if (hist[q]) atomicAdd(ghist + q, static_cast<unsigned long long>(hist[q]));
 }
 
+// ---- Top-k with one radix pass over M --------------------------------------
+// After the first (top-digit) histogram pass, one more pass over M does the
+// rest of the selection's reading: every row counts its entries kept without
+// a tie slot among those outside the boundary bucket b1 (top digit above b1,
+// or preserved; nonzero), and the entries inside it -- the only ones the
+// k-th key can be among -- are gathered with their positions.  The remaining
+// radix passes, the ties (key == k-th key) and the per-row counts then run
+// on the candidates alone; the CSR write is the third and last read of M.
+
+constexpr int kCandStage = 1024;   // one 1024-column sub-step of a row
+
+// Row i (one workgroup, 4096 columns per step): gcnt[i] = 2 x the stored
+// entries kept without a tie slot outside bucket b1, tcnt[i] = 0 (the
+// candidates' ties are added later); entries of bucket b1 are staged in LDS
+// and appended to cand (one global reservation per flush, at most cap
+// entries in all).
+template <typename TM>
+__global__ __launch_bounds__(256) void k_topk_gather(const TM* __restrict__ M, long ldm, ElemCtx c,
+                                                     int shift1, unsigned long long b1,
+                                                     int preserve, int* __restrict__ tcnt,
+                                                     int* __restrict__ gcnt,
+                                                     TopkCand* __restrict__ cand,
+                                                     unsigned long long* __restrict__ ncand,
+                                                     long long cap)
+{
+    using E = Elem<TM>;
+    __shared__ unsigned long long skey[kCandStage];
+    __shared__ int scol[kCandStage];
+    __shared__ int scnt;
+    __shared__ long long sbase;
+    const int i = blockIdx.x;
+    const TM* row = M + static_cast<long>(i) * ldm;
+    const double ai = c.alpha[i];
+    const int m1 = c.m - 1;
+    if (threadIdx.x == 0) scnt = 0;
+    __syncthreads();
+    int ng = 0;
+    // 4096 columns per step (four 16-byte loads per thread in flight); each
+    // 1024-column sub-step stages at most 1024 candidates, flushed after it
+    for (int s0 = 0; s0 < m1; s0 += 4096) {
+        TM v[4][4];
+        double b4[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int cu = s0 + 1024 * u + threadIdx.x * 4;
+            if (cu < m1) {
+                load4(row, cu, c.m, v[u]);
+                load4(c.beta, cu, c.m, b4[u]);
+            }
+        }
+        const int lane = threadIdx.x & 63;
+        const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (s0 + 1024 * u >= m1) break;   // uniform
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int j = s0 + 1024 * u + threadIdx.x * 4 + q;
+                const bool in = j < m1;
+                bool z;
+                const TM t = in ? E::val(ai + b4[u][q], v[u][q], c.k, c.reg, z) : TM(0);
+                const unsigned long long key = static_cast<unsigned long long>(E::key(t));
+                const unsigned long long top = key >> shift1;
+                const bool isc = in && top == b1;
+                if (in && !isc) {
+                    const bool base = top > b1 || (preserve && (i == 0 || j == 0));
+                    ng += (base && t != TM(0)) ? 1 : 0;
+                }
+                // wave-aggregated append: one LDS atomic per wave
+                const unsigned long long mk = __ballot(isc);
+                if (mk) {
+                    int wb = 0;
+                    if (lane == 0) wb = atomicAdd(&scnt, __popcll(mk));
+                    wb = __shfl(wb, 0);
+                    if (isc) {
+                        const int pos = wb + __popcll(mk & lt);
+                        skey[pos] = key;
+                        scol[pos] = j;
+                    }
+                }
+            }
+            // a sub-step stages at most 1024 entries: flush them now
+            __syncthreads();
+            const int cnt = scnt;
+            if (cnt > 0) {
+                if (threadIdx.x == 0)
+                    sbase = static_cast<long long>(
+                        atomicAdd(ncand, static_cast<unsigned long long>(cnt)));
+                __syncthreads();
+                const long long base = sbase;
+                for (int e = threadIdx.x; e < cnt; e += 256)
+                    if (base + e < cap) {
+                        TopkCand cd;
+                        cd.key = skey[e];
+                        cd.i = i;
+                        cd.j = scol[e];
+                        cand[base + e] = cd;
+                    }
+                __syncthreads();
+                if (threadIdx.x == 0) scnt = 0;
+                __syncthreads();
+            }
+        }
+    }
+    int tot;
+    (void)block_excl_scan_int(ng, &tot);
+    if (threadIdx.x == 0) {
+        gcnt[i] = 2 * tot;
+        tcnt[i] = 0;
+    }
+}
+
+// Count histogram of one digit of the candidates' keys whose higher digits
+// equal prefix (run-merged LDS adds, then global adds).
+__global__ __launch_bounds__(256) void k_cand_hist(const TopkCand* __restrict__ cand, long long cap,
+                                                   const unsigned long long* __restrict__ ncand,
+                                                   int shift, int width,
+                                                   unsigned long long prefix,
+                                                   unsigned long long* __restrict__ ghist)
+{
+    __shared__ unsigned int hist[RB];
+    for (int q = threadIdx.x; q < RB; q += 256) hist[q] = 0u;
+    __syncthreads();
+    // the gather's own count, never more than the slots it could write
+    const long long nc = static_cast<long long>(*ncand) < cap ? static_cast<long long>(*ncand) : cap;
+    const unsigned long long dmask = (1ull << width) - 1ull;
+    unsigned int run_b = 0u, run_c = 0u;
+    for (long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; e < nc;
+         e += static_cast<long long>(gridDim.x) * 256) {
+        const unsigned long long key = cand[e].key;
+        if ((key >> (shift + width)) != prefix) continue;
+        const unsigned int bk = static_cast<unsigned int>((key >> shift) & dmask);
+        if (bk != run_b) {
+            if (run_c) atomicAdd(&hist[run_b], run_c);
+            run_b = bk;
+            run_c = 0u;
+        }
+        run_c++;
+    }
+    if (run_c) atomicAdd(&hist[run_b], run_c);
+    __syncthreads();
+    for (int q = threadIdx.x; q < (1 << width); q += 256)
+        if (hist[q]) atomicAdd(ghist + q, static_cast<unsigned long long>(hist[q]));
+}
+
+// The candidates' share of the row tallies under the k-th key kstar: ties
+// into tcnt; stored entries kept without a tie slot (key > kstar, or
+// preserved; nonzero) add 2 to gcnt; a preserved tie in column 0 of a row
+// i > 0 sets gcnt's low bit (k_row_tally's encoding).
+__global__ __launch_bounds__(256) void k_cand_classify(const TopkCand* __restrict__ cand,
+                                                       long long cap,
+                                                       const unsigned long long* __restrict__ ncand,
+                                                       unsigned long long kstar, int preserve,
+                                                       int n, int* __restrict__ tcnt,
+                                                       int* __restrict__ gcnt)
+{
+    const long long nc = static_cast<long long>(*ncand) < cap ? static_cast<long long>(*ncand) : cap;
+    const long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= nc) return;
+    const TopkCand cd = cand[e];
+    if (cd.i < 0 || cd.i >= n) return;
+    const bool pres = preserve && (cd.i == 0 || cd.j == 0);
+    const bool base = cd.key > kstar || pres;
+    if (base && cd.key != 0ull) atomicAdd(gcnt + cd.i, 2);
+    if (cd.key == kstar) {
+        atomicAdd(tcnt + cd.i, 1);
+        if (cd.j == 0 && pres && cd.i > 0) atomicOr(gcnt + cd.i, 1);
+    }
+}
+
 // Top-k tally of row i (one workgroup, 4096 columns per step): ties
 // (key == kstar) into tcnt[i]; stored entries kept without a tie slot
 // (key > kstar, or preserved; nonzero) times 2, plus 1 if column 0 is a tie of
@@ -1307,6 +1477,89 @@ void topk_select(int bulk, const void* M, long ldm, const ElemCtx& c, long long 
         if (b < 0) fail(E_INTERNAL, "top-k radix select lost the k-th entry");
         prefix = (prefix << width) | static_cast<unsigned long long>(b);
     }
+    *kstar = prefix;
+    *n_greater = above;
+}
+
+void topk_select_rows(int bulk, const void* M, long ldm, const ElemCtx& c, long long k,
+                      int preserve, SparseHessian& H, unsigned long long* kstar,
+                      long long* n_greater, int* tcnt, int* gcnt, hipStream_t s)
+{
+    const int KB = bulk == 0 ? 32 : 64;
+    const int npass = (KB + RD - 1) / RD;
+    const int rows_per_block = std::max(1, ceil_div(c.n, 2048));
+    const int nblocks = ceil_div(c.n, rows_per_block);
+    H.hist.ensure(RB);
+    H.hll.ensure(RB);
+    // pass over M 1: the top digit
+    // digit p covers key bits [KB - p RD - w, KB - p RD), w = min(RD, KB - p RD)
+    auto host_digit = [KB](int p, int* w) {
+        const int hi = KB - p * RD;
+        *w = hi < RD ? hi : RD;
+        return hi - *w;
+    };
+    int width;
+    const int shift1 = host_digit(0, &width);
+    HIP_CHECK(hipMemsetAsync(H.hist.p, 0, RB * sizeof(unsigned long long), s));
+    if (bulk == 0)
+        hipLaunchKernelGGL(k_topk_hist<float>, dim3(nblocks), dim3(256), 0, s, cM<float>(M), ldm,
+                           c, rows_per_block, shift1, width, 0ull, 1, H.hist.p);
+    else
+        hipLaunchKernelGGL(k_topk_hist<double>, dim3(nblocks), dim3(256), 0, s, cM<double>(M),
+                           ldm, c, rows_per_block, shift1, width, 0ull, 1, H.hist.p);
+    KERNEL_CHECK();
+    HIP_CHECK(hipMemcpyAsync(H.hll.p, H.hist.p, RB * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    long long above = 0;
+    int b = (1 << width) - 1;
+    for (; b >= 0; b--) {
+        const long long h = H.hll.p[b];
+        if (above + h >= k) break;
+        above += h;
+    }
+    if (b < 0) fail(E_INTERNAL, "top-k radix select lost the k-th entry");
+    const long long nc = H.hll.p[b];   // entries of the boundary bucket
+    // pass over M 2: the row tallies outside the bucket and its entries
+    H.topk_cand.ensure(static_cast<size_t>(nc));
+    H.topk_ncand.ensure(1);
+    HIP_CHECK(hipMemsetAsync(H.topk_ncand.p, 0, sizeof(unsigned long long), s));
+    if (bulk == 0)
+        hipLaunchKernelGGL(k_topk_gather<float>, dim3(c.n), dim3(256), 0, s, cM<float>(M), ldm, c,
+                           shift1, static_cast<unsigned long long>(b), preserve, tcnt, gcnt,
+                           H.topk_cand.p, H.topk_ncand.p, nc);
+    else
+        hipLaunchKernelGGL(k_topk_gather<double>, dim3(c.n), dim3(256), 0, s, cM<double>(M), ldm,
+                           c, shift1, static_cast<unsigned long long>(b), preserve, tcnt, gcnt,
+                           H.topk_cand.p, H.topk_ncand.p, nc);
+    KERNEL_CHECK();
+    // the remaining digits over the candidates
+    unsigned long long prefix = static_cast<unsigned long long>(b);
+    const int nb = std::max(1, std::min(1024, ceil_div(nc, 2048)));
+    for (int p = 1; p < npass; p++) {
+        int w;
+        const int shift = host_digit(p, &w);
+        HIP_CHECK(hipMemsetAsync(H.hist.p, 0, RB * sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(k_cand_hist, dim3(nb), dim3(256), 0, s, H.topk_cand.p, nc,
+                           H.topk_ncand.p, shift, w, prefix, H.hist.p);
+        KERNEL_CHECK();
+        HIP_CHECK(hipMemcpyAsync(H.hll.p, H.hist.p, RB * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        int bb = (1 << w) - 1;
+        for (; bb >= 0; bb--) {
+            const long long h = H.hll.p[bb];
+            if (above + h >= k) break;
+            above += h;
+        }
+        if (bb < 0) fail(E_INTERNAL, "top-k radix select lost the k-th entry");
+        prefix = (prefix << w) | static_cast<unsigned long long>(bb);
+    }
+    // the candidates' share of the tallies
+    if (nc > 0)
+        hipLaunchKernelGGL(k_cand_classify, dim3(ceil_div(nc, 256)), dim3(256), 0, s,
+                           H.topk_cand.p, nc, H.topk_ncand.p, prefix, preserve, c.n, tcnt, gcnt);
+    KERNEL_CHECK();
     *kstar = prefix;
     *n_greater = above;
 }
