@@ -164,20 +164,20 @@ struct Work {
     }
 
     // what a Hessian build writes: CSR + CSC (12 bytes per entry each), or
-    // the kept-set bitmask of a matrix-free build (one bit per entry)
+    // the kept matrix of a matrix-free build (4 bytes per entry)
     double build_bytes()
     {
         const SparseHessian& H = hess_of(p);
-        if (H.mf) return static_cast<double>(n) * (m - 1) / 8.0;
+        if (H.mf) return static_cast<double>(n) * (m - 1) * 4.0;
         return 2.0 * H.nnz * 12.0;
     }
 
-    // one product with sigma and sigma': the CSR + CSC reads, or a matrix-free
-    // pass (M's first m - 1 columns in FP32 plus the bitmask)
+    // one product with sigma and sigma': the CSR + CSC reads, or a
+    // matrix-free pass (the FP32 kept matrix)
     double product_bytes()
     {
         const SparseHessian& H = hess_of(p);
-        if (H.mf) return static_cast<double>(n) * (m - 1) * (4.0 + 1.0 / 8.0);
+        if (H.mf) return static_cast<double>(n) * (m - 1) * 4.0;
         return 2.0 * H.nnz * 12.0;
     }
 

@@ -1,25 +1,30 @@
-// hess_mf.hip -- products with the sparsified Hessian of the threshold rule
-// straight from M (FP32 bulk), without its CSR / CSC copies.
+// hess_mf.hip -- products with the sparsified Hessian of the FP32 bulk as a
+// dense FP32 matrix of its kept entries (the "matrix-free" storage: no CSR /
+// CSC, no indices).
 //
 // The threshold rule keeps entry (i, j), j < m-1, iff T_ij >= min(tau_j,
 // rho_i) (or T_ij underflows, which adds nothing to a product), and sigma_ij
-// = T_ij / reg.  T_ij is recomputed from M with the sparsifier's own element
-// arithmetic (log2 T formed in FP64, exp2 in FP32), so the kept set and the
-// values are the stored copies' bit for bit.  At the densities SSNS reaches
-// (tens of percent) the two stored copies cost 2 x 12 bytes per kept entry
-// and product, more than one read of M (4 bytes per entry), and the
-// Schur-complement product Delta u = (h2 + s) u - sigma' ((sigma u) / (h1 +
-// s)) needs both halves: here one pass does both, since a workgroup that
+// = T_ij / reg.  At the densities SSNS reaches (tens of percent) the two
+// stored copies cost 2 x 12 bytes per kept entry and product, more than a
+// dense FP32 read (4 bytes per entry).  The build writes the kept matrix K
+// (K_ij = T_ij where kept, else 0; row pitch ldt) from the same pass that
+// decides the kept set, with the sparsifier's own element arithmetic (log2 T
+// in FP64, exp2f), so its values are the stored copies' bit for bit; within
+// one Newton step every product then reads K instead of re-forming T from M
+// (no exponent arguments, no exponentials, no kept test per element).
+//
+// The Schur-complement product Delta u = (h2 + s) u - sigma' ((sigma u) /
+// (h1 + s)) needs both halves: one pass does both, since a workgroup that
 // holds whole rows knows y_i = (sigma u)_i / (h1_i + s) as soon as row i is
 // summed, and adds sigma_ij y_i into column accumulators from the same
 // registers.
 //
 // Layout: one workgroup of 512 threads per compute unit holds a contiguous
 // block of rows; thread t owns the 4-column vectors at 4t + 2048c (c < NC,
-// m - 1 <= 2048 NC <= 16384), with beta_j (FP64), tau_j (FP32) and its FP64
-// column accumulators in registers; the row input vector is staged in LDS
-// (FP64).  Products and sums are FP64.  Each workgroup writes one FP64 column
-// partial per column; mf_fold sums them in a fixed order.
+// m - 1 <= 2048 NC <= 16384) with FP64 column accumulators in registers; the
+// row input vector is staged in LDS (FP64).  Products and sums are FP64.
+// Each workgroup writes one FP64 column partial per column; mf_fold sums
+// them in a fixed order.
 #include "dense_device.hpp"
 #include "hess_mf.hpp"
 
@@ -32,20 +37,6 @@ using namespace dev;
 
 constexpr int kMaxCols = 16384;          // m - 1 <= 16384
 
-// T_ij of the FP32-bulk sparsifier (Elem<float>::val in sparse_kernels.hip).
-__device__ __forceinline__ float tval(double ab, float mij, double k)
-{
-    return exp2f(static_cast<float>(fma(-k, static_cast<double>(mij), ab * k)));
-}
-
-// T_ij in the products: the same FP64 argument, the hardware exp2.  It equals
-// exp2f wherever T_ij >= 2^-126; below, where exp2f returns a subnormal, it
-// returns 0, which moves a product by less than 2^-126 max|x| per entry.
-__device__ __forceinline__ float tval_pass(double ab, float mij, double k)
-{
-    return __builtin_amdgcn_exp2f(static_cast<float>(fma(-k, static_cast<double>(mij), ab * k)));
-}
-
 // Butterfly sum over the wave; lane 0's value is the one used (fixed order).
 __device__ __forceinline__ double wave_total(double v)
 {
@@ -54,6 +45,8 @@ __device__ __forceinline__ double wave_total(double v)
     return v;
 }
 
+// Row r of an FP32 matrix (row pointer Mi) in the pass layout: out[c] holds
+// columns 4t + SPAN c .. + 3; the last chunk past m - 1 is zero.
 template <int NC, bool FULL, int SPAN>
 __device__ __forceinline__ void load_row(const float* __restrict__ Mi, int t, int vlast,
                                          float4* out, bool nt)
@@ -67,12 +60,26 @@ __device__ __forceinline__ void load_row(const float* __restrict__ Mi, int t, in
     }
 }
 
-// Kept-set bitmask of the threshold rule in the pass layout: word (i, t)
-// holds bit 4c + q for column 4t + 2048c + q of row i (kept <=> T_ij >=
-// min(tau_j, rho_i) and T_ij != 0 in FP32, the entries a product sees; with
-// tau and rho null every nonzero entry, the full Hessian).
+// Store chunk c of a row in the pass layout (the last chunk's columns past
+// m - 1, vlast of them valid, are left alone).
 template <int NC, bool FULL>
-__global__ __launch_bounds__(512) void k_mf_mask(MfArgs a, unsigned int* __restrict__ mask)
+__device__ __forceinline__ void store4(float* __restrict__ Ti, int t, int c, int span, int vlast,
+                                       const float (&v)[4])
+{
+    float* p = Ti + 4 * t + c * span;
+    if (FULL || c < NC - 1 || vlast >= 4) {
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (q < vlast) p[q] = v[q];
+    }
+}
+
+// The full Hessian's kept matrix: K_ij = T_ij for every j < m - 1 (the
+// sparsifier's arithmetic), the build of the top-k rule at density 1.
+template <int NC, bool FULL>
+__global__ __launch_bounds__(512) void k_mf_full(MfArgs a)
 {
     constexpr int kSpan = 2048;
     const int t = threadIdx.x;
@@ -83,23 +90,18 @@ __global__ __launch_bounds__(512) void k_mf_mask(MfArgs a, unsigned int* __restr
     vlast = FULL ? 4 : (vlast < 0 ? 0 : (vlast > 4 ? 4 : vlast));
     auto ok = [vlast](int c, int q) { return FULL || c < NC - 1 || q < vlast; };
     double bj[NC][4];
-    float tau[NC][4];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int j0 = 4 * t + c * kSpan;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            bj[c][q] = ok(c, q) ? a.beta[j0 + q] : 0.0;
-            tau[c][q] = !ok(c, q) ? INFINITY : a.tau ? static_cast<float>(a.tau[j0 + q]) : 0.f;
-        }
+        for (int q = 0; q < 4; q++) bj[c][q] = ok(c, q) ? a.beta[j0 + q] : 0.0;
     }
-    // the row in flight and its alpha_i, rho_i load together (see k_mf_pass)
+    // the row in flight and its alpha_i load together
     float4 pf[NC];
-    double pa = 0.0, pr = 0.0;
+    double pa = 0.0;
     auto prefetch = [&](long r) {
         load_row<NC, FULL, kSpan>(a.M + r * a.ldm, t, vlast, pf, a.nt != 0);
         pa = a.alpha[r];
-        if (a.rho) pr = a.rho[r];
     };
     if (r0 < r1) prefetch(r0);
 #pragma unroll 1
@@ -110,18 +112,17 @@ __global__ __launch_bounds__(512) void k_mf_mask(MfArgs a, unsigned int* __restr
 #pragma unroll
             for (int q = 0; q < 4; q++) e[c][q] = vget<float4, float>(pf[c], q);
         const double ai = pa;
-        const float ri = static_cast<float>(pr);
         if (i + 1 < r1) prefetch(i + 1);
-        unsigned int bits = 0u;
+        float* Ti = a.Tout + i * a.ldt;
 #pragma unroll
-        for (int c = 0; c < NC; c++)
+        for (int c = 0; c < NC; c++) {
+            float k4[4];
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                if (ok(c, q)) {
-                    const float tv = tval(ai + bj[c][q], e[c][q], a.k);
-                    if (tv >= fminf(tau[c][q], ri) && tv != 0.f) bits |= 1u << (4 * c + q);
-                }
-        mask[i * 512 + t] = bits;
+                k4[q] = exp2f(static_cast<float>(
+                    fma(-a.k, static_cast<double>(e[c][q]), (ai + bj[c][q]) * a.k)));
+            store4<NC, FULL>(Ti, t, c, kSpan, vlast, k4);
+        }
     }
 }
 
@@ -198,13 +199,13 @@ __device__ int select_desc512(const double* hist, int nb, double above, double t
 // k_row_threshold (the largest dropped value v with sum_{d >= v} d >= D_i -
 // 2 delta: 11-, 11- and 10-bit digits, FP64 masses) over the row held in
 // registers; the stored count (T_ij >= min(tau_j, rho_i), T_ij != 0), the
-// FP64-zero count (exp2 argument < -1075) and, when mask is not null, the
-// kept bits.  The element arithmetic is Elem<float>::val's.
+// FP64-zero count (exp2 argument < -1075) and, when a.Tout is not null, the
+// row of the dense kept matrix (T_ij where kept, else 0).  The element
+// arithmetic is Elem<float>::val's.
 template <int NC, bool FULL>
 __global__ __launch_bounds__(512) void k_mf_rule(MfArgs a, double delta2,
                                                  double* __restrict__ rho, int* __restrict__ cnt,
-                                                 int* __restrict__ zcnt,
-                                                 unsigned int* __restrict__ mask)
+                                                 int* __restrict__ zcnt)
 {
     constexpr int kSpan = 2048;
     constexpr int kWaves = 8;
@@ -357,16 +358,29 @@ __global__ __launch_bounds__(512) void k_mf_rule(MfArgs a, double delta2,
             cnt[i] = NS;
             zcnt[i] = NZ;
         }
-        if (mask != nullptr) mask[i * 512 + t] = bits;
+        if (a.Tout != nullptr) {
+            float* Ti = a.Tout + i * a.ldt;
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                float k4[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) k4[q] = ((bits >> (4 * c + q)) & 1u) ? e[c][q] : 0.f;
+                store4<NC, FULL>(Ti, t, c, kSpan, vlast, k4);
+            }
+        }
     }
 }
 
 // MODE (see hess_mf.hpp): which halves run and what weights the columns.
+// K rows are read two ahead (they and the per-row operands load together:
+// a load of its own inside the loop would wait for the whole prefetch).
 template <int MODE, int NC, bool FULL, int TH>
 __global__ __launch_bounds__(TH) void k_mf_pass(MfArgs a)
 {
     constexpr bool ROW = MODE == MF_CG || MODE == MF_HS || MODE == MF_ROWS;
     constexpr bool COL = MODE != MF_ROWS;
+    constexpr bool WH1 = MODE == MF_CG || MODE == MF_DIAG;
+    constexpr bool WXR = MODE == MF_HS || MODE == MF_COLS;
     constexpr int kThreads = TH;
     constexpr int kWaves = TH / 64;
     constexpr int kSpan = 4 * TH;
@@ -409,70 +423,43 @@ __global__ __launch_bounds__(TH) void k_mf_pass(MfArgs a)
 
     if (ROW)
         for (int j = t; j < m1; j += kThreads) sx[j] = a.xc[j];
-    double bj[NC][4];
     double acc[NC][4];
 #pragma unroll
-    for (int c = 0; c < NC; c++) {
-        const int j0 = 4 * t + c * kSpan;
+    for (int c = 0; c < NC; c++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            bj[c][q] = ok(c, q) ? a.beta[j0 + q] : 0.0;
-            acc[c][q] = 0.0;
-        }
-    }
+        for (int q = 0; q < 4; q++) acc[c][q] = 0.0;
     if (ROW) __syncthreads();
-    const unsigned int* mk = a.mask + t;
 
-    // One row in flight: its M values, kept bits, alpha_i and the row input
-    // of the column half (h1_i or xr_i) are loaded together while the row
-    // before is computed.  (Every per-row operand goes through this group: a
-    // load of its own inside the loop would wait for the whole prefetch.)
-    constexpr bool WH1 = MODE == MF_CG || MODE == MF_DIAG;
-    constexpr bool WXR = MODE == MF_HS || MODE == MF_COLS;
-    float4 pf[NC];
-    unsigned int pbits = 0u;
-    double pa = 0.0, pw = 0.0;
-    auto prefetch = [&](int r) {
-        load_row<NC, FULL, kSpan>(a.M + static_cast<long>(r) * a.ldm, t, vlast, pf, a.nt != 0);
-        pbits = mk[static_cast<long>(r) * kThreads];
-        pa = a.alpha[r];
-        if (WH1) pw = a.h1[r];
-        if (WXR) pw = a.xr[r];
+    float4 pf0[NC], pf1[NC];
+    double w0 = 0.0, w1 = 0.0;   // the row weight input (h1_i or xr_i) of the rows in flight
+    auto load = [&](int r, float4* pf, double* w) {
+        load_row<NC, FULL, kSpan>(a.T + static_cast<long>(r) * a.ldt, t, vlast, pf, a.nt != 0);
+        if (WH1) *w = a.h1[r];
+        if (WXR) *w = a.xr[r];
     };
-    if (r0 < r1) prefetch(r0);
+    if (r0 < r1) load(r0, pf0, &w0);
+    if (r0 + 1 < r1) load(r0 + 1, pf1, &w1);
 #pragma unroll 1
     for (int i = r0; i < r1; i++) {
         float e[NC][4];
 #pragma unroll
-        for (int c = 0; c < NC; c++)
+        for (int c = 0; c < NC; c++) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) e[c][q] = vget<float4, float>(pf[c], q);
-        const unsigned int bits = pbits;
-        const double ai = pa;
-        const double wi = pw;
-        if (i + 1 < r1) prefetch(i + 1);
-        double racc = 0.0;
-        // T is formed for every element and then masked: a branch per element
-        // keeps each exp's operands live across it and spills at 8 chunks
-        const double* vx = sx;
-#pragma unroll
-        for (int c = 0; c < NC; c++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const float tx = tval_pass(ai + bj[c][q], e[c][q], a.k);
-                const float tv = ((bits >> (4 * c + q)) & 1u) ? tx : 0.f;
-                e[c][q] = tv;
-                if (ROW && ok(c, q))
-                    racc = fma(static_cast<double>(tv), vx[4 * t + c * kSpan + q], racc);
-            }
-        // keep the row's terms as FP32 across the row reduction (the FP64
-        // copies the row half converted would otherwise double their registers)
-#pragma unroll
-        for (int c = 0; c < NC; c++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) asm volatile("" : "+v"(e[c][q]));
+            for (int q = 0; q < 4; q++) e[c][q] = ok(c, q) ? vget<float4, float>(pf0[c], q) : 0.f;
+            pf0[c] = pf1[c];
+        }
+        const double wi = w0;
+        w0 = w1;
+        if (i + 2 < r1) load(i + 2, pf1, &w1);
         double cw = 0.0;
         if (ROW) {
+            double racc = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (ok(c, q))
+                        racc = fma(static_cast<double>(e[c][q]), sx[4 * t + c * kSpan + q], racc);
             const int slot = (i - r0) & 1;
             const double S = wave_total(racc);
             if (lane == 0) red[slot][wave] = S;
@@ -578,6 +565,8 @@ __global__ __launch_bounds__(256) void k_mf_fold(const double* __restrict__ part
 
 bool mf_shape_ok(int n, int m) { return n >= 1 && m >= 2 && m - 1 <= kMaxCols; }
 
+long mf_pitch(int m) { return (static_cast<long>(m - 1) + 63) / 64 * 64; }
+
 int mf_blocks(int n, int ncu)
 {
     int g = ncu > 0 ? ncu : 256;
@@ -587,63 +576,65 @@ int mf_blocks(int n, int ncu)
 }
 
 template <int NC>
-void mask_launch(const MfArgs& a, int nwg, bool full, unsigned int* mask, hipStream_t s)
+void full_launch(const MfArgs& a, int nwg, bool full, hipStream_t s)
 {
     if (full)
-        hipLaunchKernelGGL((k_mf_mask<NC, true>), dim3(nwg), dim3(512), 0, s, a, mask);
+        hipLaunchKernelGGL((k_mf_full<NC, true>), dim3(nwg), dim3(512), 0, s, a);
     else
-        hipLaunchKernelGGL((k_mf_mask<NC, false>), dim3(nwg), dim3(512), 0, s, a, mask);
+        hipLaunchKernelGGL((k_mf_full<NC, false>), dim3(nwg), dim3(512), 0, s, a);
 }
 
-void launch_mf_mask(MfArgs a, int nwg, unsigned int* mask, hipStream_t s)
+void launch_mf_full(MfArgs a, int nwg, hipStream_t s)
 {
-    if (!mf_shape_ok(a.n, a.m1 + 1)) fail(E_INTERNAL, "matrix-free mask: %d x %d", a.n, a.m1 + 1);
+    if (!mf_shape_ok(a.n, a.m1 + 1)) fail(E_INTERNAL, "kept matrix: %d x %d", a.n, a.m1 + 1);
+    if (a.Tout == nullptr || a.ldt < a.m1) fail(E_INTERNAL, "kept matrix: no output");
     a.rpw = ceil_div(a.n, nwg);
     a.nt = 0;
     const int NC = ceil_div(a.m1, 2048);
     const bool full = a.m1 == NC * 2048;
     switch (NC) {
-    case 1: mask_launch<1>(a, nwg, full, mask, s); break;
-    case 2: mask_launch<2>(a, nwg, full, mask, s); break;
-    case 3: mask_launch<3>(a, nwg, full, mask, s); break;
-    case 4: mask_launch<4>(a, nwg, full, mask, s); break;
-    case 5: mask_launch<5>(a, nwg, full, mask, s); break;
-    case 6: mask_launch<6>(a, nwg, full, mask, s); break;
-    case 7: mask_launch<7>(a, nwg, full, mask, s); break;
-    default: mask_launch<8>(a, nwg, full, mask, s); break;
+    case 1: full_launch<1>(a, nwg, full, s); break;
+    case 2: full_launch<2>(a, nwg, full, s); break;
+    case 3: full_launch<3>(a, nwg, full, s); break;
+    case 4: full_launch<4>(a, nwg, full, s); break;
+    case 5: full_launch<5>(a, nwg, full, s); break;
+    case 6: full_launch<6>(a, nwg, full, s); break;
+    case 7: full_launch<7>(a, nwg, full, s); break;
+    default: full_launch<8>(a, nwg, full, s); break;
     }
     KERNEL_CHECK();
 }
 
 template <int NC>
 void rule_launch(const MfArgs& a, int nwg, bool full, double delta2, double* rho, int* cnt,
-                 int* zcnt, unsigned int* mask, hipStream_t s)
+                 int* zcnt, hipStream_t s)
 {
     if (full)
         hipLaunchKernelGGL((k_mf_rule<NC, true>), dim3(nwg), dim3(512), 0, s, a, delta2, rho, cnt,
-                           zcnt, mask);
+                           zcnt);
     else
         hipLaunchKernelGGL((k_mf_rule<NC, false>), dim3(nwg), dim3(512), 0, s, a, delta2, rho,
-                           cnt, zcnt, mask);
+                           cnt, zcnt);
 }
 
 void launch_mf_rule(MfArgs a, int nwg, double delta2, double* rho, int* cnt, int* zcnt,
-                    unsigned int* mask, hipStream_t s)
+                    hipStream_t s)
 {
     if (!mf_shape_ok(a.n, a.m1 + 1)) fail(E_INTERNAL, "threshold rows: %d x %d", a.n, a.m1 + 1);
+    if (a.Tout != nullptr && a.ldt < a.m1) fail(E_INTERNAL, "kept matrix pitch %ld", a.ldt);
     a.rpw = ceil_div(a.n, nwg);
     a.nt = 0;
     const int NC = ceil_div(a.m1, 2048);
     const bool full = a.m1 == NC * 2048;
     switch (NC) {
-    case 1: rule_launch<1>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
-    case 2: rule_launch<2>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
-    case 3: rule_launch<3>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
-    case 4: rule_launch<4>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
-    case 5: rule_launch<5>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
-    case 6: rule_launch<6>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
-    case 7: rule_launch<7>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
-    default: rule_launch<8>(a, nwg, full, delta2, rho, cnt, zcnt, mask, s); break;
+    case 1: rule_launch<1>(a, nwg, full, delta2, rho, cnt, zcnt, s); break;
+    case 2: rule_launch<2>(a, nwg, full, delta2, rho, cnt, zcnt, s); break;
+    case 3: rule_launch<3>(a, nwg, full, delta2, rho, cnt, zcnt, s); break;
+    case 4: rule_launch<4>(a, nwg, full, delta2, rho, cnt, zcnt, s); break;
+    case 5: rule_launch<5>(a, nwg, full, delta2, rho, cnt, zcnt, s); break;
+    case 6: rule_launch<6>(a, nwg, full, delta2, rho, cnt, zcnt, s); break;
+    case 7: rule_launch<7>(a, nwg, full, delta2, rho, cnt, zcnt, s); break;
+    default: rule_launch<8>(a, nwg, full, delta2, rho, cnt, zcnt, s); break;
     }
     KERNEL_CHECK();
 }
@@ -651,8 +642,9 @@ void launch_mf_rule(MfArgs a, int nwg, double delta2, double* rho, int* cnt, int
 void launch_mf_pass(int mode, MfArgs a, int nwg, hipStream_t s)
 {
     if (!mf_shape_ok(a.n, a.m1 + 1)) fail(E_INTERNAL, "matrix-free pass: %d x %d", a.n, a.m1 + 1);
+    if (a.T == nullptr || a.ldt < a.m1) fail(E_INTERNAL, "matrix-free pass: no kept matrix");
     a.rpw = ceil_div(a.n, nwg);
-    a.nt = static_cast<long>(a.n) * a.ldm * 4 > (256L << 20) ? 1 : 0;
+    a.nt = static_cast<long>(a.n) * a.ldt * 4 > (256L << 20) ? 1 : 0;
     switch (mode) {
     case MF_CG: pass_nc<MF_CG>(a, nwg, s); break;
     case MF_HS: pass_nc<MF_HS>(a, nwg, s); break;

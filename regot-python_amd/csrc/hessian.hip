@@ -401,7 +401,8 @@ static MfArgs mf_args(regot_problem* p, const SparseHessian& H)
     a.beta = H.vm;
     a.k = p->k;
     a.inv_reg = 1.0 / p->reg;
-    a.mask = H.mf_mask;
+    a.T = H.mf_t;
+    a.ldt = H.mf_ldt;
     a.h1 = H.h1;
     return a;
 }
@@ -415,21 +416,23 @@ static bool mf_allowed(const regot_problem* p, bool driver)
     return p->hess_storage == 1 || driver;
 }
 
-// Buffers of a matrix-free build: the kept-set bitmask (n x 512 words) and
+// Buffers of a matrix-free build: the kept matrix (n x mf_pitch(m) FP32) and
 // the column partials of the passes.
 static void mf_buffers(regot_problem* p, SparseHessian& H)
 {
     H.mf_nwg = mf_blocks(p->n, p->ncu);
-    H.mf_mask.ensure(static_cast<size_t>(p->n) * 512);
+    H.mf_ldt = mf_pitch(p->m);
+    H.mf_t.ensure(static_cast<size_t>(p->n) * H.mf_ldt);
     H.mf_part.ensure(static_cast<size_t>(H.mf_nwg) * std::max(p->m - 1, 1));
 }
 
-// Matrix-free build with every nonzero entry kept (the full Hessian).
+// Matrix-free build with every entry kept (the full Hessian).
 static void build_mf_full(regot_problem* p, SparseHessian& H)
 {
     mf_buffers(p, H);
     MfArgs a = mf_args(p, H);
-    launch_mf_mask(a, H.mf_nwg, H.mf_mask, p->stream);
+    a.Tout = H.mf_t;
+    launch_mf_full(a, H.mf_nwg, p->stream);
     H.mf = true;
 }
 
@@ -502,8 +505,8 @@ void hess_threshold(regot_problem* p, const double* d_gamma, const double* d_gra
             if (may_mf) mf_buffers(p, H);
             MfArgs a = mf_args(p, H);
             a.tau = H.tau;
-            launch_mf_rule(a, mf_blocks(p->n, p->ncu), 2.0 * delta, H.rho, H.rcnt, H.zcnt,
-                           may_mf ? H.mf_mask.p : nullptr, s);
+            a.Tout = may_mf ? H.mf_t.p : nullptr;
+            launch_mf_rule(a, mf_blocks(p->n, p->ncu), 2.0 * delta, H.rho, H.rcnt, H.zcnt, s);
         } else {
             launch_row_threshold(p->bulk, p->dM, p->ldm, c, H.tau, 2.0 * delta, H.rho, H.rcnt,
                                  H.zcnt, s);

@@ -68,12 +68,14 @@ struct SparseHessian {
     DBuf<float> tau_cand;
     DBuf<double> cg_r, cg_p, cg_q, cg_s, cg_u, cg_y_buf, cg_part, cg_scal, cg_dinv;
     DBuf<int> cg_ctl;
-    // Matrix-free build (threshold rule, FP32 bulk): no CSR / CSC; the kept
-    // set is the bitmask mf_mask (n x 512 words) and every product is a pass
-    // over M by mf_nwg workgroups, whose column partials go to mf_part.
+    // Matrix-free build (FP32 bulk): no CSR / CSC; the kept entries as the
+    // dense FP32 matrix mf_t (K_ij = T_ij where kept, else 0; row pitch
+    // mf_ldt) and every product is a pass over it by mf_nwg workgroups,
+    // whose column partials go to mf_part.
     bool mf = false;
     int mf_nwg = 0;
-    DBuf<unsigned int> mf_mask;
+    long mf_ldt = 0;
+    DBuf<float> mf_t;
     DBuf<double> mf_part;
     HBuf<int> cg_hctl;
     HBuf<double> hbuf;
