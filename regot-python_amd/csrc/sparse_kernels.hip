@@ -9,6 +9,7 @@
 // placement and the output is identical run to run.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "sparse.hpp"
 
@@ -347,21 +348,33 @@ __global__ __launch_bounds__(256) void k_row_threshold(const TM* __restrict__ M,
 //
 // tau_j is the smallest key K of column j with sum_{key <= K} T_ij > delta.
 // Level 1 (k_tau_hist): per column, FP64 mass and count of T over 64 buckets
-// of the key's top seven bits (T >= 0: the sign and six exponent bits, four
-// binades per bucket) and the largest key, by strips of 64 columns (lane =
-// column) over row blocks.  k_tau_fold sums the row blocks in a fixed order
-// and finds each column's boundary bucket (or settles tau_j when the column's
-// mass never passes delta); k_tau_gather collects the entries of the boundary
-// bucket; k_tau_select ends the selection on those candidates with the
-// radix passes of k_col_threshold over the remaining 25 key bits.
+// by binade (tau_bucket: every tau_j exceeds delta / n, since n entries of
+// at most delta / n cannot pass delta, so bucket 0 takes T below the binade
+// of delta / n, buckets 1..62 one binade each upwards, bucket 63 the rest)
+// and the largest key, by strips of 64 columns (lane = column) over row
+// blocks.  k_tau_fold sums the row blocks in a fixed order and finds each
+// column's boundary bucket (or settles tau_j when the column's mass never
+// passes delta); k_tau_gather collects the entries of the boundary bucket;
+// k_tau_select ends the selection on those candidates with the radix passes
+// of k_col_threshold over their full keys.  Any bucketing monotone in the key
+// gives the same tau_j; this one keeps the boundary buckets to one binade.
 // ---------------------------------------------------------------------------
 
-constexpr int TB = 64;    // level-1 buckets (key >> 25)
+constexpr int TB = 64;    // level-1 buckets (tau_bucket)
 constexpr int TC = 64;    // columns per strip workgroup
 constexpr int TU = 16;    // rows per wave step (loads in flight)
 
+// Level-1 bucket of a key (T >= 0): its binade relative to elo (the biased
+// exponent of delta / n), clamped to [0, 63]; monotone in the key.
+__device__ __forceinline__ int tau_bucket(unsigned int key, int elo)
+{
+    const int b = static_cast<int>((key >> 23) & 255u) - elo + 1;
+    return b < 0 ? 0 : (b > TB - 1 ? TB - 1 : b);
+}
+
 __global__ __launch_bounds__(256) void k_tau_hist(const float* __restrict__ M, long ldm, ElemCtx c,
-                                                  int rows_per_blk, double* __restrict__ pmass,
+                                                  int elo, int rows_per_blk,
+                                                  double* __restrict__ pmass,
                                                   unsigned int* __restrict__ pcnt,
                                                   unsigned int* __restrict__ pmax)
 {
@@ -403,7 +416,7 @@ __global__ __launch_bounds__(256) void k_tau_hist(const float* __restrict__ M, l
                 bool z;
                 const float t = Elem<float>::val(av[u] + bj, mv[u], c.k, c.reg, z);
                 const unsigned int key = __float_as_uint(t);
-                const int b = static_cast<int>(key >> 25) & (TB - 1);
+                const int b = tau_bucket(key, elo);
                 atomicAdd(&hm[b * P + lane], static_cast<double>(t));
                 atomicAdd(&hc[b * P + lane], 1u);
                 kmax = key > kmax ? key : kmax;
@@ -474,7 +487,7 @@ __global__ __launch_bounds__(256) void k_tau_fold(const double* __restrict__ pma
 // block rb writes after the blocks before it (their level-1 counts of the
 // bucket), the four waves of a workgroup claim slots by LDS counters.
 __global__ __launch_bounds__(256) void k_tau_gather(const float* __restrict__ M, long ldm,
-                                                    ElemCtx c, int rows_per_blk,
+                                                    ElemCtx c, int elo, int rows_per_blk,
                                                     const int* __restrict__ bucket,
                                                     const int* __restrict__ off,
                                                     const unsigned int* __restrict__ pcnt,
@@ -515,7 +528,7 @@ __global__ __launch_bounds__(256) void k_tau_gather(const float* __restrict__ M,
             if (ib + 4L * u >= i1) break;
             bool z;
             const float t = Elem<float>::val(av[u] + bj, mv[u], c.k, c.reg, z);
-            if ((static_cast<int>(__float_as_uint(t) >> 25) & (TB - 1)) == B) {
+            if (tau_bucket(__float_as_uint(t), elo) == B) {
                 const int pos = atomicAdd(&lfill[lane], 1);
                 if (pos < cap) cand[o + pos] = t;
             }
@@ -524,9 +537,10 @@ __global__ __launch_bounds__(256) void k_tau_gather(const float* __restrict__ M,
 }
 
 // Radix passes of k_col_threshold over the candidates of a column's boundary
-// bucket (its key bits 24..0: 9, 8 and 8 bits), from the bucket's base mass.
-// If rounding in the re-summed masses leaves no bucket passing delta, the
-// largest candidate key is taken (the entry whose mass makes the sum pass).
+// bucket (their full keys: 11-, 11- and 10-bit digits), from the bucket's
+// base mass.  If rounding in the re-summed masses leaves no bucket passing
+// delta, the largest candidate key is taken (the entry whose mass makes the
+// sum pass).
 __global__ __launch_bounds__(256) void k_tau_select(const int* __restrict__ bucket,
                                                     const double* __restrict__ base_in,
                                                     const int* __restrict__ count,
@@ -534,28 +548,28 @@ __global__ __launch_bounds__(256) void k_tau_select(const int* __restrict__ buck
                                                     const float* __restrict__ cand, double delta,
                                                     double* __restrict__ tau)
 {
-    __shared__ double hist[512];
+    __shared__ double hist[RB];
     __shared__ unsigned int kmax_s[4];
     const int j = blockIdx.x;
     const int B = bucket[j];
     if (B < 0) return;
     const float* cj = cand + off[j];
     const int cn = count[j];
-    unsigned int prefix = static_cast<unsigned int>(B);
+    unsigned int prefix = 0u;
     double base = base_in[j];
-    const int shifts[3] = {16, 8, 0};
-    const int widths[3] = {9, 8, 8};
     bool none = false;
     unsigned int kmax = 0u;
+#pragma unroll 1
     for (int p = 0; p < 3; p++) {
-        const int shift = shifts[p], width = widths[p];
-        for (int q = threadIdx.x; q < 512; q += 256) hist[q] = 0.0;
+        const int shift = p == 0 ? 21 : (p == 1 ? 10 : 0);
+        const int width = p == 2 ? 10 : 11;
+        for (int q = threadIdx.x; q < RB; q += 256) hist[q] = 0.0;
         __syncthreads();
         for (int e = threadIdx.x; e < cn; e += 256) {
             const float t = cj[e];
             const unsigned int key = __float_as_uint(t) & 0x7fffffffu;
             if (p == 0) kmax = key > kmax ? key : kmax;
-            if ((key >> (shift + width)) == prefix)
+            if (p == 0 || (key >> (shift + width)) == prefix)
                 atomicAdd(&hist[(key >> shift) & ((1u << width) - 1u)], static_cast<double>(t));
         }
         __syncthreads();
@@ -1403,7 +1417,15 @@ void threshold_tau_rows(const void* M, long ldm, const ElemCtx& c, double delta,
     H.ltmp.ensure(4);
     H.hll.ensure(4);
     const float* Mf = static_cast<const float*>(M);
-    hipLaunchKernelGGL(k_tau_hist, dim3(strips, nrb), dim3(256), 0, s, Mf, ldm, c, rpb,
+    // the biased FP32 exponent of delta / n: the binade below which no tau_j lies
+    int elo = 0;
+    {
+        const float lo = static_cast<float>(delta / n);
+        unsigned int bits;
+        std::memcpy(&bits, &lo, sizeof bits);
+        elo = (std::isfinite(lo) && lo > 0.f) ? static_cast<int>((bits >> 23) & 255u) : 0;
+    }
+    hipLaunchKernelGGL(k_tau_hist, dim3(strips, nrb), dim3(256), 0, s, Mf, ldm, c, elo, rpb,
                        H.tau_pm.p, H.tau_pc.p, H.tau_px.p);
     hipLaunchKernelGGL(k_tau_fold, dim3(ceil_div(m1, 4)), dim3(256), 0, s, H.tau_pm.p,
                        H.tau_pc.p, H.tau_px.p, nrb, m1, delta, find_small, H.tau_bkt.p,
@@ -1418,7 +1440,8 @@ void threshold_tau_rows(const void* M, long ldm, const ElemCtx& c, double delta,
         fail(E_INVALID, "threshold rule: %lld boundary candidates (> 2^31)", total);
     if (total > 0) {
         H.tau_cand.ensure(static_cast<size_t>(total));
-        hipLaunchKernelGGL(k_tau_gather, dim3(strips, nrb), dim3(256), 0, s, Mf, ldm, c, rpb,
+        hipLaunchKernelGGL(k_tau_gather, dim3(strips, nrb), dim3(256), 0, s, Mf, ldm, c, elo,
+                           rpb,
                            H.tau_bkt.p, H.tau_off.p, H.tau_pc.p, H.tau_cand.p);
         hipLaunchKernelGGL(k_tau_select, dim3(m1), dim3(256), 0, s, H.tau_bkt.p, H.tau_base.p,
                            H.tau_cnt.p, H.tau_off.p, H.tau_cand.p, delta, tau);
