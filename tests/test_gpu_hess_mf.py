@@ -1,11 +1,12 @@
 """Matrix-free sparsified Hessian (FP32 bulk) against the stored CSR / CSC.
 
-The matrix-free build keeps the kept set of the threshold rule (or every entry,
-the full Hessian) as a bitmask and recomputes sigma from M in each product, so
-on one handle and one dual point both storages must give the same counts and
-the same products up to the order of the FP64 sums; the CG solves and the
-drivers that use them must agree to their own tolerances.  The stored form is
-checked against the oracle in test_gpu_hessian.py.
+The matrix-free build keeps the kept entries of the threshold rule (or every
+entry, the full Hessian) as a dense FP32 matrix with zeros where an entry is
+dropped, written by the pass that decides the kept set, and every product reads
+it once; so on one handle and one dual point both storages must give the same
+counts and the same products up to the order of the FP64 sums; the CG solves
+and the drivers that use them must agree to their own tolerances.  The stored
+form is checked against the oracle in test_gpu_hessian.py.
 """
 import numpy as np
 import pytest
