@@ -144,13 +144,28 @@ def bench_batched(P, steps, tol, sequential=True):
     t0 = time.perf_counter()
     og = B.solve_bcd(tol=tol, max_iter=100000, autotune=True, graph=True)
     dt_graph = time.perf_counter() - t0
+    # the same problems on two-pass rounds only (two reads of M per iteration)
+    B.solve_bcd(tol=tol, max_iter=100000, autotune=True, single_read=False)
+    t0 = time.perf_counter()
+    o2 = B.solve_bcd(tol=tol, max_iter=100000, autotune=True, single_read=False)
+    dt_two = time.perf_counter() - t0
     it = out["niter"].astype(np.float64)
-    streamed = float((it * ns * ms).sum()) * 2 * 4
+    # algorithmic bytes: one read of M per BCD iteration; streamed: what the
+    # rounds read (once per iteration on single-read rounds, twice otherwise)
+    alg = float((it * ns * ms).sum()) * 4
+    reads = np.where(out["single_read"], 1.0, 2.0)
+    streamed = float((it * ns * ms * reads).sum()) * 4
+    it2 = o2["niter"].astype(np.float64)
     res = {"problems": P, "elements": int((ns * ms).sum()), "tol": tol,
            "ms_per_batch": dt * 1e3, "solves_per_s": P / dt, "rounds": out["rounds"],
            "niter_median": float(np.median(it)), "niter_max": int(it.max()),
            "mar_err_max": float(out["mar_err"].max()),
+           "single_read_problems": int(out["single_read"].sum()),
+           "alg_GB": alg / 1e9, "alg_GBps": alg / dt / 1e9, "hbm_frac": alg / dt / 8e12,
            "streamed_GB": streamed / 1e9, "streamed_GBps": streamed / dt / 1e9,
+           "two_pass": {"ms_per_batch": dt_two * 1e3,
+                        "streamed_GBps": float((it2 * ns * ms).sum()) * 8 / dt_two / 1e9,
+                        "niter_max_diff": int(np.abs(it2 - it).max())},
            "tiles": {"rows_per_wave": out["rows_per_wave"], "block_rows": out["block_rows"],
                      "autotuned": out["tuned"], "candidate_solve_ms": out["tune_ms"]},
            "graph": {"ms_per_batch": dt_graph * 1e3,

@@ -270,14 +270,26 @@ int regot_batch_destroy(regot_batch* batch);
  *                       {1, 2} FP64 x column-block rows {256, 512, 1024}) and
  *                       keeps the fastest for the handle; 0 (default): fixed
  *                       rules (4 / 2 rows per wave, 512 rows).  Results depend
- *                       on the tiles at rounding level. */
+ *                       on the tiles at rounding level.
+ * REGOT_BATCH_FUSED     1 (default): FP32 problems with 1024 < m <= 4096 and
+ *                       min a min b >= 2^-60 sum a sum b run single-read
+ *                       rounds (one read of M per BCD iteration, the
+ *                       iteration of the single-problem solver's single-read
+ *                       pass); 0: every problem runs two-pass rounds (row
+ *                       tiles, then column tiles).  Results agree to rounding
+ *                       (the sums run in another order). */
 #define REGOT_BATCH_GRAPH 0
 #define REGOT_BATCH_AUTOTUNE 1
+#define REGOT_BATCH_FUSED 2
 int regot_batch_set_option(regot_batch* batch, int option, long long value);
-/* out[11]: rows per wave, column-block rows, tuned (0/1), graph
- * instantiations, graph launches, then the six candidates' solve times (ms;
- * 0 for candidates not run). */
+/* out[12]: rows per wave, column-block rows, tuned (0/1), graph
+ * instantiations, graph launches, the six candidates' solve times (ms; 0 for
+ * candidates not run), then the number of problems on single-read rounds. */
 int regot_batch_stats(const regot_batch* batch, double* out);
+/* kind[p] = 1 if problem p runs single-read rounds (one read of its cost
+ * matrix per iteration), 0 if two-pass rounds (two reads), under the current
+ * options; kind has nprob entries. */
+int regot_batch_round_kinds(const regot_batch* batch, int* kind);
 /* Number of problems, total n*m elements, and the packed dual length
  * sum_p (n[p] + m[p] - 1). */
 int regot_batch_info(const regot_batch* batch, int* nprob, long long* total_elems,

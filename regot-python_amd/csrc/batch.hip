@@ -60,6 +60,11 @@ struct Desc {
     int ct0, nct;       // combine tiles of p: residual slots (2 parities)
     double reg, k;      // k = log2(e) / reg
     double sum_a, sum_b1;
+    // single-read rounds (fused = 1): nft tiles of fr whole rows, column
+    // chunks fnc = ceil(m / 1024), column sums at fpoff (nft x m doubles);
+    // then nrt = nft and the tiles' row partials take the row tiles' slots
+    int fused, fnc, fr, nft;
+    long fpoff;
 };
 
 struct Vecs {
@@ -71,7 +76,8 @@ struct Vecs {
     double* ap1;
     double* beta;
     double* rpart;      // 3 per row tile
-    double* part;       // column partials
+    double* part;       // column partials (log2 sums, two-pass rounds)
+    double* fpart;      // column sums of the single-read tiles
     double* cpart;      // 2 per combine tile per parity
     double* bm;         // beta_k[m-1] per problem (stored by row tile 0)
     int* done;          // per problem
@@ -155,6 +161,326 @@ __global__ __launch_bounds__(256) void k_batch_colpart(const TM* __restrict__ M,
                                 v.part + d.poff + static_cast<long>(tl.z) * d.m, nt != 0);
 }
 
+// ---------------------------------------------------------------------------
+// Single-read round (FP32, 1024 < m <= 4096): one BCD iteration in one read of
+// M.  alpha_{k+1,i} needs only row i, and the beta half-step needs only
+// alpha_{k+1,i} for the rows it sums, so tile (p, t) owns rows
+// t fr .. t fr + fr - 1 of problem p whole: thread th holds the 4-column
+// vectors at 4 th + 1024 c (c < fnc) and beta_{k,j} k for them in FP64
+// registers, and per row
+//
+//   e_ij  = 2^((beta_j - M_ij) k - sh_i)   (FP64 argument, FP32 exp2)
+//   tot_i = sum_j e_ij                     (FP32 per thread, FP64 across)
+//   alpha_{k+1,i} = reg (log a_i - ln2 (sh_i + log2 tot_i))
+//   u_ij  = e_ij a_i / tot_i = exp((alpha_{k+1,i} + beta_j - M_ij) / reg)
+//
+// u is the plan at (alpha_{k+1}, beta_k): its FP32 column accumulators move
+// into FP64 LDS accumulators every 16 rows and the tile writes one FP64
+// column sum per column; the combine adds the tiles' sums in tile order.
+// sh_i = log2 a_i - k alpha_{k,i} makes tot_i = r_i / a_i (about 1 near the
+// solution); a row whose tot_i leaves [2^-20, 2^20] is redone with its
+// largest term as the shift.  The host puts a problem on these rounds only
+// when min a min b >= 2^-60 sum a sum b, so the terms flushed below 2^-126
+// (u_ij <= a_i) drop less than 2^-46 of any column sum (which is at least
+// b_j min a / sum b after an exact beta update, as every round's beta is).
+// ---------------------------------------------------------------------------
+constexpr int kBFThreads = 256;
+constexpr int kBFWaves = kBFThreads / 64;
+constexpr int kBFSpan = 4 * kBFThreads;   // columns per vector step
+constexpr int kBFMaxNC = 4;               // m <= 4 * 1024
+constexpr int kBFMinM = 1025;             // narrower problems keep two-pass rounds
+constexpr int kBFMaxRows = 128;           // rows per tile (the row record below)
+constexpr int kBFMinRows = 16;
+constexpr int kBFTiles = 4096;            // tile count aimed at once tiles shrink
+constexpr int kBFFlush = 16;              // rows between FP32 -> FP64 flushes
+constexpr double kBFTotLo = 0x1p-20;
+constexpr double kBFTotHi = 0x1p20;
+
+// Sum / maximum of the 4 wave values v[] in a fixed butterfly order, bitwise
+// the same on every thread.
+__device__ __forceinline__ double bf_sum_waves(const double* v, int lane)
+{
+    double x = v[lane & (kBFWaves - 1)];
+#pragma unroll
+    for (int o = kBFWaves / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kBFWaves);
+    return __shfl(x, 0);
+}
+
+__device__ __forceinline__ double bf_max_waves(const double* v, int lane)
+{
+    double x = v[lane & (kBFWaves - 1)];
+#pragma unroll
+    for (int o = kBFWaves / 2; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, kBFWaves));
+    return __shfl(x, 0);
+}
+
+// One cross-lane step of an FP64 value by a DPP move of each half.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// R wave sums together: lane pairs xor 1 and xor 2 (quad permutes), then the
+// mirrored halves of each 8 and each 16 lanes (DPP), then xor 16 and xor 32
+// (shuffles issued for all R values before their adds).  Each step adds two
+// values that are equal on every lane of a group in either order, so every
+// lane ends with the same bits.
+template <int R>
+__device__ __forceinline__ void wave_sums(double (&x)[R])
+{
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += dpp_f64<0xB1>(x[r]);    // quad_perm [1,0,3,2]
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += dpp_f64<0x4E>(x[r]);    // quad_perm [2,3,0,1]
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += dpp_f64<0x141>(x[r]);   // row_half_mirror
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += dpp_f64<0x140>(x[r]);   // row_mirror
+    double y[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) y[r] = __shfl_xor(x[r], 16);
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += y[r];
+#pragma unroll
+    for (int r = 0; r < R; r++) y[r] = __shfl_xor(x[r], 32);
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += y[r];
+}
+
+// Rows per step for NC column chunks: a step's rows arrive together (about
+// 32 KB of M per workgroup) and are reduced with one barrier.
+template <int NC>
+constexpr int kBFRows = NC == 1 ? 8 : (NC == 2 ? 4 : (NC == 3 ? 3 : 2));
+
+// Thread th's NC 4-column vectors of one row.  Every load is issued (the
+// compiler then counts the loads in flight exactly): a vector that would
+// start at or past m reads the row's first vector instead, and the caller
+// masks it.  A vector that starts below m ends inside the row pitch (a
+// multiple of 64 floats).
+template <int NC, bool NT>
+__device__ __forceinline__ void bf_load(const float* __restrict__ Mi, const int* off,
+                                        float4* out)
+{
+#pragma unroll
+    for (int c = 0; c < NC; c++) out[c] = ld_m(Mi + off[c], NT);
+}
+
+// NT: nontemporal loads (the active matrices exceed the Infinity Cache).
+template <int NC, bool NT>
+__global__ __launch_bounds__(kBFThreads) void k_batch_fused(const float* __restrict__ M,
+                                                            const Desc* __restrict__ D,
+                                                            const int2* __restrict__ tiles,
+                                                            Vecs v, const int* __restrict__ kp)
+{
+    constexpr int R = kBFRows<NC>;
+    constexpr int FLUSH = kBFFlush / R;   // steps between FP32 -> FP64 flushes
+    const int2 tl = tiles[blockIdx.x];
+    const int p = tl.x;
+    if (v.done[p]) return;
+    const int k = *kp;
+    const Desc d = D[p];
+    __shared__ double acc[4 * NC][kBFThreads];
+    __shared__ double rsh[kBFMaxRows], rtot[kBFMaxRows], ra[kBFMaxRows];
+    __shared__ double red[2][R][kBFWaves];
+    __shared__ double redx[2][kBFWaves];
+
+    const int th = threadIdx.x;
+    const int lane = th & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(th >> 6);
+    const int m = d.m;
+    const long ldm = d.ldm;
+    const long r0 = static_cast<long>(tl.y) * d.fr;
+    const long r1 = (r0 + d.fr < d.n) ? r0 + d.fr : d.n;
+    const float* Mp = M + d.moff;
+    const double kk = d.k;
+    const double* alpha = ((k & 1) ? v.ap1 : v.ap0) + d.aoff;   // alpha_k
+    double* aout = ((k & 1) ? v.ap0 : v.ap1) + d.aoff;          // alpha_{k+1}
+    const double* loga = v.loga + d.aoff;
+    const double* a = v.a + d.aoff;
+    const double* beta = v.beta + d.boff;
+
+    // beta_j k, or -inf for columns at or past m: their terms are 0 (M is
+    // finite there: row padding, or the row's first vector)
+    int off[NC];
+    double bk[NC][4];
+    float acc32[NC][4];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int j0 = 4 * th + c * kBFSpan;
+        off[c] = j0 < m ? j0 : 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            bk[c][q] = j0 + q < m ? beta[j0 + q] * kk : -INFINITY;
+            acc32[c][q] = 0.f;
+            acc[4 * c + q][th] = 0.0;   // each thread's own slots: no barrier needed
+        }
+    }
+    // the rows' shifts and marginals in LDS: inside the row loop only the
+    // loads of M go to memory
+    for (int rr = th; rr < r1 - r0; rr += kBFThreads) {
+        const long i = r0 + rr;
+        rsh[rr] = loga[i] * kLog2e - alpha[i] * kk;
+        ra[rr] = a[i];
+    }
+    __syncthreads();
+
+    // the next step's rows are in flight while a step is reduced (past the
+    // tile's end a load repeats the tile's last row)
+    float4 pf[R][NC];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const long ir = (r0 + r < r1) ? r0 + r : r1 - 1;
+        bf_load<NC, NT>(Mp + ir * ldm, off, pf[r]);
+    }
+
+    int step = 0;
+#pragma unroll 1
+    for (long i0 = r0; i0 < r1; i0 += R, step++) {
+        float e[R][NC][4];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) e[r][c][q] = vget<float4, float>(pf[r][c], q);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const long ir = (i0 + R + r < r1) ? i0 + R + r : r1 - 1;
+            bf_load<NC, NT>(Mp + ir * ldm, off, pf[r]);
+        }
+
+        // e_ij in place of M_ij; rows past r1 (the same for the whole
+        // workgroup) contribute nothing
+        double sh[R];
+        float s[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const bool rv = i0 + r < r1;
+            sh[r] = rv ? rsh[i0 + r - r0] : 0.0;
+            s[r] = 0.f;
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const float x = rv ? static_cast<float>(fma(-kk,
+                                                                static_cast<double>(e[r][c][q]),
+                                                                bk[c][q] - sh[r]))
+                                       : ninf<float>();
+                    e[r][c][q] = ex2(x);
+                    s[r] += e[r][c][q];
+                }
+        }
+        const int slot = step & 1;
+        double S[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) S[r] = static_cast<double>(s[r]);
+        wave_sums<R>(S);
+        if (lane == 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) red[slot][r][wave] = S[r];
+        }
+        __syncthreads();
+
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const long i = i0 + r;
+            if (i >= r1) break;   // the same for the whole workgroup
+            // the four wave totals in a fixed order (the same on every thread)
+            double tot = ((red[slot][r][0] + red[slot][r][1]) + red[slot][r][2]) + red[slot][r][3];
+            if (!(tot >= kBFTotLo && tot <= kBFTotHi)) {
+                // Rare (first rounds): shift by the row's exact largest term.
+                // tot is the same on every thread, so the whole workgroup is
+                // here; the row is read again.
+                float4 rw[NC];
+#pragma unroll
+                for (int c = 0; c < NC; c++) rw[c] = ld_m(Mp + i * ldm + off[c], false);
+                double tm = -INFINITY;
+#pragma unroll
+                for (int c = 0; c < NC; c++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        tm = fmax(tm, fma(-kk, static_cast<double>(vget<float4, float>(rw[c], q)),
+                                          bk[c][q]));
+                const double dmw = wave_max(tm);
+                if (lane == 0) redx[0][wave] = dmw;
+                __syncthreads();
+                const double dm = bf_max_waves(redx[0], lane);
+                tot = 0.0;
+                if (dm > -INFINITY && dm < INFINITY) {
+                    sh[r] = dm;   // log2 of the row's largest term
+                    float s2 = 0.f;
+#pragma unroll
+                    for (int c = 0; c < NC; c++)
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const float x = static_cast<float>(
+                                fma(-kk, static_cast<double>(vget<float4, float>(rw[c], q)),
+                                    bk[c][q] - sh[r]));
+                            e[r][c][q] = ex2(x);
+                            s2 += e[r][c][q];
+                        }
+                    const double S2 = wave_sum0(static_cast<double>(s2));
+                    if (lane == 0) redx[1][wave] = S2;
+                    __syncthreads();
+                    tot = bf_sum_waves(redx[1], lane);
+                }
+            }
+            // rsh[i - r0] was read by every thread before the step's barrier
+            if (th == 0) {
+                rsh[i - r0] = sh[r];
+                rtot[i - r0] = tot;
+            }
+            const float wi = tot > 0.0 ? static_cast<float>(ra[i - r0] / tot) : 0.f;
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc32[c][q] = fmaf(e[r][c][q], wi, acc32[c][q]);
+        }
+        if ((step % FLUSH) == FLUSH - 1) {
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    acc[4 * c + q][th] += static_cast<double>(acc32[c][q]);
+                    acc32[c][q] = 0.f;
+                }
+        }
+    }
+
+    double* pw = v.fpart + d.fpoff + static_cast<long>(tl.y) * m;
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (4 * th + c * kBFSpan + q < m)
+                pw[4 * th + c * kBFSpan + q] = acc[4 * c + q][th] + acc32[c][q];
+
+    // Row epilogue, rows strided over the threads: alpha_{k+1} and the row
+    // partials of g_k (r_i = sum_j exp((alpha_{k,i} + beta_j - M_ij) / reg)).
+    __syncthreads();
+    double s3[3] = {0.0, 0.0, 0.0};
+    for (int rr = th; rr < r1 - r0; rr += kBFThreads) {
+        const long i = r0 + rr;
+        const double L = rtot[rr] > 0.0 ? rsh[rr] + log2(rtot[rr]) : -INFINITY;
+        const double ali = alpha[i], ai = a[i];
+        aout[i] = d.reg * (loga[i] - kLn2 * L);
+        const double ri = exp2(ali * kk + L);
+        const double g = ri - ai;
+        s3[0] += g * g;
+        s3[1] += ri;
+        s3[2] += ali * ai;
+    }
+    block_sums_256<3>(s3);
+    if (th == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) v.rpart[3L * (d.rt0 + tl.y) + q] = s3[q];
+        if (tl.y == 0) v.bm[p] = beta[m - 1];
+    }
+}
+
 // Stop decision for iterate k of problem p (identical in every tile of p):
 // g_k = [r - a ; residual of the update that produced beta_k].
 __device__ void batch_decide(const Desc& d, const Vecs& v, int p, int k, double tol,
@@ -205,7 +531,32 @@ __global__ __launch_bounds__(256) void k_batch_combine(const Desc* __restrict__ 
     const int sub = threadIdx.x & 3;
     const int j = tl.y * kCombCols + (threadIdx.x >> 2);
     const bool valid = j < d.m;
-    const double L = combine_partials(v.part + d.poff, d.nblk, d.m, j, valid, sub);
+    double L;
+    if (d.fused && !init) {
+        // the single-read tiles' plain column sums of exp((alpha_{k+1,i} +
+        // beta_{k,j} - M_ij) / reg), added in tile order p = sub, sub + 4, ...
+        // (8 loads in flight), so log2 sum_i 2^((alpha_{k+1,i} - M_ij) k) =
+        // log2(sum) - beta_{k,j} k; beta_{k,j} is read before sub 0 of the
+        // same column overwrites it below
+        double cs = 0.0;
+        if (valid) {
+            const double* fp = v.fpart + d.fpoff + j;
+            int q = sub;
+            for (; q + 28 < d.nft; q += 32) {
+                double x[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) x[u] = fp[static_cast<long>(q + 4 * u) * d.m];
+#pragma unroll
+                for (int u = 0; u < 8; u++) cs += x[u];
+            }
+            for (; q < d.nft; q += 4) cs += fp[static_cast<long>(q) * d.m];
+        }
+        cs += __shfl_xor(cs, 1);
+        cs += __shfl_xor(cs, 2);
+        L = (valid && cs > 0.0) ? log2(cs) - v.beta[d.boff + j] * d.k : -INFINITY;
+    } else {
+        L = combine_partials(v.part + d.poff, d.nblk, d.m, j, valid, sub);
+    }
     double r2 = 0.0, bb = 0.0;
     if (sub == 0 && valid) {
         const double lb = v.logb[d.boff + j];
@@ -295,12 +646,12 @@ struct regot_batch {
     long total_n = 0, total_m = 0, total_dual = 0;
     DBuf<unsigned char> M;
     DBuf<Desc> d_desc;
-    DBuf<double> a, b, loga, logb, ap0, ap1, beta, rpart, part, cpart, bm, fval, gval;
+    DBuf<double> a, b, loga, logb, ap0, ap1, beta, rpart, part, fpart, cpart, bm, fval, gval;
     DBuf<int> done, niter, kdev;
-    DBuf<int2> t_row, t_comb;
+    DBuf<int2> t_row, t_comb, t_fus;
     DBuf<int4> t_col;
     HBuf<int> h_done;
-    HBuf<int2> h_row[2], h_comb[2];
+    HBuf<int2> h_row[2], h_comb[2], h_fus[2];
     HBuf<int4> h_col[2];
     HBuf<Desc> h_desc[2];
     int hd_slot = 0;
@@ -315,11 +666,17 @@ struct regot_batch {
     bool started_fine = false;   // the last solve began in fine tiles
     double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool nt = true;      // nontemporal loads of M (active matrices beyond the cache)
+    // single-read rounds: problems that qualify (fused_ok) take them unless
+    // the option turns them off; fpart holds fpart_cap doubles
+    std::vector<char> fused_ok;
+    bool use_fused = true;
+    long fpart_cap = 0;
     // K rounds captured once per tile-table configuration (and tol,
     // max_iter) and replayed
     bool use_graph = false;
     hipGraphExec_t gexec = nullptr;
-    int g_key[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};
+    static constexpr int kGraphKey = 13;
+    int g_key[kGraphKey] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     double g_tol = -1.0;
     long long graph_builds = 0, graph_launches = 0;
 
@@ -344,6 +701,7 @@ struct regot_batch {
         v.beta = beta;
         v.rpart = rpart;
         v.part = part;
+        v.fpart = fpart;
         v.cpart = cpart;
         v.bm = bm;
         v.done = done;
@@ -353,7 +711,7 @@ struct regot_batch {
         return v;
     }
 
-    // Tile granularity of every problem's descriptor (uploaded on change).
+    // Tile granularity of the two-pass tiles in every problem's descriptor.
     void set_mode(bool f)
     {
         for (Desc& d : desc) {
@@ -363,69 +721,150 @@ struct regot_batch {
             d.rpb = ceil_div(d.n, d.nblk);
         }
         fine = f;
-        h_desc[hd_slot].ensure(desc.size());
-        std::memcpy(h_desc[hd_slot].p, desc.data(), sizeof(Desc) * desc.size());
-        HIP_CHECK(hipMemcpyAsync(d_desc.p, h_desc[hd_slot].p, sizeof(Desc) * desc.size(),
-                                 hipMemcpyHostToDevice, stream));
-        hd_slot ^= 1;
     }
 
-    // Tile tables of the listed problems into staging slot s; switches to
-    // fine tiles when the coarse tables would be small.
+    bool takes_fused(int p) const { return use_fused && fused_ok[p]; }
+
+    // Tile tables of the listed problems into staging slot s, and the
+    // descriptors.  Problems on single-read rounds get single-read tiles of
+    // fr rows, fr chosen so that about kBFTiles tiles cover their rows
+    // (kBFMinRows .. kBFMaxRows); the others get row and column tiles, which
+    // switch to fine tiles when the coarse tables would be small.  init_cols:
+    // every problem gets column tiles (the init round computes beta_0 by
+    // column tiles alone).
     void build_tables(const std::vector<int>& act, int s, int* nrow, int* ncol, int* ncomb,
-                      bool refresh = false)
+                      int* nfus, bool init_cols, bool refresh = false)
     {
         // the coarse / fine switch counts tiles of the default rows per wave,
         // so it does not move with the tuned tile shape
         const int R0 = bulk == 0 ? 4 : 2;
-        long ntiles = 0;
+        long ntiles = 0, frows = 0;
         double bytes = 0.0;
         for (int p : act) {
-            ntiles += ceil_div(desc[p].n, 4 * R0);
+            if (takes_fused(p)) frows += desc[p].n;
+            if (!takes_fused(p) || init_cols) ntiles += ceil_div(desc[p].n, 4 * R0);
             bytes += static_cast<double>(desc[p].n) * desc[p].ldm * (bulk == 0 ? 4 : 8);
         }
         nt = bytes > static_cast<double>(256L << 20);
         const bool want_fine = ntiles < kFineBelow;
         if (want_fine != fine || refresh) set_mode(want_fine);
-        long cr = 0, cc = 0, cb = 0;
+        const int fr = static_cast<int>(std::min<long>(
+            kBFMaxRows, std::max<long>(kBFMinRows, ceil_div(frows, static_cast<long>(kBFTiles)))));
+        long cr = 0, cc = 0, cb = 0, cf = 0, foff = 0;
+        long cfc[kBFMaxNC] = {0, 0, 0, 0};
         const int vw = bulk == 0 ? 4 : 2;
         for (int p : act) {
-            const Desc& d = desc[p];
-            cr += d.nrt;
-            cc += static_cast<long>(ceil_div(d.m, 64 * vw)) * d.nblk;
+            Desc& d = desc[p];
+            d.fused = takes_fused(p) ? 1 : 0;
+            if (d.fused) {
+                d.nft = ceil_div(d.n, fr);
+                d.fr = ceil_div(d.n, d.nft);
+                d.fnc = ceil_div(d.m, kBFSpan);
+                d.fpoff = foff;
+                foff += static_cast<long>(d.nft) * d.m;
+                // the single-read tiles' row partials use the row-tile slots
+                d.nrt = d.nft;
+                d.rpt = d.fr;
+                cf += d.nft;
+                cfc[d.fnc - 1] += d.nft;
+            } else {
+                cr += d.nrt;
+            }
+            if (!d.fused || init_cols) cc += static_cast<long>(ceil_div(d.m, 64 * vw)) * d.nblk;
             cb += d.nct;
         }
+        if (foff > fpart_cap)
+            fail(E_INTERNAL, "batched rounds: %ld single-read column sums above the %ld held",
+                 foff, fpart_cap);
         h_row[s].ensure(std::max<long>(cr, 1));
         h_col[s].ensure(std::max<long>(cc, 1));
         h_comb[s].ensure(std::max<long>(cb, 1));
-        long ir = 0, ic = 0, ib = 0;
+        h_fus[s].ensure(std::max<long>(cf, 1));
+        // single-read tiles grouped by column chunks (one launch per group)
+        long ir = 0, ic = 0, ib = 0, jf[kBFMaxNC];
+        jf[0] = 0;
+        for (int c = 1; c < kBFMaxNC; c++) jf[c] = jf[c - 1] + cfc[c - 1];
         for (int p : act) {
             const Desc& d = desc[p];
-            for (int t = 0; t < d.nrt; t++) h_row[s].p[ir++] = make_int2(p, t);
-            const int strips = ceil_div(d.m, 64 * vw);
-            for (int bk = 0; bk < d.nblk; bk++)
-                for (int st = 0; st < strips; st++) h_col[s].p[ic++] = make_int4(p, st, bk, 0);
+            if (d.fused) {
+                for (int t = 0; t < d.nft; t++) h_fus[s].p[jf[d.fnc - 1]++] = make_int2(p, t);
+            } else {
+                for (int t = 0; t < d.nrt; t++) h_row[s].p[ir++] = make_int2(p, t);
+            }
+            if (!d.fused || init_cols) {
+                const int strips = ceil_div(d.m, 64 * vw);
+                for (int bk = 0; bk < d.nblk; bk++)
+                    for (int st = 0; st < strips; st++)
+                        h_col[s].p[ic++] = make_int4(p, st, bk, 0);
+            }
             for (int c = 0; c < d.nct; c++) h_comb[s].p[ib++] = make_int2(p, c);
         }
         *nrow = static_cast<int>(cr);
         *ncol = static_cast<int>(cc);
         *ncomb = static_cast<int>(cb);
+        for (int c = 0; c < kBFMaxNC; c++) nfus[c] = static_cast<int>(cfc[c]);
+        h_desc[hd_slot].ensure(desc.size());
+        std::memcpy(h_desc[hd_slot].p, desc.data(), sizeof(Desc) * desc.size());
+        HIP_CHECK(hipMemcpyAsync(d_desc.p, h_desc[hd_slot].p, sizeof(Desc) * desc.size(),
+                                 hipMemcpyHostToDevice, stream));
+        hd_slot ^= 1;
         HIP_CHECK(hipMemcpyAsync(t_row.p, h_row[s].p, sizeof(int2) * std::max<long>(cr, 1),
                                  hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipMemcpyAsync(t_col.p, h_col[s].p, sizeof(int4) * std::max<long>(cc, 1),
                                  hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipMemcpyAsync(t_comb.p, h_comb[s].p, sizeof(int2) * std::max<long>(cb, 1),
                                  hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(t_fus.p, h_fus[s].p, sizeof(int2) * std::max<long>(cf, 1),
+                                 hipMemcpyHostToDevice, stream));
+    }
+
+    template <bool NT>
+    void launch_fused(int nc, int cnt, const float* Mf, const int2* tb, const Vecs& v,
+                      const int* kp)
+    {
+        const dim3 g(cnt), b(kBFThreads);
+        switch (nc) {
+        case 1:
+            hipLaunchKernelGGL((k_batch_fused<1, NT>), g, b, 0, stream, Mf, d_desc.p, tb, v, kp);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_batch_fused<2, NT>), g, b, 0, stream, Mf, d_desc.p, tb, v, kp);
+            break;
+        case 3:
+            hipLaunchKernelGGL((k_batch_fused<3, NT>), g, b, 0, stream, Mf, d_desc.p, tb, v, kp);
+            break;
+        default:
+            hipLaunchKernelGGL((k_batch_fused<4, NT>), g, b, 0, stream, Mf, d_desc.p, tb, v, kp);
+            break;
+        }
+        KERNEL_CHECK();
     }
 
     // One BCD round (round index from the device counter, advanced at its end).
     template <typename TM>
-    void launch_round(int init, int nrow, int ncol, int ncomb, double tol, int max_iter)
+    void launch_round(int init, int nrow, int ncol, int ncomb, const int* nfus, double tol,
+                      int max_iter)
     {
         const TM* Mp = reinterpret_cast<const TM*>(M.p);
         const Vecs v = vecs();
         const int ntf = nt ? 1 : 0;
         const int* kp = kdev.p;
+        if (!init) {
+            const float* Mf = reinterpret_cast<const float*>(M.p);
+            long base = 0;
+            for (int c = 0; c < kBFMaxNC; c++) {
+                const int cnt = nfus[c];
+                if (cnt > 0) {
+                    if (sizeof(TM) != 4) fail(E_INTERNAL, "single-read batched round on FP64 bulk");
+                    const int2* tb = t_fus.p + base;
+                    if (nt)
+                        launch_fused<true>(c + 1, cnt, Mf, tb, v, kp);
+                    else
+                        launch_fused<false>(c + 1, cnt, Mf, tb, v, kp);
+                }
+                base += cnt;
+            }
+        }
         if (!init && nrow > 0) {
             const int R = fine ? 1 : coarse_R;
             if (R == 1)
@@ -453,12 +892,13 @@ struct regot_batch {
         KERNEL_CHECK();
     }
 
-    void round(int init, int nrow, int ncol, int ncomb, double tol, int max_iter)
+    void round(int init, int nrow, int ncol, int ncomb, const int* nfus, double tol,
+               int max_iter)
     {
         if (bulk == 0)
-            launch_round<float>(init, nrow, ncol, ncomb, tol, max_iter);
+            launch_round<float>(init, nrow, ncol, ncomb, nfus, tol, max_iter);
         else
-            launch_round<double>(init, nrow, ncol, ncomb, tol, max_iter);
+            launch_round<double>(init, nrow, ncol, ncomb, nfus, tol, max_iter);
     }
 
     // Reference initial point of every problem: alpha_0 = 0, flags cleared,
@@ -481,6 +921,13 @@ struct regot_batch {
     // (it starts in fine tiles) keeps the defaults.
     void tune(double tol, int max_iter)
     {
+        // the candidates shape only the two-pass tiles
+        bool any_two_pass = false;
+        for (int p = 0; p < nprob; p++) any_two_pass = any_two_pass || !takes_fused(p);
+        if (!any_two_pass) {
+            tuned = true;
+            return;
+        }
         const int R32[2] = {2, 4}, R64[2] = {1, 2};
         const int rows[3] = {256, 512, 1024};
         double best = 1e300;
@@ -519,10 +966,15 @@ struct regot_batch {
         std::vector<int> act(np);
         for (int p = 0; p < np; p++) act[p] = p;
         int slot = 0, nrow = 0, ncol = 0, ncomb = 0;
-        build_tables(act, slot, &nrow, &ncol, &ncomb, true);
+        int nfus[kBFMaxNC] = {0, 0, 0, 0};
+        build_tables(act, slot, &nrow, &ncol, &ncomb, nfus, true, true);
         started_fine = fine;
-        // beta_0 = optimal_beta(alpha_0 = 0) and its residual (parity 0)
-        round(1, nrow, ncol, ncomb, tol, max_iter);
+        // beta_0 = optimal_beta(alpha_0 = 0) and its residual (parity 0), by
+        // column tiles for every problem
+        round(1, nrow, ncol, ncomb, nfus, tol, max_iter);
+        // the rounds' tables: single-read problems leave the column tiles
+        slot ^= 1;
+        build_tables(act, slot, &nrow, &ncol, &ncomb, nfus, false);
 
         // K rounds per batch, two batches in flight.  Every problem is stopped
         // by round max_iter, so rounds past it (the tail of the last batch)
@@ -531,7 +983,7 @@ struct regot_batch {
         int k = 0;
         long long nround = 0;
         auto launch_batch = [&](int e) {
-            rounds(K, nrow, ncol, ncomb, tol, max_iter);
+            rounds(K, nrow, ncol, ncomb, nfus, tol, max_iter);
             const int work = std::min(K, max_iter + 1 - k);
             nround += work;
             k += work;
@@ -564,7 +1016,7 @@ struct regot_batch {
                 if (still.size() != act.size()) {
                     act.swap(still);
                     slot ^= 1;
-                    build_tables(act, slot, &nrow, &ncol, &ncomb);
+                    build_tables(act, slot, &nrow, &ncol, &ncomb, nfus, false);
                 }
                 launch_batch(e);
                 q.push_back(e);
@@ -582,15 +1034,17 @@ struct regot_batch {
     // descriptors are device buffers refreshed outside it) and replayed;
     // the round index lives on the device, so replays need no parameter
     // updates.  The rounds contain kernels only (no memset nodes).
-    void rounds(int K, int nrow, int ncol, int ncomb, double tol, int max_iter)
+    void rounds(int K, int nrow, int ncol, int ncomb, const int* nfus, double tol,
+                int max_iter)
     {
         if (!use_graph) {
-            for (int q = 0; q < K; q++) round(0, nrow, ncol, ncomb, tol, max_iter);
+            for (int q = 0; q < K; q++) round(0, nrow, ncol, ncomb, nfus, tol, max_iter);
             return;
         }
-        const int key[9] = {K,         nrow,     ncol,     ncomb,      fine ? 1 : 0,
-                            nt ? 1 : 0, max_iter, coarse_R, coarse_rows};
-        if (!gexec || !std::equal(key, key + 9, g_key) || tol != g_tol) {
+        const int key[kGraphKey] = {K,          nrow,     ncol,     ncomb,       fine ? 1 : 0,
+                                    nt ? 1 : 0, max_iter, coarse_R, coarse_rows, nfus[0],
+                                    nfus[1],    nfus[2],  nfus[3]};
+        if (!gexec || !std::equal(key, key + kGraphKey, g_key) || tol != g_tol) {
             if (gexec) {
                 HIP_CHECK(hipGraphExecDestroy(gexec));
                 gexec = nullptr;
@@ -598,7 +1052,7 @@ struct regot_batch {
             hipGraph_t g = nullptr;
             HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             try {
-                for (int q = 0; q < K; q++) round(0, nrow, ncol, ncomb, tol, max_iter);
+                for (int q = 0; q < K; q++) round(0, nrow, ncol, ncomb, nfus, tol, max_iter);
             } catch (...) {
                 (void)hipStreamEndCapture(stream, &g);
                 if (g) (void)hipGraphDestroy(g);
@@ -608,7 +1062,7 @@ struct regot_batch {
             const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             HIP_CHECK(e);
-            std::copy(key, key + 9, g_key);
+            std::copy(key, key + kGraphKey, g_key);
             g_tol = tol;
             graph_builds++;
         }
@@ -656,6 +1110,8 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
     const int R = bulk == REGOT_F32 ? 4 : 2;
     B->desc.resize(nprob);
     long moff = 0, aoff = 0, boff = 0, rt = 0, poff = 0, ct = 0;
+    long fbig = 0, fsum_m = 0;
+    int fmax_m = 0;
     for (int p = 0; p < nprob; p++) {
         if (n[p] < 1 || m[p] < 1 || !M[p] || !a[p] || !b[p] || !(reg[p] > 0.0))
             fail(E_INVALID, "problem %d: bad sizes, null pointer or reg <= 0", p);
@@ -685,13 +1141,36 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
         ct += d.nct;
         d.reg = reg[p];
         d.k = 1.4426950408889634074 / reg[p];
-        double sa = 0.0, sb = 0.0;
-        for (int i = 0; i < d.n; i++) sa += a[p][i];
-        for (int j = 0; j < d.m - 1; j++) sb += b[p][j];
+        double sa = 0.0, sb = 0.0, sbf = 0.0, mina = a[p][0], minb = b[p][0];
+        for (int i = 0; i < d.n; i++) {
+            sa += a[p][i];
+            mina = std::min(mina, a[p][i]);
+        }
+        for (int j = 0; j < d.m; j++) {
+            if (j < d.m - 1) sb += b[p][j];
+            sbf += b[p][j];
+            minb = std::min(minb, b[p][j]);
+        }
         d.sum_a = sa;
         d.sum_b1 = sb;
+        d.fused = d.fnc = d.fr = d.nft = 0;
+        d.fpoff = 0;
+        // single-read rounds: FP32 bulk, 1024 < m <= 4096, and every column
+        // sum far above the terms the FP32 exponential flushes
+        const bool fok = bulk == REGOT_F32 && d.m >= kBFMinM && d.m <= kBFMaxNC * kBFSpan &&
+                         mina > 0.0 && minb > 0.0 && mina * minb >= 0x1p-60 * sa * sbf;
+        B->fused_ok.push_back(fok ? 1 : 0);
+        if (fok) {
+            fbig += static_cast<long>(ceil_div(d.n, kBFMaxRows)) * d.m;
+            fsum_m += d.m;
+            fmax_m = std::max(fmax_m, d.m);
+        }
         B->total_elems += static_cast<long long>(d.n) * d.m;
     }
+    // Single-read column sums: with fr = kBFMaxRows the tables hold at most
+    // fbig; below it fr >= max(kBFMinRows, rows / kBFTiles) bounds the tiles'
+    // column sums by kBFTiles max m + sum m (build_tables checks the bound).
+    B->fpart_cap = std::max(fbig, static_cast<long>(kBFTiles) * fmax_m + fsum_m);
     if (aoff > 0x7fffffffL || boff > 0x7fffffffL || rt > 0x7fffffffL)
         fail(E_INVALID, "batch too large (sum of n or m above 2^31)");
     B->total_n = aoff;
@@ -750,6 +1229,7 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
     B->beta.alloc(boff);
     B->rpart.alloc(3 * rt);
     B->part.alloc(poff);
+    B->fpart.alloc(std::max(B->fpart_cap, 1L));
     B->cpart.alloc(4 * ct);
     B->bm.alloc(nprob);
     B->fval.alloc(nprob);
@@ -772,6 +1252,10 @@ int regot_batch_create(regot_batch** out, int nprob, const int* n, const int* m,
     B->t_row.alloc(tr);
     B->t_col.alloc(tc);
     B->t_comb.alloc(tb);
+    // single-read tiles: at most ceil(n / kBFMinRows) per problem
+    long tf = 0;
+    for (const Desc& d : B->desc) tf += ceil_div(d.n, kBFMinRows);
+    B->t_fus.alloc(tf);
     B->h_done.ensure(2L * nprob);
     for (auto& e : B->ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(hipStreamSynchronize(B->stream));
@@ -786,6 +1270,9 @@ int regot_batch_set_option(regot_batch* B, int option, long long value)
     switch (option) {
         case REGOT_BATCH_GRAPH:
             B->use_graph = value != 0;
+            break;
+        case REGOT_BATCH_FUSED:
+            B->use_fused = value != 0;
             break;
         case REGOT_BATCH_AUTOTUNE:
             B->autotune = value != 0;
@@ -811,6 +1298,17 @@ int regot_batch_stats(const regot_batch* B, double* out)
     out[3] = static_cast<double>(B->graph_builds);
     out[4] = static_cast<double>(B->graph_launches);
     for (int c = 0; c < 6; c++) out[5 + c] = B->tune_ms[c];
+    int nf = 0;
+    for (int p = 0; p < B->nprob; p++) nf += B->takes_fused(p) ? 1 : 0;
+    out[11] = nf;
+    CAPI_END
+}
+
+int regot_batch_round_kinds(const regot_batch* B, int* kind)
+{
+    CAPI_BEGIN
+    if (!B || !kind) fail(E_INVALID, "null argument");
+    for (int p = 0; p < B->nprob; p++) kind[p] = B->takes_fused(p) ? 1 : 0;
     CAPI_END
 }
 

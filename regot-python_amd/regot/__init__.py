@@ -603,17 +603,21 @@ class BatchProblem:
         check(lib().regot_batch_info(h, None, _lib.ctypes.byref(tot), _lib.ctypes.byref(dl)))
         self.total_elems, self.dual_len = tot.value, dl.value
 
-    def solve_bcd(self, tol=1e-6, max_iter=1000, dual=True, graph=False, autotune=False):
+    def solve_bcd(self, tol=1e-6, max_iter=1000, dual=True, graph=False, autotune=False,
+                  single_read=True):
         """BCD on every problem.  Returns a dict of per-problem arrays:
         ``niter``, ``obj`` and ``mar_err`` (at the final iterate), ``dual``
         (a list of (n+m-1,) arrays, or None), ``rounds`` launched, the
         hipGraph instantiations / launches so far (``graph=True`` replays
         captured groups of 8 rounds instead of launching the rounds' kernels
-        one by one; results are the same) and the coarse
+        one by one; results are the same), the coarse
         tile configuration (``autotune=True`` times the candidates once per
-        handle and keeps the fastest)."""
+        handle and keeps the fastest) and ``single_read``, per problem whether
+        its rounds read its cost matrix once per iteration (FP32 problems
+        with 1024 < m <= 4096 when ``single_read=True``) or twice."""
         check(lib().regot_batch_set_option(self._h, 0, 1 if graph else 0))
         check(lib().regot_batch_set_option(self._h, 1, 1 if autotune else 0))
+        check(lib().regot_batch_set_option(self._h, 2, 1 if single_read else 0))
         P = self.nprob
         it = np.zeros(P, dtype=np.int32)
         ob = np.zeros(P)
@@ -623,12 +627,14 @@ class BatchProblem:
         check(lib().regot_batch_solve_bcd(self._h, float(tol), int(max_iter), _lib.iptr(it),
                                           dptr(ob), dptr(me), dptr(du) if dual else None,
                                           _lib.ctypes.byref(rounds)))
-        st = np.zeros(11)
+        st = np.zeros(12)
         check(lib().regot_batch_stats(self._h, dptr(st)))
+        kinds = np.zeros(P, dtype=np.int32)
+        check(lib().regot_batch_round_kinds(self._h, _lib.iptr(kinds)))
         out = {"niter": it, "obj": ob, "mar_err": me, "rounds": rounds.value, "dual": None,
                "graph_builds": int(st[3]), "graph_launches": int(st[4]),
                "rows_per_wave": int(st[0]), "block_rows": int(st[1]), "tuned": bool(st[2]),
-               "tune_ms": [float(x) for x in st[5:11]]}
+               "tune_ms": [float(x) for x in st[5:11]], "single_read": kinds.astype(bool)}
         if dual:
             sizes = self.n.astype(np.int64) + self.m - 1
             out["dual"] = np.split(du, np.cumsum(sizes)[:-1])

@@ -147,6 +147,7 @@ def lib():
     L.regot_batch_info.argtypes = [_vp, _ip, _llp, _llp]
     L.regot_batch_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_longlong]
     L.regot_batch_stats.argtypes = [_vp, _dp]
+    L.regot_batch_round_kinds.argtypes = [_vp, _ip]
     L.regot_batch_solve_bcd.argtypes = [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp, _dp,
                                         _llp]
     L.regot_sqeuclid_bf16.argtypes = [_vp, ctypes.c_longlong, _vp, ctypes.c_longlong,

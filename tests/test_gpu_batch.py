@@ -156,3 +156,75 @@ def test_batch_autotuned_tiles(gpu_lib):
                                                               out["block_rows"])
     np.testing.assert_array_equal(np.concatenate(again["dual"]), np.concatenate(out["dual"]))
     B.close()
+
+
+def single_read_batch(seed):
+    # FP32 problems around the single-read range 1024 < m <= 4096: ragged
+    # widths (one to four column chunks, partial last vectors), tiny and
+    # ragged n, and neighbours that keep two-pass rounds (m = 1024, 4097)
+    shapes = [(300, 1025), (17, 1027), (1, 2049), (3, 1500), (260, 2048), (129, 3001),
+              (64, 4095), (40, 4096), (90, 1024), (33, 4097), (200, 600)]
+    Ms, As, Bs, regs = [], [], [], []
+    rng = np.random.default_rng(seed)
+    for q, (n, m) in enumerate(shapes):
+        M, a, b = cloud_problem(n, m, d=8, seed=7000 * seed + q, dtype=np.float32)
+        Ms.append(M)
+        As.append(a)
+        Bs.append(b)
+        regs.append(float(np.exp(rng.uniform(np.log(0.02), np.log(0.2)))))
+    return shapes, Ms, As, Bs, regs
+
+
+def test_batch_single_read_rounds_match_oracle_and_two_pass(gpu_lib):
+    shapes, Ms, As, Bs, regs = single_read_batch(8)
+    tol = 1e-7
+    B = gpu_lib.BatchProblem(Ms, As, Bs, regs)
+    print("single-read batch: fused rounds", flush=True)
+    f = B.solve_bcd(tol=tol, max_iter=5000)
+    print("single-read batch: two-pass rounds", flush=True)
+    t = B.solve_bcd(tol=tol, max_iter=5000, single_read=False)
+    want = np.array([1024 < m <= 4096 for _, m in shapes])
+    np.testing.assert_array_equal(f["single_read"], want)
+    assert not t["single_read"].any()
+    for p, (n, m) in enumerate(shapes):
+        M64 = Ms[p].astype(np.float64)
+        o = O.sinkhorn_bcd(M64, As[p], Bs[p], regs[p], tol=tol, max_iter=5000)
+        for r in (f, t):
+            assert r["mar_err"][p] < tol, (p, r["mar_err"][p])
+            assert abs(int(r["niter"][p]) - o["niter"]) <= max(2, 0.05 * o["niter"]), (p, n, m)
+            # FP64 re-check of the returned dual on the CPU
+            _, g = O.dual_obj_grad(M64, As[p], Bs[p], regs[p], r["dual"][p])
+            assert np.linalg.norm(g) < 2 * tol, (p, n, m, np.linalg.norm(g))
+        # both round kinds stop within a crossing of tol of each other
+        assert abs(int(f["niter"][p]) - int(t["niter"][p])) <= max(1, 0.02 * o["niter"]), p
+        if not want[p]:
+            # problems outside the range run the same two-pass rounds either way
+            assert int(f["niter"][p]) == int(t["niter"][p])
+            np.testing.assert_array_equal(f["dual"][p], t["dual"][p])
+    B.close()
+
+
+def test_batch_single_read_tail_and_graph_replays(gpu_lib):
+    # One problem converges slowly: the others leave the tables and its tiles
+    # shrink to the minimum row count; graph replays give bitwise the same
+    # results as launches, and a second solve repeats the first.
+    M, a, b = cloud_problem(400, 2500, d=8, seed=91, dtype=np.float32)
+    Ms, As, Bs = [M, M, M], [a, a, a], [b, b, b]
+    regs = [0.2, 0.05, 0.01]
+    B = gpu_lib.BatchProblem(Ms, As, Bs, regs)
+    e = B.solve_bcd(tol=1e-7, max_iter=4000)
+    assert e["single_read"].all()
+    g = B.solve_bcd(tol=1e-7, max_iter=4000, graph=True)
+    np.testing.assert_array_equal(g["niter"], e["niter"])
+    np.testing.assert_array_equal(np.concatenate(g["dual"]), np.concatenate(e["dual"]))
+    e2 = B.solve_bcd(tol=1e-7, max_iter=4000)
+    np.testing.assert_array_equal(np.concatenate(e2["dual"]), np.concatenate(e["dual"]))
+    M64 = M.astype(np.float64)
+    for p in range(3):
+        o = O.sinkhorn_bcd(M64, a, b, regs[p], tol=1e-7, max_iter=4000)
+        assert abs(int(e["niter"][p]) - o["niter"]) <= max(2, 0.05 * o["niter"]), p
+        assert e["mar_err"][p] < 1e-7
+    # max_iter cuts the slow problem: niter = max_iter exactly
+    c = B.solve_bcd(tol=1e-12, max_iter=37)
+    assert int(c["niter"][2]) == 37
+    B.close()
