@@ -3,8 +3,12 @@
 //
 // All cost matrices live in one packed device buffer (problem p row-major at
 // moff[p] with a 256-byte row pitch).  One BCD round of every active problem
-// is three launches whose workgroups come from tile tables:
+// is a few launches whose workgroups come from tile tables:
 //
+//   single-read    (p, t): rows t fr .. t fr + fr - 1 of problem p whole ->
+//   tiles          alpha_{k+1}, the row partials of g_k and the column sums at
+//                  alpha_{k+1} (FP32 problems with 1024 < m <= 4096: one read
+//                  of M per iteration; one launch per column-chunk count);
 //   row tiles      (p, t): rows 4R t .. 4R t + 4R - 1 of problem p -> alpha_{k+1}
 //                  and the row partials of g_k;
 //   column tiles   (p, s, b): column strip s (64V columns) over row block b of
@@ -12,6 +16,8 @@
 //   combine tiles  (p, c): columns 64c .. 64c + 63 of problem p -> every tile
 //                  of p takes p's stop decision for iterate k itself (same data,
 //                  same order), then beta_{k+1} and its column residual.
+//
+// The other problems' rounds are row tiles then column tiles (two reads of M).
 //
 // Each problem follows the single-problem semantics of sinkhorn_bcd
 // (src/sinkhorn_bcd.cpp:14-124): test ||g_k|| < tol first, then update, and
