@@ -73,6 +73,13 @@ __device__ __forceinline__ float targ(float mij, double bkj, double sh, double k
     return (FULL || ok) ? x : ninf<float>();
 }
 
+// Sum of the 8 wave totals v[] in a fixed order, read by every thread (the
+// same bits everywhere).
+__device__ __forceinline__ double sum8(const double* v)
+{
+    return ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+}
+
 // Sum of the kFWaves wave totals v[] in a fixed butterfly order, computed by
 // the low lanes of every wave: the result is bitwise the same on every thread.
 __device__ __forceinline__ double sum_waves(const double* v, int lane)
@@ -165,10 +172,11 @@ __global__ __launch_bounds__(kFThreads) void k_bcd_fused(
                 s += e[c][q];
             }
         const int slot = static_cast<int>((i - r0) & 1);
-        const double S = wave_sum0(static_cast<double>(s));
-        if (lane == 0) red[slot][wave] = S;
+        double S[1] = {static_cast<double>(s)};
+        wave_sums<1>(S);
+        if (lane == 0) red[slot][wave] = S[0];
         __syncthreads();
-        double tot = sum_waves(red[slot], lane);
+        double tot = sum8(red[slot]);
 
         if (!(tot >= kTotLo && tot <= kTotHi)) {
             // Rare (first iterations from a poor start): shift by the row's

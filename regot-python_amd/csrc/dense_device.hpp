@@ -115,6 +115,42 @@ __device__ __forceinline__ double wave_sum0(double v)
     return __shfl(v, 0);
 }
 
+// One cross-lane step of an FP64 value by a DPP move of each half.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// R wave sums together: lane pairs xor 1 and xor 2 (quad permutes), then the
+// mirrored halves of each 8 and each 16 lanes (DPP), then xor 16 and xor 32
+// (shuffles issued for all R values before their adds).  Each step adds two
+// values that are equal on every lane of a group in either order, so every
+// lane ends with the same bits.
+template <int R>
+__device__ __forceinline__ void wave_sums(double (&x)[R])
+{
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += dpp_f64<0xB1>(x[r]);    // quad_perm [1,0,3,2]
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += dpp_f64<0x4E>(x[r]);    // quad_perm [2,3,0,1]
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += dpp_f64<0x141>(x[r]);   // row_half_mirror
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += dpp_f64<0x140>(x[r]);   // row_mirror
+    double y[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) y[r] = __shfl_xor(x[r], 16);
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += y[r];
+#pragma unroll
+    for (int r = 0; r < R; r++) y[r] = __shfl_xor(x[r], 32);
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] += y[r];
+}
+
 __device__ __forceinline__ double acc_log2(double sh, double s)
 {
     return (s > 0.0) ? sh + log2(s) : -INFINITY;
