@@ -151,6 +151,21 @@ __device__ __forceinline__ void wave_sums(double (&x)[R])
     for (int r = 0; r < R; r++) x[r] += y[r];
 }
 
+// Sum over each group of L consecutive lanes (L a power of two <= 64), the
+// same bits on every lane of the group: DPP steps inside rows of 16 lanes,
+// shuffles across rows.
+template <int L>
+__device__ __forceinline__ double group_sum(double x)
+{
+    if (L >= 2) x += dpp_f64<0xB1>(x);    // quad_perm [1,0,3,2]
+    if (L >= 4) x += dpp_f64<0x4E>(x);    // quad_perm [2,3,0,1]
+    if (L >= 8) x += dpp_f64<0x141>(x);   // row_half_mirror
+    if (L >= 16) x += dpp_f64<0x140>(x);  // row_mirror
+    if (L >= 32) x += __shfl_xor(x, 16);
+    if (L >= 64) x += __shfl_xor(x, 32);
+    return x;
+}
+
 __device__ __forceinline__ double acc_log2(double sh, double s)
 {
     return (s > 0.0) ? sh + log2(s) : -INFINITY;

@@ -5,6 +5,7 @@
 #include <deque>
 #include <vector>
 
+#include "dense_device.hpp"
 #include "hess_mf.hpp"
 #include "hessian.hpp"
 #include "regot_hip.h"
@@ -147,44 +148,55 @@ __global__ __launch_bounds__(256) void k_pcg_start(const double* __restrict__ r,
     }
 }
 
-// Sum of v over lanes [0, 64) of wave 0 in a fixed butterfly order,
-// broadcast to the workgroup through sm.
-__device__ __forceinline__ double wave0_sum_bcast(double v, double* sm)
+// Fixed-order sums of NV values over the 256 threads: each wave's by DPP and
+// shuffle steps, then the four wave totals in order from LDS (one barrier);
+// the same bits on every thread.  sm must not be reused before a barrier.
+template <int NV>
+__device__ __forceinline__ void wg_sums(double (&v)[NV], double (*sm)[4])
 {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (threadIdx.x == 0) *sm = v;
+    for (int q = 0; q < NV; q++) v[q] = dev::group_sum<64>(v[q]);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) sm[q][wave] = v[q];
     __syncthreads();
-    const double r = *sm;
-    __syncthreads();
-    return r;
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] = ((sm[q][0] + sm[q][1]) + sm[q][2]) + sm[q][3];
 }
 
 __global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int it)
 {
-    __shared__ double sm[2][256];
+    __shared__ double sm1[2][4], sm2[1][4];
     if (a.ctl[0]) return;
     double al, be = 0.0;
     const int cur = it & 1, prev = cur ^ 1;
     if (it == 0) {
         al = a.scal[0];
     } else {
-        double v0 = 0.0, v1 = 0.0;
-        for (int b = threadIdx.x; b < a.nbc; b += 256) {
-            v0 += a.part[2 * b];
-            v1 += a.part[2 * b + 1];
-        }
-        sm[0][threadIdx.x] = v0;
-        sm[1][threadIdx.x] = v1;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (threadIdx.x < o) {
-                sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
-                sm[1][threadIdx.x] += sm[1][threadIdx.x + o];
+        // the column workgroups' partials, b = t, t + 256, ... in order, eight
+        // loads in flight
+        double v[2] = {0.0, 0.0};
+        int b = threadIdx.x;
+        for (; b + 7 * 256 < a.nbc; b += 8 * 256) {
+            double x0[8], x1[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                x0[u] = a.part[2 * (b + 256 * u)];
+                x1[u] = a.part[2 * (b + 256 * u) + 1];
             }
-            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                v[0] += x0[u];
+                v[1] += x1[u];
+            }
         }
-        const double g = sm[0][0], d = sm[1][0];
+        for (; b < a.nbc; b += 256) {
+            v[0] += a.part[2 * b];
+            v[1] += a.part[2 * b + 1];
+        }
+        wg_sums<2>(v, sm1);
+        const double g = v[0], d = v[1];
         const double al_prev = a.scal[prev], g_prev = a.scal[2 + prev];
         be = g / g_prev;
         al = g / (d - be * g / al_prev);
@@ -206,15 +218,9 @@ __global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int it)
         a.u[j] = a.dinv ? a.dinv[j] * rj : rj;
         rr = rj * rj;
     }
-    // fixed-order workgroup sum
-    __syncthreads();
-    sm[0][threadIdx.x] = rr;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) a.part[2 * a.nbc + blockIdx.x] = sm[0][0];
+    double v1[1] = {rr};
+    wg_sums<1>(v1, sm2);
+    if (threadIdx.x == 0) a.part[2 * a.nbc + blockIdx.x] = v1[0];
 }
 
 // y = (sigma u) / (h1 + shift); LPR lanes per row, 64 / LPR rows per wave.
@@ -231,8 +237,7 @@ __global__ __launch_bounds__(256) void k_cg_rows(CgArgs a)
 #pragma unroll 4
         for (int k = a.rowptr[i] + sl; k < e; k += LPR) acc += a.rval[k] * a.u[a.colind[k]];
     }
-#pragma unroll
-    for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    acc = dev::group_sum<LPR>(acc);
     if (sl == 0 && i < a.n) a.y[i] = acc / (a.h1[i] + a.shift);
 }
 
@@ -244,12 +249,11 @@ __global__ __launch_bounds__(256) void k_cg_cols(CgArgs a, int it)
 {
     constexpr int CPW = 64 / LPR;   // columns per wave
     __shared__ double sm[2][4 * CPW];
-    __shared__ double bc;
     if (a.ctl[0]) return;
+    // every wave sums the update's r.r partials itself, in the same order
     double v = 0.0;
-    if (threadIdx.x < 64)
-        for (int b = threadIdx.x; b < a.nbu; b += 64) v += a.part[2 * a.nbc + b];
-    const double rr = wave0_sum_bcast(v, &bc);
+    for (int b = threadIdx.x & 63; b < a.nbu; b += 64) v += a.part[2 * a.nbc + b];
+    const double rr = dev::group_sum<64>(v);
     if (rr < a.scal[5]) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             a.scal[4] = rr;
@@ -269,8 +273,7 @@ __global__ __launch_bounds__(256) void k_cg_cols(CgArgs a, int it)
 #pragma unroll 4
         for (int k = a.colptr[j] + sl; k < e; k += LPR) t += a.cval[k] * a.y[a.rowind[k]];
     }
-#pragma unroll
-    for (int o = LPR / 2; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    t = dev::group_sum<LPR>(t);
     if (sl == 0) {
         double g = 0.0, d = 0.0;
         if (j < a.m1) {
@@ -302,7 +305,7 @@ __global__ __launch_bounds__(256) void k_cg_cols(CgArgs a, int it)
 __global__ __launch_bounds__(256) void k_cg_cols_mf(CgArgs a, const double* __restrict__ mpart,
                                                     int nwg)
 {
-    __shared__ double sm[2][256];
+    __shared__ double sm[2][4];
     if (a.ctl[0]) return;
     const int m1 = a.m1;
     const int j = blockIdx.x * 256 + threadIdx.x;
@@ -324,19 +327,11 @@ __global__ __launch_bounds__(256) void k_cg_cols_mf(CgArgs a, const double* __re
         g = a.r[j] * uj;
         d = wj * uj;
     }
-    sm[0][threadIdx.x] = g;
-    sm[1][threadIdx.x] = d;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
-            sm[1][threadIdx.x] += sm[1][threadIdx.x + o];
-        }
-        __syncthreads();
-    }
+    double v[2] = {g, d};
+    wg_sums<2>(v, sm);
     if (threadIdx.x == 0) {
-        a.part[2 * blockIdx.x] = sm[0][0];
-        a.part[2 * blockIdx.x + 1] = sm[1][0];
+        a.part[2 * blockIdx.x] = v[0];
+        a.part[2 * blockIdx.x + 1] = v[1];
     }
 }
 
