@@ -18,18 +18,15 @@
 // kernels fold strips, blocks and the scalars in a fixed order.
 #include <algorithm>
 
+#include "dense_device.hpp"
 #include "dexp2.hpp"
 #include "problem.hpp"
 
 namespace regot {
 namespace {
 
-__device__ __forceinline__ double wsum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
+// Wave sum by DPP and shuffle steps (the same bits on every lane).
+__device__ __forceinline__ double wsum(double v) { return dev::group_sum<64>(v); }
 
 __device__ __forceinline__ void ldm4(const float* p, int c0, int m, double (&v)[4])
 {
