@@ -100,21 +100,6 @@ __device__ __forceinline__ typename VecOf<T>::type ld_m(const T* p, bool nt)
     return nt ? ld_stream(p) : *reinterpret_cast<const VT*>(p);
 }
 
-__device__ __forceinline__ double wave_max(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
-}
-
-__device__ __forceinline__ double wave_sum0(double v)
-{
-    // Butterfly sum; lane 0's value is used so the order is fixed.
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return __shfl(v, 0);
-}
-
 // One cross-lane step of an FP64 value by a DPP move of each half.
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v)
@@ -164,6 +149,31 @@ __device__ __forceinline__ double group_sum(double x)
     if (L >= 32) x += __shfl_xor(x, 16);
     if (L >= 64) x += __shfl_xor(x, 32);
     return x;
+}
+
+// Wave maximum (any order gives the same value) and wave sum (fixed order;
+// the same bits on every lane) by the same steps.
+__device__ __forceinline__ double wave_max(double v)
+{
+    v = fmax(v, dpp_f64<0xB1>(v));
+    v = fmax(v, dpp_f64<0x4E>(v));
+    v = fmax(v, dpp_f64<0x141>(v));
+    v = fmax(v, dpp_f64<0x140>(v));
+    v = fmax(v, __shfl_xor(v, 16));
+    return fmax(v, __shfl_xor(v, 32));
+}
+
+__device__ __forceinline__ double wave_sum0(double v) { return group_sum<64>(v); }
+
+// Integer sum over the wave, the same on every lane.
+__device__ __forceinline__ int wave_isum(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
 }
 
 __device__ __forceinline__ double acc_log2(double sh, double s)
@@ -279,12 +289,7 @@ __device__ __forceinline__ void block_sums_256(double (&v)[NV])
 {
     __shared__ double sm[4][NV];
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-        double x = v[q];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-        v[q] = __shfl(x, 0);
-    }
+    for (int q = 0; q < NV; q++) v[q] = group_sum<64>(v[q]);
     const int wave = threadIdx.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63) == 0)

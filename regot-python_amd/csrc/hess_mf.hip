@@ -37,13 +37,8 @@ using namespace dev;
 
 constexpr int kMaxCols = 16384;          // m - 1 <= 16384
 
-// Butterfly sum over the wave; lane 0's value is the one used (fixed order).
-__device__ __forceinline__ double wave_total(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
+// Sum over the wave in a fixed order (the same bits on every lane).
+__device__ __forceinline__ double wave_total(double v) { return group_sum<64>(v); }
 
 // Row r of an FP32 matrix (row pointer Mi) in the pass layout: out[c] holds
 // columns 4t + SPAN c .. + 3; the last chunk past m - 1 is zero.
@@ -281,13 +276,8 @@ __global__ __launch_bounds__(512) void k_mf_rule(MfArgs a, double delta2,
             }
         const int slot = static_cast<int>((i - r0) & 1);
         {
-            double dw = dsum;
-            int cw = __popc(bits) | (nz << 16);   // both <= 2^14 per wave
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                dw += __shfl_xor(dw, o);
-                cw += __shfl_xor(cw, o);
-            }
+            const double dw = group_sum<64>(dsum);
+            const int cw = wave_isum(__popc(bits) | (nz << 16));   // both <= 2^14 per wave
             if (lane == 0) {
                 rd[slot][wave] = dw;
                 rn[slot][wave] = cw;
