@@ -17,7 +17,7 @@ def bf16_bits_to_float(t):
     return t.float().cpu().numpy().astype(np.float64)
 
 
-@pytest.mark.parametrize("n,m,d", [(300, 200, 64), (97, 131, 20), (64, 64, 8)])
+@pytest.mark.parametrize("n,m,d", [(300, 200, 64), (97, 131, 20), (64, 64, 8), (130, 70, 100), (65, 129, 128)])
 def test_sqeuclid_bf16_matches_fp64(gpu_lib, n, m, d):
     from regot.autograd import sqeuclid
     g = torch.Generator().manual_seed(n + m + d)
