@@ -265,3 +265,63 @@ def test_hessian_products_and_cg(gpu_lib, bulk):
         # iteration cap is honoured
         _, it2, err2 = P.hess_cg(rhs, shift, tol=1e-14, max_iter=3)
         assert it2 == 3 and err2 > 1e-14
+
+
+@pytest.mark.parametrize("bulk", ["fp64", "fp32"])
+def test_topk_rule_repeated_builds(gpu_lib, bulk):
+    # Successive builds on one handle (its candidate and tally buffers reused):
+    # a nearby point, a shifted point, jumps in density and the drivers'
+    # schedule (density down 10% per build at a drifting point).  Every build
+    # must keep the oracle's set.
+    n, m, reg = 300, 257, 0.03
+    M, a, b = cloud_problem(n, m, seed=29)
+    Mg = M.astype(np.float32) if bulk == "fp32" else M
+    Mo = Mg.astype(np.float64)
+    gamma0 = point(Mo, a, b, reg)
+    rng = np.random.default_rng(5)
+    steps = [(gamma0, 0.01),
+             (gamma0 + 1e-6 * reg * rng.standard_normal(gamma0.shape), 0.01),
+             (gamma0 + np.concatenate([np.full(n, 5 * reg), np.zeros(m - 1)]), 0.01),
+             (gamma0, 0.1),
+             (gamma0 + 1e-5 * reg * rng.standard_normal(gamma0.shape), 0.1),
+             (gamma0, 0.003)]
+    g = gamma0
+    for q in range(8):
+        g = g + 0.02 * reg * rng.standard_normal(g.shape)
+        steps.append((g, 0.05 * 0.9 ** q))
+    with gpu_lib.Problem(Mg, a, b, reg) as P:
+        for q, (g, density) in enumerate(steps):
+            print(f"top-k repeated build {q}: density {density:.4f}", flush=True)
+            T, _ = O.compute_T(Mo, reg, g)
+            cp, ri, va = O.sparsify_density(T, density, True)
+            nnz = P.sparsified_hess_with_density(g, density)
+            H = P.hess_export()
+            So = csc_dense(n, m, cp, ri, va)
+            Sg = csc_dense(n, m, H["colptr"], H["rowind"], H["cval"]) * reg
+            assert nnz == len(ri), q
+            if bulk == "fp64":
+                assert np.array_equal(Sg != 0, So != 0), q
+                np.testing.assert_allclose(Sg, So, rtol=1e-12)
+            else:
+                # FP32 values may reorder near the k-th value; the count is exact
+                k = int(n * (m - 1) * density)
+                assert np.sum((Sg != 0) != (So != 0)) <= max(4, 0.002 * k), q
+
+
+def test_topk_rule_ties_repeated_build(gpu_lib):
+    # the same tied build twice on one handle: both take the same ties in
+    # row-major order
+    n, m, density = 150, 121, 0.3
+    rng = np.random.default_rng(7)
+    Mq = 0.25 * rng.integers(0, 4, size=(n, m)).astype(np.float64)
+    a = np.full(n, 1.0 / n)
+    b = np.full(m, 1.0 / m)
+    gamma = np.zeros(n + m - 1)
+    kept = topk_ties_expected(Mq, 0.5, density)
+    with gpu_lib.Problem(Mq, a, b, 0.5) as P:
+        for _ in range(2):
+            nnz = P.sparsified_hess_with_density(gamma, density)
+            H = P.hess_export()
+            Sg = csc_dense(n, m, H["colptr"], H["rowind"], H["cval"])
+            assert nnz == int(kept.sum())
+            assert np.array_equal(Sg != 0, kept)
