@@ -32,6 +32,7 @@ problem beside the GPU's, and it records the host (CPU model, cores, threads,
 memory).
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -50,12 +51,20 @@ HBM_PEAK_GBS = 8000.0
 def measured_traffic(kernel, n):
     """HBM bytes per launch of the dominant pass at n = m = 16384 FP32, from the
     committed rocprofv3 PMC measurement (FETCH_SIZE and WRITE_SIZE runs scaled
-    by a copy kernel of known size); None for other sizes."""
+    by a copy kernel of known size); None for other sizes, and None when the
+    kernels' sources no longer hash to the ones measured."""
     path = os.path.join(ROOT, "profiles", "bcd_n16384_fp32_pmc_traffic.json")
     if n != 16384 or not os.path.exists(path):
         return None
     with open(path) as f:
-        k = json.load(f)["kernels"]
+        rec = json.load(f)
+    h = hashlib.sha256()
+    for src in rec.get("sources", []):
+        with open(os.path.join(ROOT, src), "rb") as f:
+            h.update(f.read())
+    if not rec.get("sources") or h.hexdigest() != rec.get("sources_sha256"):
+        return None
+    k = rec["kernels"]
     names = {"row_pass": ["k_row_pass<float, float, 4, true>"],
              "col_pass": ["k_col_partial<float, float>", "k_col_combine<true, false>"],
              "bcd_fused": ["k_bcd_fused<8, true, 2>"]}[kernel]
@@ -643,6 +652,7 @@ def main():
               flush=True)
         diff = bench_differentiable(args.diff_n, 64, 0.05, 1e-6, args.diff_steps)
 
+    traffic = measured_traffic(dom_name, n)
     out = {
         "metric": "s to mar_err<=1e-8, sinkhorn_bcd, n=m=%d EROT" % n,
         "value": ms_per_step / 1e3,
@@ -669,8 +679,11 @@ def main():
                    "two_pass": two_pass},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dom_name, n),
-                     "traffic_source": "profiles/bcd_n16384_fp32_pmc_traffic.md",
+                     "traffic": traffic,
+                     "traffic_source": ("profiles/bcd_n16384_fp32_pmc_traffic.md"
+                                        if traffic is not None else
+                                        "none: no PMC measurement of these kernel sources "
+                                        "at this size"),
                      "avg_ms": dom_ms,
                      "row_pass_avg_ms": st[0][1], "col_pass_avg_ms": st[1][1],
                      "fused_pass_avg_ms": st[2][1], "fused_combine_avg_ms": st[3][1],
