@@ -100,12 +100,15 @@ __device__ __forceinline__ typename VecOf<T>::type ld_m(const T* p, bool nt)
     return nt ? ld_stream(p) : *reinterpret_cast<const VT*>(p);
 }
 
-// One cross-lane step of an FP64 value by a DPP move of each half.
+// One cross-lane step of an FP64 value by a DPP move of each half.  Every
+// control used here (quad permutes, half-row and row mirrors) gives every
+// lane a valid source, so with bound_ctrl and full masks the old value is
+// dead and the move needs no zeroed destination first.
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 

@@ -12,7 +12,7 @@
 // which also rounds subnormal results.  On 2e7 random arguments 0.2% of the
 // results are one ulp from the correctly rounded value, none further.
 // x < -1100 gives 0 and x > 1100 gives +inf (the argument is clamped there),
-// NaN propagates.  The tables and the
+// a NaN argument gives a quiet NaN.  The tables and the
 // coefficients were computed with 60-digit decimal arithmetic and rounded to
 // nearest.  Callers stage the 128 table doubles in LDS (exp2_tab_load) and
 // pass that copy.
@@ -73,20 +73,35 @@ __device__ __forceinline__ void exp2_tab_load(double* lds)
         lds[i] = (i & 1) ? kExp2TabLo[i >> 1] : kExp2Tab[i >> 1];
 }
 
-__device__ __forceinline__ double dexp2(double x, const double* tab)
+// 2^(y / 64) for y = 64 x.  Callers that form x as a product with k pass
+// 64 k instead: scaling by a power of two commutes with every rounding (no
+// overflow or underflow at these magnitudes), so y is exactly 64 x and the
+// result is bit for bit dexp2(x).  Then k = rint(y) comes from the
+// 1.5 * 2^52 shifter (one add; |y| < 2^51 after the clamp, ties to even as
+// rint; the low word of the sum is k), and the polynomial runs on
+// d = y - k = 64 f (exact) with the coefficients scaled by 64^-n, which again
+// changes no rounding.
+__device__ __forceinline__ double dexp2_64(double y, const double* tab)
 {
-    const double c1 = 0x1.62e42fefa39efp-1, c2 = 0x1.ebfbdff82c58fp-3;
-    const double c3 = 0x1.c6b08d704a0c0p-5, c4 = 0x1.3b2ab6fba4e77p-7;
-    const double c5 = 0x1.5d87fe78a6731p-10, c6 = 0x1.430912f86c787p-13;
-    // clamp: beyond +-1100 the result is already 0 or +inf (NaN restored last)
-    const double xc = fmin(fmax(x, -1100.0), 1100.0);
-    const double kx = rint(xc * 64.0);
-    const int i = static_cast<int>(kx);
-    const double f = fma(kx, -0.015625, xc);
-    const double q = f * (c1 + f * (c2 + f * (c3 + f * (c4 + f * (c5 + f * c6)))));
+    constexpr double c1 = 0x1.62e42fefa39efp-1 * 0x1p-6, c2 = 0x1.ebfbdff82c58fp-3 * 0x1p-12;
+    constexpr double c3 = 0x1.c6b08d704a0c0p-5 * 0x1p-18, c4 = 0x1.3b2ab6fba4e77p-7 * 0x1p-24;
+    constexpr double c5 = 0x1.5d87fe78a6731p-10 * 0x1p-30, c6 = 0x1.430912f86c787p-13 * 0x1p-36;
+    // clamp: beyond +-1100 * 64 the result is already 0 or +inf (NaN last)
+    const double yc = fmin(fmax(y, -70400.0), 70400.0);
+    const double sh = yc + 0x1.8p52;
+    const double kx = sh - 0x1.8p52;
+    const int i = __double2loint(sh);
+    const double d = yc - kx;
+    const double q = d * (c1 + d * (c2 + d * (c3 + d * (c4 + d * (c5 + d * c6)))));
     const double2 t = *reinterpret_cast<const double2*>(tab + 2 * (i & 63));
     const double r = ldexp(t.x + fma(t.x, q, t.y), i >> 6);
-    return (x != x) ? x : r;
+    // a NaN argument gives a quiet NaN (exponent all ones, top mantissa bit set)
+    return __hiloint2double((y != y) ? 0x7ff80000 : __double2hiint(r), __double2loint(r));
+}
+
+__device__ __forceinline__ double dexp2(double x, const double* tab)
+{
+    return dexp2_64(x * 64.0, tab);
 }
 
 }  // namespace dev
