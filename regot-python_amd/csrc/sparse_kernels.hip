@@ -961,36 +961,137 @@ __global__ void k_topk_count(const int* tcnt, const int* __restrict__ gcnt,
 }
 
 // Exclusive prefix, over the lanes of a wave, of per-lane flag counts (four
-// flags per lane, lane-major order) by ballots; *total gets the wave's count.
-__device__ __forceinline__ int wave_excl_count4(const bool (&f)[4], int* total)
+// flags per lane, lane-major order) by ballots and mbcnt; *total gets the
+// wave's count.
+__device__ __forceinline__ int wave_prefix4(const bool (&f)[4], int* total)
 {
-    const int lane = threadIdx.x & 63;
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int pre = 0, tot = 0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const unsigned long long b = __ballot(f[q]);
-        pre += __popcll(b & lt);
+        pre = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(b >> 32),
+                                        __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(b),
+                                                                  pre));
         tot += __popcll(b);
     }
     *total = tot;
     return pre;
 }
 
+// T of one FP32-bulk element with the raw v_exp_f32: bit for bit
+// Elem<float>::val wherever that result is at least 2^-126; below, 0.
+__device__ __forceinline__ float val_raw(double ab, float mij, double k)
+{
+    return __builtin_amdgcn_exp2f(static_cast<float>(fma(-k, static_cast<double>(mij), ab * k)));
+}
+
+// One 256-column step of a row (4 columns per lane, from cj = s0 + 4 lane):
+// full steps (s0 + 256 <= m1) load unconditionally, the last partial step
+// reads only columns < m1.
+template <typename TM, bool TOPK>
+__device__ __forceinline__ void rw_load(const TM* __restrict__ row, const ElemCtx& c,
+                                        const KeepRule& rule, int s0, int m1, TM (&v)[4],
+                                        double (&b)[4], double (&t)[4])
+{
+    const int cj = s0 + (threadIdx.x & 63) * 4;
+    if (s0 + 256 <= m1) {   // wave-uniform
+        load4(row, cj, cj + 4, v);
+        load4(c.beta, cj, cj + 4, b);
+        if (!TOPK) load4(rule.tau, cj, cj + 4, t);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            v[q] = TM(0);
+            b[q] = 0.0;
+            t[q] = 0.0;
+        }
+        if (cj < m1) {
+            load4(row, cj, m1, v);
+            load4(c.beta, cj, m1, b);
+            if (!TOPK) load4(rule.tau, cj, m1, t);
+        }
+    }
+}
+
+// The kept flags, values and the ordered write of one step (see k_row_write).
+template <typename TM, bool TOPK, bool RAW>
+__device__ __forceinline__ void rw_step(const ElemCtx& c, const KeepRule& rule, int i, int cj,
+                                        int m1, double ai, double r_i, long long rem0,
+                                        double inv_reg, long base_pos, long end_pos,
+                                        const TM (&v)[4], const double (&b)[4],
+                                        const double (&tt)[4], long long* tie_seen,
+                                        long* written, int* __restrict__ colind,
+                                        double* __restrict__ val)
+{
+    using E = Elem<TM>;
+    TM t[4];
+    bool base[4], tie[4];
+    bool any_tie = false;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int j = cj + q;
+        const bool in = j < m1;
+        bool z = false;
+        if constexpr (RAW)
+            t[q] = val_raw(ai + b[q], v[q], c.k);
+        else
+            t[q] = E::val(ai + b[q], v[q], c.k, c.reg, z);
+        if (TOPK) {
+            const unsigned long long key = static_cast<unsigned long long>(E::key(t[q]));
+            base[q] = in && (key > rule.kstar || (rule.preserve && (i == 0 || j == 0)));
+            tie[q] = in && key == rule.kstar;
+            any_tie = any_tie || tie[q];
+        } else {
+            base[q] = in && ((static_cast<double>(t[q]) >= fmin(tt[q], r_i)) || z);
+            tie[q] = false;
+        }
+    }
+    if (TOPK && __ballot(any_tie) != 0ull) {
+        int tot;
+        int r = wave_prefix4(tie, &tot);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (tie[q]) {
+                if (*tie_seen + r < rem0) base[q] = true;
+                r++;
+            }
+        }
+        *tie_seen += tot;
+    }
+    bool st[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) st[q] = base[q] && t[q] != TM(0);
+    int tot;
+    long q0 = base_pos + *written + wave_prefix4(st, &tot);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (st[q] && q0 < end_pos) {
+            colind[q0] = cj + q;
+            val[q0] = static_cast<double>(t[q]) * inv_reg;
+        }
+        q0 += st[q] ? 1 : 0;
+    }
+    *written += tot;
+}
+
 // Ordered CSR write of row i under either rule: one wave per row, 256
 // columns (4 per lane) per step, positions from ballots (no LDS, no
-// barriers), the next step's loads in flight during the current step.
-// Top-k ties are taken in row-major order: a tie of chunk rank r is kept when
-// tie_off[i] + ties of earlier chunks + r < tie_take.  Only columns < m - 1
-// are read (of M, beta and tau), so the kernel also runs on M' with the roles
-// of alpha / beta and tau / rho exchanged, writing the CSC.
-template <typename TM>
+// barriers), the next step's loads in flight during the current step (full
+// steps load unconditionally).  Top-k ties are taken in row-major order: a
+// tie of chunk rank r is kept when tie_off[i] + ties of earlier chunks + r <
+// tie_take.  RAW (FP32 top-k, k-th value at least 1.125 * 2^-126): T by the
+// raw v_exp_f32, which equals the exact value on every entry the rule can
+// keep (key >= the k-th key) and leaves the others below it; row 0 and the
+// first step (column 0), where preserved entries are kept at any value,
+// use the exact function.  Only columns < m - 1 are read (of M, beta and
+// tau), so the kernel also runs on M' with the roles of alpha / beta and
+// tau / rho exchanged, writing the CSC.
+template <typename TM, bool TOPK, bool RAW>
 __global__ __launch_bounds__(256) void k_row_write(const TM* __restrict__ M, long ldm, ElemCtx c,
                                                    KeepRule rule, const int* __restrict__ rowptr,
                                                    int* __restrict__ colind,
                                                    double* __restrict__ val)
 {
-    using E = Elem<TM>;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= c.n) return;   // wave-uniform
     const int lane = threadIdx.x & 63;
@@ -998,86 +1099,53 @@ __global__ __launch_bounds__(256) void k_row_write(const TM* __restrict__ M, lon
     const double ai = c.alpha[i];
     const double inv_reg = 1.0 / c.reg;
     const int m1 = c.m - 1;
-    const bool topk = rule.mode == 1;
-    const double r_i = topk ? 0.0 : rule.rho[i];
-    const long long rem0 = topk ? rule.tie_take - rule.tie_off[i] : 0;
+    const double r_i = TOPK ? 0.0 : rule.rho[i];
+    const long long rem0 = TOPK ? rule.tie_take - rule.tie_off[i] : 0;
     long long tie_seen = 0;
     long written = 0;
     const long base_pos = rowptr[i];
     const long end_pos = rowptr[i + 1];   // writes never pass the counted end
-    TM v[4] = {TM(0), TM(0), TM(0), TM(0)};
-    double b4[4] = {0.0, 0.0, 0.0, 0.0};
-    double t4[4] = {0.0, 0.0, 0.0, 0.0};
-    int cj = lane * 4;
-    if (cj < m1) {
-        load4(row, cj, m1, v);
-        load4(c.beta, cj, m1, b4);
-        if (!topk) load4(rule.tau, cj, m1, t4);
-    }
-    for (int s0 = 0; s0 < m1; s0 += 256, cj += 256) {
-        TM vn[4] = {TM(0), TM(0), TM(0), TM(0)};
-        double bn[4] = {0.0, 0.0, 0.0, 0.0};
-        double tn[4] = {0.0, 0.0, 0.0, 0.0};
-        if (cj + 256 < m1) {
-            load4(row, cj + 256, m1, vn);
-            load4(c.beta, cj + 256, m1, bn);
-            if (!topk) load4(rule.tau, cj + 256, m1, tn);
-        }
-        TM t[4];
-        bool base[4], tie[4];
-        bool any_tie = false;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int j = cj + q;
-            base[q] = false;
-            tie[q] = false;
-            t[q] = TM(0);
-            if (j < m1) {
-                bool z;
-                t[q] = E::val(ai + b4[q], v[q], c.k, c.reg, z);
-                if (topk) {
-                    const unsigned long long key = static_cast<unsigned long long>(E::key(t[q]));
-                    base[q] = key > rule.kstar || (rule.preserve && (i == 0 || j == 0));
-                    tie[q] = key == rule.kstar;
-                    any_tie = any_tie || tie[q];
-                } else {
-                    base[q] = (static_cast<double>(t[q]) >= fmin(t4[q], r_i)) || z;
-                }
-            }
-        }
-        if (topk && __ballot(any_tie) != 0ull) {
-            int tot;
-            int r = wave_excl_count4(tie, &tot);
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                if (tie[q]) {
-                    if (tie_seen + r < rem0) base[q] = true;
-                    r++;
-                }
-            }
-            tie_seen += tot;
-        }
-        bool st[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) st[q] = base[q] && t[q] != TM(0);
-        int tot;
-        long q0 = base_pos + written + wave_excl_count4(st, &tot);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            if (st[q] && q0 < end_pos) {
-                colind[q0] = cj + q;
-                val[q0] = static_cast<double>(t[q]) * inv_reg;
-            }
-            q0 += st[q] ? 1 : 0;
-        }
-        written += tot;
+    const int last = m1 > 0 ? ((m1 - 1) / 256) * 256 : 0;
+    TM v[4];
+    double b[4], tt[4];
+    rw_load<TM, TOPK>(row, c, rule, 0, m1, v, b, tt);
+    for (int s0 = 0; s0 < m1; s0 += 256) {
+        TM vn[4];
+        double bn[4], tn[4];
+        rw_load<TM, TOPK>(row, c, rule, s0 + 256 <= last ? s0 + 256 : last, m1, vn, bn, tn);
+        const int cj = s0 + lane * 4;
+        if (RAW && i != 0 && s0 != 0)   // wave-uniform
+            rw_step<TM, TOPK, RAW>(c, rule, i, cj, m1, ai, r_i, rem0, inv_reg, base_pos, end_pos,
+                                   v, b, tt, &tie_seen, &written, colind, val);
+        else
+            rw_step<TM, TOPK, false>(c, rule, i, cj, m1, ai, r_i, rem0, inv_reg, base_pos,
+                                     end_pos, v, b, tt, &tie_seen, &written, colind, val);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             v[q] = vn[q];
-            b4[q] = bn[q];
-            t4[q] = tn[q];
+            b[q] = bn[q];
+            tt[q] = tn[q];
         }
     }
+}
+
+// The row writer for a bulk type and rule (RAW for FP32 top-k when the k-th
+// value is at least 1.125 * 2^-126: the raw exp2 of an argument below -126,
+// whose exact value is subnormal, stays well below it).
+template <typename TM>
+void launch_row_write_t(const TM* M, long ldm, const ElemCtx& c, const KeepRule& r, int rows,
+                        const int* rowptr, int* colind, double* val, hipStream_t s)
+{
+    const dim3 grid(ceil_div(rows, 4));
+    if (r.mode != 1)
+        hipLaunchKernelGGL((k_row_write<TM, false, false>), grid, dim3(256), 0, s, M, ldm, c, r,
+                           rowptr, colind, val);
+    else if (sizeof(TM) == 4 && r.kstar >= 0x00900000ull)
+        hipLaunchKernelGGL((k_row_write<TM, true, sizeof(TM) == 4>), grid, dim3(256), 0, s, M,
+                           ldm, c, r, rowptr, colind, val);
+    else
+        hipLaunchKernelGGL((k_row_write<TM, true, false>), grid, dim3(256), 0, s, M, ldm, c, r,
+                           rowptr, colind, val);
 }
 
 // ---------------------------------------------------------------------------
@@ -1612,11 +1680,9 @@ void launch_row_write(int bulk, const void* M, long ldm, const ElemCtx& c, const
                       const int* rowptr, int* colind, double* val, hipStream_t s)
 {
     if (bulk == 0)
-        hipLaunchKernelGGL(k_row_write<float>, dim3(ceil_div(c.n, 4)), dim3(256), 0, s,
-                           cM<float>(M), ldm, c, r, rowptr, colind, val);
+        launch_row_write_t(cM<float>(M), ldm, c, r, c.n, rowptr, colind, val, s);
     else
-        hipLaunchKernelGGL(k_row_write<double>, dim3(ceil_div(c.n, 4)), dim3(256), 0, s,
-                           cM<double>(M), ldm, c, r, rowptr, colind, val);
+        launch_row_write_t(cM<double>(M), ldm, c, r, c.n, rowptr, colind, val, s);
     KERNEL_CHECK();
 }
 
@@ -1650,11 +1716,9 @@ void launch_build_csc(int bulk, const void* M, long ldm, const void* MT, long ld
         rt.tau = r.rho;
         rt.rho = r.tau;
         if (bulk == 0)
-            hipLaunchKernelGGL(k_row_write<float>, dim3(ceil_div(m1, 4)), dim3(256), 0, s,
-                               cM<float>(MT), ldt, ct, rt, colptr, rowind, cval);
+            launch_row_write_t(cM<float>(MT), ldt, ct, rt, m1, colptr, rowind, cval, s);
         else
-            hipLaunchKernelGGL(k_row_write<double>, dim3(ceil_div(m1, 4)), dim3(256), 0, s,
-                               cM<double>(MT), ldt, ct, rt, colptr, rowind, cval);
+            launch_row_write_t(cM<double>(MT), ldt, ct, rt, m1, colptr, rowind, cval, s);
     } else {
         // about 512 rows per workgroup: long front-to-back runs per column
         const int bpw = std::max(1, 512 / rows_per_block);
