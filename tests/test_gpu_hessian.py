@@ -198,6 +198,38 @@ def topk_ties_expected(Mq, reg, density):
     return kept
 
 
+@pytest.mark.parametrize("top_log2", [-124.0, -130.0])
+def test_topk_rule_values_near_fp32_subnormals(gpu_lib, top_log2):
+    # FP32 bulk, four cost levels scaled so the largest T is 2^top_log2: at
+    # -124 the levels are 2^-124, 2^-124.7 (the k-th value), 2^-125.4 (the
+    # lowest normal binade) and 2^-126.2 (subnormal), so the CSR writer may
+    # use the raw exponential, while the preserved row 0 and column 0 keep
+    # subnormal entries; at -130 every value is subnormal and the writer runs
+    # with the exact exponential.  m > 4096 puts columns past the first step
+    # of every pass.
+    n, m, reg, density = 40, 5000, 0.5, 0.4
+    rng = np.random.default_rng(71)
+    Mq = 0.25 * rng.integers(0, 4, size=(n, m)).astype(np.float64)
+    a = np.full(n, 1.0 / n)
+    b = np.full(m, 1.0 / m)
+    c = -reg * top_log2 * np.log(2.0)       # alpha_i = -c: T = 2^top_log2 exp(-M / reg)
+    gamma = np.concatenate([np.full(n, -c), np.zeros(m - 1)])
+    with gpu_lib.Problem(Mq.astype(np.float32), a, b, reg) as P:
+        P.set_hess_storage(0)
+        print(f"top-k near subnormals: top 2^{top_log2}", flush=True)
+        nnz = P.sparsified_hess_with_density(gamma, density)
+        H = P.hess_export()
+    kept = topk_ties_expected(Mq, reg, density)
+    Sg = csc_dense(n, m, H["colptr"], H["rowind"], H["cval"])
+    assert nnz == int(kept.sum())
+    assert np.array_equal(Sg != 0, kept)
+    T = np.exp(-c / reg - Mq[:, :-1] / reg) / reg
+    # the FP32 exponent argument near -128 is rounded to 2^-17: up to
+    # ln2 * 2^-17 = 5.3e-6 relative; FP32 subnormals carry fewer bits (2^-149
+    # absolute against values >= 2^-133)
+    np.testing.assert_allclose(Sg[kept], T[kept], rtol=3e-3 if top_log2 < -126 else 1e-5)
+
+
 @pytest.mark.parametrize("bulk", ["fp64", "fp32"])
 @pytest.mark.parametrize("n,m,density", [(150, 121, 0.3), (400, 300, 0.005), (64, 50, 1.0)])
 def test_topk_rule_ties(gpu_lib, bulk, n, m, density):
