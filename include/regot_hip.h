@@ -164,6 +164,14 @@ int regot_hess_apply_hs(regot_problem* p, const double* x, double* res);
  * src/sinkhorn_cg.cpp:9-50. */
 int regot_hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
                   double* res, int* iters, double* tol_error);
+/* Two right-hand sides of the same (Hs + shift I) system, their CGs run in
+ * the same kernel launches (the reference's SPLR solves its two systems with
+ * one factorisation, src/sinkhorn_linsolve.cpp:160-225); each result, count
+ * and residual equals regot_hess_cg's for that right-hand side bit for bit.
+ * iters[2] and tol_error[2] (nullable). */
+int regot_hess_cg2(regot_problem* p, const double* rhs0, const double* rhs1, double shift,
+                   double tol, int max_iter, double* res0, double* res1, int* iters,
+                   double* tol_error);
 
 /*
  * Solver options; zero-initialise, then call regot_opts_default().

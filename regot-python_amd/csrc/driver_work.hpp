@@ -222,8 +222,18 @@ struct Work {
     void lin_solve_low_rank(const regot_opts& o, const double* r, double shift, const double* yv,
                             const double* sv_, double* out)
     {
-        lin_solve(o, r, shift, lr0);
-        lin_solve(o, yv, shift, lr1);
+        // the two solves share the system: their CGs run in the same launches
+        // (each with the iterates of its own lin_solve)
+        const double tol = o.cg_tol > 0 ? o.cg_tol : (o.method == 0 ? 1e-8 : 1e-10);
+        const int maxit = o.cg_max_iter > 0 ? o.cg_max_iter
+                                            : (o.method == 0 ? 2 * m : 10 * m + 1000);
+        const auto t0 = Clock::now();
+        int it2[2] = {0, 0};
+        hess_cg2(p, r, yv, shift, tol, maxit, lr0, lr1, it2, nullptr, o.method != 0);
+        count_cg(it2[0]);
+        count_cg(it2[1]);
+        phase_ms[2] += ms_since(t0);
+        phase_n[2] += 2;
         const double* xs[4] = {sv_, yv, yv, yv};
         const double* ys[4] = {r, sv_, lr1.p, lr0.p};
         double d[4];

@@ -165,9 +165,13 @@ __device__ __forceinline__ void wg_sums(double (&v)[NV], double (*sm)[4])
     for (int q = 0; q < NV; q++) v[q] = ((sm[q][0] + sm[q][1]) + sm[q][2]) + sm[q][3];
 }
 
-__global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int it)
+// The CG kernels serve one or two right-hand sides: grid row blockIdx.y
+// takes a0 or a1 (two independent CGs on the same Delta in the same
+// launches; every right-hand side runs exactly the single-CG arithmetic).
+__global__ __launch_bounds__(256) void k_cg_update(CgArgs a0, CgArgs a1, int it)
 {
     __shared__ double sm1[2][4], sm2[1][4];
+    const CgArgs a = blockIdx.y == 0 ? a0 : a1;
     if (a.ctl[0]) return;
     double al, be = 0.0;
     const int cur = it & 1, prev = cur ^ 1;
@@ -225,8 +229,9 @@ __global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int it)
 
 // y = (sigma u) / (h1 + shift); LPR lanes per row, 64 / LPR rows per wave.
 template <int LPR>
-__global__ __launch_bounds__(256) void k_cg_rows(CgArgs a)
+__global__ __launch_bounds__(256) void k_cg_rows(CgArgs a0, CgArgs a1)
 {
+    const CgArgs a = blockIdx.y == 0 ? a0 : a1;
     if (a.ctl[0]) return;
     const int lane = threadIdx.x & 63;
     const int sl = lane & (LPR - 1);
@@ -245,10 +250,11 @@ __global__ __launch_bounds__(256) void k_cg_rows(CgArgs a)
 // workgroup), then w = (h2 + shift) u - sigma' y with LPR lanes per column and
 // the partials of r.u and w.u per workgroup (fixed order).
 template <int LPR>
-__global__ __launch_bounds__(256) void k_cg_cols(CgArgs a, int it)
+__global__ __launch_bounds__(256) void k_cg_cols(CgArgs a0, CgArgs a1, int it)
 {
     constexpr int CPW = 64 / LPR;   // columns per wave
     __shared__ double sm[2][4 * CPW];
+    const CgArgs a = blockIdx.y == 0 ? a0 : a1;
     if (a.ctl[0]) return;
     // every wave sums the update's r.r partials itself, in the same order
     double v = 0.0;
@@ -653,66 +659,88 @@ static int lanes_for(double avg, int count)
     return l;
 }
 
+// a0 / a1: the right-hand sides (ny of them, 1 or 2) in grid rows 0 / 1.
 template <int LPR>
-static void launch_cg_rows(const CgArgs& a, hipStream_t s)
+static void launch_cg_rows(const CgArgs& a0, const CgArgs& a1, int ny, hipStream_t s)
 {
     const int rows_per_wg = 4 * (64 / LPR);
-    hipLaunchKernelGGL(k_cg_rows<LPR>, dim3(ceil_div(a.n, rows_per_wg)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_cg_rows<LPR>, dim3(ceil_div(a0.n, rows_per_wg), ny), dim3(256), 0, s,
+                       a0, a1);
 }
 
 template <int LPR>
-static void launch_cg_cols(const CgArgs& a, int it, hipStream_t s)
+static void launch_cg_cols(const CgArgs& a0, const CgArgs& a1, int ny, int it, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cg_cols<LPR>, dim3(a.nbc), dim3(256), 0, s, a, it);
+    hipLaunchKernelGGL(k_cg_cols<LPR>, dim3(a0.nbc, ny), dim3(256), 0, s, a0, a1, it);
 }
 
-static void cg_rows(int lpr, const CgArgs& a, hipStream_t s)
+static void cg_rows(int lpr, const CgArgs& a0, const CgArgs& a1, int ny, hipStream_t s)
 {
     switch (lpr) {
-    case 4: launch_cg_rows<4>(a, s); break;
-    case 8: launch_cg_rows<8>(a, s); break;
-    case 16: launch_cg_rows<16>(a, s); break;
-    case 32: launch_cg_rows<32>(a, s); break;
-    default: launch_cg_rows<64>(a, s); break;
+    case 4: launch_cg_rows<4>(a0, a1, ny, s); break;
+    case 8: launch_cg_rows<8>(a0, a1, ny, s); break;
+    case 16: launch_cg_rows<16>(a0, a1, ny, s); break;
+    case 32: launch_cg_rows<32>(a0, a1, ny, s); break;
+    default: launch_cg_rows<64>(a0, a1, ny, s); break;
     }
 }
 
-static void cg_cols(int lpr, const CgArgs& a, int it, hipStream_t s)
+static void cg_cols(int lpr, const CgArgs& a0, const CgArgs& a1, int ny, int it, hipStream_t s)
 {
     switch (lpr) {
-    case 4: launch_cg_cols<4>(a, it, s); break;
-    case 8: launch_cg_cols<8>(a, it, s); break;
-    case 16: launch_cg_cols<16>(a, it, s); break;
-    case 32: launch_cg_cols<32>(a, it, s); break;
-    default: launch_cg_cols<64>(a, it, s); break;
+    case 4: launch_cg_cols<4>(a0, a1, ny, it, s); break;
+    case 8: launch_cg_cols<8>(a0, a1, ny, it, s); break;
+    case 16: launch_cg_cols<16>(a0, a1, ny, it, s); break;
+    case 32: launch_cg_cols<32>(a0, a1, ny, it, s); break;
+    default: launch_cg_cols<64>(a0, a1, ny, it, s); break;
     }
 }
 
-int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
-                  double* x, double* tol_error, const double* x0, bool jacobi)
+// One right-hand side of the CG on Delta: its inputs, its device state and
+// its outcome.
+struct CgSide {
+    const double* rhs = nullptr;   // device, m-1
+    double* x = nullptr;           // device, m-1 (the solution)
+    const double* x0 = nullptr;    // device, m-1, or null (start from 0)
+    DBuf<double>*r, *pv, *q, *sv, *u, *y, *part, *scal;
+    DBuf<int>* ctl;
+    HBuf<int>* hctl;
+    HBuf<double>* hbuf;
+    std::vector<double> hpart;
+    bool active = false;
+    double rhs_norm2 = 0.0;
+    int iters = 0;
+    double tol_error = 0.0;
+    CgArgs a;
+};
+
+// CG on Delta for nr (1 or 2) right-hand sides, each to its own stop rule
+// ||r||^2 < max(tol^2 ||rhs||^2, DBL_MIN) and cap (Eigen's counting, the
+// converged update not counted; src/sinkhorn_cg.cpp:29-38).  Two sides run
+// in the same launches (grid rows 0 and 1) and each does exactly the
+// arithmetic of a CG of its own, so the iterates are the same bit for bit.
+static void cg_delta_sides(regot_problem* p, int nr, CgSide* S, double shift, double tol,
+                           int max_iter, bool jacobi)
 {
     SparseHessian& H = hess_of(p);
     const int n = p->n, m1 = p->m - 1;
     hipStream_t s = p->stream;
+    if (nr < 1 || nr > 2 || (nr == 2 && H.mf))
+        fail(E_INTERNAL, "CG over %d right-hand sides (matrix-free: %d)", nr, H.mf ? 1 : 0);
     if (m1 <= 0) {
-        if (tol_error) *tol_error = 0.0;
-        return 0;
+        for (int k = 0; k < nr; k++) {
+            S[k].iters = 0;
+            S[k].tol_error = 0.0;
+        }
+        return;
     }
-    H.cg_r.ensure(m1);
-    H.cg_p.ensure(m1);
-    H.cg_q.ensure(m1);
-    H.cg_s.ensure(m1);
-    H.cg_u.ensure(m1);
     H.vn.ensure(n);
-    H.cg_y_buf.ensure(n);
     const int lpr_r = lanes_for(static_cast<double>(H.nnz) / std::max(n, 1), n);
     const int lpr_c = lanes_for(static_cast<double>(H.nnz) / m1, m1);
     // column workgroups: 256 columns each when the products are passes over M
     const int nbc = H.mf ? ceil_div(m1, 256) : ceil_div(m1, 4 * (64 / lpr_c));
     const int nbu = ceil_div(m1, 256);
     const int nbs = ceil_div(m1, 1024);   // set-up partials
-    H.cg_part.ensure(std::max(2L * nbc + nbu, 2L * nbs));
-    H.cg_scal.ensure(8);
     if (jacobi) {
         H.cg_dinv.ensure(m1);
         if (H.mf) {
@@ -725,84 +753,106 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
         }
     }
     const double* dinv = jacobi ? H.cg_dinv.p : nullptr;
-    H.cg_ctl.ensure(4);
-    H.cg_hctl.ensure(16);
-    H.hbuf.ensure(8);
 
-    // x = x0 (or 0), r = rhs - Delta x, u = D^-1 r (or r), w = Delta u (in q),
-    // p = s = 0 (beta_0 = 0)
-    if (x0) {
-        vec_copy(x, x0, m1, s);
-        hess_apply_delta(p, x0, shift, H.cg_q);
-        vec_axpby(H.cg_r, 1.0, rhs, -1.0, H.cg_q, m1, s);
-    } else {
-        launch_fill(x, 0.0, m1, s);
-        vec_copy(H.cg_r, rhs, m1, s);
+    // per side: x = x0 (or 0), r = rhs - Delta x, u = D^-1 r (or r),
+    // w = Delta u (in q), p = s = 0 (beta_0 = 0)
+    for (int k = 0; k < nr; k++) {
+        CgSide& c = S[k];
+        c.r->ensure(m1);
+        c.pv->ensure(m1);
+        c.q->ensure(m1);
+        c.sv->ensure(m1);
+        c.u->ensure(m1);
+        c.y->ensure(n);
+        c.part->ensure(std::max(2L * nbc + nbu, 2L * nbs));
+        c.scal->ensure(8);
+        c.ctl->ensure(4);
+        c.hctl->ensure(16);
+        c.hbuf->ensure(8);
+        if (c.x0) {
+            vec_copy(c.x, c.x0, m1, s);
+            hess_apply_delta(p, c.x0, shift, *c.q);
+            vec_axpby(*c.r, 1.0, c.rhs, -1.0, *c.q, m1, s);
+        } else {
+            launch_fill(c.x, 0.0, m1, s);
+            vec_copy(*c.r, c.rhs, m1, s);
+        }
+        hipLaunchKernelGGL(k_pcg_start, dim3(nbs), dim3(256), 0, s, c.r->p, dinv, m1, c.u->p,
+                           c.part->p);
+        KERNEL_CHECK();
+        hess_apply_delta(p, *c.u, shift, *c.q);
+        const double* xs[2] = {c.rhs, c.q->p};
+        const double* ys[2] = {c.rhs, c.u->p};
+        vec_dots(xs, ys, m1, 2, c.scal->p + 6, s);
+        c.hpart.assign(2L * nbs, 0.0);
+        HIP_CHECK(hipMemcpyAsync(c.hpart.data(), c.part->p, 2L * nbs * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(c.hbuf->p, c.scal->p + 6, 2 * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+        launch_fill(*c.pv, 0.0, m1, s);
+        launch_fill(*c.sv, 0.0, m1, s);
     }
-    hipLaunchKernelGGL(k_pcg_start, dim3(nbs), dim3(256), 0, s, H.cg_r.p, dinv, m1, H.cg_u.p,
-                       H.cg_part.p);
-    KERNEL_CHECK();
-    hess_apply_delta(p, H.cg_u, shift, H.cg_q);
-    const double* xs[2] = {rhs, H.cg_q.p};
-    const double* ys[2] = {rhs, H.cg_u.p};
-    vec_dots(xs, ys, m1, 2, H.cg_scal.p + 6, s);
-    std::vector<double> hpart(2L * nbs);
-    HIP_CHECK(hipMemcpyAsync(hpart.data(), H.cg_part.p, 2L * nbs * sizeof(double),
-                             hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(H.hbuf.p, H.cg_scal.p + 6, 2 * sizeof(double),
-                             hipMemcpyDeviceToHost, s));
-    launch_fill(H.cg_p, 0.0, m1, s);
-    launch_fill(H.cg_s, 0.0, m1, s);
     HIP_CHECK(hipStreamSynchronize(s));
-    const double rhs_norm2 = H.hbuf.p[0];
-    const double delta0 = H.hbuf.p[1];
-    double rz0 = 0.0, rr0 = 0.0;
-    for (int b = 0; b < nbs; b++) {
-        rz0 += hpart[2 * b];
-        rr0 += hpart[2 * b + 1];
+    int act[2] = {0, 0};
+    int na = 0;
+    for (int k = 0; k < nr; k++) {
+        CgSide& c = S[k];
+        c.rhs_norm2 = c.hbuf->p[0];
+        const double delta0 = c.hbuf->p[1];
+        double rz0 = 0.0, rr0 = 0.0;
+        for (int b = 0; b < nbs; b++) {
+            rz0 += c.hpart[2 * b];
+            rr0 += c.hpart[2 * b + 1];
+        }
+        c.iters = 0;
+        c.active = false;
+        if (c.rhs_norm2 == 0.0) {
+            if (c.x0) launch_fill(c.x, 0.0, m1, s);
+            c.tol_error = 0.0;
+            continue;
+        }
+        double thr = tol * tol * c.rhs_norm2;
+        if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+        // converged at the start (tol >= 1 without a guess, or an exact guess)
+        if (rr0 < thr || max_iter <= 0) {
+            c.tol_error = std::sqrt(rr0 / c.rhs_norm2);
+            continue;
+        }
+        double hs[7] = {rz0 / delta0, 0.0, rz0, 0.0, rr0, thr, c.rhs_norm2};
+        HIP_CHECK(hipMemcpyAsync(c.scal->p, hs, 7 * sizeof(double), hipMemcpyHostToDevice, s));
+        int hc[4] = {0, 0, 0, 0};
+        HIP_CHECK(hipMemcpyAsync(c.ctl->p, hc, 4 * sizeof(int), hipMemcpyHostToDevice, s));
+        CgArgs& a = c.a;
+        a.rowptr = H.rowptr;
+        a.colind = H.colind;
+        a.rval = H.rval;
+        a.colptr = H.colptr;
+        a.rowind = H.rowind;
+        a.cval = H.cval;
+        a.h1 = H.h1;
+        a.h2 = H.h2;
+        a.shift = shift;
+        a.n = n;
+        a.m1 = m1;
+        a.x = c.x;
+        a.r = *c.r;
+        a.p = *c.pv;
+        a.s = *c.sv;
+        a.w = *c.q;
+        a.u = *c.u;
+        a.y = *c.y;
+        a.part = *c.part;
+        a.scal = *c.scal;
+        a.ctl = *c.ctl;
+        a.nbc = nbc;
+        a.nbu = nbu;
+        a.dinv = dinv;
+        c.active = true;
+        act[na++] = k;
     }
-    if (rhs_norm2 == 0.0) {
-        if (x0) launch_fill(x, 0.0, m1, s);
-        if (tol_error) *tol_error = 0.0;
-        return 0;
-    }
-    double thr = tol * tol * rhs_norm2;
-    if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
-    // converged at the start (tol >= 1 without a guess, or an exact guess)
-    if (rr0 < thr || max_iter <= 0) {
-        if (tol_error) *tol_error = std::sqrt(rr0 / rhs_norm2);
-        return 0;
-    }
-    double hs[7] = {rz0 / delta0, 0.0, rz0, 0.0, rr0, thr, rhs_norm2};
-    HIP_CHECK(hipMemcpyAsync(H.cg_scal.p, hs, 7 * sizeof(double), hipMemcpyHostToDevice, s));
-    int hc[4] = {0, 0, 0, 0};
-    HIP_CHECK(hipMemcpyAsync(H.cg_ctl.p, hc, 4 * sizeof(int), hipMemcpyHostToDevice, s));
-
-    CgArgs a;
-    a.rowptr = H.rowptr;
-    a.colind = H.colind;
-    a.rval = H.rval;
-    a.colptr = H.colptr;
-    a.rowind = H.rowind;
-    a.cval = H.cval;
-    a.h1 = H.h1;
-    a.h2 = H.h2;
-    a.shift = shift;
-    a.n = n;
-    a.m1 = m1;
-    a.x = x;
-    a.r = H.cg_r;
-    a.p = H.cg_p;
-    a.s = H.cg_s;
-    a.w = H.cg_q;
-    a.u = H.cg_u;
-    a.y = H.cg_y_buf;
-    a.part = H.cg_part;
-    a.scal = H.cg_scal;
-    a.ctl = H.cg_ctl;
-    a.nbc = nbc;
-    a.nbu = nbu;
-    a.dinv = dinv;
+    if (na == 0) return;
+    const CgArgs& a0 = S[act[0]].a;
+    const CgArgs& a1 = S[act[na - 1]].a;
 
     // batches of K iterations, two in flight; iteration indices run 0 ..
     // max_iter - 1 and nothing is enqueued past them
@@ -816,48 +866,56 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     MfArgs ma;
     if (H.mf) {
         ma = mf_args(p, H);
-        ma.xc = H.cg_u;
+        ma.xc = a0.u;
         ma.shift = shift;
         ma.part = H.mf_part;
-        ma.ctl = H.cg_ctl;
-        ma.rr_part = H.cg_part.p + 2 * nbc;
+        ma.ctl = a0.ctl;
+        ma.rr_part = a0.part + 2 * nbc;
         ma.rr_n = nbu;
-        ma.scal = H.cg_scal;
+        ma.scal = a0.scal;
     }
     auto launch_batch = [&](int slot) {
         for (int q = 0; q < K && k_enq < max_iter; q++, k_enq++) {
-            hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, a, k_enq);
+            hipLaunchKernelGGL(k_cg_update, dim3(nbu, na), dim3(256), 0, s, a0, a1, k_enq);
             if (H.mf) {
                 ma.it = k_enq;
                 launch_mf_pass(MF_CG, ma, H.mf_nwg, s);
-                hipLaunchKernelGGL(k_cg_cols_mf, dim3(nbc), dim3(256), 0, s, a, H.mf_part.p,
+                hipLaunchKernelGGL(k_cg_cols_mf, dim3(nbc), dim3(256), 0, s, a0, H.mf_part.p,
                                    H.mf_nwg);
             } else {
-                cg_rows(lpr_r, a, s);
-                cg_cols(lpr_c, a, k_enq, s);
+                cg_rows(lpr_r, a0, a1, na, s);
+                cg_cols(lpr_c, a0, a1, na, k_enq, s);
             }
         }
         KERNEL_CHECK();
-        HIP_CHECK(hipMemcpyAsync(H.cg_hctl.p + 4 * slot, H.cg_ctl.p, 4 * sizeof(int),
-                                 hipMemcpyDeviceToHost, s));
+        for (int j = 0; j < na; j++) {
+            CgSide& c = S[act[j]];
+            HIP_CHECK(hipMemcpyAsync(c.hctl->p + 4 * slot, c.ctl->p, 4 * sizeof(int),
+                                     hipMemcpyDeviceToHost, s));
+        }
         HIP_CHECK(hipEventRecord(ev[slot], s));
     };
+    auto all_done = [&](int slot) {
+        for (int j = 0; j < na; j++)
+            if (!S[act[j]].hctl->p[4 * slot]) return false;
+        return true;
+    };
     try {
-        std::deque<int> q;
+        std::deque<int> qu;
         launch_batch(0);
-        q.push_back(0);
+        qu.push_back(0);
         if (k_enq < max_iter) {
             launch_batch(1);
-            q.push_back(1);
+            qu.push_back(1);
         }
-        while (!q.empty()) {
-            const int slot = q.front();
-            q.pop_front();
+        while (!qu.empty()) {
+            const int slot = qu.front();
+            qu.pop_front();
             HIP_CHECK(hipEventSynchronize(ev[slot]));
-            if (H.cg_hctl.p[4 * slot]) break;
+            if (all_done(slot)) break;
             if (k_enq < max_iter) {
                 launch_batch(slot);
-                q.push_back(slot);
+                qu.push_back(slot);
             }
         }
         HIP_CHECK(hipStreamSynchronize(s));
@@ -869,11 +927,91 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
     }
     HIP_CHECK(hipEventDestroy(ev[0]));
     HIP_CHECK(hipEventDestroy(ev[1]));
-    HIP_CHECK(hipMemcpy(H.cg_hctl.p + 8, H.cg_ctl.p, 4 * sizeof(int), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(H.hbuf.p, H.cg_scal.p, 5 * sizeof(double), hipMemcpyDeviceToHost));
-    const int iters = H.cg_hctl.p[8] ? H.cg_hctl.p[9] : max_iter;
-    if (tol_error) *tol_error = std::sqrt(H.hbuf.p[4] / rhs_norm2);
-    return iters;
+    for (int j = 0; j < na; j++) {
+        CgSide& c = S[act[j]];
+        HIP_CHECK(hipMemcpy(c.hctl->p + 8, c.ctl->p, 4 * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(c.hbuf->p, c.scal->p, 5 * sizeof(double), hipMemcpyDeviceToHost));
+        c.iters = c.hctl->p[8] ? c.hctl->p[9] : max_iter;
+        c.tol_error = std::sqrt(c.hbuf->p[4] / c.rhs_norm2);
+    }
+}
+
+// Side k of the Hessian's CG state (0: cg_*, 1: cg1_*).
+static CgSide cg_side(SparseHessian& H, int k)
+{
+    CgSide c;
+    if (k == 0) {
+        c.r = &H.cg_r;
+        c.pv = &H.cg_p;
+        c.q = &H.cg_q;
+        c.sv = &H.cg_s;
+        c.u = &H.cg_u;
+        c.y = &H.cg_y_buf;
+        c.part = &H.cg_part;
+        c.scal = &H.cg_scal;
+        c.ctl = &H.cg_ctl;
+        c.hctl = &H.cg_hctl;
+        c.hbuf = &H.hbuf;
+    } else {
+        c.r = &H.cg1_r;
+        c.pv = &H.cg1_p;
+        c.q = &H.cg1_q;
+        c.sv = &H.cg1_s;
+        c.u = &H.cg1_u;
+        c.y = &H.cg1_y;
+        c.part = &H.cg1_part;
+        c.scal = &H.cg1_scal;
+        c.ctl = &H.cg1_ctl;
+        c.hctl = &H.cg1_hctl;
+        c.hbuf = &H.hbuf1;
+    }
+    return c;
+}
+
+int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
+                  double* x, double* tol_error, const double* x0, bool jacobi)
+{
+    CgSide c = cg_side(hess_of(p), 0);
+    c.rhs = rhs;
+    c.x = x;
+    c.x0 = x0;
+    cg_delta_sides(p, 1, &c, shift, tol, max_iter, jacobi);
+    if (tol_error) *tol_error = c.tol_error;
+    return c.iters;
+}
+
+// The Schur right-hand side z - sigma' (w / (h1 + s)) of rhs = [w; z] into
+// zs (m-1), with ybuf (n) as scratch.
+static void cg_schur_rhs(regot_problem* p, SparseHessian& H, const double* rhs, double shift,
+                         double* ybuf, double* zs)
+{
+    const int n = p->n, m1 = p->m - 1;
+    hipStream_t s = p->stream;
+    hipLaunchKernelGGL(k_diag_axpy, dim3(ceil_div(n, 256)), dim3(256), 0, s, ybuf, H.h1.p, rhs,
+                       shift, n, 1);
+    KERNEL_CHECK();
+    if (H.mf) {
+        mf_pass(p, H, MF_COLS, nullptr, ybuf, 0.0, nullptr);
+        launch_mf_fold(H.mf_part, H.mf_nwg, m1, 1, rhs + n, nullptr, 0.0, zs, s);
+    } else {
+        launch_spmv_cols(H, ybuf, nullptr, 0.0, 0, zs, s);
+        vec_axpby(zs, -1.0, zs, 1.0, rhs + n, m1, s);
+    }
+}
+
+// out[0:n] = (w - sigma r2) / (h1 + s) from the Schur solution r2 = out[n:].
+static void cg_back(regot_problem* p, SparseHessian& H, const double* rhs, double shift,
+                    double* ybuf, double* out)
+{
+    const int n = p->n;
+    hipStream_t s = p->stream;
+    if (H.mf)
+        mf_pass(p, H, MF_ROWS, out + n, nullptr, 0.0, ybuf);
+    else
+        launch_spmv_rows(H, out + n, ybuf, false, 0.0, s);
+    hipLaunchKernelGGL(k_sub_div, dim3(ceil_div(n, 256)), dim3(256), 0, s, rhs, ybuf, H.h1.p,
+                       shift, n, out);
+    KERNEL_CHECK();
 }
 
 // (Hs + shift I)^{-1} rhs by block elimination (src/sinkhorn_cg.cpp:9-50):
@@ -883,31 +1021,57 @@ int hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int m
 {
     SparseHessian& H = hess_of(p);
     const int n = p->n, m1 = p->m - 1;
-    hipStream_t s = p->stream;
     if (max_iter <= 0) max_iter = 2 * p->m;
     H.vn.ensure(n);
     H.vm2.ensure(std::max(m1, 1));
     H.cg_y_buf.ensure(n);
-    hipLaunchKernelGGL(k_diag_axpy, dim3(ceil_div(n, 256)), dim3(256), 0, s, H.cg_y_buf.p, H.h1.p,
-                       rhs, shift, n, 1);
-    KERNEL_CHECK();
-    if (H.mf) {
-        mf_pass(p, H, MF_COLS, nullptr, H.cg_y_buf, 0.0, nullptr);
-        launch_mf_fold(H.mf_part, H.mf_nwg, m1, 1, rhs + n, nullptr, 0.0, H.vm2, s);
-    } else {
-        launch_spmv_cols(H, H.cg_y_buf, nullptr, 0.0, 0, H.vm2, s);
-        vec_axpby(H.vm2, -1.0, H.vm2, 1.0, rhs + n, m1, s);
-    }
+    cg_schur_rhs(p, H, rhs, shift, H.cg_y_buf, H.vm2);
     const int it = hess_cg_delta(p, H.vm2, shift, tol, max_iter, out + n, tol_error,
                                  x0 ? x0 + n : nullptr, jacobi);
-    if (H.mf)
-        mf_pass(p, H, MF_ROWS, out + n, nullptr, 0.0, H.cg_y_buf);
-    else
-        launch_spmv_rows(H, out + n, H.cg_y_buf, false, 0.0, s);
-    hipLaunchKernelGGL(k_sub_div, dim3(ceil_div(n, 256)), dim3(256), 0, s, rhs, H.cg_y_buf.p,
-                       H.h1.p, shift, n, out);
-    KERNEL_CHECK();
+    cg_back(p, H, rhs, shift, H.cg_y_buf, out);
     return it;
+}
+
+int hess_cg2(regot_problem* p, const double* rhs0, const double* rhs1, double shift, double tol,
+             int max_iter, double* out0, double* out1, int* iters, double* tol_error,
+             bool jacobi)
+{
+    SparseHessian& H = hess_of(p);
+    const int n = p->n, m1 = p->m - 1;
+    if (H.mf) {
+        // the matrix-free CG serves one right-hand side: two solves
+        double e0 = 0.0, e1 = 0.0;
+        iters[0] = hess_cg(p, rhs0, shift, tol, max_iter, out0, &e0, nullptr, jacobi);
+        iters[1] = hess_cg(p, rhs1, shift, tol, max_iter, out1, &e1, nullptr, jacobi);
+        if (tol_error) {
+            tol_error[0] = e0;
+            tol_error[1] = e1;
+        }
+        return iters[0] + iters[1];
+    }
+    if (max_iter <= 0) max_iter = 2 * p->m;
+    H.vn.ensure(n);
+    H.vm2.ensure(std::max(m1, 1));
+    H.cg1_rhs.ensure(std::max(m1, 1));
+    H.cg_y_buf.ensure(n);
+    H.cg1_y.ensure(n);
+    cg_schur_rhs(p, H, rhs0, shift, H.cg_y_buf, H.vm2);
+    cg_schur_rhs(p, H, rhs1, shift, H.cg1_y, H.cg1_rhs);
+    CgSide c[2] = {cg_side(H, 0), cg_side(H, 1)};
+    c[0].rhs = H.vm2;
+    c[0].x = out0 + n;
+    c[1].rhs = H.cg1_rhs;
+    c[1].x = out1 + n;
+    cg_delta_sides(p, 2, c, shift, tol, max_iter, jacobi);
+    cg_back(p, H, rhs0, shift, H.cg_y_buf, out0);
+    cg_back(p, H, rhs1, shift, H.cg1_y, out1);
+    iters[0] = c[0].iters;
+    iters[1] = c[1].iters;
+    if (tol_error) {
+        tol_error[0] = c[0].tol_error;
+        tol_error[1] = c[1].tol_error;
+    }
+    return iters[0] + iters[1];
 }
 
 }  // namespace regot
@@ -1040,6 +1204,29 @@ int regot_hess_apply_hs(regot_problem* p, const double* x, double* res)
     hess_apply_hs(p, dx, dr);
     HIP_CHECK(hipMemcpyAsync(res, dr, N * sizeof(double), hipMemcpyDeviceToHost, p->stream));
     p->sync();
+    CAPI_END
+}
+
+int regot_hess_cg2(regot_problem* p, const double* rhs0, const double* rhs1, double shift,
+                   double tol, int max_iter, double* res0, double* res1, int* iters,
+                   double* tol_error)
+{
+    CAPI_BEGIN
+    hess_check(p, true);
+    if (!rhs0 || !rhs1 || !res0 || !res1 || !iters) fail(E_INVALID, "null argument");
+    const long N = p->n + p->m - 1;
+    DBuf<double> d0(N), d1(N), r0(N), r1(N);
+    HIP_CHECK(hipMemcpyAsync(d0, rhs0, N * sizeof(double), hipMemcpyHostToDevice, p->stream));
+    HIP_CHECK(hipMemcpyAsync(d1, rhs1, N * sizeof(double), hipMemcpyHostToDevice, p->stream));
+    double te[2] = {0.0, 0.0};
+    hess_cg2(p, d0, d1, shift, tol, max_iter, r0, r1, iters, te);
+    HIP_CHECK(hipMemcpyAsync(res0, r0, N * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    HIP_CHECK(hipMemcpyAsync(res1, r1, N * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    p->sync();
+    if (tol_error) {
+        tol_error[0] = te[0];
+        tol_error[1] = te[1];
+    }
     CAPI_END
 }
 

@@ -30,5 +30,12 @@ int hess_cg_delta(regot_problem* p, const double* rhs, double shift, double tol,
 // only its last m-1 entries are used) warm-starts the CG; returns CG iterations.
 int hess_cg(regot_problem* p, const double* rhs, double shift, double tol, int max_iter,
             double* out, double* tol_error, const double* x0 = nullptr, bool jacobi = false);
+// Two right-hand sides of the same system, their CGs run in the same
+// launches (stored Hessian; two hess_cg solves when matrix-free); each gives
+// the iterates of its own hess_cg bit for bit.  iters[2], tol_error[2]
+// (nullable); returns the sum of the iterations.
+int hess_cg2(regot_problem* p, const double* rhs0, const double* rhs1, double shift, double tol,
+             int max_iter, double* out0, double* out1, int* iters, double* tol_error,
+             bool jacobi = false);
 
 }  // namespace regot

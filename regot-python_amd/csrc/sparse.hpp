@@ -68,6 +68,11 @@ struct SparseHessian {
     DBuf<float> tau_cand;
     DBuf<double> cg_r, cg_p, cg_q, cg_s, cg_u, cg_y_buf, cg_part, cg_scal, cg_dinv;
     DBuf<int> cg_ctl;
+    // the second right-hand side of a paired CG (hess_cg2): the same state
+    DBuf<double> cg1_r, cg1_p, cg1_q, cg1_s, cg1_u, cg1_y, cg1_part, cg1_scal, cg1_rhs;
+    DBuf<int> cg1_ctl;
+    HBuf<int> cg1_hctl;
+    HBuf<double> hbuf1;
     // Matrix-free build (FP32 bulk): no CSR / CSC; the kept entries as the
     // dense FP32 matrix mf_t (K_ij = T_ij where kept, else 0; row pitch
     // mf_ldt) and every product is a pass over it by mf_nwg workgroups,

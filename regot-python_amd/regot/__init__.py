@@ -451,6 +451,20 @@ class Problem:
                                   dptr(out), _lib.ctypes.byref(it), _lib.ctypes.byref(te)))
         return out, it.value, te.value
 
+    def hess_cg2(self, rhs0, rhs1, shift, tol=1e-8, max_iter=0):
+        """Two right-hand sides in the same CG launches -> ((x0, x1), (it0, it1),
+        (err0, err1)); each as hess_cg would return it."""
+        N = self.n + self.m - 1
+        rhs0 = _vec(rhs0, N, "rhs0")
+        rhs1 = _vec(rhs1, N, "rhs1")
+        out0 = np.empty(N)
+        out1 = np.empty(N)
+        it = (_lib.ctypes.c_int * 2)()
+        te = (_lib.ctypes.c_double * 2)()
+        check(lib().regot_hess_cg2(self._h, dptr(rhs0), dptr(rhs1), float(shift), float(tol),
+                                   int(max_iter), dptr(out0), dptr(out1), it, te))
+        return (out0, out1), (it[0], it[1]), (te[0], te[1])
+
     def profile(self, on=True):
         check(lib().regot_profile_enable(self._h, int(bool(on))))
 

@@ -137,6 +137,8 @@ def lib():
     L.regot_hess_export.argtypes = [_vp, _dp, _dp, _ip, _ip, _dp, _ip, _ip, _dp]
     L.regot_hess_apply_delta.argtypes = [_vp, _dp, ctypes.c_double, _dp]
     L.regot_hess_apply_hs.argtypes = [_vp, _dp, _dp]
+    L.regot_hess_cg2.argtypes = [_vp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                 _dp, _dp, _ip, _dp]
     L.regot_hess_cg.argtypes = [_vp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
                                 _ip, _dp]
     _pp = ctypes.POINTER(_vp)
