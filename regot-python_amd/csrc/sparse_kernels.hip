@@ -656,6 +656,24 @@ __device__ int block_excl_scan_int(int v, int* total)
     return off + inc - v;
 }
 
+// Exclusive prefix, over the lanes of a wave, of per-lane flag counts (four
+// flags per lane, lane-major order) by ballots and mbcnt; *total gets the
+// wave's count.
+__device__ __forceinline__ int wave_prefix4(const bool (&f)[4], int* total)
+{
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const unsigned long long b = __ballot(f[q]);
+        pre = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(b >> 32),
+                                        __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(b),
+                                                                  pre));
+        tot += __popcll(b);
+    }
+    *total = tot;
+    return pre;
+}
+
 // Count histogram of the current digit over entries whose higher digits match
 // prefix.  Workgroup b covers rows [b R, b R + R); thread t owns the column
 // quads 4t + 1024 s, keeps their beta in registers and walks the rows four at
@@ -754,47 +772,66 @@ __global__ __launch_bounds__(256) void k_topk_gather(const TM* __restrict__ M, l
     if (threadIdx.x == 0) scnt = 0;
     __syncthreads();
     int ng = 0;
-    // 4096 columns per step (four 16-byte loads per thread in flight); each
-    // 1024-column sub-step stages at most 1024 candidates, flushed after it
+    // 4096 columns per step (four 16-byte loads per thread in flight,
+    // unconditional on full steps); each 1024-column sub-step forms its
+    // values and flags branch-free, appends its candidates with one prefix
+    // (ballots and mbcnt) and one LDS reservation per wave, and stages at
+    // most 1024 of them, flushed after it
     for (int s0 = 0; s0 < m1; s0 += 4096) {
         TM v[4][4];
         double b4[4][4];
+        if (s0 + 4096 <= m1) {   // workgroup-uniform
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int cu = s0 + 1024 * u + threadIdx.x * 4;
-            if (cu < m1) {
-                load4(row, cu, c.m, v[u]);
-                load4(c.beta, cu, c.m, b4[u]);
+            for (int u = 0; u < 4; u++) {
+                const int cu = s0 + 1024 * u + threadIdx.x * 4;
+                load4(row, cu, cu + 4, v[u]);
+                load4(c.beta, cu, cu + 4, b4[u]);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int cu = s0 + 1024 * u + threadIdx.x * 4;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    v[u][q] = TM(0);
+                    b4[u][q] = 0.0;
+                }
+                if (cu < m1) {
+                    load4(row, cu, c.m, v[u]);
+                    load4(c.beta, cu, c.m, b4[u]);
+                }
             }
         }
         const int lane = threadIdx.x & 63;
-        const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             if (s0 + 1024 * u >= m1) break;   // uniform
+            unsigned long long key[4];
+            bool isc[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int j = s0 + 1024 * u + threadIdx.x * 4 + q;
                 const bool in = j < m1;
                 bool z;
-                const TM t = in ? E::val(ai + b4[u][q], v[u][q], c.k, c.reg, z) : TM(0);
-                const unsigned long long key = static_cast<unsigned long long>(E::key(t));
-                const unsigned long long top = key >> shift1;
-                const bool isc = in && top == b1;
-                if (in && !isc) {
-                    const bool base = top > b1 || (preserve && (i == 0 || j == 0));
-                    ng += (base && t != TM(0)) ? 1 : 0;
-                }
-                // wave-aggregated append: one LDS atomic per wave
-                const unsigned long long mk = __ballot(isc);
-                if (mk) {
-                    int wb = 0;
-                    if (lane == 0) wb = atomicAdd(&scnt, __popcll(mk));
-                    wb = __shfl(wb, 0);
-                    if (isc) {
-                        const int pos = wb + __popcll(mk & lt);
-                        skey[pos] = key;
-                        scol[pos] = j;
+                const TM t = E::val(ai + b4[u][q], v[u][q], c.k, c.reg, z);
+                key[q] = static_cast<unsigned long long>(E::key(t));
+                const unsigned long long top = key[q] >> shift1;
+                isc[q] = in && top == b1;
+                const bool base = top > b1 || (preserve && (i == 0 || j == 0));
+                ng += (in && !isc[q] && base && t != TM(0)) ? 1 : 0;
+            }
+            int tot;
+            int pos = wave_prefix4(isc, &tot);
+            if (tot > 0) {   // wave-uniform
+                int wb = 0;
+                if (lane == 0) wb = atomicAdd(&scnt, tot);
+                pos += __shfl(wb, 0);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (isc[q]) {
+                        skey[pos] = key[q];
+                        scol[pos] = s0 + 1024 * u + threadIdx.x * 4 + q;
+                        pos++;
                     }
                 }
             }
@@ -960,23 +997,6 @@ __global__ void k_topk_count(const int* tcnt, const int* __restrict__ gcnt,
     cnt[i] = (gcnt[i] >> 1) + static_cast<int>(extra);
 }
 
-// Exclusive prefix, over the lanes of a wave, of per-lane flag counts (four
-// flags per lane, lane-major order) by ballots and mbcnt; *total gets the
-// wave's count.
-__device__ __forceinline__ int wave_prefix4(const bool (&f)[4], int* total)
-{
-    int pre = 0, tot = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const unsigned long long b = __ballot(f[q]);
-        pre = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(b >> 32),
-                                        __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(b),
-                                                                  pre));
-        tot += __popcll(b);
-    }
-    *total = tot;
-    return pre;
-}
 
 // T of one FP32-bulk element with the raw v_exp_f32: bit for bit
 // Elem<float>::val wherever that result is at least 2^-126; below, 0.
