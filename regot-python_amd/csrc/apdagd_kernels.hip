@@ -181,13 +181,30 @@ __global__ __launch_bounds__(256) void k_apd_marg(const double* __restrict__ row
     double e = 0.0;
     if (t < n) {
         double r = 0.0;
-        for (int s = 0; s < nstrip; s++) r += rowpart[static_cast<long>(s) * n + t];
+        // eight partials in flight, added in order
+        int s = 0;
+        for (; s + 8 <= nstrip; s += 8) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) x[u] = rowpart[static_cast<long>(s + u) * n + t];
+#pragma unroll
+            for (int u = 0; u < 8; u++) r += x[u];
+        }
+        for (; s < nstrip; s++) r += rowpart[static_cast<long>(s) * n + t];
         if (rowsum) rowsum[t] = r;
         e += (r - a[t]) * (r - a[t]);
     }
     if (t < m) {
         double c = 0.0;
-        for (int q = 0; q < nblk; q++) c += colpart[static_cast<long>(q) * m + t];
+        int q = 0;
+        for (; q + 8 <= nblk; q += 8) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) x[u] = colpart[static_cast<long>(q + u) * m + t];
+#pragma unroll
+            for (int u = 0; u < 8; u++) c += x[u];
+        }
+        for (; q < nblk; q++) c += colpart[static_cast<long>(q) * m + t];
         if (colsum) colsum[t] = c;
         e += (c - b[t]) * (c - b[t]);
     }

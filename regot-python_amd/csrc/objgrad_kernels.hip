@@ -164,14 +164,18 @@ __global__ __launch_bounds__(256) void k_sums_tile(const TM* __restrict__ M, lon
     }
 }
 
-__device__ __forceinline__ double bsum(double v, double* sm4)
+// Four workgroup sums at once, each in bsum's order (wave sums, then the
+// four wave totals left to right): one barrier instead of eight.
+__device__ __forceinline__ void bsum4(double (&v)[4], double (*sm)[4])
 {
-    v = wsum(v);
-    if ((threadIdx.x & 63) == 0) sm4[threadIdx.x >> 6] = v;
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = wsum(v[q]);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < 4; q++) sm[q][threadIdx.x >> 6] = v[q];
     __syncthreads();
-    const double r = ((sm4[0] + sm4[1]) + sm4[2]) + sm4[3];
-    __syncthreads();
-    return r;
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = ((sm[q][0] + sm[q][1]) + sm[q][2]) + sm[q][3];
 }
 
 // Thread t < n: r_t = sum over strips, grad_t = r_t - a_t; thread t < m - 1:
@@ -187,12 +191,21 @@ __global__ __launch_bounds__(256) void k_sums_fold(const double* __restrict__ ro
                                                    double* __restrict__ grad,
                                                    double* __restrict__ part)
 {
-    __shared__ double sm4[4];
+    __shared__ double sm44[4][4];
     const int t = blockIdx.x * 256 + threadIdx.x;
     double sr = 0.0, sa = 0.0, sb = 0.0, g2 = 0.0;
     if (t < n) {
+        // eight partials in flight, added in order
         double r = 0.0;
-        for (int s = 0; s < nstrip; s++) r += rowpart[static_cast<long>(s) * n + t];
+        int s = 0;
+        for (; s + 8 <= nstrip; s += 8) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) x[u] = rowpart[static_cast<long>(s + u) * n + t];
+#pragma unroll
+            for (int u = 0; u < 8; u++) r += x[u];
+        }
+        for (; s < nstrip; s++) r += rowpart[static_cast<long>(s) * n + t];
         const double g = r - a[t];
         if (grad) grad[t] = g;
         sr = r;
@@ -203,21 +216,27 @@ __global__ __launch_bounds__(256) void k_sums_fold(const double* __restrict__ ro
         sb = beta[t] * b[t];
         if (colpart) {
             double c = 0.0;
-            for (int q = 0; q < nblk; q++) c += colpart[static_cast<long>(q) * m + t];
+            int q = 0;
+            for (; q + 8 <= nblk; q += 8) {
+                double x[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) x[u] = colpart[static_cast<long>(q + u) * m + t];
+#pragma unroll
+                for (int u = 0; u < 8; u++) c += x[u];
+            }
+            for (; q < nblk; q++) c += colpart[static_cast<long>(q) * m + t];
             const double g = c - b[t];
             if (grad) grad[n + t] = g;
             g2 += g * g;
         }
     }
-    sr = bsum(sr, sm4);
-    sa = bsum(sa, sm4);
-    sb = bsum(sb, sm4);
-    g2 = bsum(g2, sm4);
+    double v[4] = {sr, sa, sb, g2};
+    bsum4(v, sm44);
     if (threadIdx.x == 0) {
-        part[4 * blockIdx.x] = sr;
-        part[4 * blockIdx.x + 1] = sa;
-        part[4 * blockIdx.x + 2] = sb;
-        part[4 * blockIdx.x + 3] = g2;
+        part[4 * blockIdx.x] = v[0];
+        part[4 * blockIdx.x + 1] = v[1];
+        part[4 * blockIdx.x + 2] = v[2];
+        part[4 * blockIdx.x + 3] = v[3];
     }
 }
 
@@ -225,21 +244,18 @@ __global__ __launch_bounds__(256) void k_sums_fold(const double* __restrict__ ro
 __global__ __launch_bounds__(256) void k_sums_final(const double* __restrict__ part, int npart,
                                                     double reg, double* __restrict__ out)
 {
-    __shared__ double sm4[4];
-    double sr = 0.0, sa = 0.0, sb = 0.0, g2 = 0.0;
+    __shared__ double sm44[4][4];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < npart; i += 256) {
-        sr += part[4 * i];
-        sa += part[4 * i + 1];
-        sb += part[4 * i + 2];
-        g2 += part[4 * i + 3];
+        v[0] += part[4 * i];
+        v[1] += part[4 * i + 1];
+        v[2] += part[4 * i + 2];
+        v[3] += part[4 * i + 3];
     }
-    sr = bsum(sr, sm4);
-    sa = bsum(sa, sm4);
-    sb = bsum(sb, sm4);
-    g2 = bsum(g2, sm4);
+    bsum4(v, sm44);
     if (threadIdx.x == 0) {
-        out[0] = reg * sr - sa - sb;
-        out[1] = g2;
+        out[0] = reg * v[0] - v[1] - v[2];
+        out[1] = v[3];
     }
 }
 
