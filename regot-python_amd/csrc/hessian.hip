@@ -172,6 +172,19 @@ __global__ __launch_bounds__(256) void k_cg_update(CgArgs a0, CgArgs a1, int it)
 {
     __shared__ double sm1[2][4], sm2[1][4];
     const CgArgs a = blockIdx.y == 0 ? a0 : a1;
+    // this thread's entries are loaded first, so that they arrive during the
+    // stop-flag read and the partial sums
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    double uj = 0.0, pj0 = 0.0, wj = 0.0, sj0 = 0.0, rj0 = 0.0, xj = 0.0, dj = 0.0;
+    if (j < a.m1) {
+        uj = a.u[j];
+        pj0 = a.p[j];
+        wj = a.w[j];
+        sj0 = a.s[j];
+        rj0 = a.r[j];
+        xj = a.x[j];
+        if (a.dinv) dj = a.dinv[j];
+    }
     if (a.ctl[0]) return;
     double al, be = 0.0;
     const int cur = it & 1, prev = cur ^ 1;
@@ -209,17 +222,16 @@ __global__ __launch_bounds__(256) void k_cg_update(CgArgs a0, CgArgs a1, int it)
             a.scal[2 + cur] = g;
         }
     }
-    const int j = blockIdx.x * 256 + threadIdx.x;
     double rr = 0.0;
     if (j < a.m1) {
-        const double pj = a.u[j] + be * a.p[j];
-        const double sj = a.w[j] + be * a.s[j];
-        const double rj = a.r[j] - al * sj;
+        const double pj = uj + be * pj0;
+        const double sj = wj + be * sj0;
+        const double rj = rj0 - al * sj;
         a.p[j] = pj;
         a.s[j] = sj;
-        a.x[j] += al * pj;
+        a.x[j] = xj + al * pj;
         a.r[j] = rj;
-        a.u[j] = a.dinv ? a.dinv[j] * rj : rj;
+        a.u[j] = a.dinv ? dj * rj : rj;
         rr = rj * rj;
     }
     double v1[1] = {rr};
@@ -232,18 +244,23 @@ template <int LPR>
 __global__ __launch_bounds__(256) void k_cg_rows(CgArgs a0, CgArgs a1)
 {
     const CgArgs a = blockIdx.y == 0 ? a0 : a1;
-    if (a.ctl[0]) return;
     const int lane = threadIdx.x & 63;
     const int sl = lane & (LPR - 1);
     const int i = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / LPR) + lane / LPR;
-    double acc = 0.0;
+    // the row's extent and diagonal are loaded with the stop flag
+    int k0 = 0, e = 0;
+    double h1 = 0.0;
     if (i < a.n) {
-        const int e = a.rowptr[i + 1];
-#pragma unroll 4
-        for (int k = a.rowptr[i] + sl; k < e; k += LPR) acc += a.rval[k] * a.u[a.colind[k]];
+        k0 = a.rowptr[i];
+        e = a.rowptr[i + 1];
+        h1 = a.h1[i];
     }
+    if (a.ctl[0]) return;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int k = k0 + sl; k < e; k += LPR) acc += a.rval[k] * a.u[a.colind[k]];
     acc = dev::group_sum<LPR>(acc);
-    if (sl == 0 && i < a.n) a.y[i] = acc / (a.h1[i] + a.shift);
+    if (sl == 0 && i < a.n) a.y[i] = acc / (h1 + a.shift);
 }
 
 // Stop test on the r.r partials of the update kernel (same order in every
@@ -255,10 +272,25 @@ __global__ __launch_bounds__(256) void k_cg_cols(CgArgs a0, CgArgs a1, int it)
     constexpr int CPW = 64 / LPR;   // columns per wave
     __shared__ double sm[2][4 * CPW];
     const CgArgs a = blockIdx.y == 0 ? a0 : a1;
-    if (a.ctl[0]) return;
+    const int lane = threadIdx.x & 63;
+    const int sl = lane & (LPR - 1);
+    const int slot = (threadIdx.x >> 6) * CPW + lane / LPR;
+    const int j = blockIdx.x * (4 * CPW) + slot;
+    // the column's extent and entries are loaded with the stop flag and the
+    // partials
+    int k0 = 0, e = 0;
+    double uj = 0.0, h2j = 0.0, rj = 0.0;
+    if (j < a.m1) {
+        k0 = a.colptr[j];
+        e = a.colptr[j + 1];
+        uj = a.u[j];
+        h2j = a.h2[j];
+        rj = a.r[j];
+    }
     // every wave sums the update's r.r partials itself, in the same order
     double v = 0.0;
-    for (int b = threadIdx.x & 63; b < a.nbu; b += 64) v += a.part[2 * a.nbc + b];
+    for (int b = lane; b < a.nbu; b += 64) v += a.part[2 * a.nbc + b];
+    if (a.ctl[0]) return;
     const double rr = dev::group_sum<64>(v);
     if (rr < a.scal[5]) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -269,24 +301,16 @@ __global__ __launch_bounds__(256) void k_cg_cols(CgArgs a0, CgArgs a1, int it)
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) a.scal[4] = rr;
-    const int lane = threadIdx.x & 63;
-    const int sl = lane & (LPR - 1);
-    const int slot = (threadIdx.x >> 6) * CPW + lane / LPR;
-    const int j = blockIdx.x * (4 * CPW) + slot;
     double t = 0.0;
-    if (j < a.m1) {
-        const int e = a.colptr[j + 1];
 #pragma unroll 4
-        for (int k = a.colptr[j] + sl; k < e; k += LPR) t += a.cval[k] * a.y[a.rowind[k]];
-    }
+    for (int k = k0 + sl; k < e; k += LPR) t += a.cval[k] * a.y[a.rowind[k]];
     t = dev::group_sum<LPR>(t);
     if (sl == 0) {
         double g = 0.0, d = 0.0;
         if (j < a.m1) {
-            const double uj = a.u[j];
-            const double wj = a.h2[j] * uj + a.shift * uj - t;
+            const double wj = h2j * uj + a.shift * uj - t;
             a.w[j] = wj;
-            g = a.r[j] * uj;
+            g = rj * uj;
             d = wj * uj;
         }
         sm[0][slot] = g;
