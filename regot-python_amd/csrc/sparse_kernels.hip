@@ -1369,39 +1369,54 @@ __global__ __launch_bounds__(256) void k_csc_strip(const TM* __restrict__ M, lon
 // Scans
 // ---------------------------------------------------------------------------
 
-// One 1024-thread workgroup; thread t scans a contiguous chunk.
+// One 1024-thread workgroup, in passes of 16384 counts: thread t holds counts
+// 16t .. 16t + 15 of the pass in registers (sixteen loads in flight), the
+// thread totals are scanned across the workgroup, and each thread writes its
+// sixteen exclusive prefixes from the registers.  out[count] = the total for
+// the int form (a CSR pointer array).
 template <typename TO>
 __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ cnt, int count,
                                                TO* __restrict__ out, long long* total)
 {
+    constexpr int kPer = 16;
     __shared__ long long wt[16];
     const int t = threadIdx.x;
-    const int per = (count + 1023) / 1024;
-    const int b = t * per, e = (b + per < count) ? b + per : count;
-    long long own = 0;
-    for (int q = b; q < e; q++) own += cnt[q];
-    // inclusive wave scan
-    long long inc = own;
     const int lane = t & 63, wave = t >> 6;
+    long long carry = 0;
+    for (int base = 0; base < count; base += 1024 * kPer) {
+        const int b = base + kPer * t;
+        int c[kPer];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const long long u = __shfl_up(inc, o);
-        if (lane >= o) inc += u;
+        for (int q = 0; q < kPer; q++) c[q] = (b + q < count) ? cnt[b + q] : 0;
+        long long own = 0;
+#pragma unroll
+        for (int q = 0; q < kPer; q++) own += c[q];
+        // inclusive wave scan of the thread totals
+        long long inc = own;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const long long u = __shfl_up(inc, o);
+            if (lane >= o) inc += u;
+        }
+        if (lane == 63) wt[wave] = inc;
+        __syncthreads();
+        long long off = carry, pass = 0;
+        for (int w = 0; w < 16; w++) {
+            if (w < wave) off += wt[w];
+            pass += wt[w];
+        }
+        long long run = off + inc - own;
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            if (b + q < count) out[b + q] = static_cast<TO>(run);
+            run += c[q];
+        }
+        carry += pass;
+        __syncthreads();   // wt is rewritten by the next pass
     }
-    if (lane == 63) wt[wave] = inc;
-    __syncthreads();
-    long long off = 0;
-    for (int w = 0; w < wave; w++) off += wt[w];
-    long long run = off + inc - own;
-    for (int q = b; q < e; q++) {
-        out[q] = static_cast<TO>(run);
-        run += cnt[q];
-    }
-    if (t == 1023) {
-        long long tot = 0;
-        for (int w = 0; w < 16; w++) tot += wt[w];
-        if (total) *total = tot;
-        if (sizeof(TO) == sizeof(int)) out[count] = static_cast<TO>(tot);
+    if (t == 0) {
+        if (total) *total = carry;
+        if (sizeof(TO) == sizeof(int)) out[count] = static_cast<TO>(carry);
     }
 }
 

@@ -357,3 +357,40 @@ def test_topk_rule_ties_repeated_build(gpu_lib):
             Sg = csc_dense(n, m, H["colptr"], H["rowind"], H["cval"])
             assert nnz == int(kept.sum())
             assert np.array_equal(Sg != 0, kept)
+
+
+@pytest.mark.parametrize("n,m", [(40000, 50), (50, 40000)])
+@pytest.mark.parametrize("rule", ["topk", "threshold"])
+def test_rules_tall_and_wide(gpu_lib, n, m, rule):
+    # Row counts, column counts and tie offsets are scanned in passes of
+    # 16384 by one workgroup: 40000 rows or 39999 columns take three passes.
+    # FP64 bulk keeps exactly the oracle's set, and CSR and CSC agree.
+    reg = 0.05
+    M, a, b = cloud_problem(n, m, seed=n + 3 * m)
+    gamma = point(M, a, b, reg, iters=2)
+    T, _ = O.compute_T(M, reg, gamma)
+    print(f"{rule} {n}x{m}", flush=True)
+    with gpu_lib.Problem(M, a, b, reg, bulk="fp64") as P:
+        if rule == "topk":
+            cp, ri, va = O.sparsify_density(T, 0.02, True)
+            nnz = P.sparsified_hess_with_density(gamma, 0.02)
+            scale = reg
+        else:
+            _, g = O.dual_obj_grad(M, a, b, reg, gamma)
+            delta = 0.01 * np.linalg.norm(g)
+            cp, ri, va = O.sparsify_threshold(T, delta, 0.001)
+            nnz = P.sparsified_hess(gamma, delta, 0.001)
+            scale = 1.0
+        H = P.hess_export()
+    assert nnz == len(ri)
+    C = csc_dense(n, m, H["colptr"], H["rowind"], H["cval"])
+    Sg = C * scale
+    So = csc_dense(n, m, cp, ri, va) / (1.0 if rule == "topk" else reg)
+    assert np.array_equal(Sg != 0, So != 0)
+    np.testing.assert_allclose(Sg, So, rtol=1e-12, atol=0)
+    rp = H["rowptr"]
+    assert rp[0] == 0 and rp[-1] == H["colptr"][-1] and np.all(np.diff(rp) >= 0)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    Sr = np.zeros((n, m - 1))
+    Sr[rows, H["colind"][:rp[-1]]] = H["rval"][:rp[-1]]
+    assert np.array_equal(Sr, C)
