@@ -1242,6 +1242,99 @@ __global__ __launch_bounds__(256) void k_csc_scatter(const int* __restrict__ row
     }
 }
 
+// k_csc_scatter with the workgroup's column offsets, row pointers and a
+// chunk of its entries in LDS (m - 1 <= kScatterCols, at most kScatterRows
+// rows per workgroup): the global loads go out in batches, sixteen column
+// offsets or four entries per thread at a time, and the rows, still taken in
+// order, depend on each other only through LDS.  Same placement as
+// k_csc_scatter: colptr[j] + off[b][j] + the entries of column j in earlier
+// rows of the workgroup.  off is not updated.
+constexpr int kScatterCols = 16384;
+constexpr int kScatterRows = 511;
+constexpr int kScatterChunk = 1024;
+
+__global__ __launch_bounds__(256) void k_csc_scatter_lds(const int* __restrict__ rowptr,
+                                                         const int* __restrict__ colind,
+                                                         const double* __restrict__ rval, int n,
+                                                         int rows_per_block, int m1,
+                                                         const int* __restrict__ colptr,
+                                                         const int* __restrict__ off,
+                                                         int* __restrict__ rowind,
+                                                         double* __restrict__ cval)
+{
+    __shared__ int soff[kScatterCols];
+    __shared__ int srp[kScatterRows + 1];
+    __shared__ int scol[kScatterChunk];
+    __shared__ double sval[kScatterChunk];
+    const int t = threadIdx.x;
+    const int* mine = off + static_cast<long>(blockIdx.x) * m1;
+    const int r0 = blockIdx.x * rows_per_block;
+    const int r1 = (r0 + rows_per_block < n) ? r0 + rows_per_block : n;
+    if (r0 >= r1) return;
+    const int nr = r1 - r0;
+    for (int q = t; q <= nr; q += 256) srp[q] = rowptr[r0 + q];
+    for (int j0 = 0; j0 < m1; j0 += 256 * 16) {
+        int c[16], o[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int j = j0 + 256 * u + t;
+            c[u] = (j < m1) ? colptr[j] : 0;
+            o[u] = (j < m1) ? mine[j] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int j = j0 + 256 * u + t;
+            if (j < m1) soff[j] = c[u] + o[u];
+        }
+    }
+    __syncthreads();
+    const int k_end = srp[nr];
+    int i = 0;   // the next row to place (the same on every thread)
+    for (int c0 = srp[0]; c0 < k_end; c0 += kScatterChunk) {
+        const int c1 = (c0 + kScatterChunk < k_end) ? c0 + kScatterChunk : k_end;
+        int cc[kScatterChunk / 256];
+        double vv[kScatterChunk / 256];
+#pragma unroll
+        for (int u = 0; u < kScatterChunk / 256; u++) {
+            const int k = c0 + 256 * u + t;
+            if (k < c1) {
+                cc[u] = colind[k];
+                vv[u] = rval[k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kScatterChunk / 256; u++) {
+            const int k = c0 + 256 * u + t;
+            if (k < c1) {
+                scol[k - c0] = cc[u];
+                sval[k - c0] = vv[u];
+            }
+        }
+        __syncthreads();
+        // the rows with entries in [c0, c1), in order; a row that runs past c1
+        // is finished in the next chunk
+        while (i < nr) {
+            const int rb = srp[i], re = srp[i + 1];
+            if (rb >= c1) break;
+            const int b = (rb > c0) ? rb : c0;
+            const int e = (re < c1) ? re : c1;
+            for (int k = b + t; k < e; k += 256) {
+                const int j = scol[k - c0];
+                if (static_cast<unsigned>(j) < static_cast<unsigned>(m1)) {
+                    const int q = soff[j];
+                    soff[j] = q + 1;
+                    rowind[q] = r0 + i;
+                    cval[q] = sval[k - c0];
+                }
+            }
+            __syncthreads();
+            if (re > c1) break;
+            i++;
+        }
+        __syncthreads();   // scol and sval are rewritten by the next chunk
+    }
+}
+
 // True when row i of the CSR stores column j (binary search; the columns of
 // a row are ascending).
 __device__ __forceinline__ bool csr_has(const int* __restrict__ rowptr,
@@ -1737,8 +1830,13 @@ void launch_build_csc(int bulk, const void* M, long ldm, const void* MT, long ld
     KERNEL_CHECK();
     launch_scan(colcnt, m1, colptr, total, s);
     if (how == 0) {
-        hipLaunchKernelGGL(k_csc_scatter, dim3(nb), dim3(256), 0, s, H.rowptr.p, H.colind.p,
-                           H.rval.p, n, rows_per_block, m1, colptr, work, rowind, cval);
+        if (m1 <= kScatterCols && rows_per_block <= kScatterRows)
+            hipLaunchKernelGGL(k_csc_scatter_lds, dim3(nb), dim3(256), 0, s, H.rowptr.p,
+                               H.colind.p, H.rval.p, n, rows_per_block, m1, colptr, work, rowind,
+                               cval);
+        else
+            hipLaunchKernelGGL(k_csc_scatter, dim3(nb), dim3(256), 0, s, H.rowptr.p, H.colind.p,
+                               H.rval.p, n, rows_per_block, m1, colptr, work, rowind, cval);
     } else if (how == 2) {
         // threshold rule on M': row j of M' is column j of M; the rule is
         // symmetric in (tau_j, rho_i) and alpha_i + beta_j is exact either way
