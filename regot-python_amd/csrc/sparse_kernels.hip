@@ -1214,6 +1214,74 @@ __global__ __launch_bounds__(256) void k_csc_prefix(int* __restrict__ cnt, int n
     total[j] = run;
 }
 
+// k_csc_count with the workgroup's column counts in LDS (m - 1 <= 16384):
+// LDS atomics, eight entries per thread in flight, then every count of the
+// workgroup is written, zeros included, so cnt needs no clearing first.
+__global__ __launch_bounds__(256) void k_csc_count_lds(const int* __restrict__ rowptr,
+                                                       const int* __restrict__ colind, int n,
+                                                       int rows_per_block, int m1,
+                                                       int* __restrict__ cnt)
+{
+    __shared__ int sc[16384];
+    const int t = threadIdx.x;
+    int* mine = cnt + static_cast<long>(blockIdx.x) * m1;
+    const int r0 = blockIdx.x * rows_per_block;
+    const int r1 = (r0 + rows_per_block < n) ? r0 + rows_per_block : n;
+    for (int j = t; j < m1; j += 256) sc[j] = 0;
+    __syncthreads();
+    if (r0 < r1) {
+        const int k1 = rowptr[r1];
+        for (int k = rowptr[r0] + t; k < k1; k += 256 * 8) {
+            int jj[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) jj[u] = (k + 256 * u < k1) ? colind[k + 256 * u] : -1;
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (static_cast<unsigned>(jj[u]) < static_cast<unsigned>(m1)) atomicAdd(&sc[jj[u]], 1);
+        }
+    }
+    __syncthreads();
+    for (int j = t; j < m1; j += 256) mine[j] = sc[j];
+}
+
+// k_csc_prefix for nb <= 512 with 64 columns per 1024-thread workgroup: wave w
+// holds counts b = w L .. w L + L - 1 (L = ceil(nb / 16) <= 32) of its lane's
+// column in registers, the sixteen segment totals are exchanged through LDS,
+// and each wave writes its exclusive prefixes.
+__global__ __launch_bounds__(1024) void k_csc_prefix16(int* __restrict__ cnt, int nb, int m1,
+                                                       int* __restrict__ total)
+{
+    __shared__ int st[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + lane;
+    const int len = (nb + 15) / 16;
+    const int b0 = w * len;
+    int c[32];
+    int own = 0;
+#pragma unroll
+    for (int q = 0; q < 32; q++) {
+        const int b = b0 + q;
+        c[q] = (q < len && b < nb && j < m1) ? cnt[static_cast<long>(b) * m1 + j] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 32; q++) own += c[q];
+    st[w][lane] = own;
+    __syncthreads();
+    int run = 0, tot = 0;
+    for (int q = 0; q < 16; q++) {
+        const int v = st[q][lane];
+        if (q < w) run += v;
+        tot += v;
+    }
+#pragma unroll
+    for (int q = 0; q < 32; q++) {
+        const int b = b0 + q;
+        if (q < len && b < nb && j < m1) cnt[static_cast<long>(b) * m1 + j] = run;
+        run += c[q];
+    }
+    if (w == 0 && j < m1) total[j] = tot;
+}
+
 // Workgroup b scatters its rows in order: entry (i, j) goes to
 // colptr[j] + off[b][j], then off[b][j] += 1; rows are separated by a barrier,
 // so within a column the entries land in row order.
@@ -1822,11 +1890,20 @@ void launch_build_csc(int bulk, const void* M, long ldm, const void* MT, long ld
     const int n = H.n, m1 = H.m - 1;
     if (m1 <= 0) return;
     const int nb = ceil_div(n, rows_per_block);
-    HIP_CHECK(hipMemsetAsync(work, 0, sizeof(int) * static_cast<size_t>(nb) * m1, s));
-    hipLaunchKernelGGL(k_csc_count, dim3(nb), dim3(256), 0, s, H.rowptr.p, H.colind.p, n,
-                       rows_per_block, m1, work);
-    hipLaunchKernelGGL(k_csc_prefix, dim3(ceil_div(m1, 256)), dim3(256), 0, s, work, nb, m1,
-                       colcnt);
+    if (m1 <= 16384) {
+        hipLaunchKernelGGL(k_csc_count_lds, dim3(nb), dim3(256), 0, s, H.rowptr.p, H.colind.p, n,
+                           rows_per_block, m1, work);
+    } else {
+        HIP_CHECK(hipMemsetAsync(work, 0, sizeof(int) * static_cast<size_t>(nb) * m1, s));
+        hipLaunchKernelGGL(k_csc_count, dim3(nb), dim3(256), 0, s, H.rowptr.p, H.colind.p, n,
+                           rows_per_block, m1, work);
+    }
+    if (nb <= 512)
+        hipLaunchKernelGGL(k_csc_prefix16, dim3(ceil_div(m1, 64)), dim3(1024), 0, s, work, nb,
+                           m1, colcnt);
+    else
+        hipLaunchKernelGGL(k_csc_prefix, dim3(ceil_div(m1, 256)), dim3(256), 0, s, work, nb, m1,
+                           colcnt);
     KERNEL_CHECK();
     launch_scan(colcnt, m1, colptr, total, s);
     if (how == 0) {
