@@ -39,15 +39,38 @@ struct DotArgs {
     const double* y[8];
 };
 
+// out[p] = x_p . y_p for p < npairs: thread t sums its elements t, t + 1024,
+// ... in order (sixteen loads of each vector in flight), then the 1024
+// partials of every pair go through the same halving tree in LDS together.
 __global__ __launch_bounds__(1024) void k_dots(DotArgs d, long count, int npairs, double* out)
 {
-    __shared__ double sm[1024];
+    __shared__ double sm[8][1024];
+    const int t = threadIdx.x;
     for (int p = 0; p < npairs; p++) {
+        const double* __restrict__ x = d.x[p];
+        const double* __restrict__ y = d.y[p];
         double s = 0.0;
-        for (long i = threadIdx.x; i < count; i += 1024) s += d.x[p][i] * d.y[p][i];
-        s = block_reduce(s, sm);
-        if (threadIdx.x == 0) out[p] = s;
+        for (long i0 = t; i0 < count; i0 += 16 * 1024) {
+            double xv[16], yv[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const long i = i0 + 1024L * u;
+                xv[u] = (i < count) ? x[i] : 0.0;
+                yv[u] = (i < count) ? y[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; u++)
+                if (i0 + 1024L * u < count) s += xv[u] * yv[u];
+        }
+        sm[p][t] = s;
     }
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (t < o)
+            for (int p = 0; p < npairs; p++) sm[p][t] += sm[p][t + o];
+        __syncthreads();
+    }
+    if (t < npairs) out[t] = sm[t][0];
 }
 
 __global__ __launch_bounds__(1024) void k_objgrad_finish(const double* r, const double* c,
