@@ -39,38 +39,35 @@ struct DotArgs {
     const double* y[8];
 };
 
-// out[p] = x_p . y_p for p < npairs: thread t sums its elements t, t + 1024,
-// ... in order (sixteen loads of each vector in flight), then the 1024
-// partials of every pair go through the same halving tree in LDS together.
-__global__ __launch_bounds__(1024) void k_dots(DotArgs d, long count, int npairs, double* out)
+// out[p] = x_p . y_p, one 1024-thread workgroup per pair: thread t sums its
+// elements t, t + 1024, ... in order (sixteen loads of each vector in
+// flight), then the 1024 partials go through a halving tree in LDS.
+__global__ __launch_bounds__(1024) void k_dots(DotArgs d, long count, double* out)
 {
-    __shared__ double sm[8][1024];
-    const int t = threadIdx.x;
-    for (int p = 0; p < npairs; p++) {
-        const double* __restrict__ x = d.x[p];
-        const double* __restrict__ y = d.y[p];
-        double s = 0.0;
-        for (long i0 = t; i0 < count; i0 += 16 * 1024) {
-            double xv[16], yv[16];
+    __shared__ double sm[1024];
+    const int t = threadIdx.x, p = blockIdx.x;
+    const double* __restrict__ x = d.x[p];
+    const double* __restrict__ y = d.y[p];
+    double s = 0.0;
+    for (long i0 = t; i0 < count; i0 += 16 * 1024) {
+        double xv[16], yv[16];
 #pragma unroll
-            for (int u = 0; u < 16; u++) {
-                const long i = i0 + 1024L * u;
-                xv[u] = (i < count) ? x[i] : 0.0;
-                yv[u] = (i < count) ? y[i] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 16; u++)
-                if (i0 + 1024L * u < count) s += xv[u] * yv[u];
+        for (int u = 0; u < 16; u++) {
+            const long i = i0 + 1024L * u;
+            xv[u] = (i < count) ? x[i] : 0.0;
+            yv[u] = (i < count) ? y[i] : 0.0;
         }
-        sm[p][t] = s;
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            if (i0 + 1024L * u < count) s += xv[u] * yv[u];
     }
+    sm[t] = s;
     __syncthreads();
     for (int o = 512; o > 0; o >>= 1) {
-        if (t < o)
-            for (int p = 0; p < npairs; p++) sm[p][t] += sm[p][t + o];
+        if (t < o) sm[t] += sm[t + o];
         __syncthreads();
     }
-    if (t < npairs) out[t] = sm[t][0];
+    if (t == 0) out[p] = sm[0];
 }
 
 __global__ __launch_bounds__(1024) void k_objgrad_finish(const double* r, const double* c,
@@ -238,7 +235,7 @@ void vec_dots(const double* const* xs, const double* const* ys, long count, int 
         d.x[p] = xs[p];
         d.y[p] = ys[p];
     }
-    hipLaunchKernelGGL(k_dots, dim3(1), dim3(1024), 0, s, d, count, npairs, out);
+    hipLaunchKernelGGL(k_dots, dim3(npairs), dim3(1024), 0, s, d, count, out);
     KERNEL_CHECK();
 }
 
