@@ -49,6 +49,11 @@ int regot_device_count(void);
  * self-test of that routine against a correctly rounded exp2). */
 int regot_check_exp2(const double* x, double* y, long long count, int device);
 
+/* y[i] = ln x[i] by the device FP64 logarithm of the APDAGD plan pass (host
+ * arrays; no reference counterpart: a self-test of that routine against a
+ * correctly rounded log). */
+int regot_check_log(const double* x, double* y, long long count, int device);
+
 /*
  * Problem handle: M (n x m, row-major, row pitch ldm elements) resident on
  * one GPU in the bulk precision, plus a, b, log a, log b and reg.

@@ -20,6 +20,7 @@
 #include <algorithm>
 
 #include "dexp2.hpp"
+#include "dlog.hpp"
 #include "problem.hpp"
 
 namespace regot {
@@ -100,7 +101,9 @@ __global__ __launch_bounds__(256) void k_apd_plan(const TM* __restrict__ M, long
     __shared__ double sm[4][256];
     __shared__ double ss[2][4];
     __shared__ __align__(16) double tab[128];
+    __shared__ double ltab[384];
     dev::exp2_tab_load(tab);
+    dev::log_tab_load(ltab);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int strip = blockIdx.x, blk = blockIdx.y;
@@ -128,7 +131,7 @@ __global__ __launch_bounds__(256) void k_apd_plan(const TM* __restrict__ M, long
                 rs += v;
                 cs[q] += v;
                 mp += mv[q] * v;
-                en += v * (1.0 - log(v));
+                en += v * (1.0 - dev::dlog(v, ltab));
             }
         }
         if (c0 < m) st4(prow, c0, m, pv);
@@ -235,7 +238,26 @@ __global__ __launch_bounds__(256) void k_apd_final(const double* __restrict__ pa
     }
 }
 
+__global__ __launch_bounds__(256) void k_log_check(const double* __restrict__ x,
+                                                   double* __restrict__ y, long count)
+{
+    __shared__ double ltab[384];
+    dev::log_tab_load(ltab);
+    __syncthreads();
+    for (long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; i < count;
+         i += static_cast<long>(gridDim.x) * 256)
+        y[i] = dev::dlog(x[i], ltab);
+}
+
 }  // namespace
+
+void launch_log_check(const double* x, double* y, long count, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_log_check, dim3(std::min(4096L, (count + 255) / 256)), dim3(256), 0, s,
+                       x, y, count);
+    KERNEL_CHECK();
+}
 
 // Scratch sizes (doubles) for a plan pass on an n x m problem.
 void apd_scratch_sizes(int n, int m, long* rowpart, long* colpart, long* scal, long* part)

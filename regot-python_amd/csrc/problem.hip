@@ -430,6 +430,19 @@ int regot_check_exp2(const double* x, double* y, long long count, int device)
     CAPI_END
 }
 
+int regot_check_log(const double* x, double* y, long long count, int device)
+{
+    CAPI_BEGIN
+    if (count < 0 || (count > 0 && (!x || !y))) fail(E_INVALID, "bad arguments");
+    if (count == 0) return OK;
+    HIP_CHECK(hipSetDevice(device));
+    DBuf<double> dx(count), dy(count);
+    HIP_CHECK(hipMemcpy(dx.p, x, count * sizeof(double), hipMemcpyHostToDevice));
+    launch_log_check(dx, dy, count, nullptr);
+    HIP_CHECK(hipMemcpy(y, dy.p, count * sizeof(double), hipMemcpyDeviceToHost));
+    CAPI_END
+}
+
 int regot_problem_create(regot_problem** out, int n, int m, const void* M, int M_dtype,
                          int bulk, int M_device, long long ldm, const double* a,
                          const double* b, double reg, int device)

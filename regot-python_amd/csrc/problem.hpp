@@ -76,6 +76,7 @@ void launch_wsums(int bulk, const void* M, long ldm, int n, int m, const double*
 
 // y = 2^x by the device FP64 exponential of the one-pass kernels (dexp2.hpp).
 void launch_exp2_check(const double* x, double* y, long count, hipStream_t s);
+void launch_log_check(const double* x, double* y, long count, hipStream_t s);
 
 // Buffers of the on-device BCD loop, kept across solves on one handle.
 struct BcdWork {

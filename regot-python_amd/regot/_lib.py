@@ -92,6 +92,7 @@ def lib():
         ctypes.c_int, ctypes.c_longlong, _dp, _dp, ctypes.c_double, ctypes.c_int]
     L.regot_problem_destroy.argtypes = [_vp]
     L.regot_check_exp2.argtypes = [_dp, _dp, ctypes.c_longlong, ctypes.c_int]
+    L.regot_check_log.argtypes = [_dp, _dp, ctypes.c_longlong, ctypes.c_int]
     L.regot_problem_dims.argtypes = [_vp, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     for name in ("regot_optimal_alpha", "regot_optimal_beta"):

@@ -177,3 +177,30 @@ def test_round_plan_and_primal_value(gpu_lib, bulk):
             pv = P.primal_val(g)
             assert abs(pv - O.primal_val(M, a, b, reg, g)) < 1e-12 * max(1.0, abs(pv))
         assert abs(P.primal_val(o["dual"]) + o["obj_vals"][-1]) < 1e-9
+
+
+def test_device_fp64_log_accuracy(gpu_lib):
+    # The table-driven FP64 log of the APDAGD plan pass (entropy term), against
+    # log in 80-bit extended precision rounded to double, on log-uniform
+    # arguments over the whole double range, subnormals, arguments next to 1
+    # and special values: at most one ulp, and rarely that.
+    from regot import _lib
+    rng = np.random.default_rng(4)
+    x = np.concatenate([np.exp2(rng.uniform(-1074.0, 1023.9, 500000)), rng.uniform(0.5, 2.0, 300000),
+                        1.0 + rng.uniform(-1e-3, 1e-3, 100000), 1.0 + rng.uniform(-1e-12, 1e-12, 10000),
+                        rng.uniform(5e-324, 2.3e-308, 50000),
+                        [1.0, 2.0, 0.5, 0.75, 1.5, np.nextafter(1.0, 0.0), np.nextafter(1.0, 2.0),
+                         5e-324, 2.2250738585072014e-308, 1.7976931348623157e308]])
+    y = np.empty_like(x)
+    _lib.check(_lib.lib().regot_check_log(_lib.dptr(x), _lib.dptr(y), len(x), 0))
+    ref = np.log(x.astype(np.longdouble)).astype(np.float64)
+    assert np.all(y[x == 1.0] == 0.0)
+    nz = ref != 0
+    ulp = np.abs(y[nz].view(np.int64) - ref[nz].view(np.int64))
+    assert ulp.max() <= 1
+    assert np.mean(ulp > 0) < 0.01
+    z = np.array([0.0, -0.0, np.inf, -1.0, -np.inf, np.nan])
+    w = np.empty_like(z)
+    _lib.check(_lib.lib().regot_check_log(_lib.dptr(z), _lib.dptr(w), len(z), 0))
+    assert w[0] == -np.inf and w[1] == -np.inf and w[2] == np.inf
+    assert np.all(np.isnan(w[3:]))
